@@ -1,0 +1,36 @@
+"""box2d_amd: Box2D v3's world step on AMD Instinct MI355X (gfx950).
+
+The native library ``lib/libbox2d_amd.so`` exports the Box2D C API (see include/box2d/box2d.h);
+``b2World_Step`` runs its arithmetic in hand-written HIP kernels. This package only locates the
+built libraries and wraps the scene driver for tests and benchmarks.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+LIBRARY_PATH = os.path.join(LIB_DIR, "libbox2d_amd.so")
+SCENES_PATH = os.path.join(LIB_DIR, "libb2amd_scenes.so")
+
+_lib = None
+
+
+def library():
+    """Load libbox2d_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise RuntimeError(
+                f"{LIBRARY_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        _lib = ctypes.CDLL(LIBRARY_PATH)
+    return _lib
+
+
+from .scenes_api import SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HELLO, SCENE_MIXED_PILE  # noqa: E402
+
+
+def scenes():
+    """Scene driver bound to the GPU library."""
+    library()
+    return SceneLib(SCENES_PATH)

@@ -1,0 +1,87 @@
+// GPU half of a world: device arrays, host<->device synchronization and the stage launches of
+// one b2World_Step. Only plain types cross this interface so the host bookkeeping (world.cpp)
+// compiles without HIP headers.
+#pragma once
+
+#include "layout.h"
+
+#include <cstdint>
+#include <vector>
+
+namespace b2g
+{
+
+class World;
+
+// Work lists the host rebuilds when the set of awake bodies or the graph coloring changes.
+struct SolverLists
+{
+	std::vector<int> awakeBodies;		  // body ids in awake-set order
+	std::vector<int> collideContacts;	  // every awake contact (graph colors + non-touching)
+	std::vector<int> constraintContacts; // colored contacts by active color, then overflow contacts
+	std::vector<int> colorContactStart;	  // per active color: first constraint index (+ sentinel)
+	std::vector<int> jointList;			  // colored joints by active color, then overflow joints
+	std::vector<int> colorJointStart;	  // per active color: first joint index (+ sentinel)
+	int activeColorCount = 0;
+	int overflowContactStart = 0;
+	int overflowContactCount = 0;
+	int overflowJointStart = 0;
+	int overflowJointCount = 0;
+};
+
+// Results the host needs after the collide stage.
+struct CollideResult
+{
+	// contact id * 4 + kind (1 disjoint, 2 started touching, 3 stopped touching), ascending id
+	std::vector<int> changes;
+	int recycledCount = 0;
+};
+
+// Results the host needs after body finalization.
+struct FinalizeResult
+{
+	std::vector<int> enlargedBodies; // awake indices of bodies with an enlarged shape, ascending
+	std::vector<int> enlargedShapes; // shape ids whose fat AABB grew
+	std::vector<b2AABB> enlargedShapeAABBs;
+	std::vector<float> sleepTimes; // per awake index (only when sleep is enabled)
+	int fastBodyCount = 0;
+};
+
+class DeviceWorld
+{
+public:
+	DeviceWorld();
+	~DeviceWorld();
+
+	// Host -> device uploads of id-indexed arrays (whole-array copies; used when the host edited them).
+	void UploadBodies( const World& world );
+	void UploadShapes( const World& world );
+	void UploadJoints( const World& world );
+	void UploadContactInits( const std::vector<ContactInit>& inits, int contactCapacity );
+	void UploadLists( const SolverLists& lists );
+
+	// Device -> host downloads (after a step, on demand).
+	void DownloadBodies( World& world );
+	void DownloadShapes( World& world );
+	void DownloadJoints( World& world );
+	void DownloadContact( int contactId, ContactSim* out );
+
+	// Stages
+	void Collide( const StepParams& params, CollideResult* result );
+	void Solve( const StepParams& params, bool anyRestitution, bool needSleepTimes, FinalizeResult* result );
+
+	// Per-stage GPU timings of the last step in milliseconds (collide, solve, finalize).
+	float lastCollideMs = 0.0f;
+	float lastSolveMs = 0.0f;
+
+	void* stream = nullptr;
+
+private:
+	struct Impl;
+	Impl* impl;
+};
+
+// Last HIP error text (empty if none) for diagnostics.
+const char* LastDeviceError();
+
+} // namespace b2g

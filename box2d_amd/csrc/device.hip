@@ -1,0 +1,1458 @@
+// GPU half of b2World_Step for gfx950: device arrays, kernels and the per-step launch sequence.
+//
+// Stage order per step (reference src/physics_world.c:845-1005, src/solver.c:1010-1198):
+//   collide        one lane per awake contact: AABB test, recycling, manifold, warm-start id match
+//   prepare        one lane per graph constraint (contacts) / joint
+//   sub-steps      integrate velocities; warm start, solve, integrate positions, relax
+//                  (overflow constraints first, single lane in order, then one launch per color)
+//   restitution    only when some shape has restitution
+//   store          impulses back into the contact manifolds
+//   finalize       one lane per awake body: transforms, sleep velocity, fat AABBs, enlarge flags
+// Colors are Gauss-Seidel phases: constraints inside a color touch disjoint dynamic bodies, so a
+// color is solved by one fully parallel launch and the result is independent of lane order.
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "narrowphase.h"
+#include "solver_device.h"
+#include "world.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace b2g
+{
+
+static char s_lastError[256] = "";
+
+const char* LastDeviceError()
+{
+	return s_lastError;
+}
+
+#define HIP_CHECK( expr )                                                                                                        \
+	do                                                                                                                           \
+	{                                                                                                                            \
+		hipError_t _e = ( expr );                                                                                                \
+		if ( _e != hipSuccess )                                                                                                  \
+		{                                                                                                                        \
+			snprintf( s_lastError, sizeof( s_lastError ), "%s failed: %s", #expr, hipGetErrorString( _e ) );                   \
+			::b2g::Fatal( "HIP error: %s (%s:%d)", s_lastError, __FILE__, __LINE__ );                                             \
+		}                                                                                                                        \
+	}                                                                                                                            \
+	while ( 0 )
+
+constexpr int kBlock = 256;
+
+static inline int Blocks( int n )
+{
+	return ( n + kBlock - 1 ) / kBlock;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernels
+
+__global__ void InitContactsKernel( ContactSim* contacts, const ContactInit* inits, int count )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= count )
+	{
+		return;
+	}
+	ContactInit ci = inits[i];
+	ContactSim c;
+	memset( &c, 0, sizeof( c ) );
+	c.shapeIdA = ci.shapeIdA;
+	c.shapeIdB = ci.shapeIdB;
+	c.bodyIdA = ci.bodyIdA;
+	c.bodyIdB = ci.bodyIdB;
+	c.simFlags = ci.simFlags;
+	c.friction = ci.friction;
+	c.restitution = ci.restitution;
+	contacts[ci.contactId] = c;
+}
+
+struct CollideArgs
+{
+	const int* list;
+	int count;
+	ContactSim* contacts;
+	const ShapeDev* shapes;
+	const BodySim* sims;
+	const BodyAux* aux;
+	int* changes;
+	int* counters; // [0] change count, [1] recycled count
+	float recycleDistance;
+};
+
+B2G_DEV float ShapeRadius( const ShapeDev& s )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			return s.capsule.radius;
+		case b2_circleShape:
+			return s.circle.radius;
+		case b2_polygonShape:
+			return s.polygon.radius;
+		default:
+			return 0.0f;
+	}
+}
+
+// Narrowphase update for one contact (reference src/physics_world.c:451-609 and
+// src/contact.c:519-698).
+__global__ void CollideKernel( CollideArgs a )
+{
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( k >= a.count )
+	{
+		return;
+	}
+
+	int contactId = a.list[k];
+	ContactSim* cs = a.contacts + contactId;
+	const ShapeDev& shapeA = a.shapes[cs->shapeIdA];
+	const ShapeDev& shapeB = a.shapes[cs->shapeIdB];
+
+	bool overlap = b2AABB_Overlaps( shapeA.fatAABB, shapeB.fatAABB );
+	if ( overlap == false )
+	{
+		cs->simFlags |= kSimDisjoint;
+		cs->simFlags &= ~kSimTouching;
+		int slot = atomicAdd( a.counters + 0, 1 );
+		a.changes[slot] = contactId * 4 + 1;
+		return;
+	}
+
+	uint32_t simFlags = cs->simFlags;
+	bool wasTouching = ( simFlags & kSimTouching ) != 0;
+
+	int bodyIdA = shapeA.bodyId;
+	int bodyIdB = shapeB.bodyId;
+	const BodySim& simA = a.sims[bodyIdA];
+	const BodySim& simB = a.sims[bodyIdB];
+	b2Transform transformA = simA.transform;
+	b2Transform transformB = simB.transform;
+	const BodyAux& auxA = a.aux[bodyIdA];
+	const BodyAux& auxB = a.aux[bodyIdB];
+
+	float recycleDistance = a.recycleDistance;
+	float recycleDistanceNonTouching = b2MinFloat( recycleDistance, kSpeculativeDistance );
+
+	bool isFast = ( auxA.flags & kIsFast ) || ( auxB.flags & kIsFast );
+
+	if ( isFast == false && recycleDistance > 0.0f && ( simFlags & kSimRelativeTransformValid ) && ( simFlags & kContactRecycle ) )
+	{
+		b2Rot cachedQA = cs->cachedRotationA;
+		b2Rot cachedQB = cs->cachedRotationB;
+		b2Transform xfc = cs->cachedRelativePose;
+		b2Transform xf = b2InvMulWorldTransforms( transformA, transformB );
+
+		float cosA = transformA.q.c * cachedQA.c + transformA.q.s * cachedQA.s;
+		float cosB = transformB.q.c * cachedQB.c + transformB.q.s * cachedQB.s;
+		float minCos = b2MinFloat( cosA, cosB );
+
+		float maxExtentA = auxA.type == b2_staticBody ? 0.0f : simA.maxExtent;
+		float maxExtentB = auxB.type == b2_staticBody ? 0.0f : simB.maxExtent;
+		float maxExtent = b2MaxFloat( maxExtentA, maxExtentB );
+		float distance = b2Distance( xf.p, xfc.p );
+		b2Rot qr = b2InvMulRot( xf.q, xfc.q );
+
+		float tolerance = wasTouching ? recycleDistance : recycleDistanceNonTouching;
+
+		if ( minCos > 0.98f && distance + maxExtent * b2AbsFloat( qr.s ) < tolerance )
+		{
+			b2Rot dqA = b2MulRot( transformA.q, b2InvertRot( cachedQA ) );
+			b2Rot dqB = b2MulRot( transformB.q, b2InvertRot( cachedQB ) );
+			b2Vec2 normal = cs->manifold.normal;
+			b2Vec2 dc = b2SubPos( simB.center, simA.center );
+
+			for ( int i = 0; i < cs->manifold.pointCount; ++i )
+			{
+				b2ManifoldPoint* mp = cs->manifold.points + i;
+				b2Vec2 rA = b2RotateVector( dqA, mp->anchorA );
+				b2Vec2 rB = b2RotateVector( dqB, mp->anchorB );
+				b2Vec2 dp = b2Add( dc, b2Sub( rB, rA ) );
+				mp->separation = mp->baseSeparation + b2Dot( dp, normal );
+				mp->persisted = true;
+			}
+
+			atomicAdd( a.counters + 1, 1 );
+			return;
+		}
+	}
+
+	cs->cachedRotationA = transformA.q;
+	cs->cachedRotationB = transformB.q;
+	cs->cachedRelativePose = b2InvMulWorldTransforms( transformA, transformB );
+	simFlags |= kSimRelativeTransformValid;
+
+	b2Vec2 centerOffsetA = b2RotateVector( transformA.q, simA.localCenter );
+	b2Vec2 centerOffsetB = b2RotateVector( transformB.q, simB.localCenter );
+
+	// b2UpdateContact
+	b2Manifold oldManifold = cs->manifold;
+	b2Transform relativeTransform = b2InvMulWorldTransforms( transformA, transformB );
+	b2LocalManifold local = ComputeManifold( shapeA, shapeB, relativeTransform );
+
+	b2Manifold m;
+	memset( &m, 0, sizeof( m ) );
+	m.normal = b2RotateVector( transformA.q, local.normal );
+	m.pointCount = local.pointCount;
+
+	b2Vec2 originDelta = b2SubPos( transformA.p, transformB.p );
+	for ( int i = 0; i < local.pointCount; ++i )
+	{
+		b2ManifoldPoint* mp = m.points + i;
+		mp->anchorA = b2RotateVector( transformA.q, local.points[i].point );
+		mp->anchorB = b2Add( mp->anchorA, originDelta );
+		mp->separation = local.points[i].separation;
+		mp->id = local.points[i].id;
+	}
+
+	cs->friction = sqrtf( shapeA.friction * shapeB.friction );
+	cs->restitution = b2MaxFloat( shapeA.restitution, shapeB.restitution );
+
+	if ( shapeA.rollingResistance > 0.0f || shapeB.rollingResistance > 0.0f )
+	{
+		float radiusA = ShapeRadius( shapeA );
+		float radiusB = ShapeRadius( shapeB );
+		float maxRadius = b2MaxFloat( radiusA, radiusB );
+		cs->rollingResistance = b2MaxFloat( shapeA.rollingResistance, shapeB.rollingResistance ) * maxRadius;
+	}
+	else
+	{
+		cs->rollingResistance = 0.0f;
+	}
+
+	cs->tangentSpeed = shapeA.tangentSpeed + shapeB.tangentSpeed;
+
+	int pointCount = m.pointCount;
+	bool touching = pointCount > 0;
+
+	if ( touching && ( ( shapeA.flags | shapeB.flags ) & kShapeEnableHitEvents ) )
+	{
+		simFlags |= kSimEnableHitEvent;
+	}
+	else
+	{
+		simFlags &= ~kSimEnableHitEvent;
+	}
+
+	if ( pointCount > 0 )
+	{
+		m.rollingImpulse = oldManifold.rollingImpulse;
+	}
+
+	for ( int i = 0; i < pointCount; ++i )
+	{
+		b2ManifoldPoint* mp2 = m.points + i;
+		mp2->anchorA = b2Sub( mp2->anchorA, centerOffsetA );
+		mp2->anchorB = b2Sub( mp2->anchorB, centerOffsetB );
+		mp2->normalImpulse = 0.0f;
+		mp2->tangentImpulse = 0.0f;
+		mp2->totalNormalImpulse = 0.0f;
+		mp2->normalVelocity = 0.0f;
+		mp2->persisted = false;
+
+		uint16_t id2 = mp2->id;
+		for ( int j = 0; j < oldManifold.pointCount; ++j )
+		{
+			b2ManifoldPoint* mp1 = oldManifold.points + j;
+			if ( mp1->id == id2 )
+			{
+				mp2->normalImpulse = mp1->normalImpulse;
+				mp2->tangentImpulse = mp1->tangentImpulse;
+				mp2->persisted = true;
+				mp1->normalImpulse = 0.0f;
+				mp1->tangentImpulse = 0.0f;
+				break;
+			}
+		}
+	}
+
+	if ( touching )
+	{
+		simFlags |= kSimTouching;
+	}
+	else
+	{
+		simFlags &= ~kSimTouching;
+	}
+
+	for ( int i = 0; i < m.pointCount; ++i )
+	{
+		m.points[i].baseSeparation = m.points[i].separation;
+	}
+
+	cs->manifold = m;
+	cs->simFlags = simFlags;
+
+	if ( touching == true && wasTouching == false )
+	{
+		int slot = atomicAdd( a.counters + 0, 1 );
+		a.changes[slot] = contactId * 4 + 2;
+	}
+	else if ( touching == false && wasTouching == true )
+	{
+		int slot = atomicAdd( a.counters + 0, 1 );
+		a.changes[slot] = contactId * 4 + 3;
+	}
+}
+
+struct PrepareArgs
+{
+	ConstraintSoA c;
+	int count;
+	const ContactSim* contacts;
+	const BodySim* sims;
+	const BodyState* states;
+	const BodyAux* aux;
+	StepParams p;
+};
+
+// Constraint preparation (reference src/contact_solver.c:1573-1809; the overflow variant at
+// :24-160 computes the same values).
+__global__ void PrepareContactsKernel( PrepareArgs a )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= a.count )
+	{
+		return;
+	}
+
+	const ConstraintSoA& c = a.c;
+	const ContactSim* cs = a.contacts + c.contactId[i];
+	const b2Manifold& manifold = cs->manifold;
+
+	int bodyIdA = cs->bodyIdA;
+	int bodyIdB = cs->bodyIdB;
+	c.bodyA[i] = bodyIdA;
+	c.bodyB[i] = bodyIdB;
+	c.pointCount[i] = manifold.pointCount;
+
+	bool staticA = a.aux[bodyIdA].type == b2_staticBody;
+	bool staticB = a.aux[bodyIdB].type == b2_staticBody;
+
+	float mA = a.sims[bodyIdA].invMass;
+	float iA = a.sims[bodyIdA].invInertia;
+	float mB = a.sims[bodyIdB].invMass;
+	float iB = a.sims[bodyIdB].invInertia;
+
+	b2Vec2 vA = a.states[bodyIdA].linearVelocity;
+	float wA = a.states[bodyIdA].angularVelocity;
+	b2Vec2 vB = a.states[bodyIdB].linearVelocity;
+	float wB = a.states[bodyIdB].angularVelocity;
+
+	c.f[kcInvMassA][i] = mA;
+	c.f[kcInvMassB][i] = mB;
+	c.f[kcInvIA][i] = iA;
+	c.f[kcInvIB][i] = iB;
+
+	{
+		float k = iA + iB;
+		c.f[kcRollingMass][i] = k > 0.0f ? 1.0f / k : 0.0f;
+	}
+
+	Softness soft = a.p.contactSoftness;
+	if ( staticA || staticB )
+	{
+		soft = a.p.staticSoftness;
+	}
+	else if ( a.p.enableContactSoftening )
+	{
+		float contactHertz = b2MinFloat( a.p.contactHertz, 0.125f * a.p.inv_h );
+		float ratio = 1.0f;
+		if ( mA < mB )
+		{
+			ratio = b2MaxFloat( 0.5f, mA / mB );
+		}
+		else if ( mB < mA )
+		{
+			ratio = b2MaxFloat( 0.5f, mB / mA );
+		}
+		soft = MakeSoft( ratio * contactHertz, ratio * a.p.contactDampingRatio, a.p.h );
+	}
+
+	float warmStartScale = a.p.enableWarmStarting ? 1.0f : 0.0f;
+
+	b2Vec2 normal = manifold.normal;
+	c.f[kcNormalX][i] = normal.x;
+	c.f[kcNormalY][i] = normal.y;
+	c.f[kcFriction][i] = cs->friction;
+	c.f[kcTangentSpeed][i] = cs->tangentSpeed;
+	c.f[kcRestitution][i] = cs->restitution;
+	c.f[kcRollingResistance][i] = cs->rollingResistance;
+	c.f[kcRollingImpulse][i] = warmStartScale * manifold.rollingImpulse;
+	c.f[kcBiasRate][i] = soft.biasRate;
+	c.f[kcMassScale][i] = soft.massScale;
+	c.f[kcImpulseScale][i] = soft.impulseScale;
+
+	b2Vec2 tangent = b2RightPerp( normal );
+
+	for ( int j = 0; j < 2; ++j )
+	{
+		int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+		if ( j < manifold.pointCount )
+		{
+			const b2ManifoldPoint* mp = manifold.points + j;
+			b2Vec2 rA = mp->anchorA;
+			b2Vec2 rB = mp->anchorB;
+			c.f[kcAnchorA1X + o][i] = rA.x;
+			c.f[kcAnchorA1Y + o][i] = rA.y;
+			c.f[kcAnchorB1X + o][i] = rB.x;
+			c.f[kcAnchorB1Y + o][i] = rB.y;
+			c.f[kcBaseSeparation1 + o][i] = mp->separation - b2Dot( b2Sub( rB, rA ), normal );
+			c.f[kcNormalImpulse1 + o][i] = warmStartScale * mp->normalImpulse;
+			c.f[kcTangentImpulse1 + o][i] = warmStartScale * mp->tangentImpulse;
+			c.f[kcTotalNormalImpulse1 + o][i] = 0.0f;
+
+			float rnA = b2Cross( rA, normal );
+			float rnB = b2Cross( rB, normal );
+			float kNormal = mA + mB + iA * rnA * rnA + iB * rnB * rnB;
+			c.f[kcNormalMass1 + o][i] = kNormal > 0.0f ? 1.0f / kNormal : 0.0f;
+
+			float rtA = b2Cross( rA, tangent );
+			float rtB = b2Cross( rB, tangent );
+			float kTangent = mA + mB + iA * rtA * rtA + iB * rtB * rtB;
+			c.f[kcTangentMass1 + o][i] = kTangent > 0.0f ? 1.0f / kTangent : 0.0f;
+
+			b2Vec2 vrA = b2Add( vA, b2CrossSV( wA, rA ) );
+			b2Vec2 vrB = b2Add( vB, b2CrossSV( wB, rB ) );
+			c.f[kcRelativeVelocity1 + o][i] = b2Dot( normal, b2Sub( vrB, vrA ) );
+		}
+		else
+		{
+			c.f[kcBaseSeparation1 + o][i] = 0.0f;
+			c.f[kcNormalImpulse1 + o][i] = 0.0f;
+			c.f[kcTangentImpulse1 + o][i] = 0.0f;
+			c.f[kcTotalNormalImpulse1 + o][i] = 0.0f;
+			c.f[kcAnchorA1X + o][i] = 0.0f;
+			c.f[kcAnchorA1Y + o][i] = 0.0f;
+			c.f[kcAnchorB1X + o][i] = 0.0f;
+			c.f[kcAnchorB1Y + o][i] = 0.0f;
+			c.f[kcNormalMass1 + o][i] = 0.0f;
+			c.f[kcTangentMass1 + o][i] = 0.0f;
+			c.f[kcRelativeVelocity1 + o][i] = 0.0f;
+		}
+	}
+}
+
+struct PrepareJointArgs
+{
+	JointSim* joints;
+	const int* list;
+	int count;
+	const BodySim* sims;
+	StepParams p;
+};
+
+// Joint preparation (reference src/joint.c:1406-1452, src/revolute_joint.c:217-281).
+__global__ void PrepareJointsKernel( PrepareJointArgs a )
+{
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( k >= a.count )
+	{
+		return;
+	}
+
+	JointSim* joint = a.joints + a.list[k];
+	float hertz = b2MinFloat( joint->constraintHertz, 0.25f * a.p.inv_h );
+	joint->constraintSoftness = MakeSoft( hertz, joint->constraintDampingRatio, a.p.h );
+
+	const BodySim& simA = a.sims[joint->bodyIdA];
+	const BodySim& simB = a.sims[joint->bodyIdB];
+
+	float mA = simA.invMass;
+	float iA = simA.invInertia;
+	float mB = simB.invMass;
+	float iB = simB.invInertia;
+	joint->invMassA = mA;
+	joint->invMassB = mB;
+	joint->invIA = iA;
+	joint->invIB = iB;
+
+	joint->indexA = joint->bodyIdA;
+	joint->indexB = joint->bodyIdB;
+
+	joint->frameA.q = b2MulRot( simA.transform.q, joint->localFrameA.q );
+	joint->frameA.p = b2RotateVector( simA.transform.q, b2Sub( joint->localFrameA.p, simA.localCenter ) );
+	joint->frameB.q = b2MulRot( simB.transform.q, joint->localFrameB.q );
+	joint->frameB.p = b2RotateVector( simB.transform.q, b2Sub( joint->localFrameB.p, simB.localCenter ) );
+	joint->deltaCenter = b2SubPos( simB.center, simA.center );
+
+	float k2 = iA + iB;
+	joint->axialMass = k2 > 0.0f ? 1.0f / k2 : 0.0f;
+	joint->springSoftness = MakeSoft( joint->hertz, joint->dampingRatio, a.p.h );
+
+	if ( a.p.enableWarmStarting == 0 )
+	{
+		joint->linearImpulse = { 0.0f, 0.0f };
+		joint->springImpulse = 0.0f;
+		joint->motorImpulse = 0.0f;
+		joint->lowerImpulse = 0.0f;
+		joint->upperImpulse = 0.0f;
+	}
+}
+
+struct BodyArgs
+{
+	const int* awake;
+	int count;
+	BodySim* sims;
+	BodyState* states;
+	BodyAux* aux;
+	StepParams p;
+};
+
+// reference src/solver.c:66-112
+__global__ void IntegrateVelocitiesKernel( BodyArgs a )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= a.count )
+	{
+		return;
+	}
+	int id = a.awake[i];
+	const BodySim& sim = a.sims[id];
+	BodyState* state = a.states + id;
+
+	b2Vec2 v = state->linearVelocity;
+	float w = state->angularVelocity;
+	float h = a.p.h;
+
+	float linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
+	float angularDamping = 1.0f / ( 1.0f + h * sim.angularDamping );
+	float gravityScale = sim.invMass > 0.0f ? sim.gravityScale : 0.0f;
+
+	b2Vec2 linearVelocityDelta = b2Add( b2MulSV( h * sim.invMass, sim.force ), b2MulSV( h * gravityScale, a.p.gravity ) );
+	float angularVelocityDelta = h * sim.invInertia * sim.torque;
+
+	v = b2MulAdd( linearVelocityDelta, linearDamping, v );
+	w = angularVelocityDelta + angularDamping * w;
+
+	state->linearVelocity = v;
+	state->angularVelocity = w;
+}
+
+// reference src/solver.c:114-162
+__global__ void IntegratePositionsKernel( BodyArgs a )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= a.count )
+	{
+		return;
+	}
+	BodyState* state = a.states + a.awake[i];
+
+	float h = a.p.h;
+	float maxLinearSpeed = a.p.maxLinearVelocity;
+	float maxAngularSpeed = ( 0.25f * B2_PI ) * a.p.inv_dt;
+	float maxLinearSpeedSquared = maxLinearSpeed * maxLinearSpeed;
+	float maxAngularSpeedSquared = maxAngularSpeed * maxAngularSpeed;
+
+	b2Vec2 v = state->linearVelocity;
+	float w = state->angularVelocity;
+	uint32_t flags = state->flags;
+
+	v.x = ( flags & kLockLinearX ) ? 0.0f : v.x;
+	v.y = ( flags & kLockLinearY ) ? 0.0f : v.y;
+	w = ( flags & kLockAngularZ ) ? 0.0f : w;
+
+	if ( b2Dot( v, v ) > maxLinearSpeedSquared )
+	{
+		float ratio = maxLinearSpeed / b2Length( v );
+		v = b2MulSV( ratio, v );
+		flags |= kIsSpeedCapped;
+	}
+
+	if ( w * w > maxAngularSpeedSquared && ( flags & kAllowFastRotation ) == 0 )
+	{
+		float ratio = maxAngularSpeed / b2AbsFloat( w );
+		w *= ratio;
+		flags |= kIsSpeedCapped;
+	}
+
+	state->flags = flags;
+	state->linearVelocity = v;
+	state->angularVelocity = w;
+	state->deltaPosition = b2MulAdd( state->deltaPosition, h, v );
+	state->deltaRotation = b2IntegrateRotation( state->deltaRotation, h * w );
+}
+
+enum ColorStage
+{
+	kStageWarmStart,
+	kStageSolve,
+	kStageRelax,
+	kStageRestitution,
+};
+
+struct ColorArgs
+{
+	ConstraintSoA c;
+	JointSim* joints;
+	const int* jointList;
+	int jointStart;
+	int jointCount;
+	int contactStart;
+	int contactCount;
+	BodyState* states;
+	StepParams p;
+	int stage;
+};
+
+// True when any constraint in the 4-lane group of local index k has a nonzero value in field f
+// (the wide solver skips or runs rolling and restitution per lane group).
+B2G_DEV bool GroupAnyNonZero( const ConstraintSoA& c, int f, int colorStart, int colorCount, int k )
+{
+	int base = k & ~3;
+	int end = b2MinInt( base + 4, colorCount );
+	for ( int j = base; j < end; ++j )
+	{
+		if ( c.f[f][colorStart + j] != 0.0f )
+		{
+			return true;
+		}
+	}
+	return false;
+}
+
+// One graph color of one stage: joints occupy the first jointCount lanes, contacts the rest.
+__global__ void ColorKernel( ColorArgs a )
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( t < a.jointCount )
+	{
+		JointSim* joint = a.joints + a.jointList[a.jointStart + t];
+		if ( a.stage == kStageWarmStart )
+		{
+			WarmStartRevolute( joint, a.states );
+		}
+		else if ( a.stage == kStageSolve )
+		{
+			SolveRevolute( joint, a.states, true, a.p.h, a.p.inv_h );
+		}
+		else if ( a.stage == kStageRelax )
+		{
+			SolveRevolute( joint, a.states, false, a.p.h, a.p.inv_h );
+		}
+		return;
+	}
+
+	int k = t - a.jointCount;
+	if ( k >= a.contactCount )
+	{
+		return;
+	}
+	int i = a.contactStart + k;
+
+	switch ( a.stage )
+	{
+		case kStageWarmStart:
+			WarmStartContact( a.c, i, a.states );
+			break;
+		case kStageSolve:
+			SolveContact( a.c, i, a.states, true, a.p.inv_h, -a.p.contactSpeed, false );
+			break;
+		case kStageRelax:
+		{
+			bool rolling = GroupAnyNonZero( a.c, kcRollingResistance, a.contactStart, a.contactCount, k );
+			SolveContact( a.c, i, a.states, false, a.p.inv_h, -a.p.contactSpeed, rolling );
+		}
+		break;
+		case kStageRestitution:
+			if ( GroupAnyNonZero( a.c, kcRestitution, a.contactStart, a.contactCount, k ) )
+			{
+				RestitutionContact( a.c, i, a.states, a.p.restitutionThreshold );
+			}
+			break;
+		default:
+			break;
+	}
+}
+
+// Overflow constraints: a single lane walks joints then contacts in order, as the reference's
+// single-threaded overflow passes do (src/solver.c:1100-1153).
+__global__ void OverflowKernel( ColorArgs a )
+{
+	if ( blockIdx.x != 0 || threadIdx.x != 0 )
+	{
+		return;
+	}
+
+	for ( int t = 0; t < a.jointCount; ++t )
+	{
+		JointSim* joint = a.joints + a.jointList[a.jointStart + t];
+		if ( a.stage == kStageWarmStart )
+		{
+			WarmStartRevolute( joint, a.states );
+		}
+		else if ( a.stage == kStageSolve )
+		{
+			SolveRevolute( joint, a.states, true, a.p.h, a.p.inv_h );
+		}
+		else if ( a.stage == kStageRelax )
+		{
+			SolveRevolute( joint, a.states, false, a.p.h, a.p.inv_h );
+		}
+	}
+
+	for ( int k = 0; k < a.contactCount; ++k )
+	{
+		int i = a.contactStart + k;
+		switch ( a.stage )
+		{
+			case kStageWarmStart:
+				OverflowWarmStart( a.c, i, a.states );
+				break;
+			case kStageSolve:
+				OverflowSolve( a.c, i, a.states, true, a.p.inv_h, a.p.contactSpeed );
+				break;
+			case kStageRelax:
+				OverflowSolve( a.c, i, a.states, false, a.p.inv_h, a.p.contactSpeed );
+				break;
+			case kStageRestitution:
+				OverflowRestitution( a.c, i, a.states, a.p.restitutionThreshold );
+				break;
+			default:
+				break;
+		}
+	}
+}
+
+struct StoreArgs
+{
+	ConstraintSoA c;
+	ContactSim* contacts;
+	int coloredCount;
+	int totalCount;
+};
+
+// Impulses back to the manifolds (reference src/contact_solver.c:2238-2331 for colored
+// constraints, which write both points, and :516-545 for overflow, which write pointCount points).
+__global__ void StoreImpulsesKernel( StoreArgs a )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= a.totalCount )
+	{
+		return;
+	}
+
+	const ConstraintSoA& c = a.c;
+	b2Manifold* m = &a.contacts[c.contactId[i]].manifold;
+	int writeCount = i < a.coloredCount ? 2 : m->pointCount;
+
+	for ( int j = 0; j < writeCount; ++j )
+	{
+		int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+		m->points[j].normalImpulse = c.f[kcNormalImpulse1 + o][i];
+		m->points[j].tangentImpulse = c.f[kcTangentImpulse1 + o][i];
+		m->points[j].totalNormalImpulse = c.f[kcTotalNormalImpulse1 + o][i];
+		m->points[j].normalVelocity = c.f[kcRelativeVelocity1 + o][i];
+	}
+	m->rollingImpulse = c.f[kcRollingImpulse][i];
+}
+
+// Fat shape AABB: float world points, double extremes and inflation, one narrowing cast
+// (reference src/geometry.c:402-505).
+B2G_DEV b2AABB FatShapeAABB( const ShapeDev& s, b2Transform xf, float extra )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+		{
+			b2Vec2 v1 = b2TransformWorldPoint( xf, s.capsule.center1 );
+			b2Vec2 v2 = b2TransformWorldPoint( xf, s.capsule.center2 );
+			double r = (double)s.capsule.radius + (double)extra;
+			double v1x = v1.x, v1y = v1.y, v2x = v2.x, v2y = v2.y;
+			b2AABB aabb = { { (float)( ( v1x < v2x ? v1x : v2x ) - r ), (float)( ( v1y < v2y ? v1y : v2y ) - r ) },
+							{ (float)( ( v1x > v2x ? v1x : v2x ) + r ), (float)( ( v1y > v2y ? v1y : v2y ) + r ) } };
+			return aabb;
+		}
+		case b2_circleShape:
+		{
+			b2Vec2 c = b2TransformWorldPoint( xf, s.circle.center );
+			double r = (double)s.circle.radius + (double)extra;
+			double cx = c.x, cy = c.y;
+			b2AABB aabb = { { (float)( cx - r ), (float)( cy - r ) }, { (float)( cx + r ), (float)( cy + r ) } };
+			return aabb;
+		}
+		case b2_polygonShape:
+		{
+			b2Vec2 v = b2TransformWorldPoint( xf, s.polygon.vertices[0] );
+			double lx = v.x, ly = v.y, ux = v.x, uy = v.y;
+			for ( int i = 1; i < s.polygon.count; ++i )
+			{
+				v = b2TransformWorldPoint( xf, s.polygon.vertices[i] );
+				double vx = v.x, vy = v.y;
+				lx = vx < lx ? vx : lx;
+				ly = vy < ly ? vy : ly;
+				ux = vx > ux ? vx : ux;
+				uy = vy > uy ? vy : uy;
+			}
+			double r = (double)s.polygon.radius + (double)extra;
+			b2AABB aabb = { { (float)( lx - r ), (float)( ly - r ) }, { (float)( ux + r ), (float)( uy + r ) } };
+			return aabb;
+		}
+		case b2_segmentShape:
+		case b2_chainSegmentShape:
+		{
+			const b2Segment& seg = s.type == b2_segmentShape ? s.segment : s.chainSegment.segment;
+			b2Vec2 v1 = b2TransformWorldPoint( xf, seg.point1 );
+			b2Vec2 v2 = b2TransformWorldPoint( xf, seg.point2 );
+			double e = (double)extra;
+			double v1x = v1.x, v1y = v1.y, v2x = v2.x, v2y = v2.y;
+			b2AABB aabb = { { (float)( ( v1x < v2x ? v1x : v2x ) - e ), (float)( ( v1y < v2y ? v1y : v2y ) - e ) },
+							{ (float)( ( v1x > v2x ? v1x : v2x ) + e ), (float)( ( v1y > v2y ? v1y : v2y ) + e ) } };
+			return aabb;
+		}
+		default:
+		{
+			b2AABB empty = { { 0.0f, 0.0f }, { 0.0f, 0.0f } };
+			return empty;
+		}
+	}
+}
+
+struct FinalizeArgs
+{
+	const int* awake;
+	int count;
+	BodySim* sims;
+	BodyState* states;
+	BodyAux* aux;
+	ShapeDev* shapes;
+	StepParams p;
+	int* counters; // [0] enlarged bodies, [1] enlarged shapes, [2] fast bodies
+	int* enlargedBodies;
+	int* enlargedShapes;
+	b2AABB* enlargedShapeAABBs;
+	float* sleepTimes;
+};
+
+// Body finalization (reference src/solver.c:553-743).
+__global__ void FinalizeBodiesKernel( FinalizeArgs a )
+{
+	int simIndex = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( simIndex >= a.count )
+	{
+		return;
+	}
+
+	int bodyId = a.awake[simIndex];
+	BodyState* state = a.states + bodyId;
+	BodySim* sim = a.sims + bodyId;
+	BodyAux* aux = a.aux + bodyId;
+
+	b2Vec2 v = state->linearVelocity;
+	float w = state->angularVelocity;
+
+	sim->center = b2OffsetPos( sim->center, state->deltaPosition );
+	sim->transform.q = b2NormalizeRot( b2MulRot( state->deltaRotation, sim->transform.q ) );
+
+	float maxVelocity = b2Length( v ) + b2AbsFloat( w ) * sim->maxExtent;
+	float maxDeltaPosition = b2Length( state->deltaPosition ) + b2AbsFloat( state->deltaRotation.s ) * sim->maxExtent;
+	float positionSleepFactor = 0.5f;
+	float sleepVelocity = b2MaxFloat( maxVelocity, positionSleepFactor * a.p.inv_dt * maxDeltaPosition );
+
+	state->deltaPosition = { 0.0f, 0.0f };
+	state->deltaRotation = { 1.0f, 0.0f };
+
+	sim->transform.p = b2OffsetPos( sim->center, b2Neg( b2RotateVector( sim->transform.q, sim->localCenter ) ) );
+
+	sim->force = { 0.0f, 0.0f };
+	sim->torque = 0.0f;
+
+	uint32_t bodyFlags = aux->flags;
+	bodyFlags &= ~kTransientFlags;
+	bodyFlags |= ( sim->flags & ( kIsSpeedCapped | kHadTimeOfImpact ) );
+	bodyFlags |= ( state->flags & ( kIsSpeedCapped | kHadTimeOfImpact ) );
+	sim->flags &= ~kTransientFlags;
+	state->flags &= ~kTransientFlags;
+
+	float sleepTime = aux->sleepTime;
+	if ( a.p.enableSleep == 0 || ( bodyFlags & kEnableSleep ) == 0 || sleepVelocity > aux->sleepThreshold )
+	{
+		sleepTime = 0.0f;
+		float maxMotion = b2MaxFloat( maxDeltaPosition, maxVelocity * a.p.dt );
+		if ( aux->type == b2_dynamicBody && a.p.enableContinuous && maxMotion > aux->safetyFactor * sim->minExtent )
+		{
+			bodyFlags |= kIsFast;
+			atomicAdd( a.counters + 2, 1 );
+		}
+		else
+		{
+			sim->center0 = sim->center;
+			sim->rotation0 = sim->transform.q;
+		}
+	}
+	else
+	{
+		sim->center0 = sim->center;
+		sim->rotation0 = sim->transform.q;
+		sleepTime += a.p.dt;
+	}
+
+	aux->sleepTime = sleepTime;
+	aux->flags = bodyFlags;
+	if ( a.sleepTimes != nullptr )
+	{
+		a.sleepTimes[simIndex] = sleepTime;
+	}
+
+	b2Transform transform = sim->transform;
+	bool isFast = ( bodyFlags & kIsFast ) != 0;
+	bool enlarged = false;
+	int shapeId = sim->headShapeId;
+	while ( shapeId != B2_NULL_INDEX )
+	{
+		ShapeDev* shape = a.shapes + shapeId;
+		if ( isFast )
+		{
+			enlarged = true;
+		}
+		else
+		{
+			b2AABB aabb = FatShapeAABB( *shape, transform, kSpeculativeDistance );
+			shape->aabb = aabb;
+			shape->enlargedAABB = 0;
+			if ( b2AABB_Contains( shape->fatAABB, aabb ) == false )
+			{
+				float margin = shape->aabbMargin;
+				b2AABB fatAABB;
+				fatAABB.lowerBound.x = aabb.lowerBound.x - margin;
+				fatAABB.lowerBound.y = aabb.lowerBound.y - margin;
+				fatAABB.upperBound.x = aabb.upperBound.x + margin;
+				fatAABB.upperBound.y = aabb.upperBound.y + margin;
+				shape->fatAABB = fatAABB;
+				shape->enlargedAABB = 1;
+				enlarged = true;
+				int slot = atomicAdd( a.counters + 1, 1 );
+				a.enlargedShapes[slot] = shapeId;
+				a.enlargedShapeAABBs[slot] = fatAABB;
+			}
+		}
+		shapeId = shape->nextShapeId;
+	}
+
+	aux->enlarged = enlarged ? 1 : 0;
+	if ( enlarged )
+	{
+		int slot = atomicAdd( a.counters + 0, 1 );
+		a.enlargedBodies[slot] = simIndex;
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Device buffers
+
+template <typename T>
+struct DevArray
+{
+	T* ptr = nullptr;
+	size_t capacity = 0;
+
+	void Reserve( size_t n, bool keep, hipStream_t stream, size_t used )
+	{
+		if ( n <= capacity )
+		{
+			return;
+		}
+		size_t newCap = std::max<size_t>( n, capacity + capacity / 2 );
+		newCap = std::max<size_t>( newCap, 16 );
+		T* p = nullptr;
+		HIP_CHECK( hipMalloc( &p, newCap * sizeof( T ) ) );
+		if ( keep && ptr != nullptr && used > 0 )
+		{
+			HIP_CHECK( hipMemcpyAsync( p, ptr, used * sizeof( T ), hipMemcpyDeviceToDevice, stream ) );
+			HIP_CHECK( hipStreamSynchronize( stream ) );
+		}
+		if ( ptr != nullptr )
+		{
+			HIP_CHECK( hipFree( ptr ) );
+		}
+		ptr = p;
+		capacity = newCap;
+	}
+
+	void Free()
+	{
+		if ( ptr != nullptr )
+		{
+			(void)hipFree( ptr );
+		}
+		ptr = nullptr;
+		capacity = 0;
+	}
+};
+
+struct DeviceWorld::Impl
+{
+	hipStream_t stream = nullptr;
+
+	DevArray<BodySim> sims;
+	DevArray<BodyState> states;
+	DevArray<BodyAux> aux;
+	int bodyCount = 0;
+
+	DevArray<ShapeDev> shapes;
+	int shapeCount = 0;
+
+	DevArray<ContactSim> contacts;
+	int contactCapacityUsed = 0;
+
+	DevArray<JointSim> joints;
+	int jointCount = 0;
+
+	DevArray<ContactInit> inits;
+
+	// work lists
+	DevArray<int> awakeBodies;
+	DevArray<int> collideList;
+	DevArray<int> jointList;
+	DevArray<int> changes;
+	SolverLists lists;
+
+	// constraints
+	DevArray<float> cfields;
+	DevArray<int> cints;
+	size_t constraintCapacity = 0;
+	ConstraintSoA soa = {};
+
+	// finalize outputs
+	DevArray<int> enlargedBodies;
+	DevArray<int> enlargedShapes;
+	DevArray<b2AABB> enlargedShapeAABBs;
+	DevArray<float> sleepTimes;
+
+	// counters: [0..1] collide, [4..6] finalize
+	int* counters = nullptr;
+	int* hostCounters = nullptr;
+
+	hipEvent_t ev[4] = {};
+
+	void EnsureConstraints( size_t n )
+	{
+		if ( n <= constraintCapacity )
+		{
+			return;
+		}
+		size_t cap = std::max<size_t>( n, constraintCapacity + constraintCapacity / 2 );
+		cap = std::max<size_t>( cap, 64 );
+		cfields.Free();
+		cints.Free();
+		cfields.Reserve( cap * kcFieldCount, false, stream, 0 );
+		cints.Reserve( cap * 4, false, stream, 0 );
+		constraintCapacity = cap;
+		for ( int f = 0; f < kcFieldCount; ++f )
+		{
+			soa.f[f] = cfields.ptr + f * cap;
+		}
+		soa.bodyA = cints.ptr;
+		soa.bodyB = cints.ptr + cap;
+		soa.contactId = cints.ptr + 2 * cap;
+		soa.pointCount = cints.ptr + 3 * cap;
+	}
+};
+
+DeviceWorld::DeviceWorld()
+{
+	impl = new Impl();
+	HIP_CHECK( hipStreamCreateWithFlags( &impl->stream, hipStreamNonBlocking ) );
+	HIP_CHECK( hipMalloc( &impl->counters, 16 * sizeof( int ) ) );
+	HIP_CHECK( hipHostMalloc( &impl->hostCounters, 16 * sizeof( int ), hipHostMallocDefault ) );
+	for ( int i = 0; i < 4; ++i )
+	{
+		HIP_CHECK( hipEventCreate( &impl->ev[i] ) );
+	}
+	stream = impl->stream;
+}
+
+DeviceWorld::~DeviceWorld()
+{
+	if ( impl == nullptr )
+	{
+		return;
+	}
+	(void)hipStreamSynchronize( impl->stream );
+	impl->sims.Free();
+	impl->states.Free();
+	impl->aux.Free();
+	impl->shapes.Free();
+	impl->contacts.Free();
+	impl->joints.Free();
+	impl->inits.Free();
+	impl->awakeBodies.Free();
+	impl->collideList.Free();
+	impl->jointList.Free();
+	impl->changes.Free();
+	impl->cfields.Free();
+	impl->cints.Free();
+	impl->enlargedBodies.Free();
+	impl->enlargedShapes.Free();
+	impl->enlargedShapeAABBs.Free();
+	impl->sleepTimes.Free();
+	(void)hipFree( impl->counters );
+	(void)hipHostFree( impl->hostCounters );
+	for ( int i = 0; i < 4; ++i )
+	{
+		(void)hipEventDestroy( impl->ev[i] );
+	}
+	(void)hipStreamDestroy( impl->stream );
+	delete impl;
+}
+
+void DeviceWorld::UploadBodies( const World& world )
+{
+	int n = (int)world.bodySims.size();
+	impl->sims.Reserve( n, false, impl->stream, 0 );
+	impl->states.Reserve( n, false, impl->stream, 0 );
+	impl->aux.Reserve( n, false, impl->stream, 0 );
+	if ( n > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( impl->sims.ptr, world.bodySims.data(), n * sizeof( BodySim ), hipMemcpyHostToDevice, impl->stream ) );
+		HIP_CHECK( hipMemcpyAsync( impl->states.ptr, world.bodyStates.data(), n * sizeof( BodyState ), hipMemcpyHostToDevice,
+								   impl->stream ) );
+		HIP_CHECK( hipMemcpyAsync( impl->aux.ptr, world.bodyAux.data(), n * sizeof( BodyAux ), hipMemcpyHostToDevice, impl->stream ) );
+	}
+	impl->bodyCount = n;
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::DownloadBodies( World& world )
+{
+	int n = impl->bodyCount;
+	B2G_CHECK( n <= (int)world.bodySims.size() );
+	if ( n > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( world.bodySims.data(), impl->sims.ptr, n * sizeof( BodySim ), hipMemcpyDeviceToHost, impl->stream ) );
+		HIP_CHECK( hipMemcpyAsync( world.bodyStates.data(), impl->states.ptr, n * sizeof( BodyState ), hipMemcpyDeviceToHost,
+								   impl->stream ) );
+		HIP_CHECK( hipMemcpyAsync( world.bodyAux.data(), impl->aux.ptr, n * sizeof( BodyAux ), hipMemcpyDeviceToHost, impl->stream ) );
+	}
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::UploadShapes( const World& world )
+{
+	int n = (int)world.shapeData.size();
+	impl->shapes.Reserve( n, false, impl->stream, 0 );
+	if ( n > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( impl->shapes.ptr, world.shapeData.data(), n * sizeof( ShapeDev ), hipMemcpyHostToDevice,
+								   impl->stream ) );
+	}
+	impl->shapeCount = n;
+	impl->enlargedShapes.Reserve( std::max<size_t>( n, 1 ), false, impl->stream, 0 );
+	impl->enlargedShapeAABBs.Reserve( std::max<size_t>( n, 1 ), false, impl->stream, 0 );
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::DownloadShapes( World& world )
+{
+	int n = impl->shapeCount;
+	B2G_CHECK( n <= (int)world.shapeData.size() );
+	if ( n > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( world.shapeData.data(), impl->shapes.ptr, n * sizeof( ShapeDev ), hipMemcpyDeviceToHost,
+								   impl->stream ) );
+	}
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::UploadJoints( const World& world )
+{
+	int n = (int)world.jointSims.size();
+	impl->joints.Reserve( n, false, impl->stream, 0 );
+	if ( n > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( impl->joints.ptr, world.jointSims.data(), n * sizeof( JointSim ), hipMemcpyHostToDevice,
+								   impl->stream ) );
+	}
+	impl->jointCount = n;
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::DownloadJoints( World& world )
+{
+	int n = impl->jointCount;
+	B2G_CHECK( n <= (int)world.jointSims.size() );
+	if ( n > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( world.jointSims.data(), impl->joints.ptr, n * sizeof( JointSim ), hipMemcpyDeviceToHost,
+								   impl->stream ) );
+	}
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::UploadContactInits( const std::vector<ContactInit>& inits, int contactCapacity )
+{
+	impl->contacts.Reserve( contactCapacity, true, impl->stream, impl->contactCapacityUsed );
+	impl->contactCapacityUsed = std::max( impl->contactCapacityUsed, contactCapacity );
+
+	int n = (int)inits.size();
+	if ( n == 0 )
+	{
+		return;
+	}
+	B2G_CHECK( impl->shapeCount > 0 );
+
+	impl->inits.Reserve( n, false, impl->stream, 0 );
+	HIP_CHECK( hipMemcpyAsync( impl->inits.ptr, inits.data(), n * sizeof( ContactInit ), hipMemcpyHostToDevice, impl->stream ) );
+	InitContactsKernel<<<Blocks( n ), kBlock, 0, impl->stream>>>( impl->contacts.ptr, impl->inits.ptr, n );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::DownloadContact( int contactId, ContactSim* out )
+{
+	HIP_CHECK( hipMemcpyAsync( out, impl->contacts.ptr + contactId, sizeof( ContactSim ), hipMemcpyDeviceToHost, impl->stream ) );
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+template <typename T>
+static void UploadVector( DevArray<T>& dst, const std::vector<T>& src, hipStream_t stream )
+{
+	dst.Reserve( std::max<size_t>( src.size(), 1 ), false, stream, 0 );
+	if ( src.empty() == false )
+	{
+		HIP_CHECK( hipMemcpyAsync( dst.ptr, src.data(), src.size() * sizeof( T ), hipMemcpyHostToDevice, stream ) );
+	}
+}
+
+void DeviceWorld::UploadLists( const SolverLists& lists )
+{
+	impl->lists = lists;
+	UploadVector( impl->awakeBodies, lists.awakeBodies, impl->stream );
+	UploadVector( impl->collideList, lists.collideContacts, impl->stream );
+	UploadVector( impl->jointList, lists.jointList, impl->stream );
+
+	size_t nc = lists.constraintContacts.size();
+	impl->EnsureConstraints( std::max<size_t>( nc, 1 ) );
+	if ( nc > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( impl->soa.contactId, lists.constraintContacts.data(), nc * sizeof( int ), hipMemcpyHostToDevice,
+								   impl->stream ) );
+	}
+
+	size_t nb = lists.awakeBodies.size();
+	impl->enlargedBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
+	impl->sleepTimes.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
+	impl->changes.Reserve( std::max<size_t>( lists.collideContacts.size(), 1 ), false, impl->stream, 0 );
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
+{
+	result->changes.clear();
+	result->recycledCount = 0;
+
+	int n = (int)impl->lists.collideContacts.size();
+	if ( n == 0 )
+	{
+		return;
+	}
+
+	hipStream_t s = impl->stream;
+	HIP_CHECK( hipEventRecord( impl->ev[0], s ) );
+	HIP_CHECK( hipMemsetAsync( impl->counters, 0, 2 * sizeof( int ), s ) );
+
+	CollideArgs a;
+	a.list = impl->collideList.ptr;
+	a.count = n;
+	a.contacts = impl->contacts.ptr;
+	a.shapes = impl->shapes.ptr;
+	a.sims = impl->sims.ptr;
+	a.aux = impl->aux.ptr;
+	a.changes = impl->changes.ptr;
+	a.counters = impl->counters;
+	a.recycleDistance = params.contactRecycleDistance;
+	CollideKernel<<<Blocks( n ), kBlock, 0, s>>>( a );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipEventRecord( impl->ev[1], s ) );
+
+	HIP_CHECK( hipMemcpyAsync( impl->hostCounters, impl->counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+
+	int changeCount = impl->hostCounters[0];
+	result->recycledCount = impl->hostCounters[1];
+	if ( changeCount > 0 )
+	{
+		result->changes.resize( changeCount );
+		HIP_CHECK( hipMemcpyAsync( result->changes.data(), impl->changes.ptr, changeCount * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+		std::sort( result->changes.begin(), result->changes.end() );
+	}
+
+	HIP_CHECK( hipEventElapsedTime( &lastCollideMs, impl->ev[0], impl->ev[1] ) );
+}
+
+void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool needSleepTimes, FinalizeResult* result )
+{
+	const SolverLists& L = impl->lists;
+	hipStream_t s = impl->stream;
+	int nb = (int)L.awakeBodies.size();
+	int nc = (int)L.constraintContacts.size();
+	int nj = (int)L.jointList.size();
+
+	HIP_CHECK( hipEventRecord( impl->ev[2], s ) );
+
+	if ( nc > 0 )
+	{
+		PrepareArgs pa;
+		pa.c = impl->soa;
+		pa.count = nc;
+		pa.contacts = impl->contacts.ptr;
+		pa.sims = impl->sims.ptr;
+		pa.states = impl->states.ptr;
+		pa.aux = impl->aux.ptr;
+		pa.p = params;
+		PrepareContactsKernel<<<Blocks( nc ), kBlock, 0, s>>>( pa );
+		HIP_CHECK( hipGetLastError() );
+	}
+
+	if ( nj > 0 )
+	{
+		PrepareJointArgs ja;
+		ja.joints = impl->joints.ptr;
+		ja.list = impl->jointList.ptr;
+		ja.count = nj;
+		ja.sims = impl->sims.ptr;
+		ja.p = params;
+		PrepareJointsKernel<<<Blocks( nj ), kBlock, 0, s>>>( ja );
+		HIP_CHECK( hipGetLastError() );
+	}
+
+	BodyArgs ba;
+	ba.awake = impl->awakeBodies.ptr;
+	ba.count = nb;
+	ba.sims = impl->sims.ptr;
+	ba.states = impl->states.ptr;
+	ba.aux = impl->aux.ptr;
+	ba.p = params;
+
+	ColorArgs ca;
+	ca.c = impl->soa;
+	ca.joints = impl->joints.ptr;
+	ca.jointList = impl->jointList.ptr;
+	ca.states = impl->states.ptr;
+	ca.p = params;
+
+	auto runStage = [&]( int stage ) {
+		if ( L.overflowContactCount > 0 || L.overflowJointCount > 0 )
+		{
+			ca.stage = stage;
+			ca.jointStart = L.overflowJointStart;
+			ca.jointCount = L.overflowJointCount;
+			ca.contactStart = L.overflowContactStart;
+			ca.contactCount = L.overflowContactCount;
+			OverflowKernel<<<1, 64, 0, s>>>( ca );
+			HIP_CHECK( hipGetLastError() );
+		}
+
+		for ( int c = 0; c < L.activeColorCount; ++c )
+		{
+			ca.stage = stage;
+			ca.jointStart = L.colorJointStart[c];
+			ca.jointCount = L.colorJointStart[c + 1] - L.colorJointStart[c];
+			ca.contactStart = L.colorContactStart[c];
+			ca.contactCount = L.colorContactStart[c + 1] - L.colorContactStart[c];
+			int lanes = ca.jointCount + ca.contactCount;
+			if ( lanes == 0 )
+			{
+				continue;
+			}
+			if ( stage == kStageRestitution && ca.contactCount == 0 )
+			{
+				continue;
+			}
+			ColorKernel<<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+			HIP_CHECK( hipGetLastError() );
+		}
+	};
+
+	for ( int sub = 0; sub < params.subStepCount; ++sub )
+	{
+		IntegrateVelocitiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba );
+		HIP_CHECK( hipGetLastError() );
+		runStage( kStageWarmStart );
+		runStage( kStageSolve );
+		IntegratePositionsKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba );
+		HIP_CHECK( hipGetLastError() );
+		runStage( kStageRelax );
+	}
+
+	if ( anyRestitution )
+	{
+		runStage( kStageRestitution );
+	}
+
+	if ( nc > 0 )
+	{
+		StoreArgs sa;
+		sa.c = impl->soa;
+		sa.contacts = impl->contacts.ptr;
+		sa.coloredCount = L.overflowContactStart;
+		sa.totalCount = nc;
+		StoreImpulsesKernel<<<Blocks( nc ), kBlock, 0, s>>>( sa );
+		HIP_CHECK( hipGetLastError() );
+	}
+
+	HIP_CHECK( hipMemsetAsync( impl->counters + 4, 0, 3 * sizeof( int ), s ) );
+
+	FinalizeArgs fa;
+	fa.awake = impl->awakeBodies.ptr;
+	fa.count = nb;
+	fa.sims = impl->sims.ptr;
+	fa.states = impl->states.ptr;
+	fa.aux = impl->aux.ptr;
+	fa.shapes = impl->shapes.ptr;
+	fa.p = params;
+	fa.counters = impl->counters + 4;
+	fa.enlargedBodies = impl->enlargedBodies.ptr;
+	fa.enlargedShapes = impl->enlargedShapes.ptr;
+	fa.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
+	fa.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
+	FinalizeBodiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( fa );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
+
+	HIP_CHECK( hipMemcpyAsync( impl->hostCounters + 4, impl->counters + 4, 3 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+
+	int enlargedBodyCount = impl->hostCounters[4];
+	int enlargedShapeCount = impl->hostCounters[5];
+	result->fastBodyCount = impl->hostCounters[6];
+
+	result->enlargedBodies.resize( enlargedBodyCount );
+	result->enlargedShapes.resize( enlargedShapeCount );
+	result->enlargedShapeAABBs.resize( enlargedShapeCount );
+	if ( enlargedBodyCount > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( result->enlargedBodies.data(), impl->enlargedBodies.ptr, enlargedBodyCount * sizeof( int ),
+								   hipMemcpyDeviceToHost, s ) );
+	}
+	if ( enlargedShapeCount > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( result->enlargedShapes.data(), impl->enlargedShapes.ptr, enlargedShapeCount * sizeof( int ),
+								   hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipMemcpyAsync( result->enlargedShapeAABBs.data(), impl->enlargedShapeAABBs.ptr,
+								   enlargedShapeCount * sizeof( b2AABB ), hipMemcpyDeviceToHost, s ) );
+	}
+	if ( needSleepTimes && nb > 0 )
+	{
+		result->sleepTimes.resize( nb );
+		HIP_CHECK( hipMemcpyAsync( result->sleepTimes.data(), impl->sleepTimes.ptr, nb * sizeof( float ), hipMemcpyDeviceToHost, s ) );
+	}
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	std::sort( result->enlargedBodies.begin(), result->enlargedBodies.end() );
+
+	HIP_CHECK( hipEventElapsedTime( &lastSolveMs, impl->ev[2], impl->ev[3] ) );
+}
+
+} // namespace b2g

@@ -1,0 +1,795 @@
+// GPU solver arithmetic for one contact or joint constraint.
+//
+// These functions are the per-lane bodies of the constraint kernels. The colored path reproduces
+// the reference's wide (SIMD lane) arithmetic, including its operation order, its a >= b / a <= b
+// min-max semantics and the fact that a one-point manifold still runs the second point with zeros
+// (reference src/contact_solver.c:936-1063, 1811-2228). The overflow path reproduces the scalar
+// overflow solver (src/contact_solver.c:24-545), whose expressions differ from the wide ones.
+// Revolute joints follow src/revolute_joint.c:217-500.
+#pragma once
+
+#include "layout.h"
+
+namespace b2g
+{
+
+#define B2G_DEVI __device__ __forceinline__
+
+struct ConstraintSoA
+{
+	float* f[kcFieldCount];
+	int* bodyA;
+	int* bodyB;
+	int* contactId;
+	int* pointCount;
+};
+
+B2G_DEVI float MaxW( float a, float b )
+{
+	return a >= b ? a : b;
+}
+
+B2G_DEVI float MinW( float a, float b )
+{
+	return a <= b ? a : b;
+}
+
+B2G_DEVI Softness MakeSoft( float hertz, float zeta, float h )
+{
+	if ( hertz == 0.0f )
+	{
+		return Softness{ 0.0f, 0.0f, 0.0f };
+	}
+	float omega = 2.0f * B2_PI * hertz;
+	float a1 = 2.0f * zeta + h * omega;
+	float a2 = h * omega * a1;
+	float a3 = 1.0f / ( 1.0f + a2 );
+	return Softness{ omega / a1, a2 * a3, a3 };
+}
+
+// Deterministic arctangent (reference src/math_functions.c:96-136).
+B2G_DEVI float Atan2( float y, float x )
+{
+	if ( x == 0.0f && y == 0.0f )
+	{
+		return 0.0f;
+	}
+	float ax = b2AbsFloat( x );
+	float ay = b2AbsFloat( y );
+	float mx = b2MaxFloat( ay, ax );
+	float mn = b2MinFloat( ay, ax );
+	float a = mn / mx;
+	float s = a * a;
+	float c = s * a;
+	float q = s * s;
+	float r = 0.024840285f * q + 0.18681418f;
+	float t = -0.094097948f * q - 0.33213072f;
+	r = r * s + t;
+	r = r * c + a;
+	if ( ay > ax )
+	{
+		r = 1.57079637f - r;
+	}
+	if ( x < 0 )
+	{
+		r = 3.14159274f - r;
+	}
+	if ( y < 0 )
+	{
+		r = -r;
+	}
+	return r;
+}
+
+struct LaneBody
+{
+	float vx, vy, w;
+	uint32_t flags;
+	float dpx, dpy, dqc, dqs;
+};
+
+B2G_DEVI LaneBody LoadBody( const BodyState* states, int index )
+{
+	const float4* p = reinterpret_cast<const float4*>( states + index );
+	float4 a = p[0];
+	float4 b = p[1];
+	LaneBody r;
+	r.vx = a.x;
+	r.vy = a.y;
+	r.w = a.z;
+	r.flags = __float_as_uint( a.w );
+	r.dpx = b.x;
+	r.dpy = b.y;
+	r.dqc = b.z;
+	r.dqs = b.w;
+	return r;
+}
+
+B2G_DEVI void StoreVelocity( BodyState* states, int index, const LaneBody& b )
+{
+	if ( b.flags & kDynamicFlag )
+	{
+		BodyState* s = states + index;
+		s->linearVelocity.x = b.vx;
+		s->linearVelocity.y = b.vy;
+		s->angularVelocity = b.w;
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Colored (wide-lane) contact arithmetic
+
+B2G_DEVI void WarmStartContact( const ConstraintSoA& c, int i, BodyState* states )
+{
+	int ia = c.bodyA[i];
+	int ib = c.bodyB[i];
+	LaneBody bA = LoadBody( states, ia );
+	LaneBody bB = LoadBody( states, ib );
+
+	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
+	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
+	float nX = c.f[kcNormalX][i], nY = c.f[kcNormalY][i];
+	float tX = nY;
+	float tY = 0.0f - nX;
+
+	{
+		float rAx = c.f[kcAnchorA1X][i], rAy = c.f[kcAnchorA1Y][i];
+		float rBx = c.f[kcAnchorB1X][i], rBy = c.f[kcAnchorB1Y][i];
+		float ni = c.f[kcNormalImpulse1][i], ti = c.f[kcTangentImpulse1][i];
+		float Px = ni * nX + ti * tX;
+		float Py = ni * nY + ti * tY;
+		bA.w = bA.w - iA * ( rAx * Py - rAy * Px );
+		bA.vx = bA.vx - mA * Px;
+		bA.vy = bA.vy - mA * Py;
+		bB.w = bB.w + iB * ( rBx * Py - rBy * Px );
+		bB.vx = bB.vx + mB * Px;
+		bB.vy = bB.vy + mB * Py;
+		c.f[kcTotalNormalImpulse1][i] = c.f[kcTotalNormalImpulse1][i] + ni;
+	}
+
+	{
+		float rAx = c.f[kcAnchorA2X][i], rAy = c.f[kcAnchorA2Y][i];
+		float rBx = c.f[kcAnchorB2X][i], rBy = c.f[kcAnchorB2Y][i];
+		float ni = c.f[kcNormalImpulse2][i], ti = c.f[kcTangentImpulse2][i];
+		float Px = ni * nX + ti * tX;
+		float Py = ni * nY + ti * tY;
+		bA.w = bA.w - iA * ( rAx * Py - rAy * Px );
+		bA.vx = bA.vx - mA * Px;
+		bA.vy = bA.vy - mA * Py;
+		bB.w = bB.w + iB * ( rBx * Py - rBy * Px );
+		bB.vx = bB.vx + mB * Px;
+		bB.vy = bB.vy + mB * Py;
+		c.f[kcTotalNormalImpulse2][i] = c.f[kcTotalNormalImpulse2][i] + ni;
+	}
+
+	float rolling = c.f[kcRollingImpulse][i];
+	bA.w = bA.w - iA * rolling;
+	bB.w = bB.w + iB * rolling;
+
+	StoreVelocity( states, ia, bA );
+	StoreVelocity( states, ib, bB );
+}
+
+// One normal point of the wide solve. Returns the new accumulated impulse for the rolling term.
+B2G_DEVI float SolveNormalPoint( const ConstraintSoA& c, int i, LaneBody& bA, LaneBody& bB, float mA, float mB, float iA,
+								 float iB, float nX, float nY, float dpX, float dpY, float biasRate, float massScale,
+								 float impulseScale, float inv_h, float contactSpeed, int fAx, int fAy, int fBx, int fBy,
+								 int fBase, int fNormalImpulse, int fTotal, int fNormalMass )
+{
+	float rAx = c.f[fAx][i], rAy = c.f[fAy][i];
+	float rBx = c.f[fBx][i], rBy = c.f[fBy][i];
+
+	// moving anchors for the current separation
+	float rsAx = bA.dqc * rAx - bA.dqs * rAy;
+	float rsAy = bA.dqs * rAx + bA.dqc * rAy;
+	float rsBx = bB.dqc * rBx - bB.dqs * rBy;
+	float rsBy = bB.dqs * rBx + bB.dqc * rBy;
+
+	float dsx = dpX + ( rsBx - rsAx );
+	float dsy = dpY + ( rsBy - rsAy );
+	float s = ( nX * dsx + nY * dsy ) + c.f[fBase][i];
+
+	bool mask = s > 0.0f;
+	float specBias = s * inv_h;
+	float softBias = MaxW( biasRate * s, contactSpeed );
+	float bias = mask ? specBias : softBias;
+	float pointMassScale = mask ? 1.0f : massScale;
+	float pointImpulseScale = mask ? 0.0f : impulseScale;
+
+	float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
+	float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
+	float vn = dvx * nX + dvy * nY;
+
+	float normalImpulse = c.f[fNormalImpulse][i];
+	float negImpulse = c.f[fNormalMass][i] * ( pointMassScale * vn + bias ) + pointImpulseScale * normalImpulse;
+	float newImpulse = MaxW( normalImpulse - negImpulse, 0.0f );
+	float impulse = newImpulse - normalImpulse;
+	c.f[fNormalImpulse][i] = newImpulse;
+	c.f[fTotal][i] = c.f[fTotal][i] + impulse;
+
+	float Px = impulse * nX;
+	float Py = impulse * nY;
+	bA.vx = bA.vx - mA * Px;
+	bA.vy = bA.vy - mA * Py;
+	bA.w = bA.w - iA * ( rAx * Py - rAy * Px );
+	bB.vx = bB.vx + mB * Px;
+	bB.vy = bB.vy + mB * Py;
+	bB.w = bB.w + iB * ( rBx * Py - rBy * Px );
+	return newImpulse;
+}
+
+B2G_DEVI void FrictionPoint( const ConstraintSoA& c, int i, LaneBody& bA, LaneBody& bB, float mA, float mB, float iA, float iB,
+							 float tX, float tY, float friction, float tangentSpeed, int fAx, int fAy, int fBx, int fBy,
+							 int fNormalImpulse, int fTangentImpulse, int fTangentMass )
+{
+	float rAx = c.f[fAx][i], rAy = c.f[fAy][i];
+	float rBx = c.f[fBx][i], rBy = c.f[fBy][i];
+
+	float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
+	float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
+	float vt = dvx * tX + dvy * tY;
+	vt = vt - tangentSpeed;
+
+	float negImpulse = c.f[fTangentMass][i] * vt;
+	float maxFriction = friction * c.f[fNormalImpulse][i];
+	float tangentImpulse = c.f[fTangentImpulse][i];
+	float newImpulse = tangentImpulse - negImpulse;
+	newImpulse = MaxW( 0.0f - maxFriction, MinW( newImpulse, maxFriction ) );
+	float impulse = newImpulse - tangentImpulse;
+	c.f[fTangentImpulse][i] = newImpulse;
+
+	float Px = impulse * tX;
+	float Py = impulse * tY;
+	bA.vx = bA.vx - mA * Px;
+	bA.vy = bA.vy - mA * Py;
+	bA.w = bA.w - iA * ( rAx * Py - rAy * Px );
+	bB.vx = bB.vx + mB * Px;
+	bB.vy = bB.vy + mB * Py;
+	bB.w = bB.w + iB * ( rBx * Py - rBy * Px );
+}
+
+// laneGroupRolling: true when any constraint of this constraint's 4-lane group has rolling
+// resistance (the wide solver tests the whole group).
+B2G_DEVI void SolveContact( const ConstraintSoA& c, int i, BodyState* states, bool useBias, float inv_h, float contactSpeed,
+							bool laneGroupRolling )
+{
+	int ia = c.bodyA[i];
+	int ib = c.bodyB[i];
+	LaneBody bA = LoadBody( states, ia );
+	LaneBody bB = LoadBody( states, ib );
+
+	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
+	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
+	float nX = c.f[kcNormalX][i], nY = c.f[kcNormalY][i];
+
+	float biasRate, massScale, impulseScale;
+	if ( useBias )
+	{
+		biasRate = c.f[kcMassScale][i] * c.f[kcBiasRate][i];
+		massScale = c.f[kcMassScale][i];
+		impulseScale = c.f[kcImpulseScale][i];
+	}
+	else
+	{
+		biasRate = 0.0f;
+		massScale = 1.0f;
+		impulseScale = 0.0f;
+	}
+
+	float dpX = bB.dpx - bA.dpx;
+	float dpY = bB.dpy - bA.dpy;
+
+	float totalNormalImpulse = 0.0f;
+	float n1 = SolveNormalPoint( c, i, bA, bB, mA, mB, iA, iB, nX, nY, dpX, dpY, biasRate, massScale, impulseScale, inv_h,
+								 contactSpeed, kcAnchorA1X, kcAnchorA1Y, kcAnchorB1X, kcAnchorB1Y, kcBaseSeparation1,
+								 kcNormalImpulse1, kcTotalNormalImpulse1, kcNormalMass1 );
+	totalNormalImpulse = totalNormalImpulse + n1;
+	float n2 = SolveNormalPoint( c, i, bA, bB, mA, mB, iA, iB, nX, nY, dpX, dpY, biasRate, massScale, impulseScale, inv_h,
+								 contactSpeed, kcAnchorA2X, kcAnchorA2Y, kcAnchorB2X, kcAnchorB2Y, kcBaseSeparation2,
+								 kcNormalImpulse2, kcTotalNormalImpulse2, kcNormalMass2 );
+	totalNormalImpulse = totalNormalImpulse + n2;
+
+	if ( useBias == false )
+	{
+		if ( laneGroupRolling )
+		{
+			float deltaLambda = c.f[kcRollingMass][i] * ( bA.w - bB.w );
+			float lambda = c.f[kcRollingImpulse][i];
+			float maxLambda = c.f[kcRollingResistance][i] * totalNormalImpulse;
+			float rolling = b2ClampFloat( lambda + deltaLambda, -maxLambda, maxLambda );
+			c.f[kcRollingImpulse][i] = rolling;
+			deltaLambda = rolling - lambda;
+			bA.w = bA.w - iA * deltaLambda;
+			bB.w = bB.w + iB * deltaLambda;
+		}
+
+		float tX = nY;
+		float tY = 0.0f - nX;
+		float friction = c.f[kcFriction][i];
+		float tangentSpeed = c.f[kcTangentSpeed][i];
+		FrictionPoint( c, i, bA, bB, mA, mB, iA, iB, tX, tY, friction, tangentSpeed, kcAnchorA1X, kcAnchorA1Y, kcAnchorB1X,
+					   kcAnchorB1Y, kcNormalImpulse1, kcTangentImpulse1, kcTangentMass1 );
+		FrictionPoint( c, i, bA, bB, mA, mB, iA, iB, tX, tY, friction, tangentSpeed, kcAnchorA2X, kcAnchorA2Y, kcAnchorB2X,
+					   kcAnchorB2Y, kcNormalImpulse2, kcTangentImpulse2, kcTangentMass2 );
+	}
+
+	StoreVelocity( states, ia, bA );
+	StoreVelocity( states, ib, bB );
+}
+
+B2G_DEVI void RestitutionPoint( const ConstraintSoA& c, int i, LaneBody& bA, LaneBody& bB, float mA, float mB, float iA,
+								float iB, float nX, float nY, float threshold, float restitution, bool restitutionMask,
+								int fAx, int fAy, int fBx, int fBy, int fRelVel, int fTotal, int fNormalMass,
+								int fNormalImpulse )
+{
+	float relativeVelocity = c.f[fRelVel][i];
+	bool mask1 = ( relativeVelocity + threshold ) > 0.0f;
+	bool mask2 = c.f[fTotal][i] == 0.0f;
+	bool mask = mask1 || mask2 || restitutionMask;
+	float mass = mask ? 0.0f : c.f[fNormalMass][i];
+
+	float rAx = c.f[fAx][i], rAy = c.f[fAy][i];
+	float rBx = c.f[fBx][i], rBy = c.f[fBy][i];
+
+	float dvx = ( bB.vx - bB.w * rBy ) - ( bA.vx - bA.w * rAy );
+	float dvy = ( bB.vy + bB.w * rBx ) - ( bA.vy + bA.w * rAx );
+	float vn = dvx * nX + dvy * nY;
+
+	float negImpulse = mass * ( vn + restitution * relativeVelocity );
+	float normalImpulse = c.f[fNormalImpulse][i];
+	float newImpulse = MaxW( normalImpulse - negImpulse, 0.0f );
+	float deltaImpulse = newImpulse - normalImpulse;
+	c.f[fNormalImpulse][i] = newImpulse;
+	c.f[fTotal][i] = c.f[fTotal][i] + deltaImpulse;
+
+	float Px = deltaImpulse * nX;
+	float Py = deltaImpulse * nY;
+	bA.vx = bA.vx - mA * Px;
+	bA.vy = bA.vy - mA * Py;
+	bA.w = bA.w - iA * ( rAx * Py - rAy * Px );
+	bB.vx = bB.vx + mB * Px;
+	bB.vy = bB.vy + mB * Py;
+	bB.w = bB.w + iB * ( rBx * Py - rBy * Px );
+}
+
+B2G_DEVI void RestitutionContact( const ConstraintSoA& c, int i, BodyState* states, float threshold )
+{
+	int ia = c.bodyA[i];
+	int ib = c.bodyB[i];
+	LaneBody bA = LoadBody( states, ia );
+	LaneBody bB = LoadBody( states, ib );
+
+	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
+	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
+	float nX = c.f[kcNormalX][i], nY = c.f[kcNormalY][i];
+	float restitution = c.f[kcRestitution][i];
+	bool restitutionMask = restitution == 0.0f;
+
+	RestitutionPoint( c, i, bA, bB, mA, mB, iA, iB, nX, nY, threshold, restitution, restitutionMask, kcAnchorA1X, kcAnchorA1Y,
+					  kcAnchorB1X, kcAnchorB1Y, kcRelativeVelocity1, kcTotalNormalImpulse1, kcNormalMass1, kcNormalImpulse1 );
+	RestitutionPoint( c, i, bA, bB, mA, mB, iA, iB, nX, nY, threshold, restitution, restitutionMask, kcAnchorA2X, kcAnchorA2Y,
+					  kcAnchorB2X, kcAnchorB2Y, kcRelativeVelocity2, kcTotalNormalImpulse2, kcNormalMass2, kcNormalImpulse2 );
+
+	StoreVelocity( states, ia, bA );
+	StoreVelocity( states, ib, bB );
+}
+
+// ---------------------------------------------------------------------------------------------
+// Overflow (scalar) contact arithmetic, solved one constraint after another by a single lane.
+
+struct OverflowVel
+{
+	float vx, vy, w;
+};
+
+B2G_DEVI void OverflowWarmStart( const ConstraintSoA& c, int i, BodyState* states )
+{
+	int ia = c.bodyA[i];
+	int ib = c.bodyB[i];
+	BodyState* stateA = states + ia;
+	BodyState* stateB = states + ib;
+
+	b2Vec2 vA = stateA->linearVelocity;
+	float wA = stateA->angularVelocity;
+	b2Vec2 vB = stateB->linearVelocity;
+	float wB = stateB->angularVelocity;
+
+	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
+	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
+	b2Vec2 normal = { c.f[kcNormalX][i], c.f[kcNormalY][i] };
+	b2Vec2 tangent = b2RightPerp( normal );
+	int pointCount = c.pointCount[i];
+
+	for ( int j = 0; j < pointCount; ++j )
+	{
+		int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+		b2Vec2 rA = { c.f[kcAnchorA1X + o][i], c.f[kcAnchorA1Y + o][i] };
+		b2Vec2 rB = { c.f[kcAnchorB1X + o][i], c.f[kcAnchorB1Y + o][i] };
+		float ni = c.f[kcNormalImpulse1 + o][i];
+		float ti = c.f[kcTangentImpulse1 + o][i];
+		b2Vec2 P = b2Add( b2MulSV( ni, normal ), b2MulSV( ti, tangent ) );
+		c.f[kcTotalNormalImpulse1 + o][i] = c.f[kcTotalNormalImpulse1 + o][i] + ni;
+		wA -= iA * b2Cross( rA, P );
+		vA = b2MulAdd( vA, -mA, P );
+		wB += iB * b2Cross( rB, P );
+		vB = b2MulAdd( vB, mB, P );
+	}
+
+	float rolling = c.f[kcRollingImpulse][i];
+	wA -= iA * rolling;
+	wB += iB * rolling;
+
+	if ( stateA->flags & kDynamicFlag )
+	{
+		stateA->linearVelocity = vA;
+		stateA->angularVelocity = wA;
+	}
+	if ( stateB->flags & kDynamicFlag )
+	{
+		stateB->linearVelocity = vB;
+		stateB->angularVelocity = wB;
+	}
+}
+
+B2G_DEVI void OverflowSolve( const ConstraintSoA& c, int i, BodyState* states, bool useBias, float inv_h, float contactSpeed )
+{
+	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
+	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
+
+	int ia = c.bodyA[i];
+	int ib = c.bodyB[i];
+	BodyState* stateA = states + ia;
+	BodyState* stateB = states + ib;
+
+	b2Vec2 vA = stateA->linearVelocity;
+	float wA = stateA->angularVelocity;
+	b2Rot dqA = stateA->deltaRotation;
+	b2Vec2 vB = stateB->linearVelocity;
+	float wB = stateB->angularVelocity;
+	b2Rot dqB = stateB->deltaRotation;
+
+	b2Vec2 dp = b2Sub( stateB->deltaPosition, stateA->deltaPosition );
+
+	b2Vec2 normal = { c.f[kcNormalX][i], c.f[kcNormalY][i] };
+	b2Vec2 tangent = b2RightPerp( normal );
+	float friction = c.f[kcFriction][i];
+	Softness softness = { c.f[kcBiasRate][i], c.f[kcMassScale][i], c.f[kcImpulseScale][i] };
+	int pointCount = c.pointCount[i];
+	float totalNormalImpulse = 0.0f;
+
+	for ( int j = 0; j < pointCount; ++j )
+	{
+		int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+		b2Vec2 rA = { c.f[kcAnchorA1X + o][i], c.f[kcAnchorA1Y + o][i] };
+		b2Vec2 rB = { c.f[kcAnchorB1X + o][i], c.f[kcAnchorB1Y + o][i] };
+
+		b2Vec2 ds = b2Add( dp, b2Sub( b2RotateVector( dqB, rB ), b2RotateVector( dqA, rA ) ) );
+		float s = c.f[kcBaseSeparation1 + o][i] + b2Dot( ds, normal );
+
+		float velocityBias = 0.0f;
+		float massScale = 1.0f;
+		float impulseScale = 0.0f;
+		if ( s > 0.0f )
+		{
+			velocityBias = s * inv_h;
+		}
+		else if ( useBias )
+		{
+			velocityBias = b2MaxFloat( softness.massScale * softness.biasRate * s, -contactSpeed );
+			massScale = softness.massScale;
+			impulseScale = softness.impulseScale;
+		}
+
+		b2Vec2 vrA = b2Add( vA, b2CrossSV( wA, rA ) );
+		b2Vec2 vrB = b2Add( vB, b2CrossSV( wB, rB ) );
+		float vn = b2Dot( b2Sub( vrB, vrA ), normal );
+
+		float normalImpulse = c.f[kcNormalImpulse1 + o][i];
+		float impulse = -c.f[kcNormalMass1 + o][i] * ( massScale * vn + velocityBias ) - impulseScale * normalImpulse;
+		float newImpulse = b2MaxFloat( normalImpulse + impulse, 0.0f );
+		impulse = newImpulse - normalImpulse;
+		c.f[kcNormalImpulse1 + o][i] = newImpulse;
+		c.f[kcTotalNormalImpulse1 + o][i] = c.f[kcTotalNormalImpulse1 + o][i] + impulse;
+		totalNormalImpulse += newImpulse;
+
+		b2Vec2 P = b2MulSV( impulse, normal );
+		vA = b2MulSub( vA, mA, P );
+		wA -= iA * b2Cross( rA, P );
+		vB = b2MulAdd( vB, mB, P );
+		wB += iB * b2Cross( rB, P );
+	}
+
+	if ( useBias == false )
+	{
+		float tangentSpeed = c.f[kcTangentSpeed][i];
+		for ( int j = 0; j < pointCount; ++j )
+		{
+			int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+			b2Vec2 rA = { c.f[kcAnchorA1X + o][i], c.f[kcAnchorA1Y + o][i] };
+			b2Vec2 rB = { c.f[kcAnchorB1X + o][i], c.f[kcAnchorB1Y + o][i] };
+
+			b2Vec2 vrB = b2Add( vB, b2CrossSV( wB, rB ) );
+			b2Vec2 vrA = b2Add( vA, b2CrossSV( wA, rA ) );
+			float vt = b2Dot( b2Sub( vrB, vrA ), tangent ) - tangentSpeed;
+			float impulse = c.f[kcTangentMass1 + o][i] * ( -vt );
+			float maxFriction = friction * c.f[kcNormalImpulse1 + o][i];
+			float tangentImpulse = c.f[kcTangentImpulse1 + o][i];
+			float newImpulse = b2ClampFloat( tangentImpulse + impulse, -maxFriction, maxFriction );
+			impulse = newImpulse - tangentImpulse;
+			c.f[kcTangentImpulse1 + o][i] = newImpulse;
+
+			b2Vec2 P = b2MulSV( impulse, tangent );
+			vA = b2MulSub( vA, mA, P );
+			wA -= iA * b2Cross( rA, P );
+			vB = b2MulAdd( vB, mB, P );
+			wB += iB * b2Cross( rB, P );
+		}
+
+		{
+			float deltaLambda = -c.f[kcRollingMass][i] * ( wB - wA );
+			float lambda = c.f[kcRollingImpulse][i];
+			float maxLambda = c.f[kcRollingResistance][i] * totalNormalImpulse;
+			float rolling = b2ClampFloat( lambda + deltaLambda, -maxLambda, maxLambda );
+			c.f[kcRollingImpulse][i] = rolling;
+			deltaLambda = rolling - lambda;
+			wA -= iA * deltaLambda;
+			wB += iB * deltaLambda;
+		}
+	}
+
+	if ( stateA->flags & kDynamicFlag )
+	{
+		stateA->linearVelocity = vA;
+		stateA->angularVelocity = wA;
+	}
+	if ( stateB->flags & kDynamicFlag )
+	{
+		stateB->linearVelocity = vB;
+		stateB->angularVelocity = wB;
+	}
+}
+
+B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, BodyState* states, float threshold )
+{
+	float restitution = c.f[kcRestitution][i];
+	if ( restitution == 0.0f )
+	{
+		return;
+	}
+
+	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
+	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
+	int ia = c.bodyA[i];
+	int ib = c.bodyB[i];
+	BodyState* stateA = states + ia;
+	BodyState* stateB = states + ib;
+	b2Vec2 vA = stateA->linearVelocity;
+	float wA = stateA->angularVelocity;
+	b2Vec2 vB = stateB->linearVelocity;
+	float wB = stateB->angularVelocity;
+	b2Vec2 normal = { c.f[kcNormalX][i], c.f[kcNormalY][i] };
+	int pointCount = c.pointCount[i];
+
+	for ( int j = 0; j < pointCount; ++j )
+	{
+		int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+		float relativeVelocity = c.f[kcRelativeVelocity1 + o][i];
+		if ( relativeVelocity > -threshold || c.f[kcTotalNormalImpulse1 + o][i] == 0.0f )
+		{
+			continue;
+		}
+
+		b2Vec2 rA = { c.f[kcAnchorA1X + o][i], c.f[kcAnchorA1Y + o][i] };
+		b2Vec2 rB = { c.f[kcAnchorB1X + o][i], c.f[kcAnchorB1Y + o][i] };
+		b2Vec2 vrB = b2Add( vB, b2CrossSV( wB, rB ) );
+		b2Vec2 vrA = b2Add( vA, b2CrossSV( wA, rA ) );
+		float vn = b2Dot( b2Sub( vrB, vrA ), normal );
+
+		float normalImpulse = c.f[kcNormalImpulse1 + o][i];
+		float impulse = -c.f[kcNormalMass1 + o][i] * ( vn + restitution * relativeVelocity );
+		float newImpulse = b2MaxFloat( normalImpulse + impulse, 0.0f );
+		impulse = newImpulse - normalImpulse;
+		c.f[kcNormalImpulse1 + o][i] = newImpulse;
+		c.f[kcTotalNormalImpulse1 + o][i] = c.f[kcTotalNormalImpulse1 + o][i] + impulse;
+
+		b2Vec2 P = b2MulSV( impulse, normal );
+		vA = b2MulSub( vA, mA, P );
+		wA -= iA * b2Cross( rA, P );
+		vB = b2MulAdd( vB, mB, P );
+		wB += iB * b2Cross( rB, P );
+	}
+
+	if ( stateA->flags & kDynamicFlag )
+	{
+		stateA->linearVelocity = vA;
+		stateA->angularVelocity = wA;
+	}
+	if ( stateB->flags & kDynamicFlag )
+	{
+		stateB->linearVelocity = vB;
+		stateB->angularVelocity = wB;
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Revolute joint
+
+B2G_DEVI void WarmStartRevolute( JointSim* joint, BodyState* states )
+{
+	float mA = joint->invMassA;
+	float mB = joint->invMassB;
+	float iA = joint->invIA;
+	float iB = joint->invIB;
+
+	BodyState* stateA = states + joint->indexA;
+	BodyState* stateB = states + joint->indexB;
+
+	b2Vec2 rA = b2RotateVector( stateA->deltaRotation, joint->frameA.p );
+	b2Vec2 rB = b2RotateVector( stateB->deltaRotation, joint->frameB.p );
+
+	float axialImpulse = joint->springImpulse + joint->motorImpulse + joint->lowerImpulse - joint->upperImpulse;
+
+	if ( stateA->flags & kDynamicFlag )
+	{
+		stateA->linearVelocity = b2MulSub( stateA->linearVelocity, mA, joint->linearImpulse );
+		stateA->angularVelocity -= iA * ( b2Cross( rA, joint->linearImpulse ) + axialImpulse );
+	}
+	if ( stateB->flags & kDynamicFlag )
+	{
+		stateB->linearVelocity = b2MulAdd( stateB->linearVelocity, mB, joint->linearImpulse );
+		stateB->angularVelocity += iB * ( b2Cross( rB, joint->linearImpulse ) + axialImpulse );
+	}
+}
+
+B2G_DEVI void SolveRevolute( JointSim* joint, BodyState* states, bool useBias, float h, float inv_h )
+{
+	float mA = joint->invMassA;
+	float mB = joint->invMassB;
+	float iA = joint->invIA;
+	float iB = joint->invIB;
+
+	BodyState* stateA = states + joint->indexA;
+	BodyState* stateB = states + joint->indexB;
+
+	b2Vec2 vA = stateA->linearVelocity;
+	float wA = stateA->angularVelocity;
+	b2Vec2 vB = stateB->linearVelocity;
+	float wB = stateB->angularVelocity;
+
+	b2Rot qA = b2MulRot( stateA->deltaRotation, joint->frameA.q );
+	b2Rot qB = b2MulRot( stateB->deltaRotation, joint->frameB.q );
+	b2Rot relQ = b2InvMulRot( qA, qB );
+
+	bool fixedRotation = ( iA + iB == 0.0f );
+
+	if ( joint->enableSpring && fixedRotation == false )
+	{
+		float jointAngle = Atan2( relQ.s, relQ.c );
+		float jointAngleDelta = b2UnwindAngle( jointAngle - joint->targetAngle );
+		float C = jointAngleDelta;
+		float bias = joint->springSoftness.biasRate * C;
+		float massScale = joint->springSoftness.massScale;
+		float impulseScale = joint->springSoftness.impulseScale;
+		float Cdot = wB - wA;
+		float impulse = -massScale * joint->axialMass * ( Cdot + bias ) - impulseScale * joint->springImpulse;
+		joint->springImpulse += impulse;
+		wA -= iA * impulse;
+		wB += iB * impulse;
+	}
+
+	if ( joint->enableMotor && fixedRotation == false )
+	{
+		float Cdot = wB - wA - joint->motorSpeed;
+		float impulse = -joint->axialMass * Cdot;
+		float oldImpulse = joint->motorImpulse;
+		float maxImpulse = h * joint->maxMotorTorque;
+		joint->motorImpulse = b2ClampFloat( joint->motorImpulse + impulse, -maxImpulse, maxImpulse );
+		impulse = joint->motorImpulse - oldImpulse;
+		wA -= iA * impulse;
+		wB += iB * impulse;
+	}
+
+	if ( joint->enableLimit && fixedRotation == false )
+	{
+		float jointAngle = Atan2( relQ.s, relQ.c );
+
+		{
+			float C = jointAngle - joint->lowerAngle;
+			float bias = 0.0f;
+			float massScale = 1.0f;
+			float impulseScale = 0.0f;
+			if ( C > 0.0f )
+			{
+				bias = C * inv_h;
+			}
+			else if ( useBias )
+			{
+				bias = joint->constraintSoftness.biasRate * C;
+				massScale = joint->constraintSoftness.massScale;
+				impulseScale = joint->constraintSoftness.impulseScale;
+			}
+			float Cdot = wB - wA;
+			float oldImpulse = joint->lowerImpulse;
+			float impulse = -massScale * joint->axialMass * ( Cdot + bias ) - impulseScale * oldImpulse;
+			joint->lowerImpulse = b2MaxFloat( oldImpulse + impulse, 0.0f );
+			impulse = joint->lowerImpulse - oldImpulse;
+			wA -= iA * impulse;
+			wB += iB * impulse;
+		}
+
+		{
+			float C = joint->upperAngle - jointAngle;
+			float bias = 0.0f;
+			float massScale = 1.0f;
+			float impulseScale = 0.0f;
+			if ( C > 0.0f )
+			{
+				bias = C * inv_h;
+			}
+			else if ( useBias )
+			{
+				bias = joint->constraintSoftness.biasRate * C;
+				massScale = joint->constraintSoftness.massScale;
+				impulseScale = joint->constraintSoftness.impulseScale;
+			}
+			float Cdot = wA - wB;
+			float oldImpulse = joint->upperImpulse;
+			float impulse = -massScale * joint->axialMass * ( Cdot + bias ) - impulseScale * oldImpulse;
+			joint->upperImpulse = b2MaxFloat( oldImpulse + impulse, 0.0f );
+			impulse = joint->upperImpulse - oldImpulse;
+			wA += iA * impulse;
+			wB -= iB * impulse;
+		}
+	}
+
+	{
+		b2Vec2 rA = b2RotateVector( stateA->deltaRotation, joint->frameA.p );
+		b2Vec2 rB = b2RotateVector( stateB->deltaRotation, joint->frameB.p );
+
+		b2Vec2 Cdot = b2Sub( b2Add( vB, b2CrossSV( wB, rB ) ), b2Add( vA, b2CrossSV( wA, rA ) ) );
+
+		b2Vec2 bias = { 0.0f, 0.0f };
+		float massScale = 1.0f;
+		float impulseScale = 0.0f;
+		if ( useBias )
+		{
+			b2Vec2 dcA = stateA->deltaPosition;
+			b2Vec2 dcB = stateB->deltaPosition;
+			b2Vec2 separation = b2Add( b2Add( b2Sub( dcB, dcA ), b2Sub( rB, rA ) ), joint->deltaCenter );
+			bias = b2MulSV( joint->constraintSoftness.biasRate, separation );
+			massScale = joint->constraintSoftness.massScale;
+			impulseScale = joint->constraintSoftness.impulseScale;
+		}
+
+		b2Mat22 K;
+		K.cx.x = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+		K.cy.x = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+		K.cx.y = K.cy.x;
+		K.cy.y = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+		b2Vec2 b = b2Solve22( K, b2Add( Cdot, bias ) );
+
+		b2Vec2 impulse;
+		impulse.x = -massScale * b.x - impulseScale * joint->linearImpulse.x;
+		impulse.y = -massScale * b.y - impulseScale * joint->linearImpulse.y;
+		joint->linearImpulse.x += impulse.x;
+		joint->linearImpulse.y += impulse.y;
+
+		vA = b2MulSub( vA, mA, impulse );
+		wA -= iA * b2Cross( rA, impulse );
+		vB = b2MulAdd( vB, mB, impulse );
+		wB += iB * b2Cross( rB, impulse );
+	}
+
+	if ( stateA->flags & kDynamicFlag )
+	{
+		stateA->linearVelocity = vA;
+		stateA->angularVelocity = wA;
+	}
+	if ( stateB->flags & kDynamicFlag )
+	{
+		stateB->linearVelocity = vB;
+		stateB->angularVelocity = wB;
+	}
+}
+
+} // namespace b2g

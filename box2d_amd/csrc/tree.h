@@ -1,0 +1,131 @@
+// Host-side bounding volume tree that fixes the broadphase pair order.
+//
+// Box2D orders new contacts by the order in which b2DynamicTree_Query visits leaves, and that
+// order is a function of the tree topology: surface-area-heuristic insertion with rotations
+// (reference src/dynamic_tree.c:164-673), enlarge-only refits (:798-837) and a per-step partial
+// median-split rebuild (:1433-1539, :1723-1996). This class keeps that topology (and the node
+// allocation order, which fixes proxy ids) identical to the reference so the pair order matches.
+#pragma once
+
+#include "core.h"
+
+#include <cstdint>
+#include <vector>
+
+namespace b2g
+{
+
+enum TreeNodeFlags : uint16_t
+{
+	kNodeAllocated = 0x0001,
+	kNodeEnlarged = 0x0002,
+	kNodeLeaf = 0x0004,
+};
+
+struct TreeNode
+{
+	b2AABB aabb;
+	uint64_t categoryBits;
+	int child1;
+	int child2;
+	uint64_t userData;
+	int parent; // doubles as the free-list link
+	uint16_t height;
+	uint16_t flags;
+};
+
+class DynamicTree
+{
+public:
+	DynamicTree();
+
+	int CreateProxy( b2AABB aabb, uint64_t categoryBits, uint64_t userData );
+	void DestroyProxy( int proxyId );
+	void MoveProxy( int proxyId, b2AABB aabb );
+	void EnlargeProxy( int proxyId, b2AABB aabb );
+
+	// Stack-based depth-first query; child2 is visited before child1. The callback returns false
+	// to stop the query.
+	template <typename F>
+	void Query( b2AABB aabb, uint64_t maskBits, F&& callback ) const
+	{
+		if ( nodeCount == 0 )
+		{
+			return;
+		}
+
+		int stack[1024];
+		int stackCount = 0;
+		stack[stackCount++] = root;
+
+		while ( stackCount > 0 )
+		{
+			int nodeId = stack[--stackCount];
+			const TreeNode& node = nodes[nodeId];
+
+			if ( b2AABB_Overlaps( node.aabb, aabb ) && ( node.categoryBits & maskBits ) != 0 )
+			{
+				if ( node.flags & kNodeLeaf )
+				{
+					if ( callback( nodeId, node.userData ) == false )
+					{
+						return;
+					}
+				}
+				else if ( stackCount < 1023 )
+				{
+					stack[stackCount++] = node.child1;
+					stack[stackCount++] = node.child2;
+				}
+			}
+		}
+	}
+
+	// Rebuild the enlarged part of the tree (or all of it with fullBuild). Returns leaf count.
+	int Rebuild( bool fullBuild );
+
+	b2AABB GetAABB( int proxyId ) const
+	{
+		return nodes[proxyId].aabb;
+	}
+
+	uint64_t GetUserData( int proxyId ) const
+	{
+		return nodes[proxyId].userData;
+	}
+
+	int GetHeight() const
+	{
+		return root == B2_NULL_INDEX ? 0 : nodes[root].height;
+	}
+
+	int ProxyCount() const
+	{
+		return proxyCount;
+	}
+
+	// Leaf proxy ids in query order (child2 first depth-first traversal). Used by the GPU pair
+	// finder to order pairs exactly like the host query would.
+	void LeafOrder( std::vector<int>& out ) const;
+
+private:
+	int AllocateNode();
+	void FreeNode( int nodeId );
+	int FindBestSibling( b2AABB boxD ) const;
+	void RotateNodes( int iA );
+	void InsertLeaf( int leaf, bool shouldRotate );
+	void RemoveLeaf( int leaf );
+	int BuildTree( int leafCount );
+
+	std::vector<TreeNode> nodes;
+	std::vector<int> freeStack; // freed node indices, reused last-in first-out
+	int freshIndex = 0;			// next never-used node index
+	int root = B2_NULL_INDEX;
+	int nodeCount = 0;
+	int proxyCount = 0;
+
+	std::vector<int> leafIndices;
+	std::vector<b2Vec2> leafCenters;
+};
+
+} // namespace b2g

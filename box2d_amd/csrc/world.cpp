@@ -1,0 +1,2615 @@
+// Host-side world bookkeeping for b2World_Step (see world.h).
+//
+// Each block below restates a reference routine whose ordering decisions the GPU results depend
+// on; the reference file:line is given at each function. Arithmetic that runs every step lives in
+// the GPU kernels (device.hip); the host only runs creation-time arithmetic (shape margins, mass
+// properties, initial AABBs) and integer bookkeeping.
+
+#include "world.h"
+
+#include "device.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+
+namespace b2g
+{
+
+static b2LogFcn* s_logFcn = nullptr;
+static b2AssertFcn* s_assertFcn = nullptr;
+
+void Log( const char* fmt, ... )
+{
+	char buffer[512];
+	va_list args;
+	va_start( args, fmt );
+	vsnprintf( buffer, sizeof( buffer ), fmt, args );
+	va_end( args );
+	if ( s_logFcn != nullptr )
+	{
+		s_logFcn( buffer );
+	}
+	else
+	{
+		fprintf( stderr, "box2d_amd: %s\n", buffer );
+	}
+}
+
+void Fatal( const char* fmt, ... )
+{
+	char buffer[512];
+	va_list args;
+	va_start( args, fmt );
+	vsnprintf( buffer, sizeof( buffer ), fmt, args );
+	va_end( args );
+	fprintf( stderr, "box2d_amd fatal: %s\n", buffer );
+	fflush( stderr );
+	abort();
+}
+
+// ---------------------------------------------------------------------------------------------
+// World registry
+
+static World* s_worlds[B2_MAX_WORLDS] = {};
+static uint16_t s_generations[B2_MAX_WORLDS] = {};
+
+World* GetWorld( b2WorldId id )
+{
+	if ( id.index1 < 1 || id.index1 > B2_MAX_WORLDS )
+	{
+		return nullptr;
+	}
+	World* world = s_worlds[id.index1 - 1];
+	if ( world == nullptr || world->generation != id.generation )
+	{
+		return nullptr;
+	}
+	return world;
+}
+
+World* GetWorldFromIndex( int index )
+{
+	if ( index < 0 || index >= B2_MAX_WORLDS )
+	{
+		return nullptr;
+	}
+	return s_worlds[index];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shape helpers (reference src/shape.c:36-87, 704-721, 826-887)
+
+static float ComputeShapeMargin( const ShapeDev& s )
+{
+	float margin = 0.0f;
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			margin = 0.5f * b2Distance( s.capsule.center2, s.capsule.center1 ) + s.capsule.radius;
+			break;
+		case b2_circleShape:
+			margin = s.circle.radius;
+			break;
+		case b2_polygonShape:
+		{
+			float maxExtentSqr = 0.0f;
+			for ( int i = 0; i < s.polygon.count; ++i )
+			{
+				float distanceSqr = b2DistanceSquared( s.polygon.vertices[i], s.polygon.centroid );
+				maxExtentSqr = b2MaxFloat( maxExtentSqr, distanceSqr );
+			}
+			margin = sqrtf( maxExtentSqr );
+		}
+		break;
+		case b2_segmentShape:
+			margin = 0.5f * b2Distance( s.segment.point1, s.segment.point2 );
+			break;
+		case b2_chainSegmentShape:
+			margin = 0.5f * b2Distance( s.chainSegment.segment.point1, s.chainSegment.segment.point2 );
+			break;
+		default:
+			return B2_MAX_AABB_MARGIN;
+	}
+	return b2MinFloat( B2_MAX_AABB_MARGIN, B2_AABB_MARGIN_FRACTION * margin );
+}
+
+static b2Vec2 ShapeCentroid( const ShapeDev& s )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			return b2Lerp( s.capsule.center1, s.capsule.center2, 0.5f );
+		case b2_circleShape:
+			return s.circle.center;
+		case b2_polygonShape:
+			return s.polygon.centroid;
+		case b2_segmentShape:
+			return b2Lerp( s.segment.point1, s.segment.point2, 0.5f );
+		case b2_chainSegmentShape:
+			return b2Lerp( s.chainSegment.segment.point1, s.chainSegment.segment.point2, 0.5f );
+		default:
+			return b2Vec2{ 0.0f, 0.0f };
+	}
+}
+
+struct ShapeExtent
+{
+	float minExtent;
+	float maxExtent;
+};
+
+static ShapeExtent ComputeShapeExtent( const ShapeDev& s, b2Vec2 localCenter )
+{
+	ShapeExtent extent = { 0.0f, 0.0f };
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+		{
+			float radius = s.capsule.radius;
+			extent.minExtent = radius;
+			b2Vec2 c1 = b2Sub( s.capsule.center1, localCenter );
+			b2Vec2 c2 = b2Sub( s.capsule.center2, localCenter );
+			extent.maxExtent = sqrtf( b2MaxFloat( b2LengthSquared( c1 ), b2LengthSquared( c2 ) ) ) + radius;
+		}
+		break;
+		case b2_circleShape:
+		{
+			float radius = s.circle.radius;
+			extent.minExtent = radius;
+			extent.maxExtent = b2Length( b2Sub( s.circle.center, localCenter ) ) + radius;
+		}
+		break;
+		case b2_polygonShape:
+		{
+			const b2Polygon& poly = s.polygon;
+			float minExtent = B2_HUGE;
+			float maxExtentSqr = 0.0f;
+			for ( int i = 0; i < poly.count; ++i )
+			{
+				b2Vec2 v = poly.vertices[i];
+				float planeOffset = b2Dot( poly.normals[i], b2Sub( v, poly.centroid ) );
+				minExtent = b2MinFloat( minExtent, planeOffset );
+				float distanceSqr = b2LengthSquared( b2Sub( v, localCenter ) );
+				maxExtentSqr = b2MaxFloat( maxExtentSqr, distanceSqr );
+			}
+			extent.minExtent = minExtent + poly.radius;
+			extent.maxExtent = sqrtf( maxExtentSqr ) + poly.radius;
+		}
+		break;
+		case b2_segmentShape:
+		{
+			b2Vec2 c1 = b2Sub( s.segment.point1, localCenter );
+			b2Vec2 c2 = b2Sub( s.segment.point2, localCenter );
+			extent.maxExtent = sqrtf( b2MaxFloat( b2LengthSquared( c1 ), b2LengthSquared( c2 ) ) );
+		}
+		break;
+		case b2_chainSegmentShape:
+		{
+			b2Vec2 c1 = b2Sub( s.chainSegment.segment.point1, localCenter );
+			b2Vec2 c2 = b2Sub( s.chainSegment.segment.point2, localCenter );
+			extent.maxExtent = sqrtf( b2MaxFloat( b2LengthSquared( c1 ), b2LengthSquared( c2 ) ) );
+		}
+		break;
+		default:
+			break;
+	}
+	return extent;
+}
+
+static b2MassData ComputeShapeMass( const ShapeDev& s, float density )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			return b2ComputeCapsuleMass( &s.capsule, density );
+		case b2_circleShape:
+			return b2ComputeCircleMass( &s.circle, density );
+		case b2_polygonShape:
+			return b2ComputePolygonMass( &s.polygon, density );
+		default:
+			return b2MassData{ 0.0f, { 0.0f, 0.0f }, 0.0f };
+	}
+}
+
+// Fat AABB on the host for shape creation (same construction as the GPU finalize stage,
+// reference src/geometry.c:402-505).
+static b2AABB HostFatShapeAABB( const ShapeDev& s, b2Transform xf, float extra )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+		{
+			b2Vec2 v1 = b2TransformWorldPoint( xf, s.capsule.center1 );
+			b2Vec2 v2 = b2TransformWorldPoint( xf, s.capsule.center2 );
+			double r = (double)s.capsule.radius + (double)extra;
+			double v1x = v1.x, v1y = v1.y, v2x = v2.x, v2y = v2.y;
+			return b2AABB{ { (float)( ( v1x < v2x ? v1x : v2x ) - r ), (float)( ( v1y < v2y ? v1y : v2y ) - r ) },
+						   { (float)( ( v1x > v2x ? v1x : v2x ) + r ), (float)( ( v1y > v2y ? v1y : v2y ) + r ) } };
+		}
+		case b2_circleShape:
+		{
+			b2Vec2 c = b2TransformWorldPoint( xf, s.circle.center );
+			double r = (double)s.circle.radius + (double)extra;
+			double cx = c.x, cy = c.y;
+			return b2AABB{ { (float)( cx - r ), (float)( cy - r ) }, { (float)( cx + r ), (float)( cy + r ) } };
+		}
+		case b2_polygonShape:
+		{
+			b2Vec2 v = b2TransformWorldPoint( xf, s.polygon.vertices[0] );
+			double lx = v.x, ly = v.y, ux = v.x, uy = v.y;
+			for ( int i = 1; i < s.polygon.count; ++i )
+			{
+				v = b2TransformWorldPoint( xf, s.polygon.vertices[i] );
+				double vx = v.x, vy = v.y;
+				lx = vx < lx ? vx : lx;
+				ly = vy < ly ? vy : ly;
+				ux = vx > ux ? vx : ux;
+				uy = vy > uy ? vy : uy;
+			}
+			double r = (double)s.polygon.radius + (double)extra;
+			return b2AABB{ { (float)( lx - r ), (float)( ly - r ) }, { (float)( ux + r ), (float)( uy + r ) } };
+		}
+		case b2_segmentShape:
+		case b2_chainSegmentShape:
+		{
+			const b2Segment& seg = s.type == b2_segmentShape ? s.segment : s.chainSegment.segment;
+			b2Vec2 v1 = b2TransformWorldPoint( xf, seg.point1 );
+			b2Vec2 v2 = b2TransformWorldPoint( xf, seg.point2 );
+			double e = (double)extra;
+			double v1x = v1.x, v1y = v1.y, v2x = v2.x, v2y = v2.y;
+			return b2AABB{ { (float)( ( v1x < v2x ? v1x : v2x ) - e ), (float)( ( v1y < v2y ? v1y : v2y ) - e ) },
+						   { (float)( ( v1x > v2x ? v1x : v2x ) + e ), (float)( ( v1y > v2y ? v1y : v2y ) + e ) } };
+		}
+		default:
+			return b2AABB{ { 0.0f, 0.0f }, { 0.0f, 0.0f } };
+	}
+}
+
+// Contact registry: which shape type pairs have a manifold and which order is primary
+// (reference src/contact.c:166-206).
+struct ContactRegister
+{
+	bool exists;
+	bool primary;
+};
+
+static ContactRegister s_registers[b2_shapeTypeCount][b2_shapeTypeCount];
+
+static void AddType( b2ShapeType type1, b2ShapeType type2 )
+{
+	s_registers[type1][type2] = { true, true };
+	if ( type1 != type2 )
+	{
+		s_registers[type2][type1] = { true, false };
+	}
+}
+
+static void InitializeContactRegisters()
+{
+	static bool initialized = false;
+	if ( initialized )
+	{
+		return;
+	}
+	AddType( b2_circleShape, b2_circleShape );
+	AddType( b2_capsuleShape, b2_circleShape );
+	AddType( b2_capsuleShape, b2_capsuleShape );
+	AddType( b2_polygonShape, b2_circleShape );
+	AddType( b2_polygonShape, b2_capsuleShape );
+	AddType( b2_polygonShape, b2_polygonShape );
+	AddType( b2_segmentShape, b2_circleShape );
+	AddType( b2_segmentShape, b2_capsuleShape );
+	AddType( b2_segmentShape, b2_polygonShape );
+	AddType( b2_chainSegmentShape, b2_circleShape );
+	AddType( b2_chainSegmentShape, b2_capsuleShape );
+	AddType( b2_chainSegmentShape, b2_polygonShape );
+	initialized = true;
+}
+
+static bool CanCollide( b2ShapeType a, b2ShapeType b )
+{
+	return s_registers[a][b].exists;
+}
+
+static bool ShouldShapesCollide( b2Filter filterA, b2Filter filterB )
+{
+	if ( filterA.groupIndex == filterB.groupIndex && filterA.groupIndex != 0 )
+	{
+		return filterA.groupIndex > 0;
+	}
+	return ( filterA.maskBits & filterB.categoryBits ) != 0 && ( filterA.categoryBits & filterB.maskBits ) != 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Broadphase proxy bookkeeping (reference src/broad_phase.c:82-147)
+
+void BroadPhase::UnBufferMove( int proxyKey )
+{
+	b2BodyType proxyType = B2G_PROXY_TYPE( proxyKey );
+	int proxyId = B2G_PROXY_ID( proxyKey );
+	if ( movedProxies[proxyType].GetBit( proxyId ) )
+	{
+		movedProxies[proxyType].ClearBit( proxyId );
+		for ( size_t i = 0; i < moveArray.size(); ++i )
+		{
+			if ( moveArray[i] == proxyKey )
+			{
+				RemoveSwap( moveArray, (int)i );
+				break;
+			}
+		}
+	}
+}
+
+int BroadPhase::CreateProxy( b2AABB aabb, b2BodyType proxyType, uint64_t categoryBits, int shapeIndex, bool forcePairCreation )
+{
+	int proxyId = trees[proxyType].CreateProxy( aabb, categoryBits, (uint64_t)shapeIndex );
+	int proxyKey = B2G_PROXY_KEY( proxyId, proxyType );
+	if ( proxyType != b2_staticBody || forcePairCreation )
+	{
+		BufferMove( proxyKey );
+	}
+	return proxyKey;
+}
+
+void BroadPhase::DestroyProxy( int proxyKey )
+{
+	UnBufferMove( proxyKey );
+	trees[B2G_PROXY_TYPE( proxyKey )].DestroyProxy( B2G_PROXY_ID( proxyKey ) );
+}
+
+void BroadPhase::EnlargeProxy( int proxyKey, b2AABB aabb )
+{
+	trees[B2G_PROXY_TYPE( proxyKey )].EnlargeProxy( B2G_PROXY_ID( proxyKey ), aabb );
+	BufferMove( proxyKey );
+}
+
+// ---------------------------------------------------------------------------------------------
+// World
+
+World::World( const b2WorldDef* def, int worldIndex, uint16_t gen )
+{
+	InitializeContactRegisters();
+
+	worldId = (uint16_t)worldIndex;
+	generation = gen;
+
+	// static, disabled and awake solver sets (reference src/physics_world.c:206-227)
+	for ( int i = 0; i < 3; ++i )
+	{
+		int setId = solverSetIdPool.Alloc();
+		SolverSet set;
+		set.setIndex = setId;
+		solverSets.push_back( set );
+	}
+
+	gravity = def->gravity;
+	hitEventThreshold = def->hitEventThreshold;
+	restitutionThreshold = def->restitutionThreshold;
+	maxLinearSpeed = def->maximumLinearSpeed;
+	contactSpeed = def->contactSpeed;
+	contactHertz = def->contactHertz;
+	contactDampingRatio = def->contactDampingRatio;
+	contactRecycleDistance = B2_CONTACT_RECYCLE_DISTANCE;
+	enableSleep = def->enableSleep;
+	enableContinuous = def->enableContinuous;
+	enableContactSoftening = def->enableContactSoftening;
+	userData = def->userData;
+	workerCount = def->workerCount;
+
+	if ( def->frictionCallback != nullptr || def->restitutionCallback != nullptr )
+	{
+		Fatal( "custom friction/restitution callbacks are not supported by the GPU step" );
+	}
+
+	device = new DeviceWorld();
+}
+
+World::~World()
+{
+	delete device;
+}
+
+Body* World::GetBody( b2BodyId id )
+{
+	int index = id.index1 - 1;
+	if ( index < 0 || index >= (int)bodies.size() )
+	{
+		return nullptr;
+	}
+	Body* body = &bodies[index];
+	if ( body->id != index || body->generation != id.generation )
+	{
+		return nullptr;
+	}
+	return body;
+}
+
+void World::PullBodies()
+{
+	if ( bodiesStale )
+	{
+		device->DownloadBodies( *this );
+		bodiesStale = false;
+	}
+}
+
+void World::PushBodiesIfDirty()
+{
+	if ( bodiesDirty )
+	{
+		device->UploadBodies( *this );
+		bodiesDirty = false;
+	}
+}
+
+void World::PullShapes()
+{
+	if ( shapesStale )
+	{
+		device->DownloadShapes( *this );
+		shapesStale = false;
+	}
+}
+
+void World::PullJoints()
+{
+	if ( jointsStale )
+	{
+		device->DownloadJoints( *this );
+		jointsStale = false;
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Islands (reference src/island.c)
+
+Island* World::CreateIsland( int setIndex )
+{
+	int islandId = islandIdPool.Alloc();
+	if ( islandId == (int)islands.size() )
+	{
+		islands.emplace_back();
+	}
+
+	SolverSet& set = solverSets[setIndex];
+	Island* island = &islands[islandId];
+	island->setIndex = setIndex;
+	island->localIndex = (int)set.islandIds.size();
+	island->islandId = islandId;
+	island->bodies.clear();
+	island->contacts.clear();
+	island->joints.clear();
+	island->constraintRemoveCount = 0;
+	set.islandIds.push_back( islandId );
+	return island;
+}
+
+void World::DestroyIsland( int islandId )
+{
+	if ( splitIslandId == islandId )
+	{
+		splitIslandId = B2_NULL_INDEX;
+	}
+
+	Island* island = &islands[islandId];
+	SolverSet& set = solverSets[island->setIndex];
+	int localIndex = island->localIndex;
+	int lastIndex = (int)set.islandIds.size() - 1;
+	int moveIslandId = set.islandIds[lastIndex];
+	set.islandIds[localIndex] = set.islandIds[lastIndex];
+	islands[moveIslandId].localIndex = localIndex;
+	set.islandIds.pop_back();
+
+	island->bodies.clear();
+	island->contacts.clear();
+	island->joints.clear();
+	island->constraintRemoveCount = 0;
+	island->localIndex = B2_NULL_INDEX;
+	island->islandId = B2_NULL_INDEX;
+	island->setIndex = B2_NULL_INDEX;
+	islandIdPool.Free( islandId );
+}
+
+void World::CreateIslandForBody( int setIndex, Body* body )
+{
+	Island* island = CreateIsland( setIndex );
+	island->bodies.push_back( body->id );
+	body->islandId = island->islandId;
+	body->islandIndex = 0;
+}
+
+int World::MergeIslands( int islandIdA, int islandIdB )
+{
+	if ( islandIdA == islandIdB )
+	{
+		return islandIdA;
+	}
+	if ( islandIdA == B2_NULL_INDEX )
+	{
+		return islandIdB;
+	}
+	if ( islandIdB == B2_NULL_INDEX )
+	{
+		return islandIdA;
+	}
+
+	Island* islandA = &islands[islandIdA];
+	Island* islandB = &islands[islandIdB];
+	Island* bigIsland;
+	Island* smallIsland;
+	if ( islandA->bodies.size() >= islandB->bodies.size() )
+	{
+		bigIsland = islandA;
+		smallIsland = islandB;
+	}
+	else
+	{
+		bigIsland = islandB;
+		smallIsland = islandA;
+	}
+
+	int bigIslandId = bigIsland->islandId;
+
+	for ( int bodyId : smallIsland->bodies )
+	{
+		Body& body = bodies[bodyId];
+		body.islandId = bigIslandId;
+		body.islandIndex = (int)bigIsland->bodies.size();
+		bigIsland->bodies.push_back( bodyId );
+	}
+
+	for ( const ContactLink& link : smallIsland->contacts )
+	{
+		Contact& contact = contacts[link.contactId];
+		contact.islandId = bigIslandId;
+		contact.islandIndex = (int)bigIsland->contacts.size();
+		bigIsland->contacts.push_back( link );
+	}
+
+	for ( const JointLink& link : smallIsland->joints )
+	{
+		Joint& joint = joints[link.jointId];
+		joint.islandId = bigIslandId;
+		joint.islandIndex = (int)bigIsland->joints.size();
+		bigIsland->joints.push_back( link );
+	}
+
+	bigIsland->constraintRemoveCount += smallIsland->constraintRemoveCount;
+	DestroyIsland( smallIsland->islandId );
+	return bigIslandId;
+}
+
+void World::AddContactToIsland( int islandId, Contact* contact )
+{
+	Island* island = &islands[islandId];
+	contact->islandId = islandId;
+	contact->islandIndex = (int)island->contacts.size();
+	island->contacts.push_back( ContactLink{ contact->contactId, contact->edges[0].bodyId, contact->edges[1].bodyId } );
+}
+
+void World::LinkContact( Contact* contact )
+{
+	int bodyIdA = contact->edges[0].bodyId;
+	int bodyIdB = contact->edges[1].bodyId;
+
+	if ( bodies[bodyIdA].setIndex == kAwakeSet && bodies[bodyIdB].setIndex >= kFirstSleepingSet )
+	{
+		WakeSolverSet( bodies[bodyIdB].setIndex );
+	}
+	if ( bodies[bodyIdB].setIndex == kAwakeSet && bodies[bodyIdA].setIndex >= kFirstSleepingSet )
+	{
+		WakeSolverSet( bodies[bodyIdA].setIndex );
+	}
+
+	int islandIdA = bodies[bodyIdA].islandId;
+	int islandIdB = bodies[bodyIdB].islandId;
+	int finalIslandId = MergeIslands( islandIdA, islandIdB );
+	AddContactToIsland( finalIslandId, contact );
+}
+
+void World::UnlinkContact( Contact* contact )
+{
+	int islandId = contact->islandId;
+	Island* island = &islands[islandId];
+	int removeIndex = contact->islandIndex;
+	int movedIndex = RemoveSwap( island->contacts, removeIndex );
+	if ( movedIndex != B2_NULL_INDEX )
+	{
+		contacts[island->contacts[removeIndex].contactId].islandIndex = removeIndex;
+	}
+	contact->islandId = B2_NULL_INDEX;
+	contact->islandIndex = B2_NULL_INDEX;
+	island->constraintRemoveCount += 1;
+}
+
+void World::AddJointToIsland( int islandId, Joint* joint )
+{
+	Island* island = &islands[islandId];
+	joint->islandId = islandId;
+	joint->islandIndex = (int)island->joints.size();
+	island->joints.push_back( JointLink{ joint->jointId, joint->edges[0].bodyId, joint->edges[1].bodyId } );
+}
+
+void World::LinkJoint( Joint* joint )
+{
+	Body* bodyA = &bodies[joint->edges[0].bodyId];
+	Body* bodyB = &bodies[joint->edges[1].bodyId];
+
+	if ( bodyA->setIndex == kAwakeSet && bodyB->setIndex >= kFirstSleepingSet )
+	{
+		WakeSolverSet( bodyB->setIndex );
+	}
+	else if ( bodyB->setIndex == kAwakeSet && bodyA->setIndex >= kFirstSleepingSet )
+	{
+		WakeSolverSet( bodyA->setIndex );
+	}
+
+	int finalIslandId = MergeIslands( bodyA->islandId, bodyB->islandId );
+	AddJointToIsland( finalIslandId, joint );
+}
+
+static inline int IslandFindParent( std::vector<int>& parents, int node )
+{
+	while ( parents[node] != node )
+	{
+		int grandParent = parents[parents[node]];
+		parents[node] = grandParent;
+		node = grandParent;
+	}
+	return node;
+}
+
+// Union-find split of an island whose constraints were removed (reference src/island.c:389-651).
+void World::SplitIsland( int baseId )
+{
+	Island* baseIsland = &islands[baseId];
+	int baseBodyCount = (int)baseIsland->bodies.size();
+	std::vector<int> baseBodyIds = baseIsland->bodies;
+	std::vector<ContactLink> baseContacts = baseIsland->contacts;
+	std::vector<JointLink> baseJoints = baseIsland->joints;
+
+	std::vector<int> parents( baseBodyCount ), contactCounts( baseBodyCount, 0 ), jointCounts( baseBodyCount, 0 ),
+		ranks( baseBodyCount, 0 );
+	for ( int i = 0; i < baseBodyCount; ++i )
+	{
+		parents[i] = i;
+	}
+
+	auto unite = [&]( int node1, int node2 ) {
+		int root1 = IslandFindParent( parents, node1 );
+		int root2 = IslandFindParent( parents, node2 );
+		if ( root1 != root2 )
+		{
+			if ( ranks[root1] < ranks[root2] )
+			{
+				parents[root1] = root2;
+				contactCounts[root2] += contactCounts[root1];
+				jointCounts[root2] += jointCounts[root1];
+			}
+			else if ( ranks[root1] > ranks[root2] )
+			{
+				parents[root2] = root1;
+				contactCounts[root1] += contactCounts[root2];
+				jointCounts[root1] += jointCounts[root2];
+			}
+			else
+			{
+				parents[root2] = root1;
+				ranks[root1] += 1;
+				contactCounts[root1] += contactCounts[root2];
+				jointCounts[root1] += jointCounts[root2];
+			}
+		}
+	};
+
+	for ( const ContactLink& link : baseContacts )
+	{
+		int islandIndexA = bodies[link.bodyIdA].islandIndex;
+		int islandIndexB = bodies[link.bodyIdB].islandIndex;
+		if ( islandIndexA != B2_NULL_INDEX && islandIndexB != B2_NULL_INDEX )
+		{
+			unite( islandIndexA, islandIndexB );
+			contactCounts[IslandFindParent( parents, islandIndexA )] += 1;
+		}
+		else
+		{
+			int islandIndex = islandIndexA != B2_NULL_INDEX ? islandIndexA : islandIndexB;
+			contactCounts[IslandFindParent( parents, islandIndex )] += 1;
+		}
+	}
+
+	for ( const JointLink& link : baseJoints )
+	{
+		int islandIndexA = bodies[link.bodyIdA].islandIndex;
+		int islandIndexB = bodies[link.bodyIdB].islandIndex;
+		if ( islandIndexA != B2_NULL_INDEX && islandIndexB != B2_NULL_INDEX )
+		{
+			unite( islandIndexA, islandIndexB );
+			jointCounts[IslandFindParent( parents, islandIndexA )] += 1;
+		}
+		else
+		{
+			int islandIndex = islandIndexA != B2_NULL_INDEX ? islandIndexA : islandIndexB;
+			jointCounts[IslandFindParent( parents, islandIndex )] += 1;
+		}
+	}
+
+	int componentCount = 0;
+	for ( int i = 0; i < baseBodyCount; ++i )
+	{
+		parents[i] = IslandFindParent( parents, i );
+		if ( parents[i] == i )
+		{
+			componentCount += 1;
+		}
+	}
+
+	if ( componentCount == 1 )
+	{
+		baseIsland->constraintRemoveCount = 0;
+		return;
+	}
+
+	baseIsland->bodies.clear();
+	baseIsland->contacts.clear();
+	baseIsland->joints.clear();
+
+	std::vector<int> rootMap( baseBodyCount, B2_NULL_INDEX );
+	int islandCount = 0;
+	for ( int i = 0; i < baseBodyCount; ++i )
+	{
+		int rootIndex = parents[i];
+		if ( rootMap[rootIndex] == B2_NULL_INDEX )
+		{
+			rootMap[rootIndex] = islandCount;
+			islandCount += 1;
+		}
+	}
+
+	std::vector<int> islandIds( islandCount );
+	for ( int i = 0; i < islandCount; ++i )
+	{
+		Island* newIsland = CreateIsland( kAwakeSet );
+		islandIds[i] = newIsland->islandId;
+	}
+
+	for ( int i = 0; i < baseBodyCount; ++i )
+	{
+		int bodyId = baseBodyIds[i];
+		int root = IslandFindParent( parents, i );
+		int newIslandId = islandIds[rootMap[root]];
+		Body& body = bodies[bodyId];
+		Island& newIsland = islands[newIslandId];
+		body.islandId = newIslandId;
+		body.islandIndex = (int)newIsland.bodies.size();
+		newIsland.bodies.push_back( bodyId );
+	}
+
+	for ( const ContactLink& link : baseContacts )
+	{
+		Contact& contact = contacts[link.contactId];
+		int targetIslandId = bodies[link.bodyIdA].islandId != B2_NULL_INDEX ? bodies[link.bodyIdA].islandId : bodies[link.bodyIdB].islandId;
+		Island& targetIsland = islands[targetIslandId];
+		contact.islandId = targetIslandId;
+		contact.islandIndex = (int)targetIsland.contacts.size();
+		targetIsland.contacts.push_back( link );
+	}
+
+	for ( const JointLink& link : baseJoints )
+	{
+		Joint& joint = joints[link.jointId];
+		int targetIslandId = bodies[link.bodyIdA].islandId != B2_NULL_INDEX ? bodies[link.bodyIdA].islandId : bodies[link.bodyIdB].islandId;
+		Island& targetIsland = islands[targetIslandId];
+		joint.islandId = targetIslandId;
+		joint.islandIndex = (int)targetIsland.joints.size();
+		targetIsland.joints.push_back( link );
+	}
+
+	DestroyIsland( baseId );
+}
+
+// ---------------------------------------------------------------------------------------------
+// Solver sets and sleep (reference src/solver_set.c)
+
+void World::DestroySolverSet( int setIndex )
+{
+	SolverSet& set = solverSets[setIndex];
+	set.bodyIds.clear();
+	set.contactIds.clear();
+	set.jointIds.clear();
+	set.islandIds.clear();
+	set.setIndex = B2_NULL_INDEX;
+	solverSetIdPool.Free( setIndex );
+}
+
+void World::WakeSolverSet( int setIndex )
+{
+	// Waking rewrites body states, so the host copy must be current first.
+	PullBodies();
+
+	SolverSet& awakeSet = solverSets[kAwakeSet];
+	SolverSet& disabledSet = solverSets[kDisabledSet];
+
+	std::vector<int> setBodies = solverSets[setIndex].bodyIds;
+	for ( int bodyId : setBodies )
+	{
+		Body& body = bodies[bodyId];
+		body.setIndex = kAwakeSet;
+		body.localIndex = (int)awakeSet.bodyIds.size();
+		bodyAux[bodyId].sleepTime = 0.0f;
+		awakeSet.bodyIds.push_back( bodyId );
+
+		BodyState& state = bodyStates[bodyId];
+		state = BodyState{ { 0.0f, 0.0f }, 0.0f, bodyAux[bodyId].flags, { 0.0f, 0.0f }, { 1.0f, 0.0f } };
+
+		int contactKey = body.headContactKey;
+		while ( contactKey != B2_NULL_INDEX )
+		{
+			int edgeIndex = contactKey & 1;
+			int contactId = contactKey >> 1;
+			Contact& contact = contacts[contactId];
+			contactKey = contact.edges[edgeIndex].nextKey;
+
+			if ( contact.setIndex != kDisabledSet )
+			{
+				continue;
+			}
+
+			int localIndex = contact.localIndex;
+			contact.setIndex = kAwakeSet;
+			contact.localIndex = (int)awakeSet.contactIds.size();
+			awakeSet.contactIds.push_back( contactId );
+
+			int movedLocalIndex = RemoveSwap( disabledSet.contactIds, localIndex );
+			if ( movedLocalIndex != B2_NULL_INDEX )
+			{
+				contacts[disabledSet.contactIds[localIndex]].localIndex = localIndex;
+			}
+		}
+	}
+
+	std::vector<int> setContacts = solverSets[setIndex].contactIds;
+	for ( int contactId : setContacts )
+	{
+		Contact& contact = contacts[contactId];
+		AddContactToGraph( &contact );
+		contact.setIndex = kAwakeSet;
+	}
+
+	std::vector<int> setJoints = solverSets[setIndex].jointIds;
+	for ( int jointId : setJoints )
+	{
+		Joint& joint = joints[jointId];
+		AddJointToGraph( &joint );
+		joint.setIndex = kAwakeSet;
+	}
+
+	std::vector<int> setIslands = solverSets[setIndex].islandIds;
+	for ( int islandId : setIslands )
+	{
+		Island& island = islands[islandId];
+		island.setIndex = kAwakeSet;
+		island.localIndex = (int)solverSets[kAwakeSet].islandIds.size();
+		solverSets[kAwakeSet].islandIds.push_back( islandId );
+	}
+
+	DestroySolverSet( setIndex );
+	bodiesDirty = true;
+	listsDirty = true;
+}
+
+void World::RemoveBodyFromAwakeSet( Body* body )
+{
+	SolverSet& awakeSet = solverSets[kAwakeSet];
+	int localIndex = body->localIndex;
+	int movedIndex = RemoveSwap( awakeSet.bodyIds, localIndex );
+	if ( movedIndex != B2_NULL_INDEX )
+	{
+		bodies[awakeSet.bodyIds[localIndex]].localIndex = localIndex;
+	}
+}
+
+void World::TrySleepIsland( int islandId )
+{
+	Island* island = &islands[islandId];
+	if ( island->constraintRemoveCount > 0 && island->bodies.size() > 1 )
+	{
+		return;
+	}
+
+	int sleepSetId = solverSetIdPool.Alloc();
+	if ( sleepSetId == (int)solverSets.size() )
+	{
+		solverSets.emplace_back();
+	}
+	SolverSet& sleepSet = solverSets[sleepSetId];
+	sleepSet = SolverSet{};
+	sleepSet.setIndex = sleepSetId;
+
+	SolverSet& awakeSet = solverSets[kAwakeSet];
+	SolverSet& disabledSet = solverSets[kDisabledSet];
+
+	for ( int bodyId : island->bodies )
+	{
+		Body* body = &bodies[bodyId];
+
+		if ( body->bodyMoveIndex != B2_NULL_INDEX )
+		{
+			moveEvents[body->bodyMoveIndex].fellAsleep = true;
+			body->bodyMoveIndex = B2_NULL_INDEX;
+		}
+
+		int sleepBodyIndex = (int)sleepSet.bodyIds.size();
+		sleepSet.bodyIds.push_back( bodyId );
+		RemoveBodyFromAwakeSet( body );
+		body->setIndex = sleepSetId;
+		body->localIndex = sleepBodyIndex;
+
+		int contactKey = body->headContactKey;
+		while ( contactKey != B2_NULL_INDEX )
+		{
+			int contactId = contactKey >> 1;
+			int edgeIndex = contactKey & 1;
+			Contact* contact = &contacts[contactId];
+			contactKey = contact->edges[edgeIndex].nextKey;
+
+			if ( contact->setIndex == kDisabledSet )
+			{
+				continue;
+			}
+			if ( contact->colorIndex != B2_NULL_INDEX )
+			{
+				continue;
+			}
+
+			int otherEdgeIndex = edgeIndex ^ 1;
+			int otherBodyId = contact->edges[otherEdgeIndex].bodyId;
+			if ( bodies[otherBodyId].setIndex == kAwakeSet )
+			{
+				continue;
+			}
+
+			int localIndex = contact->localIndex;
+			contact->setIndex = kDisabledSet;
+			contact->localIndex = (int)disabledSet.contactIds.size();
+			disabledSet.contactIds.push_back( contactId );
+
+			int movedLocalIndex = RemoveSwap( awakeSet.contactIds, localIndex );
+			if ( movedLocalIndex != B2_NULL_INDEX )
+			{
+				contacts[awakeSet.contactIds[localIndex]].localIndex = localIndex;
+			}
+		}
+	}
+
+	for ( const ContactLink& link : island->contacts )
+	{
+		Contact* contact = &contacts[link.contactId];
+		int colorIndex = contact->colorIndex;
+		GraphColor& color = colors[colorIndex];
+
+		if ( colorIndex != kOverflowIndex )
+		{
+			color.bodySet.ClearBit( contact->edges[0].bodyId );
+			color.bodySet.ClearBit( contact->edges[1].bodyId );
+		}
+
+		int localIndex = contact->localIndex;
+		int sleepContactIndex = (int)sleepSet.contactIds.size();
+		sleepSet.contactIds.push_back( link.contactId );
+
+		int movedLocalIndex = RemoveSwap( color.contactIds, localIndex );
+		if ( movedLocalIndex != B2_NULL_INDEX )
+		{
+			contacts[color.contactIds[localIndex]].localIndex = localIndex;
+		}
+
+		contact->setIndex = sleepSetId;
+		contact->colorIndex = B2_NULL_INDEX;
+		contact->localIndex = sleepContactIndex;
+	}
+
+	for ( const JointLink& link : island->joints )
+	{
+		Joint* joint = &joints[link.jointId];
+		int colorIndex = joint->colorIndex;
+		int localIndex = joint->localIndex;
+		GraphColor& color = colors[colorIndex];
+
+		if ( colorIndex != kOverflowIndex )
+		{
+			color.bodySet.ClearBit( joint->edges[0].bodyId );
+			color.bodySet.ClearBit( joint->edges[1].bodyId );
+		}
+
+		int sleepJointIndex = (int)sleepSet.jointIds.size();
+		sleepSet.jointIds.push_back( link.jointId );
+
+		int movedIndex = RemoveSwap( color.jointIds, localIndex );
+		if ( movedIndex != B2_NULL_INDEX )
+		{
+			joints[color.jointIds[localIndex]].localIndex = localIndex;
+		}
+
+		joint->setIndex = sleepSetId;
+		joint->colorIndex = B2_NULL_INDEX;
+		joint->localIndex = sleepJointIndex;
+	}
+
+	{
+		int islandIndex = island->localIndex;
+		sleepSet.islandIds.push_back( islandId );
+		int movedIslandIndex = RemoveSwap( awakeSet.islandIds, islandIndex );
+		if ( movedIslandIndex != B2_NULL_INDEX )
+		{
+			islands[awakeSet.islandIds[islandIndex]].localIndex = islandIndex;
+		}
+		island->setIndex = sleepSetId;
+		island->localIndex = 0;
+	}
+
+	if ( splitIslandId == islandId )
+	{
+		splitIslandId = B2_NULL_INDEX;
+	}
+
+	listsDirty = true;
+}
+
+void World::MergeSolverSets( int setId1, int setId2 )
+{
+	if ( solverSets[setId1].bodyIds.size() < solverSets[setId2].bodyIds.size() )
+	{
+		std::swap( setId1, setId2 );
+	}
+
+	SolverSet& set1 = solverSets[setId1];
+	SolverSet& set2 = solverSets[setId2];
+
+	for ( int bodyId : set2.bodyIds )
+	{
+		Body& body = bodies[bodyId];
+		body.setIndex = setId1;
+		body.localIndex = (int)set1.bodyIds.size();
+		set1.bodyIds.push_back( bodyId );
+	}
+
+	for ( int contactId : set2.contactIds )
+	{
+		Contact& contact = contacts[contactId];
+		contact.setIndex = setId1;
+		contact.localIndex = (int)set1.contactIds.size();
+		set1.contactIds.push_back( contactId );
+	}
+
+	for ( int jointId : set2.jointIds )
+	{
+		Joint& joint = joints[jointId];
+		joint.setIndex = setId1;
+		joint.localIndex = (int)set1.jointIds.size();
+		set1.jointIds.push_back( jointId );
+	}
+
+	for ( int islandId : set2.islandIds )
+	{
+		Island& island = islands[islandId];
+		island.setIndex = setId1;
+		island.localIndex = (int)set1.islandIds.size();
+		set1.islandIds.push_back( islandId );
+	}
+
+	DestroySolverSet( setId2 );
+}
+
+bool World::WakeBody( Body* body )
+{
+	if ( body->setIndex >= kFirstSleepingSet )
+	{
+		WakeSolverSet( body->setIndex );
+		return true;
+	}
+	return false;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Bodies (reference src/body.c:174-332, 524-656)
+
+b2BodyId World::CreateBody( const b2BodyDef* def )
+{
+	if ( locked )
+	{
+		return b2_nullBodyId;
+	}
+
+	PullBodies();
+
+	bool isAwake = ( def->isAwake || def->enableSleep == false ) && def->isEnabled;
+
+	int setId;
+	if ( def->isEnabled == false )
+	{
+		setId = kDisabledSet;
+	}
+	else if ( def->type == b2_staticBody )
+	{
+		setId = kStaticSet;
+	}
+	else if ( isAwake )
+	{
+		setId = kAwakeSet;
+	}
+	else
+	{
+		setId = solverSetIdPool.Alloc();
+		if ( setId == (int)solverSets.size() )
+		{
+			solverSets.emplace_back();
+		}
+		solverSets[setId] = SolverSet{};
+		solverSets[setId].setIndex = setId;
+	}
+
+	int bodyId = bodyIdPool.Alloc();
+
+	uint32_t lockFlags = 0;
+	lockFlags |= def->motionLocks.linearX ? kLockLinearX : 0;
+	lockFlags |= def->motionLocks.linearY ? kLockLinearY : 0;
+	lockFlags |= def->motionLocks.angularZ ? kLockAngularZ : 0;
+
+	if ( bodyId == (int)bodies.size() )
+	{
+		bodies.emplace_back();
+		bodySims.emplace_back();
+		bodyStates.emplace_back();
+		bodyAux.emplace_back();
+	}
+
+	BodySim& sim = bodySims[bodyId];
+	sim = BodySim{};
+	sim.transform.p = def->position;
+	sim.transform.q = def->rotation;
+	sim.center = def->position;
+	sim.rotation0 = sim.transform.q;
+	sim.center0 = sim.center;
+	sim.minExtent = B2_HUGE;
+	sim.maxExtent = 0.0f;
+	sim.linearDamping = def->linearDamping;
+	sim.angularDamping = def->angularDamping;
+	sim.gravityScale = def->gravityScale;
+	sim.flags = lockFlags;
+	sim.flags |= def->isBullet ? kIsBullet : 0;
+	sim.flags |= def->allowFastRotation ? kAllowFastRotation : 0;
+	sim.flags |= def->type == b2_dynamicBody ? kDynamicFlag : 0;
+	sim.flags |= def->enableSleep ? kEnableSleep : 0;
+	sim.flags |= def->enableContactRecycling ? kEnableContactRecycling : 0;
+	sim.headShapeId = B2_NULL_INDEX;
+
+	BodyState& state = bodyStates[bodyId];
+	state = BodyState{ { 0.0f, 0.0f }, 0.0f, 0u, { 0.0f, 0.0f }, { 1.0f, 0.0f } };
+	if ( setId == kAwakeSet )
+	{
+		state.linearVelocity = def->linearVelocity;
+		state.angularVelocity = def->angularVelocity;
+		state.flags = sim.flags;
+	}
+
+	Body& body = bodies[bodyId];
+	uint16_t gen = body.generation;
+	body = Body{};
+	body.generation = gen + 1;
+	if ( def->name != nullptr )
+	{
+		strncpy( body.name, def->name, B2_NAME_LENGTH );
+		body.name[B2_NAME_LENGTH] = 0;
+	}
+	body.userData = def->userData;
+	body.setIndex = setId;
+	body.localIndex = (int)solverSets[setId].bodyIds.size();
+	solverSets[setId].bodyIds.push_back( bodyId );
+	body.id = bodyId;
+	body.type = def->type;
+	body.flags = sim.flags;
+
+	BodyAux& aux = bodyAux[bodyId];
+	aux = BodyAux{};
+	aux.sleepThreshold = def->sleepThreshold;
+	aux.sleepTime = 0.0f;
+	aux.safetyFactor = def->safetyFactor;
+	aux.flags = sim.flags;
+	aux.type = def->type;
+
+	if ( setId >= kAwakeSet )
+	{
+		CreateIslandForBody( setId, &body );
+	}
+
+	bodiesDirty = true;
+	listsDirty = true;
+
+	return b2BodyId{ bodyId + 1, worldId, body.generation };
+}
+
+void World::UpdateBodyMassData( Body* body )
+{
+	BodySim& sim = bodySims[body->id];
+	body->flags &= ~kDirtyMass;
+	bodyAux[body->id].flags &= ~kDirtyMass;
+
+	body->mass = 0.0f;
+	body->inertia = 0.0f;
+	sim.invMass = 0.0f;
+	sim.invInertia = 0.0f;
+	sim.localCenter = b2Vec2{ 0.0f, 0.0f };
+	sim.minExtent = B2_HUGE;
+	sim.maxExtent = 0.0f;
+
+	if ( body->type != b2_dynamicBody )
+	{
+		sim.center = sim.transform.p;
+		sim.center0 = sim.center;
+
+		if ( body->type == b2_kinematicBody )
+		{
+			int shapeId = body->headShapeId;
+			while ( shapeId != B2_NULL_INDEX )
+			{
+				ShapeExtent extent = ComputeShapeExtent( shapeData[shapeId], b2Vec2{ 0.0f, 0.0f } );
+				sim.minExtent = b2MinFloat( sim.minExtent, extent.minExtent );
+				sim.maxExtent = b2MaxFloat( sim.maxExtent, extent.maxExtent );
+				shapeId = shapes[shapeId].nextShapeId;
+			}
+		}
+		bodiesDirty = true;
+		return;
+	}
+
+	std::vector<b2MassData> masses;
+	b2Vec2 localCenter = b2Vec2{ 0.0f, 0.0f };
+	int shapeId = body->headShapeId;
+	while ( shapeId != B2_NULL_INDEX )
+	{
+		const Shape& s = shapes[shapeId];
+		shapeId = s.nextShapeId;
+		if ( s.density == 0.0f )
+		{
+			masses.push_back( b2MassData{ 0.0f, { 0.0f, 0.0f }, 0.0f } );
+			continue;
+		}
+		b2MassData massData = ComputeShapeMass( shapeData[s.id], s.density );
+		body->mass += massData.mass;
+		localCenter = b2MulAdd( localCenter, massData.mass, massData.center );
+		masses.push_back( massData );
+	}
+
+	if ( body->mass > 0.0f )
+	{
+		sim.invMass = 1.0f / body->mass;
+		localCenter = b2MulSV( sim.invMass, localCenter );
+	}
+
+	for ( const b2MassData& massData : masses )
+	{
+		if ( massData.mass == 0.0f )
+		{
+			continue;
+		}
+		b2Vec2 offset = b2Sub( localCenter, massData.center );
+		float inertia = massData.rotationalInertia + massData.mass * b2Dot( offset, offset );
+		body->inertia += inertia;
+	}
+
+	if ( body->inertia > 0.0f && ( body->flags & kLockAngularZ ) == 0 )
+	{
+		sim.invInertia = 1.0f / body->inertia;
+	}
+	else
+	{
+		body->inertia = 0.0f;
+		sim.invInertia = 0.0f;
+	}
+
+	b2Vec2 oldCenter = sim.center;
+	sim.localCenter = localCenter;
+	sim.center = b2TransformWorldPoint( sim.transform, sim.localCenter );
+	sim.center0 = sim.center;
+
+	if ( body->setIndex == kAwakeSet )
+	{
+		BodyState& state = bodyStates[body->id];
+		b2Vec2 deltaLinear = b2CrossSV( state.angularVelocity, b2SubPos( sim.center, oldCenter ) );
+		state.linearVelocity = b2Add( state.linearVelocity, deltaLinear );
+	}
+
+	shapeId = body->headShapeId;
+	while ( shapeId != B2_NULL_INDEX )
+	{
+		ShapeExtent extent = ComputeShapeExtent( shapeData[shapeId], localCenter );
+		sim.minExtent = b2MinFloat( sim.minExtent, extent.minExtent );
+		sim.maxExtent = b2MaxFloat( sim.maxExtent, extent.maxExtent );
+		shapeId = shapes[shapeId].nextShapeId;
+	}
+
+	bodiesDirty = true;
+}
+
+bool World::ShouldBodiesCollide( Body* bodyA, Body* bodyB )
+{
+	if ( bodyA->type != b2_dynamicBody && bodyB->type != b2_dynamicBody )
+	{
+		return false;
+	}
+
+	int jointKey;
+	int otherBodyId;
+	if ( bodyA->jointCount < bodyB->jointCount )
+	{
+		jointKey = bodyA->headJointKey;
+		otherBodyId = bodyB->id;
+	}
+	else
+	{
+		jointKey = bodyB->headJointKey;
+		otherBodyId = bodyA->id;
+	}
+
+	while ( jointKey != B2_NULL_INDEX )
+	{
+		int jointId = jointKey >> 1;
+		int edgeIndex = jointKey & 1;
+		int otherEdgeIndex = edgeIndex ^ 1;
+		Joint& joint = joints[jointId];
+		if ( joint.collideConnected == false && joint.edges[otherEdgeIndex].bodyId == otherBodyId )
+		{
+			return false;
+		}
+		jointKey = joint.edges[edgeIndex].nextKey;
+	}
+	return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shapes (reference src/shape.c:89-305)
+
+void World::CreateShapeProxy( Shape* shape, bool forcePairCreation )
+{
+	Body& body = bodies[shape->bodyId];
+	ShapeDev& sd = shapeData[shape->id];
+	b2Transform transform = bodySims[body.id].transform;
+
+	b2AABB aabb = HostFatShapeAABB( sd, transform, B2_SPECULATIVE_DISTANCE );
+	sd.aabb = aabb;
+	float margin = body.type == b2_staticBody ? B2_SPECULATIVE_DISTANCE : sd.aabbMargin;
+	b2AABB fatAABB;
+	fatAABB.lowerBound.x = aabb.lowerBound.x - margin;
+	fatAABB.lowerBound.y = aabb.lowerBound.y - margin;
+	fatAABB.upperBound.x = aabb.upperBound.x + margin;
+	fatAABB.upperBound.y = aabb.upperBound.y + margin;
+	sd.fatAABB = fatAABB;
+
+	shape->proxyKey = broadPhase.CreateProxy( fatAABB, body.type, shape->filter.categoryBits, shape->id, forcePairCreation );
+}
+
+b2ShapeId World::CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void* geometry, b2ShapeType shapeType )
+{
+	if ( locked )
+	{
+		return b2_nullShapeId;
+	}
+
+	Body* body = GetBody( bodyId );
+	if ( body == nullptr )
+	{
+		return b2_nullShapeId;
+	}
+
+	if ( def->isSensor )
+	{
+		Fatal( "sensor shapes are not supported by the GPU step" );
+	}
+	if ( def->enablePreSolveEvents || def->enableCustomFiltering )
+	{
+		Fatal( "pre-solve callbacks and custom filtering are not supported by the GPU step" );
+	}
+
+	PullBodies();
+	PullShapes();
+
+	int shapeId = shapeIdPool.Alloc();
+	if ( shapeId == (int)shapes.size() )
+	{
+		shapes.emplace_back();
+		shapeData.emplace_back();
+	}
+
+	Shape* shape = &shapes[shapeId];
+	uint16_t gen = shape->generation;
+	*shape = Shape{};
+	shape->generation = gen + 1;
+
+	ShapeDev& sd = shapeData[shapeId];
+	memset( &sd, 0, sizeof( sd ) );
+	switch ( shapeType )
+	{
+		case b2_capsuleShape:
+			sd.capsule = *(const b2Capsule*)geometry;
+			break;
+		case b2_circleShape:
+			sd.circle = *(const b2Circle*)geometry;
+			break;
+		case b2_polygonShape:
+			sd.polygon = *(const b2Polygon*)geometry;
+			break;
+		case b2_segmentShape:
+			sd.segment = *(const b2Segment*)geometry;
+			break;
+		case b2_chainSegmentShape:
+			sd.chainSegment = *(const b2ChainSegment*)geometry;
+			break;
+		default:
+			break;
+	}
+
+	shape->id = shapeId;
+	shape->bodyId = body->id;
+	shape->type = shapeType;
+	shape->density = def->density;
+	shape->material = def->material;
+	shape->filter = def->filter;
+	shape->userData = def->userData;
+	shape->enableSensorEvents = def->enableSensorEvents;
+	shape->enableContactEvents = def->enableContactEvents;
+	shape->enableCustomFiltering = def->enableCustomFiltering;
+	shape->enableHitEvents = def->enableHitEvents;
+	shape->enablePreSolveEvents = def->enablePreSolveEvents;
+	shape->proxyKey = B2_NULL_INDEX;
+
+	sd.type = shapeType;
+	sd.bodyId = body->id;
+	sd.flags = ( def->enableHitEvents ? kShapeEnableHitEvents : 0 ) | ( def->enableContactEvents ? kShapeEnableContactEvents : 0 );
+	sd.friction = def->material.friction;
+	sd.restitution = def->material.restitution;
+	sd.rollingResistance = def->material.rollingResistance;
+	sd.tangentSpeed = def->material.tangentSpeed;
+	sd.localCentroid = ShapeCentroid( sd );
+	sd.aabbMargin = ComputeShapeMargin( sd );
+	sd.enlargedAABB = 0;
+
+	if ( def->material.restitution > 0.0f )
+	{
+		restitutionShapeCount += 1;
+	}
+
+	if ( body->setIndex != kDisabledSet )
+	{
+		CreateShapeProxy( shape, def->invokeContactCreation || def->isSensor );
+	}
+
+	// head insert into the body's shape list
+	if ( body->headShapeId != B2_NULL_INDEX )
+	{
+		shapes[body->headShapeId].prevShapeId = shapeId;
+	}
+	shape->prevShapeId = B2_NULL_INDEX;
+	shape->nextShapeId = body->headShapeId;
+	sd.nextShapeId = body->headShapeId;
+	body->headShapeId = shapeId;
+	bodySims[body->id].headShapeId = shapeId;
+	body->shapeCount += 1;
+
+	if ( def->updateBodyMass == true )
+	{
+		UpdateBodyMassData( body );
+	}
+	else if ( ( body->flags & kDirtyMass ) == 0 )
+	{
+		body->flags |= kDirtyMass;
+		bodyAux[body->id].flags |= kDirtyMass;
+	}
+
+	shapesDirty = true;
+	bodiesDirty = true;
+	return b2ShapeId{ shapeId + 1, worldId, shape->generation };
+}
+
+b2ChainId World::CreateChain( b2BodyId bodyId, const b2ChainDef* def )
+{
+	(void)bodyId;
+	(void)def;
+	Fatal( "chain shapes are not supported by the GPU step yet" );
+}
+
+// ---------------------------------------------------------------------------------------------
+// Contacts (reference src/contact.c:214-501)
+
+void World::CreateContact( Shape* shapeA, Shape* shapeB )
+{
+	b2ShapeType type1 = shapeA->type;
+	b2ShapeType type2 = shapeB->type;
+
+	if ( s_registers[type1][type2].exists == false )
+	{
+		return;
+	}
+
+	if ( s_registers[type1][type2].primary == false )
+	{
+		CreateContact( shapeB, shapeA );
+		return;
+	}
+
+	if ( type1 == b2_chainSegmentShape )
+	{
+		Fatal( "chain segment contacts are not supported by the GPU step yet" );
+	}
+
+	Body* bodyA = &bodies[shapeA->bodyId];
+	Body* bodyB = &bodies[shapeB->bodyId];
+
+	int setIndex;
+	if ( bodyA->setIndex == kAwakeSet || bodyB->setIndex == kAwakeSet )
+	{
+		setIndex = kAwakeSet;
+	}
+	else
+	{
+		setIndex = kDisabledSet;
+	}
+
+	SolverSet& set = solverSets[setIndex];
+
+	int contactId = contactIdPool.Alloc();
+	if ( contactId == (int)contacts.size() )
+	{
+		contacts.emplace_back();
+	}
+
+	int shapeIdA = shapeA->id;
+	int shapeIdB = shapeB->id;
+
+	Contact* contact = &contacts[contactId];
+	uint32_t gen = contact->generation;
+	*contact = Contact{};
+	contact->contactId = contactId;
+	contact->generation = gen + 1;
+	contact->setIndex = setIndex;
+	contact->colorIndex = B2_NULL_INDEX;
+	contact->localIndex = (int)set.contactIds.size();
+	contact->islandId = B2_NULL_INDEX;
+	contact->islandIndex = B2_NULL_INDEX;
+	contact->shapeIdA = shapeIdA;
+	contact->shapeIdB = shapeIdB;
+	contact->flags = 0;
+
+	if ( ( bodyA->flags & kEnableContactRecycling ) != 0 && ( bodyB->flags & kEnableContactRecycling ) != 0 )
+	{
+		contact->flags |= kContactRecycle;
+	}
+
+	if ( shapeA->enableContactEvents || shapeB->enableContactEvents )
+	{
+		contact->flags |= kContactEnableContactEvents;
+	}
+
+	{
+		contact->edges[0].bodyId = shapeA->bodyId;
+		contact->edges[0].prevKey = B2_NULL_INDEX;
+		contact->edges[0].nextKey = bodyA->headContactKey;
+		int keyA = ( contactId << 1 ) | 0;
+		int headContactKey = bodyA->headContactKey;
+		if ( headContactKey != B2_NULL_INDEX )
+		{
+			contacts[headContactKey >> 1].edges[headContactKey & 1].prevKey = keyA;
+		}
+		bodyA->headContactKey = keyA;
+		bodyA->contactCount += 1;
+	}
+
+	{
+		contact->edges[1].bodyId = shapeB->bodyId;
+		contact->edges[1].prevKey = B2_NULL_INDEX;
+		contact->edges[1].nextKey = bodyB->headContactKey;
+		int keyB = ( contactId << 1 ) | 1;
+		int headContactKey = bodyB->headContactKey;
+		if ( headContactKey != B2_NULL_INDEX )
+		{
+			contacts[headContactKey >> 1].edges[headContactKey & 1].prevKey = keyB;
+		}
+		bodyB->headContactKey = keyB;
+		bodyB->contactCount += 1;
+	}
+
+	broadPhase.pairSet.insert( B2G_SHAPE_PAIR_KEY( shapeIdA, shapeIdB ) );
+
+	set.contactIds.push_back( contactId );
+
+	ContactInit init;
+	init.contactId = contactId;
+	init.shapeIdA = shapeIdA;
+	init.shapeIdB = shapeIdB;
+	init.bodyIdA = shapeA->bodyId;
+	init.bodyIdB = shapeB->bodyId;
+	init.friction = sqrtf( shapeA->material.friction * shapeB->material.friction );
+	init.restitution = b2MaxFloat( shapeA->material.restitution, shapeB->material.restitution );
+	init.simFlags = contact->flags;
+	if ( shapeA->enablePreSolveEvents || shapeB->enablePreSolveEvents )
+	{
+		init.simFlags |= kSimEnablePreSolveEvents;
+	}
+	pendingContactInits.push_back( init );
+
+	listsDirty = true;
+}
+
+void World::DestroyContact( Contact* contact )
+{
+	broadPhase.pairSet.erase( B2G_SHAPE_PAIR_KEY( contact->shapeIdA, contact->shapeIdB ) );
+
+	ContactEdge* edgeA = contact->edges + 0;
+	ContactEdge* edgeB = contact->edges + 1;
+	int bodyIdA = edgeA->bodyId;
+	int bodyIdB = edgeB->bodyId;
+	Body* bodyA = &bodies[bodyIdA];
+	Body* bodyB = &bodies[bodyIdB];
+
+	bool touching = ( contact->flags & kContactTouching ) != 0;
+
+	if ( edgeA->prevKey != B2_NULL_INDEX )
+	{
+		contacts[edgeA->prevKey >> 1].edges[edgeA->prevKey & 1].nextKey = edgeA->nextKey;
+	}
+	if ( edgeA->nextKey != B2_NULL_INDEX )
+	{
+		contacts[edgeA->nextKey >> 1].edges[edgeA->nextKey & 1].prevKey = edgeA->prevKey;
+	}
+
+	int contactId = contact->contactId;
+	int edgeKeyA = ( contactId << 1 ) | 0;
+	if ( bodyA->headContactKey == edgeKeyA )
+	{
+		bodyA->headContactKey = edgeA->nextKey;
+	}
+	bodyA->contactCount -= 1;
+
+	if ( edgeB->prevKey != B2_NULL_INDEX )
+	{
+		contacts[edgeB->prevKey >> 1].edges[edgeB->prevKey & 1].nextKey = edgeB->nextKey;
+	}
+	if ( edgeB->nextKey != B2_NULL_INDEX )
+	{
+		contacts[edgeB->nextKey >> 1].edges[edgeB->nextKey & 1].prevKey = edgeB->prevKey;
+	}
+
+	int edgeKeyB = ( contactId << 1 ) | 1;
+	if ( bodyB->headContactKey == edgeKeyB )
+	{
+		bodyB->headContactKey = edgeB->nextKey;
+	}
+	bodyB->contactCount -= 1;
+
+	if ( contact->islandId != B2_NULL_INDEX )
+	{
+		UnlinkContact( contact );
+	}
+
+	if ( contact->colorIndex != B2_NULL_INDEX )
+	{
+		RemoveContactFromGraph( bodyIdA, bodyIdB, contact->colorIndex, contact->localIndex );
+	}
+	else
+	{
+		SolverSet& set = solverSets[contact->setIndex];
+		int movedIndex = RemoveSwap( set.contactIds, contact->localIndex );
+		if ( movedIndex != B2_NULL_INDEX )
+		{
+			contacts[set.contactIds[contact->localIndex]].localIndex = contact->localIndex;
+		}
+	}
+
+	contact->contactId = B2_NULL_INDEX;
+	contact->setIndex = B2_NULL_INDEX;
+	contact->colorIndex = B2_NULL_INDEX;
+	contact->localIndex = B2_NULL_INDEX;
+	contactIdPool.Free( contactId );
+
+	if ( touching )
+	{
+		WakeBody( bodyA );
+		WakeBody( bodyB );
+	}
+
+	listsDirty = true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Constraint graph coloring (reference src/constraint_graph.c:66-291)
+
+void World::AddContactToGraph( Contact* contact )
+{
+	int colorIndex = kOverflowIndex;
+	int bodyIdA = contact->edges[0].bodyId;
+	int bodyIdB = contact->edges[1].bodyId;
+	b2BodyType typeA = bodies[bodyIdA].type;
+	b2BodyType typeB = bodies[bodyIdB].type;
+
+	if ( typeA == b2_dynamicBody && typeB == b2_dynamicBody )
+	{
+		for ( int i = 0; i < kDynamicColorCount; ++i )
+		{
+			GraphColor& color = colors[i];
+			if ( color.bodySet.GetBit( bodyIdA ) || color.bodySet.GetBit( bodyIdB ) )
+			{
+				continue;
+			}
+			color.bodySet.SetBitGrow( bodyIdA );
+			color.bodySet.SetBitGrow( bodyIdB );
+			colorIndex = i;
+			break;
+		}
+	}
+	else if ( typeA == b2_dynamicBody )
+	{
+		for ( int i = kOverflowIndex - 1; i >= 1; --i )
+		{
+			GraphColor& color = colors[i];
+			if ( color.bodySet.GetBit( bodyIdA ) )
+			{
+				continue;
+			}
+			color.bodySet.SetBitGrow( bodyIdA );
+			colorIndex = i;
+			break;
+		}
+	}
+	else if ( typeB == b2_dynamicBody )
+	{
+		for ( int i = kOverflowIndex - 1; i >= 1; --i )
+		{
+			GraphColor& color = colors[i];
+			if ( color.bodySet.GetBit( bodyIdB ) )
+			{
+				continue;
+			}
+			color.bodySet.SetBitGrow( bodyIdB );
+			colorIndex = i;
+			break;
+		}
+	}
+
+	GraphColor& color = colors[colorIndex];
+	contact->colorIndex = colorIndex;
+	contact->localIndex = (int)color.contactIds.size();
+	color.contactIds.push_back( contact->contactId );
+	listsDirty = true;
+}
+
+void World::RemoveContactFromGraph( int bodyIdA, int bodyIdB, int colorIndex, int localIndex )
+{
+	GraphColor& color = colors[colorIndex];
+	if ( colorIndex != kOverflowIndex )
+	{
+		color.bodySet.ClearBit( bodyIdA );
+		color.bodySet.ClearBit( bodyIdB );
+	}
+
+	int movedIndex = RemoveSwap( color.contactIds, localIndex );
+	if ( movedIndex != B2_NULL_INDEX )
+	{
+		contacts[color.contactIds[localIndex]].localIndex = localIndex;
+	}
+	listsDirty = true;
+}
+
+int World::AssignJointColor( int bodyIdA, int bodyIdB, b2BodyType typeA, b2BodyType typeB )
+{
+	if ( typeA == b2_dynamicBody && typeB == b2_dynamicBody )
+	{
+		for ( int i = 0; i < kDynamicColorCount; ++i )
+		{
+			GraphColor& color = colors[i];
+			if ( color.bodySet.GetBit( bodyIdA ) || color.bodySet.GetBit( bodyIdB ) )
+			{
+				continue;
+			}
+			color.bodySet.SetBitGrow( bodyIdA );
+			color.bodySet.SetBitGrow( bodyIdB );
+			return i;
+		}
+	}
+	else if ( typeA == b2_dynamicBody )
+	{
+		for ( int i = kOverflowIndex - 1; i >= 1; --i )
+		{
+			GraphColor& color = colors[i];
+			if ( color.bodySet.GetBit( bodyIdA ) )
+			{
+				continue;
+			}
+			color.bodySet.SetBitGrow( bodyIdA );
+			return i;
+		}
+	}
+	else if ( typeB == b2_dynamicBody )
+	{
+		for ( int i = kOverflowIndex - 1; i >= 1; --i )
+		{
+			GraphColor& color = colors[i];
+			if ( color.bodySet.GetBit( bodyIdB ) )
+			{
+				continue;
+			}
+			color.bodySet.SetBitGrow( bodyIdB );
+			return i;
+		}
+	}
+	return kOverflowIndex;
+}
+
+void World::AddJointToGraph( Joint* joint )
+{
+	int bodyIdA = joint->edges[0].bodyId;
+	int bodyIdB = joint->edges[1].bodyId;
+	int colorIndex = AssignJointColor( bodyIdA, bodyIdB, bodies[bodyIdA].type, bodies[bodyIdB].type );
+	GraphColor& color = colors[colorIndex];
+	joint->colorIndex = colorIndex;
+	joint->localIndex = (int)color.jointIds.size();
+	color.jointIds.push_back( joint->jointId );
+	listsDirty = true;
+}
+
+void World::RemoveJointFromGraph( int bodyIdA, int bodyIdB, int colorIndex, int localIndex )
+{
+	GraphColor& color = colors[colorIndex];
+	if ( colorIndex != kOverflowIndex )
+	{
+		color.bodySet.ClearBit( bodyIdA );
+		color.bodySet.ClearBit( bodyIdB );
+	}
+	int movedIndex = RemoveSwap( color.jointIds, localIndex );
+	if ( movedIndex != B2_NULL_INDEX )
+	{
+		joints[color.jointIds[localIndex]].localIndex = localIndex;
+	}
+	listsDirty = true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Joints (reference src/joint.c:161-391, 605-646)
+
+void World::DestroyContactsBetweenBodies( Body* bodyA, Body* bodyB )
+{
+	int contactKey;
+	int otherBodyId;
+	if ( bodyA->contactCount < bodyB->contactCount )
+	{
+		contactKey = bodyA->headContactKey;
+		otherBodyId = bodyB->id;
+	}
+	else
+	{
+		contactKey = bodyB->headContactKey;
+		otherBodyId = bodyA->id;
+	}
+
+	while ( contactKey != B2_NULL_INDEX )
+	{
+		int contactId = contactKey >> 1;
+		int edgeIndex = contactKey & 1;
+		Contact* contact = &contacts[contactId];
+		contactKey = contact->edges[edgeIndex].nextKey;
+		int otherEdgeIndex = edgeIndex ^ 1;
+		if ( contact->edges[otherEdgeIndex].bodyId == otherBodyId )
+		{
+			DestroyContact( contact );
+		}
+	}
+}
+
+b2JointId World::CreateRevoluteJoint( const b2RevoluteJointDef* def )
+{
+	if ( locked )
+	{
+		return b2_nullJointId;
+	}
+
+	const b2JointDef* base = &def->base;
+	Body* bodyA = GetBody( base->bodyIdA );
+	Body* bodyB = GetBody( base->bodyIdB );
+	if ( bodyA == nullptr || bodyB == nullptr )
+	{
+		return b2_nullJointId;
+	}
+
+	PullJoints();
+
+	if ( base->collideConnected == false )
+	{
+		DestroyContactsBetweenBodies( bodyA, bodyB );
+	}
+
+	int bodyIdA = bodyA->id;
+	int bodyIdB = bodyB->id;
+	int maxSetIndex = b2MaxInt( bodyA->setIndex, bodyB->setIndex );
+
+	int jointId = jointIdPool.Alloc();
+	if ( jointId == (int)joints.size() )
+	{
+		joints.emplace_back();
+		jointSims.emplace_back();
+	}
+
+	Joint* joint = &joints[jointId];
+	uint16_t gen = joint->generation;
+	*joint = Joint{};
+	joint->generation = gen + 1;
+	joint->jointId = jointId;
+	joint->userData = base->userData;
+	joint->type = b2_revoluteJoint;
+	joint->collideConnected = base->collideConnected;
+
+	joint->edges[0].bodyId = bodyIdA;
+	joint->edges[0].prevKey = B2_NULL_INDEX;
+	joint->edges[0].nextKey = bodyA->headJointKey;
+	int keyA = ( jointId << 1 ) | 0;
+	if ( bodyA->headJointKey != B2_NULL_INDEX )
+	{
+		joints[bodyA->headJointKey >> 1].edges[bodyA->headJointKey & 1].prevKey = keyA;
+	}
+	bodyA->headJointKey = keyA;
+	bodyA->jointCount += 1;
+
+	joint->edges[1].bodyId = bodyIdB;
+	joint->edges[1].prevKey = B2_NULL_INDEX;
+	joint->edges[1].nextKey = bodyB->headJointKey;
+	int keyB = ( jointId << 1 ) | 1;
+	if ( bodyB->headJointKey != B2_NULL_INDEX )
+	{
+		joints[bodyB->headJointKey >> 1].edges[bodyB->headJointKey & 1].prevKey = keyB;
+	}
+	bodyB->headJointKey = keyB;
+	bodyB->jointCount += 1;
+
+	if ( bodyA->setIndex == kDisabledSet || bodyB->setIndex == kDisabledSet )
+	{
+		joint->setIndex = kDisabledSet;
+		joint->localIndex = (int)solverSets[kDisabledSet].jointIds.size();
+		solverSets[kDisabledSet].jointIds.push_back( jointId );
+	}
+	else if ( bodyA->type != b2_dynamicBody && bodyB->type != b2_dynamicBody )
+	{
+		joint->setIndex = kStaticSet;
+		joint->localIndex = (int)solverSets[kStaticSet].jointIds.size();
+		solverSets[kStaticSet].jointIds.push_back( jointId );
+	}
+	else if ( bodyA->setIndex == kAwakeSet || bodyB->setIndex == kAwakeSet )
+	{
+		if ( maxSetIndex >= kFirstSleepingSet )
+		{
+			WakeSolverSet( maxSetIndex );
+		}
+		joint->setIndex = kAwakeSet;
+		AddJointToGraph( joint );
+	}
+	else
+	{
+		int setIndex = maxSetIndex;
+		joint->setIndex = setIndex;
+		joint->localIndex = (int)solverSets[setIndex].jointIds.size();
+		solverSets[setIndex].jointIds.push_back( jointId );
+
+		if ( bodyA->setIndex != bodyB->setIndex && bodyA->setIndex >= kFirstSleepingSet && bodyB->setIndex >= kFirstSleepingSet )
+		{
+			MergeSolverSets( bodyA->setIndex, bodyB->setIndex );
+		}
+	}
+
+	JointSim& js = jointSims[jointId];
+	memset( &js, 0, sizeof( js ) );
+	js.jointId = jointId;
+	js.bodyIdA = bodyIdA;
+	js.bodyIdB = bodyIdB;
+	js.type = b2_revoluteJoint;
+	js.localFrameA = base->localFrameA;
+	js.localFrameB = base->localFrameB;
+	js.constraintHertz = base->constraintHertz;
+	js.constraintDampingRatio = base->constraintDampingRatio;
+	js.constraintSoftness = Softness{ 0.0f, 1.0f, 0.0f };
+	js.forceThreshold = base->forceThreshold;
+	js.torqueThreshold = base->torqueThreshold;
+
+	js.targetAngle = b2ClampFloat( def->targetAngle, -B2_PI, B2_PI );
+	js.hertz = def->hertz;
+	js.dampingRatio = def->dampingRatio;
+	float lowerAngle = b2MinFloat( def->lowerAngle, def->upperAngle );
+	float upperAngle = b2MaxFloat( def->lowerAngle, def->upperAngle );
+	js.lowerAngle = b2ClampFloat( lowerAngle, -0.99f * B2_PI, 0.99f * B2_PI );
+	js.upperAngle = b2ClampFloat( upperAngle, -0.99f * B2_PI, 0.99f * B2_PI );
+	js.maxMotorTorque = def->maxMotorTorque;
+	js.motorSpeed = def->motorSpeed;
+	js.enableSpring = def->enableSpring;
+	js.enableLimit = def->enableLimit;
+	js.enableMotor = def->enableMotor;
+
+	if ( joint->setIndex > kDisabledSet )
+	{
+		LinkJoint( joint );
+	}
+
+	jointsDirty = true;
+	listsDirty = true;
+	return b2JointId{ jointId + 1, worldId, joint->generation };
+}
+
+// ---------------------------------------------------------------------------------------------
+// Broadphase pair update (reference src/broad_phase.c:172-529)
+
+void World::UpdateBroadPhasePairs()
+{
+	BroadPhase& bp = broadPhase;
+	int moveCount = (int)bp.moveArray.size();
+	if ( moveCount == 0 )
+	{
+		return;
+	}
+
+	std::vector<std::vector<MovePair>> results( moveCount );
+
+	for ( int i = 0; i < moveCount; ++i )
+	{
+		std::vector<MovePair>& pairList = results[i];
+		int queryProxyKey = bp.moveArray[i];
+		if ( queryProxyKey == B2_NULL_INDEX )
+		{
+			continue;
+		}
+
+		b2BodyType proxyType = B2G_PROXY_TYPE( queryProxyKey );
+		int proxyId = B2G_PROXY_ID( queryProxyKey );
+		const DynamicTree& baseTree = bp.trees[proxyType];
+		b2AABB fatAABB = baseTree.GetAABB( proxyId );
+		int queryShapeIndex = (int)baseTree.GetUserData( proxyId );
+
+		b2BodyType treeType = b2_staticBody;
+		auto callback = [&]( int hitProxyId, uint64_t userData ) -> bool {
+			int shapeId = (int)userData;
+			int proxyKey = B2G_PROXY_KEY( hitProxyId, treeType );
+			if ( proxyKey == queryProxyKey )
+			{
+				return true;
+			}
+
+			if ( proxyType == b2_dynamicBody )
+			{
+				if ( treeType == b2_dynamicBody && proxyKey < queryProxyKey )
+				{
+					if ( bp.movedProxies[treeType].GetBit( hitProxyId ) )
+					{
+						return true;
+					}
+				}
+			}
+			else
+			{
+				if ( bp.movedProxies[treeType].GetBit( hitProxyId ) )
+				{
+					return true;
+				}
+			}
+
+			uint64_t pairKey = B2G_SHAPE_PAIR_KEY( shapeId, queryShapeIndex );
+			if ( bp.pairSet.count( pairKey ) != 0 )
+			{
+				return true;
+			}
+
+			int shapeIdA, shapeIdB;
+			if ( proxyKey < queryProxyKey )
+			{
+				shapeIdA = shapeId;
+				shapeIdB = queryShapeIndex;
+			}
+			else
+			{
+				shapeIdA = queryShapeIndex;
+				shapeIdB = shapeId;
+			}
+
+			Shape* shapeA = &shapes[shapeIdA];
+			Shape* shapeB = &shapes[shapeIdB];
+			if ( shapeA->bodyId == shapeB->bodyId )
+			{
+				return true;
+			}
+			if ( shapeA->sensorIndex != B2_NULL_INDEX || shapeB->sensorIndex != B2_NULL_INDEX )
+			{
+				return true;
+			}
+			if ( ShouldShapesCollide( shapeA->filter, shapeB->filter ) == false )
+			{
+				return true;
+			}
+			if ( CanCollide( shapeA->type, shapeB->type ) == false )
+			{
+				return true;
+			}
+			if ( ShouldBodiesCollide( &bodies[shapeA->bodyId], &bodies[shapeB->bodyId] ) == false )
+			{
+				return true;
+			}
+
+			pairList.push_back( MovePair{ shapeIdA, shapeIdB } );
+			return true;
+		};
+
+		if ( proxyType == b2_dynamicBody )
+		{
+			treeType = b2_kinematicBody;
+			bp.trees[b2_kinematicBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
+			treeType = b2_staticBody;
+			bp.trees[b2_staticBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
+		}
+
+		treeType = b2_dynamicBody;
+		bp.trees[b2_dynamicBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
+	}
+
+	// The tree rebuild runs after the queries (the reference overlaps it with the narrowphase).
+	bp.trees[b2_dynamicBody].Rebuild( false );
+	bp.trees[b2_kinematicBody].Rebuild( false );
+
+	// Create contacts serially in move-buffer order; each proxy's pairs were linked head-first,
+	// so they are created in reverse query order.
+	for ( int i = 0; i < moveCount; ++i )
+	{
+		std::vector<MovePair>& pairList = results[i];
+		for ( int j = (int)pairList.size() - 1; j >= 0; --j )
+		{
+			CreateContact( &shapes[pairList[j].shapeIndexA], &shapes[pairList[j].shapeIndexB] );
+		}
+	}
+
+	for ( int proxyKey : bp.moveArray )
+	{
+		bp.movedProxies[B2G_PROXY_TYPE( proxyKey )].ClearBit( B2G_PROXY_ID( proxyKey ) );
+	}
+	bp.moveArray.clear();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Contact state changes reported by the GPU narrowphase, applied in ascending contact id
+// (reference src/physics_world.c:726-836)
+
+void World::ProcessContactStates( const int* changes, int count )
+{
+	SolverSet& awakeSet = solverSets[kAwakeSet];
+
+	for ( int k = 0; k < count; ++k )
+	{
+		int contactId = changes[k] >> 2;
+		int kind = changes[k] & 3;
+		Contact* contact = &contacts[contactId];
+
+		int colorIndex = contact->colorIndex;
+		int localIndex = contact->localIndex;
+
+		if ( kind == 1 )
+		{
+			DestroyContact( contact );
+		}
+		else if ( kind == 2 )
+		{
+			contact->flags |= kContactTouching;
+			LinkContact( contact );
+			AddContactToGraph( contact );
+
+			int movedIndex = RemoveSwap( awakeSet.contactIds, localIndex );
+			if ( movedIndex != B2_NULL_INDEX )
+			{
+				contacts[awakeSet.contactIds[localIndex]].localIndex = localIndex;
+			}
+		}
+		else if ( kind == 3 )
+		{
+			contact->flags &= ~kContactTouching;
+			UnlinkContact( contact );
+			int bodyIdA = contact->edges[0].bodyId;
+			int bodyIdB = contact->edges[1].bodyId;
+
+			contact->colorIndex = B2_NULL_INDEX;
+			contact->localIndex = (int)awakeSet.contactIds.size();
+			awakeSet.contactIds.push_back( contactId );
+
+			RemoveContactFromGraph( bodyIdA, bodyIdB, colorIndex, localIndex );
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Step
+
+static Softness HostMakeSoft( float hertz, float zeta, float h )
+{
+	if ( hertz == 0.0f )
+	{
+		return Softness{ 0.0f, 0.0f, 0.0f };
+	}
+	float omega = 2.0f * B2_PI * hertz;
+	float a1 = 2.0f * zeta + h * omega;
+	float a2 = h * omega * a1;
+	float a3 = 1.0f / ( 1.0f + a2 );
+	return Softness{ omega / a1, a2 * a3, a3 };
+}
+
+static void BuildSolverLists( World& w, SolverLists& L )
+{
+	L = SolverLists{};
+	L.awakeBodies = w.solverSets[kAwakeSet].bodyIds;
+
+	for ( int i = 0; i < kGraphColorCount; ++i )
+	{
+		L.collideContacts.insert( L.collideContacts.end(), w.colors[i].contactIds.begin(), w.colors[i].contactIds.end() );
+	}
+	const std::vector<int>& nonTouching = w.solverSets[kAwakeSet].contactIds;
+	L.collideContacts.insert( L.collideContacts.end(), nonTouching.begin(), nonTouching.end() );
+
+	for ( int i = 0; i < kOverflowIndex; ++i )
+	{
+		const GraphColor& color = w.colors[i];
+		if ( color.contactIds.size() + color.jointIds.size() == 0 )
+		{
+			continue;
+		}
+		L.colorContactStart.push_back( (int)L.constraintContacts.size() );
+		L.colorJointStart.push_back( (int)L.jointList.size() );
+		L.constraintContacts.insert( L.constraintContacts.end(), color.contactIds.begin(), color.contactIds.end() );
+		L.jointList.insert( L.jointList.end(), color.jointIds.begin(), color.jointIds.end() );
+		L.activeColorCount += 1;
+	}
+	L.colorContactStart.push_back( (int)L.constraintContacts.size() );
+	L.colorJointStart.push_back( (int)L.jointList.size() );
+
+	const GraphColor& overflow = w.colors[kOverflowIndex];
+	L.overflowContactStart = (int)L.constraintContacts.size();
+	L.overflowContactCount = (int)overflow.contactIds.size();
+	L.constraintContacts.insert( L.constraintContacts.end(), overflow.contactIds.begin(), overflow.contactIds.end() );
+	L.overflowJointStart = (int)L.jointList.size();
+	L.overflowJointCount = (int)overflow.jointIds.size();
+	L.jointList.insert( L.jointList.end(), overflow.jointIds.begin(), overflow.jointIds.end() );
+}
+
+void World::Step( float timeStep, int subStepCount )
+{
+	if ( locked )
+	{
+		return;
+	}
+
+	moveEvents.clear();
+	moveEventBodyIds.clear();
+	moveEventsNeedTransforms = false;
+	profile = b2Profile{};
+	recycledContactCount = 0;
+
+	locked = true;
+	uint64_t stepTicks = b2GetTicks();
+
+	{
+		uint64_t pairTicks = b2GetTicks();
+		UpdateBroadPhasePairs();
+		profile.pairs = b2GetMilliseconds( pairTicks );
+	}
+
+	StepParams params = {};
+	params.dt = timeStep;
+	params.subStepCount = b2MaxInt( 1, subStepCount );
+	if ( timeStep > 0.0f )
+	{
+		params.inv_dt = 1.0f / timeStep;
+		params.h = timeStep / params.subStepCount;
+		params.inv_h = params.subStepCount * params.inv_dt;
+	}
+	inv_h = params.inv_h;
+	inv_dt = params.inv_dt;
+
+	float hertz = b2MinFloat( contactHertz, 0.125f * params.inv_h );
+	params.contactSoftness = HostMakeSoft( hertz, contactDampingRatio, params.h );
+	params.staticSoftness = HostMakeSoft( 2.0f * hertz, contactDampingRatio, params.h );
+	params.restitutionThreshold = restitutionThreshold;
+	params.maxLinearVelocity = maxLinearSpeed;
+	params.contactSpeed = contactSpeed;
+	params.contactHertz = contactHertz;
+	params.contactDampingRatio = contactDampingRatio;
+	params.hitEventThreshold = hitEventThreshold;
+	params.gravity = gravity;
+	params.contactRecycleDistance = contactRecycleDistance;
+	params.enableWarmStarting = enableWarmStarting ? 1 : 0;
+	params.enableSleep = enableSleep ? 1 : 0;
+	params.enableContinuous = enableContinuous ? 1 : 0;
+	params.enableContactSoftening = enableContactSoftening ? 1 : 0;
+
+	// Host edits made since the last step go to the GPU before any stage reads them.
+	PushBodiesIfDirty();
+	if ( shapesDirty )
+	{
+		device->UploadShapes( *this );
+		shapesDirty = false;
+	}
+	if ( jointsDirty )
+	{
+		device->UploadJoints( *this );
+		jointsDirty = false;
+	}
+	device->UploadContactInits( pendingContactInits, contactIdPool.Capacity() );
+	pendingContactInits.clear();
+
+	SolverLists lists;
+	if ( listsDirty )
+	{
+		BuildSolverLists( *this, lists );
+		device->UploadLists( lists );
+		listsDirty = false;
+	}
+
+	{
+		uint64_t collideTicks = b2GetTicks();
+		CollideResult cr;
+		device->Collide( params, &cr );
+		recycledContactCount = cr.recycledCount;
+		if ( cr.changes.empty() == false )
+		{
+			ProcessContactStates( cr.changes.data(), (int)cr.changes.size() );
+		}
+		profile.collide = b2GetMilliseconds( collideTicks );
+	}
+
+	if ( timeStep > 0.0f )
+	{
+		uint64_t solveTicks = b2GetTicks();
+		Solve( params );
+		profile.solve = b2GetMilliseconds( solveTicks );
+	}
+
+	profile.step = b2GetMilliseconds( stepTicks );
+	locked = false;
+}
+
+void World::Solve( StepParams& params )
+{
+	stepIndex += 1;
+
+	SolverSet& awakeSet = solverSets[kAwakeSet];
+	int awakeBodyCount = (int)awakeSet.bodyIds.size();
+	if ( awakeBodyCount == 0 )
+	{
+		return;
+	}
+
+	if ( splitIslandId != B2_NULL_INDEX )
+	{
+		SplitIsland( splitIslandId );
+		splitIslandId = B2_NULL_INDEX;
+	}
+
+	// Woken bodies may have rewritten states on the host.
+	PushBodiesIfDirty();
+
+	if ( listsDirty )
+	{
+		SolverLists lists;
+		BuildSolverLists( *this, lists );
+		device->UploadLists( lists );
+		listsDirty = false;
+	}
+
+	FinalizeResult fr;
+	uint64_t constraintTicks = b2GetTicks();
+	device->Solve( params, restitutionShapeCount > 0, enableSleep, &fr );
+	profile.constraints = b2GetMilliseconds( constraintTicks );
+
+	bodiesStale = true;
+	shapesStale = true;
+	jointsStale = true;
+
+	if ( fr.fastBodyCount > 0 )
+	{
+		Fatal( "continuous collision for fast bodies is not supported by the GPU step yet (%d fast bodies)", fr.fastBodyCount );
+	}
+
+	// Move events: one per awake body in awake order (transforms fetched lazily).
+	moveEventBodyIds = awakeSet.bodyIds;
+	moveEvents.resize( awakeBodyCount );
+	for ( int i = 0; i < awakeBodyCount; ++i )
+	{
+		int bodyId = moveEventBodyIds[i];
+		Body& body = bodies[bodyId];
+		body.bodyMoveIndex = i;
+		moveEvents[i].userData = body.userData;
+		moveEvents[i].bodyId = b2BodyId{ bodyId + 1, worldId, body.generation };
+		moveEvents[i].fellAsleep = false;
+	}
+	moveEventsNeedTransforms = true;
+
+	// Refit: enlarge proxies of enlarged bodies in awake order (reference src/solver.c:1822-1915).
+	{
+		uint64_t refitTicks = b2GetTicks();
+		std::vector<int> shapeSlot;
+		if ( fr.enlargedShapes.empty() == false )
+		{
+			for ( size_t k = 0; k < fr.enlargedShapes.size(); ++k )
+			{
+				int shapeId = fr.enlargedShapes[k];
+				if ( (int)shapeSlot.size() <= shapeId )
+				{
+					shapeSlot.resize( shapeId + 1, -1 );
+				}
+				shapeSlot[shapeId] = (int)k;
+			}
+		}
+
+		for ( int awakeIndex : fr.enlargedBodies )
+		{
+			int bodyId = moveEventBodyIds[awakeIndex];
+			int shapeId = bodies[bodyId].headShapeId;
+			while ( shapeId != B2_NULL_INDEX )
+			{
+				Shape& shape = shapes[shapeId];
+				if ( shapeId < (int)shapeSlot.size() && shapeSlot[shapeId] >= 0 )
+				{
+					b2AABB fatAABB = fr.enlargedShapeAABBs[shapeSlot[shapeId]];
+					broadPhase.EnlargeProxy( shape.proxyKey, fatAABB );
+				}
+				shapeId = shape.nextShapeId;
+			}
+		}
+		profile.refit = b2GetMilliseconds( refitTicks );
+	}
+
+	// Island sleep (reference src/solver.c:2024-2080)
+	if ( enableSleep )
+	{
+		uint64_t sleepTicks = b2GetTicks();
+		int islandCount = (int)awakeSet.islandIds.size();
+		std::vector<char> awakeIslands( islandCount, 0 );
+		float splitSleepTimer = 0.0f;
+		int candidate = B2_NULL_INDEX;
+		for ( int i = 0; i < awakeBodyCount; ++i )
+		{
+			int bodyId = moveEventBodyIds[i];
+			Body& body = bodies[bodyId];
+			float sleepTime = fr.sleepTimes[i];
+			Island& island = islands[body.islandId];
+			if ( sleepTime < B2_TIME_TO_SLEEP )
+			{
+				awakeIslands[island.localIndex] = 1;
+			}
+			else if ( island.constraintRemoveCount > 0 )
+			{
+				if ( sleepTime > splitSleepTimer || ( sleepTime == splitSleepTimer && body.islandId > candidate ) )
+				{
+					candidate = body.islandId;
+					splitSleepTimer = sleepTime;
+				}
+			}
+		}
+		splitIslandId = candidate;
+
+		for ( int islandIndex = islandCount - 1; islandIndex >= 0; islandIndex -= 1 )
+		{
+			if ( awakeIslands[islandIndex] )
+			{
+				continue;
+			}
+			int islandId = awakeSet.islandIds[islandIndex];
+			TrySleepIsland( islandId );
+		}
+		profile.sleepIslands = b2GetMilliseconds( sleepTicks );
+	}
+}
+
+void World::EnableSleeping( bool flag )
+{
+	if ( locked )
+	{
+		return;
+	}
+	if ( flag == enableSleep )
+	{
+		return;
+	}
+	enableSleep = flag;
+	if ( flag == false )
+	{
+		int setCount = (int)solverSets.size();
+		for ( int i = kFirstSleepingSet; i < setCount; ++i )
+		{
+			if ( solverSets[i].setIndex != B2_NULL_INDEX && solverSets[i].bodyIds.empty() == false )
+			{
+				WakeSolverSet( i );
+			}
+		}
+	}
+}
+
+b2BodyEvents World::GetBodyEvents()
+{
+	if ( moveEventsNeedTransforms )
+	{
+		PullBodies();
+		for ( size_t i = 0; i < moveEvents.size(); ++i )
+		{
+			moveEvents[i].transform = bodySims[moveEventBodyIds[i]].transform;
+		}
+		moveEventsNeedTransforms = false;
+	}
+	return b2BodyEvents{ moveEvents.data(), (int)moveEvents.size() };
+}
+
+b2Counters World::GetCounters()
+{
+	b2Counters s = {};
+	s.bodyCount = bodyIdPool.Count();
+	s.shapeCount = shapeIdPool.Count();
+	s.contactCount = contactIdPool.Count();
+	s.jointCount = jointIdPool.Count();
+	s.islandCount = islandIdPool.Count();
+	s.staticTreeHeight = broadPhase.trees[b2_staticBody].GetHeight();
+	s.treeHeight = b2MaxInt( broadPhase.trees[b2_dynamicBody].GetHeight(), broadPhase.trees[b2_kinematicBody].GetHeight() );
+	s.recycledContactCount = recycledContactCount;
+	s.awakeContactCount = 0;
+	for ( int i = 0; i < kGraphColorCount; ++i )
+	{
+		s.colorCounts[i] = (int)( colors[i].contactIds.size() + colors[i].jointIds.size() );
+		s.awakeContactCount += (int)colors[i].contactIds.size();
+	}
+	s.awakeContactCount += (int)solverSets[kAwakeSet].contactIds.size();
+	return s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// World registry entry points used by the C API
+
+World* CreateWorldInternal( const b2WorldDef* def, b2WorldId* outId )
+{
+	int index = -1;
+	for ( int i = 0; i < B2_MAX_WORLDS; ++i )
+	{
+		if ( s_worlds[i] == nullptr )
+		{
+			index = i;
+			break;
+		}
+	}
+	if ( index < 0 )
+	{
+		*outId = b2_nullWorldId;
+		return nullptr;
+	}
+	uint16_t gen = s_generations[index];
+	World* world = new World( def, index, gen );
+	s_worlds[index] = world;
+	*outId = b2WorldId{ (uint16_t)( index + 1 ), gen };
+	return world;
+}
+
+void DestroyWorldInternal( World* world )
+{
+	int index = world->worldId;
+	s_generations[index] = (uint16_t)( world->generation + 1 );
+	s_worlds[index] = nullptr;
+	delete world;
+}
+
+void SetLogFcn( b2LogFcn* fcn )
+{
+	s_logFcn = fcn;
+}
+
+void SetAssertFcn( b2AssertFcn* fcn )
+{
+	s_assertFcn = fcn;
+}
+
+} // namespace b2g

@@ -1,0 +1,349 @@
+// Host-side world: the bookkeeping half of b2World_Step.
+//
+// The GPU runs the arithmetic of the step (narrowphase, constraint preparation, the sub-stepped
+// soft-step solver and body finalization). The host keeps what decides *order*: id pools, the
+// broadphase trees and move buffer, contact creation and destruction, graph coloring, islands and
+// solver sets. Every ordering decision follows the reference so that the GPU solves the same
+// constraints in the same Gauss-Seidel order and the run reproduces the reference bit for bit.
+#pragma once
+
+#include "core.h"
+#include "layout.h"
+#include "tree.h"
+
+#include <unordered_set>
+#include <vector>
+
+namespace b2g
+{
+
+class DeviceWorld;
+
+enum SolverSetType
+{
+	kStaticSet = 0,
+	kDisabledSet = 1,
+	kAwakeSet = 2,
+	kFirstSleepingSet = 3,
+};
+
+constexpr int kGraphColorCount = B2_GRAPH_COLOR_COUNT;
+constexpr int kOverflowIndex = kGraphColorCount - 1;
+constexpr int kDynamicColorCount = kGraphColorCount - 4;
+
+#define B2G_PROXY_TYPE( KEY ) ( (b2BodyType)( ( KEY ) & 3 ) )
+#define B2G_PROXY_ID( KEY ) ( ( KEY ) >> 2 )
+#define B2G_PROXY_KEY( ID, TYPE ) ( ( ( ID ) << 2 ) | ( TYPE ) )
+#define B2G_SHAPE_PAIR_KEY( K1, K2 ) ( ( K1 ) < ( K2 ) ? (uint64_t)( K1 ) << 32 | (uint64_t)( K2 ) : (uint64_t)( K2 ) << 32 | (uint64_t)( K1 ) )
+
+struct Body
+{
+	void* userData = nullptr;
+	int setIndex = B2_NULL_INDEX;
+	int localIndex = B2_NULL_INDEX;
+	int headContactKey = B2_NULL_INDEX;
+	int contactCount = 0;
+	int headShapeId = B2_NULL_INDEX;
+	int shapeCount = 0;
+	int headChainId = B2_NULL_INDEX;
+	int headJointKey = B2_NULL_INDEX;
+	int jointCount = 0;
+	int islandId = B2_NULL_INDEX;
+	int islandIndex = B2_NULL_INDEX;
+	float mass = 0.0f;
+	float inertia = 0.0f;
+	int bodyMoveIndex = B2_NULL_INDEX;
+	int id = B2_NULL_INDEX;
+	uint32_t flags = 0;
+	b2BodyType type = b2_staticBody;
+	uint16_t generation = 0;
+	char name[B2_NAME_LENGTH + 1] = {};
+};
+
+struct Shape
+{
+	int id = B2_NULL_INDEX;
+	int bodyId = B2_NULL_INDEX;
+	int prevShapeId = B2_NULL_INDEX;
+	int nextShapeId = B2_NULL_INDEX;
+	int sensorIndex = B2_NULL_INDEX;
+	b2ShapeType type = b2_circleShape;
+	b2SurfaceMaterial material = {};
+	float density = 0.0f;
+	int proxyKey = B2_NULL_INDEX;
+	b2Filter filter = {};
+	void* userData = nullptr;
+	uint16_t generation = 0;
+	bool enableSensorEvents = false;
+	bool enableContactEvents = false;
+	bool enableCustomFiltering = false;
+	bool enableHitEvents = false;
+	bool enablePreSolveEvents = false;
+};
+
+struct ChainShape
+{
+	int id = B2_NULL_INDEX;
+	int bodyId = B2_NULL_INDEX;
+	int nextChainId = B2_NULL_INDEX;
+	std::vector<int> shapeIndices;
+	uint16_t generation = 0;
+};
+
+struct ContactEdge
+{
+	int bodyId;
+	int prevKey;
+	int nextKey;
+};
+
+struct Contact
+{
+	ContactEdge edges[2];
+	int islandId = B2_NULL_INDEX;
+	int islandIndex = B2_NULL_INDEX;
+	int setIndex = B2_NULL_INDEX;
+	int colorIndex = B2_NULL_INDEX;
+	int localIndex = B2_NULL_INDEX;
+	int shapeIdA = B2_NULL_INDEX;
+	int shapeIdB = B2_NULL_INDEX;
+	int contactId = B2_NULL_INDEX;
+	uint32_t flags = 0;
+	uint32_t generation = 0;
+};
+
+struct JointEdge
+{
+	int bodyId;
+	int prevKey;
+	int nextKey;
+};
+
+struct Joint
+{
+	void* userData = nullptr;
+	int setIndex = B2_NULL_INDEX;
+	int colorIndex = B2_NULL_INDEX;
+	int localIndex = B2_NULL_INDEX;
+	JointEdge edges[2];
+	int jointId = B2_NULL_INDEX;
+	int islandId = B2_NULL_INDEX;
+	int islandIndex = B2_NULL_INDEX;
+	b2JointType type = b2_revoluteJoint;
+	uint16_t generation = 0;
+	bool collideConnected = false;
+};
+
+struct ContactLink
+{
+	int contactId;
+	int bodyIdA;
+	int bodyIdB;
+};
+
+struct JointLink
+{
+	int jointId;
+	int bodyIdA;
+	int bodyIdB;
+};
+
+struct Island
+{
+	int setIndex = B2_NULL_INDEX;
+	int localIndex = B2_NULL_INDEX;
+	int islandId = B2_NULL_INDEX;
+	int constraintRemoveCount = 0;
+	std::vector<int> bodies;
+	std::vector<ContactLink> contacts;
+	std::vector<JointLink> joints;
+};
+
+// A solver set lists the ids that live in it, in the reference's array order
+// (reference src/solver_set.h:17-29). Only the order matters on the host; body and contact data
+// stay in the id-indexed arrays.
+struct SolverSet
+{
+	std::vector<int> bodyIds;
+	std::vector<int> contactIds;
+	std::vector<int> jointIds;
+	std::vector<int> islandIds;
+	int setIndex = B2_NULL_INDEX;
+};
+
+struct GraphColor
+{
+	BitSet bodySet;
+	std::vector<int> contactIds;
+	std::vector<int> jointIds;
+};
+
+struct MovePair
+{
+	int shapeIndexA;
+	int shapeIndexB;
+};
+
+class BroadPhase
+{
+public:
+	DynamicTree trees[b2_bodyTypeCount];
+	BitSet movedProxies[b2_bodyTypeCount];
+	std::vector<int> moveArray;
+	std::unordered_set<uint64_t> pairSet;
+
+	void BufferMove( int queryProxy )
+	{
+		b2BodyType proxyType = B2G_PROXY_TYPE( queryProxy );
+		int proxyId = B2G_PROXY_ID( queryProxy );
+		if ( movedProxies[proxyType].GetBit( proxyId ) == false )
+		{
+			movedProxies[proxyType].SetBitGrow( proxyId );
+			moveArray.push_back( queryProxy );
+		}
+	}
+
+	void UnBufferMove( int proxyKey );
+	int CreateProxy( b2AABB aabb, b2BodyType proxyType, uint64_t categoryBits, int shapeIndex, bool forcePairCreation );
+	void DestroyProxy( int proxyKey );
+	void EnlargeProxy( int proxyKey, b2AABB aabb );
+};
+
+class World
+{
+public:
+	explicit World( const b2WorldDef* def, int worldIndex, uint16_t generation );
+	~World();
+
+	// Public API helpers
+	b2BodyId CreateBody( const b2BodyDef* def );
+	b2ShapeId CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void* geometry, b2ShapeType type );
+	b2ChainId CreateChain( b2BodyId bodyId, const b2ChainDef* def );
+	b2JointId CreateRevoluteJoint( const b2RevoluteJointDef* def );
+	void Step( float timeStep, int subStepCount );
+	void EnableSleeping( bool flag );
+
+	// Host copies of GPU state are refreshed lazily after a step.
+	void PullBodies();
+	void PushBodiesIfDirty();
+
+	b2BodyEvents GetBodyEvents();
+	b2Counters GetCounters();
+	bool WakeBodyPublic( Body* body )
+	{
+		return WakeBody( body );
+	}
+
+	Body* GetBody( b2BodyId id );
+
+	// identity
+	uint16_t worldId;
+	uint16_t generation;
+	bool locked = false;
+
+	// settings
+	b2Vec2 gravity;
+	float hitEventThreshold;
+	float restitutionThreshold;
+	float maxLinearSpeed;
+	float contactSpeed;
+	float contactHertz;
+	float contactDampingRatio;
+	float contactRecycleDistance;
+	bool enableSleep;
+	bool enableWarmStarting = true;
+	bool enableContinuous;
+	bool enableContactSoftening;
+	void* userData = nullptr;
+	int workerCount = 1;
+	int stepIndex = 0;
+	int splitIslandId = B2_NULL_INDEX;
+	float inv_h = 0.0f;
+	float inv_dt = 0.0f;
+
+	// id pools and host records
+	IdPool bodyIdPool, shapeIdPool, chainIdPool, contactIdPool, jointIdPool, islandIdPool, solverSetIdPool;
+	std::vector<Body> bodies;
+	std::vector<Shape> shapes;
+	std::vector<ChainShape> chains;
+	std::vector<Contact> contacts;
+	std::vector<Joint> joints;
+	std::vector<Island> islands;
+	std::vector<SolverSet> solverSets;
+	GraphColor colors[kGraphColorCount];
+	BroadPhase broadPhase;
+
+	// host mirrors of the id-indexed GPU arrays
+	std::vector<BodySim> bodySims;
+	std::vector<BodyState> bodyStates;
+	std::vector<BodyAux> bodyAux;
+	std::vector<ShapeDev> shapeData;
+	std::vector<JointSim> jointSims;
+	std::vector<ContactInit> pendingContactInits;
+
+	bool bodiesDirty = true;	// host copy newer than device
+	bool bodiesStale = false;	// device copy newer than host
+	bool shapesDirty = true;
+	bool shapesStale = false;
+	bool jointsDirty = true;
+	bool jointsStale = false;
+	bool listsDirty = true; // solver work lists (awake bodies, colors, contacts) changed
+	int restitutionShapeCount = 0;
+
+	// events and statistics
+	std::vector<b2BodyMoveEvent> moveEvents;
+	std::vector<int> moveEventBodyIds;
+	bool moveEventsNeedTransforms = false;
+	b2Profile profile = {};
+	int recycledContactCount = 0;
+
+	DeviceWorld* device = nullptr;
+
+private:
+	friend class DeviceWorld;
+
+	// bodies / sets / islands (reference body.c, solver_set.c, island.c)
+	void UpdateBodyMassData( Body* body );
+	void CreateIslandForBody( int setIndex, Body* body );
+	Island* CreateIsland( int setIndex );
+	void DestroyIsland( int islandId );
+	int MergeIslands( int islandIdA, int islandIdB );
+	void AddContactToIsland( int islandId, Contact* contact );
+	void LinkContact( Contact* contact );
+	void UnlinkContact( Contact* contact );
+	void AddJointToIsland( int islandId, Joint* joint );
+	void LinkJoint( Joint* joint );
+	void SplitIsland( int baseId );
+	void WakeSolverSet( int setIndex );
+	void TrySleepIsland( int islandId );
+	void MergeSolverSets( int setId1, int setId2 );
+	void DestroySolverSet( int setIndex );
+	bool WakeBody( Body* body );
+	void RemoveBodyFromAwakeSet( Body* body );
+	bool ShouldBodiesCollide( Body* bodyA, Body* bodyB );
+
+	// shapes / contacts / graph
+	void CreateShapeProxy( Shape* shape, bool forcePairCreation );
+	void CreateContact( Shape* shapeA, Shape* shapeB );
+	void DestroyContact( Contact* contact );
+	void AddContactToGraph( Contact* contact );
+	void RemoveContactFromGraph( int bodyIdA, int bodyIdB, int colorIndex, int localIndex );
+	int AssignJointColor( int bodyIdA, int bodyIdB, b2BodyType typeA, b2BodyType typeB );
+	void AddJointToGraph( Joint* joint );
+	void RemoveJointFromGraph( int bodyIdA, int bodyIdB, int colorIndex, int localIndex );
+	void DestroyContactsBetweenBodies( Body* bodyA, Body* bodyB );
+
+	// step stages
+	void UpdateBroadPhasePairs();
+	void ProcessContactStates( const int* changedIds, int count );
+	void Solve( StepParams& params );
+	void FinishSolve( StepParams& params );
+
+	void PullShapes();
+	void PullJoints();
+};
+
+World* GetWorld( b2WorldId id );
+World* GetWorldFromIndex( int index );
+
+} // namespace b2g

@@ -1,0 +1,506 @@
+// Scene driver for the Box2D step path. Public Box2D C API only.
+//
+// The pyramid follows the reference benchmark scene (shared/benchmarks.c:88-131), the falling
+// hinges follow the reference determinism scene (shared/determinism.c:11-134) and the hello
+// scene follows test/test_world.c:19-98. The random generator is XorShift32 seeded with 12345,
+// the generator the reference scene library uses (shared/utils.h:9-36).
+
+#include "scenes.h"
+
+#include "box2d/box2d.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define B2S_MAX_HANDLES 4096
+
+typedef struct b2sScene
+{
+	b2WorldId worldId;
+	b2BodyId* bodies;
+	int bodyCount;
+	int inUse;
+} b2sScene;
+
+static b2sScene s_scenes[B2S_MAX_HANDLES];
+
+static uint32_t s_rngState = 12345u;
+
+static uint32_t b2sNextRandom( void )
+{
+	uint32_t x = s_rngState;
+	x ^= x << 13;
+	x ^= x >> 17;
+	x ^= x << 5;
+	s_rngState = x;
+	return x;
+}
+
+// Uniform in [lo, hi] with 15-bit resolution.
+static float b2sRandomRange( float lo, float hi )
+{
+	float r = (float)( b2sNextRandom() % 32768u ) / 32767.0f;
+	return lo + r * ( hi - lo );
+}
+
+static void b2sPushBody( b2sScene* scene, int* capacity, b2BodyId id )
+{
+	if ( scene->bodyCount == *capacity )
+	{
+		*capacity = *capacity == 0 ? 64 : 2 * *capacity;
+		scene->bodies = (b2BodyId*)realloc( scene->bodies, (size_t)*capacity * sizeof( b2BodyId ) );
+	}
+	scene->bodies[scene->bodyCount++] = id;
+}
+
+static void b2sBuildPyramid( b2sScene* scene, int rows, float groundHalfWidth, int enableSleep )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Pos ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( groundHalfWidth, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.density = 1.0f;
+
+	float a = 0.5f;
+	b2Polygon box = b2MakeSquare( a );
+	float shift = 1.0f * a;
+	int capacity = 0;
+
+	for ( int i = 0; i < rows; ++i )
+	{
+		float y = ( 2.0f * i + 1.0f ) * shift;
+		for ( int j = i; j < rows; ++j )
+		{
+			float x = ( i + 1.0f ) * shift + 2.0f * ( j - i ) * shift - a * rows;
+			bodyDef.position = ( b2Pos ){ x, y };
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			b2CreatePolygonShape( bodyId, &shapeDef, &box );
+			b2sPushBody( scene, &capacity, bodyId );
+		}
+	}
+}
+
+static void b2sBuildFallingHinges( b2sScene* scene )
+{
+	b2WorldId worldId = scene->worldId;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Pos ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( 40.0f, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+
+	int columnCount = 4;
+	int rowCount = 20;
+	float h = 0.25f;
+	b2Polygon box = b2MakeSquare( h );
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+
+	b2RevoluteJointDef jointDef = b2DefaultRevoluteJointDef();
+	jointDef.enableLimit = true;
+	jointDef.lowerAngle = -0.1f * B2_PI;
+	jointDef.upperAngle = 0.2f * B2_PI;
+	jointDef.enableSpring = true;
+	jointDef.hertz = 1.0f;
+	jointDef.dampingRatio = 1.0f;
+	jointDef.enableMotor = true;
+	jointDef.maxMotorTorque = 0.25f;
+	jointDef.base.localFrameA.p = ( b2Vec2 ){ -h, h };
+	jointDef.base.localFrameB.p = ( b2Vec2 ){ -h, -h };
+	jointDef.base.constraintHertz = 60.0f;
+	jointDef.base.constraintDampingRatio = 0.0f;
+	jointDef.base.drawScale = 0.5f;
+
+	float offset = 0.4f * h;
+	float dx = 10.0f * h;
+	float xBase = -0.5f * dx * ( columnCount - 1.0f );
+	int capacity = 0;
+
+	for ( int j = 0; j < columnCount; ++j )
+	{
+		float x = xBase + j * dx;
+		b2BodyId prevBodyId = b2_nullBodyId;
+		for ( int i = 0; i < rowCount; ++i )
+		{
+			b2BodyDef bodyDef = b2DefaultBodyDef();
+			bodyDef.type = b2_dynamicBody;
+			bodyDef.position.x = x + offset * i;
+			bodyDef.position.y = h + 2.0f * h * i;
+			float angle = ( i & 1 ) == 0 ? -0.1f : 0.1f;
+			bodyDef.rotation = b2MakeRot( angle );
+
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			if ( ( i & 1 ) == 0 )
+			{
+				prevBodyId = bodyId;
+			}
+			else
+			{
+				jointDef.base.bodyIdA = prevBodyId;
+				jointDef.base.bodyIdB = bodyId;
+				b2CreateRevoluteJoint( worldId, &jointDef );
+				prevBodyId = b2_nullBodyId;
+			}
+
+			b2CreatePolygonShape( bodyId, &shapeDef, &box );
+			b2sPushBody( scene, &capacity, bodyId );
+		}
+	}
+}
+
+static void b2sBuildHello( b2sScene* scene )
+{
+	b2WorldId worldId = scene->worldId;
+	b2BodyDef groundBodyDef = b2DefaultBodyDef();
+	groundBodyDef.position = ( b2Pos ){ 0.0f, -10.0f };
+	b2BodyId groundId = b2CreateBody( worldId, &groundBodyDef );
+	b2Polygon groundBox = b2MakeBox( 50.0f, 10.0f );
+	b2ShapeDef groundShapeDef = b2DefaultShapeDef();
+	b2CreatePolygonShape( groundId, &groundShapeDef, &groundBox );
+
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	bodyDef.position = ( b2Pos ){ 0.0f, 4.0f };
+	b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+	b2Polygon dynamicBox = b2MakeBox( 1.0f, 1.0f );
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.density = 1.0f;
+	shapeDef.material.friction = 0.3f;
+	b2CreatePolygonShape( bodyId, &shapeDef, &dynamicBox );
+
+	int capacity = 0;
+	b2sPushBody( scene, &capacity, bodyId );
+}
+
+static void b2sBuildMixedPile( b2sScene* scene, int count, int enableSleep, uint32_t seed )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+	s_rngState = seed != 0 ? seed : 12345u;
+
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Pos ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( 40.0f, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+
+	int columns = 20;
+	int capacity = 0;
+	for ( int k = 0; k < count; ++k )
+	{
+		int col = k % columns;
+		int row = k / columns;
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ -19.0f + 2.0f * col + b2sRandomRange( -0.2f, 0.2f ), 1.5f + 2.0f * row };
+		bodyDef.rotation = b2MakeRot( b2sRandomRange( -3.0f, 3.0f ) );
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.density = b2sRandomRange( 0.5f, 2.0f );
+
+		int kind = k & 3;
+		if ( kind == 0 )
+		{
+			b2Circle circle = { { 0.0f, 0.0f }, b2sRandomRange( 0.2f, 0.5f ) };
+			b2CreateCircleShape( bodyId, &shapeDef, &circle );
+		}
+		else if ( kind == 1 )
+		{
+			float halfLength = b2sRandomRange( 0.2f, 0.6f );
+			b2Capsule capsule = { { -halfLength, 0.0f }, { halfLength, 0.0f }, b2sRandomRange( 0.1f, 0.25f ) };
+			b2CreateCapsuleShape( bodyId, &shapeDef, &capsule );
+		}
+		else if ( kind == 2 )
+		{
+			b2Vec2 points[8];
+			int n = 3 + (int)( b2sNextRandom() % 6u );
+			for ( int i = 0; i < n; ++i )
+			{
+				float angle = b2sRandomRange( -3.14f, 3.14f );
+				float radius = b2sRandomRange( 0.3f, 0.9f );
+				b2Rot q = b2MakeRot( angle );
+				points[i] = ( b2Vec2 ){ radius * q.c, radius * q.s };
+			}
+			b2Hull hull = b2ComputeHull( points, n );
+			b2Polygon polygon = hull.count >= 3 ? b2MakePolygon( &hull, 0.0f ) : b2MakeSquare( 0.4f );
+			b2CreatePolygonShape( bodyId, &shapeDef, &polygon );
+		}
+		else
+		{
+			b2Polygon box = b2MakeRoundedBox( b2sRandomRange( 0.2f, 0.5f ), b2sRandomRange( 0.2f, 0.5f ), 0.05f );
+			b2CreatePolygonShape( bodyId, &shapeDef, &box );
+		}
+
+		b2sPushBody( scene, &capacity, bodyId );
+	}
+}
+
+int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount )
+{
+	int handle = -1;
+	for ( int i = 0; i < B2S_MAX_HANDLES; ++i )
+	{
+		if ( s_scenes[i].inUse == 0 )
+		{
+			handle = i;
+			break;
+		}
+	}
+
+	if ( handle < 0 )
+	{
+		return -1;
+	}
+
+	b2WorldDef worldDef = b2DefaultWorldDef();
+	worldDef.workerCount = workerCount;
+	b2WorldId worldId = b2CreateWorld( &worldDef );
+	if ( worldId.index1 == 0 )
+	{
+		return -1;
+	}
+
+	b2sScene* scene = s_scenes + handle;
+	memset( scene, 0, sizeof( *scene ) );
+	scene->worldId = worldId;
+	scene->inUse = 1;
+
+	switch ( kind )
+	{
+		case B2S_PYRAMID:
+		{
+			int rows = iparamCount > 0 ? iparams[0] : 20;
+			int sleep = iparamCount > 1 ? iparams[1] : 0;
+			float groundHalfWidth = fparamCount > 0 ? fparams[0] : 40.0f;
+			b2sBuildPyramid( scene, rows, groundHalfWidth, sleep );
+		}
+		break;
+
+		case B2S_FALLING_HINGES:
+			b2sBuildFallingHinges( scene );
+			break;
+
+		case B2S_HELLO:
+			b2sBuildHello( scene );
+			break;
+
+		case B2S_MIXED_PILE:
+		{
+			int count = iparamCount > 0 ? iparams[0] : 100;
+			int sleep = iparamCount > 1 ? iparams[1] : 0;
+			uint32_t seed = iparamCount > 2 ? (uint32_t)iparams[2] : 12345u;
+			b2sBuildMixedPile( scene, count, sleep, seed );
+		}
+		break;
+
+		default:
+			b2DestroyWorld( worldId );
+			scene->inUse = 0;
+			return -1;
+	}
+
+	return handle;
+}
+
+static b2sScene* b2sGet( int handle )
+{
+	if ( handle < 0 || handle >= B2S_MAX_HANDLES || s_scenes[handle].inUse == 0 )
+	{
+		return NULL;
+	}
+	return s_scenes + handle;
+}
+
+void b2s_destroy( int handle )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return;
+	}
+	b2DestroyWorld( scene->worldId );
+	free( scene->bodies );
+	memset( scene, 0, sizeof( *scene ) );
+}
+
+void b2s_step( int handle, float timeStep, int subStepCount, int count )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return;
+	}
+	for ( int i = 0; i < count; ++i )
+	{
+		b2World_Step( scene->worldId, timeStep, subStepCount );
+	}
+}
+
+static double b2sNowMs( void )
+{
+	struct timespec ts;
+	clock_gettime( CLOCK_MONOTONIC, &ts );
+	return 1000.0 * (double)ts.tv_sec + 1.0e-6 * (double)ts.tv_nsec;
+}
+
+double b2s_step_timed( int handle, float timeStep, int subStepCount, int count )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return -1.0;
+	}
+	double start = b2sNowMs();
+	for ( int i = 0; i < count; ++i )
+	{
+		b2World_Step( scene->worldId, timeStep, subStepCount );
+	}
+	return b2sNowMs() - start;
+}
+
+int b2s_body_count( int handle )
+{
+	b2sScene* scene = b2sGet( handle );
+	return scene == NULL ? 0 : scene->bodyCount;
+}
+
+uint32_t b2s_hash( int handle )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	uint32_t hash = B2_HASH_INIT;
+	for ( int i = 0; i < scene->bodyCount; ++i )
+	{
+		b2WorldTransform transform = b2Body_GetTransform( scene->bodies[i] );
+		hash = b2Hash( hash, (uint8_t*)( &transform ), sizeof( b2WorldTransform ) );
+	}
+	return hash;
+}
+
+int b2s_get_transforms( int handle, float* out, int capacity )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	int n = scene->bodyCount < capacity ? scene->bodyCount : capacity;
+	for ( int i = 0; i < n; ++i )
+	{
+		b2WorldTransform t = b2Body_GetTransform( scene->bodies[i] );
+		out[4 * i + 0] = (float)t.p.x;
+		out[4 * i + 1] = (float)t.p.y;
+		out[4 * i + 2] = t.q.c;
+		out[4 * i + 3] = t.q.s;
+	}
+	return n;
+}
+
+int b2s_get_velocities( int handle, float* out, int capacity )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	int n = scene->bodyCount < capacity ? scene->bodyCount : capacity;
+	for ( int i = 0; i < n; ++i )
+	{
+		b2Vec2 v = b2Body_GetLinearVelocity( scene->bodies[i] );
+		out[3 * i + 0] = v.x;
+		out[3 * i + 1] = v.y;
+		out[3 * i + 2] = b2Body_GetAngularVelocity( scene->bodies[i] );
+	}
+	return n;
+}
+
+int b2s_move_count( int handle )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	b2BodyEvents events = b2World_GetBodyEvents( scene->worldId );
+	return events.moveCount;
+}
+
+int b2s_awake_body_count( int handle )
+{
+	b2sScene* scene = b2sGet( handle );
+	return scene == NULL ? 0 : b2World_GetAwakeBodyCount( scene->worldId );
+}
+
+int b2s_counters( int handle, int* out, int capacity )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL || capacity < 30 )
+	{
+		return 0;
+	}
+	b2Counters c = b2World_GetCounters( scene->worldId );
+	out[0] = c.bodyCount;
+	out[1] = c.shapeCount;
+	out[2] = c.contactCount;
+	out[3] = c.jointCount;
+	out[4] = c.islandCount;
+	out[5] = c.awakeContactCount;
+	for ( int i = 0; i < 24; ++i )
+	{
+		out[6 + i] = c.colorCounts[i];
+	}
+	return 30;
+}
+
+int b2s_falling_hinges( int workerCount, int stepLimit, int* sleepStep, uint32_t* hash )
+{
+	int handle = b2s_create( B2S_FALLING_HINGES, workerCount, NULL, 0, NULL, 0 );
+	if ( handle < 0 )
+	{
+		return 0;
+	}
+
+	b2sScene* scene = s_scenes + handle;
+	uint32_t h = 0;
+	int step = 0;
+	int slept = 0;
+	for ( int i = 0; i < stepLimit; ++i )
+	{
+		b2World_Step( scene->worldId, 1.0f / 60.0f, 4 );
+		b2BodyEvents events = b2World_GetBodyEvents( scene->worldId );
+		if ( events.moveCount == 0 )
+		{
+			h = b2s_hash( handle );
+			slept = 1;
+			break;
+		}
+		step += 1;
+	}
+
+	*sleepStep = slept ? step : -1;
+	*hash = h;
+	b2s_destroy( handle );
+	return slept;
+}

@@ -1,0 +1,82 @@
+// Scene driver for the Box2D step path.
+//
+// Written only against the public Box2D C API (box2d/box2d.h), so the same source compiles
+// against the reference library and against box2d_amd, and drives both through identical
+// calls. Each scene is addressed by a small integer handle so that callers (Python ctypes,
+// C benchmarks) never need Box2D's struct layouts.
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined( _WIN32 )
+#define B2S_API __declspec( dllexport )
+#else
+#define B2S_API __attribute__( ( visibility( "default" ) ) )
+#endif
+
+// Scene kinds
+enum
+{
+	// Triangular box pyramid on a static ground box.
+	// iparams: [0] rows, [1] enableSleep. fparams: [0] ground half width (half height is 1).
+	// Follows the reference benchmark construction (shared/benchmarks.c:88-131).
+	B2S_PYRAMID = 0,
+
+	// Falling hinged boxes used by the reference determinism test (shared/determinism.c:11-107).
+	B2S_FALLING_HINGES = 1,
+
+	// A single dynamic box resting on a ground box (test/test_world.c:19-98 "HelloWorld").
+	B2S_HELLO = 2,
+
+	// Small mixed scene: circles, capsules, polygons and rounded boxes dropped on a ground box.
+	// iparams: [0] body count, [1] enableSleep, [2] seed. fparams: unused.
+	B2S_MIXED_PILE = 3,
+};
+
+// Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.
+// Returns a handle >= 0, or -1 on failure.
+B2S_API int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount );
+
+// Destroy the world behind a handle.
+B2S_API void b2s_destroy( int handle );
+
+// Step the world count times with the given time step and sub-step count.
+B2S_API void b2s_step( int handle, float timeStep, int subStepCount, int count );
+
+// Wall-clock milliseconds spent in count steps (monotonic clock around b2World_Step calls).
+B2S_API double b2s_step_timed( int handle, float timeStep, int subStepCount, int count );
+
+// Number of bodies created by the scene (static ground excluded).
+B2S_API int b2s_body_count( int handle );
+
+// djb2 hash (b2Hash) of the world transforms of the scene bodies in creation order,
+// the same construction as the reference determinism check (shared/determinism.c:110-125).
+B2S_API uint32_t b2s_hash( int handle );
+
+// Copy the world transforms (p.x, p.y, q.c, q.s) of the scene bodies in creation order.
+// Returns the number of bodies written.
+B2S_API int b2s_get_transforms( int handle, float* out, int capacity );
+
+// Copy the linear and angular velocities (v.x, v.y, w) of the scene bodies.
+B2S_API int b2s_get_velocities( int handle, float* out, int capacity );
+
+// Body move event count and awake body count of the last step.
+B2S_API int b2s_move_count( int handle );
+B2S_API int b2s_awake_body_count( int handle );
+
+// World counters: out[0..5] = bodyCount, shapeCount, contactCount, jointCount, islandCount,
+// awakeContactCount; out[6..29] = colorCounts[0..23]. Returns the number of ints written (30).
+B2S_API int b2s_counters( int handle, int* out, int capacity );
+
+// Run the reference determinism protocol on the falling hinges scene: step at 1/60 with 4
+// sub-steps until every body sleeps or stepLimit is reached.
+// Writes the sleep step and the transform hash at that step. Returns 1 if the scene slept.
+B2S_API int b2s_falling_hinges( int workerCount, int stepLimit, int* sleepStep, uint32_t* hash );
+
+#ifdef __cplusplus
+}
+#endif

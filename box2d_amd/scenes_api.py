@@ -1,0 +1,92 @@
+"""ctypes wrapper for the public-API scene driver (box2d_amd/scenes/scenes.c).
+
+The same driver source is compiled against this library and against the reference library, so a
+SceneLib can drive either one through identical Box2D API calls.
+"""
+
+import ctypes
+
+import numpy as np
+
+SCENE_PYRAMID = 0
+SCENE_FALLING_HINGES = 1
+SCENE_HELLO = 2
+SCENE_MIXED_PILE = 3
+
+
+class SceneLib:
+    def __init__(self, path):
+        self.path = path
+        lib = ctypes.CDLL(path)
+        lib.b2s_create.restype = ctypes.c_int
+        lib.b2s_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+        lib.b2s_destroy.argtypes = [ctypes.c_int]
+        lib.b2s_step.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int]
+        lib.b2s_step_timed.restype = ctypes.c_double
+        lib.b2s_step_timed.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int]
+        lib.b2s_body_count.restype = ctypes.c_int
+        lib.b2s_hash.restype = ctypes.c_uint32
+        lib.b2s_get_transforms.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+        lib.b2s_get_velocities.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+        lib.b2s_counters.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        lib.b2s_falling_hinges.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_uint32)]
+        self.lib = lib
+
+    def create(self, kind, iparams=(), fparams=(), workers=1):
+        ip = (ctypes.c_int * max(1, len(iparams)))(*iparams)
+        fp = (ctypes.c_float * max(1, len(fparams)))(*fparams)
+        h = self.lib.b2s_create(kind, workers, ip, len(iparams), fp, len(fparams))
+        if h < 0:
+            raise RuntimeError("scene creation failed")
+        return h
+
+    def pyramid(self, rows, ground_half_width=40.0, sleep=False, workers=1):
+        return self.create(SCENE_PYRAMID, (rows, 1 if sleep else 0), (ground_half_width,), workers)
+
+    def destroy(self, h):
+        self.lib.b2s_destroy(h)
+
+    def step(self, h, count=1, dt=1.0 / 60.0, substeps=4):
+        self.lib.b2s_step(h, ctypes.c_float(dt), substeps, count)
+
+    def step_timed(self, h, count, dt=1.0 / 60.0, substeps=4):
+        return self.lib.b2s_step_timed(h, ctypes.c_float(dt), substeps, count)
+
+    def body_count(self, h):
+        return self.lib.b2s_body_count(h)
+
+    def hash(self, h):
+        return self.lib.b2s_hash(h)
+
+    def transforms(self, h):
+        n = self.body_count(h)
+        out = np.zeros((n, 4), dtype=np.float32)
+        self.lib.b2s_get_transforms(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n)
+        return out
+
+    def velocities(self, h):
+        n = self.body_count(h)
+        out = np.zeros((n, 3), dtype=np.float32)
+        self.lib.b2s_get_velocities(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n)
+        return out
+
+    def counters(self, h):
+        out = (ctypes.c_int * 30)()
+        self.lib.b2s_counters(h, out, 30)
+        vals = list(out)
+        return {"bodies": vals[0], "shapes": vals[1], "contacts": vals[2], "joints": vals[3], "islands": vals[4],
+                "awake_contacts": vals[5], "colors": vals[6:30]}
+
+    def move_count(self, h):
+        return self.lib.b2s_move_count(h)
+
+    def awake_body_count(self, h):
+        return self.lib.b2s_awake_body_count(h)
+
+    def falling_hinges(self, workers=1, step_limit=500):
+        s = ctypes.c_int()
+        hsh = ctypes.c_uint32()
+        ok = self.lib.b2s_falling_hinges(workers, step_limit, ctypes.byref(s), ctypes.byref(hsh))
+        return bool(ok), s.value, hsh.value

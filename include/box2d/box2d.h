@@ -1,0 +1,74 @@
+// box2d_amd public API: the Box2D world, body, shape and joint calls.
+//
+// Drop-in counterpart of the reference's include/box2d/box2d.h for the world step path. Each
+// declaration keeps the reference's name, signature and meaning; the file:line after each group
+// names the reference declaration it replaces. b2World_Step runs the step on the GPU.
+#pragma once
+
+#include "base.h"
+#include "collision.h"
+#include "id.h"
+#include "types.h"
+
+/// World (reference: include/box2d/box2d.h:34-228)
+B2_API b2WorldId b2CreateWorld( const b2WorldDef* def );
+B2_API void b2DestroyWorld( b2WorldId worldId );
+B2_API bool b2World_IsValid( b2WorldId id );
+B2_API void b2World_Step( b2WorldId worldId, float timeStep, int subStepCount );
+B2_API b2BodyEvents b2World_GetBodyEvents( b2WorldId worldId );
+B2_API b2ContactEvents b2World_GetContactEvents( b2WorldId worldId );
+B2_API void b2World_EnableSleeping( b2WorldId worldId, bool flag );
+B2_API bool b2World_IsSleepingEnabled( b2WorldId worldId );
+B2_API void b2World_EnableContinuous( b2WorldId worldId, bool flag );
+B2_API bool b2World_IsContinuousEnabled( b2WorldId worldId );
+B2_API void b2World_EnableWarmStarting( b2WorldId worldId, bool flag );
+B2_API bool b2World_IsWarmStartingEnabled( b2WorldId worldId );
+B2_API void b2World_SetGravity( b2WorldId worldId, b2Vec2 gravity );
+B2_API b2Vec2 b2World_GetGravity( b2WorldId worldId );
+B2_API void b2World_SetContactTuning( b2WorldId worldId, float hertz, float dampingRatio, float pushSpeed );
+B2_API void b2World_SetMaximumLinearSpeed( b2WorldId worldId, float maximumLinearSpeed );
+B2_API float b2World_GetMaximumLinearSpeed( b2WorldId worldId );
+B2_API int b2World_GetAwakeBodyCount( b2WorldId worldId );
+B2_API b2Profile b2World_GetProfile( b2WorldId worldId );
+B2_API b2Counters b2World_GetCounters( b2WorldId worldId );
+B2_API void b2World_SetUserData( b2WorldId worldId, void* userData );
+B2_API void* b2World_GetUserData( b2WorldId worldId );
+
+/// Bodies (reference: include/box2d/box2d.h:355-645)
+B2_API b2BodyId b2CreateBody( b2WorldId worldId, const b2BodyDef* def );
+B2_API bool b2Body_IsValid( b2BodyId id );
+B2_API b2BodyType b2Body_GetType( b2BodyId bodyId );
+B2_API b2Pos b2Body_GetPosition( b2BodyId bodyId );
+B2_API b2Rot b2Body_GetRotation( b2BodyId bodyId );
+B2_API b2WorldTransform b2Body_GetTransform( b2BodyId bodyId );
+B2_API b2Pos b2Body_GetWorldCenterOfMass( b2BodyId bodyId );
+B2_API b2Vec2 b2Body_GetLocalCenterOfMass( b2BodyId bodyId );
+B2_API b2Vec2 b2Body_GetLinearVelocity( b2BodyId bodyId );
+B2_API float b2Body_GetAngularVelocity( b2BodyId bodyId );
+B2_API void b2Body_SetLinearVelocity( b2BodyId bodyId, b2Vec2 linearVelocity );
+B2_API void b2Body_SetAngularVelocity( b2BodyId bodyId, float angularVelocity );
+B2_API float b2Body_GetMass( b2BodyId bodyId );
+B2_API float b2Body_GetRotationalInertia( b2BodyId bodyId );
+B2_API bool b2Body_IsAwake( b2BodyId bodyId );
+B2_API int b2Body_GetShapeCount( b2BodyId bodyId );
+B2_API void b2Body_SetUserData( b2BodyId bodyId, void* userData );
+B2_API void* b2Body_GetUserData( b2BodyId bodyId );
+B2_API b2WorldId b2Body_GetWorld( b2BodyId bodyId );
+
+/// Shapes (reference: include/box2d/box2d.h:659-908)
+B2_API b2ShapeId b2CreateCircleShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Circle* circle );
+B2_API b2ShapeId b2CreateCapsuleShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Capsule* capsule );
+B2_API b2ShapeId b2CreatePolygonShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Polygon* polygon );
+B2_API b2ShapeId b2CreateSegmentShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Segment* segment );
+B2_API bool b2Shape_IsValid( b2ShapeId id );
+B2_API b2ShapeType b2Shape_GetType( b2ShapeId shapeId );
+B2_API b2BodyId b2Shape_GetBody( b2ShapeId shapeId );
+B2_API b2AABB b2Shape_GetAABB( b2ShapeId shapeId );
+B2_API b2ChainId b2CreateChain( b2BodyId bodyId, const b2ChainDef* def );
+
+/// Joints (reference: include/box2d/box2d.h:919-1120)
+B2_API b2JointId b2CreateRevoluteJoint( b2WorldId worldId, const b2RevoluteJointDef* def );
+B2_API bool b2Joint_IsValid( b2JointId id );
+B2_API b2JointType b2Joint_GetType( b2JointId jointId );
+B2_API b2BodyId b2Joint_GetBodyA( b2JointId jointId );
+B2_API b2BodyId b2Joint_GetBodyB( b2JointId jointId );

@@ -1,0 +1,152 @@
+// box2d_amd public header: geometry and contact manifold types.
+//
+// Drop-in counterpart of the subset of the reference's include/box2d/collision.h that the world
+// step uses: shape primitives, hulls, mass properties, shape AABBs and the manifold layout that
+// b2Contact_GetData / events report. Struct layouts match the reference (collision.h:124-197,
+// 527-586).
+#pragma once
+
+#include "base.h"
+#include "math_functions.h"
+
+#include <stdbool.h>
+
+#define B2_MAX_POLYGON_VERTICES 8
+
+typedef struct b2ShapeProxy
+{
+	b2Vec2 points[B2_MAX_POLYGON_VERTICES];
+	int count;
+	float radius;
+} b2ShapeProxy;
+
+typedef struct b2MassData
+{
+	float mass;
+	b2Vec2 center;
+	float rotationalInertia;
+} b2MassData;
+
+typedef struct b2Circle
+{
+	b2Vec2 center;
+	float radius;
+} b2Circle;
+
+typedef struct b2Capsule
+{
+	b2Vec2 center1;
+	b2Vec2 center2;
+	float radius;
+} b2Capsule;
+
+typedef struct b2Polygon
+{
+	b2Vec2 vertices[B2_MAX_POLYGON_VERTICES];
+	b2Vec2 normals[B2_MAX_POLYGON_VERTICES];
+	b2Vec2 centroid;
+	float radius;
+	int count;
+} b2Polygon;
+
+typedef struct b2Segment
+{
+	b2Vec2 point1;
+	b2Vec2 point2;
+} b2Segment;
+
+typedef struct b2ChainSegment
+{
+	b2Vec2 ghost1;
+	b2Segment segment;
+	b2Vec2 ghost2;
+	int chainId;
+} b2ChainSegment;
+
+typedef struct b2Hull
+{
+	b2Vec2 points[B2_MAX_POLYGON_VERTICES];
+	int count;
+} b2Hull;
+
+/// Polygon construction (reference: src/geometry.c:57-252)
+B2_API b2Polygon b2MakePolygon( const b2Hull* hull, float radius );
+B2_API b2Polygon b2MakeOffsetPolygon( const b2Hull* hull, b2Vec2 position, b2Rot rotation );
+B2_API b2Polygon b2MakeOffsetRoundedPolygon( const b2Hull* hull, b2Vec2 position, b2Rot rotation, float radius );
+B2_API b2Polygon b2MakeSquare( float halfWidth );
+B2_API b2Polygon b2MakeBox( float halfWidth, float halfHeight );
+B2_API b2Polygon b2MakeRoundedBox( float halfWidth, float halfHeight, float radius );
+B2_API b2Polygon b2MakeOffsetBox( float halfWidth, float halfHeight, b2Vec2 center, b2Rot rotation );
+B2_API b2Polygon b2MakeOffsetRoundedBox( float halfWidth, float halfHeight, b2Vec2 center, b2Rot rotation, float radius );
+B2_API b2Polygon b2TransformPolygon( b2Transform transform, const b2Polygon* polygon );
+
+/// Mass properties (reference: src/geometry.c:254-400)
+B2_API b2MassData b2ComputeCircleMass( const b2Circle* shape, float density );
+B2_API b2MassData b2ComputeCapsuleMass( const b2Capsule* shape, float density );
+B2_API b2MassData b2ComputePolygonMass( const b2Polygon* shape, float density );
+
+/// Shape bounding boxes (reference: src/geometry.c:402-482)
+B2_API b2AABB b2ComputeCircleAABB( const b2Circle* shape, b2WorldTransform transform );
+B2_API b2AABB b2ComputeCapsuleAABB( const b2Capsule* shape, b2WorldTransform transform );
+B2_API b2AABB b2ComputePolygonAABB( const b2Polygon* shape, b2WorldTransform transform );
+B2_API b2AABB b2ComputeSegmentAABB( const b2Segment* shape, b2WorldTransform transform );
+
+/// Convex hull with quickhull and collinear point removal (reference: src/hull.c:87-328)
+B2_API b2Hull b2ComputeHull( const b2Vec2* points, int count );
+B2_API bool b2ValidateHull( const b2Hull* hull );
+
+typedef struct b2SegmentDistanceResult
+{
+	b2Vec2 closest1;
+	b2Vec2 closest2;
+	float fraction1;
+	float fraction2;
+	float distanceSquared;
+} b2SegmentDistanceResult;
+
+typedef struct b2SimplexCache
+{
+	uint16_t count;
+	uint8_t indexA[3];
+	uint8_t indexB[3];
+} b2SimplexCache;
+
+static const b2SimplexCache b2_emptySimplexCache = B2_ZERO_INIT;
+
+/// A contact point in world space relative to each body's center of mass
+/// (reference: include/box2d/collision.h:527-560)
+typedef struct b2ManifoldPoint
+{
+	b2Vec2 anchorA;
+	b2Vec2 anchorB;
+	float separation;
+	float baseSeparation;
+	float normalImpulse;
+	float tangentImpulse;
+	float totalNormalImpulse;
+	float normalVelocity;
+	uint16_t id;
+	bool persisted;
+} b2ManifoldPoint;
+
+typedef struct b2Manifold
+{
+	b2Vec2 normal;
+	float rollingImpulse;
+	b2ManifoldPoint points[2];
+	int pointCount;
+} b2Manifold;
+
+typedef struct b2LocalManifoldPoint
+{
+	b2Vec2 point;
+	float separation;
+	uint16_t id;
+} b2LocalManifoldPoint;
+
+typedef struct b2LocalManifold
+{
+	b2Vec2 normal;
+	b2LocalManifoldPoint points[2];
+	int pointCount;
+} b2LocalManifold;

@@ -1,0 +1,324 @@
+// box2d_amd public header: definition structs, events, profile and counters.
+//
+// Drop-in counterpart of the reference's include/box2d/types.h for the calls the world step path
+// uses. Every struct keeps the reference layout (checked with static_assert in the library) so
+// definitions filled by existing C code are read field-for-field.
+#pragma once
+
+#include "base.h"
+#include "collision.h"
+#include "id.h"
+#include "math_functions.h"
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#define B2_DEFAULT_CATEGORY_BITS 1
+#define B2_DEFAULT_MASK_BITS UINT64_MAX
+
+typedef void b2TaskCallback( void* taskContext );
+typedef void* b2EnqueueTaskCallback( b2TaskCallback* task, void* taskContext, void* userContext );
+typedef void b2FinishTaskCallback( void* userTask, void* userContext );
+typedef float b2FrictionCallback( float frictionA, uint64_t userMaterialIdA, float frictionB, uint64_t userMaterialIdB );
+typedef float b2RestitutionCallback( float restitutionA, uint64_t userMaterialIdA, float restitutionB, uint64_t userMaterialIdB );
+
+typedef struct b2Capacity
+{
+	int staticShapeCount;
+	int dynamicShapeCount;
+	int staticBodyCount;
+	int dynamicBodyCount;
+	int contactCount;
+} b2Capacity;
+
+/// World definition (reference: include/box2d/types.h:92-160)
+typedef struct b2WorldDef
+{
+	b2Vec2 gravity;
+	float restitutionThreshold;
+	float hitEventThreshold;
+	float contactHertz;
+	float contactDampingRatio;
+	float contactSpeed;
+	float maximumLinearSpeed;
+	b2FrictionCallback* frictionCallback;
+	b2RestitutionCallback* restitutionCallback;
+	bool enableSleep;
+	bool enableContinuous;
+	bool enableContactSoftening;
+	int workerCount;
+	b2EnqueueTaskCallback* enqueueTask;
+	b2FinishTaskCallback* finishTask;
+	void* userTaskContext;
+	void* userData;
+	b2Capacity capacity;
+	int internalValue;
+} b2WorldDef;
+
+B2_API b2WorldDef b2DefaultWorldDef( void );
+
+typedef enum b2BodyType
+{
+	b2_staticBody = 0,
+	b2_kinematicBody = 1,
+	b2_dynamicBody = 2,
+	b2_bodyTypeCount,
+} b2BodyType;
+
+typedef struct b2MotionLocks
+{
+	bool linearX;
+	bool linearY;
+	bool angularZ;
+} b2MotionLocks;
+
+/// Body definition (reference: include/box2d/types.h:186-262)
+typedef struct b2BodyDef
+{
+	b2BodyType type;
+	b2Pos position;
+	b2Rot rotation;
+	b2Vec2 linearVelocity;
+	float angularVelocity;
+	float linearDamping;
+	float angularDamping;
+	float gravityScale;
+	float sleepThreshold;
+	float safetyFactor;
+	const char* name;
+	void* userData;
+	b2MotionLocks motionLocks;
+	bool enableSleep;
+	bool isAwake;
+	bool isBullet;
+	bool isEnabled;
+	bool allowFastRotation;
+	bool enableContactRecycling;
+	int internalValue;
+} b2BodyDef;
+
+B2_API b2BodyDef b2DefaultBodyDef( void );
+
+typedef struct b2Filter
+{
+	uint64_t categoryBits;
+	uint64_t maskBits;
+	int groupIndex;
+} b2Filter;
+
+B2_API b2Filter b2DefaultFilter( void );
+
+typedef struct b2QueryFilter
+{
+	uint64_t categoryBits;
+	uint64_t maskBits;
+} b2QueryFilter;
+
+B2_API b2QueryFilter b2DefaultQueryFilter( void );
+
+typedef enum b2ShapeType
+{
+	b2_circleShape,
+	b2_capsuleShape,
+	b2_segmentShape,
+	b2_polygonShape,
+	b2_chainSegmentShape,
+	b2_shapeTypeCount
+} b2ShapeType;
+
+typedef struct b2SurfaceMaterial
+{
+	float friction;
+	float restitution;
+	float rollingResistance;
+	float tangentSpeed;
+	uint64_t userMaterialId;
+	uint32_t customColor;
+} b2SurfaceMaterial;
+
+B2_API b2SurfaceMaterial b2DefaultSurfaceMaterial( void );
+
+/// Shape definition (reference: include/box2d/types.h:340-401)
+typedef struct b2ShapeDef
+{
+	void* userData;
+	b2SurfaceMaterial material;
+	float density;
+	b2Filter filter;
+	bool enableCustomFiltering;
+	bool isSensor;
+	bool enableSensorEvents;
+	bool enableContactEvents;
+	bool enableHitEvents;
+	bool enablePreSolveEvents;
+	bool invokeContactCreation;
+	bool updateBodyMass;
+	int internalValue;
+} b2ShapeDef;
+
+B2_API b2ShapeDef b2DefaultShapeDef( void );
+
+typedef struct b2ChainDef
+{
+	void* userData;
+	const b2Vec2* points;
+	int count;
+	const b2SurfaceMaterial* materials;
+	int materialCount;
+	b2Filter filter;
+	bool isLoop;
+	bool enableSensorEvents;
+	int internalValue;
+} b2ChainDef;
+
+B2_API b2ChainDef b2DefaultChainDef( void );
+
+/// Per-step stage timings in milliseconds (reference: include/box2d/types.h:526-551)
+typedef struct b2Profile
+{
+	float step;
+	float pairs;
+	float collide;
+	float solve;
+	float solverSetup;
+	float constraints;
+	float prepareConstraints;
+	float integrateVelocities;
+	float warmStart;
+	float solveImpulses;
+	float integratePositions;
+	float relaxImpulses;
+	float applyRestitution;
+	float storeImpulses;
+	float splitIslands;
+	float transforms;
+	float sensorHits;
+	float jointEvents;
+	float hitEvents;
+	float refit;
+	float bullets;
+	float sleepIslands;
+	float sensors;
+} b2Profile;
+
+/// World counters (reference: include/box2d/types.h:554-575)
+typedef struct b2Counters
+{
+	int64_t byteCount;
+	int bodyCount;
+	int shapeCount;
+	int contactCount;
+	int jointCount;
+	int islandCount;
+	int stackUsed;
+	int staticTreeHeight;
+	int treeHeight;
+	int taskCount;
+	int colorCounts[24];
+	int awakeContactCount;
+	int recycledContactCount;
+} b2Counters;
+
+typedef enum b2JointType
+{
+	b2_distanceJoint,
+	b2_filterJoint,
+	b2_motorJoint,
+	b2_moverJoint,
+	b2_pogoJoint,
+	b2_prismaticJoint,
+	b2_revoluteJoint,
+	b2_weldJoint,
+	b2_wheelJoint,
+} b2JointType;
+
+/// Base joint definition (reference: include/box2d/types.h:605-650)
+typedef struct b2JointDef
+{
+	void* userData;
+	b2BodyId bodyIdA;
+	b2BodyId bodyIdB;
+	b2Transform localFrameA;
+	b2Transform localFrameB;
+	float forceThreshold;
+	float torqueThreshold;
+	float constraintHertz;
+	float constraintDampingRatio;
+	float drawScale;
+	bool collideConnected;
+} b2JointDef;
+
+/// Revolute joint definition (reference: include/box2d/types.h:845-898)
+typedef struct b2RevoluteJointDef
+{
+	b2JointDef base;
+	float targetAngle;
+	bool enableSpring;
+	float hertz;
+	float dampingRatio;
+	bool enableLimit;
+	float lowerAngle;
+	float upperAngle;
+	bool enableMotor;
+	float maxMotorTorque;
+	float motorSpeed;
+	int internalValue;
+} b2RevoluteJointDef;
+
+B2_API b2RevoluteJointDef b2DefaultRevoluteJointDef( void );
+
+typedef struct b2ContactBeginTouchEvent
+{
+	b2ShapeId shapeIdA;
+	b2ShapeId shapeIdB;
+	b2ContactId contactId;
+} b2ContactBeginTouchEvent;
+
+typedef struct b2ContactEndTouchEvent
+{
+	b2ShapeId shapeIdA;
+	b2ShapeId shapeIdB;
+	b2ContactId contactId;
+} b2ContactEndTouchEvent;
+
+typedef struct b2ContactHitEvent
+{
+	b2ShapeId shapeIdA;
+	b2ShapeId shapeIdB;
+	b2ContactId contactId;
+	b2Pos point;
+	b2Vec2 normal;
+	float approachSpeed;
+} b2ContactHitEvent;
+
+typedef struct b2ContactEvents
+{
+	b2ContactBeginTouchEvent* beginEvents;
+	b2ContactEndTouchEvent* endEvents;
+	b2ContactHitEvent* hitEvents;
+	int beginCount;
+	int endCount;
+	int hitCount;
+} b2ContactEvents;
+
+typedef struct b2BodyMoveEvent
+{
+	void* userData;
+	b2WorldTransform transform;
+	b2BodyId bodyId;
+	bool fellAsleep;
+} b2BodyMoveEvent;
+
+typedef struct b2BodyEvents
+{
+	b2BodyMoveEvent* moveEvents;
+	int moveCount;
+} b2BodyEvents;
+
+typedef struct b2ContactData
+{
+	b2ContactId contactId;
+	b2ShapeId shapeIdA;
+	b2ShapeId shapeIdB;
+	b2Manifold manifold;
+} b2ContactData;
