@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the Box2D world step on one AMD Instinct MI355X.
+
+Workload: the reference's large_pyramid benchmark scene (shared/benchmarks.c:88-131): a
+100-row triangular pyramid of 5050 dynamic 0.5 m boxes (density 1, friction 0.6) on a static
+100 x 1 m (half-extent) ground box, gravity (0, -10), 1/60 s steps with 4 substeps, sleep off.
+Bodies are created through the public Box2D C API by the scene driver and stepped with
+b2World_Step; all body, shape and contact state stays resident in GPU memory between steps.
+
+Protocol (benchmark/main.c:307-377 style): W untimed warm-up steps, then K timed steps bracketed
+by device synchronization; one JSON line on stdout. value = body-substeps per second
+(bodies x substeps / seconds per step). After the timed region a second pass of the same scene
+records HIP events around every kernel launch on the world's stream to price the dominant kernel
+against HBM bandwidth (algorithmic bytes per launch / average launch duration). The CPU baseline
+is the reference library compiled from its own sources (oracle/_ref) stepping the same scene on
+the host's cores, timed on a bounded sample of steps.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+SUBSTEPS = 4
+DT = 1.0 / 60.0
+# Published reference figure for this scene and metric: large_pyramid, 8 threads, SSE2, AMD Ryzen 9
+# 7950X: 242.273 ms over 499 steps (benchmark/amd7950x_sse2/large_pyramid.csv:9) -> 41.6 M body-substeps/s.
+PUBLISHED_BODY_SUBSTEPS = 5050 * SUBSTEPS / (242.273e-3 / 499)
+METRIC = "body-substeps/s & ms/step: batched+adjoint vs plain GPU, hash-matched; CPU ref"
+
+
+def golden_hash(rows, ground, steps):
+    path = os.path.join(ROOT, "tests", "golden", "world_runs.json")
+    try:
+        with open(path) as f:
+            runs = json.load(f)
+    except OSError:
+        return None
+    for s in runs["scenes"]:
+        if s["kind"] == 0 and s["iparams"][:2] == [rows, 0] and abs(s["fparams"][0] - ground) < 1e-6:
+            return s["hashes"].get(str(steps))
+    return None
+
+
+def kernel_stats(lib, world_id):
+    from b2types import KernelStats
+
+    arr = (KernelStats * 32)()
+    lib.b2amd_World_GetKernelStats.restype = ctypes.c_int
+    n = lib.b2amd_World_GetKernelStats(world_id, arr, 32)
+    out = []
+    for i in range(n):
+        k = arr[i]
+        if k.launches > 0:
+            out.append({"name": k.name.decode(), "launches": k.launches, "total_ms": k.totalMs,
+                        "bytes": k.algorithmicBytes})
+    return out
+
+
+def cpu_baseline(rows, ground, seconds, threads):
+    """Reference library (compiled from the reference sources) on this host's cores."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
+    if not os.path.exists(path):
+        return None
+    from box2d_amd.scenes_api import SceneLib
+
+    ref = SceneLib(path)
+    h = ref.pyramid(rows, ground, sleep=False, workers=threads)
+    bodies = ref.body_count(h)
+    ref.step(h)  # untimed warm-up step, as benchmark/main.c does
+    total_ms = 0.0
+    steps = 0
+    while steps < 499 and total_ms < seconds * 1000.0:
+        total_ms += ref.step_timed(h, 10)
+        steps += 10
+    ref.destroy(h)
+    ms = total_ms / steps
+    return {"value": bodies * SUBSTEPS / (ms * 1e-3), "unit": "body-substeps/s", "cores": threads,
+            "kind": "reference", "ms_per_step": ms,
+            "sample": f"steps 2..{steps + 1} of the same {rows}-row pyramid, workerCount={threads} "
+                      f"(reference built-in scheduler), monotonic clock"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=499)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--rows", type=int, default=100)
+    ap.add_argument("--ground", type=float, default=100.0, help="ground box half width (m)")
+    ap.add_argument("--profile-steps", type=int, default=60, help="steps of the per-kernel timing pass")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bound on the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported. A single Box2D world "
+              "does not shard across GPUs; independent worlds would be split by world range.", file=sys.stderr)
+        sys.exit(2)
+
+    import box2d_amd
+    from b2types import WorldId
+
+    lib = box2d_amd.library()
+    scenes = box2d_amd.scenes()
+    scenes.lib.b2s_world_id.restype = ctypes.c_uint32
+
+    h = scenes.pyramid(args.rows, args.ground, sleep=False)
+    bodies = scenes.body_count(h)
+    packed = scenes.lib.b2s_world_id(h)
+    wid = WorldId(packed & 0xFFFF, packed >> 16)
+
+    scenes.step(h, args.warmup, DT, SUBSTEPS)
+    lib.b2amd_Synchronize()
+    t0 = time.perf_counter()
+    scenes.step(h, args.steps, DT, SUBSTEPS)
+    lib.b2amd_Synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = bodies * SUBSTEPS * args.steps / elapsed
+
+    total_steps = args.warmup + args.steps
+    want = golden_hash(args.rows, args.ground, total_steps)
+    got = scenes.hash(h)
+    hash_match = None if want is None else bool(got == want)
+    counters = scenes.counters(h)
+
+    # per-kernel timing pass on a fresh copy of the scene (same steps as the timed region)
+    h2 = scenes.pyramid(args.rows, args.ground, sleep=False)
+    p2 = scenes.lib.b2s_world_id(h2)
+    wid2 = WorldId(p2 & 0xFFFF, p2 >> 16)
+    scenes.step(h2, args.warmup, DT, SUBSTEPS)
+    lib.b2amd_World_EnableKernelTiming(wid2, ctypes.c_bool(True))
+    lib.b2amd_World_ResetKernelStats(wid2)
+    scenes.step(h2, min(args.profile_steps, args.steps), DT, SUBSTEPS)
+    stats = kernel_stats(lib, wid2)
+    lib.b2amd_World_EnableKernelTiming(wid2, ctypes.c_bool(False))
+    scenes.destroy(h2)
+
+    dominant = max(stats, key=lambda k: k["total_ms"]) if stats else None
+    roofline = None
+    if dominant is not None:
+        avg_ms = dominant["total_ms"] / dominant["launches"]
+        bytes_per_launch = dominant["bytes"] / dominant["launches"]
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant["name"],
+                    "avg_launch_us": avg_ms * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch}
+    kernels = [{"name": k["name"], "launches_per_step": k["launches"] / min(args.profile_steps, args.steps),
+                "avg_us": 1e3 * k["total_ms"] / k["launches"],
+                "gbs": (k["bytes"] / (k["total_ms"] * 1e-3) / 1e9) if k["total_ms"] > 0 else 0.0} for k in stats]
+    gpu_busy_ms = sum(k["total_ms"] for k in stats) / max(1, min(args.profile_steps, args.steps))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.rows, args.ground, args.cpu_seconds, threads)
+
+    scenes.destroy(h)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "body-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_BODY_SUBSTEPS if (args.rows == 100) else None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"large_pyramid: {args.rows}-row box pyramid ({bodies} bodies) on a "
+                               f"{args.ground:g}x1 m ground, plain single-world GPU step",
+                   "bodies": bodies, "substeps": SUBSTEPS, "dt": DT, "sleep": False,
+                   "contacts": counters["contacts"], "colors": [c for c in counters["colors"] if c],
+                   "baseline": "reference large_pyramid 8T SSE2 on Ryzen 9 7950X, 41.6M body-substeps/s"},
+        "hash_match": hash_match,
+        "hash": f"{got:#010x}",
+        "roofline": roofline,
+        "gpu_busy_ms_per_step": gpu_busy_ms,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

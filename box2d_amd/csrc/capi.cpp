@@ -2,6 +2,7 @@
 // its GPU step. Each function keeps the reference's meaning; the reference implementation is at
 // the file:line given in the header next to the declaration.
 
+#include "box2d_amd.h"
 #include "device.h"
 #include "world.h"
 
@@ -13,6 +14,9 @@ World* CreateWorldInternal( const b2WorldDef* def, b2WorldId* outId );
 void DestroyWorldInternal( World* world );
 void SetLogFcn( b2LogFcn* fcn );
 void SetAssertFcn( b2AssertFcn* fcn );
+int CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
+				  b2LocalManifold* manifolds );
+void DeviceSynchronize();
 } // namespace b2g
 
 using namespace b2g;
@@ -596,6 +600,47 @@ b2BodyId b2Joint_GetBodyB( b2JointId jointId )
 	}
 	Body& body = world->bodies[joint->edges[1].bodyId];
 	return b2BodyId{ body.id + 1, jointId.world0, body.generation };
+}
+
+// Extensions (include/box2d_amd.h) -----------------------------------------------------------
+
+void b2amd_World_EnableKernelTiming( b2WorldId worldId, bool flag )
+{
+	World* world = GetWorld( worldId );
+	if ( world != nullptr )
+	{
+		world->device->EnableKernelTiming( flag );
+	}
+}
+
+int b2amd_World_GetKernelStats( b2WorldId worldId, b2amd_KernelStats* stats, int capacity )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr || stats == nullptr )
+	{
+		return 0;
+	}
+	return world->device->GetKernelStats( stats, capacity );
+}
+
+void b2amd_World_ResetKernelStats( b2WorldId worldId )
+{
+	World* world = GetWorld( worldId );
+	if ( world != nullptr )
+	{
+		world->device->ResetKernelStats();
+	}
+}
+
+void b2amd_Synchronize( void )
+{
+	DeviceSynchronize();
+}
+
+int b2amd_CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
+						b2LocalManifold* manifolds )
+{
+	return CollideBatch( shapesA, shapesB, transforms, count, manifolds );
 }
 
 } // extern "C"

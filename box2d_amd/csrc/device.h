@@ -74,6 +74,11 @@ public:
 	float lastCollideMs = 0.0f;
 	float lastSolveMs = 0.0f;
 
+	// Optional per-kernel timing (HIP events around every launch).
+	void EnableKernelTiming( bool flag );
+	int GetKernelStats( void* stats, int capacity ); // b2amd_KernelStats array
+	void ResetKernelStats();
+
 	void* stream = nullptr;
 
 private:

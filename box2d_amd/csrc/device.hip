@@ -13,6 +13,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "box2d_amd.h"
 #include "device.h"
 #include "narrowphase.h"
 #include "solver_device.h"
@@ -623,21 +624,23 @@ B2G_DEV bool GroupAnyNonZero( const ConstraintSoA& c, int f, int colorStart, int
 }
 
 // One graph color of one stage: joints occupy the first jointCount lanes, contacts the rest.
+// One instantiation per stage so each stage is its own kernel (and its own profiler row).
+template <int Stage>
 __global__ void ColorKernel( ColorArgs a )
 {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( t < a.jointCount )
 	{
 		JointSim* joint = a.joints + a.jointList[a.jointStart + t];
-		if ( a.stage == kStageWarmStart )
+		if ( Stage == kStageWarmStart )
 		{
 			WarmStartRevolute( joint, a.states );
 		}
-		else if ( a.stage == kStageSolve )
+		else if ( Stage == kStageSolve )
 		{
 			SolveRevolute( joint, a.states, true, a.p.h, a.p.inv_h );
 		}
-		else if ( a.stage == kStageRelax )
+		else if ( Stage == kStageRelax )
 		{
 			SolveRevolute( joint, a.states, false, a.p.h, a.p.inv_h );
 		}
@@ -651,28 +654,25 @@ __global__ void ColorKernel( ColorArgs a )
 	}
 	int i = a.contactStart + k;
 
-	switch ( a.stage )
+	if ( Stage == kStageWarmStart )
 	{
-		case kStageWarmStart:
-			WarmStartContact( a.c, i, a.states );
-			break;
-		case kStageSolve:
-			SolveContact( a.c, i, a.states, true, a.p.inv_h, -a.p.contactSpeed, false );
-			break;
-		case kStageRelax:
+		WarmStartContact( a.c, i, a.states );
+	}
+	else if ( Stage == kStageSolve )
+	{
+		SolveContact( a.c, i, a.states, true, a.p.inv_h, -a.p.contactSpeed, false );
+	}
+	else if ( Stage == kStageRelax )
+	{
+		bool rolling = GroupAnyNonZero( a.c, kcRollingResistance, a.contactStart, a.contactCount, k );
+		SolveContact( a.c, i, a.states, false, a.p.inv_h, -a.p.contactSpeed, rolling );
+	}
+	else
+	{
+		if ( GroupAnyNonZero( a.c, kcRestitution, a.contactStart, a.contactCount, k ) )
 		{
-			bool rolling = GroupAnyNonZero( a.c, kcRollingResistance, a.contactStart, a.contactCount, k );
-			SolveContact( a.c, i, a.states, false, a.p.inv_h, -a.p.contactSpeed, rolling );
+			RestitutionContact( a.c, i, a.states, a.p.restitutionThreshold );
 		}
-		break;
-		case kStageRestitution:
-			if ( GroupAnyNonZero( a.c, kcRestitution, a.contactStart, a.contactCount, k ) )
-			{
-				RestitutionContact( a.c, i, a.states, a.p.restitutionThreshold );
-			}
-			break;
-		default:
-			break;
 	}
 }
 
@@ -949,6 +949,46 @@ __global__ void FinalizeBodiesKernel( FinalizeArgs a )
 }
 
 // ---------------------------------------------------------------------------------------------
+// Kernel classes for timing, with the algorithmic bytes each unit of work must move: every field
+// a lane reads or writes counted once, body states gathered and scattered per constraint
+// (per-unit figures of the step's data layout; see the roofline notes in the design document).
+
+enum KernelClass
+{
+	kkCollide,
+	kkPrepareContacts,
+	kkPrepareJoints,
+	kkIntegrateVelocities,
+	kkWarmStart,
+	kkSolve,
+	kkIntegratePositions,
+	kkRelax,
+	kkRestitution,
+	kkOverflow,
+	kkStore,
+	kkFinalize,
+	kKernelClassCount
+};
+
+static const char* s_kernelClassNames[kKernelClassCount] = {
+	"collide", "prepare_contacts", "prepare_joints", "integrate_velocities", "warm_start", "solve",
+	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies",
+};
+
+constexpr double kBytesCollideFull = 800.0;		// per contact with a new manifold
+constexpr double kBytesCollideRecycled = 180.0; // per recycled contact
+constexpr double kBytesPrepare = 292.0;			// per contact constraint
+constexpr double kBytesIntegrateVelocities = 56.0;
+constexpr double kBytesIntegratePositions = 60.0;
+constexpr double kBytesWarmStart = 148.0;
+constexpr double kBytesSolve = 212.0;
+constexpr double kBytesRelax = 236.0;
+constexpr double kBytesRestitution = 236.0;
+constexpr double kBytesStore = 76.0;
+constexpr double kBytesFinalize = 370.0;
+constexpr double kBytesJoint = 256.0; // revolute joint record read and written plus two body states
+
+// ---------------------------------------------------------------------------------------------
 // Device buffers
 
 template <typename T>
@@ -1035,6 +1075,69 @@ struct DeviceWorld::Impl
 	int* hostCounters = nullptr;
 
 	hipEvent_t ev[4] = {};
+
+	// Per-kernel timing
+	bool timing = false;
+	struct Pending
+	{
+		int cls;
+		double bytes;
+		int ev0;
+		int ev1;
+	};
+	std::vector<Pending> pending;
+	std::vector<hipEvent_t> eventPool;
+	int eventUsed = 0;
+	double statMs[kKernelClassCount] = {};
+	double statBytes[kKernelClassCount] = {};
+	int statLaunches[kKernelClassCount] = {};
+
+	int NextEvent()
+	{
+		if ( eventUsed == (int)eventPool.size() )
+		{
+			hipEvent_t e;
+			HIP_CHECK( hipEventCreate( &e ) );
+			eventPool.push_back( e );
+		}
+		return eventUsed++;
+	}
+
+	// Launch through this wrapper so the optional timing brackets every kernel.
+	template <typename F>
+	void Launch( int cls, double bytes, F&& launch )
+	{
+		if ( timing )
+		{
+			int e0 = NextEvent();
+			int e1 = NextEvent();
+			HIP_CHECK( hipEventRecord( eventPool[e0], stream ) );
+			launch();
+			HIP_CHECK( hipGetLastError() );
+			HIP_CHECK( hipEventRecord( eventPool[e1], stream ) );
+			pending.push_back( Pending{ cls, bytes, e0, e1 } );
+		}
+		else
+		{
+			launch();
+			HIP_CHECK( hipGetLastError() );
+		}
+	}
+
+	// Call after the stream has been synchronized.
+	void FlushTiming()
+	{
+		for ( const Pending& p : pending )
+		{
+			float ms = 0.0f;
+			HIP_CHECK( hipEventElapsedTime( &ms, eventPool[p.ev0], eventPool[p.ev1] ) );
+			statMs[p.cls] += ms;
+			statBytes[p.cls] += p.bytes;
+			statLaunches[p.cls] += 1;
+		}
+		pending.clear();
+		eventUsed = 0;
+	}
 
 	void EnsureConstraints( size_t n )
 	{
@@ -1272,8 +1375,8 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 	a.changes = impl->changes.ptr;
 	a.counters = impl->counters;
 	a.recycleDistance = params.contactRecycleDistance;
-	CollideKernel<<<Blocks( n ), kBlock, 0, s>>>( a );
-	HIP_CHECK( hipGetLastError() );
+	// Bytes are fixed up after the launch once the recycled count is known.
+	impl->Launch( kkCollide, 0.0, [&]() { CollideKernel<<<Blocks( n ), kBlock, 0, s>>>( a ); } );
 	HIP_CHECK( hipEventRecord( impl->ev[1], s ) );
 
 	HIP_CHECK( hipMemcpyAsync( impl->hostCounters, impl->counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
@@ -1281,6 +1384,13 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 
 	int changeCount = impl->hostCounters[0];
 	result->recycledCount = impl->hostCounters[1];
+	if ( impl->timing && impl->pending.empty() == false )
+	{
+		int recycled = result->recycledCount;
+		impl->pending.back().bytes = kBytesCollideFull * ( n - recycled ) + kBytesCollideRecycled * recycled;
+	}
+	impl->FlushTiming();
+
 	if ( changeCount > 0 )
 	{
 		result->changes.resize( changeCount );
@@ -1312,8 +1422,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		pa.states = impl->states.ptr;
 		pa.aux = impl->aux.ptr;
 		pa.p = params;
-		PrepareContactsKernel<<<Blocks( nc ), kBlock, 0, s>>>( pa );
-		HIP_CHECK( hipGetLastError() );
+		impl->Launch( kkPrepareContacts, kBytesPrepare * nc, [&]() { PrepareContactsKernel<<<Blocks( nc ), kBlock, 0, s>>>( pa ); } );
 	}
 
 	if ( nj > 0 )
@@ -1324,8 +1433,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		ja.count = nj;
 		ja.sims = impl->sims.ptr;
 		ja.p = params;
-		PrepareJointsKernel<<<Blocks( nj ), kBlock, 0, s>>>( ja );
-		HIP_CHECK( hipGetLastError() );
+		impl->Launch( kkPrepareJoints, kBytesJoint * nj, [&]() { PrepareJointsKernel<<<Blocks( nj ), kBlock, 0, s>>>( ja ); } );
 	}
 
 	BodyArgs ba;
@@ -1343,6 +1451,33 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 	ca.states = impl->states.ptr;
 	ca.p = params;
 
+	auto stageClass = []( int stage ) {
+		switch ( stage )
+		{
+			case kStageWarmStart:
+				return (int)kkWarmStart;
+			case kStageSolve:
+				return (int)kkSolve;
+			case kStageRelax:
+				return (int)kkRelax;
+			default:
+				return (int)kkRestitution;
+		}
+	};
+	auto stageBytes = []( int stage ) {
+		switch ( stage )
+		{
+			case kStageWarmStart:
+				return kBytesWarmStart;
+			case kStageSolve:
+				return kBytesSolve;
+			case kStageRelax:
+				return kBytesRelax;
+			default:
+				return kBytesRestitution;
+		}
+	};
+
 	auto runStage = [&]( int stage ) {
 		if ( L.overflowContactCount > 0 || L.overflowJointCount > 0 )
 		{
@@ -1351,8 +1486,8 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 			ca.jointCount = L.overflowJointCount;
 			ca.contactStart = L.overflowContactStart;
 			ca.contactCount = L.overflowContactCount;
-			OverflowKernel<<<1, 64, 0, s>>>( ca );
-			HIP_CHECK( hipGetLastError() );
+			double bytes = stageBytes( stage ) * ca.contactCount + kBytesJoint * ca.jointCount;
+			impl->Launch( kkOverflow, bytes, [&]() { OverflowKernel<<<1, 64, 0, s>>>( ca ); } );
 		}
 
 		for ( int c = 0; c < L.activeColorCount; ++c )
@@ -1371,19 +1506,35 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 			{
 				continue;
 			}
-			ColorKernel<<<Blocks( lanes ), kBlock, 0, s>>>( ca );
-			HIP_CHECK( hipGetLastError() );
+			double bytes = stageBytes( stage ) * ca.contactCount + kBytesJoint * ca.jointCount;
+			impl->Launch( stageClass( stage ), bytes, [&]() {
+				switch ( stage )
+				{
+					case kStageWarmStart:
+						ColorKernel<kStageWarmStart><<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+						break;
+					case kStageSolve:
+						ColorKernel<kStageSolve><<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+						break;
+					case kStageRelax:
+						ColorKernel<kStageRelax><<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+						break;
+					default:
+						ColorKernel<kStageRestitution><<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+						break;
+				}
+			} );
 		}
 	};
 
 	for ( int sub = 0; sub < params.subStepCount; ++sub )
 	{
-		IntegrateVelocitiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba );
-		HIP_CHECK( hipGetLastError() );
+		impl->Launch( kkIntegrateVelocities, kBytesIntegrateVelocities * nb,
+					  [&]() { IntegrateVelocitiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba ); } );
 		runStage( kStageWarmStart );
 		runStage( kStageSolve );
-		IntegratePositionsKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba );
-		HIP_CHECK( hipGetLastError() );
+		impl->Launch( kkIntegratePositions, kBytesIntegratePositions * nb,
+					  [&]() { IntegratePositionsKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba ); } );
 		runStage( kStageRelax );
 	}
 
@@ -1399,8 +1550,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		sa.contacts = impl->contacts.ptr;
 		sa.coloredCount = L.overflowContactStart;
 		sa.totalCount = nc;
-		StoreImpulsesKernel<<<Blocks( nc ), kBlock, 0, s>>>( sa );
-		HIP_CHECK( hipGetLastError() );
+		impl->Launch( kkStore, kBytesStore * nc, [&]() { StoreImpulsesKernel<<<Blocks( nc ), kBlock, 0, s>>>( sa ); } );
 	}
 
 	HIP_CHECK( hipMemsetAsync( impl->counters + 4, 0, 3 * sizeof( int ), s ) );
@@ -1418,12 +1568,12 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 	fa.enlargedShapes = impl->enlargedShapes.ptr;
 	fa.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 	fa.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
-	FinalizeBodiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( fa );
-	HIP_CHECK( hipGetLastError() );
+	impl->Launch( kkFinalize, kBytesFinalize * nb, [&]() { FinalizeBodiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( fa ); } );
 	HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
 
 	HIP_CHECK( hipMemcpyAsync( impl->hostCounters + 4, impl->counters + 4, 3 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
 	HIP_CHECK( hipStreamSynchronize( s ) );
+	impl->FlushTiming();
 
 	int enlargedBodyCount = impl->hostCounters[4];
 	int enlargedShapeCount = impl->hostCounters[5];
@@ -1455,4 +1605,179 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 	HIP_CHECK( hipEventElapsedTime( &lastSolveMs, impl->ev[2], impl->ev[3] ) );
 }
 
+void DeviceWorld::EnableKernelTiming( bool flag )
+{
+	impl->timing = flag;
+}
+
+int DeviceWorld::GetKernelStats( void* out, int capacity )
+{
+	b2amd_KernelStats* stats = static_cast<b2amd_KernelStats*>( out );
+	int n = 0;
+	for ( int k = 0; k < kKernelClassCount && n < capacity; ++k )
+	{
+		if ( impl->statLaunches[k] == 0 )
+		{
+			continue;
+		}
+		b2amd_KernelStats& st = stats[n++];
+		memset( &st, 0, sizeof( st ) );
+		strncpy( st.name, s_kernelClassNames[k], sizeof( st.name ) - 1 );
+		st.launches = impl->statLaunches[k];
+		st.totalMs = impl->statMs[k];
+		st.algorithmicBytes = impl->statBytes[k];
+	}
+	return n;
+}
+
+void DeviceWorld::ResetKernelStats()
+{
+	for ( int k = 0; k < kKernelClassCount; ++k )
+	{
+		impl->statMs[k] = 0.0;
+		impl->statBytes[k] = 0.0;
+		impl->statLaunches[k] = 0;
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched narrowphase for callers outside the world step.
+
+__global__ void CollideBatchKernel( const ShapeDev* shapesA, const ShapeDev* shapesB, const b2Transform* xfs, int count,
+									b2LocalManifold* out )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= count )
+	{
+		return;
+	}
+	out[i] = ComputeManifold( shapesA[i], shapesB[i], xfs[i] );
+}
+
+void DeviceSynchronize()
+{
+	HIP_CHECK( hipDeviceSynchronize() );
+}
+
+int CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
+				  b2LocalManifold* manifolds )
+{
+	if ( count <= 0 )
+	{
+		return 0;
+	}
+
+	std::vector<ShapeDev> a( count ), b( count );
+	for ( int i = 0; i < count; ++i )
+	{
+		const b2amd_Shape* src[2] = { shapesA + i, shapesB + i };
+		ShapeDev* dst[2] = { &a[i], &b[i] };
+		for ( int k = 0; k < 2; ++k )
+		{
+			memset( dst[k], 0, sizeof( ShapeDev ) );
+			dst[k]->type = src[k]->type;
+			switch ( src[k]->type )
+			{
+				case b2_circleShape:
+					dst[k]->circle = src[k]->circle;
+					break;
+				case b2_capsuleShape:
+					dst[k]->capsule = src[k]->capsule;
+					break;
+				case b2_segmentShape:
+					dst[k]->segment = src[k]->segment;
+					break;
+				case b2_polygonShape:
+					dst[k]->polygon = src[k]->polygon;
+					break;
+				default:
+					return -1;
+			}
+		}
+		int tA = a[i].type, tB = b[i].type;
+		bool primary = ( tA == tB ) || ( tA == b2_capsuleShape && tB == b2_circleShape ) ||
+					   ( tA == b2_polygonShape && ( tB == b2_circleShape || tB == b2_capsuleShape ) ) ||
+					   ( tA == b2_segmentShape && tB != b2_segmentShape );
+		if ( primary == false || ( tA == b2_segmentShape && tB == b2_segmentShape ) )
+		{
+			return -1;
+		}
+	}
+
+	ShapeDev* dA = nullptr;
+	ShapeDev* dB = nullptr;
+	b2Transform* dX = nullptr;
+	b2LocalManifold* dM = nullptr;
+	HIP_CHECK( hipMalloc( &dA, count * sizeof( ShapeDev ) ) );
+	HIP_CHECK( hipMalloc( &dB, count * sizeof( ShapeDev ) ) );
+	HIP_CHECK( hipMalloc( &dX, count * sizeof( b2Transform ) ) );
+	HIP_CHECK( hipMalloc( &dM, count * sizeof( b2LocalManifold ) ) );
+	HIP_CHECK( hipMemcpy( dA, a.data(), count * sizeof( ShapeDev ), hipMemcpyHostToDevice ) );
+	HIP_CHECK( hipMemcpy( dB, b.data(), count * sizeof( ShapeDev ), hipMemcpyHostToDevice ) );
+	HIP_CHECK( hipMemcpy( dX, transforms, count * sizeof( b2Transform ), hipMemcpyHostToDevice ) );
+	CollideBatchKernel<<<Blocks( count ), kBlock>>>( dA, dB, dX, count, dM );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipMemcpy( manifolds, dM, count * sizeof( b2LocalManifold ), hipMemcpyDeviceToHost ) );
+	HIP_CHECK( hipFree( dA ) );
+	HIP_CHECK( hipFree( dB ) );
+	HIP_CHECK( hipFree( dX ) );
+	HIP_CHECK( hipFree( dM ) );
+	return 0;
+}
+
 } // namespace b2g
+
+// Host forms of the manifold functions (the same source the collide kernel runs per lane).
+extern "C" {
+
+B2_API b2SegmentDistanceResult b2SegmentDistance( b2Vec2 p1, b2Vec2 q1, b2Vec2 p2, b2Vec2 q2 )
+{
+	return b2g::SegmentDistance( p1, q1, p2, q2 );
+}
+
+B2_API b2LocalManifold b2CollideCircles( const b2Circle* circleA, const b2Circle* circleB, b2Transform xf )
+{
+	return b2g::CollideCircles( *circleA, *circleB, xf );
+}
+
+B2_API b2LocalManifold b2CollideCapsuleAndCircle( const b2Capsule* capsuleA, const b2Circle* circleB, b2Transform xf )
+{
+	return b2g::CollideCapsuleAndCircle( *capsuleA, *circleB, xf );
+}
+
+B2_API b2LocalManifold b2CollideSegmentAndCircle( const b2Segment* segmentA, const b2Circle* circleB, b2Transform xf )
+{
+	return b2g::CollideSegmentAndCircle( *segmentA, *circleB, xf );
+}
+
+B2_API b2LocalManifold b2CollidePolygonAndCircle( const b2Polygon* polygonA, const b2Circle* circleB, b2Transform xf )
+{
+	return b2g::CollidePolygonAndCircle( *polygonA, *circleB, xf );
+}
+
+B2_API b2LocalManifold b2CollideCapsules( const b2Capsule* capsuleA, const b2Capsule* capsuleB, b2Transform xf )
+{
+	return b2g::CollideCapsules( *capsuleA, *capsuleB, xf );
+}
+
+B2_API b2LocalManifold b2CollideSegmentAndCapsule( const b2Segment* segmentA, const b2Capsule* capsuleB, b2Transform xf )
+{
+	return b2g::CollideSegmentAndCapsule( *segmentA, *capsuleB, xf );
+}
+
+B2_API b2LocalManifold b2CollidePolygonAndCapsule( const b2Polygon* polygonA, const b2Capsule* capsuleB, b2Transform xf )
+{
+	return b2g::CollidePolygonAndCapsule( *polygonA, *capsuleB, xf );
+}
+
+B2_API b2LocalManifold b2CollidePolygons( const b2Polygon* polygonA, const b2Polygon* polygonB, b2Transform xf )
+{
+	return b2g::CollidePolygons( *polygonA, *polygonB, xf );
+}
+
+B2_API b2LocalManifold b2CollideSegmentAndPolygon( const b2Segment* segmentA, const b2Polygon* polygonB, b2Transform xf )
+{
+	return b2g::CollideSegmentAndPolygon( *segmentA, *polygonB, xf );
+}
+
+} // extern "C"

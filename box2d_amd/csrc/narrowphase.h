@@ -21,7 +21,7 @@
 namespace b2g
 {
 
-#define B2G_DEV __device__ __forceinline__
+#define B2G_DEV __host__ __device__ __forceinline__
 #define B2G_MAKE_ID( A, B ) ( (uint16_t)( ( (uint8_t)( A ) << 8 ) | (uint8_t)( B ) ) )
 
 constexpr float kLinearSlop = 0.005f;

@@ -330,6 +330,16 @@ static b2sScene* b2sGet( int handle )
 	return s_scenes + handle;
 }
 
+uint32_t b2s_world_id( int handle )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	return (uint32_t)scene->worldId.index1 | ( (uint32_t)scene->worldId.generation << 16 );
+}
+
 void b2s_destroy( int handle )
 {
 	b2sScene* scene = b2sGet( handle );

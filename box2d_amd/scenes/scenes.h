@@ -41,6 +41,9 @@ enum
 // Returns a handle >= 0, or -1 on failure.
 B2S_API int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount );
 
+// The b2WorldId behind a handle packed as index1 | generation << 16 (0 if the handle is invalid).
+B2S_API uint32_t b2s_world_id( int handle );
+
 // Destroy the world behind a handle.
 B2S_API void b2s_destroy( int handle );
 

@@ -1,0 +1,65 @@
+// box2d_amd extensions to the Box2D C API.
+//
+// These entry points have no counterpart in the reference library. They expose what a GPU step
+// adds: per-kernel timing of the step for roofline measurement, and batched GPU evaluation of the
+// narrowphase manifold functions (the GPU form of the reference's b2Collide* routines,
+// reference include/box2d/collision.h:615-680) for callers that collide many shape pairs at once.
+#pragma once
+
+#include "box2d/box2d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/// Aggregated GPU time of one kernel class of the step.
+typedef struct b2amd_KernelStats
+{
+	char name[32];
+	/// number of launches recorded
+	int launches;
+	/// summed launch durations in milliseconds (HIP events on the world's stream)
+	double totalMs;
+	/// summed algorithmic bytes the launches had to move (fields read and written once)
+	double algorithmicBytes;
+} b2amd_KernelStats;
+
+/// Record per-kernel HIP event timings during b2World_Step (adds a small overhead per launch).
+B2_API void b2amd_World_EnableKernelTiming( b2WorldId worldId, bool flag );
+
+/// Copy the accumulated kernel statistics. Returns the number of kernel classes written.
+B2_API int b2amd_World_GetKernelStats( b2WorldId worldId, b2amd_KernelStats* stats, int capacity );
+
+/// Clear the accumulated kernel statistics.
+B2_API void b2amd_World_ResetKernelStats( b2WorldId worldId );
+
+/// Block until all GPU work issued by this library on the current device has completed
+/// (a device-wide synchronize; b2World_Step already returns with its own work complete).
+B2_API void b2amd_Synchronize( void );
+
+/// A shape for batched narrowphase evaluation. type is a b2ShapeType; the union holds the
+/// geometry in the shape's local frame.
+typedef struct b2amd_Shape
+{
+	int type;
+	int padding;
+	union
+	{
+		b2Circle circle;
+		b2Capsule capsule;
+		b2Polygon polygon;
+		b2Segment segment;
+	};
+} b2amd_Shape;
+
+/// Evaluate count manifolds on the GPU: manifold i collides shapesA[i] with shapesB[i], where
+/// transforms[i] is the pose of B in A's frame (as for the reference b2Collide* functions).
+/// Pairs must be in the reference's primary order (A's type >= B's type in
+/// circle < capsule < segment < polygon, with segment-vs-polygon taking the segment as A).
+/// Host arrays in, host array out; synchronous. Returns 0 on success, -1 for an unsupported pair.
+B2_API int b2amd_CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
+							   b2LocalManifold* manifolds );
+
+#ifdef __cplusplus
+}
+#endif

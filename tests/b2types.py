@@ -1,0 +1,130 @@
+"""ctypes mirrors of the Box2D value types the tests pass across the C ABI.
+
+Layouts follow the reference's public headers (include/box2d/math_functions.h, collision.h,
+id.h); include/box2d_amd.h adds b2amd_Shape and b2amd_KernelStats.
+"""
+
+import ctypes
+
+import numpy as np
+
+c_float = ctypes.c_float
+
+
+class Vec2(ctypes.Structure):
+    _fields_ = [("x", c_float), ("y", c_float)]
+
+
+class Rot(ctypes.Structure):
+    _fields_ = [("c", c_float), ("s", c_float)]
+
+
+class Transform(ctypes.Structure):
+    _fields_ = [("p", Vec2), ("q", Rot)]
+
+
+class Circle(ctypes.Structure):
+    _fields_ = [("center", Vec2), ("radius", c_float)]
+
+
+class Capsule(ctypes.Structure):
+    _fields_ = [("center1", Vec2), ("center2", Vec2), ("radius", c_float)]
+
+
+class Segment(ctypes.Structure):
+    _fields_ = [("point1", Vec2), ("point2", Vec2)]
+
+
+class Polygon(ctypes.Structure):
+    _fields_ = [("vertices", Vec2 * 8), ("normals", Vec2 * 8), ("centroid", Vec2), ("radius", c_float),
+                ("count", ctypes.c_int)]
+
+
+class Hull(ctypes.Structure):
+    _fields_ = [("points", Vec2 * 8), ("count", ctypes.c_int)]
+
+
+class ShapeUnion(ctypes.Union):
+    _fields_ = [("circle", Circle), ("capsule", Capsule), ("polygon", Polygon), ("segment", Segment)]
+
+
+class Shape(ctypes.Structure):
+    """b2amd_Shape: a b2ShapeType and the geometry in the shape's frame."""
+    _anonymous_ = ("u",)
+    _fields_ = [("type", ctypes.c_int), ("padding", ctypes.c_int), ("u", ShapeUnion)]
+
+
+class LocalManifoldPoint(ctypes.Structure):
+    _fields_ = [("point", Vec2), ("separation", c_float), ("id", ctypes.c_uint16)]
+
+
+class LocalManifold(ctypes.Structure):
+    _fields_ = [("normal", Vec2), ("points", LocalManifoldPoint * 2), ("pointCount", ctypes.c_int)]
+
+
+class SegmentDistanceResult(ctypes.Structure):
+    _fields_ = [("closest1", Vec2), ("closest2", Vec2), ("fraction1", c_float), ("fraction2", c_float),
+                ("distanceSquared", c_float)]
+
+
+class WorldId(ctypes.Structure):
+    _fields_ = [("index1", ctypes.c_uint16), ("generation", ctypes.c_uint16)]
+
+
+class KernelStats(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int), ("totalMs", ctypes.c_double),
+                ("algorithmicBytes", ctypes.c_double)]
+
+
+# b2ShapeType values (include/box2d/types.h)
+CIRCLE, CAPSULE, SEGMENT, POLYGON, CHAIN_SEGMENT = 0, 1, 2, 3, 4
+
+# primary pair order of the reference contact registry (src/contact.c:188-206)
+PAIR_TYPES = [(CIRCLE, CIRCLE), (CAPSULE, CIRCLE), (CAPSULE, CAPSULE), (POLYGON, CIRCLE), (POLYGON, CAPSULE),
+              (POLYGON, POLYGON), (SEGMENT, CIRCLE), (SEGMENT, CAPSULE), (SEGMENT, POLYGON)]
+
+MANIFOLD_FN = {
+    (CIRCLE, CIRCLE): ("b2CollideCircles", "circle", "circle"),
+    (CAPSULE, CIRCLE): ("b2CollideCapsuleAndCircle", "capsule", "circle"),
+    (CAPSULE, CAPSULE): ("b2CollideCapsules", "capsule", "capsule"),
+    (POLYGON, CIRCLE): ("b2CollidePolygonAndCircle", "polygon", "circle"),
+    (POLYGON, CAPSULE): ("b2CollidePolygonAndCapsule", "polygon", "capsule"),
+    (POLYGON, POLYGON): ("b2CollidePolygons", "polygon", "polygon"),
+    (SEGMENT, CIRCLE): ("b2CollideSegmentAndCircle", "segment", "circle"),
+    (SEGMENT, CAPSULE): ("b2CollideSegmentAndCapsule", "segment", "capsule"),
+    (SEGMENT, POLYGON): ("b2CollideSegmentAndPolygon", "segment", "polygon"),
+}
+
+_FIELD_TYPE = {"circle": Circle, "capsule": Capsule, "polygon": Polygon, "segment": Segment}
+
+
+def bind_manifold_functions(lib):
+    """Set ctypes signatures of the b2Collide* functions on a loaded library."""
+    for (ta, tb), (name, fa, fb) in MANIFOLD_FN.items():
+        fn = getattr(lib, name)
+        fn.restype = LocalManifold
+        fn.argtypes = [ctypes.POINTER(_FIELD_TYPE[fa]), ctypes.POINTER(_FIELD_TYPE[fb]), Transform]
+    return lib
+
+
+def collide_host(lib, shape_a, shape_b, xf):
+    """Call the b2Collide* function of lib for one pair of b2amd_Shape records."""
+    name, fa, fb = MANIFOLD_FN[(shape_a.type, shape_b.type)]
+    fn = getattr(lib, name)
+    return fn(ctypes.byref(getattr(shape_a, fa)), ctypes.byref(getattr(shape_b, fb)), xf)
+
+
+def manifold_words(m):
+    """A b2LocalManifold as 11 uint32 words (normal, 2 x (point, separation, id), pointCount)."""
+    raw = np.frombuffer(bytes(m), dtype=np.uint32).copy()
+    # the id shares a word with 2 bytes of padding: keep only the low 16 bits
+    raw[5] &= 0xFFFF
+    raw[9] &= 0xFFFF
+    return raw
+
+
+def manifold_array_words(arr):
+    raw = np.frombuffer(bytes(arr), dtype=np.uint32).reshape(len(arr), 11).copy()
+    raw[:, 5] &= 0xFFFF
+    raw[:, 9] &= 0xFFFF
+    return raw

@@ -1,0 +1,90 @@
+"""Write the whole-run golden fixtures from the reference library.
+
+Run in a container that holds the reference sources, after `make -C oracle ref` has compiled
+them into oracle/_ref/libb2ref_scenes.so (the reference's own src/*.c plus the public-API scene
+driver box2d_amd/scenes/scenes.c). Every fixture is data only: scene parameters, the step
+protocol, and the reference's outputs (djb2 transform hashes as in shared/determinism.c:110-125,
+world counters, and final transform bits for the small scenes).
+
+    python tests/golden/gen_golden.py
+"""
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from box2d_amd.scenes_api import SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE  # noqa: E402
+
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).reshape(-1).tolist()
+
+
+def trace(lib, h, steps, every):
+    """Hash after each step listed by `every` (1-based step numbers)."""
+    out = {}
+    for s in range(1, steps + 1):
+        lib.step(h)
+        if s in every:
+            out[str(s)] = lib.hash(h)
+    return out
+
+
+def scene_fixture(lib, name, kind, iparams, fparams, steps, every, keep_transforms):
+    h = lib.create(kind, iparams, fparams)
+    fx = {
+        "name": name,
+        "kind": kind,
+        "iparams": list(iparams),
+        "fparams": list(fparams),
+        "dt": 1.0 / 60.0,
+        "substeps": 4,
+        "steps": steps,
+        "body_count": lib.body_count(h),
+        "hash_initial": lib.hash(h),
+    }
+    fx["hashes"] = trace(lib, h, steps, every)
+    fx["counters"] = lib.counters(h)
+    fx["awake_body_count"] = lib.awake_body_count(h)
+    if keep_transforms:
+        fx["transforms_bits"] = bits(lib.transforms(h))
+        fx["velocities_bits"] = bits(lib.velocities(h))
+    lib.destroy(h)
+    return fx
+
+
+def main():
+    if not os.path.exists(REF_LIB):
+        sys.exit(f"{REF_LIB} missing: run `make -C oracle ref` first")
+    lib = SceneLib(REF_LIB)
+    fixtures = []
+    fixtures.append(scene_fixture(lib, "hello", SCENE_HELLO, (), (), 120, set(range(1, 121)), True))
+    fixtures.append(scene_fixture(lib, "pyramid20", SCENE_PYRAMID, (20, 0), (40.0,), 500,
+                                  set(range(1, 501)), True))
+    fixtures.append(scene_fixture(lib, "pyramid20_sleep", SCENE_PYRAMID, (20, 1), (40.0,), 500,
+                                  set(range(1, 501)), True))
+    fixtures.append(scene_fixture(lib, "pyramid100", SCENE_PYRAMID, (100, 0), (100.0,), 500,
+                                  {1, 2, 5, 10, 25, 50, 100, 200, 300, 400, 500}, False))
+    for seed in (1, 2, 3):
+        fixtures.append(scene_fixture(lib, f"mixed_pile_{seed}", SCENE_MIXED_PILE, (64, 1, seed), (), 300,
+                                      set(range(1, 301)), True))
+    ok, step, hsh = lib.falling_hinges(1, 500)
+    hinges = {"name": "falling_hinges", "slept": ok, "sleep_step": step, "hash": hsh}
+    out = {"generator": "tests/golden/gen_golden.py", "reference": "erincatto/box2d v3.2 (src/*.c, -O2 -ffp-contract=off)",
+           "scenes": fixtures, "falling_hinges": hinges}
+    path = os.path.join(HERE, "world_runs.json")
+    with open(path, "w") as f:
+        json.dump(out, f, separators=(",", ":"))
+    print(f"wrote {path} ({os.path.getsize(path)} bytes); falling hinges {ok} {step} {hsh:#x}")
+
+
+if __name__ == "__main__":
+    main()
