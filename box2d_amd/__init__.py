@@ -30,6 +30,16 @@ def library():
 from .scenes_api import SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HELLO, SCENE_MIXED_PILE  # noqa: E402
 
 
+SOLVER_AUTO = 0
+SOLVER_COLOR_KERNELS = 1
+SOLVER_FUSED = 2
+
+
+def set_solver_path(world_id, path):
+    """Select the solver path of a world (SOLVER_AUTO, SOLVER_COLOR_KERNELS or SOLVER_FUSED)."""
+    library().b2amd_World_SetSolverPath(world_id, ctypes.c_int(path))
+
+
 def scenes():
     """Scene driver bound to the GPU library."""
     library()

@@ -632,6 +632,15 @@ void b2amd_World_ResetKernelStats( b2WorldId worldId )
 	}
 }
 
+void b2amd_World_SetSolverPath( b2WorldId worldId, b2amd_SolverPath path )
+{
+	World* world = GetWorld( worldId );
+	if ( world != nullptr )
+	{
+		world->device->SetSolverPath( (int)path );
+	}
+}
+
 void b2amd_Synchronize( void )
 {
 	DeviceSynchronize();

@@ -79,6 +79,9 @@ public:
 	int GetKernelStats( void* stats, int capacity ); // b2amd_KernelStats array
 	void ResetKernelStats();
 
+	// Solver path: 0 auto, 1 color kernels, 2 fused when the world fits.
+	void SetSolverPath( int path );
+
 	void* stream = nullptr;
 
 private:

@@ -48,6 +48,13 @@ const char* LastDeviceError()
 
 constexpr int kBlock = 256;
 
+static int ColorBlockFromEnv()
+{
+	const char* v = getenv( "B2AMD_COLOR_BLOCK" );
+	int b = v != nullptr ? atoi( v ) : 128;
+	return ( b == 64 || b == 128 || b == 256 || b == 512 || b == 1024 ) ? b : 128;
+}
+
 static inline int Blocks( int n )
 {
 	return ( n + kBlock - 1 ) / kBlock;
@@ -313,6 +320,7 @@ struct PrepareArgs
 	const BodySim* sims;
 	const BodyState* states;
 	const BodyAux* aux;
+	const int* bodySlot;
 	StepParams p;
 };
 
@@ -338,6 +346,8 @@ __global__ void PrepareContactsKernel( PrepareArgs a )
 
 	bool staticA = a.aux[bodyIdA].type == b2_staticBody;
 	bool staticB = a.aux[bodyIdB].type == b2_staticBody;
+	c.slotA[i] = staticA ? kStaticSlot : a.bodySlot[bodyIdA];
+	c.slotB[i] = staticB ? kStaticSlot : a.bodySlot[bodyIdB];
 
 	float mA = a.sims[bodyIdA].invMass;
 	float iA = a.sims[bodyIdA].invInertia;
@@ -449,6 +459,8 @@ struct PrepareJointArgs
 	const int* list;
 	int count;
 	const BodySim* sims;
+	const BodyAux* aux;
+	const int* bodySlot;
 	StepParams p;
 };
 
@@ -479,6 +491,8 @@ __global__ void PrepareJointsKernel( PrepareJointArgs a )
 
 	joint->indexA = joint->bodyIdA;
 	joint->indexB = joint->bodyIdB;
+	joint->slotA = a.aux[joint->bodyIdA].type == b2_staticBody ? kStaticSlot : a.bodySlot[joint->bodyIdA];
+	joint->slotB = a.aux[joint->bodyIdB].type == b2_staticBody ? kStaticSlot : a.bodySlot[joint->bodyIdB];
 
 	joint->frameA.q = b2MulRot( simA.transform.q, joint->localFrameA.q );
 	joint->frameA.p = b2RotateVector( simA.transform.q, b2Sub( joint->localFrameA.p, simA.localCenter ) );
@@ -510,27 +524,18 @@ struct BodyArgs
 	StepParams p;
 };
 
-// reference src/solver.c:66-112
-__global__ void IntegrateVelocitiesKernel( BodyArgs a )
+// Velocity integration of one body (reference src/solver.c:66-112).
+B2G_DEV void IntegrateVelocity( const BodySim& sim, BodyState* state, const StepParams& p )
 {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if ( i >= a.count )
-	{
-		return;
-	}
-	int id = a.awake[i];
-	const BodySim& sim = a.sims[id];
-	BodyState* state = a.states + id;
-
 	b2Vec2 v = state->linearVelocity;
 	float w = state->angularVelocity;
-	float h = a.p.h;
+	float h = p.h;
 
 	float linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
 	float angularDamping = 1.0f / ( 1.0f + h * sim.angularDamping );
 	float gravityScale = sim.invMass > 0.0f ? sim.gravityScale : 0.0f;
 
-	b2Vec2 linearVelocityDelta = b2Add( b2MulSV( h * sim.invMass, sim.force ), b2MulSV( h * gravityScale, a.p.gravity ) );
+	b2Vec2 linearVelocityDelta = b2Add( b2MulSV( h * sim.invMass, sim.force ), b2MulSV( h * gravityScale, p.gravity ) );
 	float angularVelocityDelta = h * sim.invInertia * sim.torque;
 
 	v = b2MulAdd( linearVelocityDelta, linearDamping, v );
@@ -540,19 +545,13 @@ __global__ void IntegrateVelocitiesKernel( BodyArgs a )
 	state->angularVelocity = w;
 }
 
-// reference src/solver.c:114-162
-__global__ void IntegratePositionsKernel( BodyArgs a )
+// Position integration of one body: locks, speed caps, delta position and rotation
+// (reference src/solver.c:114-162).
+B2G_DEV void IntegratePosition( BodyState* state, const StepParams& p )
 {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if ( i >= a.count )
-	{
-		return;
-	}
-	BodyState* state = a.states + a.awake[i];
-
-	float h = a.p.h;
-	float maxLinearSpeed = a.p.maxLinearVelocity;
-	float maxAngularSpeed = ( 0.25f * B2_PI ) * a.p.inv_dt;
+	float h = p.h;
+	float maxLinearSpeed = p.maxLinearVelocity;
+	float maxAngularSpeed = ( 0.25f * B2_PI ) * p.inv_dt;
 	float maxLinearSpeedSquared = maxLinearSpeed * maxLinearSpeed;
 	float maxAngularSpeedSquared = maxAngularSpeed * maxAngularSpeed;
 
@@ -583,6 +582,27 @@ __global__ void IntegratePositionsKernel( BodyArgs a )
 	state->angularVelocity = w;
 	state->deltaPosition = b2MulAdd( state->deltaPosition, h, v );
 	state->deltaRotation = b2IntegrateRotation( state->deltaRotation, h * w );
+}
+
+__global__ void IntegrateVelocitiesKernel( BodyArgs a )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= a.count )
+	{
+		return;
+	}
+	int id = a.awake[i];
+	IntegrateVelocity( a.sims[id], a.states + id, a.p );
+}
+
+__global__ void IntegratePositionsKernel( BodyArgs a )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= a.count )
+	{
+		return;
+	}
+	IntegratePosition( a.states + a.awake[i], a.p );
 }
 
 enum ColorStage
@@ -623,26 +643,25 @@ B2G_DEV bool GroupAnyNonZero( const ConstraintSoA& c, int f, int colorStart, int
 	return false;
 }
 
-// One graph color of one stage: joints occupy the first jointCount lanes, contacts the rest.
-// One instantiation per stage so each stage is its own kernel (and its own profiler row).
-template <int Stage>
-__global__ void ColorKernel( ColorArgs a )
+// Lane t of one graph color of one stage: joints occupy the first jointCount lanes, contacts
+// the rest (joint and contact blocks of one color touch disjoint bodies).
+template <int Stage, typename Bodies>
+B2G_DEV void ColorLane( const ColorArgs& a, int t, const Bodies& B )
 {
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( t < a.jointCount )
 	{
 		JointSim* joint = a.joints + a.jointList[a.jointStart + t];
 		if ( Stage == kStageWarmStart )
 		{
-			WarmStartRevolute( joint, a.states );
+			WarmStartRevolute( joint, B );
 		}
 		else if ( Stage == kStageSolve )
 		{
-			SolveRevolute( joint, a.states, true, a.p.h, a.p.inv_h );
+			SolveRevolute( joint, B, true, a.p.h, a.p.inv_h );
 		}
 		else if ( Stage == kStageRelax )
 		{
-			SolveRevolute( joint, a.states, false, a.p.h, a.p.inv_h );
+			SolveRevolute( joint, B, false, a.p.h, a.p.inv_h );
 		}
 		return;
 	}
@@ -656,72 +675,239 @@ __global__ void ColorKernel( ColorArgs a )
 
 	if ( Stage == kStageWarmStart )
 	{
-		WarmStartContact( a.c, i, a.states );
+		WarmStartContact( a.c, i, B );
 	}
 	else if ( Stage == kStageSolve )
 	{
-		SolveContact( a.c, i, a.states, true, a.p.inv_h, -a.p.contactSpeed, false );
+		SolveContact( a.c, i, B, true, a.p.inv_h, -a.p.contactSpeed, false );
 	}
 	else if ( Stage == kStageRelax )
 	{
 		bool rolling = GroupAnyNonZero( a.c, kcRollingResistance, a.contactStart, a.contactCount, k );
-		SolveContact( a.c, i, a.states, false, a.p.inv_h, -a.p.contactSpeed, rolling );
+		SolveContact( a.c, i, B, false, a.p.inv_h, -a.p.contactSpeed, rolling );
 	}
 	else
 	{
 		if ( GroupAnyNonZero( a.c, kcRestitution, a.contactStart, a.contactCount, k ) )
 		{
-			RestitutionContact( a.c, i, a.states, a.p.restitutionThreshold );
+			RestitutionContact( a.c, i, B, a.p.restitutionThreshold );
 		}
 	}
 }
 
-// Overflow constraints: a single lane walks joints then contacts in order, as the reference's
-// single-threaded overflow passes do (src/solver.c:1100-1153).
-__global__ void OverflowKernel( ColorArgs a )
+// Overflow constraints of one stage, walked by a single lane: joints then contacts in order, as
+// the reference's single-threaded overflow passes do (src/solver.c:1100-1168).
+template <int Stage, typename Bodies>
+B2G_DEV void OverflowSerial( const ColorArgs& a, const Bodies& B )
 {
-	if ( blockIdx.x != 0 || threadIdx.x != 0 )
-	{
-		return;
-	}
-
 	for ( int t = 0; t < a.jointCount; ++t )
 	{
 		JointSim* joint = a.joints + a.jointList[a.jointStart + t];
-		if ( a.stage == kStageWarmStart )
+		if ( Stage == kStageWarmStart )
 		{
-			WarmStartRevolute( joint, a.states );
+			WarmStartRevolute( joint, B );
 		}
-		else if ( a.stage == kStageSolve )
+		else if ( Stage == kStageSolve )
 		{
-			SolveRevolute( joint, a.states, true, a.p.h, a.p.inv_h );
+			SolveRevolute( joint, B, true, a.p.h, a.p.inv_h );
 		}
-		else if ( a.stage == kStageRelax )
+		else if ( Stage == kStageRelax )
 		{
-			SolveRevolute( joint, a.states, false, a.p.h, a.p.inv_h );
+			SolveRevolute( joint, B, false, a.p.h, a.p.inv_h );
 		}
 	}
 
 	for ( int k = 0; k < a.contactCount; ++k )
 	{
 		int i = a.contactStart + k;
-		switch ( a.stage )
+		if ( Stage == kStageWarmStart )
 		{
-			case kStageWarmStart:
-				OverflowWarmStart( a.c, i, a.states );
-				break;
-			case kStageSolve:
-				OverflowSolve( a.c, i, a.states, true, a.p.inv_h, a.p.contactSpeed );
-				break;
-			case kStageRelax:
-				OverflowSolve( a.c, i, a.states, false, a.p.inv_h, a.p.contactSpeed );
-				break;
-			case kStageRestitution:
-				OverflowRestitution( a.c, i, a.states, a.p.restitutionThreshold );
-				break;
-			default:
-				break;
+			OverflowWarmStart( a.c, i, B );
 		}
+		else if ( Stage == kStageSolve )
+		{
+			OverflowSolve( a.c, i, B, true, a.p.inv_h, a.p.contactSpeed );
+		}
+		else if ( Stage == kStageRelax )
+		{
+			OverflowSolve( a.c, i, B, false, a.p.inv_h, a.p.contactSpeed );
+		}
+		else
+		{
+			OverflowRestitution( a.c, i, B, a.p.restitutionThreshold );
+		}
+	}
+}
+
+// One graph color of one stage over the id-indexed body states in device memory.
+template <int Stage>
+__global__ void ColorKernel( ColorArgs a )
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	ColorLane<Stage>( a, t, GlobalBodies{ a.states } );
+}
+
+template <int Stage>
+__global__ void OverflowKernel( ColorArgs a )
+{
+	if ( blockIdx.x != 0 || threadIdx.x != 0 )
+	{
+		return;
+	}
+	OverflowSerial<Stage>( a, GlobalBodies{ a.states } );
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused solver: the whole substep loop of one world in one workgroup.
+//
+// The world's awake body states live in LDS for the duration of the solve (7 floats and 16 flag
+// bits per body, 30 bytes, so kFusedBodyCapacity bodies fit in 160 KB). Colors and stages are
+// separated by workgroup barriers instead of kernel boundaries; constraint fields stream from
+// device memory. Stage order and per-constraint arithmetic are those of the color-kernel path
+// (reference src/solver.c:1090-1170).
+
+constexpr int kFusedThreads = 1024;
+constexpr int kFusedMaxContacts = 4096;
+constexpr int kMaxColorSlots = 24;
+
+struct FusedArgs
+{
+	ColorArgs color; // c (with slot arrays), joints, jointList, p; ranges set per color below
+	const int* awake;
+	int bodyCount;
+	const BodySim* sims;
+	BodyState* states;
+	int activeColorCount;
+	int colorContactStart[kMaxColorSlots + 1];
+	int colorJointStart[kMaxColorSlots + 1];
+	int overflowContactStart;
+	int overflowContactCount;
+	int overflowJointStart;
+	int overflowJointCount;
+	int subStepCount;
+	int anyRestitution;
+};
+
+template <int Stage>
+B2G_DEV void FusedStage( const FusedArgs& a, ColorArgs& ca, const LdsBodies& B )
+{
+	if ( a.overflowContactCount > 0 || a.overflowJointCount > 0 )
+	{
+		if ( threadIdx.x == 0 )
+		{
+			ColorArgs oa = ca;
+			oa.jointStart = a.overflowJointStart;
+			oa.jointCount = a.overflowJointCount;
+			oa.contactStart = a.overflowContactStart;
+			oa.contactCount = a.overflowContactCount;
+			OverflowSerial<Stage>( oa, B );
+		}
+		__syncthreads();
+	}
+
+	for ( int color = 0; color < a.activeColorCount; ++color )
+	{
+		ca.jointStart = a.colorJointStart[color];
+		ca.jointCount = a.colorJointStart[color + 1] - ca.jointStart;
+		ca.contactStart = a.colorContactStart[color];
+		ca.contactCount = a.colorContactStart[color + 1] - ca.contactStart;
+		int lanes = ca.jointCount + ca.contactCount;
+		if ( lanes == 0 || ( Stage == kStageRestitution && ca.contactCount == 0 ) )
+		{
+			continue;
+		}
+		for ( int t = threadIdx.x; t < lanes; t += kFusedThreads )
+		{
+			ColorLane<Stage>( ca, t, B );
+		}
+		__syncthreads();
+	}
+}
+
+__global__ __launch_bounds__( kFusedThreads ) void SolveFusedKernel( FusedArgs a )
+{
+	__shared__ float s_vx[kFusedBodyCapacity + 1];
+	__shared__ float s_vy[kFusedBodyCapacity + 1];
+	__shared__ float s_w[kFusedBodyCapacity + 1];
+	__shared__ float s_dpx[kFusedBodyCapacity + 1];
+	__shared__ float s_dpy[kFusedBodyCapacity + 1];
+	__shared__ float s_dqc[kFusedBodyCapacity + 1];
+	__shared__ float s_dqs[kFusedBodyCapacity + 1];
+	__shared__ uint16_t s_flags[kFusedBodyCapacity + 1];
+
+	LdsBodies B{ s_vx, s_vy, s_w, s_dpx, s_dpy, s_dqc, s_dqs, s_flags };
+	const int n = a.bodyCount;
+	const GlobalBodies G{ a.states };
+
+	auto put = [&]( int slot, const BodyState& st ) {
+		s_vx[slot] = st.linearVelocity.x;
+		s_vy[slot] = st.linearVelocity.y;
+		s_w[slot] = st.angularVelocity;
+		s_flags[slot] = (uint16_t)st.flags;
+		s_dpx[slot] = st.deltaPosition.x;
+		s_dpy[slot] = st.deltaPosition.y;
+		s_dqc[slot] = st.deltaRotation.c;
+		s_dqs[slot] = st.deltaRotation.s;
+	};
+
+	for ( int i = threadIdx.x; i < n; i += kFusedThreads )
+	{
+		put( i, ToState( G.Load( a.awake[i] ) ) );
+	}
+	if ( threadIdx.x == 0 )
+	{
+		// identity state of static bodies (reference b2_identityBodyState)
+		BodyState id;
+		id.linearVelocity = { 0.0f, 0.0f };
+		id.angularVelocity = 0.0f;
+		id.flags = 0;
+		id.deltaPosition = { 0.0f, 0.0f };
+		id.deltaRotation = { 1.0f, 0.0f };
+		put( kStaticSlot, id );
+	}
+	__syncthreads();
+
+	ColorArgs ca = a.color;
+	for ( int sub = 0; sub < a.subStepCount; ++sub )
+	{
+		for ( int i = threadIdx.x; i < n; i += kFusedThreads )
+		{
+			BodyState st = ToState( B.Load( i ) );
+			IntegrateVelocity( a.sims[a.awake[i]], &st, a.color.p );
+			s_vx[i] = st.linearVelocity.x;
+			s_vy[i] = st.linearVelocity.y;
+			s_w[i] = st.angularVelocity;
+		}
+		__syncthreads();
+
+		FusedStage<kStageWarmStart>( a, ca, B );
+		FusedStage<kStageSolve>( a, ca, B );
+
+		for ( int i = threadIdx.x; i < n; i += kFusedThreads )
+		{
+			BodyState st = ToState( B.Load( i ) );
+			IntegratePosition( &st, a.color.p );
+			put( i, st );
+		}
+		__syncthreads();
+
+		FusedStage<kStageRelax>( a, ca, B );
+	}
+
+	if ( a.anyRestitution )
+	{
+		FusedStage<kStageRestitution>( a, ca, B );
+	}
+
+	// write the whole state back (flags carry the speed-cap bit to finalize)
+	for ( int i = threadIdx.x; i < n; i += kFusedThreads )
+	{
+		BodyState* dst = a.states + a.awake[i];
+		dst->linearVelocity = { s_vx[i], s_vy[i] };
+		dst->angularVelocity = s_w[i];
+		dst->flags = ( dst->flags & 0xFFFF0000u ) | s_flags[i];
+		dst->deltaPosition = { s_dpx[i], s_dpy[i] };
+		dst->deltaRotation = { s_dqc[i], s_dqs[i] };
 	}
 }
 
@@ -967,12 +1153,13 @@ enum KernelClass
 	kkOverflow,
 	kkStore,
 	kkFinalize,
+	kkSolveFused,
 	kKernelClassCount
 };
 
 static const char* s_kernelClassNames[kKernelClassCount] = {
 	"collide", "prepare_contacts", "prepare_joints", "integrate_velocities", "warm_start", "solve",
-	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies",
+	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies", "solve_fused",
 };
 
 constexpr double kBytesCollideFull = 800.0;		// per contact with a new manifold
@@ -987,6 +1174,91 @@ constexpr double kBytesRestitution = 236.0;
 constexpr double kBytesStore = 76.0;
 constexpr double kBytesFinalize = 370.0;
 constexpr double kBytesJoint = 256.0; // revolute joint record read and written plus two body states
+// Fused solver (body states in LDS): constraint fields of warm start 100 B, solve 124 B and relax
+// 156 B per contact per substep; 32 B of body sim per body per substep; 64 B of body state read
+// and written once per step.
+constexpr double kBytesFusedContactSubstep = 380.0;
+constexpr double kBytesFusedBodySubstep = 32.0;
+constexpr double kBytesFusedBodyStep = 64.0;
+
+// ---------------------------------------------------------------------------------------------
+// Readback: one small kernel per host sync point copies counters and variable-length result
+// lists into a pinned host mailbox and clears the counters for the next step, so each sync
+// point costs one launch and one stream synchronize.
+
+constexpr int kMailHeader = 8;
+constexpr int kPackThreads = 1024;
+
+__global__ __launch_bounds__( kPackThreads ) void PackCollideKernel( int* counters, const int* changes, int* mail )
+{
+	int count = counters[0];
+	int recycled = counters[1];
+	for ( int i = threadIdx.x; i < count; i += kPackThreads )
+	{
+		mail[kMailHeader + i] = changes[i];
+	}
+	__syncthreads();
+	if ( threadIdx.x == 0 )
+	{
+		mail[0] = count;
+		mail[1] = recycled;
+		counters[0] = 0;
+		counters[1] = 0;
+	}
+}
+
+struct PackFinalizeArgs
+{
+	int* counters; // enlarged bodies, enlarged shapes, fast bodies
+	const int* enlargedBodies;
+	const int* enlargedShapes;
+	const b2AABB* enlargedShapeAABBs;
+	const float* sleepTimes; // null when not needed
+	int bodyCount;
+	int shapeOffset; // mailbox offsets in ints
+	int aabbOffset;
+	int sleepOffset;
+	int* mail;
+};
+
+__global__ __launch_bounds__( kPackThreads ) void PackFinalizeKernel( PackFinalizeArgs a )
+{
+	int bodies = a.counters[0];
+	int shapes = a.counters[1];
+	int fast = a.counters[2];
+	for ( int i = threadIdx.x; i < bodies; i += kPackThreads )
+	{
+		a.mail[kMailHeader + i] = a.enlargedBodies[i];
+	}
+	for ( int i = threadIdx.x; i < shapes; i += kPackThreads )
+	{
+		a.mail[a.shapeOffset + i] = a.enlargedShapes[i];
+		b2AABB box = a.enlargedShapeAABBs[i];
+		float* dst = reinterpret_cast<float*>( a.mail + a.aabbOffset + 4 * i );
+		dst[0] = box.lowerBound.x;
+		dst[1] = box.lowerBound.y;
+		dst[2] = box.upperBound.x;
+		dst[3] = box.upperBound.y;
+	}
+	if ( a.sleepTimes != nullptr )
+	{
+		float* dst = reinterpret_cast<float*>( a.mail + a.sleepOffset );
+		for ( int i = threadIdx.x; i < a.bodyCount; i += kPackThreads )
+		{
+			dst[i] = a.sleepTimes[i];
+		}
+	}
+	__syncthreads();
+	if ( threadIdx.x == 0 )
+	{
+		a.mail[0] = bodies;
+		a.mail[1] = shapes;
+		a.mail[2] = fast;
+		a.counters[0] = 0;
+		a.counters[1] = 0;
+		a.counters[2] = 0;
+	}
+}
 
 // ---------------------------------------------------------------------------------------------
 // Device buffers
@@ -1053,10 +1325,15 @@ struct DeviceWorld::Impl
 
 	// work lists
 	DevArray<int> awakeBodies;
+	DevArray<int> bodySlot; // body id -> fused solver LDS slot (awake index), kStaticSlot otherwise
 	DevArray<int> collideList;
 	DevArray<int> jointList;
 	DevArray<int> changes;
 	SolverLists lists;
+	std::vector<int> slotStaging;
+	int solverPath = 0;
+	// Workgroup size of the color kernels (64..1024; launches are latency-bound, so it matters little).
+	int colorBlock = ColorBlockFromEnv();
 
 	// constraints
 	DevArray<float> cfields;
@@ -1073,6 +1350,27 @@ struct DeviceWorld::Impl
 	// counters: [0..1] collide, [4..6] finalize
 	int* counters = nullptr;
 	int* hostCounters = nullptr;
+
+	// pinned host mailbox written by the pack kernels
+	int* mail = nullptr;
+	size_t mailCapacity = 0; // ints
+
+	void EnsureMail( size_t ints )
+	{
+		if ( ints <= mailCapacity )
+		{
+			return;
+		}
+		size_t cap = std::max<size_t>( ints, mailCapacity + mailCapacity / 2 );
+		cap = std::max<size_t>( cap, 4096 );
+		HIP_CHECK( hipStreamSynchronize( stream ) );
+		if ( mail != nullptr )
+		{
+			HIP_CHECK( hipHostFree( mail ) );
+		}
+		HIP_CHECK( hipHostMalloc( (void**)&mail, cap * sizeof( int ), hipHostMallocDefault ) );
+		mailCapacity = cap;
+	}
 
 	hipEvent_t ev[4] = {};
 
@@ -1150,7 +1448,7 @@ struct DeviceWorld::Impl
 		cfields.Free();
 		cints.Free();
 		cfields.Reserve( cap * kcFieldCount, false, stream, 0 );
-		cints.Reserve( cap * 4, false, stream, 0 );
+		cints.Reserve( cap * 6, false, stream, 0 );
 		constraintCapacity = cap;
 		for ( int f = 0; f < kcFieldCount; ++f )
 		{
@@ -1160,6 +1458,8 @@ struct DeviceWorld::Impl
 		soa.bodyB = cints.ptr + cap;
 		soa.contactId = cints.ptr + 2 * cap;
 		soa.pointCount = cints.ptr + 3 * cap;
+		soa.slotA = cints.ptr + 4 * cap;
+		soa.slotB = cints.ptr + 5 * cap;
 	}
 };
 
@@ -1168,6 +1468,8 @@ DeviceWorld::DeviceWorld()
 	impl = new Impl();
 	HIP_CHECK( hipStreamCreateWithFlags( &impl->stream, hipStreamNonBlocking ) );
 	HIP_CHECK( hipMalloc( &impl->counters, 16 * sizeof( int ) ) );
+	// zero once: from then on the pack kernels clear the counters after reading them
+	HIP_CHECK( hipMemset( impl->counters, 0, 16 * sizeof( int ) ) );
 	HIP_CHECK( hipHostMalloc( &impl->hostCounters, 16 * sizeof( int ), hipHostMallocDefault ) );
 	for ( int i = 0; i < 4; ++i )
 	{
@@ -1191,6 +1493,7 @@ DeviceWorld::~DeviceWorld()
 	impl->joints.Free();
 	impl->inits.Free();
 	impl->awakeBodies.Free();
+	impl->bodySlot.Free();
 	impl->collideList.Free();
 	impl->jointList.Free();
 	impl->changes.Free();
@@ -1202,6 +1505,10 @@ DeviceWorld::~DeviceWorld()
 	impl->sleepTimes.Free();
 	(void)hipFree( impl->counters );
 	(void)hipHostFree( impl->hostCounters );
+	if ( impl->mail != nullptr )
+	{
+		(void)hipHostFree( impl->mail );
+	}
 	for ( int i = 0; i < 4; ++i )
 	{
 		(void)hipEventDestroy( impl->ev[i] );
@@ -1332,6 +1639,19 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 {
 	impl->lists = lists;
 	UploadVector( impl->awakeBodies, lists.awakeBodies, impl->stream );
+	{
+		int slots = impl->bodyCount;
+		for ( int id : lists.awakeBodies )
+		{
+			slots = std::max( slots, id + 1 );
+		}
+		impl->slotStaging.assign( std::max( slots, 1 ), kStaticSlot );
+		for ( size_t k = 0; k < lists.awakeBodies.size(); ++k )
+		{
+			impl->slotStaging[lists.awakeBodies[k]] = (int)std::min<size_t>( k, kStaticSlot );
+		}
+		UploadVector( impl->bodySlot, impl->slotStaging, impl->stream );
+	}
 	UploadVector( impl->collideList, lists.collideContacts, impl->stream );
 	UploadVector( impl->jointList, lists.jointList, impl->stream );
 
@@ -1363,7 +1683,7 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 
 	hipStream_t s = impl->stream;
 	HIP_CHECK( hipEventRecord( impl->ev[0], s ) );
-	HIP_CHECK( hipMemsetAsync( impl->counters, 0, 2 * sizeof( int ), s ) );
+	impl->EnsureMail( kMailHeader + (size_t)n );
 
 	CollideArgs a;
 	a.list = impl->collideList.ptr;
@@ -1379,11 +1699,12 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 	impl->Launch( kkCollide, 0.0, [&]() { CollideKernel<<<Blocks( n ), kBlock, 0, s>>>( a ); } );
 	HIP_CHECK( hipEventRecord( impl->ev[1], s ) );
 
-	HIP_CHECK( hipMemcpyAsync( impl->hostCounters, impl->counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	PackCollideKernel<<<1, kPackThreads, 0, s>>>( impl->counters, impl->changes.ptr, impl->mail );
+	HIP_CHECK( hipGetLastError() );
 	HIP_CHECK( hipStreamSynchronize( s ) );
 
-	int changeCount = impl->hostCounters[0];
-	result->recycledCount = impl->hostCounters[1];
+	int changeCount = impl->mail[0];
+	result->recycledCount = impl->mail[1];
 	if ( impl->timing && impl->pending.empty() == false )
 	{
 		int recycled = result->recycledCount;
@@ -1393,9 +1714,7 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 
 	if ( changeCount > 0 )
 	{
-		result->changes.resize( changeCount );
-		HIP_CHECK( hipMemcpyAsync( result->changes.data(), impl->changes.ptr, changeCount * sizeof( int ), hipMemcpyDeviceToHost, s ) );
-		HIP_CHECK( hipStreamSynchronize( s ) );
+		result->changes.assign( impl->mail + kMailHeader, impl->mail + kMailHeader + changeCount );
 		std::sort( result->changes.begin(), result->changes.end() );
 	}
 
@@ -1421,6 +1740,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		pa.sims = impl->sims.ptr;
 		pa.states = impl->states.ptr;
 		pa.aux = impl->aux.ptr;
+		pa.bodySlot = impl->bodySlot.ptr;
 		pa.p = params;
 		impl->Launch( kkPrepareContacts, kBytesPrepare * nc, [&]() { PrepareContactsKernel<<<Blocks( nc ), kBlock, 0, s>>>( pa ); } );
 	}
@@ -1432,6 +1752,8 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		ja.list = impl->jointList.ptr;
 		ja.count = nj;
 		ja.sims = impl->sims.ptr;
+		ja.aux = impl->aux.ptr;
+		ja.bodySlot = impl->bodySlot.ptr;
 		ja.p = params;
 		impl->Launch( kkPrepareJoints, kBytesJoint * nj, [&]() { PrepareJointsKernel<<<Blocks( nj ), kBlock, 0, s>>>( ja ); } );
 	}
@@ -1478,6 +1800,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		}
 	};
 
+	const int cb = impl->colorBlock;
 	auto runStage = [&]( int stage ) {
 		if ( L.overflowContactCount > 0 || L.overflowJointCount > 0 )
 		{
@@ -1487,7 +1810,23 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 			ca.contactStart = L.overflowContactStart;
 			ca.contactCount = L.overflowContactCount;
 			double bytes = stageBytes( stage ) * ca.contactCount + kBytesJoint * ca.jointCount;
-			impl->Launch( kkOverflow, bytes, [&]() { OverflowKernel<<<1, 64, 0, s>>>( ca ); } );
+			impl->Launch( kkOverflow, bytes, [&]() {
+				switch ( stage )
+				{
+					case kStageWarmStart:
+						OverflowKernel<kStageWarmStart><<<1, 64, 0, s>>>( ca );
+						break;
+					case kStageSolve:
+						OverflowKernel<kStageSolve><<<1, 64, 0, s>>>( ca );
+						break;
+					case kStageRelax:
+						OverflowKernel<kStageRelax><<<1, 64, 0, s>>>( ca );
+						break;
+					default:
+						OverflowKernel<kStageRestitution><<<1, 64, 0, s>>>( ca );
+						break;
+				}
+			} );
 		}
 
 		for ( int c = 0; c < L.activeColorCount; ++c )
@@ -1511,36 +1850,70 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 				switch ( stage )
 				{
 					case kStageWarmStart:
-						ColorKernel<kStageWarmStart><<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+						ColorKernel<kStageWarmStart><<<( lanes + cb - 1 ) / cb, cb, 0, s>>>( ca );
 						break;
 					case kStageSolve:
-						ColorKernel<kStageSolve><<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+						ColorKernel<kStageSolve><<<( lanes + cb - 1 ) / cb, cb, 0, s>>>( ca );
 						break;
 					case kStageRelax:
-						ColorKernel<kStageRelax><<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+						ColorKernel<kStageRelax><<<( lanes + cb - 1 ) / cb, cb, 0, s>>>( ca );
 						break;
 					default:
-						ColorKernel<kStageRestitution><<<Blocks( lanes ), kBlock, 0, s>>>( ca );
+						ColorKernel<kStageRestitution><<<( lanes + cb - 1 ) / cb, cb, 0, s>>>( ca );
 						break;
 				}
 			} );
 		}
 	};
 
-	for ( int sub = 0; sub < params.subStepCount; ++sub )
+	// The fused solver runs on one compute unit: it wins while the per-color launch latency of the
+	// color kernels dominates, and loses once a world has enough constraints to need the whole GPU.
+	bool fits = nb > 0 && nb <= kFusedBodyCapacity && L.activeColorCount <= kMaxColorSlots;
+	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && nc <= kFusedMaxContacts ) );
+	if ( fused )
 	{
-		impl->Launch( kkIntegrateVelocities, kBytesIntegrateVelocities * nb,
-					  [&]() { IntegrateVelocitiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba ); } );
-		runStage( kStageWarmStart );
-		runStage( kStageSolve );
-		impl->Launch( kkIntegratePositions, kBytesIntegratePositions * nb,
-					  [&]() { IntegratePositionsKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba ); } );
-		runStage( kStageRelax );
+		FusedArgs fa;
+		fa.color = ca;
+		fa.color.c.bodyA = impl->soa.slotA;
+		fa.color.c.bodyB = impl->soa.slotB;
+		fa.awake = impl->awakeBodies.ptr;
+		fa.bodyCount = nb;
+		fa.sims = impl->sims.ptr;
+		fa.states = impl->states.ptr;
+		fa.activeColorCount = L.activeColorCount;
+		for ( int c = 0; c <= L.activeColorCount; ++c )
+		{
+			fa.colorContactStart[c] = L.colorContactStart[c];
+			fa.colorJointStart[c] = L.colorJointStart[c];
+		}
+		fa.overflowContactStart = L.overflowContactStart;
+		fa.overflowContactCount = L.overflowContactCount;
+		fa.overflowJointStart = L.overflowJointStart;
+		fa.overflowJointCount = L.overflowJointCount;
+		fa.subStepCount = params.subStepCount;
+		fa.anyRestitution = anyRestitution ? 1 : 0;
+		// constraint fields per substep (warm start, solve, relax) + body sims per substep + state in/out
+		double bytes = params.subStepCount * ( kBytesFusedContactSubstep * nc + kBytesFusedBodySubstep * nb + kBytesJoint * nj ) +
+					   kBytesFusedBodyStep * nb;
+		impl->Launch( kkSolveFused, bytes, [&]() { SolveFusedKernel<<<1, kFusedThreads, 0, s>>>( fa ); } );
 	}
-
-	if ( anyRestitution )
+	else
 	{
-		runStage( kStageRestitution );
+		for ( int sub = 0; sub < params.subStepCount; ++sub )
+		{
+			impl->Launch( kkIntegrateVelocities, kBytesIntegrateVelocities * nb,
+						  [&]() { IntegrateVelocitiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba ); } );
+			runStage( kStageWarmStart );
+			runStage( kStageSolve );
+			impl->Launch( kkIntegratePositions, kBytesIntegratePositions * nb,
+						  [&]() { IntegratePositionsKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba ); } );
+			runStage( kStageRelax );
+		}
+
+		if ( anyRestitution )
+		{
+			runStage( kStageRestitution );
+		}
 	}
 
 	if ( nc > 0 )
@@ -1552,8 +1925,6 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		sa.totalCount = nc;
 		impl->Launch( kkStore, kBytesStore * nc, [&]() { StoreImpulsesKernel<<<Blocks( nc ), kBlock, 0, s>>>( sa ); } );
 	}
-
-	HIP_CHECK( hipMemsetAsync( impl->counters + 4, 0, 3 * sizeof( int ), s ) );
 
 	FinalizeArgs fa;
 	fa.awake = impl->awakeBodies.ptr;
@@ -1571,36 +1942,43 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 	impl->Launch( kkFinalize, kBytesFinalize * nb, [&]() { FinalizeBodiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( fa ); } );
 	HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
 
-	HIP_CHECK( hipMemcpyAsync( impl->hostCounters + 4, impl->counters + 4, 3 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
-	HIP_CHECK( hipStreamSynchronize( s ) );
-	impl->FlushTiming();
+	{
+		size_t shapeCap = std::max<size_t>( impl->shapeCount, 1 );
+		PackFinalizeArgs pk;
+		pk.counters = impl->counters + 4;
+		pk.enlargedBodies = impl->enlargedBodies.ptr;
+		pk.enlargedShapes = impl->enlargedShapes.ptr;
+		pk.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
+		pk.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
+		pk.bodyCount = nb;
+		pk.shapeOffset = kMailHeader + nb;
+		pk.aabbOffset = pk.shapeOffset + (int)shapeCap;
+		pk.sleepOffset = pk.aabbOffset + 4 * (int)shapeCap;
+		impl->EnsureMail( (size_t)pk.sleepOffset + nb );
+		pk.mail = impl->mail;
+		PackFinalizeKernel<<<1, kPackThreads, 0, s>>>( pk );
+		HIP_CHECK( hipGetLastError() );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+		impl->FlushTiming();
 
-	int enlargedBodyCount = impl->hostCounters[4];
-	int enlargedShapeCount = impl->hostCounters[5];
-	result->fastBodyCount = impl->hostCounters[6];
-
-	result->enlargedBodies.resize( enlargedBodyCount );
-	result->enlargedShapes.resize( enlargedShapeCount );
-	result->enlargedShapeAABBs.resize( enlargedShapeCount );
-	if ( enlargedBodyCount > 0 )
-	{
-		HIP_CHECK( hipMemcpyAsync( result->enlargedBodies.data(), impl->enlargedBodies.ptr, enlargedBodyCount * sizeof( int ),
-								   hipMemcpyDeviceToHost, s ) );
+		const int* mail = impl->mail;
+		int enlargedBodyCount = mail[0];
+		int enlargedShapeCount = mail[1];
+		result->fastBodyCount = mail[2];
+		result->enlargedBodies.assign( mail + kMailHeader, mail + kMailHeader + enlargedBodyCount );
+		result->enlargedShapes.assign( mail + pk.shapeOffset, mail + pk.shapeOffset + enlargedShapeCount );
+		result->enlargedShapeAABBs.resize( enlargedShapeCount );
+		if ( enlargedShapeCount > 0 )
+		{
+			memcpy( result->enlargedShapeAABBs.data(), mail + pk.aabbOffset, enlargedShapeCount * sizeof( b2AABB ) );
+		}
+		if ( needSleepTimes && nb > 0 )
+		{
+			result->sleepTimes.resize( nb );
+			memcpy( result->sleepTimes.data(), mail + pk.sleepOffset, nb * sizeof( float ) );
+		}
+		std::sort( result->enlargedBodies.begin(), result->enlargedBodies.end() );
 	}
-	if ( enlargedShapeCount > 0 )
-	{
-		HIP_CHECK( hipMemcpyAsync( result->enlargedShapes.data(), impl->enlargedShapes.ptr, enlargedShapeCount * sizeof( int ),
-								   hipMemcpyDeviceToHost, s ) );
-		HIP_CHECK( hipMemcpyAsync( result->enlargedShapeAABBs.data(), impl->enlargedShapeAABBs.ptr,
-								   enlargedShapeCount * sizeof( b2AABB ), hipMemcpyDeviceToHost, s ) );
-	}
-	if ( needSleepTimes && nb > 0 )
-	{
-		result->sleepTimes.resize( nb );
-		HIP_CHECK( hipMemcpyAsync( result->sleepTimes.data(), impl->sleepTimes.ptr, nb * sizeof( float ), hipMemcpyDeviceToHost, s ) );
-	}
-	HIP_CHECK( hipStreamSynchronize( s ) );
-	std::sort( result->enlargedBodies.begin(), result->enlargedBodies.end() );
 
 	HIP_CHECK( hipEventElapsedTime( &lastSolveMs, impl->ev[2], impl->ev[3] ) );
 }
@@ -1652,6 +2030,11 @@ __global__ void CollideBatchKernel( const ShapeDev* shapesA, const ShapeDev* sha
 		return;
 	}
 	out[i] = ComputeManifold( shapesA[i], shapesB[i], xfs[i] );
+}
+
+void DeviceWorld::SetSolverPath( int path )
+{
+	impl->solverPath = path;
 }
 
 void DeviceSynchronize()

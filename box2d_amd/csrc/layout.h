@@ -217,7 +217,9 @@ struct JointSim
 	int enableSpring;
 	int enableMotor;
 	int enableLimit;
-	int pad;
+	int slotA; // LDS slots of the two bodies in the fused solver
+	int slotB;
+	int pad[3];
 };
 
 // Contact constraint fields, one device array per field (structure of arrays).

@@ -18,11 +18,18 @@ namespace b2g
 struct ConstraintSoA
 {
 	float* f[kcFieldCount];
-	int* bodyA;
+	int* bodyA; // body ids (GlobalBodies view)
 	int* bodyB;
 	int* contactId;
 	int* pointCount;
+	int* slotA; // LDS slots of the fused solver (LdsBodies view)
+	int* slotB;
 };
+
+// LDS body slots of the fused solver: awake bodies occupy [0, count); every static body reads
+// the identity state kept in slot kFusedBodyCapacity.
+constexpr int kFusedBodyCapacity = 5400;
+constexpr int kStaticSlot = kFusedBodyCapacity;
 
 B2G_DEVI float MaxW( float a, float b )
 {
@@ -88,43 +95,108 @@ struct LaneBody
 	float dpx, dpy, dqc, dqs;
 };
 
-B2G_DEVI LaneBody LoadBody( const BodyState* states, int index )
+// Body state access of the solver functions. GlobalBodies reads and writes the id-indexed state
+// array in device memory; LdsBodies reads and writes a world's awake bodies staged in LDS by
+// solver-set index (the fused solver kernel). Constraint and joint body indices must match the
+// view: body ids for GlobalBodies, LDS slots for LdsBodies.
+struct GlobalBodies
 {
-	const float4* p = reinterpret_cast<const float4*>( states + index );
-	float4 a = p[0];
-	float4 b = p[1];
-	LaneBody r;
-	r.vx = a.x;
-	r.vy = a.y;
-	r.w = a.z;
-	r.flags = __float_as_uint( a.w );
-	r.dpx = b.x;
-	r.dpy = b.y;
-	r.dqc = b.z;
-	r.dqs = b.w;
-	return r;
-}
+	BodyState* states;
 
-B2G_DEVI void StoreVelocity( BodyState* states, int index, const LaneBody& b )
-{
-	if ( b.flags & kDynamicFlag )
+	B2G_DEVI LaneBody Load( int index ) const
 	{
-		BodyState* s = states + index;
-		s->linearVelocity.x = b.vx;
-		s->linearVelocity.y = b.vy;
-		s->angularVelocity = b.w;
+		const float4* p = reinterpret_cast<const float4*>( states + index );
+		float4 a = p[0];
+		float4 b = p[1];
+		LaneBody r;
+		r.vx = a.x;
+		r.vy = a.y;
+		r.w = a.z;
+		r.flags = __float_as_uint( a.w );
+		r.dpx = b.x;
+		r.dpy = b.y;
+		r.dqc = b.z;
+		r.dqs = b.w;
+		return r;
 	}
-}
+
+	// Velocities are written back for dynamic bodies only (kinematic and static states are read-only).
+	B2G_DEVI void Store( int index, const LaneBody& b ) const
+	{
+		if ( b.flags & kDynamicFlag )
+		{
+			BodyState* s = states + index;
+			s->linearVelocity.x = b.vx;
+			s->linearVelocity.y = b.vy;
+			s->angularVelocity = b.w;
+		}
+	}
+
+	B2G_DEVI static int JointA( const JointSim* j )
+	{
+		return j->indexA;
+	}
+	B2G_DEVI static int JointB( const JointSim* j )
+	{
+		return j->indexB;
+	}
+};
+
+struct LdsBodies
+{
+	float* vx;
+	float* vy;
+	float* w;
+	float* dpx;
+	float* dpy;
+	float* dqc;
+	float* dqs;
+	uint16_t* flags;
+
+	B2G_DEVI LaneBody Load( int index ) const
+	{
+		LaneBody r;
+		r.vx = vx[index];
+		r.vy = vy[index];
+		r.w = w[index];
+		r.flags = flags[index];
+		r.dpx = dpx[index];
+		r.dpy = dpy[index];
+		r.dqc = dqc[index];
+		r.dqs = dqs[index];
+		return r;
+	}
+
+	B2G_DEVI void Store( int index, const LaneBody& b ) const
+	{
+		if ( b.flags & kDynamicFlag )
+		{
+			vx[index] = b.vx;
+			vy[index] = b.vy;
+			w[index] = b.w;
+		}
+	}
+
+	B2G_DEVI static int JointA( const JointSim* j )
+	{
+		return j->slotA;
+	}
+	B2G_DEVI static int JointB( const JointSim* j )
+	{
+		return j->slotB;
+	}
+};
 
 // ---------------------------------------------------------------------------------------------
 // Colored (wide-lane) contact arithmetic
 
-B2G_DEVI void WarmStartContact( const ConstraintSoA& c, int i, BodyState* states )
+template <typename Bodies>
+B2G_DEVI void WarmStartContact( const ConstraintSoA& c, int i, const Bodies& B )
 {
 	int ia = c.bodyA[i];
 	int ib = c.bodyB[i];
-	LaneBody bA = LoadBody( states, ia );
-	LaneBody bB = LoadBody( states, ib );
+	LaneBody bA = B.Load( ia );
+	LaneBody bB = B.Load( ib );
 
 	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
 	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
@@ -166,8 +238,8 @@ B2G_DEVI void WarmStartContact( const ConstraintSoA& c, int i, BodyState* states
 	bA.w = bA.w - iA * rolling;
 	bB.w = bB.w + iB * rolling;
 
-	StoreVelocity( states, ia, bA );
-	StoreVelocity( states, ib, bB );
+	B.Store( ia, bA );
+	B.Store( ib, bB );
 }
 
 // One normal point of the wide solve. Returns the new accumulated impulse for the rolling term.
@@ -250,13 +322,14 @@ B2G_DEVI void FrictionPoint( const ConstraintSoA& c, int i, LaneBody& bA, LaneBo
 
 // laneGroupRolling: true when any constraint of this constraint's 4-lane group has rolling
 // resistance (the wide solver tests the whole group).
-B2G_DEVI void SolveContact( const ConstraintSoA& c, int i, BodyState* states, bool useBias, float inv_h, float contactSpeed,
+template <typename Bodies>
+B2G_DEVI void SolveContact( const ConstraintSoA& c, int i, const Bodies& B, bool useBias, float inv_h, float contactSpeed,
 							bool laneGroupRolling )
 {
 	int ia = c.bodyA[i];
 	int ib = c.bodyB[i];
-	LaneBody bA = LoadBody( states, ia );
-	LaneBody bB = LoadBody( states, ib );
+	LaneBody bA = B.Load( ia );
+	LaneBody bB = B.Load( ib );
 
 	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
 	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
@@ -313,8 +386,8 @@ B2G_DEVI void SolveContact( const ConstraintSoA& c, int i, BodyState* states, bo
 					   kcAnchorB2Y, kcNormalImpulse2, kcTangentImpulse2, kcTangentMass2 );
 	}
 
-	StoreVelocity( states, ia, bA );
-	StoreVelocity( states, ib, bB );
+	B.Store( ia, bA );
+	B.Store( ib, bB );
 }
 
 B2G_DEVI void RestitutionPoint( const ConstraintSoA& c, int i, LaneBody& bA, LaneBody& bB, float mA, float mB, float iA,
@@ -352,12 +425,13 @@ B2G_DEVI void RestitutionPoint( const ConstraintSoA& c, int i, LaneBody& bA, Lan
 	bB.w = bB.w + iB * ( rBx * Py - rBy * Px );
 }
 
-B2G_DEVI void RestitutionContact( const ConstraintSoA& c, int i, BodyState* states, float threshold )
+template <typename Bodies>
+B2G_DEVI void RestitutionContact( const ConstraintSoA& c, int i, const Bodies& B, float threshold )
 {
 	int ia = c.bodyA[i];
 	int ib = c.bodyB[i];
-	LaneBody bA = LoadBody( states, ia );
-	LaneBody bB = LoadBody( states, ib );
+	LaneBody bA = B.Load( ia );
+	LaneBody bB = B.Load( ib );
 
 	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
 	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
@@ -370,24 +444,52 @@ B2G_DEVI void RestitutionContact( const ConstraintSoA& c, int i, BodyState* stat
 	RestitutionPoint( c, i, bA, bB, mA, mB, iA, iB, nX, nY, threshold, restitution, restitutionMask, kcAnchorA2X, kcAnchorA2Y,
 					  kcAnchorB2X, kcAnchorB2Y, kcRelativeVelocity2, kcTotalNormalImpulse2, kcNormalMass2, kcNormalImpulse2 );
 
-	StoreVelocity( states, ia, bA );
-	StoreVelocity( states, ib, bB );
+	B.Store( ia, bA );
+	B.Store( ib, bB );
 }
 
 // ---------------------------------------------------------------------------------------------
 // Overflow (scalar) contact arithmetic, solved one constraint after another by a single lane.
+
+B2G_DEVI BodyState ToState( const LaneBody& b )
+{
+	BodyState s;
+	s.linearVelocity = { b.vx, b.vy };
+	s.angularVelocity = b.w;
+	s.flags = b.flags;
+	s.deltaPosition = { b.dpx, b.dpy };
+	s.deltaRotation = { b.dqc, b.dqs };
+	return s;
+}
+
+B2G_DEVI LaneBody ToLane( const BodyState& s )
+{
+	LaneBody b;
+	b.vx = s.linearVelocity.x;
+	b.vy = s.linearVelocity.y;
+	b.w = s.angularVelocity;
+	b.flags = s.flags;
+	b.dpx = s.deltaPosition.x;
+	b.dpy = s.deltaPosition.y;
+	b.dqc = s.deltaRotation.c;
+	b.dqs = s.deltaRotation.s;
+	return b;
+}
 
 struct OverflowVel
 {
 	float vx, vy, w;
 };
 
-B2G_DEVI void OverflowWarmStart( const ConstraintSoA& c, int i, BodyState* states )
+template <typename Bodies>
+B2G_DEVI void OverflowWarmStart( const ConstraintSoA& c, int i, const Bodies& B )
 {
 	int ia = c.bodyA[i];
 	int ib = c.bodyB[i];
-	BodyState* stateA = states + ia;
-	BodyState* stateB = states + ib;
+	BodyState sA = ToState( B.Load( ia ) );
+	BodyState* stateA = &sA;
+	BodyState sB = ToState( B.Load( ib ) );
+	BodyState* stateB = &sB;
 
 	b2Vec2 vA = stateA->linearVelocity;
 	float wA = stateA->angularVelocity;
@@ -429,17 +531,23 @@ B2G_DEVI void OverflowWarmStart( const ConstraintSoA& c, int i, BodyState* state
 		stateB->linearVelocity = vB;
 		stateB->angularVelocity = wB;
 	}
+
+	B.Store( ia, ToLane( sA ) );
+	B.Store( ib, ToLane( sB ) );
 }
 
-B2G_DEVI void OverflowSolve( const ConstraintSoA& c, int i, BodyState* states, bool useBias, float inv_h, float contactSpeed )
+template <typename Bodies>
+B2G_DEVI void OverflowSolve( const ConstraintSoA& c, int i, const Bodies& B, bool useBias, float inv_h, float contactSpeed )
 {
 	float mA = c.f[kcInvMassA][i], mB = c.f[kcInvMassB][i];
 	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
 
 	int ia = c.bodyA[i];
 	int ib = c.bodyB[i];
-	BodyState* stateA = states + ia;
-	BodyState* stateB = states + ib;
+	BodyState sA = ToState( B.Load( ia ) );
+	BodyState* stateA = &sA;
+	BodyState sB = ToState( B.Load( ib ) );
+	BodyState* stateB = &sB;
 
 	b2Vec2 vA = stateA->linearVelocity;
 	float wA = stateA->angularVelocity;
@@ -547,9 +655,13 @@ B2G_DEVI void OverflowSolve( const ConstraintSoA& c, int i, BodyState* states, b
 		stateB->linearVelocity = vB;
 		stateB->angularVelocity = wB;
 	}
+
+	B.Store( ia, ToLane( sA ) );
+	B.Store( ib, ToLane( sB ) );
 }
 
-B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, BodyState* states, float threshold )
+template <typename Bodies>
+B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, const Bodies& B, float threshold )
 {
 	float restitution = c.f[kcRestitution][i];
 	if ( restitution == 0.0f )
@@ -561,8 +673,10 @@ B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, BodyState* sta
 	float iA = c.f[kcInvIA][i], iB = c.f[kcInvIB][i];
 	int ia = c.bodyA[i];
 	int ib = c.bodyB[i];
-	BodyState* stateA = states + ia;
-	BodyState* stateB = states + ib;
+	BodyState sA = ToState( B.Load( ia ) );
+	BodyState* stateA = &sA;
+	BodyState sB = ToState( B.Load( ib ) );
+	BodyState* stateB = &sB;
 	b2Vec2 vA = stateA->linearVelocity;
 	float wA = stateA->angularVelocity;
 	b2Vec2 vB = stateB->linearVelocity;
@@ -609,20 +723,28 @@ B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, BodyState* sta
 		stateB->linearVelocity = vB;
 		stateB->angularVelocity = wB;
 	}
+
+	B.Store( ia, ToLane( sA ) );
+	B.Store( ib, ToLane( sB ) );
 }
 
 // ---------------------------------------------------------------------------------------------
 // Revolute joint
 
-B2G_DEVI void WarmStartRevolute( JointSim* joint, BodyState* states )
+template <typename Bodies>
+B2G_DEVI void WarmStartRevolute( JointSim* joint, const Bodies& B )
 {
 	float mA = joint->invMassA;
 	float mB = joint->invMassB;
 	float iA = joint->invIA;
 	float iB = joint->invIB;
+	int ia = Bodies::JointA( joint );
+	int ib = Bodies::JointB( joint );
 
-	BodyState* stateA = states + joint->indexA;
-	BodyState* stateB = states + joint->indexB;
+	BodyState sA = ToState( B.Load( ia ) );
+	BodyState* stateA = &sA;
+	BodyState sB = ToState( B.Load( ib ) );
+	BodyState* stateB = &sB;
 
 	b2Vec2 rA = b2RotateVector( stateA->deltaRotation, joint->frameA.p );
 	b2Vec2 rB = b2RotateVector( stateB->deltaRotation, joint->frameB.p );
@@ -639,17 +761,25 @@ B2G_DEVI void WarmStartRevolute( JointSim* joint, BodyState* states )
 		stateB->linearVelocity = b2MulAdd( stateB->linearVelocity, mB, joint->linearImpulse );
 		stateB->angularVelocity += iB * ( b2Cross( rB, joint->linearImpulse ) + axialImpulse );
 	}
+
+	B.Store( ia, ToLane( sA ) );
+	B.Store( ib, ToLane( sB ) );
 }
 
-B2G_DEVI void SolveRevolute( JointSim* joint, BodyState* states, bool useBias, float h, float inv_h )
+template <typename Bodies>
+B2G_DEVI void SolveRevolute( JointSim* joint, const Bodies& B, bool useBias, float h, float inv_h )
 {
 	float mA = joint->invMassA;
 	float mB = joint->invMassB;
 	float iA = joint->invIA;
 	float iB = joint->invIB;
+	int ia = Bodies::JointA( joint );
+	int ib = Bodies::JointB( joint );
 
-	BodyState* stateA = states + joint->indexA;
-	BodyState* stateB = states + joint->indexB;
+	BodyState sA = ToState( B.Load( ia ) );
+	BodyState* stateA = &sA;
+	BodyState sB = ToState( B.Load( ib ) );
+	BodyState* stateB = &sB;
 
 	b2Vec2 vA = stateA->linearVelocity;
 	float wA = stateA->angularVelocity;
@@ -790,6 +920,9 @@ B2G_DEVI void SolveRevolute( JointSim* joint, BodyState* states, bool useBias, f
 		stateB->linearVelocity = vB;
 		stateB->angularVelocity = wB;
 	}
+
+	B.Store( ia, ToLane( sA ) );
+	B.Store( ib, ToLane( sB ) );
 }
 
 } // namespace b2g

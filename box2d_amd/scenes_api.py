@@ -8,6 +8,10 @@ import ctypes
 
 import numpy as np
 
+class WorldId(ctypes.Structure):
+    _fields_ = [("index1", ctypes.c_uint16), ("generation", ctypes.c_uint16)]
+
+
 SCENE_PYRAMID = 0
 SCENE_FALLING_HINGES = 1
 SCENE_HELLO = 2
@@ -32,6 +36,8 @@ class SceneLib:
         lib.b2s_counters.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         lib.b2s_falling_hinges.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                            ctypes.POINTER(ctypes.c_uint32)]
+        lib.b2s_world_id.restype = ctypes.c_uint32
+        lib.b2s_world_id.argtypes = [ctypes.c_int]
         self.lib = lib
 
     def create(self, kind, iparams=(), fparams=(), workers=1):
@@ -44,6 +50,11 @@ class SceneLib:
 
     def pyramid(self, rows, ground_half_width=40.0, sleep=False, workers=1):
         return self.create(SCENE_PYRAMID, (rows, 1 if sleep else 0), (ground_half_width,), workers)
+
+    def world_id(self, h):
+        """The scene's b2WorldId as a ctypes struct (index1, generation)."""
+        packed = self.lib.b2s_world_id(h)
+        return WorldId(packed & 0xFFFF, packed >> 16)
 
     def destroy(self, h):
         self.lib.b2s_destroy(h)

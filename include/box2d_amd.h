@@ -33,6 +33,20 @@ B2_API int b2amd_World_GetKernelStats( b2WorldId worldId, b2amd_KernelStats* sta
 /// Clear the accumulated kernel statistics.
 B2_API void b2amd_World_ResetKernelStats( b2WorldId worldId );
 
+/// Solver path of a world's step. The fused path runs the whole substep loop in one workgroup
+/// with the awake body states in LDS (up to 5400 awake bodies); the color-kernel path launches one
+/// kernel per graph color and stage. Auto picks the fused path for small worlds (up to 4096
+/// contact constraints). Results are bit-identical across paths; the switch exists for testing
+/// and measurement. Fused falls back to color kernels for worlds beyond its LDS budget.
+typedef enum b2amd_SolverPath
+{
+	b2amd_solverAuto = 0,
+	b2amd_solverColorKernels = 1,
+	b2amd_solverFused = 2,
+} b2amd_SolverPath;
+
+B2_API void b2amd_World_SetSolverPath( b2WorldId worldId, b2amd_SolverPath path );
+
 /// Block until all GPU work issued by this library on the current device has completed
 /// (a device-wide synchronize; b2World_Step already returns with its own work complete).
 B2_API void b2amd_Synchronize( void );

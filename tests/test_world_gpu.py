@@ -21,8 +21,11 @@ def scene(world_runs, name):
     return next(s for s in world_runs["scenes"] if s["name"] == name)
 
 
-def run_and_compare(gpu_scenes, fx):
+def run_and_compare(gpu_scenes, fx, path=0):
+    import box2d_amd
+
     h = gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"])
+    box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
     try:
         assert gpu_scenes.body_count(h) == fx["body_count"]
         assert gpu_scenes.hash(h) == fx["hash_initial"]
@@ -45,8 +48,23 @@ def run_and_compare(gpu_scenes, fx):
 
 @pytest.mark.parametrize("name", ["hello", "pyramid20", "pyramid100", "pyramid20_sleep"])
 def test_scene_matches_reference(gpu_scenes, world_runs, name):
-    print(f"[gpu] scene {name}", flush=True)
+    """Default path: the fused single-workgroup solver for worlds that fit its LDS budget."""
+    print(f"[gpu] scene {name} (auto path)", flush=True)
     run_and_compare(gpu_scenes, scene(world_runs, name))
+
+
+@pytest.mark.parametrize("name", ["pyramid100"])
+def test_scene_matches_reference_fused(gpu_scenes, world_runs, name):
+    """Fused single-workgroup solver forced on a world above the auto threshold."""
+    print(f"[gpu] scene {name} (fused)", flush=True)
+    run_and_compare(gpu_scenes, scene(world_runs, name), path=2)
+
+
+@pytest.mark.parametrize("name", ["hello", "pyramid20", "pyramid100", "pyramid20_sleep"])
+def test_scene_matches_reference_color_kernels(gpu_scenes, world_runs, name):
+    """Per-color kernel path (the path of worlds too large for the fused solver)."""
+    print(f"[gpu] scene {name} (color kernels)", flush=True)
+    run_and_compare(gpu_scenes, scene(world_runs, name), path=1)
 
 
 @pytest.mark.parametrize("name", sorted(NEEDS_CONTINUOUS))
