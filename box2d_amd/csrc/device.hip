@@ -17,6 +17,7 @@
 #include "device.h"
 #include "narrowphase.h"
 #include "solver_device.h"
+#include "toi.h"
 #include "world.h"
 
 #include <algorithm>
@@ -1015,6 +1016,7 @@ struct FinalizeArgs
 	ShapeDev* shapes;
 	StepParams p;
 	int* counters; // [0] enlarged bodies, [1] enlarged shapes, [2] fast bodies
+	int* fastBodies;
 	int* enlargedBodies;
 	int* enlargedShapes;
 	b2AABB* enlargedShapeAABBs;
@@ -1069,7 +1071,8 @@ __global__ void FinalizeBodiesKernel( FinalizeArgs a )
 		if ( aux->type == b2_dynamicBody && a.p.enableContinuous && maxMotion > aux->safetyFactor * sim->minExtent )
 		{
 			bodyFlags |= kIsFast;
-			atomicAdd( a.counters + 2, 1 );
+			int slot = atomicAdd( a.counters + 2, 1 );
+			a.fastBodies[slot] = simIndex;
 		}
 		else
 		{
@@ -1135,6 +1138,302 @@ __global__ void FinalizeBodiesKernel( FinalizeArgs a )
 }
 
 // ---------------------------------------------------------------------------------------------
+// Continuous collision of fast bodies against static shapes (reference src/solver.c:181-550).
+//
+// One lane per fast non-bullet body. The reference queries the static tree with the swept box and
+// keeps the smallest time of impact; the minimum does not depend on the visit order, so the lane
+// tests the static shapes of the world in a flat loop against the same fat-AABB overlap. All
+// geometry runs in a frame centered on the body's start position (the reference's base frame).
+
+struct ContinuousArgs
+{
+	const int* fastBodies;
+	const int* fastCount;
+	const int* awake;
+	BodySim* sims;
+	ShapeDev* shapes;
+	const int* staticShapes;
+	int staticShapeCount;
+	const int2* noCollide; // (body, static body) pairs joined with collideConnected off
+	int noCollideCount;
+	int* enlargedShapeCount;
+	int* enlargedShapes;
+	b2AABB* enlargedShapeAABBs;
+	float huge;
+};
+
+B2G_DEV bool ShouldShapesCollide( const b2Filter& a, const b2Filter& b )
+{
+	if ( a.groupIndex == b.groupIndex && a.groupIndex != 0 )
+	{
+		return a.groupIndex > 0;
+	}
+	return ( a.maskBits & b.categoryBits ) != 0 && ( a.categoryBits & b.maskBits ) != 0;
+}
+
+// Distance proxy of a shape in its body frame (reference src/shape.c:1055-1076).
+B2G_DEV b2ShapeProxy ShapeProxy( const ShapeDev& s )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			return MakeProxy( &s.capsule.center1, 2, s.capsule.radius );
+		case b2_circleShape:
+			return MakeProxy( &s.circle.center, 1, s.circle.radius );
+		case b2_polygonShape:
+			return MakeProxy( s.polygon.vertices, s.polygon.count, s.polygon.radius );
+		case b2_segmentShape:
+			return MakeProxy( &s.segment.point1, 2, 0.0f );
+		default:
+			return MakeProxy( &s.chainSegment.segment.point1, 2, 0.0f );
+	}
+}
+
+// Local centroid (reference src/shape.c:705-722) and smallest extent about it (src/shape.c:826-880).
+B2G_DEV b2Vec2 ShapeCentroidDev( const ShapeDev& s )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			return b2Lerp( s.capsule.center1, s.capsule.center2, 0.5f );
+		case b2_circleShape:
+			return s.circle.center;
+		case b2_polygonShape:
+			return s.polygon.centroid;
+		case b2_segmentShape:
+			return b2Lerp( s.segment.point1, s.segment.point2, 0.5f );
+		default:
+			return b2Lerp( s.chainSegment.segment.point1, s.chainSegment.segment.point2, 0.5f );
+	}
+}
+
+B2G_DEV float ShapeMinExtent( const ShapeDev& s, float huge )
+{
+	switch ( s.type )
+	{
+		case b2_capsuleShape:
+			return s.capsule.radius;
+		case b2_circleShape:
+			return s.circle.radius;
+		case b2_polygonShape:
+		{
+			float minExtent = huge;
+			for ( int i = 0; i < s.polygon.count; ++i )
+			{
+				b2Vec2 v = s.polygon.vertices[i];
+				float planeOffset = b2Dot( s.polygon.normals[i], b2Sub( v, s.polygon.centroid ) );
+				minExtent = b2MinFloat( minExtent, planeOffset );
+			}
+			return minExtent + s.polygon.radius;
+		}
+		default:
+			return 0.0f;
+	}
+}
+
+// AABB translated from the base frame to world: double sums narrowed once (reference src/aabb.h:55-63).
+B2G_DEV b2AABB OffsetAABB( b2AABB box, b2Vec2 origin )
+{
+	b2AABB r;
+	r.lowerBound.x = (float)( origin.x + (double)box.lowerBound.x );
+	r.lowerBound.y = (float)( origin.y + (double)box.lowerBound.y );
+	r.upperBound.x = (float)( origin.x + (double)box.upperBound.x );
+	r.upperBound.y = (float)( origin.y + (double)box.upperBound.y );
+	return r;
+}
+
+B2G_DEV b2Sweep RelativeSweep( const BodySim& sim, b2Vec2 base )
+{
+	b2Sweep sw;
+	sw.c1 = b2SubPos( sim.center0, base );
+	sw.c2 = b2SubPos( sim.center, base );
+	sw.q1 = sim.rotation0;
+	sw.q2 = sim.transform.q;
+	sw.localCenter = sim.localCenter;
+	return sw;
+}
+
+constexpr float kCoreFraction = 0.25f;
+
+B2G_DEV void EnlargeShape( const ContinuousArgs& a, int shapeId, ShapeDev* shape )
+{
+	if ( b2AABB_Contains( shape->fatAABB, shape->aabb ) == false )
+	{
+		float margin = shape->aabbMargin;
+		b2AABB fat;
+		fat.lowerBound.x = shape->aabb.lowerBound.x - margin;
+		fat.lowerBound.y = shape->aabb.lowerBound.y - margin;
+		fat.upperBound.x = shape->aabb.upperBound.x + margin;
+		fat.upperBound.y = shape->aabb.upperBound.y + margin;
+		shape->fatAABB = fat;
+		shape->enlargedAABB = 1;
+		int slot = atomicAdd( a.enlargedShapeCount, 1 );
+		a.enlargedShapes[slot] = shapeId;
+		a.enlargedShapeAABBs[slot] = fat;
+	}
+}
+
+__global__ void ContinuousKernel( ContinuousArgs a )
+{
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( k >= *a.fastCount )
+	{
+		return;
+	}
+
+	int bodyId = a.awake[a.fastBodies[k]];
+	BodySim* fastSim = a.sims + bodyId;
+
+	b2Vec2 base = fastSim->center0;
+	b2Sweep sweep = RelativeSweep( *fastSim, base );
+
+	b2Transform xf1;
+	xf1.q = sweep.q1;
+	xf1.p = b2Sub( sweep.c1, b2RotateVector( sweep.q1, sweep.localCenter ) );
+	b2Transform xf2;
+	xf2.q = sweep.q2;
+	xf2.p = b2Sub( sweep.c2, b2RotateVector( sweep.q2, sweep.localCenter ) );
+
+	float fraction = 1.0f;
+	bool hadImpact = false;
+
+	for ( int shapeId = fastSim->headShapeId; shapeId != B2_NULL_INDEX; shapeId = a.shapes[shapeId].nextShapeId )
+	{
+		ShapeDev* fastShape = a.shapes + shapeId;
+		fastShape->enlargedAABB = 0;
+		b2Vec2 centroid1 = b2TransformPoint( xf1, fastShape->localCentroid );
+		b2Vec2 centroid2 = b2TransformPoint( xf2, fastShape->localCentroid );
+
+		b2AABB box1 = fastShape->aabb;
+		b2AABB box2 = OffsetAABB( FatShapeAABB( *fastShape, xf2, 0.0f ), base );
+		fastShape->aabb = box2;
+		b2AABB swept = b2AABB_Union( box1, box2 );
+
+		for ( int j = 0; j < a.staticShapeCount; ++j )
+		{
+			int otherId = a.staticShapes[j];
+			const ShapeDev& other = a.shapes[otherId];
+			if ( other.filter.categoryBits == 0 || b2AABB_Overlaps( other.fatAABB, swept ) == false )
+			{
+				continue;
+			}
+			if ( otherId == shapeId || other.bodyId == fastShape->bodyId )
+			{
+				continue;
+			}
+			if ( ShouldShapesCollide( fastShape->filter, other.filter ) == false )
+			{
+				continue;
+			}
+			bool jointed = false;
+			for ( int n = 0; n < a.noCollideCount; ++n )
+			{
+				if ( a.noCollide[n].x == bodyId && a.noCollide[n].y == other.bodyId )
+				{
+					jointed = true;
+				}
+			}
+			if ( jointed )
+			{
+				continue;
+			}
+
+			const BodySim& otherSim = a.sims[other.bodyId];
+
+			if ( other.type == b2_chainSegmentShape )
+			{
+				// skip fast parallel motion over a one-sided chain (reference src/solver.c:265-290)
+				b2Transform rel = { b2SubPos( otherSim.transform.p, base ), otherSim.transform.q };
+				b2Vec2 p1 = b2TransformPoint( rel, other.chainSegment.segment.point1 );
+				b2Vec2 p2 = b2TransformPoint( rel, other.chainSegment.segment.point2 );
+				float length;
+				b2Vec2 e = b2GetLengthAndNormalize( &length, b2Sub( p2, p1 ) );
+				if ( length > kLinearSlop )
+				{
+					float separation1 = b2Cross( b2Sub( centroid1, p1 ), e );
+					float separation2 = b2Cross( b2Sub( centroid2, p1 ), e );
+					float coreDistance = kCoreFraction * fastSim->minExtent;
+					if ( separation1 < 0.0f || ( separation1 - separation2 < coreDistance && separation2 > coreDistance ) )
+					{
+						continue;
+					}
+				}
+			}
+
+			b2TOIInput input;
+			input.proxyA = ShapeProxy( other );
+			input.proxyB = ShapeProxy( *fastShape );
+			input.sweepA = RelativeSweep( otherSim, base );
+			input.sweepB = sweep;
+			input.maxFraction = fraction;
+
+			b2TOIOutput output = TimeOfImpact( input );
+			float hitFraction = fraction;
+			bool didHit = false;
+			if ( 0.0f < output.fraction && output.fraction < fraction )
+			{
+				hitFraction = output.fraction;
+				didHit = true;
+			}
+			else if ( 0.0f == output.fraction )
+			{
+				// fall back to a small circle at the fast shape's centroid
+				b2Vec2 centroid = ShapeCentroidDev( *fastShape );
+				float radius = kCoreFraction * ShapeMinExtent( *fastShape, a.huge );
+				input.proxyB = MakeProxy( &centroid, 1, radius );
+				output = TimeOfImpact( input );
+				if ( 0.0f < output.fraction && output.fraction < fraction )
+				{
+					hitFraction = output.fraction;
+					didHit = true;
+				}
+			}
+
+			if ( didHit )
+			{
+				hadImpact = true;
+				fraction = hitFraction;
+			}
+		}
+	}
+
+	if ( hadImpact )
+	{
+		fastSim->flags |= kHadTimeOfImpact;
+	}
+
+	if ( fraction < 1.0f )
+	{
+		b2Rot q = b2NLerp( sweep.q1, sweep.q2, fraction );
+		b2Vec2 c = b2Lerp( sweep.c1, sweep.c2, fraction );
+		b2Vec2 origin = b2Sub( c, b2RotateVector( q, sweep.localCenter ) );
+
+		fastSim->transform.q = q;
+		fastSim->transform.p = b2OffsetPos( base, origin );
+		fastSim->center = b2OffsetPos( base, c );
+		fastSim->rotation0 = q;
+		fastSim->center0 = fastSim->center;
+
+		b2Transform transform = fastSim->transform;
+		for ( int shapeId = fastSim->headShapeId; shapeId != B2_NULL_INDEX; shapeId = a.shapes[shapeId].nextShapeId )
+		{
+			ShapeDev* shape = a.shapes + shapeId;
+			shape->aabb = FatShapeAABB( *shape, transform, kSpeculativeDistance );
+			EnlargeShape( a, shapeId, shape );
+		}
+	}
+	else
+	{
+		fastSim->rotation0 = fastSim->transform.q;
+		fastSim->center0 = fastSim->center;
+		for ( int shapeId = fastSim->headShapeId; shapeId != B2_NULL_INDEX; shapeId = a.shapes[shapeId].nextShapeId )
+		{
+			EnlargeShape( a, shapeId, a.shapes + shapeId );
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
 // Kernel classes for timing, with the algorithmic bytes each unit of work must move: every field
 // a lane reads or writes counted once, body states gathered and scattered per constraint
 // (per-unit figures of the step's data layout; see the roofline notes in the design document).
@@ -1154,12 +1453,13 @@ enum KernelClass
 	kkStore,
 	kkFinalize,
 	kkSolveFused,
+	kkContinuous,
 	kKernelClassCount
 };
 
 static const char* s_kernelClassNames[kKernelClassCount] = {
 	"collide", "prepare_contacts", "prepare_joints", "integrate_velocities", "warm_start", "solve",
-	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies", "solve_fused",
+	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies", "solve_fused", "continuous",
 };
 
 constexpr double kBytesCollideFull = 800.0;		// per contact with a new manifold
@@ -1343,6 +1643,11 @@ struct DeviceWorld::Impl
 
 	// finalize outputs
 	DevArray<int> enlargedBodies;
+	DevArray<int> fastBodies;
+	DevArray<int> ccdStaticShapes;
+	DevArray<int2> ccdNoCollide;
+	std::vector<int> ccdStaticStaging;
+	std::vector<int2> ccdNoCollideStaging;
 	DevArray<int> enlargedShapes;
 	DevArray<b2AABB> enlargedShapeAABBs;
 	DevArray<float> sleepTimes;
@@ -1500,6 +1805,9 @@ DeviceWorld::~DeviceWorld()
 	impl->cfields.Free();
 	impl->cints.Free();
 	impl->enlargedBodies.Free();
+	impl->fastBodies.Free();
+	impl->ccdStaticShapes.Free();
+	impl->ccdNoCollide.Free();
 	impl->enlargedShapes.Free();
 	impl->enlargedShapeAABBs.Free();
 	impl->sleepTimes.Free();
@@ -1515,6 +1823,16 @@ DeviceWorld::~DeviceWorld()
 	}
 	(void)hipStreamDestroy( impl->stream );
 	delete impl;
+}
+
+template <typename T>
+static void UploadVector( DevArray<T>& dst, const std::vector<T>& src, hipStream_t stream )
+{
+	dst.Reserve( std::max<size_t>( src.size(), 1 ), false, stream, 0 );
+	if ( src.empty() == false )
+	{
+		HIP_CHECK( hipMemcpyAsync( dst.ptr, src.data(), src.size() * sizeof( T ), hipMemcpyHostToDevice, stream ) );
+	}
 }
 
 void DeviceWorld::UploadBodies( const World& world )
@@ -1560,6 +1878,17 @@ void DeviceWorld::UploadShapes( const World& world )
 	impl->shapeCount = n;
 	impl->enlargedShapes.Reserve( std::max<size_t>( n, 1 ), false, impl->stream, 0 );
 	impl->enlargedShapeAABBs.Reserve( std::max<size_t>( n, 1 ), false, impl->stream, 0 );
+
+	// static shapes in id order: the candidates of continuous collision
+	impl->ccdStaticStaging.clear();
+	for ( const Shape& shape : world.shapes )
+	{
+		if ( shape.id != B2_NULL_INDEX && world.bodies[shape.bodyId].type == b2_staticBody )
+		{
+			impl->ccdStaticStaging.push_back( shape.id );
+		}
+	}
+	UploadVector( impl->ccdStaticShapes, impl->ccdStaticStaging, impl->stream );
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
@@ -1585,6 +1914,25 @@ void DeviceWorld::UploadJoints( const World& world )
 								   impl->stream ) );
 	}
 	impl->jointCount = n;
+
+	// joints that switch off collision between a moving body and a static body
+	impl->ccdNoCollideStaging.clear();
+	for ( const Joint& joint : world.joints )
+	{
+		if ( joint.jointId == B2_NULL_INDEX || joint.collideConnected )
+		{
+			continue;
+		}
+		int a = joint.edges[0].bodyId;
+		int b = joint.edges[1].bodyId;
+		bool staticA = world.bodies[a].type == b2_staticBody;
+		bool staticB = world.bodies[b].type == b2_staticBody;
+		if ( staticA != staticB )
+		{
+			impl->ccdNoCollideStaging.push_back( staticB ? int2{ a, b } : int2{ b, a } );
+		}
+	}
+	UploadVector( impl->ccdNoCollide, impl->ccdNoCollideStaging, impl->stream );
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
@@ -1625,15 +1973,6 @@ void DeviceWorld::DownloadContact( int contactId, ContactSim* out )
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
-template <typename T>
-static void UploadVector( DevArray<T>& dst, const std::vector<T>& src, hipStream_t stream )
-{
-	dst.Reserve( std::max<size_t>( src.size(), 1 ), false, stream, 0 );
-	if ( src.empty() == false )
-	{
-		HIP_CHECK( hipMemcpyAsync( dst.ptr, src.data(), src.size() * sizeof( T ), hipMemcpyHostToDevice, stream ) );
-	}
-}
 
 void DeviceWorld::UploadLists( const SolverLists& lists )
 {
@@ -1665,6 +2004,7 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 
 	size_t nb = lists.awakeBodies.size();
 	impl->enlargedBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
+	impl->fastBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->sleepTimes.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->changes.Reserve( std::max<size_t>( lists.collideContacts.size(), 1 ), false, impl->stream, 0 );
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
@@ -1935,11 +2275,32 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 	fa.shapes = impl->shapes.ptr;
 	fa.p = params;
 	fa.counters = impl->counters + 4;
+	fa.fastBodies = impl->fastBodies.ptr;
 	fa.enlargedBodies = impl->enlargedBodies.ptr;
 	fa.enlargedShapes = impl->enlargedShapes.ptr;
 	fa.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 	fa.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
 	impl->Launch( kkFinalize, kBytesFinalize * nb, [&]() { FinalizeBodiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( fa ); } );
+
+	if ( params.enableContinuous )
+	{
+		// The fast-body count is only known on the device: size the grid for every awake body.
+		ContinuousArgs ka;
+		ka.fastBodies = impl->fastBodies.ptr;
+		ka.fastCount = impl->counters + 6;
+		ka.awake = impl->awakeBodies.ptr;
+		ka.sims = impl->sims.ptr;
+		ka.shapes = impl->shapes.ptr;
+		ka.staticShapes = impl->ccdStaticShapes.ptr;
+		ka.staticShapeCount = (int)impl->ccdStaticStaging.size();
+		ka.noCollide = impl->ccdNoCollide.ptr;
+		ka.noCollideCount = (int)impl->ccdNoCollideStaging.size();
+		ka.enlargedShapeCount = impl->counters + 5;
+		ka.enlargedShapes = impl->enlargedShapes.ptr;
+		ka.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
+		ka.huge = B2_HUGE;
+		impl->Launch( kkContinuous, 0.0, [&]() { ContinuousKernel<<<Blocks( nb ), kBlock, 0, s>>>( ka ); } );
+	}
 	HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
 
 	{
@@ -2161,6 +2522,62 @@ B2_API b2LocalManifold b2CollidePolygons( const b2Polygon* polygonA, const b2Pol
 B2_API b2LocalManifold b2CollideSegmentAndPolygon( const b2Segment* segmentA, const b2Polygon* polygonB, b2Transform xf )
 {
 	return b2g::CollideSegmentAndPolygon( *segmentA, *polygonB, xf );
+}
+
+} // extern "C"
+
+// Host forms of the distance and time-of-impact routines (the code the continuous kernel runs).
+extern "C" {
+
+B2_API b2ShapeProxy b2MakeProxy( const b2Vec2* points, int count, float radius )
+{
+	return b2g::MakeProxy( points, count, radius );
+}
+
+B2_API b2ShapeProxy b2MakeOffsetProxy( const b2Vec2* points, int count, float radius, b2Vec2 position, b2Rot rotation )
+{
+	count = count < B2_MAX_POLYGON_VERTICES ? count : B2_MAX_POLYGON_VERTICES;
+	b2Transform transform = { position, rotation };
+	b2ShapeProxy proxy;
+	for ( int i = 0; i < count; ++i )
+	{
+		proxy.points[i] = b2TransformPoint( transform, points[i] );
+	}
+	proxy.count = count;
+	proxy.radius = radius;
+	return proxy;
+}
+
+B2_API b2DistanceOutput b2ShapeDistance( const b2DistanceInput* input, b2SimplexCache* cache, b2Simplex* simplexes,
+										 int simplexCapacity )
+{
+	int recorded = 0;
+	if ( simplexes != nullptr && simplexCapacity > 0 )
+	{
+		// the reference records the starting simplex (src/distance.c:444-449)
+		b2ShapeProxy localB;
+		localB.count = input->proxyB.count;
+		localB.radius = input->proxyB.radius;
+		for ( int i = 0; i < localB.count; ++i )
+		{
+			localB.points[i] = b2TransformPoint( input->transform, input->proxyB.points[i] );
+		}
+		simplexes[0] = b2g::SimplexFromCache( *cache, input->proxyA, localB );
+		recorded = 1;
+	}
+	b2DistanceOutput output = b2g::ShapeDistance( *input, cache );
+	output.simplexCount = recorded;
+	return output;
+}
+
+B2_API b2Transform b2GetSweepTransform( const b2Sweep* sweep, float time )
+{
+	return b2g::SweepTransform( *sweep, time );
+}
+
+B2_API b2TOIOutput b2TimeOfImpact( const b2TOIInput* input )
+{
+	return b2g::TimeOfImpact( *input );
 }
 
 } // extern "C"

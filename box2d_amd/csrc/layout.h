@@ -141,6 +141,7 @@ struct ShapeDev
 		b2Segment segment;
 		b2ChainSegment chainSegment;
 	};
+	b2Filter filter;
 };
 
 // Per contact (indexed by contact id). Mirrors b2ContactSim (reference src/contact.h:103-142)

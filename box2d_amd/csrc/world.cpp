@@ -1181,7 +1181,10 @@ b2BodyId World::CreateBody( const b2BodyDef* def )
 	sim.angularDamping = def->angularDamping;
 	sim.gravityScale = def->gravityScale;
 	sim.flags = lockFlags;
-	sim.flags |= def->isBullet ? kIsBullet : 0;
+	if ( def->isBullet )
+	{
+		Fatal( "bullet bodies (continuous collision against moving bodies) are not supported by the GPU step" );
+	}
 	sim.flags |= def->allowFastRotation ? kAllowFastRotation : 0;
 	sim.flags |= def->type == b2_dynamicBody ? kDynamicFlag : 0;
 	sim.flags |= def->enableSleep ? kEnableSleep : 0;
@@ -1474,6 +1477,7 @@ b2ShapeId World::CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void
 	sd.restitution = def->material.restitution;
 	sd.rollingResistance = def->material.rollingResistance;
 	sd.tangentSpeed = def->material.tangentSpeed;
+	sd.filter = def->filter;
 	sd.localCentroid = ShapeCentroid( sd );
 	sd.aabbMargin = ComputeShapeMargin( sd );
 	sd.enlargedAABB = 0;
@@ -2412,11 +2416,6 @@ void World::Solve( StepParams& params )
 	bodiesStale = true;
 	shapesStale = true;
 	jointsStale = true;
-
-	if ( fr.fastBodyCount > 0 )
-	{
-		Fatal( "continuous collision for fast bodies is not supported by the GPU step yet (%d fast bodies)", fr.fastBodyCount );
-	}
 
 	// Move events: one per awake body in awake order (transforms fetched lazily).
 	moveEventBodyIds = awakeSet.bodyIds;

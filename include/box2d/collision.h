@@ -166,3 +166,86 @@ B2_API b2LocalManifold b2CollideSegmentAndCapsule( const b2Segment* segmentA, co
 B2_API b2LocalManifold b2CollidePolygonAndCapsule( const b2Polygon* polygonA, const b2Capsule* capsuleB, b2Transform xf );
 B2_API b2LocalManifold b2CollidePolygons( const b2Polygon* polygonA, const b2Polygon* polygonB, b2Transform xf );
 B2_API b2LocalManifold b2CollideSegmentAndPolygon( const b2Segment* segmentA, const b2Polygon* polygonB, b2Transform xf );
+
+/// Input for b2ShapeDistance (reference include/box2d/collision.h:382-396).
+typedef struct b2DistanceInput
+{
+	b2ShapeProxy proxyA;
+	b2ShapeProxy proxyB;
+	/// pose of B in A's frame
+	b2Transform transform;
+	bool useRadii;
+} b2DistanceInput;
+
+/// Output of b2ShapeDistance, in shape A's frame.
+typedef struct b2DistanceOutput
+{
+	b2Vec2 pointA;
+	b2Vec2 pointB;
+	b2Vec2 normal;
+	float distance;
+	int iterations;
+	int simplexCount;
+} b2DistanceOutput;
+
+typedef struct b2SimplexVertex
+{
+	b2Vec2 wA;
+	b2Vec2 wB;
+	b2Vec2 w;
+	float a;
+	int indexA;
+	int indexB;
+} b2SimplexVertex;
+
+typedef struct b2Simplex
+{
+	b2SimplexVertex v1, v2, v3;
+	int count;
+} b2Simplex;
+
+/// Center-of-mass sweep of a body over a step (reference include/box2d/collision.h:457-464).
+typedef struct b2Sweep
+{
+	b2Vec2 localCenter;
+	b2Vec2 c1;
+	b2Vec2 c2;
+	b2Rot q1;
+	b2Rot q2;
+} b2Sweep;
+
+typedef struct b2TOIInput
+{
+	b2ShapeProxy proxyA;
+	b2ShapeProxy proxyB;
+	b2Sweep sweepA;
+	b2Sweep sweepB;
+	float maxFraction;
+} b2TOIInput;
+
+typedef enum b2TOIState
+{
+	b2_toiStateUnknown,
+	b2_toiStateFailed,
+	b2_toiStateOverlapped,
+	b2_toiStateHit,
+	b2_toiStateSeparated
+} b2TOIState;
+
+typedef struct b2TOIOutput
+{
+	b2TOIState state;
+	b2Vec2 point;
+	b2Vec2 normal;
+	float fraction;
+} b2TOIOutput;
+
+/// Distance proxies and GJK distance (reference src/distance.c:108-611).
+B2_API b2ShapeProxy b2MakeProxy( const b2Vec2* points, int count, float radius );
+B2_API b2ShapeProxy b2MakeOffsetProxy( const b2Vec2* points, int count, float radius, b2Vec2 position, b2Rot rotation );
+B2_API b2DistanceOutput b2ShapeDistance( const b2DistanceInput* input, b2SimplexCache* cache, b2Simplex* simplexes,
+										 int simplexCapacity );
+
+/// Sweep interpolation and conservative-advancement time of impact (reference src/distance.c:14-30, 1148-1441).
+B2_API b2Transform b2GetSweepTransform( const b2Sweep* sweep, float time );
+B2_API b2TOIOutput b2TimeOfImpact( const b2TOIInput* input );

@@ -128,3 +128,44 @@ def manifold_array_words(arr):
     raw[:, 5] &= 0xFFFF
     raw[:, 9] &= 0xFFFF
     return raw
+
+
+class ShapeProxy(ctypes.Structure):
+    _fields_ = [("points", Vec2 * 8), ("count", ctypes.c_int), ("radius", c_float)]
+
+
+class SimplexCache(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint16), ("indexA", ctypes.c_uint8 * 3), ("indexB", ctypes.c_uint8 * 3)]
+
+
+class DistanceInput(ctypes.Structure):
+    _fields_ = [("proxyA", ShapeProxy), ("proxyB", ShapeProxy), ("transform", Transform), ("useRadii", ctypes.c_bool)]
+
+
+class DistanceOutput(ctypes.Structure):
+    _fields_ = [("pointA", Vec2), ("pointB", Vec2), ("normal", Vec2), ("distance", c_float), ("iterations", ctypes.c_int),
+                ("simplexCount", ctypes.c_int)]
+
+
+class Sweep(ctypes.Structure):
+    _fields_ = [("localCenter", Vec2), ("c1", Vec2), ("c2", Vec2), ("q1", Rot), ("q2", Rot)]
+
+
+class TOIInput(ctypes.Structure):
+    _fields_ = [("proxyA", ShapeProxy), ("proxyB", ShapeProxy), ("sweepA", Sweep), ("sweepB", Sweep),
+                ("maxFraction", c_float)]
+
+
+class TOIOutput(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("point", Vec2), ("normal", Vec2), ("fraction", c_float)]
+
+
+def bind_distance_functions(lib):
+    lib.b2ShapeDistance.restype = DistanceOutput
+    lib.b2ShapeDistance.argtypes = [ctypes.POINTER(DistanceInput), ctypes.POINTER(SimplexCache), ctypes.c_void_p,
+                                    ctypes.c_int]
+    lib.b2TimeOfImpact.restype = TOIOutput
+    lib.b2TimeOfImpact.argtypes = [ctypes.POINTER(TOIInput)]
+    lib.b2GetSweepTransform.restype = Transform
+    lib.b2GetSweepTransform.argtypes = [ctypes.POINTER(Sweep), c_float]
+    return lib

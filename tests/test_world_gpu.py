@@ -12,11 +12,6 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# Scenes whose bodies move fast enough to need continuous collision, which the GPU step does
-# not have yet (a world that needs it reports the missing feature and stops).
-NEEDS_CONTINUOUS = {"mixed_pile_1", "mixed_pile_2", "mixed_pile_3"}
-
-
 def scene(world_runs, name):
     return next(s for s in world_runs["scenes"] if s["name"] == name)
 
@@ -67,13 +62,37 @@ def test_scene_matches_reference_color_kernels(gpu_scenes, world_runs, name):
     run_and_compare(gpu_scenes, scene(world_runs, name), path=1)
 
 
-@pytest.mark.parametrize("name", sorted(NEEDS_CONTINUOUS))
-def test_mixed_pile_matches_reference(gpu_scenes, world_runs, name):
-    pytest.skip("continuous collision (time of impact) is not on the GPU step yet")
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("name", ["mixed_pile_1", "mixed_pile_2", "mixed_pile_3", "falling_hinges_trace"])
+def test_continuous_scenes_match_reference(gpu_scenes, world_runs, name, path):
+    """Circles, capsules, hulls and rounded boxes dropped from height (fast bodies take the
+    continuous collision kernel), and the hinged-box determinism scene with revolute joints."""
+    print(f"[gpu] scene {name} path {path}", flush=True)
+    run_and_compare(gpu_scenes, scene(world_runs, name), path=path)
 
 
-def test_falling_hinges_determinism(gpu_scenes, world_runs):
-    pytest.skip("continuous collision (time of impact) is not on the GPU step yet")
+@pytest.mark.parametrize("path", [0, 1])
+def test_falling_hinges_determinism(gpu_scenes, world_runs, path):
+    """The reference determinism protocol (test/test_determinism.c:19-24, shared/determinism.c):
+    step at 1/60 s with 4 substeps until no body moves; the scene sleeps at step 274 with
+    transform hash 0xE86690F4."""
+    import box2d_amd
+
+    print(f"[gpu] falling hinges protocol path {path}", flush=True)
+    h = gpu_scenes.create(1)
+    box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
+    try:
+        sleep_step = -1
+        for step in range(500):
+            gpu_scenes.step(h)
+            if gpu_scenes.move_count(h) == 0:
+                sleep_step = step
+                break
+        fh = world_runs["falling_hinges"]
+        assert sleep_step == fh["sleep_step"] == 274
+        assert gpu_scenes.hash(h) == fh["hash"] == 0xE86690F4
+    finally:
+        gpu_scenes.destroy(h)
 
 
 def test_interleaved_worlds_are_independent(gpu_scenes, world_runs):
