@@ -27,7 +27,7 @@ def library():
     return _lib
 
 
-from .scenes_api import SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HELLO, SCENE_MIXED_PILE  # noqa: E402
+from .scenes_api import SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HELLO, SCENE_MIXED_PILE, SCENE_RAIN  # noqa: E402
 
 
 SOLVER_AUTO = 0

@@ -17,6 +17,8 @@ void SetAssertFcn( b2AssertFcn* fcn );
 int CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
 				  b2LocalManifold* manifolds );
 void DeviceSynchronize();
+int TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs );
+int ShapeDistanceBatch( const b2DistanceInput* inputs, b2SimplexCache* caches, int count, b2DistanceOutput* outputs );
 } // namespace b2g
 
 using namespace b2g;
@@ -639,6 +641,16 @@ void b2amd_World_SetSolverPath( b2WorldId worldId, b2amd_SolverPath path )
 	{
 		world->device->SetSolverPath( (int)path );
 	}
+}
+
+int b2amd_TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs )
+{
+	return TimeOfImpactBatch( inputs, count, outputs );
+}
+
+int b2amd_ShapeDistanceBatch( const b2DistanceInput* inputs, b2SimplexCache* caches, int count, b2DistanceOutput* outputs )
+{
+	return ShapeDistanceBatch( inputs, caches, count, outputs );
 }
 
 void b2amd_Synchronize( void )

@@ -17,6 +17,7 @@
 #include "device.h"
 #include "narrowphase.h"
 #include "solver_device.h"
+#include "chain.h"
 #include "toi.h"
 #include "world.h"
 
@@ -206,7 +207,7 @@ __global__ void CollideKernel( CollideArgs a )
 	// b2UpdateContact
 	b2Manifold oldManifold = cs->manifold;
 	b2Transform relativeTransform = b2InvMulWorldTransforms( transformA, transformB );
-	b2LocalManifold local = ComputeManifold( shapeA, shapeB, relativeTransform );
+	b2LocalManifold local = ComputeManifoldCached( shapeA, shapeB, relativeTransform, &cs->cache );
 
 	b2Manifold m;
 	memset( &m, 0, sizeof( m ) );
@@ -1140,10 +1141,24 @@ __global__ void FinalizeBodiesKernel( FinalizeArgs a )
 // ---------------------------------------------------------------------------------------------
 // Continuous collision of fast bodies against static shapes (reference src/solver.c:181-550).
 //
-// One lane per fast non-bullet body. The reference queries the static tree with the swept box and
-// keeps the smallest time of impact; the minimum does not depend on the visit order, so the lane
-// tests the static shapes of the world in a flat loop against the same fat-AABB overlap. All
-// geometry runs in a frame centered on the body's start position (the reference's base frame).
+// One lane per fast non-bullet body. The lane walks a device copy of the host's static tree in
+// the reference's query order (stack, child2 popped first; src/dynamic_tree.c:1085-1135). The
+// order matters: each time of impact is bounded by the smallest fraction found so far, and the
+// root finder's iterates depend on that bound. All geometry runs in a frame centered on the
+// body's start position (the reference's base frame).
+
+// Static tree node: fat AABB, category bits (union over the subtree), children or leaf shape id.
+struct CcdNode
+{
+	b2AABB aabb;
+	uint64_t categoryBits;
+	int child1;
+	int child2;
+	int shapeId; // leaf only
+	int leaf;
+};
+
+constexpr int kCcdStackSize = 256;
 
 struct ContinuousArgs
 {
@@ -1152,8 +1167,9 @@ struct ContinuousArgs
 	const int* awake;
 	BodySim* sims;
 	ShapeDev* shapes;
-	const int* staticShapes;
-	int staticShapeCount;
+	const CcdNode* tree;
+	int treeRoot;
+	int* error; // set when a query stack would overflow
 	const int2* noCollide; // (body, static body) pairs joined with collideConnected off
 	int noCollideCount;
 	int* enlargedShapeCount;
@@ -1309,14 +1325,36 @@ __global__ void ContinuousKernel( ContinuousArgs a )
 		fastShape->aabb = box2;
 		b2AABB swept = b2AABB_Union( box1, box2 );
 
-		for ( int j = 0; j < a.staticShapeCount; ++j )
+		int stack[kCcdStackSize];
+		int stackCount = 0;
+		if ( a.treeRoot != B2_NULL_INDEX )
 		{
-			int otherId = a.staticShapes[j];
-			const ShapeDev& other = a.shapes[otherId];
-			if ( other.filter.categoryBits == 0 || b2AABB_Overlaps( other.fatAABB, swept ) == false )
+			stack[stackCount++] = a.treeRoot;
+		}
+
+		while ( stackCount > 0 )
+		{
+			const CcdNode& node = a.tree[stack[--stackCount]];
+			if ( b2AABB_Overlaps( node.aabb, swept ) == false || node.categoryBits == 0 )
 			{
 				continue;
 			}
+			if ( node.leaf == 0 )
+			{
+				if ( stackCount < kCcdStackSize - 1 )
+				{
+					stack[stackCount++] = node.child1;
+					stack[stackCount++] = node.child2;
+				}
+				else
+				{
+					*a.error = 1;
+				}
+				continue;
+			}
+
+			int otherId = node.shapeId;
+			const ShapeDev& other = a.shapes[otherId];
 			if ( otherId == shapeId || other.bodyId == fastShape->bodyId )
 			{
 				continue;
@@ -1509,7 +1547,7 @@ __global__ __launch_bounds__( kPackThreads ) void PackCollideKernel( int* counte
 
 struct PackFinalizeArgs
 {
-	int* counters; // enlarged bodies, enlarged shapes, fast bodies
+	int* counters; // enlarged bodies, enlarged shapes, fast bodies, continuous-collision error
 	const int* enlargedBodies;
 	const int* enlargedShapes;
 	const b2AABB* enlargedShapeAABBs;
@@ -1526,6 +1564,7 @@ __global__ __launch_bounds__( kPackThreads ) void PackFinalizeKernel( PackFinali
 	int bodies = a.counters[0];
 	int shapes = a.counters[1];
 	int fast = a.counters[2];
+	int error = a.counters[3];
 	for ( int i = threadIdx.x; i < bodies; i += kPackThreads )
 	{
 		a.mail[kMailHeader + i] = a.enlargedBodies[i];
@@ -1554,9 +1593,11 @@ __global__ __launch_bounds__( kPackThreads ) void PackFinalizeKernel( PackFinali
 		a.mail[0] = bodies;
 		a.mail[1] = shapes;
 		a.mail[2] = fast;
+		a.mail[3] = error;
 		a.counters[0] = 0;
 		a.counters[1] = 0;
 		a.counters[2] = 0;
+		a.counters[3] = 0;
 	}
 }
 
@@ -1644,9 +1685,10 @@ struct DeviceWorld::Impl
 	// finalize outputs
 	DevArray<int> enlargedBodies;
 	DevArray<int> fastBodies;
-	DevArray<int> ccdStaticShapes;
+	DevArray<CcdNode> ccdTree;
+	int ccdTreeRoot = B2_NULL_INDEX;
 	DevArray<int2> ccdNoCollide;
-	std::vector<int> ccdStaticStaging;
+	std::vector<CcdNode> ccdTreeStaging;
 	std::vector<int2> ccdNoCollideStaging;
 	DevArray<int> enlargedShapes;
 	DevArray<b2AABB> enlargedShapeAABBs;
@@ -1806,7 +1848,7 @@ DeviceWorld::~DeviceWorld()
 	impl->cints.Free();
 	impl->enlargedBodies.Free();
 	impl->fastBodies.Free();
-	impl->ccdStaticShapes.Free();
+	impl->ccdTree.Free();
 	impl->ccdNoCollide.Free();
 	impl->enlargedShapes.Free();
 	impl->enlargedShapeAABBs.Free();
@@ -1879,16 +1921,24 @@ void DeviceWorld::UploadShapes( const World& world )
 	impl->enlargedShapes.Reserve( std::max<size_t>( n, 1 ), false, impl->stream, 0 );
 	impl->enlargedShapeAABBs.Reserve( std::max<size_t>( n, 1 ), false, impl->stream, 0 );
 
-	// static shapes in id order: the candidates of continuous collision
-	impl->ccdStaticStaging.clear();
-	for ( const Shape& shape : world.shapes )
+	// the static tree, node for node: continuous collision walks it in the host query order
 	{
-		if ( shape.id != B2_NULL_INDEX && world.bodies[shape.bodyId].type == b2_staticBody )
+		const DynamicTree& tree = world.broadPhase.trees[b2_staticBody];
+		const std::vector<TreeNode>& nodes = tree.Nodes();
+		impl->ccdTreeStaging.resize( nodes.size() );
+		for ( size_t i = 0; i < nodes.size(); ++i )
 		{
-			impl->ccdStaticStaging.push_back( shape.id );
+			CcdNode& d = impl->ccdTreeStaging[i];
+			d.aabb = nodes[i].aabb;
+			d.categoryBits = nodes[i].categoryBits;
+			d.leaf = ( nodes[i].flags & kNodeLeaf ) ? 1 : 0;
+			d.child1 = d.leaf ? B2_NULL_INDEX : nodes[i].child1;
+			d.child2 = d.leaf ? B2_NULL_INDEX : nodes[i].child2;
+			d.shapeId = d.leaf ? (int)nodes[i].userData : B2_NULL_INDEX;
 		}
+		impl->ccdTreeRoot = tree.Root();
+		UploadVector( impl->ccdTree, impl->ccdTreeStaging, impl->stream );
 	}
-	UploadVector( impl->ccdStaticShapes, impl->ccdStaticStaging, impl->stream );
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
@@ -2291,8 +2341,9 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		ka.awake = impl->awakeBodies.ptr;
 		ka.sims = impl->sims.ptr;
 		ka.shapes = impl->shapes.ptr;
-		ka.staticShapes = impl->ccdStaticShapes.ptr;
-		ka.staticShapeCount = (int)impl->ccdStaticStaging.size();
+		ka.tree = impl->ccdTree.ptr;
+		ka.treeRoot = impl->ccdTree.ptr != nullptr ? impl->ccdTreeRoot : B2_NULL_INDEX;
+		ka.error = impl->counters + 7;
 		ka.noCollide = impl->ccdNoCollide.ptr;
 		ka.noCollideCount = (int)impl->ccdNoCollideStaging.size();
 		ka.enlargedShapeCount = impl->counters + 5;
@@ -2326,6 +2377,10 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		int enlargedBodyCount = mail[0];
 		int enlargedShapeCount = mail[1];
 		result->fastBodyCount = mail[2];
+		if ( mail[3] != 0 )
+		{
+			Fatal( "continuous collision: the static tree is deeper than the query stack (%d entries)", kCcdStackSize );
+		}
 		result->enlargedBodies.assign( mail + kMailHeader, mail + kMailHeader + enlargedBodyCount );
 		result->enlargedShapes.assign( mail + pk.shapeOffset, mail + pk.shapeOffset + enlargedShapeCount );
 		result->enlargedShapeAABBs.resize( enlargedShapeCount );
@@ -2396,6 +2451,73 @@ __global__ void CollideBatchKernel( const ShapeDev* shapesA, const ShapeDev* sha
 void DeviceWorld::SetSolverPath( int path )
 {
 	impl->solverPath = path;
+}
+
+__global__ void TimeOfImpactBatchKernel( const b2TOIInput* inputs, int count, b2TOIOutput* outputs )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i < count )
+	{
+		outputs[i] = TimeOfImpact( inputs[i] );
+	}
+}
+
+__global__ void ShapeDistanceBatchKernel( const b2DistanceInput* inputs, b2SimplexCache* caches, int count, b2DistanceOutput* outputs )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i < count )
+	{
+		b2SimplexCache cache = caches[i];
+		outputs[i] = ShapeDistance( inputs[i], &cache );
+		caches[i] = cache;
+	}
+}
+
+// Copy n records to a fresh device buffer.
+template <typename T>
+static T* ToDevice( const T* host, int n )
+{
+	T* d = nullptr;
+	HIP_CHECK( hipMalloc( &d, std::max( n, 1 ) * sizeof( T ) ) );
+	HIP_CHECK( hipMemcpy( d, host, n * sizeof( T ), hipMemcpyHostToDevice ) );
+	return d;
+}
+
+int TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs )
+{
+	if ( count <= 0 )
+	{
+		return 0;
+	}
+	b2TOIInput* dIn = ToDevice( inputs, count );
+	b2TOIOutput* dOut = nullptr;
+	HIP_CHECK( hipMalloc( &dOut, count * sizeof( b2TOIOutput ) ) );
+	TimeOfImpactBatchKernel<<<Blocks( count ), kBlock>>>( dIn, count, dOut );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipMemcpy( outputs, dOut, count * sizeof( b2TOIOutput ), hipMemcpyDeviceToHost ) );
+	HIP_CHECK( hipFree( dIn ) );
+	HIP_CHECK( hipFree( dOut ) );
+	return 0;
+}
+
+int ShapeDistanceBatch( const b2DistanceInput* inputs, b2SimplexCache* caches, int count, b2DistanceOutput* outputs )
+{
+	if ( count <= 0 )
+	{
+		return 0;
+	}
+	b2DistanceInput* dIn = ToDevice( inputs, count );
+	b2SimplexCache* dCache = ToDevice( caches, count );
+	b2DistanceOutput* dOut = nullptr;
+	HIP_CHECK( hipMalloc( &dOut, count * sizeof( b2DistanceOutput ) ) );
+	ShapeDistanceBatchKernel<<<Blocks( count ), kBlock>>>( dIn, dCache, count, dOut );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipMemcpy( outputs, dOut, count * sizeof( b2DistanceOutput ), hipMemcpyDeviceToHost ) );
+	HIP_CHECK( hipMemcpy( caches, dCache, count * sizeof( b2SimplexCache ), hipMemcpyDeviceToHost ) );
+	HIP_CHECK( hipFree( dIn ) );
+	HIP_CHECK( hipFree( dCache ) );
+	HIP_CHECK( hipFree( dOut ) );
+	return 0;
 }
 
 void DeviceSynchronize()
@@ -2578,6 +2700,28 @@ B2_API b2Transform b2GetSweepTransform( const b2Sweep* sweep, float time )
 B2_API b2TOIOutput b2TimeOfImpact( const b2TOIInput* input )
 {
 	return b2g::TimeOfImpact( *input );
+}
+
+} // extern "C"
+
+// Host forms of the chain-segment manifold functions.
+extern "C" {
+
+B2_API b2LocalManifold b2CollideChainSegmentAndCircle( const b2ChainSegment* segmentA, const b2Circle* circleB, b2Transform xf )
+{
+	return b2g::CollideChainSegmentAndCircle( *segmentA, *circleB, xf );
+}
+
+B2_API b2LocalManifold b2CollideChainSegmentAndCapsule( const b2ChainSegment* segmentA, const b2Capsule* capsuleB, b2Transform xf,
+														b2SimplexCache* cache )
+{
+	return b2g::CollideChainSegmentAndCapsule( *segmentA, *capsuleB, xf, cache );
+}
+
+B2_API b2LocalManifold b2CollideChainSegmentAndPolygon( const b2ChainSegment* segmentA, const b2Polygon* polygonB, b2Transform xf,
+														b2SimplexCache* cache )
+{
+	return b2g::CollideChainSegmentAndPolygon( *segmentA, *polygonB, xf, cache );
 }
 
 } // extern "C"

@@ -104,6 +104,16 @@ public:
 		return proxyCount;
 	}
 
+	const std::vector<TreeNode>& Nodes() const
+	{
+		return nodes;
+	}
+
+	int Root() const
+	{
+		return root;
+	}
+
 	// Leaf proxy ids in query order (child2 first depth-first traversal). Used by the GPU pair
 	// finder to order pairs exactly like the host query would.
 	void LeafOrder( std::vector<int>& out ) const;

@@ -1421,6 +1421,28 @@ b2ShapeId World::CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void
 	PullBodies();
 	PullShapes();
 
+	int shapeId = CreateShapeInternal( body, def, geometry, shapeType );
+	Shape* shape = &shapes[shapeId];
+
+	if ( def->updateBodyMass == true )
+	{
+		UpdateBodyMassData( body );
+	}
+	else if ( ( body->flags & kDirtyMass ) == 0 )
+	{
+		body->flags |= kDirtyMass;
+		bodyAux[body->id].flags |= kDirtyMass;
+	}
+
+	shapesDirty = true;
+	bodiesDirty = true;
+	return b2ShapeId{ shapeId + 1, worldId, shape->generation };
+}
+
+// Shape record, proxy and body shape-list link (reference src/shape.c:108-207). Mass is the
+// caller's business: public shapes update it, chain segments do not.
+int World::CreateShapeInternal( Body* body, const b2ShapeDef* def, const void* geometry, b2ShapeType shapeType )
+{
 	int shapeId = shapeIdPool.Alloc();
 	if ( shapeId == (int)shapes.size() )
 	{
@@ -1503,27 +1525,86 @@ b2ShapeId World::CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void
 	body->headShapeId = shapeId;
 	bodySims[body->id].headShapeId = shapeId;
 	body->shapeCount += 1;
+	return shapeId;
+}
 
-	if ( def->updateBodyMass == true )
+// Chain of one-sided segments with ghost vertices (reference src/shape.c:439-571).
+b2ChainId World::CreateChain( b2BodyId bodyId, const b2ChainDef* def )
+{
+	if ( locked )
 	{
-		UpdateBodyMassData( body );
+		return b2ChainId{ 0, 0, 0 };
 	}
-	else if ( ( body->flags & kDirtyMass ) == 0 )
+
+	Body* body = GetBody( bodyId );
+	if ( body == nullptr || def->count < 4 || ( def->materialCount != 1 && def->materialCount != def->count ) )
 	{
-		body->flags |= kDirtyMass;
-		bodyAux[body->id].flags |= kDirtyMass;
+		return b2ChainId{ 0, 0, 0 };
+	}
+
+	PullBodies();
+	PullShapes();
+
+	int chainId = chainIdPool.Alloc();
+	if ( chainId == (int)chains.size() )
+	{
+		chains.emplace_back();
+	}
+
+	ChainShape* chain = &chains[chainId];
+	chain->id = chainId;
+	chain->bodyId = body->id;
+	chain->nextChainId = body->headChainId;
+	chain->generation += 1;
+	chain->materials.assign( def->materials, def->materials + def->materialCount );
+	body->headChainId = chainId;
+
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.userData = def->userData;
+	shapeDef.filter = def->filter;
+	shapeDef.enableSensorEvents = def->enableSensorEvents;
+	shapeDef.enableContactEvents = false;
+	shapeDef.enableHitEvents = false;
+
+	const b2Vec2* points = def->points;
+	int n = def->count;
+	int materialCount = def->materialCount;
+
+	auto addSegment = [&]( b2Vec2 ghost1, b2Vec2 p1, b2Vec2 p2, b2Vec2 ghost2, int materialIndex ) {
+		b2ChainSegment segment;
+		segment.ghost1 = ghost1;
+		segment.segment.point1 = p1;
+		segment.segment.point2 = p2;
+		segment.ghost2 = ghost2;
+		segment.chainId = chainId;
+		shapeDef.material = def->materials[materialCount == 1 ? 0 : materialIndex];
+		int shapeId = CreateShapeInternal( body, &shapeDef, &segment, b2_chainSegmentShape );
+		chains[chainId].shapeIndices.push_back( shapeId );
+	};
+
+	if ( def->isLoop )
+	{
+		int prevIndex = n - 1;
+		for ( int i = 0; i < n - 2; ++i )
+		{
+			addSegment( points[prevIndex], points[i], points[i + 1], points[i + 2], i );
+			prevIndex = i;
+		}
+		addSegment( points[n - 3], points[n - 2], points[n - 1], points[0], n - 2 );
+		addSegment( points[n - 2], points[n - 1], points[0], points[1], n - 1 );
+	}
+	else
+	{
+		for ( int i = 0; i < n - 3; ++i )
+		{
+			// the material belongs to the leading point of the solid segment
+			addSegment( points[i], points[i + 1], points[i + 2], points[i + 3], i + 1 );
+		}
 	}
 
 	shapesDirty = true;
 	bodiesDirty = true;
-	return b2ShapeId{ shapeId + 1, worldId, shape->generation };
-}
-
-b2ChainId World::CreateChain( b2BodyId bodyId, const b2ChainDef* def )
-{
-	(void)bodyId;
-	(void)def;
-	Fatal( "chain shapes are not supported by the GPU step yet" );
+	return b2ChainId{ chainId + 1, worldId, chains[chainId].generation };
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1543,11 +1624,6 @@ void World::CreateContact( Shape* shapeA, Shape* shapeB )
 	{
 		CreateContact( shapeB, shapeA );
 		return;
-	}
-
-	if ( type1 == b2_chainSegmentShape )
-	{
-		Fatal( "chain segment contacts are not supported by the GPU step yet" );
 	}
 
 	Body* bodyA = &bodies[shapeA->bodyId];

@@ -87,6 +87,7 @@ struct ChainShape
 	int bodyId = B2_NULL_INDEX;
 	int nextChainId = B2_NULL_INDEX;
 	std::vector<int> shapeIndices;
+	std::vector<b2SurfaceMaterial> materials;
 	uint16_t generation = 0;
 };
 
@@ -218,6 +219,7 @@ public:
 	// Public API helpers
 	b2BodyId CreateBody( const b2BodyDef* def );
 	b2ShapeId CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void* geometry, b2ShapeType type );
+	int CreateShapeInternal( Body* body, const b2ShapeDef* def, const void* geometry, b2ShapeType type );
 	b2ChainId CreateChain( b2BodyId bodyId, const b2ChainDef* def );
 	b2JointId CreateRevoluteJoint( const b2RevoluteJointDef* def );
 	void Step( float timeStep, int subStepCount );

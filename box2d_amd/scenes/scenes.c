@@ -254,6 +254,104 @@ static void b2sBuildMixedPile( b2sScene* scene, int count, int enableSleep, uint
 	}
 }
 
+static void b2sAddMixedShape( b2BodyId bodyId, int kind )
+{
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.density = b2sRandomRange( 0.5f, 2.0f );
+
+	if ( kind == 0 )
+	{
+		b2Circle circle = { { 0.0f, 0.0f }, b2sRandomRange( 0.2f, 0.5f ) };
+		b2CreateCircleShape( bodyId, &shapeDef, &circle );
+	}
+	else if ( kind == 1 )
+	{
+		float halfLength = b2sRandomRange( 0.2f, 0.6f );
+		b2Capsule capsule = { { -halfLength, 0.0f }, { halfLength, 0.0f }, b2sRandomRange( 0.1f, 0.25f ) };
+		b2CreateCapsuleShape( bodyId, &shapeDef, &capsule );
+	}
+	else if ( kind == 2 )
+	{
+		b2Vec2 points[8];
+		int n = 3 + (int)( b2sNextRandom() % 6u );
+		for ( int i = 0; i < n; ++i )
+		{
+			float angle = b2sRandomRange( -3.14f, 3.14f );
+			float radius = b2sRandomRange( 0.0f, 0.5f );
+			b2Rot q = b2MakeRot( angle );
+			points[i] = ( b2Vec2 ){ radius * q.c, radius * q.s };
+		}
+		b2Hull hull = b2ComputeHull( points, n );
+		b2Polygon polygon = hull.count >= 3 ? b2MakePolygon( &hull, 0.0f ) : b2MakeSquare( 0.25f );
+		b2CreatePolygonShape( bodyId, &shapeDef, &polygon );
+	}
+	else
+	{
+		b2Polygon box = b2MakeRoundedBox( b2sRandomRange( 0.2f, 0.5f ), b2sRandomRange( 0.2f, 0.5f ), 0.05f );
+		b2CreatePolygonShape( bodyId, &shapeDef, &box );
+	}
+}
+
+static void b2sBuildRain( b2sScene* scene, int count, int columns, int enableSleep, uint32_t seed, float width )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+	s_rngState = seed != 0 ? seed : 12345u;
+
+	// container: right wall down, floor right to left, left wall up (interior on the right side)
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+
+		float halfWidth = 0.5f * width + 20.0f;
+		float height = 100.0f;
+		int floorSegments = (int)( 2.0f * halfWidth / 2.0f );
+		int wallSegments = 10;
+		int pointCount = 2 + ( wallSegments + 1 ) + ( floorSegments - 1 ) + ( wallSegments + 1 );
+		b2Vec2* points = (b2Vec2*)malloc( (size_t)pointCount * sizeof( b2Vec2 ) );
+		int n = 0;
+		points[n++] = ( b2Vec2 ){ halfWidth, height + 1.0f };
+		for ( int i = 0; i <= wallSegments; ++i )
+		{
+			points[n++] = ( b2Vec2 ){ halfWidth, height - height * i / wallSegments };
+		}
+		for ( int i = 1; i < floorSegments; ++i )
+		{
+			points[n++] = ( b2Vec2 ){ halfWidth - 2.0f * halfWidth * i / floorSegments, 0.0f };
+		}
+		for ( int i = 0; i <= wallSegments; ++i )
+		{
+			points[n++] = ( b2Vec2 ){ -halfWidth, height * i / wallSegments };
+		}
+		points[n++] = ( b2Vec2 ){ -halfWidth, height + 1.0f };
+
+		b2SurfaceMaterial material = b2DefaultSurfaceMaterial();
+		b2ChainDef chainDef = b2DefaultChainDef();
+		chainDef.points = points;
+		chainDef.count = n;
+		chainDef.materials = &material;
+		chainDef.materialCount = 1;
+		chainDef.isLoop = false;
+		b2CreateChain( groundId, &chainDef );
+		free( points );
+	}
+
+	float spacing = width / (float)columns;
+	int capacity = 0;
+	for ( int k = 0; k < count; ++k )
+	{
+		int col = k % columns;
+		int row = k / columns;
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ -0.5f * width + ( col + 0.5f ) * spacing, 110.0f + spacing * row };
+		bodyDef.rotation = b2MakeRot( b2sRandomRange( -3.0f, 3.0f ) );
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2sAddMixedShape( bodyId, k & 3 );
+		b2sPushBody( scene, &capacity, bodyId );
+	}
+}
+
 int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount )
 {
 	int handle = -1;
@@ -309,6 +407,17 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 			int sleep = iparamCount > 1 ? iparams[1] : 0;
 			uint32_t seed = iparamCount > 2 ? (uint32_t)iparams[2] : 12345u;
 			b2sBuildMixedPile( scene, count, sleep, seed );
+		}
+		break;
+
+		case B2S_RAIN:
+		{
+			int count = iparamCount > 0 ? iparams[0] : 20000;
+			int columns = iparamCount > 1 ? iparams[1] : 100;
+			int sleep = iparamCount > 2 ? iparams[2] : 0;
+			uint32_t seed = iparamCount > 3 ? (uint32_t)iparams[3] : 12345u;
+			float width = fparamCount > 0 ? fparams[0] : 120.0f;
+			b2sBuildRain( scene, count, columns, sleep, seed, width );
 		}
 		break;
 

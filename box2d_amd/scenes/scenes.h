@@ -35,6 +35,15 @@ enum
 	// Small mixed scene: circles, capsules, polygons and rounded boxes dropped on a ground box.
 	// iparams: [0] body count, [1] enableSleep, [2] seed. fparams: unused.
 	B2S_MIXED_PILE = 3,
+
+	// Mixed-shape rain into a static U-shaped chain container: equal quarters of circles
+	// (r in [0.2, 0.5]), capsules (half-length in [0.2, 0.6], r in [0.1, 0.25]), convex hulls of
+	// 3-8 points in a 1 m disc, and rounded boxes (radius 0.05), densities in [0.5, 2], on a grid
+	// of `columns` columns spanning a column of the given width (spacing width / columns, rows
+	// stacked upward from 110 m). The container is an open chain whose interior is on the
+	// segments' right side (walls 100 m high, floor 2 m segments).
+	// iparams: [0] body count, [1] columns, [2] enableSleep, [3] seed. fparams: [0] column width.
+	B2S_RAIN = 4,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.

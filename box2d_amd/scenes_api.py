@@ -16,6 +16,7 @@ SCENE_PYRAMID = 0
 SCENE_FALLING_HINGES = 1
 SCENE_HELLO = 2
 SCENE_MIXED_PILE = 3
+SCENE_RAIN = 4
 
 
 class SceneLib:

@@ -249,3 +249,11 @@ B2_API b2DistanceOutput b2ShapeDistance( const b2DistanceInput* input, b2Simplex
 /// Sweep interpolation and conservative-advancement time of impact (reference src/distance.c:14-30, 1148-1441).
 B2_API b2Transform b2GetSweepTransform( const b2Sweep* sweep, float time );
 B2_API b2TOIOutput b2TimeOfImpact( const b2TOIInput* input );
+
+/// One-sided chain segment manifolds; capsule and polygon forms carry the contact's GJK simplex
+/// cache (reference include/box2d/collision.h:643-652).
+B2_API b2LocalManifold b2CollideChainSegmentAndCircle( const b2ChainSegment* segmentA, const b2Circle* circleB, b2Transform xf );
+B2_API b2LocalManifold b2CollideChainSegmentAndCapsule( const b2ChainSegment* segmentA, const b2Capsule* capsuleB, b2Transform xf,
+														b2SimplexCache* cache );
+B2_API b2LocalManifold b2CollideChainSegmentAndPolygon( const b2ChainSegment* segmentA, const b2Polygon* polygonB, b2Transform xf,
+														b2SimplexCache* cache );

@@ -74,6 +74,14 @@ typedef struct b2amd_Shape
 B2_API int b2amd_CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
 							   b2LocalManifold* manifolds );
 
+/// Evaluate count time-of-impact queries on the GPU (the continuous-collision kernel's code;
+/// reference src/distance.c:1148-1441). Host arrays in, host array out; synchronous. Returns 0.
+B2_API int b2amd_TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs );
+
+/// Evaluate count GJK distance queries on the GPU, each starting from and updating caches[i]
+/// (reference src/distance.c:424-611). Host arrays in and out; synchronous. Returns 0.
+B2_API int b2amd_ShapeDistanceBatch( const b2DistanceInput* inputs, b2SimplexCache* caches, int count, b2DistanceOutput* outputs );
+
 #ifdef __cplusplus
 }
 #endif

@@ -169,3 +169,19 @@ def bind_distance_functions(lib):
     lib.b2GetSweepTransform.restype = Transform
     lib.b2GetSweepTransform.argtypes = [ctypes.POINTER(Sweep), c_float]
     return lib
+
+
+class ChainSegment(ctypes.Structure):
+    _fields_ = [("ghost1", Vec2), ("segment", Segment), ("ghost2", Vec2), ("chainId", ctypes.c_int)]
+
+
+def bind_chain_functions(lib):
+    lib.b2CollideChainSegmentAndCircle.restype = LocalManifold
+    lib.b2CollideChainSegmentAndCircle.argtypes = [ctypes.POINTER(ChainSegment), ctypes.POINTER(Circle), Transform]
+    lib.b2CollideChainSegmentAndCapsule.restype = LocalManifold
+    lib.b2CollideChainSegmentAndCapsule.argtypes = [ctypes.POINTER(ChainSegment), ctypes.POINTER(Capsule), Transform,
+                                                    ctypes.c_void_p]
+    lib.b2CollideChainSegmentAndPolygon.restype = LocalManifold
+    lib.b2CollideChainSegmentAndPolygon.argtypes = [ctypes.POINTER(ChainSegment), ctypes.POINTER(Polygon), Transform,
+                                                    ctypes.c_void_p]
+    return lib

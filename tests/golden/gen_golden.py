@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from box2d_amd.scenes_api import SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES  # noqa: E402
+from box2d_amd.scenes_api import SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES, SCENE_RAIN  # noqa: E402
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
 
@@ -39,7 +39,8 @@ def trace(lib, h, steps, every):
 
 
 def scene_fixture(lib, name, kind, iparams, fparams, steps, every, keep_transforms):
-    h = lib.create(kind, iparams, fparams)
+    # the reference's results do not depend on the worker count; 8 workers keep generation quick
+    h = lib.create(kind, iparams, fparams, workers=8)
     fx = {
         "name": name,
         "kind": kind,
@@ -76,6 +77,9 @@ def main():
     for seed in (1, 2, 3):
         fixtures.append(scene_fixture(lib, f"mixed_pile_{seed}", SCENE_MIXED_PILE, (64, 1, seed), (), 300,
                                       set(range(1, 301)), True))
+    fixtures.append(scene_fixture(lib, "rain_small", SCENE_RAIN, (2000, 40, 0, 7), (48.0,), 400, set(range(1, 401)), True))
+    fixtures.append(scene_fixture(lib, "rain", SCENE_RAIN, (20000, 100, 0, 12345), (120.0,), 1000,
+                                  set(range(10, 1001, 10)) | {1, 2, 5}, False))
     fixtures.append(scene_fixture(lib, "falling_hinges_trace", SCENE_FALLING_HINGES, (), (), 300, set(range(1, 301)),
                                   True))
     ok, step, hsh = lib.falling_hinges(1, 500)

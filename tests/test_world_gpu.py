@@ -71,6 +71,20 @@ def test_continuous_scenes_match_reference(gpu_scenes, world_runs, name, path):
     run_and_compare(gpu_scenes, scene(world_runs, name), path=path)
 
 
+@pytest.mark.parametrize("path", [0, 1, 2])
+def test_rain_small_matches_reference(gpu_scenes, world_runs, path):
+    """2000 mixed shapes raining into a static U-shaped chain container (one-sided chain segments
+    with ghost vertices, GJK simplex caches, continuous collision against the chain)."""
+    print(f"[gpu] rain_small path {path}", flush=True)
+    run_and_compare(gpu_scenes, scene(world_runs, "rain_small"), path=path)
+
+
+def test_rain_matches_reference(gpu_scenes, world_runs):
+    """The 20,000-body mixed-shape rain, 1000 steps, hash-checked every 10 steps."""
+    print("[gpu] rain 20k", flush=True)
+    run_and_compare(gpu_scenes, scene(world_runs, "rain"))
+
+
 @pytest.mark.parametrize("path", [0, 1])
 def test_falling_hinges_determinism(gpu_scenes, world_runs, path):
     """The reference determinism protocol (test/test_determinism.c:19-24, shared/determinism.c):
