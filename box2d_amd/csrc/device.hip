@@ -115,9 +115,8 @@ B2G_DEV float ShapeRadius( const ShapeDev& s )
 
 // Narrowphase update for one contact (reference src/physics_world.c:451-609 and
 // src/contact.c:519-698).
-__global__ void CollideKernel( CollideArgs a )
+B2G_DEV void CollideBody( const CollideArgs& a, int k )
 {
-	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( k >= a.count )
 	{
 		return;
@@ -328,9 +327,8 @@ struct PrepareArgs
 
 // Constraint preparation (reference src/contact_solver.c:1573-1809; the overflow variant at
 // :24-160 computes the same values).
-__global__ void PrepareContactsKernel( PrepareArgs a )
+B2G_DEV void PrepareContactsBody( const PrepareArgs& a, int i )
 {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( i >= a.count )
 	{
 		return;
@@ -467,9 +465,8 @@ struct PrepareJointArgs
 };
 
 // Joint preparation (reference src/joint.c:1406-1452, src/revolute_joint.c:217-281).
-__global__ void PrepareJointsKernel( PrepareJointArgs a )
+B2G_DEV void PrepareJointsBody( const PrepareJointArgs& a, int k )
 {
-	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( k >= a.count )
 	{
 		return;
@@ -818,7 +815,7 @@ B2G_DEV void FusedStage( const FusedArgs& a, ColorArgs& ca, const LdsBodies& B )
 		{
 			continue;
 		}
-		for ( int t = threadIdx.x; t < lanes; t += kFusedThreads )
+		for ( int t = threadIdx.x; t < lanes; t += blockDim.x )
 		{
 			ColorLane<Stage>( ca, t, B );
 		}
@@ -826,19 +823,30 @@ B2G_DEV void FusedStage( const FusedArgs& a, ColorArgs& ca, const LdsBodies& B )
 	}
 }
 
-__global__ __launch_bounds__( kFusedThreads ) void SolveFusedKernel( FusedArgs a )
+// LDS bytes of a world with `count` awake bodies: slot 0 holds the static identity state, slots
+// 1..count the awake bodies; 7 floats and 16 flag bits per slot.
+__host__ __device__ inline int FusedLdsBytes( int count )
 {
-	__shared__ float s_vx[kFusedBodyCapacity + 1];
-	__shared__ float s_vy[kFusedBodyCapacity + 1];
-	__shared__ float s_w[kFusedBodyCapacity + 1];
-	__shared__ float s_dpx[kFusedBodyCapacity + 1];
-	__shared__ float s_dpy[kFusedBodyCapacity + 1];
-	__shared__ float s_dqc[kFusedBodyCapacity + 1];
-	__shared__ float s_dqs[kFusedBodyCapacity + 1];
-	__shared__ uint16_t s_flags[kFusedBodyCapacity + 1];
+	int cap = count + 1;
+	return ( 7 * 4 * cap + 2 * cap + 15 ) & ~15;
+}
+
+// The whole substep loop of one world by the calling workgroup (any block size), body states in
+// the dynamic LDS at lds.
+B2G_DEV void FusedBody( const FusedArgs& a, char* lds )
+{
+	const int n = a.bodyCount;
+	const int cap = n + 1;
+	float* s_vx = reinterpret_cast<float*>( lds );
+	float* s_vy = s_vx + cap;
+	float* s_w = s_vy + cap;
+	float* s_dpx = s_w + cap;
+	float* s_dpy = s_dpx + cap;
+	float* s_dqc = s_dpy + cap;
+	float* s_dqs = s_dqc + cap;
+	uint16_t* s_flags = reinterpret_cast<uint16_t*>( s_dqs + cap );
 
 	LdsBodies B{ s_vx, s_vy, s_w, s_dpx, s_dpy, s_dqc, s_dqs, s_flags };
-	const int n = a.bodyCount;
 	const GlobalBodies G{ a.states };
 
 	auto put = [&]( int slot, const BodyState& st ) {
@@ -852,9 +860,9 @@ __global__ __launch_bounds__( kFusedThreads ) void SolveFusedKernel( FusedArgs a
 		s_dqs[slot] = st.deltaRotation.s;
 	};
 
-	for ( int i = threadIdx.x; i < n; i += kFusedThreads )
+	for ( int i = threadIdx.x; i < n; i += blockDim.x )
 	{
-		put( i, ToState( G.Load( a.awake[i] ) ) );
+		put( i + 1, ToState( G.Load( a.awake[i] ) ) );
 	}
 	if ( threadIdx.x == 0 )
 	{
@@ -872,24 +880,24 @@ __global__ __launch_bounds__( kFusedThreads ) void SolveFusedKernel( FusedArgs a
 	ColorArgs ca = a.color;
 	for ( int sub = 0; sub < a.subStepCount; ++sub )
 	{
-		for ( int i = threadIdx.x; i < n; i += kFusedThreads )
+		for ( int i = threadIdx.x; i < n; i += blockDim.x )
 		{
-			BodyState st = ToState( B.Load( i ) );
+			BodyState st = ToState( B.Load( i + 1 ) );
 			IntegrateVelocity( a.sims[a.awake[i]], &st, a.color.p );
-			s_vx[i] = st.linearVelocity.x;
-			s_vy[i] = st.linearVelocity.y;
-			s_w[i] = st.angularVelocity;
+			s_vx[i + 1] = st.linearVelocity.x;
+			s_vy[i + 1] = st.linearVelocity.y;
+			s_w[i + 1] = st.angularVelocity;
 		}
 		__syncthreads();
 
 		FusedStage<kStageWarmStart>( a, ca, B );
 		FusedStage<kStageSolve>( a, ca, B );
 
-		for ( int i = threadIdx.x; i < n; i += kFusedThreads )
+		for ( int i = threadIdx.x; i < n; i += blockDim.x )
 		{
-			BodyState st = ToState( B.Load( i ) );
+			BodyState st = ToState( B.Load( i + 1 ) );
 			IntegratePosition( &st, a.color.p );
-			put( i, st );
+			put( i + 1, st );
 		}
 		__syncthreads();
 
@@ -902,14 +910,45 @@ __global__ __launch_bounds__( kFusedThreads ) void SolveFusedKernel( FusedArgs a
 	}
 
 	// write the whole state back (flags carry the speed-cap bit to finalize)
-	for ( int i = threadIdx.x; i < n; i += kFusedThreads )
+	for ( int i = threadIdx.x; i < n; i += blockDim.x )
 	{
 		BodyState* dst = a.states + a.awake[i];
-		dst->linearVelocity = { s_vx[i], s_vy[i] };
-		dst->angularVelocity = s_w[i];
-		dst->flags = ( dst->flags & 0xFFFF0000u ) | s_flags[i];
-		dst->deltaPosition = { s_dpx[i], s_dpy[i] };
-		dst->deltaRotation = { s_dqc[i], s_dqs[i] };
+		dst->linearVelocity = { s_vx[i + 1], s_vy[i + 1] };
+		dst->angularVelocity = s_w[i + 1];
+		dst->flags = ( dst->flags & 0xFFFF0000u ) | s_flags[i + 1];
+		dst->deltaPosition = { s_dpx[i + 1], s_dpy[i + 1] };
+		dst->deltaRotation = { s_dqc[i + 1], s_dqs[i + 1] };
+	}
+}
+
+__global__ __launch_bounds__( kFusedThreads ) void SolveFusedKernel( FusedArgs a )
+{
+	extern __shared__ char s_lds[];
+	FusedBody( a, s_lds );
+}
+
+// Persistent form over many worlds: each workgroup takes the next world from a queue (worlds
+// ordered largest first) until the queue is empty. Every workgroup reaches the queue's end, so
+// the launch drains without any inter-group wait.
+__global__ __launch_bounds__( kFusedThreads ) void SolveFusedBatchKernel( const FusedArgs* args, const int* order, int count,
+																		  int* queue )
+{
+	extern __shared__ char s_lds[];
+	__shared__ int s_world;
+	for ( ;; )
+	{
+		if ( threadIdx.x == 0 )
+		{
+			s_world = atomicAdd( queue, 1 );
+		}
+		__syncthreads();
+		int w = s_world;
+		if ( w >= count )
+		{
+			break;
+		}
+		FusedBody( args[order[w]], s_lds );
+		__syncthreads();
 	}
 }
 
@@ -923,9 +962,8 @@ struct StoreArgs
 
 // Impulses back to the manifolds (reference src/contact_solver.c:2238-2331 for colored
 // constraints, which write both points, and :516-545 for overflow, which write pointCount points).
-__global__ void StoreImpulsesKernel( StoreArgs a )
+B2G_DEV void StoreImpulsesBody( const StoreArgs& a, int i )
 {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( i >= a.totalCount )
 	{
 		return;
@@ -1025,9 +1063,8 @@ struct FinalizeArgs
 };
 
 // Body finalization (reference src/solver.c:553-743).
-__global__ void FinalizeBodiesKernel( FinalizeArgs a )
+B2G_DEV void FinalizeBodiesBody( const FinalizeArgs& a, int simIndex )
 {
-	int simIndex = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( simIndex >= a.count )
 	{
 		return;
@@ -1289,9 +1326,8 @@ B2G_DEV void EnlargeShape( const ContinuousArgs& a, int shapeId, ShapeDev* shape
 	}
 }
 
-__global__ void ContinuousKernel( ContinuousArgs a )
+B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 {
-	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( k >= *a.fastCount )
 	{
 		return;
@@ -1469,6 +1505,77 @@ __global__ void ContinuousKernel( ContinuousArgs a )
 			EnlargeShape( a, shapeId, a.shapes + shapeId );
 		}
 	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Lane kernels: each per-item stage body runs either as a single-world launch or as one launch
+// over many worlds, where a block table maps every block to (world, block within the world).
+
+struct CollideOp
+{
+	using Args = CollideArgs;
+	static B2G_DEV void Run( const Args& a, int i )
+	{
+		CollideBody( a, i );
+	}
+};
+
+struct PrepareContactsOp
+{
+	using Args = PrepareArgs;
+	static B2G_DEV void Run( const Args& a, int i )
+	{
+		PrepareContactsBody( a, i );
+	}
+};
+
+struct PrepareJointsOp
+{
+	using Args = PrepareJointArgs;
+	static B2G_DEV void Run( const Args& a, int i )
+	{
+		PrepareJointsBody( a, i );
+	}
+};
+
+struct StoreImpulsesOp
+{
+	using Args = StoreArgs;
+	static B2G_DEV void Run( const Args& a, int i )
+	{
+		StoreImpulsesBody( a, i );
+	}
+};
+
+struct FinalizeBodiesOp
+{
+	using Args = FinalizeArgs;
+	static B2G_DEV void Run( const Args& a, int i )
+	{
+		FinalizeBodiesBody( a, i );
+	}
+};
+
+struct ContinuousOp
+{
+	using Args = ContinuousArgs;
+	static B2G_DEV void Run( const Args& a, int i )
+	{
+		ContinuousBody( a, i );
+	}
+};
+
+template <typename Op>
+__global__ void LaneKernel( typename Op::Args a )
+{
+	Op::Run( a, blockIdx.x * blockDim.x + threadIdx.x );
+}
+
+template <typename Op>
+__global__ void LaneBatchKernel( const typename Op::Args* args, const int2* blocks )
+{
+	int2 b = blocks[blockIdx.x];
+	Op::Run( args[b.x], b.y * blockDim.x + threadIdx.x );
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1810,8 +1917,23 @@ struct DeviceWorld::Impl
 	}
 };
 
+// Allow the fused solver kernels the whole 160 KB of LDS for dynamic allocation (once per process).
+static void EnableLargeLds()
+{
+	static bool done = false;
+	if ( done == false )
+	{
+		HIP_CHECK( hipFuncSetAttribute( reinterpret_cast<const void*>( SolveFusedKernel ),
+										hipFuncAttributeMaxDynamicSharedMemorySize, FusedLdsBytes( kFusedBodyCapacity ) ) );
+		HIP_CHECK( hipFuncSetAttribute( reinterpret_cast<const void*>( SolveFusedBatchKernel ),
+										hipFuncAttributeMaxDynamicSharedMemorySize, FusedLdsBytes( kFusedBodyCapacity ) ) );
+		done = true;
+	}
+}
+
 DeviceWorld::DeviceWorld()
 {
+	EnableLargeLds();
 	impl = new Impl();
 	HIP_CHECK( hipStreamCreateWithFlags( &impl->stream, hipStreamNonBlocking ) );
 	HIP_CHECK( hipMalloc( &impl->counters, 16 * sizeof( int ) ) );
@@ -2037,7 +2159,7 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 		impl->slotStaging.assign( std::max( slots, 1 ), kStaticSlot );
 		for ( size_t k = 0; k < lists.awakeBodies.size(); ++k )
 		{
-			impl->slotStaging[lists.awakeBodies[k]] = (int)std::min<size_t>( k, kStaticSlot );
+			impl->slotStaging[lists.awakeBodies[k]] = (int)std::min<size_t>( k + 1, kFusedBodyCapacity );
 		}
 		UploadVector( impl->bodySlot, impl->slotStaging, impl->stream );
 	}
@@ -2086,7 +2208,7 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 	a.counters = impl->counters;
 	a.recycleDistance = params.contactRecycleDistance;
 	// Bytes are fixed up after the launch once the recycled count is known.
-	impl->Launch( kkCollide, 0.0, [&]() { CollideKernel<<<Blocks( n ), kBlock, 0, s>>>( a ); } );
+	impl->Launch( kkCollide, 0.0, [&]() { LaneKernel<CollideOp><<<Blocks( n ), kBlock, 0, s>>>( a ); } );
 	HIP_CHECK( hipEventRecord( impl->ev[1], s ) );
 
 	PackCollideKernel<<<1, kPackThreads, 0, s>>>( impl->counters, impl->changes.ptr, impl->mail );
@@ -2132,7 +2254,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		pa.aux = impl->aux.ptr;
 		pa.bodySlot = impl->bodySlot.ptr;
 		pa.p = params;
-		impl->Launch( kkPrepareContacts, kBytesPrepare * nc, [&]() { PrepareContactsKernel<<<Blocks( nc ), kBlock, 0, s>>>( pa ); } );
+		impl->Launch( kkPrepareContacts, kBytesPrepare * nc, [&]() { LaneKernel<PrepareContactsOp><<<Blocks( nc ), kBlock, 0, s>>>( pa ); } );
 	}
 
 	if ( nj > 0 )
@@ -2145,7 +2267,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		ja.aux = impl->aux.ptr;
 		ja.bodySlot = impl->bodySlot.ptr;
 		ja.p = params;
-		impl->Launch( kkPrepareJoints, kBytesJoint * nj, [&]() { PrepareJointsKernel<<<Blocks( nj ), kBlock, 0, s>>>( ja ); } );
+		impl->Launch( kkPrepareJoints, kBytesJoint * nj, [&]() { LaneKernel<PrepareJointsOp><<<Blocks( nj ), kBlock, 0, s>>>( ja ); } );
 	}
 
 	BodyArgs ba;
@@ -2285,7 +2407,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		// constraint fields per substep (warm start, solve, relax) + body sims per substep + state in/out
 		double bytes = params.subStepCount * ( kBytesFusedContactSubstep * nc + kBytesFusedBodySubstep * nb + kBytesJoint * nj ) +
 					   kBytesFusedBodyStep * nb;
-		impl->Launch( kkSolveFused, bytes, [&]() { SolveFusedKernel<<<1, kFusedThreads, 0, s>>>( fa ); } );
+		impl->Launch( kkSolveFused, bytes, [&]() { SolveFusedKernel<<<1, kFusedThreads, FusedLdsBytes( nb ), s>>>( fa ); } );
 	}
 	else
 	{
@@ -2313,7 +2435,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		sa.contacts = impl->contacts.ptr;
 		sa.coloredCount = L.overflowContactStart;
 		sa.totalCount = nc;
-		impl->Launch( kkStore, kBytesStore * nc, [&]() { StoreImpulsesKernel<<<Blocks( nc ), kBlock, 0, s>>>( sa ); } );
+		impl->Launch( kkStore, kBytesStore * nc, [&]() { LaneKernel<StoreImpulsesOp><<<Blocks( nc ), kBlock, 0, s>>>( sa ); } );
 	}
 
 	FinalizeArgs fa;
@@ -2330,7 +2452,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 	fa.enlargedShapes = impl->enlargedShapes.ptr;
 	fa.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 	fa.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
-	impl->Launch( kkFinalize, kBytesFinalize * nb, [&]() { FinalizeBodiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( fa ); } );
+	impl->Launch( kkFinalize, kBytesFinalize * nb, [&]() { LaneKernel<FinalizeBodiesOp><<<Blocks( nb ), kBlock, 0, s>>>( fa ); } );
 
 	if ( params.enableContinuous )
 	{
@@ -2350,7 +2472,7 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		ka.enlargedShapes = impl->enlargedShapes.ptr;
 		ka.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 		ka.huge = B2_HUGE;
-		impl->Launch( kkContinuous, 0.0, [&]() { ContinuousKernel<<<Blocks( nb ), kBlock, 0, s>>>( ka ); } );
+		impl->Launch( kkContinuous, 0.0, [&]() { LaneKernel<ContinuousOp><<<Blocks( nb ), kBlock, 0, s>>>( ka ); } );
 	}
 	HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
 

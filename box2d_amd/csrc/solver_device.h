@@ -26,10 +26,10 @@ struct ConstraintSoA
 	int* slotB;
 };
 
-// LDS body slots of the fused solver: awake bodies occupy [0, count); every static body reads
-// the identity state kept in slot kFusedBodyCapacity.
+// LDS body slots of the fused solver: every static body reads the identity state kept in slot 0,
+// awake bodies occupy slots 1..count (at most kFusedBodyCapacity).
 constexpr int kFusedBodyCapacity = 5400;
-constexpr int kStaticSlot = kFusedBodyCapacity;
+constexpr int kStaticSlot = 0;
 
 B2G_DEVI float MaxW( float a, float b )
 {
