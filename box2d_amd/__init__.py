@@ -27,7 +27,34 @@ def library():
     return _lib
 
 
-from .scenes_api import SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HELLO, SCENE_MIXED_PILE, SCENE_RAIN  # noqa: E402
+from .scenes_api import (SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HELLO, SCENE_MIXED_PILE,  # noqa: E402
+                         SCENE_RAIN, SCENE_BATCH_WORLD, WorldId)
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("solveKernelMs", ctypes.c_float), ("solveThreads", ctypes.c_int), ("solveGrid", ctypes.c_int),
+                ("worlds", ctypes.c_int)]
+
+
+def step_worlds(world_ids, count=1, dt=1.0 / 60.0, substeps=4):
+    """Step the worlds (a ctypes array of WorldId) together count times with b2amd_StepWorlds.
+    Returns the BatchStats of the last step."""
+    lib = library()
+    stats = BatchStats()
+    for _ in range(count):
+        lib.b2amd_StepWorlds(world_ids, len(world_ids), ctypes.c_float(dt), substeps, ctypes.byref(stats))
+    return stats
+
+
+def batch_world_specs(count, lo=8, hi=1024, seed=12345):
+    """Sizes log-uniform in [lo, hi], a world kind per world (0 pyramid, 1 pile, 2 chains) and a
+    seed per world, all from one seeded generator."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(int)
+    kinds = rng.integers(0, 3, count)
+    return [(int(s), int(k), int(seed + 1 + i)) for i, (s, k) in enumerate(zip(sizes, kinds))]
 
 
 SOLVER_AUTO = 0

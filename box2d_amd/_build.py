@@ -39,7 +39,7 @@ HIP_FLAGS = [
 COMMON = ["-O3", "-fPIC", "-fvisibility=hidden", "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-function",
           "-Wno-missing-braces"]
 
-HOST_SOURCES = ["world.cpp", "tree.cpp", "geometry.cpp", "capi.cpp"]
+HOST_SOURCES = ["world.cpp", "tree.cpp", "geometry.cpp", "capi.cpp", "batch.cpp"]
 HIP_SOURCES = ["device.hip"]
 
 
@@ -76,7 +76,7 @@ def build(verbose=True):
         objs = list(ex.map(lambda j: _compile(*j), jobs))
 
     lib = os.path.join(LIB, "libbox2d_amd.so")
-    _run([_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", lib] + objs)
+    _run([_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", lib] + objs + ["-pthread"])
 
     scenes = os.path.join(LIB, "libb2amd_scenes.so")
     _run(["gcc", "-std=gnu17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-I", INCLUDE,

@@ -653,6 +653,30 @@ int b2amd_ShapeDistanceBatch( const b2DistanceInput* inputs, b2SimplexCache* cac
 	return ShapeDistanceBatch( inputs, caches, count, outputs );
 }
 
+int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int subStepCount, b2amd_BatchStats* stats )
+{
+	std::vector<World*> worlds;
+	worlds.reserve( count );
+	for ( int i = 0; i < count; ++i )
+	{
+		World* world = GetWorld( worldIds[i] );
+		if ( world != nullptr )
+		{
+			worlds.push_back( world );
+		}
+	}
+	BatchStats bs = {};
+	int n = StepWorlds( worlds.data(), (int)worlds.size(), timeStep, subStepCount, &bs );
+	if ( stats != nullptr )
+	{
+		stats->solveKernelMs = bs.solveKernelMs;
+		stats->solveThreads = bs.solveThreads;
+		stats->solveGrid = bs.solveGrid;
+		stats->worlds = bs.worlds;
+	}
+	return n;
+}
+
 void b2amd_Synchronize( void )
 {
 	DeviceSynchronize();

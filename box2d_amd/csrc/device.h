@@ -66,9 +66,18 @@ public:
 	void DownloadJoints( World& world );
 	void DownloadContact( int contactId, ContactSim* out );
 
-	// Stages
+	// Stages. Each runs as a launch half (enqueue, or record when recording for a batch) and a
+	// finish half (wait for the stream, read the results back).
 	void Collide( const StepParams& params, CollideResult* result );
 	void Solve( const StepParams& params, bool anyRestitution, bool needSleepTimes, FinalizeResult* result );
+	void CollideLaunch( const StepParams& params );
+	void CollideFinish( CollideResult* result );
+	void SolveLaunch( const StepParams& params, bool anyRestitution, bool needSleepTimes );
+	void SolveFinish( FinalizeResult* result );
+
+	// Record instead of launching (the batch that steps many worlds launches the records).
+	void SetRecording( bool flag );
+	void* Record();
 
 	// Per-stage GPU timings of the last step in milliseconds (collide, solve, finalize).
 	float lastCollideMs = 0.0f;
@@ -84,10 +93,49 @@ public:
 
 	void* stream = nullptr;
 
-private:
 	struct Impl;
+
+private:
 	Impl* impl;
 };
+
+// Steps many worlds together: the worlds record their stage arguments (DeviceWorld recording
+// mode) and the batch launches one kernel per stage over all of them on its own stream. The
+// substep loop runs as one persistent launch whose workgroups take worlds from a queue.
+class Batch
+{
+public:
+	Batch();
+	~Batch();
+
+	// Launch the recorded collide stage of the worlds and wait for it.
+	void Collide( DeviceWorld** worlds, int count );
+	// Launch the recorded solve stages (prepare .. finalize, continuous, readback) and wait.
+	void Solve( DeviceWorld** worlds, int count );
+
+	// Workgroup size of the persistent solver (0 while the autotuner is still trying sizes).
+	int FusedThreads() const;
+
+	float lastFusedMs = 0.0f;
+	int lastFusedThreads = 0;
+	int lastFusedGrid = 0;
+
+	struct Impl;
+
+private:
+	Impl* impl;
+};
+
+struct BatchStats
+{
+	float solveKernelMs; // persistent substep-loop launch of the last batch step
+	int solveThreads;	 // its workgroup size
+	int solveGrid;		 // its workgroup count
+	int worlds;			 // worlds stepped
+};
+
+// Step count worlds as one batch (defined in batch.cpp). Returns the number of worlds stepped.
+int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, BatchStats* stats );
 
 // Last HIP error text (empty if none) for diagnostics.
 const char* LastDeviceError();

@@ -1634,22 +1634,40 @@ constexpr double kBytesFusedBodyStep = 64.0;
 constexpr int kMailHeader = 8;
 constexpr int kPackThreads = 1024;
 
-__global__ __launch_bounds__( kPackThreads ) void PackCollideKernel( int* counters, const int* changes, int* mail )
+struct PackCollideArgs
 {
-	int count = counters[0];
-	int recycled = counters[1];
-	for ( int i = threadIdx.x; i < count; i += kPackThreads )
+	int* counters;
+	const int* changes;
+	int* mail;
+};
+
+B2G_DEV void PackCollideBody( const PackCollideArgs& a )
+{
+	int count = a.counters[0];
+	int recycled = a.counters[1];
+	for ( int i = threadIdx.x; i < count; i += blockDim.x )
 	{
-		mail[kMailHeader + i] = changes[i];
+		a.mail[kMailHeader + i] = a.changes[i];
 	}
 	__syncthreads();
 	if ( threadIdx.x == 0 )
 	{
-		mail[0] = count;
-		mail[1] = recycled;
-		counters[0] = 0;
-		counters[1] = 0;
+		a.mail[0] = count;
+		a.mail[1] = recycled;
+		a.counters[0] = 0;
+		a.counters[1] = 0;
 	}
+}
+
+__global__ __launch_bounds__( kPackThreads ) void PackCollideKernel( PackCollideArgs a )
+{
+	PackCollideBody( a );
+}
+
+// one workgroup per world
+__global__ __launch_bounds__( kPackThreads ) void PackCollideBatchKernel( const PackCollideArgs* args )
+{
+	PackCollideBody( args[blockIdx.x] );
 }
 
 struct PackFinalizeArgs
@@ -1666,17 +1684,17 @@ struct PackFinalizeArgs
 	int* mail;
 };
 
-__global__ __launch_bounds__( kPackThreads ) void PackFinalizeKernel( PackFinalizeArgs a )
+B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
 {
 	int bodies = a.counters[0];
 	int shapes = a.counters[1];
 	int fast = a.counters[2];
 	int error = a.counters[3];
-	for ( int i = threadIdx.x; i < bodies; i += kPackThreads )
+	for ( int i = threadIdx.x; i < bodies; i += blockDim.x )
 	{
 		a.mail[kMailHeader + i] = a.enlargedBodies[i];
 	}
-	for ( int i = threadIdx.x; i < shapes; i += kPackThreads )
+	for ( int i = threadIdx.x; i < shapes; i += blockDim.x )
 	{
 		a.mail[a.shapeOffset + i] = a.enlargedShapes[i];
 		b2AABB box = a.enlargedShapeAABBs[i];
@@ -1689,7 +1707,7 @@ __global__ __launch_bounds__( kPackThreads ) void PackFinalizeKernel( PackFinali
 	if ( a.sleepTimes != nullptr )
 	{
 		float* dst = reinterpret_cast<float*>( a.mail + a.sleepOffset );
-		for ( int i = threadIdx.x; i < a.bodyCount; i += kPackThreads )
+		for ( int i = threadIdx.x; i < a.bodyCount; i += blockDim.x )
 		{
 			dst[i] = a.sleepTimes[i];
 		}
@@ -1706,6 +1724,16 @@ __global__ __launch_bounds__( kPackThreads ) void PackFinalizeKernel( PackFinali
 		a.counters[2] = 0;
 		a.counters[3] = 0;
 	}
+}
+
+__global__ __launch_bounds__( kPackThreads ) void PackFinalizeKernel( PackFinalizeArgs a )
+{
+	PackFinalizeBody( a );
+}
+
+__global__ __launch_bounds__( kPackThreads ) void PackFinalizeBatchKernel( const PackFinalizeArgs* args )
+{
+	PackFinalizeBody( args[blockIdx.x] );
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1804,6 +1832,36 @@ struct DeviceWorld::Impl
 	// counters: [0..1] collide, [4..6] finalize
 	int* counters = nullptr;
 	int* hostCounters = nullptr;
+
+	// Batched stepping: instead of launching, the phase functions record each kernel's
+	// arguments here and a batch launches one kernel per stage over all worlds.
+	bool recording = false;
+	struct Record
+	{
+		bool collide = false, prepare = false, prepareJoints = false, fused = false, store = false, finalize = false,
+			 continuous = false, packCollide = false, packFinalize = false;
+		CollideArgs collideArgs;
+		int collideLanes = 0;
+		PackCollideArgs packCollideArgs;
+		PrepareArgs prepareArgs;
+		int prepareLanes = 0;
+		PrepareJointArgs prepareJointArgs;
+		int prepareJointLanes = 0;
+		FusedArgs fusedArgs;
+		StoreArgs storeArgs;
+		int storeLanes = 0;
+		FinalizeArgs finalizeArgs;
+		int finalizeLanes = 0;
+		ContinuousArgs continuousArgs;
+		int continuousLanes = 0;
+		PackFinalizeArgs packFinalizeArgs;
+	} rec;
+
+	// phase state kept from the launch half to the finish half of a stage
+	int collideCount = 0;
+	int solveBodyCount = 0;
+	bool solveNeedSleepTimes = false;
+	PackFinalizeArgs solvePack = {};
 
 	// pinned host mailbox written by the pack kernels
 	int* mail = nullptr;
@@ -2182,19 +2240,16 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
-void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
+void DeviceWorld::CollideLaunch( const StepParams& params )
 {
-	result->changes.clear();
-	result->recycledCount = 0;
-
 	int n = (int)impl->lists.collideContacts.size();
+	impl->collideCount = n;
 	if ( n == 0 )
 	{
 		return;
 	}
 
 	hipStream_t s = impl->stream;
-	HIP_CHECK( hipEventRecord( impl->ev[0], s ) );
 	impl->EnsureMail( kMailHeader + (size_t)n );
 
 	CollideArgs a;
@@ -2207,13 +2262,39 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 	a.changes = impl->changes.ptr;
 	a.counters = impl->counters;
 	a.recycleDistance = params.contactRecycleDistance;
+
+	PackCollideArgs pc = { impl->counters, impl->changes.ptr, impl->mail };
+
+	if ( impl->recording )
+	{
+		impl->rec.collide = true;
+		impl->rec.collideArgs = a;
+		impl->rec.collideLanes = n;
+		impl->rec.packCollide = true;
+		impl->rec.packCollideArgs = pc;
+		return;
+	}
+
+	HIP_CHECK( hipEventRecord( impl->ev[0], s ) );
 	// Bytes are fixed up after the launch once the recycled count is known.
 	impl->Launch( kkCollide, 0.0, [&]() { LaneKernel<CollideOp><<<Blocks( n ), kBlock, 0, s>>>( a ); } );
 	HIP_CHECK( hipEventRecord( impl->ev[1], s ) );
-
-	PackCollideKernel<<<1, kPackThreads, 0, s>>>( impl->counters, impl->changes.ptr, impl->mail );
+	PackCollideKernel<<<1, kPackThreads, 0, s>>>( pc );
 	HIP_CHECK( hipGetLastError() );
-	HIP_CHECK( hipStreamSynchronize( s ) );
+}
+
+void DeviceWorld::CollideFinish( CollideResult* result )
+{
+	result->changes.clear();
+	result->recycledCount = 0;
+
+	int n = impl->collideCount;
+	if ( n == 0 )
+	{
+		return;
+	}
+
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 
 	int changeCount = impl->mail[0];
 	result->recycledCount = impl->mail[1];
@@ -2230,18 +2311,31 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 		std::sort( result->changes.begin(), result->changes.end() );
 	}
 
-	HIP_CHECK( hipEventElapsedTime( &lastCollideMs, impl->ev[0], impl->ev[1] ) );
+	if ( impl->recording == false )
+	{
+		HIP_CHECK( hipEventElapsedTime( &lastCollideMs, impl->ev[0], impl->ev[1] ) );
+	}
 }
 
-void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool needSleepTimes, FinalizeResult* result )
+void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
+{
+	CollideLaunch( params );
+	CollideFinish( result );
+}
+
+void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bool needSleepTimes )
 {
 	const SolverLists& L = impl->lists;
 	hipStream_t s = impl->stream;
 	int nb = (int)L.awakeBodies.size();
 	int nc = (int)L.constraintContacts.size();
 	int nj = (int)L.jointList.size();
+	const bool rec = impl->recording;
 
-	HIP_CHECK( hipEventRecord( impl->ev[2], s ) );
+	if ( rec == false )
+	{
+		HIP_CHECK( hipEventRecord( impl->ev[2], s ) );
+	}
 
 	if ( nc > 0 )
 	{
@@ -2254,7 +2348,17 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		pa.aux = impl->aux.ptr;
 		pa.bodySlot = impl->bodySlot.ptr;
 		pa.p = params;
-		impl->Launch( kkPrepareContacts, kBytesPrepare * nc, [&]() { LaneKernel<PrepareContactsOp><<<Blocks( nc ), kBlock, 0, s>>>( pa ); } );
+		if ( rec )
+		{
+			impl->rec.prepare = true;
+			impl->rec.prepareArgs = pa;
+			impl->rec.prepareLanes = nc;
+		}
+		else
+		{
+			impl->Launch( kkPrepareContacts, kBytesPrepare * nc,
+						  [&]() { LaneKernel<PrepareContactsOp><<<Blocks( nc ), kBlock, 0, s>>>( pa ); } );
+		}
 	}
 
 	if ( nj > 0 )
@@ -2267,7 +2371,17 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		ja.aux = impl->aux.ptr;
 		ja.bodySlot = impl->bodySlot.ptr;
 		ja.p = params;
-		impl->Launch( kkPrepareJoints, kBytesJoint * nj, [&]() { LaneKernel<PrepareJointsOp><<<Blocks( nj ), kBlock, 0, s>>>( ja ); } );
+		if ( rec )
+		{
+			impl->rec.prepareJoints = true;
+			impl->rec.prepareJointArgs = ja;
+			impl->rec.prepareJointLanes = nj;
+		}
+		else
+		{
+			impl->Launch( kkPrepareJoints, kBytesJoint * nj,
+						  [&]() { LaneKernel<PrepareJointsOp><<<Blocks( nj ), kBlock, 0, s>>>( ja ); } );
+		}
 	}
 
 	BodyArgs ba;
@@ -2382,6 +2496,15 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 	// color kernels dominates, and loses once a world has enough constraints to need the whole GPU.
 	bool fits = nb > 0 && nb <= kFusedBodyCapacity && L.activeColorCount <= kMaxColorSlots;
 	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && nc <= kFusedMaxContacts ) );
+	if ( rec && nb > 0 && fits == false )
+	{
+		Fatal( "batched stepping needs every world's awake bodies in one workgroup's LDS (%d awake, at most %d)", nb,
+			   kFusedBodyCapacity );
+	}
+	if ( rec && nb > 0 )
+	{
+		fused = true;
+	}
 	if ( fused )
 	{
 		FusedArgs fa;
@@ -2407,7 +2530,15 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		// constraint fields per substep (warm start, solve, relax) + body sims per substep + state in/out
 		double bytes = params.subStepCount * ( kBytesFusedContactSubstep * nc + kBytesFusedBodySubstep * nb + kBytesJoint * nj ) +
 					   kBytesFusedBodyStep * nb;
-		impl->Launch( kkSolveFused, bytes, [&]() { SolveFusedKernel<<<1, kFusedThreads, FusedLdsBytes( nb ), s>>>( fa ); } );
+		if ( rec )
+		{
+			impl->rec.fused = true;
+			impl->rec.fusedArgs = fa;
+		}
+		else
+		{
+			impl->Launch( kkSolveFused, bytes, [&]() { SolveFusedKernel<<<1, kFusedThreads, FusedLdsBytes( nb ), s>>>( fa ); } );
+		}
 	}
 	else
 	{
@@ -2435,7 +2566,16 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		sa.contacts = impl->contacts.ptr;
 		sa.coloredCount = L.overflowContactStart;
 		sa.totalCount = nc;
-		impl->Launch( kkStore, kBytesStore * nc, [&]() { LaneKernel<StoreImpulsesOp><<<Blocks( nc ), kBlock, 0, s>>>( sa ); } );
+		if ( rec )
+		{
+			impl->rec.store = true;
+			impl->rec.storeArgs = sa;
+			impl->rec.storeLanes = nc;
+		}
+		else
+		{
+			impl->Launch( kkStore, kBytesStore * nc, [&]() { LaneKernel<StoreImpulsesOp><<<Blocks( nc ), kBlock, 0, s>>>( sa ); } );
+		}
 	}
 
 	FinalizeArgs fa;
@@ -2452,7 +2592,16 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 	fa.enlargedShapes = impl->enlargedShapes.ptr;
 	fa.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 	fa.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
-	impl->Launch( kkFinalize, kBytesFinalize * nb, [&]() { LaneKernel<FinalizeBodiesOp><<<Blocks( nb ), kBlock, 0, s>>>( fa ); } );
+	if ( rec )
+	{
+		impl->rec.finalize = true;
+		impl->rec.finalizeArgs = fa;
+		impl->rec.finalizeLanes = nb;
+	}
+	else
+	{
+		impl->Launch( kkFinalize, kBytesFinalize * nb, [&]() { LaneKernel<FinalizeBodiesOp><<<Blocks( nb ), kBlock, 0, s>>>( fa ); } );
+	}
 
 	if ( params.enableContinuous )
 	{
@@ -2472,9 +2621,21 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		ka.enlargedShapes = impl->enlargedShapes.ptr;
 		ka.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 		ka.huge = B2_HUGE;
-		impl->Launch( kkContinuous, 0.0, [&]() { LaneKernel<ContinuousOp><<<Blocks( nb ), kBlock, 0, s>>>( ka ); } );
+		if ( rec )
+		{
+			impl->rec.continuous = true;
+			impl->rec.continuousArgs = ka;
+			impl->rec.continuousLanes = nb;
+		}
+		else
+		{
+			impl->Launch( kkContinuous, 0.0, [&]() { LaneKernel<ContinuousOp><<<Blocks( nb ), kBlock, 0, s>>>( ka ); } );
+		}
 	}
-	HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
+	if ( rec == false )
+	{
+		HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
+	}
 
 	{
 		size_t shapeCap = std::max<size_t>( impl->shapeCount, 1 );
@@ -2490,9 +2651,29 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		pk.sleepOffset = pk.aabbOffset + 4 * (int)shapeCap;
 		impl->EnsureMail( (size_t)pk.sleepOffset + nb );
 		pk.mail = impl->mail;
-		PackFinalizeKernel<<<1, kPackThreads, 0, s>>>( pk );
-		HIP_CHECK( hipGetLastError() );
-		HIP_CHECK( hipStreamSynchronize( s ) );
+		impl->solveBodyCount = nb;
+		impl->solveNeedSleepTimes = needSleepTimes;
+		impl->solvePack = pk;
+		if ( rec )
+		{
+			impl->rec.packFinalize = true;
+			impl->rec.packFinalizeArgs = pk;
+		}
+		else
+		{
+			PackFinalizeKernel<<<1, kPackThreads, 0, s>>>( pk );
+			HIP_CHECK( hipGetLastError() );
+		}
+	}
+}
+
+void DeviceWorld::SolveFinish( FinalizeResult* result )
+{
+	{
+		int nb = impl->solveBodyCount;
+		bool needSleepTimes = impl->solveNeedSleepTimes;
+		const PackFinalizeArgs& pk = impl->solvePack;
+		HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 		impl->FlushTiming();
 
 		const int* mail = impl->mail;
@@ -2518,7 +2699,16 @@ void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool nee
 		std::sort( result->enlargedBodies.begin(), result->enlargedBodies.end() );
 	}
 
-	HIP_CHECK( hipEventElapsedTime( &lastSolveMs, impl->ev[2], impl->ev[3] ) );
+	if ( impl->recording == false )
+	{
+		HIP_CHECK( hipEventElapsedTime( &lastSolveMs, impl->ev[2], impl->ev[3] ) );
+	}
+}
+
+void DeviceWorld::Solve( const StepParams& params, bool anyRestitution, bool needSleepTimes, FinalizeResult* result )
+{
+	SolveLaunch( params, anyRestitution, needSleepTimes );
+	SolveFinish( result );
 }
 
 void DeviceWorld::EnableKernelTiming( bool flag )
@@ -2847,3 +3037,341 @@ B2_API b2LocalManifold b2CollideChainSegmentAndPolygon( const b2ChainSegment* se
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Batched stepping of many worlds: one launch per stage over every world's recorded arguments.
+
+namespace b2g
+{
+
+void DeviceWorld::SetRecording( bool flag )
+{
+	impl->recording = flag;
+	impl->rec = Impl::Record{};
+}
+
+void* DeviceWorld::Record()
+{
+	return &impl->rec;
+}
+
+struct Batch::Impl
+{
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	char* staging = nullptr; // pinned
+	size_t stagingCapacity = 0;
+	DevArray<char> buffer;
+	int* queue = nullptr;
+	int cuCount = 0;
+
+	// autotuned workgroup size of the persistent solver
+	int fusedThreads = 0;
+	int tuneIndex = 0;
+	float tuneMs[3] = {};
+
+	size_t used = 0;
+
+	void EnsureStaging( size_t bytes )
+	{
+		if ( bytes <= stagingCapacity )
+		{
+			return;
+		}
+		size_t cap = std::max( bytes, stagingCapacity + stagingCapacity / 2 );
+		if ( staging != nullptr )
+		{
+			HIP_CHECK( hipHostFree( staging ) );
+		}
+		HIP_CHECK( hipHostMalloc( (void**)&staging, cap, hipHostMallocDefault ) );
+		stagingCapacity = cap;
+	}
+
+	// Append a block of host data to the staging area (16-byte aligned); returns its offset.
+	size_t Put( const void* data, size_t bytes )
+	{
+		size_t offset = ( used + 15 ) & ~size_t( 15 );
+		memcpy( staging + offset, data, bytes );
+		used = offset + bytes;
+		return offset;
+	}
+};
+
+Batch::Batch()
+{
+	impl = new Impl();
+	HIP_CHECK( hipStreamCreateWithFlags( &impl->stream, hipStreamNonBlocking ) );
+	HIP_CHECK( hipEventCreate( &impl->ev0 ) );
+	HIP_CHECK( hipEventCreate( &impl->ev1 ) );
+	HIP_CHECK( hipMalloc( &impl->queue, sizeof( int ) ) );
+	int device = 0;
+	HIP_CHECK( hipGetDevice( &device ) );
+	HIP_CHECK( hipDeviceGetAttribute( &impl->cuCount, hipDeviceAttributeMultiprocessorCount, device ) );
+	const char* v = getenv( "B2AMD_BATCH_THREADS" );
+	if ( v != nullptr )
+	{
+		int t = atoi( v );
+		impl->fusedThreads = ( t == 256 || t == 512 || t == 1024 ) ? t : 0;
+	}
+}
+
+Batch::~Batch()
+{
+	(void)hipStreamSynchronize( impl->stream );
+	impl->buffer.Free();
+	(void)hipFree( impl->queue );
+	if ( impl->staging != nullptr )
+	{
+		(void)hipHostFree( impl->staging );
+	}
+	(void)hipEventDestroy( impl->ev0 );
+	(void)hipEventDestroy( impl->ev1 );
+	(void)hipStreamDestroy( impl->stream );
+	delete impl;
+}
+
+int Batch::FusedThreads() const
+{
+	return impl->fusedThreads;
+}
+
+static const int kBatchThreadCandidates[3] = { 256, 512, 1024 };
+
+// Per-stage lists gathered from the worlds' records.
+template <typename Args>
+struct StageList
+{
+	std::vector<Args> args;
+	std::vector<int> lanes;
+};
+
+// Upload a lane stage (arguments and block table) into the staging area; returns the offsets.
+template <typename Args>
+static void StageLane( Batch::Impl* b, const StageList<Args>& list, size_t* argsOffset, size_t* blocksOffset, int* blockCount )
+{
+	std::vector<int2> blocks;
+	for ( size_t w = 0; w < list.args.size(); ++w )
+	{
+		int n = ( list.lanes[w] + kBlock - 1 ) / kBlock;
+		for ( int k = 0; k < n; ++k )
+		{
+			blocks.push_back( int2{ (int)w, k } );
+		}
+	}
+	*argsOffset = b->Put( list.args.data(), list.args.size() * sizeof( Args ) );
+	*blocksOffset = b->Put( blocks.data(), blocks.size() * sizeof( int2 ) );
+	*blockCount = (int)blocks.size();
+}
+
+// Staging bytes of a lane stage: arguments, block table, alignment slack.
+template <typename Args>
+static size_t ListBytes( const StageList<Args>& list )
+{
+	size_t blocks = 0;
+	for ( int n : list.lanes )
+	{
+		blocks += ( n + kBlock - 1 ) / kBlock;
+	}
+	return list.args.size() * sizeof( Args ) + blocks * sizeof( int2 ) + 32;
+}
+
+void Batch::Collide( DeviceWorld** worlds, int count )
+{
+	StageList<CollideArgs> collide;
+	std::vector<PackCollideArgs> pack;
+	for ( int i = 0; i < count; ++i )
+	{
+		auto* r = static_cast<DeviceWorld::Impl::Record*>( worlds[i]->Record() );
+		if ( r->collide )
+		{
+			collide.args.push_back( r->collideArgs );
+			collide.lanes.push_back( r->collideLanes );
+			pack.push_back( r->packCollideArgs );
+		}
+	}
+	if ( collide.args.empty() )
+	{
+		return;
+	}
+
+	Impl* b = impl;
+	b->used = 0;
+	b->EnsureStaging( ListBytes( collide ) + pack.size() * sizeof( PackCollideArgs ) + 64 );
+	size_t argsOff, blocksOff, packOff;
+	int blockCount;
+	StageLane( b, collide, &argsOff, &blocksOff, &blockCount );
+	packOff = b->Put( pack.data(), pack.size() * sizeof( PackCollideArgs ) );
+
+	b->buffer.Reserve( b->used, false, b->stream, 0 );
+	HIP_CHECK( hipMemcpyAsync( b->buffer.ptr, b->staging, b->used, hipMemcpyHostToDevice, b->stream ) );
+	char* base = b->buffer.ptr;
+	LaneBatchKernel<CollideOp><<<blockCount, kBlock, 0, b->stream>>>( reinterpret_cast<const CollideArgs*>( base + argsOff ),
+																	reinterpret_cast<const int2*>( base + blocksOff ) );
+	HIP_CHECK( hipGetLastError() );
+	PackCollideBatchKernel<<<(int)pack.size(), kPackThreads, 0, b->stream>>>( reinterpret_cast<const PackCollideArgs*>( base + packOff ) );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipStreamSynchronize( b->stream ) );
+}
+
+void Batch::Solve( DeviceWorld** worlds, int count )
+{
+	StageList<PrepareArgs> prepare;
+	StageList<PrepareJointArgs> prepareJoints;
+	std::vector<FusedArgs> fused;
+	std::vector<int> fusedBodies;
+	StageList<StoreArgs> store;
+	StageList<FinalizeArgs> finalize;
+	StageList<ContinuousArgs> continuous;
+	std::vector<PackFinalizeArgs> pack;
+
+	for ( int i = 0; i < count; ++i )
+	{
+		auto* r = static_cast<DeviceWorld::Impl::Record*>( worlds[i]->Record() );
+		if ( r->prepare )
+		{
+			prepare.args.push_back( r->prepareArgs );
+			prepare.lanes.push_back( r->prepareLanes );
+		}
+		if ( r->prepareJoints )
+		{
+			prepareJoints.args.push_back( r->prepareJointArgs );
+			prepareJoints.lanes.push_back( r->prepareJointLanes );
+		}
+		if ( r->fused )
+		{
+			fused.push_back( r->fusedArgs );
+			fusedBodies.push_back( r->fusedArgs.bodyCount );
+		}
+		if ( r->store )
+		{
+			store.args.push_back( r->storeArgs );
+			store.lanes.push_back( r->storeLanes );
+		}
+		if ( r->finalize )
+		{
+			finalize.args.push_back( r->finalizeArgs );
+			finalize.lanes.push_back( r->finalizeLanes );
+		}
+		if ( r->continuous )
+		{
+			continuous.args.push_back( r->continuousArgs );
+			continuous.lanes.push_back( r->continuousLanes );
+		}
+		if ( r->packFinalize )
+		{
+			pack.push_back( r->packFinalizeArgs );
+		}
+	}
+
+	Impl* b = impl;
+	b->used = 0;
+	size_t n = (size_t)count;
+	b->EnsureStaging( ListBytes( prepare ) + ListBytes( prepareJoints ) + ListBytes( store ) + ListBytes( finalize ) +
+					  ListBytes( continuous ) + n * ( sizeof( FusedArgs ) + sizeof( int ) + sizeof( PackFinalizeArgs ) ) + 256 );
+
+	size_t prepOff = 0, prepBlocksOff = 0, jointOff = 0, jointBlocksOff = 0, storeOff = 0, storeBlocksOff = 0;
+	size_t finOff = 0, finBlocksOff = 0, ccdOff = 0, ccdBlocksOff = 0, fusedOff = 0, orderOff = 0, packOff = 0;
+	int prepBlocks = 0, jointBlocks = 0, storeBlocks = 0, finBlocks = 0, ccdBlocks = 0;
+	StageLane( b, prepare, &prepOff, &prepBlocksOff, &prepBlocks );
+	StageLane( b, prepareJoints, &jointOff, &jointBlocksOff, &jointBlocks );
+	StageLane( b, store, &storeOff, &storeBlocksOff, &storeBlocks );
+	StageLane( b, finalize, &finOff, &finBlocksOff, &finBlocks );
+	StageLane( b, continuous, &ccdOff, &ccdBlocksOff, &ccdBlocks );
+
+	// largest worlds first, so the persistent solver's queue ends with the short jobs
+	std::vector<int> order( fused.size() );
+	for ( size_t i = 0; i < order.size(); ++i )
+	{
+		order[i] = (int)i;
+	}
+	std::stable_sort( order.begin(), order.end(), [&]( int x, int y ) { return fusedBodies[x] > fusedBodies[y]; } );
+	int maxBodies = 0;
+	for ( int nb : fusedBodies )
+	{
+		maxBodies = std::max( maxBodies, nb );
+	}
+	fusedOff = b->Put( fused.data(), fused.size() * sizeof( FusedArgs ) );
+	orderOff = b->Put( order.data(), order.size() * sizeof( int ) );
+	packOff = b->Put( pack.data(), pack.size() * sizeof( PackFinalizeArgs ) );
+
+	b->buffer.Reserve( b->used, false, b->stream, 0 );
+	hipStream_t s = b->stream;
+	HIP_CHECK( hipMemcpyAsync( b->buffer.ptr, b->staging, b->used, hipMemcpyHostToDevice, s ) );
+	char* base = b->buffer.ptr;
+
+	if ( prepBlocks > 0 )
+	{
+		LaneBatchKernel<PrepareContactsOp><<<prepBlocks, kBlock, 0, s>>>( reinterpret_cast<const PrepareArgs*>( base + prepOff ),
+																	   reinterpret_cast<const int2*>( base + prepBlocksOff ) );
+	}
+	if ( jointBlocks > 0 )
+	{
+		LaneBatchKernel<PrepareJointsOp><<<jointBlocks, kBlock, 0, s>>>( reinterpret_cast<const PrepareJointArgs*>( base + jointOff ),
+																	  reinterpret_cast<const int2*>( base + jointBlocksOff ) );
+	}
+
+	if ( fused.empty() == false )
+	{
+		// Autotune the workgroup size over the first batches: try 256, 512 and 1024 threads once
+		// each and keep the fastest. The choice does not change any result.
+		int threads = b->fusedThreads != 0 ? b->fusedThreads : kBatchThreadCandidates[b->tuneIndex];
+		int lds = FusedLdsBytes( maxBodies );
+		int perCu = 0;
+		HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, SolveFusedBatchKernel, threads, lds ) );
+		int grid = std::max( 1, std::min( (int)fused.size(), perCu * b->cuCount ) );
+		HIP_CHECK( hipMemsetAsync( b->queue, 0, sizeof( int ), s ) );
+		HIP_CHECK( hipEventRecord( b->ev0, s ) );
+		SolveFusedBatchKernel<<<grid, threads, lds, s>>>( reinterpret_cast<const FusedArgs*>( base + fusedOff ),
+														  reinterpret_cast<const int*>( base + orderOff ), (int)fused.size(),
+														  b->queue );
+		HIP_CHECK( hipGetLastError() );
+		HIP_CHECK( hipEventRecord( b->ev1, s ) );
+		lastFusedThreads = threads;
+		lastFusedGrid = grid;
+	}
+
+	if ( storeBlocks > 0 )
+	{
+		LaneBatchKernel<StoreImpulsesOp><<<storeBlocks, kBlock, 0, s>>>( reinterpret_cast<const StoreArgs*>( base + storeOff ),
+																	  reinterpret_cast<const int2*>( base + storeBlocksOff ) );
+	}
+	if ( finBlocks > 0 )
+	{
+		LaneBatchKernel<FinalizeBodiesOp><<<finBlocks, kBlock, 0, s>>>( reinterpret_cast<const FinalizeArgs*>( base + finOff ),
+																	 reinterpret_cast<const int2*>( base + finBlocksOff ) );
+	}
+	if ( ccdBlocks > 0 )
+	{
+		LaneBatchKernel<ContinuousOp><<<ccdBlocks, kBlock, 0, s>>>( reinterpret_cast<const ContinuousArgs*>( base + ccdOff ),
+																 reinterpret_cast<const int2*>( base + ccdBlocksOff ) );
+	}
+	if ( pack.empty() == false )
+	{
+		PackFinalizeBatchKernel<<<(int)pack.size(), kPackThreads, 0, s>>>( reinterpret_cast<const PackFinalizeArgs*>( base + packOff ) );
+	}
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+
+	if ( fused.empty() == false )
+	{
+		float ms = 0.0f;
+		HIP_CHECK( hipEventElapsedTime( &ms, b->ev0, b->ev1 ) );
+		lastFusedMs = ms;
+		if ( b->fusedThreads == 0 )
+		{
+			b->tuneMs[b->tuneIndex] = ms;
+			b->tuneIndex += 1;
+			if ( b->tuneIndex == 3 )
+			{
+				int best = 0;
+				for ( int k = 1; k < 3; ++k )
+				{
+					best = b->tuneMs[k] < b->tuneMs[best] ? k : best;
+				}
+				b->fusedThreads = kBatchThreadCandidates[best];
+			}
+		}
+	}
+}
+
+} // namespace b2g

@@ -2362,9 +2362,20 @@ static void BuildSolverLists( World& w, SolverLists& L )
 
 void World::Step( float timeStep, int subStepCount )
 {
-	if ( locked )
+	if ( StepBegin( timeStep, subStepCount ) == false )
 	{
 		return;
+	}
+	StepMiddle();
+	StepEnd();
+}
+
+// Step phase 1: pairs, uploads, collide launch. Returns false when the world is locked.
+bool World::StepBegin( float timeStep, int subStepCount )
+{
+	if ( locked )
+	{
+		return false;
 	}
 
 	moveEvents.clear();
@@ -2374,7 +2385,7 @@ void World::Step( float timeStep, int subStepCount )
 	recycledContactCount = 0;
 
 	locked = true;
-	uint64_t stepTicks = b2GetTicks();
+	stepTicks = b2GetTicks();
 
 	{
 		uint64_t pairTicks = b2GetTicks();
@@ -2433,30 +2444,46 @@ void World::Step( float timeStep, int subStepCount )
 		listsDirty = false;
 	}
 
-	{
-		uint64_t collideTicks = b2GetTicks();
-		CollideResult cr;
-		device->Collide( params, &cr );
-		recycledContactCount = cr.recycledCount;
-		if ( cr.changes.empty() == false )
-		{
-			ProcessContactStates( cr.changes.data(), (int)cr.changes.size() );
-		}
-		profile.collide = b2GetMilliseconds( collideTicks );
-	}
+	stepParams = params;
+	collideTicks = b2GetTicks();
+	device->CollideLaunch( params );
+	return true;
+}
 
-	if ( timeStep > 0.0f )
+// Step phase 2: collide results, touch processing, solve launch.
+void World::StepMiddle()
+{
+	CollideResult cr;
+	device->CollideFinish( &cr );
+	recycledContactCount = cr.recycledCount;
+	if ( cr.changes.empty() == false )
 	{
-		uint64_t solveTicks = b2GetTicks();
-		Solve( params );
+		ProcessContactStates( cr.changes.data(), (int)cr.changes.size() );
+	}
+	profile.collide = b2GetMilliseconds( collideTicks );
+
+	solvePending = false;
+	if ( stepParams.dt > 0.0f )
+	{
+		solveTicks = b2GetTicks();
+		solvePending = SolveBegin( stepParams );
+	}
+}
+
+// Step phase 3: solve results, refit, sleep.
+void World::StepEnd()
+{
+	if ( solvePending )
+	{
+		SolveEnd();
 		profile.solve = b2GetMilliseconds( solveTicks );
+		solvePending = false;
 	}
-
 	profile.step = b2GetMilliseconds( stepTicks );
 	locked = false;
 }
 
-void World::Solve( StepParams& params )
+bool World::SolveBegin( StepParams& params )
 {
 	stepIndex += 1;
 
@@ -2464,7 +2491,7 @@ void World::Solve( StepParams& params )
 	int awakeBodyCount = (int)awakeSet.bodyIds.size();
 	if ( awakeBodyCount == 0 )
 	{
-		return;
+		return false;
 	}
 
 	if ( splitIslandId != B2_NULL_INDEX )
@@ -2484,9 +2511,18 @@ void World::Solve( StepParams& params )
 		listsDirty = false;
 	}
 
+	constraintTicks = b2GetTicks();
+	device->SolveLaunch( params, restitutionShapeCount > 0, enableSleep );
+	return true;
+}
+
+void World::SolveEnd()
+{
+	SolverSet& awakeSet = solverSets[kAwakeSet];
+	int awakeBodyCount = (int)awakeSet.bodyIds.size();
+
 	FinalizeResult fr;
-	uint64_t constraintTicks = b2GetTicks();
-	device->Solve( params, restitutionShapeCount > 0, enableSleep, &fr );
+	device->SolveFinish( &fr );
 	profile.constraints = b2GetMilliseconds( constraintTicks );
 
 	bodiesStale = true;

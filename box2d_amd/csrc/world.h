@@ -223,6 +223,12 @@ public:
 	b2ChainId CreateChain( b2BodyId bodyId, const b2ChainDef* def );
 	b2JointId CreateRevoluteJoint( const b2RevoluteJointDef* def );
 	void Step( float timeStep, int subStepCount );
+
+	// The three phases of Step, split at the two points where the host waits for the GPU, so a
+	// batch can run each phase for many worlds around one launch per stage.
+	bool StepBegin( float timeStep, int subStepCount );
+	void StepMiddle();
+	void StepEnd();
 	void EnableSleeping( bool flag );
 
 	// Host copies of GPU state are refreshed lazily after a step.
@@ -338,7 +344,11 @@ private:
 	// step stages
 	void UpdateBroadPhasePairs();
 	void ProcessContactStates( const int* changedIds, int count );
-	void Solve( StepParams& params );
+	bool SolveBegin( StepParams& params );
+	void SolveEnd();
+	StepParams stepParams = {};
+	bool solvePending = false;
+	uint64_t stepTicks = 0, collideTicks = 0, solveTicks = 0, constraintTicks = 0;
 	void FinishSolve( StepParams& params );
 
 	void PullShapes();

@@ -352,6 +352,117 @@ static void b2sBuildRain( b2sScene* scene, int count, int columns, int enableSle
 	}
 }
 
+static void b2sBuildBatchWorld( b2sScene* scene, int target, int kind, uint32_t seed )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, false );
+	s_rngState = seed != 0 ? seed : 12345u;
+	int capacity = 0;
+
+	b2BodyId groundId;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Pos ){ 0.0f, -1.0f };
+		groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( 40.0f, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+
+	if ( kind == 0 )
+	{
+		int rows = 1;
+		while ( ( rows + 1 ) * ( rows + 2 ) / 2 <= target )
+		{
+			rows += 1;
+		}
+		b2Polygon box = b2MakeSquare( 0.5f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.density = 1.0f;
+		for ( int i = 0; i < rows; ++i )
+		{
+			float y = 0.5f + i * 1.0f;
+			float x0 = -0.5f * ( rows - 1 - i );
+			for ( int j = i; j < rows; ++j )
+			{
+				b2BodyDef bodyDef = b2DefaultBodyDef();
+				bodyDef.type = b2_dynamicBody;
+				bodyDef.position = ( b2Pos ){ x0 + ( j - i ) * 1.0f, y };
+				b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+				b2CreatePolygonShape( bodyId, &shapeDef, &box );
+				b2sPushBody( scene, &capacity, bodyId );
+			}
+		}
+	}
+	else if ( kind == 1 )
+	{
+		int columns = 1;
+		while ( columns * columns < target )
+		{
+			columns += 1;
+		}
+		for ( int k = 0; k < target; ++k )
+		{
+			b2BodyDef bodyDef = b2DefaultBodyDef();
+			bodyDef.type = b2_dynamicBody;
+			bodyDef.position = ( b2Pos ){ -0.6f * columns + 1.2f * ( k % columns ) + b2sRandomRange( -0.1f, 0.1f ),
+										  1.0f + 1.2f * ( k / columns ) };
+			bodyDef.rotation = b2MakeRot( b2sRandomRange( -3.0f, 3.0f ) );
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			b2ShapeDef shapeDef = b2DefaultShapeDef();
+			shapeDef.density = b2sRandomRange( 0.5f, 2.0f );
+			if ( b2sNextRandom() & 1u )
+			{
+				b2Polygon box = b2MakeBox( b2sRandomRange( 0.2f, 0.5f ), b2sRandomRange( 0.2f, 0.5f ) );
+				b2CreatePolygonShape( bodyId, &shapeDef, &box );
+			}
+			else
+			{
+				b2Circle circle = { { 0.0f, 0.0f }, b2sRandomRange( 0.2f, 0.5f ) };
+				b2CreateCircleShape( bodyId, &shapeDef, &circle );
+			}
+			b2sPushBody( scene, &capacity, bodyId );
+		}
+	}
+	else
+	{
+		int chainCount = 1 + (int)( b2sNextRandom() % 8u );
+		int links = target / chainCount;
+		links = links < 5 ? 5 : ( links > 20 ? 20 : links );
+		float halfLength = 0.25f;
+		b2Capsule capsule = { { -halfLength, 0.0f }, { halfLength, 0.0f }, 0.125f };
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.density = 20.0f;
+		b2RevoluteJointDef jointDef = b2DefaultRevoluteJointDef();
+		for ( int c = 0; c < chainCount; ++c )
+		{
+			float x = -2.0f * chainCount + 4.0f * c;
+			float y = 2.0f * halfLength * links + 2.0f;
+			// bodies are unrotated, so a local point is the world point minus the body position
+			b2BodyId prevId = groundId;
+			b2Vec2 prevPosition = { 0.0f, -1.0f };
+			for ( int i = 0; i < links; ++i )
+			{
+				b2BodyDef bodyDef = b2DefaultBodyDef();
+				bodyDef.type = b2_dynamicBody;
+				bodyDef.position = ( b2Pos ){ x + ( 1.0f + 2.0f * i ) * halfLength, y };
+				b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+				b2CreateCapsuleShape( bodyId, &shapeDef, &capsule );
+
+				b2Vec2 pivot = { x + ( 2.0f * i ) * halfLength, y };
+				jointDef.base.bodyIdA = prevId;
+				jointDef.base.bodyIdB = bodyId;
+				jointDef.base.localFrameA.p = ( b2Vec2 ){ pivot.x - prevPosition.x, pivot.y - prevPosition.y };
+				jointDef.base.localFrameB.p = ( b2Vec2 ){ pivot.x - bodyDef.position.x, pivot.y - bodyDef.position.y };
+				b2CreateRevoluteJoint( worldId, &jointDef );
+				prevId = bodyId;
+				prevPosition = ( b2Vec2 ){ bodyDef.position.x, bodyDef.position.y };
+				b2sPushBody( scene, &capacity, bodyId );
+			}
+		}
+	}
+}
+
 int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount )
 {
 	int handle = -1;
@@ -407,6 +518,15 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 			int sleep = iparamCount > 1 ? iparams[1] : 0;
 			uint32_t seed = iparamCount > 2 ? (uint32_t)iparams[2] : 12345u;
 			b2sBuildMixedPile( scene, count, sleep, seed );
+		}
+		break;
+
+		case B2S_BATCH_WORLD:
+		{
+			int target = iparamCount > 0 ? iparams[0] : 64;
+			int which = iparamCount > 1 ? iparams[1] : 0;
+			uint32_t seed = iparamCount > 2 ? (uint32_t)iparams[2] : 12345u;
+			b2sBuildBatchWorld( scene, target, which, seed );
 		}
 		break;
 

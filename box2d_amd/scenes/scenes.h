@@ -44,6 +44,14 @@ enum
 	// segments' right side (walls 100 m high, floor 2 m segments).
 	// iparams: [0] body count, [1] columns, [2] enableSleep, [3] seed. fparams: [0] column width.
 	B2S_RAIN = 4,
+
+	// One world of the batched uneven-worlds workload, sized to about a target body count, with
+	// config-0 physics (gravity -10, friction 0.6, sleep off, ground box 40 x 1 half extents).
+	// iparams: [0] target bodies, [1] kind (0 box pyramid, 1 random box/circle pile, 2 hanging
+	// chains of capsule links on revolute joints), [2] seed.
+	// Pyramids take the largest row count whose box count fits the target; piles place the
+	// target count on a grid; chains are 1-8 chains of 5-20 links (count = chains * links).
+	B2S_BATCH_WORLD = 5,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.

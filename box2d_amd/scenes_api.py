@@ -17,6 +17,7 @@ SCENE_FALLING_HINGES = 1
 SCENE_HELLO = 2
 SCENE_MIXED_PILE = 3
 SCENE_RAIN = 4
+SCENE_BATCH_WORLD = 5
 
 
 class SceneLib:

@@ -82,6 +82,26 @@ B2_API int b2amd_TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOu
 /// (reference src/distance.c:424-611). Host arrays in and out; synchronous. Returns 0.
 B2_API int b2amd_ShapeDistanceBatch( const b2DistanceInput* inputs, b2SimplexCache* caches, int count, b2DistanceOutput* outputs );
 
+/// Timing of the last b2amd_StepWorlds call's persistent solver launch.
+typedef struct b2amd_BatchStats
+{
+	/// GPU time of the substep-loop launch over all worlds, milliseconds (HIP events)
+	float solveKernelMs;
+	/// workgroup size the autotuner chose (256, 512 or 1024)
+	int solveThreads;
+	/// number of persistent workgroups launched
+	int solveGrid;
+	/// worlds stepped
+	int worlds;
+} b2amd_BatchStats;
+
+/// Step count independent worlds by timeStep with subStepCount substeps, as one batch: host
+/// bookkeeping runs on host threads and every GPU stage is one launch over all the worlds, the
+/// substep loop a persistent launch over a world queue. Each world ends exactly as if stepped
+/// alone with b2World_Step. Every world's awake bodies must fit the fused solver (5400).
+/// Returns the number of worlds stepped; stats may be null.
+B2_API int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int subStepCount, b2amd_BatchStats* stats );
+
 #ifdef __cplusplus
 }
 #endif
