@@ -88,6 +88,39 @@ def cpu_baseline(rows, ground, seconds, threads):
                       f"(reference built-in scheduler), monotonic clock"}
 
 
+def batched_workload(lib, scenes, worlds, steps, loop_steps):
+    """Uneven batched worlds: body counts log-uniform in [8, 1024], each a box pyramid, a random
+    box/circle pile or hanging capsule chains (seeded), stepped together with b2amd_StepWorlds;
+    the same worlds stepped one by one through b2World_Step timed alongside."""
+    import box2d_amd
+
+    specs = box2d_amd.batch_world_specs(worlds, lo=8, hi=1024, seed=12345)
+    handles = [scenes.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
+    bodies = sum(scenes.body_count(h) for h in handles)
+    ids = (box2d_amd.WorldId * worlds)(*[scenes.world_id(h) for h in handles])
+    box2d_amd.step_worlds(ids, 4, DT, SUBSTEPS)  # warm-up; the first three also autotune the workgroup size
+    lib.b2amd_Synchronize()
+    solve_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = box2d_amd.step_worlds(ids, 1, DT, SUBSTEPS)
+        solve_ms += st.solveKernelMs
+    lib.b2amd_Synchronize()
+    batched_s = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for h in handles:
+            scenes.step(h, 1, DT, SUBSTEPS)
+    loop_s = (time.perf_counter() - t0) / loop_steps
+    for h in handles:
+        scenes.destroy(h)
+    return {"worlds": worlds, "bodies": bodies, "steps": steps, "ms_per_step": batched_s * 1e3,
+            "body_substeps_per_s": bodies * SUBSTEPS / batched_s, "solve_kernel_ms_per_step": solve_ms / steps,
+            "solve_workgroup_threads": st.solveThreads, "solve_workgroups": st.solveGrid,
+            "plain_loop_ms_per_step": loop_s * 1e3, "plain_loop_body_substeps_per_s": bodies * SUBSTEPS / loop_s,
+            "speedup_vs_plain_loop": loop_s / batched_s, "plain_loop_steps": loop_steps}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +132,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="bound on the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched workload (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -166,6 +201,10 @@ def main():
 
     scenes.destroy(h)
 
+    batched = None
+    if args.batched_worlds > 0:
+        batched = batched_workload(lib, scenes, args.batched_worlds, args.batched_steps, 2)
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -190,6 +229,7 @@ def main():
         "gpu_busy_ms_per_step": gpu_busy_ms,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line))
 

@@ -132,6 +132,13 @@ int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, Bat
 		s_batch = new Batch();
 	}
 
+	float phase[5] = {};
+	uint64_t t = b2GetTicks();
+	auto lap = [&]( int k ) {
+		phase[k] = b2GetMilliseconds( t );
+		t = b2GetTicks();
+	};
+
 	std::vector<char> began( count, 0 );
 	s_pool->For( count, [&]( int i ) {
 		worlds[i]->device->SetRecording( true );
@@ -153,14 +160,19 @@ int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, Bat
 		}
 	}
 	int n = (int)active.size();
+	lap( 0 );
 
 	s_batch->Collide( devices.data(), n );
+	lap( 1 );
 	s_pool->For( n, [&]( int i ) { active[i]->StepMiddle(); } );
+	lap( 2 );
 	s_batch->Solve( devices.data(), n );
+	lap( 3 );
 	s_pool->For( n, [&]( int i ) {
 		active[i]->StepEnd();
 		active[i]->device->SetRecording( false );
 	} );
+	lap( 4 );
 
 	if ( stats != nullptr )
 	{
@@ -168,6 +180,10 @@ int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, Bat
 		stats->solveThreads = s_batch->lastFusedThreads;
 		stats->solveGrid = s_batch->lastFusedGrid;
 		stats->worlds = n;
+		for ( int k = 0; k < 5; ++k )
+		{
+			stats->phaseMs[k] = phase[k];
+		}
 	}
 	return n;
 }

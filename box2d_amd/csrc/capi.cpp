@@ -673,6 +673,10 @@ int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int 
 		stats->solveThreads = bs.solveThreads;
 		stats->solveGrid = bs.solveGrid;
 		stats->worlds = bs.worlds;
+		for ( int k = 0; k < 5; ++k )
+		{
+			stats->phaseMs[k] = bs.phaseMs[k];
+		}
 	}
 	return n;
 }

@@ -132,6 +132,7 @@ struct BatchStats
 	int solveThreads;	 // its workgroup size
 	int solveGrid;		 // its workgroup count
 	int worlds;			 // worlds stepped
+	float phaseMs[5];	 // wall time: host phase 1, GPU collide, host phase 2, GPU solve, host phase 3
 };
 
 // Step count worlds as one batch (defined in batch.cpp). Returns the number of worlds stepped.

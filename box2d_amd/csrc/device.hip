@@ -65,14 +65,22 @@ static inline int Blocks( int n )
 // ---------------------------------------------------------------------------------------------
 // Kernels
 
-__global__ void InitContactsKernel( ContactSim* contacts, const ContactInit* inits, int count )
+struct InitArgs
 {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if ( i >= count )
+	ContactSim* contacts;
+	const ContactInit* inits;
+	int count;
+};
+
+// Scatter newly created contacts into the id-indexed contact array.
+B2G_DEV void InitContactsBody( const InitArgs& a, int i )
+{
+	if ( i >= a.count )
 	{
 		return;
 	}
-	ContactInit ci = inits[i];
+	ContactSim* contacts = a.contacts;
+	ContactInit ci = a.inits[i];
 	ContactSim c;
 	memset( &c, 0, sizeof( c ) );
 	c.shapeIdA = ci.shapeIdA;
@@ -1565,6 +1573,35 @@ struct ContinuousOp
 	}
 };
 
+struct InitContactsOp
+{
+	using Args = InitArgs;
+	static B2G_DEV void Run( const Args& a, int i )
+	{
+		InitContactsBody( a, i );
+	}
+};
+
+// Staged host-to-device copies of a batch: each workgroup copies one entry, word by word, from
+// the batch's uploaded staging buffer to the entry's destination.
+struct ScatterEntry
+{
+	uint32_t* dst;
+	uint64_t srcOffset; // bytes into the staging buffer
+	uint32_t words;
+	uint32_t pad;
+};
+
+__global__ void ScatterKernel( const ScatterEntry* entries, const char* base )
+{
+	ScatterEntry e = entries[blockIdx.x];
+	const uint32_t* src = reinterpret_cast<const uint32_t*>( base + e.srcOffset );
+	for ( uint32_t w = threadIdx.x; w < e.words; w += blockDim.x )
+	{
+		e.dst[w] = src[w];
+	}
+}
+
 template <typename Op>
 __global__ void LaneKernel( typename Op::Args a )
 {
@@ -1751,8 +1788,10 @@ struct DevArray
 		{
 			return;
 		}
-		size_t newCap = std::max<size_t>( n, capacity + capacity / 2 );
-		newCap = std::max<size_t>( newCap, 16 );
+		// doubling with a floor of 256 elements keeps reallocations (allocation, copy, free: each
+		// a synchronizing runtime call) rare while a world's contact set grows
+		size_t newCap = std::max<size_t>( n, 2 * capacity );
+		newCap = std::max<size_t>( newCap, 256 );
 		T* p = nullptr;
 		HIP_CHECK( hipMalloc( &p, newCap * sizeof( T ) ) );
 		if ( keep && ptr != nullptr && used > 0 )
@@ -1855,7 +1894,47 @@ struct DeviceWorld::Impl
 		ContinuousArgs continuousArgs;
 		int continuousLanes = 0;
 		PackFinalizeArgs packFinalizeArgs;
+
+		// host data the batch copies to the device before the next stage it launches
+		struct Upload
+		{
+			void* dst;
+			size_t offset;
+			size_t bytes;
+		};
+		std::vector<char> payload;
+		std::vector<Upload> uploads;
+		bool init = false;
+		InitArgs initArgs;
+		int initLanes = 0;
 	} rec;
+
+	// Host-to-device copy of host data: now on the world's stream, or recorded for the batch.
+	void Stage( void* dst, const void* src, size_t bytes )
+	{
+		if ( bytes == 0 )
+		{
+			return;
+		}
+		if ( recording )
+		{
+			size_t offset = rec.payload.size();
+			rec.payload.resize( offset + ( ( bytes + 3 ) & ~size_t( 3 ) ) );
+			memcpy( rec.payload.data() + offset, src, bytes );
+			rec.uploads.push_back( Record::Upload{ dst, offset, bytes } );
+		}
+		else
+		{
+			HIP_CHECK( hipMemcpyAsync( dst, src, bytes, hipMemcpyHostToDevice, stream ) );
+		}
+	}
+
+	template <typename T>
+	void StageVector( DevArray<T>& dst, const std::vector<T>& src )
+	{
+		dst.Reserve( std::max<size_t>( src.size(), 1 ), false, stream, 0 );
+		Stage( dst.ptr, src.data(), src.size() * sizeof( T ) );
+	}
 
 	// phase state kept from the launch half to the finish half of a stage
 	int collideCount = 0;
@@ -2191,8 +2270,16 @@ void DeviceWorld::UploadContactInits( const std::vector<ContactInit>& inits, int
 	B2G_CHECK( impl->shapeCount > 0 );
 
 	impl->inits.Reserve( n, false, impl->stream, 0 );
-	HIP_CHECK( hipMemcpyAsync( impl->inits.ptr, inits.data(), n * sizeof( ContactInit ), hipMemcpyHostToDevice, impl->stream ) );
-	InitContactsKernel<<<Blocks( n ), kBlock, 0, impl->stream>>>( impl->contacts.ptr, impl->inits.ptr, n );
+	impl->Stage( impl->inits.ptr, inits.data(), n * sizeof( ContactInit ) );
+	InitArgs ia = { impl->contacts.ptr, impl->inits.ptr, n };
+	if ( impl->recording )
+	{
+		impl->rec.init = true;
+		impl->rec.initArgs = ia;
+		impl->rec.initLanes = n;
+		return;
+	}
+	LaneKernel<InitContactsOp><<<Blocks( n ), kBlock, 0, impl->stream>>>( ia );
 	HIP_CHECK( hipGetLastError() );
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
@@ -2207,7 +2294,7 @@ void DeviceWorld::DownloadContact( int contactId, ContactSim* out )
 void DeviceWorld::UploadLists( const SolverLists& lists )
 {
 	impl->lists = lists;
-	UploadVector( impl->awakeBodies, lists.awakeBodies, impl->stream );
+	impl->StageVector( impl->awakeBodies, lists.awakeBodies );
 	{
 		int slots = impl->bodyCount;
 		for ( int id : lists.awakeBodies )
@@ -2219,25 +2306,24 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 		{
 			impl->slotStaging[lists.awakeBodies[k]] = (int)std::min<size_t>( k + 1, kFusedBodyCapacity );
 		}
-		UploadVector( impl->bodySlot, impl->slotStaging, impl->stream );
+		impl->StageVector( impl->bodySlot, impl->slotStaging );
 	}
-	UploadVector( impl->collideList, lists.collideContacts, impl->stream );
-	UploadVector( impl->jointList, lists.jointList, impl->stream );
+	impl->StageVector( impl->collideList, lists.collideContacts );
+	impl->StageVector( impl->jointList, lists.jointList );
 
 	size_t nc = lists.constraintContacts.size();
 	impl->EnsureConstraints( std::max<size_t>( nc, 1 ) );
-	if ( nc > 0 )
-	{
-		HIP_CHECK( hipMemcpyAsync( impl->soa.contactId, lists.constraintContacts.data(), nc * sizeof( int ), hipMemcpyHostToDevice,
-								   impl->stream ) );
-	}
+	impl->Stage( impl->soa.contactId, lists.constraintContacts.data(), nc * sizeof( int ) );
 
 	size_t nb = lists.awakeBodies.size();
 	impl->enlargedBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->fastBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->sleepTimes.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->changes.Reserve( std::max<size_t>( lists.collideContacts.size(), 1 ), false, impl->stream, 0 );
-	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+	if ( impl->recording == false )
+	{
+		HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+	}
 }
 
 void DeviceWorld::CollideLaunch( const StepParams& params )
@@ -2294,7 +2380,10 @@ void DeviceWorld::CollideFinish( CollideResult* result )
 		return;
 	}
 
-	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+	if ( impl->recording == false )
+	{
+		HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+	}
 
 	int changeCount = impl->mail[0];
 	result->recycledCount = impl->mail[1];
@@ -2673,7 +2762,10 @@ void DeviceWorld::SolveFinish( FinalizeResult* result )
 		int nb = impl->solveBodyCount;
 		bool needSleepTimes = impl->solveNeedSleepTimes;
 		const PackFinalizeArgs& pk = impl->solvePack;
-		HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+		if ( impl->recording == false )
+		{
+			HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+		}
 		impl->FlushTiming();
 
 		const int* mail = impl->mail;
@@ -3046,6 +3138,21 @@ namespace b2g
 
 void DeviceWorld::SetRecording( bool flag )
 {
+	// Uploads and contact inits recorded but not taken by a batch run here, in order.
+	Impl::Record& r = impl->rec;
+	if ( r.uploads.empty() == false || r.init )
+	{
+		for ( const auto& u : r.uploads )
+		{
+			HIP_CHECK( hipMemcpyAsync( u.dst, r.payload.data() + u.offset, u.bytes, hipMemcpyHostToDevice, impl->stream ) );
+		}
+		if ( r.init )
+		{
+			LaneKernel<InitContactsOp><<<Blocks( r.initLanes ), kBlock, 0, impl->stream>>>( r.initArgs );
+			HIP_CHECK( hipGetLastError() );
+		}
+		HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+	}
 	impl->recording = flag;
 	impl->rec = Impl::Record{};
 }
@@ -3175,9 +3282,50 @@ static size_t ListBytes( const StageList<Args>& list )
 	return list.args.size() * sizeof( Args ) + blocks * sizeof( int2 ) + 32;
 }
 
+// Put every world's recorded uploads into the staging area and build the scatter table; the
+// worlds' upload records are cleared. Returns the number of entries.
+static int StageUploads( Batch::Impl* b, DeviceWorld** worlds, int count, size_t* entriesOffset )
+{
+	std::vector<ScatterEntry> entries;
+	for ( int i = 0; i < count; ++i )
+	{
+		auto* r = static_cast<DeviceWorld::Impl::Record*>( worlds[i]->Record() );
+		if ( r->uploads.empty() )
+		{
+			continue;
+		}
+		size_t base = b->Put( r->payload.data(), r->payload.size() );
+		for ( const auto& u : r->uploads )
+		{
+			ScatterEntry e;
+			e.dst = static_cast<uint32_t*>( u.dst );
+			e.srcOffset = base + u.offset;
+			e.words = (uint32_t)( u.bytes / 4 );
+			e.pad = 0;
+			entries.push_back( e );
+		}
+		r->uploads.clear();
+		r->payload.clear();
+	}
+	*entriesOffset = b->Put( entries.data(), entries.size() * sizeof( ScatterEntry ) );
+	return (int)entries.size();
+}
+
+static size_t UploadBytes( DeviceWorld** worlds, int count )
+{
+	size_t bytes = 0;
+	for ( int i = 0; i < count; ++i )
+	{
+		auto* r = static_cast<DeviceWorld::Impl::Record*>( worlds[i]->Record() );
+		bytes += r->payload.size() + r->uploads.size() * sizeof( ScatterEntry ) + 16;
+	}
+	return bytes + 64;
+}
+
 void Batch::Collide( DeviceWorld** worlds, int count )
 {
 	StageList<CollideArgs> collide;
+	StageList<InitArgs> inits;
 	std::vector<PackCollideArgs> pack;
 	for ( int i = 0; i < count; ++i )
 	{
@@ -3188,23 +3336,43 @@ void Batch::Collide( DeviceWorld** worlds, int count )
 			collide.lanes.push_back( r->collideLanes );
 			pack.push_back( r->packCollideArgs );
 		}
-	}
-	if ( collide.args.empty() )
-	{
-		return;
+		if ( r->init )
+		{
+			inits.args.push_back( r->initArgs );
+			inits.lanes.push_back( r->initLanes );
+			r->init = false;
+		}
 	}
 
 	Impl* b = impl;
 	b->used = 0;
-	b->EnsureStaging( ListBytes( collide ) + pack.size() * sizeof( PackCollideArgs ) + 64 );
-	size_t argsOff, blocksOff, packOff;
-	int blockCount;
+	b->EnsureStaging( UploadBytes( worlds, count ) + ListBytes( inits ) + ListBytes( collide ) +
+					  pack.size() * sizeof( PackCollideArgs ) + 128 );
+	size_t entriesOff, initOff, initBlocksOff, argsOff, blocksOff, packOff;
+	int initBlocks, blockCount;
+	int entryCount = StageUploads( b, worlds, count, &entriesOff );
+	StageLane( b, inits, &initOff, &initBlocksOff, &initBlocks );
 	StageLane( b, collide, &argsOff, &blocksOff, &blockCount );
 	packOff = b->Put( pack.data(), pack.size() * sizeof( PackCollideArgs ) );
 
 	b->buffer.Reserve( b->used, false, b->stream, 0 );
 	HIP_CHECK( hipMemcpyAsync( b->buffer.ptr, b->staging, b->used, hipMemcpyHostToDevice, b->stream ) );
 	char* base = b->buffer.ptr;
+	if ( entryCount > 0 )
+	{
+		ScatterKernel<<<entryCount, kBlock, 0, b->stream>>>( reinterpret_cast<const ScatterEntry*>( base + entriesOff ), base );
+	}
+	if ( initBlocks > 0 )
+	{
+		LaneBatchKernel<InitContactsOp><<<initBlocks, kBlock, 0, b->stream>>>( reinterpret_cast<const InitArgs*>( base + initOff ),
+																			   reinterpret_cast<const int2*>( base + initBlocksOff ) );
+	}
+	if ( blockCount == 0 )
+	{
+		HIP_CHECK( hipGetLastError() );
+		HIP_CHECK( hipStreamSynchronize( b->stream ) );
+		return;
+	}
 	LaneBatchKernel<CollideOp><<<blockCount, kBlock, 0, b->stream>>>( reinterpret_cast<const CollideArgs*>( base + argsOff ),
 																	reinterpret_cast<const int2*>( base + blocksOff ) );
 	HIP_CHECK( hipGetLastError() );
@@ -3266,8 +3434,11 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 	Impl* b = impl;
 	b->used = 0;
 	size_t n = (size_t)count;
-	b->EnsureStaging( ListBytes( prepare ) + ListBytes( prepareJoints ) + ListBytes( store ) + ListBytes( finalize ) +
-					  ListBytes( continuous ) + n * ( sizeof( FusedArgs ) + sizeof( int ) + sizeof( PackFinalizeArgs ) ) + 256 );
+	b->EnsureStaging( UploadBytes( worlds, count ) + ListBytes( prepare ) + ListBytes( prepareJoints ) + ListBytes( store ) +
+					  ListBytes( finalize ) + ListBytes( continuous ) +
+					  n * ( sizeof( FusedArgs ) + sizeof( int ) + sizeof( PackFinalizeArgs ) ) + 256 );
+	size_t entriesOff = 0;
+	int entryCount = StageUploads( b, worlds, count, &entriesOff );
 
 	size_t prepOff = 0, prepBlocksOff = 0, jointOff = 0, jointBlocksOff = 0, storeOff = 0, storeBlocksOff = 0;
 	size_t finOff = 0, finBlocksOff = 0, ccdOff = 0, ccdBlocksOff = 0, fusedOff = 0, orderOff = 0, packOff = 0;
@@ -3299,6 +3470,10 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 	HIP_CHECK( hipMemcpyAsync( b->buffer.ptr, b->staging, b->used, hipMemcpyHostToDevice, s ) );
 	char* base = b->buffer.ptr;
 
+	if ( entryCount > 0 )
+	{
+		ScatterKernel<<<entryCount, kBlock, 0, s>>>( reinterpret_cast<const ScatterEntry*>( base + entriesOff ), base );
+	}
 	if ( prepBlocks > 0 )
 	{
 		LaneBatchKernel<PrepareContactsOp><<<prepBlocks, kBlock, 0, s>>>( reinterpret_cast<const PrepareArgs*>( base + prepOff ),

@@ -2422,6 +2422,7 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	params.enableContactSoftening = enableContactSoftening ? 1 : 0;
 
 	// Host edits made since the last step go to the GPU before any stage reads them.
+	uint64_t setupTicks = b2GetTicks();
 	PushBodiesIfDirty();
 	if ( shapesDirty )
 	{
@@ -2444,6 +2445,8 @@ bool World::StepBegin( float timeStep, int subStepCount )
 		listsDirty = false;
 	}
 
+	profile.solverSetup = b2GetMilliseconds( setupTicks );
+
 	stepParams = params;
 	collideTicks = b2GetTicks();
 	device->CollideLaunch( params );
@@ -2456,10 +2459,12 @@ void World::StepMiddle()
 	CollideResult cr;
 	device->CollideFinish( &cr );
 	recycledContactCount = cr.recycledCount;
+	uint64_t stateTicks = b2GetTicks();
 	if ( cr.changes.empty() == false )
 	{
 		ProcessContactStates( cr.changes.data(), (int)cr.changes.size() );
 	}
+	profile.prepareConstraints = b2GetMilliseconds( stateTicks );
 	profile.collide = b2GetMilliseconds( collideTicks );
 
 	solvePending = false;
@@ -2494,6 +2499,7 @@ bool World::SolveBegin( StepParams& params )
 		return false;
 	}
 
+	uint64_t splitTicks = b2GetTicks();
 	if ( splitIslandId != B2_NULL_INDEX )
 	{
 		SplitIsland( splitIslandId );
@@ -2510,6 +2516,7 @@ bool World::SolveBegin( StepParams& params )
 		device->UploadLists( lists );
 		listsDirty = false;
 	}
+	profile.splitIslands = b2GetMilliseconds( splitTicks );
 
 	constraintTicks = b2GetTicks();
 	device->SolveLaunch( params, restitutionShapeCount > 0, enableSleep );
@@ -2530,6 +2537,7 @@ void World::SolveEnd()
 	jointsStale = true;
 
 	// Move events: one per awake body in awake order (transforms fetched lazily).
+	uint64_t eventTicks = b2GetTicks();
 	moveEventBodyIds = awakeSet.bodyIds;
 	moveEvents.resize( awakeBodyCount );
 	for ( int i = 0; i < awakeBodyCount; ++i )
@@ -2542,6 +2550,7 @@ void World::SolveEnd()
 		moveEvents[i].fellAsleep = false;
 	}
 	moveEventsNeedTransforms = true;
+	profile.transforms = b2GetMilliseconds( eventTicks );
 
 	// Refit: enlarge proxies of enlarged bodies in awake order (reference src/solver.c:1822-1915).
 	{

@@ -93,6 +93,9 @@ typedef struct b2amd_BatchStats
 	int solveGrid;
 	/// worlds stepped
 	int worlds;
+	/// wall-clock milliseconds of the step's phases: host phase 1 (pairs, uploads), GPU collide,
+	/// host phase 2 (touch processing, coloring), GPU solve, host phase 3 (refit, sleep)
+	float phaseMs[5];
 } b2amd_BatchStats;
 
 /// Step count independent worlds by timeStep with subStepCount substeps, as one batch: host
