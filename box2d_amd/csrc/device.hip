@@ -652,12 +652,21 @@ B2G_DEV bool GroupAnyNonZero( const ConstraintSoA& c, int f, int colorStart, int
 
 // Lane t of one graph color of one stage: joints occupy the first jointCount lanes, contacts
 // the rest (joint and contact blocks of one color touch disjoint bodies).
-template <int Stage, typename Bodies>
-B2G_DEV void ColorLane( const ColorArgs& a, int t, const Bodies& B )
+// Joint and contact index ranges of one color (or of the overflow constraints).
+struct ColorRange
 {
-	if ( t < a.jointCount )
+	int jointStart;
+	int jointCount;
+	int contactStart;
+	int contactCount;
+};
+
+template <int Stage, typename Bodies>
+B2G_DEV void ColorLane( const ColorArgs& a, const ColorRange& r, int t, const Bodies& B )
+{
+	if ( t < r.jointCount )
 	{
-		JointSim* joint = a.joints + a.jointList[a.jointStart + t];
+		JointSim* joint = a.joints + a.jointList[r.jointStart + t];
 		if ( Stage == kStageWarmStart )
 		{
 			WarmStartRevolute( joint, B );
@@ -673,12 +682,12 @@ B2G_DEV void ColorLane( const ColorArgs& a, int t, const Bodies& B )
 		return;
 	}
 
-	int k = t - a.jointCount;
-	if ( k >= a.contactCount )
+	int k = t - r.jointCount;
+	if ( k >= r.contactCount )
 	{
 		return;
 	}
-	int i = a.contactStart + k;
+	int i = r.contactStart + k;
 
 	if ( Stage == kStageWarmStart )
 	{
@@ -690,12 +699,12 @@ B2G_DEV void ColorLane( const ColorArgs& a, int t, const Bodies& B )
 	}
 	else if ( Stage == kStageRelax )
 	{
-		bool rolling = GroupAnyNonZero( a.c, kcRollingResistance, a.contactStart, a.contactCount, k );
+		bool rolling = GroupAnyNonZero( a.c, kcRollingResistance, r.contactStart, r.contactCount, k );
 		SolveContact( a.c, i, B, false, a.p.inv_h, -a.p.contactSpeed, rolling );
 	}
 	else
 	{
-		if ( GroupAnyNonZero( a.c, kcRestitution, a.contactStart, a.contactCount, k ) )
+		if ( GroupAnyNonZero( a.c, kcRestitution, r.contactStart, r.contactCount, k ) )
 		{
 			RestitutionContact( a.c, i, B, a.p.restitutionThreshold );
 		}
@@ -705,11 +714,11 @@ B2G_DEV void ColorLane( const ColorArgs& a, int t, const Bodies& B )
 // Overflow constraints of one stage, walked by a single lane: joints then contacts in order, as
 // the reference's single-threaded overflow passes do (src/solver.c:1100-1168).
 template <int Stage, typename Bodies>
-B2G_DEV void OverflowSerial( const ColorArgs& a, const Bodies& B )
+B2G_DEV void OverflowSerial( const ColorArgs& a, const ColorRange& r, const Bodies& B )
 {
-	for ( int t = 0; t < a.jointCount; ++t )
+	for ( int t = 0; t < r.jointCount; ++t )
 	{
-		JointSim* joint = a.joints + a.jointList[a.jointStart + t];
+		JointSim* joint = a.joints + a.jointList[r.jointStart + t];
 		if ( Stage == kStageWarmStart )
 		{
 			WarmStartRevolute( joint, B );
@@ -724,9 +733,9 @@ B2G_DEV void OverflowSerial( const ColorArgs& a, const Bodies& B )
 		}
 	}
 
-	for ( int k = 0; k < a.contactCount; ++k )
+	for ( int k = 0; k < r.contactCount; ++k )
 	{
-		int i = a.contactStart + k;
+		int i = r.contactStart + k;
 		if ( Stage == kStageWarmStart )
 		{
 			OverflowWarmStart( a.c, i, B );
@@ -751,7 +760,8 @@ template <int Stage>
 __global__ void ColorKernel( ColorArgs a )
 {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	ColorLane<Stage>( a, t, GlobalBodies{ a.states } );
+	ColorRange r = { a.jointStart, a.jointCount, a.contactStart, a.contactCount };
+	ColorLane<Stage>( a, r, t, GlobalBodies{ a.states } );
 }
 
 template <int Stage>
@@ -761,7 +771,8 @@ __global__ void OverflowKernel( ColorArgs a )
 	{
 		return;
 	}
-	OverflowSerial<Stage>( a, GlobalBodies{ a.states } );
+	ColorRange r = { a.jointStart, a.jointCount, a.contactStart, a.contactCount };
+	OverflowSerial<Stage>( a, r, GlobalBodies{ a.states } );
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -796,36 +807,33 @@ struct FusedArgs
 };
 
 template <int Stage>
-B2G_DEV void FusedStage( const FusedArgs& a, ColorArgs& ca, const LdsBodies& B )
+B2G_DEV void FusedStage( const FusedArgs& a, const LdsBodies& B )
 {
 	if ( a.overflowContactCount > 0 || a.overflowJointCount > 0 )
 	{
 		if ( threadIdx.x == 0 )
 		{
-			ColorArgs oa = ca;
-			oa.jointStart = a.overflowJointStart;
-			oa.jointCount = a.overflowJointCount;
-			oa.contactStart = a.overflowContactStart;
-			oa.contactCount = a.overflowContactCount;
-			OverflowSerial<Stage>( oa, B );
+			ColorRange r = { a.overflowJointStart, a.overflowJointCount, a.overflowContactStart, a.overflowContactCount };
+			OverflowSerial<Stage>( a.color, r, B );
 		}
 		__syncthreads();
 	}
 
 	for ( int color = 0; color < a.activeColorCount; ++color )
 	{
-		ca.jointStart = a.colorJointStart[color];
-		ca.jointCount = a.colorJointStart[color + 1] - ca.jointStart;
-		ca.contactStart = a.colorContactStart[color];
-		ca.contactCount = a.colorContactStart[color + 1] - ca.contactStart;
-		int lanes = ca.jointCount + ca.contactCount;
-		if ( lanes == 0 || ( Stage == kStageRestitution && ca.contactCount == 0 ) )
+		ColorRange r;
+		r.jointStart = a.colorJointStart[color];
+		r.jointCount = a.colorJointStart[color + 1] - r.jointStart;
+		r.contactStart = a.colorContactStart[color];
+		r.contactCount = a.colorContactStart[color + 1] - r.contactStart;
+		int lanes = r.jointCount + r.contactCount;
+		if ( lanes == 0 || ( Stage == kStageRestitution && r.contactCount == 0 ) )
 		{
 			continue;
 		}
 		for ( int t = threadIdx.x; t < lanes; t += blockDim.x )
 		{
-			ColorLane<Stage>( ca, t, B );
+			ColorLane<Stage>( a.color, r, t, B );
 		}
 		__syncthreads();
 	}
@@ -885,7 +893,6 @@ B2G_DEV void FusedBody( const FusedArgs& a, char* lds )
 	}
 	__syncthreads();
 
-	ColorArgs ca = a.color;
 	for ( int sub = 0; sub < a.subStepCount; ++sub )
 	{
 		for ( int i = threadIdx.x; i < n; i += blockDim.x )
@@ -898,8 +905,8 @@ B2G_DEV void FusedBody( const FusedArgs& a, char* lds )
 		}
 		__syncthreads();
 
-		FusedStage<kStageWarmStart>( a, ca, B );
-		FusedStage<kStageSolve>( a, ca, B );
+		FusedStage<kStageWarmStart>( a, B );
+		FusedStage<kStageSolve>( a, B );
 
 		for ( int i = threadIdx.x; i < n; i += blockDim.x )
 		{
@@ -909,12 +916,12 @@ B2G_DEV void FusedBody( const FusedArgs& a, char* lds )
 		}
 		__syncthreads();
 
-		FusedStage<kStageRelax>( a, ca, B );
+		FusedStage<kStageRelax>( a, B );
 	}
 
 	if ( a.anyRestitution )
 	{
-		FusedStage<kStageRestitution>( a, ca, B );
+		FusedStage<kStageRestitution>( a, B );
 	}
 
 	// write the whole state back (flags carry the speed-cap bit to finalize)
@@ -950,12 +957,14 @@ __global__ __launch_bounds__( kFusedThreads ) void SolveFusedBatchKernel( const 
 			s_world = atomicAdd( queue, 1 );
 		}
 		__syncthreads();
-		int w = s_world;
+		// the world index is uniform across the workgroup: keep it (and the argument loads) scalar
+		int w = __builtin_amdgcn_readfirstlane( s_world );
 		if ( w >= count )
 		{
 			break;
 		}
-		FusedBody( args[order[w]], s_lds );
+		int world = __builtin_amdgcn_readfirstlane( order[w] );
+		FusedBody( args[world], s_lds );
 		__syncthreads();
 	}
 }
@@ -2041,10 +2050,8 @@ struct DeviceWorld::Impl
 		cfields.Reserve( cap * kcFieldCount, false, stream, 0 );
 		cints.Reserve( cap * 6, false, stream, 0 );
 		constraintCapacity = cap;
-		for ( int f = 0; f < kcFieldCount; ++f )
-		{
-			soa.f[f] = cfields.ptr + f * cap;
-		}
+		soa.f.base = cfields.ptr;
+		soa.f.stride = (int)cap;
 		soa.bodyA = cints.ptr;
 		soa.bodyB = cints.ptr + cap;
 		soa.contactId = cints.ptr + 2 * cap;

@@ -15,9 +15,22 @@ namespace b2g
 
 #define B2G_DEVI __device__ __forceinline__
 
+// The constraint fields live in one allocation, field-major: field k of constraint i is at
+// base[k * stride + i]. f[k] yields the field's array, so lanes read c.f[k][i].
+struct FieldTable
+{
+	float* base;
+	int stride;
+
+	__host__ __device__ float* operator[]( int field ) const
+	{
+		return base + (size_t)field * stride;
+	}
+};
+
 struct ConstraintSoA
 {
-	float* f[kcFieldCount];
+	FieldTable f;
 	int* bodyA; // body ids (GlobalBodies view)
 	int* bodyB;
 	int* contactId;
