@@ -28,7 +28,7 @@ def library():
 
 
 from .scenes_api import (SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HELLO, SCENE_MIXED_PILE,  # noqa: E402
-                         SCENE_RAIN, SCENE_BATCH_WORLD, WorldId)
+                         SCENE_RAIN, SCENE_BATCH_WORLD, SCENE_PUSH, WorldId)
 
 
 class BatchStats(ctypes.Structure):

@@ -6,12 +6,14 @@
 #include "layout.h"
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 namespace b2g
 {
 
 class World;
+class TapeStore;
 
 // Work lists the host rebuilds when the set of awake bodies or the graph coloring changes.
 struct SolverLists
@@ -91,6 +93,18 @@ public:
 	// Solver path: 0 auto, 1 color kernels, 2 fused when the world fits.
 	void SetSolverPath( int path );
 
+	// Step recording for the adjoint: while a tape is set, every solve appends one record (body
+	// state at the step's start, constraints in solve order) to the tape. index is the world's
+	// slot on the tape. The solve uses the fused path.
+	void SetTape( TapeStore* tape, int index );
+	TapeStore* Tape() const;
+	// Write the final body state record of a finished tape (bodies only) and wait for it.
+	void CaptureFinalState( char* dst, int bodyCount );
+	// Device addresses of the id-indexed shape array (ShapeDev) and body sims (BodySim).
+	const void* ShapesDevice() const;
+	const void* SimsDevice() const;
+	int AwakeBodyCount() const;
+
 	void* stream = nullptr;
 
 	struct Impl;
@@ -137,6 +151,40 @@ struct BatchStats
 
 // Step count worlds as one batch (defined in batch.cpp). Returns the number of worlds stepped.
 int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, BatchStats* stats );
+
+// Device storage of recorded steps (adjoint.hip). Records are carved out of large device chunks
+// and stay valid until the store is destroyed.
+class TapeStore
+{
+public:
+	struct WorldTape
+	{
+		World* world = nullptr;
+		std::vector<char*> steps; // device address of each step's record
+		std::vector<int> bodyCounts;
+		std::vector<int> constraintCounts;
+		std::vector<int> subStepCounts;
+		char* final = nullptr; // final state record (bodies only)
+		int finalBodyCount = 0;
+	};
+
+	~TapeStore();
+	char* Alloc( size_t bytes );
+
+	std::vector<WorldTape> worlds;
+	bool finished = false;
+
+private:
+	std::mutex mutex;
+	std::vector<char*> chunks;
+	size_t chunkUsed = 0;
+	size_t chunkSize = 0;
+};
+
+// Record layout: header at 0, bodies at kTapeBodiesOffset, constraints after the bodies.
+constexpr size_t kTapeBodiesOffset = 256;
+size_t TapeConstraintsOffset( int bodyCount );
+size_t TapeRecordBytes( int bodyCount, int constraintCount );
 
 // Last HIP error text (empty if none) for diagnostics.
 const char* LastDeviceError();

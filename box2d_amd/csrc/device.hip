@@ -197,6 +197,7 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 				mp->separation = mp->baseSeparation + b2Dot( dp, normal );
 				mp->persisted = true;
 			}
+			cs->stepInfo = 1u | ( 1u << 2 ) | ( 2u << 4 );
 
 			atomicAdd( a.counters + 1, 1 );
 			return;
@@ -265,6 +266,7 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 		m.rollingImpulse = oldManifold.rollingImpulse;
 	}
 
+	uint32_t stepInfo = 0;
 	for ( int i = 0; i < pointCount; ++i )
 	{
 		b2ManifoldPoint* mp2 = m.points + i;
@@ -287,6 +289,7 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 				mp2->persisted = true;
 				mp1->normalImpulse = 0.0f;
 				mp1->tangentImpulse = 0.0f;
+				stepInfo |= (uint32_t)( j + 1 ) << ( 2 + 2 * i );
 				break;
 			}
 		}
@@ -308,6 +311,7 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 
 	cs->manifold = m;
 	cs->simFlags = simFlags;
+	cs->stepInfo = stepInfo;
 
 	if ( touching == true && wasTouching == false )
 	{
@@ -530,66 +534,6 @@ struct BodyArgs
 	BodyAux* aux;
 	StepParams p;
 };
-
-// Velocity integration of one body (reference src/solver.c:66-112).
-B2G_DEV void IntegrateVelocity( const BodySim& sim, BodyState* state, const StepParams& p )
-{
-	b2Vec2 v = state->linearVelocity;
-	float w = state->angularVelocity;
-	float h = p.h;
-
-	float linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
-	float angularDamping = 1.0f / ( 1.0f + h * sim.angularDamping );
-	float gravityScale = sim.invMass > 0.0f ? sim.gravityScale : 0.0f;
-
-	b2Vec2 linearVelocityDelta = b2Add( b2MulSV( h * sim.invMass, sim.force ), b2MulSV( h * gravityScale, p.gravity ) );
-	float angularVelocityDelta = h * sim.invInertia * sim.torque;
-
-	v = b2MulAdd( linearVelocityDelta, linearDamping, v );
-	w = angularVelocityDelta + angularDamping * w;
-
-	state->linearVelocity = v;
-	state->angularVelocity = w;
-}
-
-// Position integration of one body: locks, speed caps, delta position and rotation
-// (reference src/solver.c:114-162).
-B2G_DEV void IntegratePosition( BodyState* state, const StepParams& p )
-{
-	float h = p.h;
-	float maxLinearSpeed = p.maxLinearVelocity;
-	float maxAngularSpeed = ( 0.25f * B2_PI ) * p.inv_dt;
-	float maxLinearSpeedSquared = maxLinearSpeed * maxLinearSpeed;
-	float maxAngularSpeedSquared = maxAngularSpeed * maxAngularSpeed;
-
-	b2Vec2 v = state->linearVelocity;
-	float w = state->angularVelocity;
-	uint32_t flags = state->flags;
-
-	v.x = ( flags & kLockLinearX ) ? 0.0f : v.x;
-	v.y = ( flags & kLockLinearY ) ? 0.0f : v.y;
-	w = ( flags & kLockAngularZ ) ? 0.0f : w;
-
-	if ( b2Dot( v, v ) > maxLinearSpeedSquared )
-	{
-		float ratio = maxLinearSpeed / b2Length( v );
-		v = b2MulSV( ratio, v );
-		flags |= kIsSpeedCapped;
-	}
-
-	if ( w * w > maxAngularSpeedSquared && ( flags & kAllowFastRotation ) == 0 )
-	{
-		float ratio = maxAngularSpeed / b2AbsFloat( w );
-		w *= ratio;
-		flags |= kIsSpeedCapped;
-	}
-
-	state->flags = flags;
-	state->linearVelocity = v;
-	state->angularVelocity = w;
-	state->deltaPosition = b2MulAdd( state->deltaPosition, h, v );
-	state->deltaRotation = b2IntegrateRotation( state->deltaRotation, h * w );
-}
 
 __global__ void IntegrateVelocitiesKernel( BodyArgs a )
 {
@@ -1528,6 +1472,120 @@ B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 // Lane kernels: each per-item stage body runs either as a single-world launch or as one launch
 // over many worlds, where a block table maps every block to (world, block within the world).
 
+// ---------------------------------------------------------------------------------------------
+// Step records for the adjoint: after constraint preparation and before the substep loop, one
+// lane per awake body and per contact constraint copies what the backward pass recomputes the
+// step from (see b2amd_TapeStep, b2amd_TapeBody, b2amd_TapeConstraint).
+
+size_t TapeConstraintsOffset( int bodyCount )
+{
+	return kTapeBodiesOffset + ( ( (size_t)bodyCount * sizeof( b2amd_TapeBody ) + 255 ) & ~(size_t)255 );
+}
+
+size_t TapeRecordBytes( int bodyCount, int constraintCount )
+{
+	return ( ( TapeConstraintsOffset( bodyCount ) + (size_t)constraintCount * sizeof( b2amd_TapeConstraint ) + 255 ) &
+			 ~(size_t)255 );
+}
+
+struct TapeRecordArgs
+{
+	b2amd_TapeStep header;
+	char* dst;
+	size_t constraintsOffset;
+	ConstraintSoA c; // bodyA/bodyB are body ids, slotA/slotB fused solver slots
+	const ContactSim* contacts;
+	const BodySim* sims;
+	const BodyState* states;
+	const int* awake;
+	int bodyCount;
+	int constraintCount;
+};
+
+B2G_DEV void TapeRecordBody( const TapeRecordArgs& a, int t )
+{
+	if ( t == 0 )
+	{
+		*reinterpret_cast<b2amd_TapeStep*>( a.dst ) = a.header;
+	}
+
+	if ( t < a.bodyCount )
+	{
+		int id = a.awake[t];
+		const BodySim& sim = a.sims[id];
+		const BodyState& st = a.states[id];
+		b2amd_TapeBody b;
+		b.bodyId = id;
+		b.flags = st.flags;
+		b.center = sim.center;
+		b.origin = sim.transform.p;
+		b.rotation = sim.transform.q;
+		b.linearVelocity = st.linearVelocity;
+		b.angularVelocity = st.angularVelocity;
+		b.invMass = sim.invMass;
+		b.invInertia = sim.invInertia;
+		b.gravityScale = sim.gravityScale;
+		b.linearDamping = sim.linearDamping;
+		b.angularDamping = sim.angularDamping;
+		b.localCenter = sim.localCenter;
+		b.force = sim.force;
+		b.torque = sim.torque;
+		b.padding = 0.0f;
+		reinterpret_cast<b2amd_TapeBody*>( a.dst + kTapeBodiesOffset )[t] = b;
+	}
+
+	if ( t < a.constraintCount )
+	{
+		const ConstraintSoA& c = a.c;
+		int contactId = c.contactId[t];
+		const ContactSim& cs = a.contacts[contactId];
+		const b2Manifold& m = cs.manifold;
+		b2amd_TapeConstraint r;
+		r.contactId = contactId;
+		r.shapeIdA = cs.shapeIdA;
+		r.shapeIdB = cs.shapeIdB;
+		r.bodyIdA = c.bodyA[t];
+		r.bodyIdB = c.bodyB[t];
+		r.slotA = c.slotA[t];
+		r.slotB = c.slotB[t];
+		r.pointCount = m.pointCount;
+		r.info = cs.stepInfo;
+		r.normal = m.normal;
+		for ( int j = 0; j < 2; ++j )
+		{
+			int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+			bool live = j < m.pointCount;
+			r.id[j] = live ? m.points[j].id : 0;
+			r.anchorA[j] = live ? m.points[j].anchorA : b2Vec2{ 0.0f, 0.0f };
+			r.anchorB[j] = live ? m.points[j].anchorB : b2Vec2{ 0.0f, 0.0f };
+			r.separation[j] = live ? m.points[j].separation : 0.0f;
+			r.baseSeparation[j] = live ? m.points[j].baseSeparation : 0.0f;
+			r.normalImpulse[j] = c.f[kcNormalImpulse1 + o][t];
+			r.tangentImpulse[j] = c.f[kcTangentImpulse1 + o][t];
+		}
+		r.friction = c.f[kcFriction][t];
+		r.restitution = c.f[kcRestitution][t];
+		r.rollingResistance = c.f[kcRollingResistance][t];
+		r.tangentSpeed = c.f[kcTangentSpeed][t];
+		r.biasRate = c.f[kcBiasRate][t];
+		r.massScale = c.f[kcMassScale][t];
+		r.impulseScale = c.f[kcImpulseScale][t];
+		r.cachedRotationA = cs.cachedRotationA;
+		r.cachedRotationB = cs.cachedRotationB;
+		r.padding = 0.0f;
+		reinterpret_cast<b2amd_TapeConstraint*>( a.dst + a.constraintsOffset )[t] = r;
+	}
+}
+
+struct TapeRecordOp
+{
+	using Args = TapeRecordArgs;
+	static B2G_DEV void Run( const Args& a, int i )
+	{
+		TapeRecordBody( a, i );
+	}
+};
+
 struct CollideOp
 {
 	using Args = CollideArgs;
@@ -1856,6 +1914,9 @@ struct DeviceWorld::Impl
 	SolverLists lists;
 	std::vector<int> slotStaging;
 	int solverPath = 0;
+	// step recording for the adjoint (null when not recording)
+	TapeStore* tape = nullptr;
+	int tapeIndex = 0;
 	// Workgroup size of the color kernels (64..1024; launches are latency-bound, so it matters little).
 	int colorBlock = ColorBlockFromEnv();
 
@@ -1887,7 +1948,9 @@ struct DeviceWorld::Impl
 	struct Record
 	{
 		bool collide = false, prepare = false, prepareJoints = false, fused = false, store = false, finalize = false,
-			 continuous = false, packCollide = false, packFinalize = false;
+			 continuous = false, packCollide = false, packFinalize = false, tapeRecord = false;
+		TapeRecordArgs tapeArgs;
+		int tapeLanes = 0;
 		CollideArgs collideArgs;
 		int collideLanes = 0;
 		PackCollideArgs packCollideArgs;
@@ -2592,6 +2655,66 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	// color kernels dominates, and loses once a world has enough constraints to need the whole GPU.
 	bool fits = nb > 0 && nb <= kFusedBodyCapacity && L.activeColorCount <= kMaxColorSlots;
 	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && nc <= kFusedMaxContacts ) );
+
+	if ( impl->tape != nullptr && nb > 0 )
+	{
+		if ( fits == false )
+		{
+			Fatal( "step recording needs the world's awake bodies in one workgroup's LDS (%d awake, at most %d)", nb,
+				   kFusedBodyCapacity );
+		}
+		fused = true;
+
+		TapeStore::WorldTape& wt = impl->tape->worlds[impl->tapeIndex];
+		char* dst = impl->tape->Alloc( TapeRecordBytes( nb, nc ) );
+		wt.steps.push_back( dst );
+		wt.bodyCounts.push_back( nb );
+		wt.constraintCounts.push_back( nc );
+		wt.subStepCounts.push_back( params.subStepCount );
+
+		TapeRecordArgs ta;
+		memset( &ta.header, 0, sizeof( ta.header ) );
+		ta.header.bodyCount = nb;
+		ta.header.constraintCount = nc;
+		ta.header.jointCount = nj;
+		ta.header.activeColorCount = L.activeColorCount;
+		for ( int c = 0; c <= L.activeColorCount && c < 25; ++c )
+		{
+			ta.header.colorStart[c] = L.colorContactStart[c];
+		}
+		ta.header.overflowStart = L.overflowContactStart;
+		ta.header.overflowCount = L.overflowContactCount;
+		ta.header.subStepCount = params.subStepCount;
+		ta.header.timeStep = params.dt;
+		ta.header.h = params.h;
+		ta.header.inv_h = params.inv_h;
+		ta.header.contactSpeed = params.contactSpeed;
+		ta.header.maxLinearSpeed = params.maxLinearVelocity;
+		ta.header.maxAngularSpeed = ( 0.25f * B2_PI ) * params.inv_dt;
+		ta.header.gravity = params.gravity;
+		ta.header.enableWarmStarting = params.enableWarmStarting;
+		ta.header.inv_dt = params.inv_dt;
+		ta.dst = dst;
+		ta.constraintsOffset = TapeConstraintsOffset( nb );
+		ta.c = impl->soa;
+		ta.contacts = impl->contacts.ptr;
+		ta.sims = impl->sims.ptr;
+		ta.states = impl->states.ptr;
+		ta.awake = impl->awakeBodies.ptr;
+		ta.bodyCount = nb;
+		ta.constraintCount = nc;
+		int lanes = std::max( nb, nc );
+		if ( rec )
+		{
+			impl->rec.tapeRecord = true;
+			impl->rec.tapeArgs = ta;
+			impl->rec.tapeLanes = lanes;
+		}
+		else
+		{
+			LaneKernel<TapeRecordOp><<<Blocks( lanes ), kBlock, 0, s>>>( ta );
+		}
+	}
 	if ( rec && nb > 0 && fits == false )
 	{
 		Fatal( "batched stepping needs every world's awake bodies in one workgroup's LDS (%d awake, at most %d)", nb,
@@ -2862,6 +2985,52 @@ __global__ void CollideBatchKernel( const ShapeDev* shapesA, const ShapeDev* sha
 void DeviceWorld::SetSolverPath( int path )
 {
 	impl->solverPath = path;
+}
+
+void DeviceWorld::SetTape( TapeStore* tape, int index )
+{
+	impl->tape = tape;
+	impl->tapeIndex = index;
+}
+
+TapeStore* DeviceWorld::Tape() const
+{
+	return impl->tape;
+}
+
+void DeviceWorld::CaptureFinalState( char* dst, int bodyCount )
+{
+	TapeRecordArgs ta;
+	memset( &ta, 0, sizeof( ta ) );
+	ta.header.bodyCount = bodyCount;
+	ta.dst = dst;
+	ta.constraintsOffset = TapeConstraintsOffset( bodyCount );
+	ta.c = impl->soa;
+	ta.contacts = impl->contacts.ptr;
+	ta.sims = impl->sims.ptr;
+	ta.states = impl->states.ptr;
+	ta.awake = impl->awakeBodies.ptr;
+	ta.bodyCount = bodyCount;
+	ta.constraintCount = 0;
+	int lanes = std::max( bodyCount, 1 );
+	LaneKernel<TapeRecordOp><<<Blocks( lanes ), kBlock, 0, impl->stream>>>( ta );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+const void* DeviceWorld::ShapesDevice() const
+{
+	return impl->shapes.ptr;
+}
+
+const void* DeviceWorld::SimsDevice() const
+{
+	return impl->sims.ptr;
+}
+
+int DeviceWorld::AwakeBodyCount() const
+{
+	return (int)impl->lists.awakeBodies.size();
 }
 
 __global__ void TimeOfImpactBatchKernel( const b2TOIInput* inputs, int count, b2TOIOutput* outputs )
@@ -3392,6 +3561,7 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 {
 	StageList<PrepareArgs> prepare;
 	StageList<PrepareJointArgs> prepareJoints;
+	StageList<TapeRecordArgs> tapeRecords;
 	std::vector<FusedArgs> fused;
 	std::vector<int> fusedBodies;
 	StageList<StoreArgs> store;
@@ -3411,6 +3581,11 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 		{
 			prepareJoints.args.push_back( r->prepareJointArgs );
 			prepareJoints.lanes.push_back( r->prepareJointLanes );
+		}
+		if ( r->tapeRecord )
+		{
+			tapeRecords.args.push_back( r->tapeArgs );
+			tapeRecords.lanes.push_back( r->tapeLanes );
 		}
 		if ( r->fused )
 		{
@@ -3441,7 +3616,8 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 	Impl* b = impl;
 	b->used = 0;
 	size_t n = (size_t)count;
-	b->EnsureStaging( UploadBytes( worlds, count ) + ListBytes( prepare ) + ListBytes( prepareJoints ) + ListBytes( store ) +
+	b->EnsureStaging( UploadBytes( worlds, count ) + ListBytes( prepare ) + ListBytes( prepareJoints ) + ListBytes( tapeRecords ) +
+					  ListBytes( store ) +
 					  ListBytes( finalize ) + ListBytes( continuous ) +
 					  n * ( sizeof( FusedArgs ) + sizeof( int ) + sizeof( PackFinalizeArgs ) ) + 256 );
 	size_t entriesOff = 0;
@@ -3450,8 +3626,11 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 	size_t prepOff = 0, prepBlocksOff = 0, jointOff = 0, jointBlocksOff = 0, storeOff = 0, storeBlocksOff = 0;
 	size_t finOff = 0, finBlocksOff = 0, ccdOff = 0, ccdBlocksOff = 0, fusedOff = 0, orderOff = 0, packOff = 0;
 	int prepBlocks = 0, jointBlocks = 0, storeBlocks = 0, finBlocks = 0, ccdBlocks = 0;
+	size_t tapeOff = 0, tapeBlocksOff = 0;
+	int tapeBlocks = 0;
 	StageLane( b, prepare, &prepOff, &prepBlocksOff, &prepBlocks );
 	StageLane( b, prepareJoints, &jointOff, &jointBlocksOff, &jointBlocks );
+	StageLane( b, tapeRecords, &tapeOff, &tapeBlocksOff, &tapeBlocks );
 	StageLane( b, store, &storeOff, &storeBlocksOff, &storeBlocks );
 	StageLane( b, finalize, &finOff, &finBlocksOff, &finBlocks );
 	StageLane( b, continuous, &ccdOff, &ccdBlocksOff, &ccdBlocks );
@@ -3490,6 +3669,11 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 	{
 		LaneBatchKernel<PrepareJointsOp><<<jointBlocks, kBlock, 0, s>>>( reinterpret_cast<const PrepareJointArgs*>( base + jointOff ),
 																	  reinterpret_cast<const int2*>( base + jointBlocksOff ) );
+	}
+	if ( tapeBlocks > 0 )
+	{
+		LaneBatchKernel<TapeRecordOp><<<tapeBlocks, kBlock, 0, s>>>( reinterpret_cast<const TapeRecordArgs*>( base + tapeOff ),
+																  reinterpret_cast<const int2*>( base + tapeBlocksOff ) );
 	}
 
 	if ( fused.empty() == false )

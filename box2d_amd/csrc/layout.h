@@ -153,7 +153,9 @@ struct ContactSim
 	int bodyIdA;
 	int bodyIdB;
 	uint32_t simFlags;
-	int pad;
+	// decisions of the last narrowphase update, read by step recording: bit 0 manifold recycled;
+	// bits 2-3 and 4-5 warm-start source of point 0 and 1 (0 none, 1 + old point index)
+	uint32_t stepInfo;
 	b2Rot cachedRotationA;
 	b2Rot cachedRotationB;
 	b2Transform cachedRelativePose;

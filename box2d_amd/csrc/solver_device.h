@@ -201,6 +201,69 @@ struct LdsBodies
 };
 
 // ---------------------------------------------------------------------------------------------
+// Body integration
+
+// Velocity integration of one body (reference src/solver.c:66-112).
+B2G_DEVI void IntegrateVelocity( const BodySim& sim, BodyState* state, const StepParams& p )
+{
+	b2Vec2 v = state->linearVelocity;
+	float w = state->angularVelocity;
+	float h = p.h;
+
+	float linearDamping = 1.0f / ( 1.0f + h * sim.linearDamping );
+	float angularDamping = 1.0f / ( 1.0f + h * sim.angularDamping );
+	float gravityScale = sim.invMass > 0.0f ? sim.gravityScale : 0.0f;
+
+	b2Vec2 linearVelocityDelta = b2Add( b2MulSV( h * sim.invMass, sim.force ), b2MulSV( h * gravityScale, p.gravity ) );
+	float angularVelocityDelta = h * sim.invInertia * sim.torque;
+
+	v = b2MulAdd( linearVelocityDelta, linearDamping, v );
+	w = angularVelocityDelta + angularDamping * w;
+
+	state->linearVelocity = v;
+	state->angularVelocity = w;
+}
+
+// Position integration of one body: locks, speed caps, delta position and rotation
+// (reference src/solver.c:114-162).
+B2G_DEVI void IntegratePosition( BodyState* state, const StepParams& p )
+{
+	float h = p.h;
+	float maxLinearSpeed = p.maxLinearVelocity;
+	float maxAngularSpeed = ( 0.25f * B2_PI ) * p.inv_dt;
+	float maxLinearSpeedSquared = maxLinearSpeed * maxLinearSpeed;
+	float maxAngularSpeedSquared = maxAngularSpeed * maxAngularSpeed;
+
+	b2Vec2 v = state->linearVelocity;
+	float w = state->angularVelocity;
+	uint32_t flags = state->flags;
+
+	v.x = ( flags & kLockLinearX ) ? 0.0f : v.x;
+	v.y = ( flags & kLockLinearY ) ? 0.0f : v.y;
+	w = ( flags & kLockAngularZ ) ? 0.0f : w;
+
+	if ( b2Dot( v, v ) > maxLinearSpeedSquared )
+	{
+		float ratio = maxLinearSpeed / b2Length( v );
+		v = b2MulSV( ratio, v );
+		flags |= kIsSpeedCapped;
+	}
+
+	if ( w * w > maxAngularSpeedSquared && ( flags & kAllowFastRotation ) == 0 )
+	{
+		float ratio = maxAngularSpeed / b2AbsFloat( w );
+		w *= ratio;
+		flags |= kIsSpeedCapped;
+	}
+
+	state->flags = flags;
+	state->linearVelocity = v;
+	state->angularVelocity = w;
+	state->deltaPosition = b2MulAdd( state->deltaPosition, h, v );
+	state->deltaRotation = b2IntegrateRotation( state->deltaRotation, h * w );
+}
+
+// ---------------------------------------------------------------------------------------------
 // Colored (wide-lane) contact arithmetic
 
 template <typename Bodies>

@@ -463,6 +463,31 @@ static void b2sBuildBatchWorld( b2sScene* scene, int target, int kind, uint32_t 
 	}
 }
 
+// Pyramid of `rows` rows of 1 m boxes with a 1 m pusher box on the ground to its left, 0.5 m
+// from the pyramid's base, launched with the given velocity and spin. The pusher's friction is a
+// parameter; every other shape keeps the default friction 0.6. Bodies in creation order: the
+// pyramid row by row (the top box last), then the pusher.
+static void b2sBuildPush( b2sScene* scene, int rows, float vx, float vy, float omega, float friction )
+{
+	b2sBuildPyramid( scene, rows, 40.0f, 0 );
+
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	bodyDef.position = ( b2Pos ){ -0.5f * rows - 1.0f, 0.5f };
+	bodyDef.linearVelocity = ( b2Vec2 ){ vx, vy };
+	bodyDef.angularVelocity = omega;
+	b2BodyId pusherId = b2CreateBody( scene->worldId, &bodyDef );
+
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.density = 1.0f;
+	shapeDef.material.friction = friction;
+	b2Polygon box = b2MakeSquare( 0.5f );
+	b2CreatePolygonShape( pusherId, &shapeDef, &box );
+
+	int capacity = scene->bodyCount;
+	b2sPushBody( scene, &capacity, pusherId );
+}
+
 int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount )
 {
 	int handle = -1;
@@ -527,6 +552,17 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 			int which = iparamCount > 1 ? iparams[1] : 0;
 			uint32_t seed = iparamCount > 2 ? (uint32_t)iparams[2] : 12345u;
 			b2sBuildBatchWorld( scene, target, which, seed );
+		}
+		break;
+
+		case B2S_PUSH:
+		{
+			int rows = iparamCount > 0 ? iparams[0] : 10;
+			float vx = fparamCount > 0 ? fparams[0] : 4.0f;
+			float vy = fparamCount > 1 ? fparams[1] : 0.0f;
+			float omega = fparamCount > 2 ? fparams[2] : 0.0f;
+			float friction = fparamCount > 3 ? fparams[3] : 0.6f;
+			b2sBuildPush( scene, rows, vx, vy, omega, friction );
 		}
 		break;
 
@@ -614,6 +650,21 @@ double b2s_step_timed( int handle, float timeStep, int subStepCount, int count )
 		b2World_Step( scene->worldId, timeStep, subStepCount );
 	}
 	return b2sNowMs() - start;
+}
+
+int b2s_body_ids( int handle, int* out, int capacity )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	int n = scene->bodyCount < capacity ? scene->bodyCount : capacity;
+	for ( int i = 0; i < n; ++i )
+	{
+		out[i] = scene->bodies[i].index1 - 1;
+	}
+	return n;
 }
 
 int b2s_body_count( int handle )

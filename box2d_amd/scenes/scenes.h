@@ -52,6 +52,12 @@ enum
 	// Pyramids take the largest row count whose box count fits the target; piles place the
 	// target count on a grid; chains are 1-8 chains of 5-20 links (count = chains * links).
 	B2S_BATCH_WORLD = 5,
+
+	// Differentiable push: a pyramid of 1 m boxes (iparams[0] rows, default 10: 55 boxes) on the
+	// 40 x 1 m ground box, sleep off, and a 1 m pusher box on the ground 0.5 m left of the pyramid
+	// with initial velocity (fparams[0], fparams[1]), spin fparams[2] and friction fparams[3].
+	// Bodies: the pyramid row by row (top box last), then the pusher.
+	B2S_PUSH = 6,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.
@@ -69,6 +75,9 @@ B2S_API void b2s_step( int handle, float timeStep, int subStepCount, int count )
 
 // Wall-clock milliseconds spent in count steps (monotonic clock around b2World_Step calls).
 B2S_API double b2s_step_timed( int handle, float timeStep, int subStepCount, int count );
+
+// Body ids (index1 - 1 of each b2BodyId) of the scene bodies in creation order. Returns the count.
+B2S_API int b2s_body_ids( int handle, int* out, int capacity );
 
 // Number of bodies created by the scene (static ground excluded).
 B2S_API int b2s_body_count( int handle );

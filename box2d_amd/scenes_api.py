@@ -18,6 +18,7 @@ SCENE_HELLO = 2
 SCENE_MIXED_PILE = 3
 SCENE_RAIN = 4
 SCENE_BATCH_WORLD = 5
+SCENE_PUSH = 6
 
 
 class SceneLib:
@@ -40,6 +41,8 @@ class SceneLib:
                                            ctypes.POINTER(ctypes.c_uint32)]
         lib.b2s_world_id.restype = ctypes.c_uint32
         lib.b2s_world_id.argtypes = [ctypes.c_int]
+        lib.b2s_body_ids.restype = ctypes.c_int
+        lib.b2s_body_ids.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         self.lib = lib
 
     def create(self, kind, iparams=(), fparams=(), workers=1):
@@ -69,6 +72,17 @@ class SceneLib:
 
     def body_count(self, h):
         return self.lib.b2s_body_count(h)
+
+    def body_ids(self, h):
+        """Body ids (b2BodyId.index1 - 1) of the scene bodies in creation order."""
+        n = self.body_count(h)
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        self.lib.b2s_body_ids(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n)
+        return out[:n]
+
+    def push(self, vx, vy, omega, friction, rows=10):
+        """Differentiable push scene: pyramid plus a launched pusher box (SCENE_PUSH)."""
+        return self.create(SCENE_PUSH, (rows,), (vx, vy, omega, friction))
 
     def hash(self, h):
         return self.lib.b2s_hash(h)

@@ -105,6 +105,162 @@ typedef struct b2amd_BatchStats
 /// Returns the number of worlds stepped; stats may be null.
 B2_API int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int subStepCount, b2amd_BatchStats* stats );
 
+// ---------------------------------------------------------------------------------------------
+// Differentiation of recorded steps.
+//
+// A tape records, for each step of a set of worlds (b2World_Step or b2amd_StepWorlds), the body
+// state at the step's start and the step's contact constraints in solve order with the discrete
+// decisions the step took (manifold recycled or recomputed, warm-start point matches). Each record
+// is a checkpoint: the backward pass recomputes the step's substeps from it on the GPU and runs
+// the hand-written adjoint of the step in reverse, step by step. The active contact set, the
+// feature ids and every clamp pattern are those of the recorded forward (held fixed). Supported:
+// dynamic bodies with polygon shapes against static or dynamic polygons, no joints, no overflow
+// constraints, no restitution or rolling resistance, no speed caps, locks or continuous events.
+
+/// Body state (or its gradient) in the form the adjoint uses: center of mass, rotation,
+/// velocities.
+typedef struct b2amd_BodyState
+{
+	b2Vec2 center;
+	b2Rot rotation;
+	b2Vec2 linearVelocity;
+	float angularVelocity;
+	float padding;
+} b2amd_BodyState;
+
+/// Step header of a record.
+typedef struct b2amd_TapeStep
+{
+	int bodyCount;		  // awake bodies, slots 1..bodyCount (slot 0 is the static identity)
+	int constraintCount;  // contact constraints in solve order
+	int jointCount;		  // joints in the step (not differentiable)
+	int activeColorCount; // graph colors in solve order
+	int colorStart[25];	  // first constraint of each active color (+ sentinel)
+	int overflowStart;	  // overflow constraints [overflowStart, overflowStart + overflowCount)
+	int overflowCount;
+	int subStepCount;
+	float timeStep;
+	float h;	 // substep
+	float inv_h;
+	float contactSpeed; // maximum contact push-out speed (the solver uses -contactSpeed)
+	float maxLinearSpeed;
+	float maxAngularSpeed; // 0.25 pi / timeStep
+	b2Vec2 gravity;
+	int enableWarmStarting;
+	float inv_dt;
+	int padding[2];
+} b2amd_TapeStep;
+
+/// One awake body of a record, by slot.
+typedef struct b2amd_TapeBody
+{
+	int bodyId;		// index1 - 1 of the body's b2BodyId
+	uint32_t flags; // solver flags (locks, speed cap, continuous)
+	b2Vec2 center;
+	b2Vec2 origin; // transform position
+	b2Rot rotation;
+	b2Vec2 linearVelocity;
+	float angularVelocity;
+	float invMass;
+	float invInertia;
+	float gravityScale;
+	float linearDamping;
+	float angularDamping;
+	b2Vec2 localCenter;
+	b2Vec2 force;
+	float torque;
+	float padding;
+} b2amd_TapeBody;
+
+/// One contact constraint of a record, after the step's narrowphase and constraint preparation.
+typedef struct b2amd_TapeConstraint
+{
+	int contactId;
+	int shapeIdA, shapeIdB;
+	int bodyIdA, bodyIdB;
+	int slotA, slotB; // body slots of the step (0: static body)
+	int pointCount;
+	/// bit 0: manifold recycled this step (anchors, normal and base separations kept, separations
+	/// updated); bits 2-3 and 4-5: warm-start source of point 0 and 1 (0 none, 1 + index of the
+	/// point of the previous step's manifold of the same contact)
+	uint32_t info;
+	uint16_t id[2];
+	b2Vec2 normal;
+	b2Vec2 anchorA[2]; // relative to body A's center of mass
+	b2Vec2 anchorB[2];
+	float separation[2];
+	float baseSeparation[2]; // separations at the manifold's last full update
+	float normalImpulse[2];	 // warm-start impulses
+	float tangentImpulse[2];
+	float friction;
+	float restitution;
+	float rollingResistance;
+	float tangentSpeed;
+	float biasRate, massScale, impulseScale; // constraint softness
+	b2Rot cachedRotationA, cachedRotationB;	 // body rotations at the manifold's last full update
+	float padding;
+} b2amd_TapeConstraint;
+
+typedef struct b2amd_Tape b2amd_Tape;
+
+/// Start recording the steps of count worlds. Every later step of these worlds (b2World_Step or
+/// b2amd_StepWorlds) appends a record until b2amd_Tape_Finish. A world is on at most one tape.
+/// Recorded worlds use the fused solver. Returns null if a world is already on a tape.
+B2_API b2amd_Tape* b2amd_Tape_Create( const b2WorldId* worldIds, int count );
+
+/// Stop recording and capture each world's final body state. Returns the number of steps.
+B2_API int b2amd_Tape_Finish( b2amd_Tape* tape );
+
+/// Release the tape's device memory.
+B2_API void b2amd_Tape_Destroy( b2amd_Tape* tape );
+
+/// Steps recorded (equal for all worlds of a tape stepped together).
+B2_API int b2amd_Tape_GetStepCount( b2amd_Tape* tape, int worldIndex );
+
+/// Array strides of the gradient buffers: body and shape id capacities over the tape's worlds.
+B2_API int b2amd_Tape_GetBodyStride( b2amd_Tape* tape );
+B2_API int b2amd_Tape_GetShapeStride( b2amd_Tape* tape );
+
+/// Copy one record to the host: step < step count gives the step's header, bodies and
+/// constraints; step == step count gives the final state (bodies only). Capacities bound the
+/// copies. Returns 0, or -1 for an invalid world or step.
+B2_API int b2amd_Tape_ReadStep( b2amd_Tape* tape, int worldIndex, int step, b2amd_TapeStep* header, b2amd_TapeBody* bodies,
+								int bodyCapacity, b2amd_TapeConstraint* constraints, int constraintCapacity );
+
+/// Final body states of all worlds: states[world * bodyStride + bodyId] (bodies not awake at the
+/// end are left untouched).
+B2_API int b2amd_Tape_GetFinalStates( b2amd_Tape* tape, b2amd_BodyState* states );
+
+/// Diagnostics of a backward pass.
+typedef struct b2amd_AdjointStats
+{
+	/// GPU time of the backward launch, milliseconds (HIP events)
+	float backwardMs;
+	/// recomputed values that differ from the recorded forward (0: the recompute is bit-exact)
+	int replayMismatches;
+	/// union over worlds of unsupported features found (0: the gradient is complete); bits:
+	/// 1 overflow constraints, 2 joints, 4 restitution or rolling resistance, 8 non-polygon
+	/// manifold, 16 speed cap, 32 awake set changed, 64 motion locks, 128 continuous collision
+	int unsupported;
+	int worlds;
+	int steps;
+} b2amd_AdjointStats;
+
+/// Reverse pass over the whole tape. finalGrad[world * bodyStride + bodyId] is the gradient of
+/// a scalar loss with respect to the final body states; the pass writes the gradient with respect
+/// to the initial body states to initialGrad (same layout) and the gradient with respect to each
+/// shape's friction to shapeFrictionGrad[world * shapeStride + shapeId] (contact friction is
+/// sqrt(frictionA * frictionB), shape frictions held constant over the run). Host arrays;
+/// synchronous. Returns 0.
+B2_API int b2amd_Tape_Backward( b2amd_Tape* tape, const b2amd_BodyState* finalGrad, b2amd_BodyState* initialGrad,
+								float* shapeFrictionGrad, b2amd_AdjointStats* stats );
+
+/// Current state of any body of a tape's world (static bodies included), in record form.
+B2_API int b2amd_Tape_GetBody( b2amd_Tape* tape, int worldIndex, int bodyId, b2amd_TapeBody* body );
+
+/// A polygon shape of a tape's world, with its body id and friction (for checkers).
+B2_API int b2amd_Tape_GetShape( b2amd_Tape* tape, int worldIndex, int shapeId, b2amd_Shape* shape, int* bodyId, float* friction );
+
 #ifdef __cplusplus
 }
 #endif
