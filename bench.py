@@ -121,6 +121,118 @@ def batched_workload(lib, scenes, worlds, steps, loop_steps):
             "speedup_vs_plain_loop": loop_s / batched_s, "plain_loop_steps": loop_steps}
 
 
+def rain_workload(lib, scenes, steps, cpu_steps, cpu_threads, no_cpu):
+    """The 20,000-body mixed-shape rain into the U-shaped chain container (20k bodies, 1000 steps):
+    whole-run GPU time and the final hash against the reference's; the reference library on the
+    host's cores over the first cpu_steps steps, with the GPU's time over the same steps beside it."""
+    want = None
+    path = os.path.join(ROOT, "tests", "golden", "world_runs.json")
+    try:
+        with open(path) as f:
+            runs = json.load(f)
+        fx = next(x for x in runs["scenes"] if x["name"] == "rain")
+        want = fx["hashes"].get(str(steps))
+    except (OSError, StopIteration):
+        pass
+    iparams, fparams = (20000, 100, 0, 12345), (120.0,)
+    h = scenes.create(4, iparams, fparams)
+    wid = scenes.world_id(h)
+    bodies = scenes.body_count(h)
+    used = {0: 0, 1: 0, 2: 0}
+    first_ms = None
+    lib.b2amd_Synchronize()
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        scenes.step(h, 1, DT, SUBSTEPS)
+        used[lib.b2amd_World_GetLastPairQuery(wid)] += 1
+        if k == cpu_steps:
+            lib.b2amd_Synchronize()
+            first_ms = (time.perf_counter() - t0) * 1e3 / cpu_steps
+    lib.b2amd_Synchronize()
+    total = time.perf_counter() - t0
+    got = scenes.hash(h)
+    scenes.destroy(h)
+    out = {"workload": "rain: 20,000 mixed shapes (circles, capsules, hulls, rounded boxes) into a U chain container",
+           "bodies": bodies, "steps": steps, "gpu_ms_per_step": total * 1e3 / steps,
+           "gpu_body_substeps_per_s": bodies * SUBSTEPS * steps / total, "hash": f"{got:#010x}",
+           "hash_match": None if want is None else bool(got == want),
+           "pair_query_steps": {"gpu_lbvh": used[2], "host_tree": used[1], "none": used[0]},
+           "gpu_ms_per_step_first_steps": first_ms, "first_steps": cpu_steps}
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
+    if not no_cpu and os.path.exists(ref_path):
+        from box2d_amd.scenes_api import SceneLib
+
+        ref = SceneLib(ref_path)
+        hr = ref.create(4, iparams, fparams, workers=cpu_threads)
+        ms = ref.step_timed(hr, cpu_steps, DT, SUBSTEPS) / cpu_steps
+        ref.destroy(hr)
+        out["cpu_baseline"] = {"value": ms, "unit": "ms/step", "cores": cpu_threads, "kind": "reference",
+                               "sample": f"steps 1..{cpu_steps} of the same rain, workerCount={cpu_threads}"}
+        out["speedup_vs_cpu_first_steps"] = ms / first_ms if first_ms else None
+    return out
+
+
+def push_workload(lib, worlds, steps, loop_worlds):
+    """Differentiable push (256 worlds of a 10-row box pyramid plus a pusher box, 60 steps): the
+    batched forward with every step recorded on a tape, the GPU adjoint over the tape, and the
+    same worlds stepped without recording and one by one through b2World_Step."""
+    import numpy as np
+
+    import box2d_amd
+    from box2d_amd.adjoint import BODY_STATE, PushWorlds
+
+    rng = np.random.default_rng(4)
+    params = np.stack([rng.uniform(2, 6, worlds), rng.uniform(-0.5, 0.5, worlds), rng.uniform(-1, 1, worlds),
+                       rng.uniform(0.3, 0.9, worlds)], 1)
+    goals = np.stack([rng.uniform(-1, 1, worlds), rng.uniform(8.5, 9.5, worlds)], 1)
+
+    # plain batched forward (no tape)
+    pw = PushWorlds(params)
+    bodies = sum(len(b) for b in pw.body_ids)
+    ids = (box2d_amd.WorldId * worlds)(*pw.ids)
+    lib.b2amd_Synchronize()
+    t0 = time.perf_counter()
+    box2d_amd.step_worlds(ids, steps, DT, SUBSTEPS)
+    lib.b2amd_Synchronize()
+    plain_s = time.perf_counter() - t0
+    pw.destroy()
+
+    # recorded forward + backward
+    pw = PushWorlds(params)
+    t0 = time.perf_counter()
+    tape = pw.record(steps)
+    lib.b2amd_Synchronize()
+    fwd_s = time.perf_counter() - t0
+    final = tape.final_states()
+    fg = np.zeros((worlds, tape.body_stride), dtype=BODY_STATE)
+    for w in range(worlds):
+        top = int(pw.body_ids[w][-2])
+        fg[w]["center"][top] = 2.0 * (final[w]["center"][top] - goals[w])
+    t0 = time.perf_counter()
+    ig, fr, stats = tape.backward(fg)
+    bwd_s = time.perf_counter() - t0
+    pw.destroy()
+
+    # plain per-world loop on a subset
+    pw = PushWorlds(params[:loop_worlds])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for h in pw.handles:
+            pw.scenes.step(h, 1, DT, SUBSTEPS)
+    lib.b2amd_Synchronize()
+    loop_s = (time.perf_counter() - t0) * worlds / loop_worlds
+    pw.destroy()
+
+    work = bodies * SUBSTEPS * steps
+    return {"workload": f"push: {worlds} worlds x (10-row pyramid + pusher box), {steps} steps, loss |top - goal|^2",
+            "worlds": worlds, "bodies": bodies, "steps": steps,
+            "forward_recorded_ms_per_step": fwd_s * 1e3 / steps, "forward_plain_ms_per_step": plain_s * 1e3 / steps,
+            "backward_ms_per_step": bwd_s * 1e3 / steps, "backward_kernel_ms": stats.backwardMs,
+            "fwd_bwd_body_substeps_per_s": work / (fwd_s + bwd_s), "plain_batched_body_substeps_per_s": work / plain_s,
+            "plain_loop_body_substeps_per_s": work / loop_s, "plain_loop_sample_worlds": loop_worlds,
+            "replay_mismatches": stats.replayMismatches, "unsupported": stats.unsupported}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +246,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched workload (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--rain-steps", type=int, default=1000, help="steps of the rain workload (0: skip)")
+    ap.add_argument("--rain-cpu-steps", type=int, default=150, help="bounded CPU-baseline sample of the rain")
+    ap.add_argument("--push-worlds", type=int, default=256, help="worlds of the differentiable push (0: skip)")
+    ap.add_argument("--push-steps", type=int, default=60)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -205,6 +321,16 @@ def main():
     if args.batched_worlds > 0:
         batched = batched_workload(lib, scenes, args.batched_worlds, args.batched_steps, 2)
 
+    lib.b2amd_World_GetLastPairQuery.restype = ctypes.c_int
+    rain = None
+    if args.rain_steps > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        rain = rain_workload(lib, scenes, args.rain_steps, min(args.rain_cpu_steps, args.rain_steps), threads, args.no_cpu)
+
+    push = None
+    if args.push_worlds > 0:
+        push = push_workload(lib, args.push_worlds, args.push_steps, min(32, args.push_worlds))
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -230,6 +356,8 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
         "batched": batched,
+        "rain": rain,
+        "push_adjoint": push,
     }
     print(json.dumps(line))
 

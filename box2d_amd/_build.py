@@ -40,7 +40,7 @@ COMMON = ["-O3", "-fPIC", "-fvisibility=hidden", "-I", INCLUDE, "-I", CSRC, "-Wa
           "-Wno-missing-braces"]
 
 HOST_SOURCES = ["world.cpp", "tree.cpp", "geometry.cpp", "capi.cpp", "batch.cpp"]
-HIP_SOURCES = ["device.hip", "adjoint.hip"]
+HIP_SOURCES = ["device.hip", "adjoint.hip", "broadphase.hip"]
 
 
 def _hipcc():

@@ -643,6 +643,21 @@ void b2amd_World_SetSolverPath( b2WorldId worldId, b2amd_SolverPath path )
 	}
 }
 
+void b2amd_World_SetPairQuery( b2WorldId worldId, b2amd_PairQuery mode )
+{
+	World* world = GetWorld( worldId );
+	if ( world != nullptr )
+	{
+		world->pairQuery = (int)mode;
+	}
+}
+
+int b2amd_World_GetLastPairQuery( b2WorldId worldId )
+{
+	World* world = GetWorld( worldId );
+	return world != nullptr ? world->lastPairQueryPath : 0;
+}
+
 int b2amd_TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs )
 {
 	return TimeOfImpactBatch( inputs, count, outputs );

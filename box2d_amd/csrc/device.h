@@ -79,6 +79,7 @@ public:
 
 	// Record instead of launching (the batch that steps many worlds launches the records).
 	void SetRecording( bool flag );
+	bool Recording() const;
 	void* Record();
 
 	// Per-stage GPU timings of the last step in milliseconds (collide, solve, finalize).
@@ -151,6 +152,39 @@ struct BatchStats
 
 // Step count worlds as one batch (defined in batch.cpp). Returns the number of worlds stepped.
 int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, BatchStats* stats );
+
+// GPU broadphase pair query (broadphase.hip): a Morton-ordered LBVH over every proxy of a world,
+// queried with the fat boxes of the moved proxies.
+struct LbvhProxy
+{
+	int proxyKey; // proxy id << 2 | body type of its tree
+	int shapeId;
+	b2AABB aabb; // the tree's fat box
+};
+
+struct PairCandidate
+{
+	int query; // index into the query list
+	int proxyKey;
+	int shapeId;
+};
+
+class PairFinder
+{
+public:
+	PairFinder();
+	~PairFinder();
+
+	// Every (query, proxy) whose fat boxes overlap and that passes the pair callback's self, tree
+	// and moved-proxy rules (moved: 1 per shape id whose proxy is in the move buffer). Unordered.
+	void Find( const std::vector<LbvhProxy>& proxies, const std::vector<LbvhProxy>& queries, const std::vector<uint8_t>& moved,
+			   std::vector<PairCandidate>* out );
+
+	struct Impl;
+
+private:
+	Impl* impl;
+};
 
 // Device storage of recorded steps (adjoint.hip). Records are carved out of large device chunks
 // and stay valid until the store is destroyed.

@@ -3333,6 +3333,11 @@ void DeviceWorld::SetRecording( bool flag )
 	impl->rec = Impl::Record{};
 }
 
+bool DeviceWorld::Recording() const
+{
+	return impl->recording;
+}
+
 void* DeviceWorld::Record()
 {
 	return &impl->rec;

@@ -410,6 +410,7 @@ World::World( const b2WorldDef* def, int worldIndex, uint16_t gen )
 
 World::~World()
 {
+	delete pairFinder;
 	delete device;
 }
 
@@ -2126,10 +2127,163 @@ b2JointId World::CreateRevoluteJoint( const b2RevoluteJointDef* def )
 // ---------------------------------------------------------------------------------------------
 // Broadphase pair update (reference src/broad_phase.c:172-529)
 
+// Remaining rules of b2PairQueryCallback for one hit (reference src/broad_phase.c:216-324):
+// existing pair, shape order, same body, sensors, filters, shape types, joints.
+bool World::AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair )
+{
+	uint64_t pairKey = B2G_SHAPE_PAIR_KEY( shapeId, queryShapeIndex );
+	if ( broadPhase.pairSet.count( pairKey ) != 0 )
+	{
+		return false;
+	}
+
+	int shapeIdA, shapeIdB;
+	if ( proxyKey < queryProxyKey )
+	{
+		shapeIdA = shapeId;
+		shapeIdB = queryShapeIndex;
+	}
+	else
+	{
+		shapeIdA = queryShapeIndex;
+		shapeIdB = shapeId;
+	}
+
+	Shape* shapeA = &shapes[shapeIdA];
+	Shape* shapeB = &shapes[shapeIdB];
+	if ( shapeA->bodyId == shapeB->bodyId )
+	{
+		return false;
+	}
+	if ( shapeA->sensorIndex != B2_NULL_INDEX || shapeB->sensorIndex != B2_NULL_INDEX )
+	{
+		return false;
+	}
+	if ( ShouldShapesCollide( shapeA->filter, shapeB->filter ) == false )
+	{
+		return false;
+	}
+	if ( CanCollide( shapeA->type, shapeB->type ) == false )
+	{
+		return false;
+	}
+	if ( ShouldBodiesCollide( &bodies[shapeA->bodyId], &bodies[shapeB->bodyId] ) == false )
+	{
+		return false;
+	}
+	*pair = MovePair{ shapeIdA, shapeIdB };
+	return true;
+}
+
+// New pairs of the moved proxies on the GPU (LBVH over all proxies), then ordered per query the
+// way the reference's stack query visits leaves: kinematic tree, static tree, dynamic tree, each
+// in depth-first child2-first leaf order (src/broad_phase.c:370-397, src/dynamic_tree.c:1085-1135).
+void World::QueryPairsGpu( std::vector<std::vector<MovePair>>& results )
+{
+	BroadPhase& bp = broadPhase;
+	if ( pairFinder == nullptr )
+	{
+		pairFinder = new PairFinder();
+	}
+
+	std::vector<LbvhProxy> proxies;
+	for ( int type = 0; type < b2_bodyTypeCount; ++type )
+	{
+		const std::vector<TreeNode>& nodes = bp.trees[type].Nodes();
+		for ( int id = 0; id < (int)nodes.size(); ++id )
+		{
+			const TreeNode& node = nodes[id];
+			if ( ( node.flags & kNodeAllocated ) && ( node.flags & kNodeLeaf ) )
+			{
+				proxies.push_back( LbvhProxy{ B2G_PROXY_KEY( id, type ), (int)node.userData, node.aabb } );
+			}
+		}
+	}
+
+	int moveCount = (int)bp.moveArray.size();
+	std::vector<LbvhProxy> queries;
+	std::vector<int> queryMove;
+	std::vector<uint8_t> moved( std::max<size_t>( shapes.size(), 1 ), 0 );
+	for ( int i = 0; i < moveCount; ++i )
+	{
+		int key = bp.moveArray[i];
+		if ( key == B2_NULL_INDEX )
+		{
+			continue;
+		}
+		const DynamicTree& tree = bp.trees[B2G_PROXY_TYPE( key )];
+		int shapeId = (int)tree.GetUserData( B2G_PROXY_ID( key ) );
+		queries.push_back( LbvhProxy{ key, shapeId, tree.GetAABB( B2G_PROXY_ID( key ) ) } );
+		queryMove.push_back( i );
+		moved[shapeId] = 1;
+	}
+
+	std::vector<PairCandidate> cands;
+	pairFinder->Find( proxies, queries, moved, &cands );
+
+	// leaf visit rank of every proxy of each tree
+	std::vector<int> rank[b2_bodyTypeCount];
+	for ( int type = 0; type < b2_bodyTypeCount; ++type )
+	{
+		std::vector<int> order;
+		bp.trees[type].LeafOrder( order );
+		rank[type].assign( bp.trees[type].Nodes().size(), 0 );
+		for ( int k = 0; k < (int)order.size(); ++k )
+		{
+			rank[type][order[k]] = k;
+		}
+	}
+
+	// bucket by query, then sort each bucket into visit order
+	int nq = (int)queries.size();
+	std::vector<int> start( nq + 1, 0 );
+	for ( const PairCandidate& c : cands )
+	{
+		start[c.query + 1] += 1;
+	}
+	for ( int q = 0; q < nq; ++q )
+	{
+		start[q + 1] += start[q];
+	}
+	std::vector<PairCandidate> sorted( cands.size() );
+	{
+		std::vector<int> fill( start.begin(), start.end() - 1 );
+		for ( const PairCandidate& c : cands )
+		{
+			sorted[fill[c.query]++] = c;
+		}
+	}
+	auto treeOrder = []( int type ) { return type == b2_kinematicBody ? 0 : ( type == b2_staticBody ? 1 : 2 ); };
+	for ( int q = 0; q < nq; ++q )
+	{
+		auto begin = sorted.begin() + start[q];
+		auto end = sorted.begin() + start[q + 1];
+		std::sort( begin, end, [&]( const PairCandidate& x, const PairCandidate& y ) {
+			int tx = B2G_PROXY_TYPE( x.proxyKey ), ty = B2G_PROXY_TYPE( y.proxyKey );
+			if ( tx != ty )
+			{
+				return treeOrder( tx ) < treeOrder( ty );
+			}
+			return rank[tx][B2G_PROXY_ID( x.proxyKey )] < rank[ty][B2G_PROXY_ID( y.proxyKey )];
+		} );
+		std::vector<MovePair>& pairList = results[queryMove[q]];
+		const LbvhProxy& query = queries[q];
+		for ( auto it = begin; it != end; ++it )
+		{
+			MovePair pair;
+			if ( AcceptPair( query.proxyKey, query.shapeId, it->proxyKey, it->shapeId, &pair ) )
+			{
+				pairList.push_back( pair );
+			}
+		}
+	}
+}
+
 void World::UpdateBroadPhasePairs()
 {
 	BroadPhase& bp = broadPhase;
 	int moveCount = (int)bp.moveArray.size();
+	lastPairQueryPath = 0;
 	if ( moveCount == 0 )
 	{
 		return;
@@ -2137,103 +2291,81 @@ void World::UpdateBroadPhasePairs()
 
 	std::vector<std::vector<MovePair>> results( moveCount );
 
-	for ( int i = 0; i < moveCount; ++i )
+	static const int s_gpuPairThreshold = []() {
+		const char* v = getenv( "B2AMD_GPU_PAIR_THRESHOLD" );
+		return v != nullptr ? atoi( v ) : 1024;
+	}();
+	bool gpu = pairQuery == 2 || ( pairQuery == 0 && moveCount >= s_gpuPairThreshold );
+	gpu = gpu && device->Recording() == false;
+	lastPairQueryPath = gpu ? 2 : 1;
+
+	if ( gpu )
 	{
-		std::vector<MovePair>& pairList = results[i];
-		int queryProxyKey = bp.moveArray[i];
-		if ( queryProxyKey == B2_NULL_INDEX )
+		QueryPairsGpu( results );
+	}
+	else
+	{
+		for ( int i = 0; i < moveCount; ++i )
 		{
-			continue;
-		}
-
-		b2BodyType proxyType = B2G_PROXY_TYPE( queryProxyKey );
-		int proxyId = B2G_PROXY_ID( queryProxyKey );
-		const DynamicTree& baseTree = bp.trees[proxyType];
-		b2AABB fatAABB = baseTree.GetAABB( proxyId );
-		int queryShapeIndex = (int)baseTree.GetUserData( proxyId );
-
-		b2BodyType treeType = b2_staticBody;
-		auto callback = [&]( int hitProxyId, uint64_t userData ) -> bool {
-			int shapeId = (int)userData;
-			int proxyKey = B2G_PROXY_KEY( hitProxyId, treeType );
-			if ( proxyKey == queryProxyKey )
+			std::vector<MovePair>& pairList = results[i];
+			int queryProxyKey = bp.moveArray[i];
+			if ( queryProxyKey == B2_NULL_INDEX )
 			{
-				return true;
+				continue;
 			}
 
-			if ( proxyType == b2_dynamicBody )
-			{
-				if ( treeType == b2_dynamicBody && proxyKey < queryProxyKey )
+			b2BodyType proxyType = B2G_PROXY_TYPE( queryProxyKey );
+			int proxyId = B2G_PROXY_ID( queryProxyKey );
+			const DynamicTree& baseTree = bp.trees[proxyType];
+			b2AABB fatAABB = baseTree.GetAABB( proxyId );
+			int queryShapeIndex = (int)baseTree.GetUserData( proxyId );
+
+			b2BodyType treeType = b2_staticBody;
+			auto callback = [&]( int hitProxyId, uint64_t userData ) -> bool {
+				int shapeId = (int)userData;
+				int proxyKey = B2G_PROXY_KEY( hitProxyId, treeType );
+				if ( proxyKey == queryProxyKey )
+				{
+					return true;
+				}
+
+				if ( proxyType == b2_dynamicBody )
+				{
+					if ( treeType == b2_dynamicBody && proxyKey < queryProxyKey )
+					{
+						if ( bp.movedProxies[treeType].GetBit( hitProxyId ) )
+						{
+							return true;
+						}
+					}
+				}
+				else
 				{
 					if ( bp.movedProxies[treeType].GetBit( hitProxyId ) )
 					{
 						return true;
 					}
 				}
-			}
-			else
-			{
-				if ( bp.movedProxies[treeType].GetBit( hitProxyId ) )
+
+				MovePair pair;
+				if ( AcceptPair( queryProxyKey, queryShapeIndex, proxyKey, shapeId, &pair ) )
 				{
-					return true;
+					pairList.push_back( pair );
 				}
+				return true;
+			};
+
+			if ( proxyType == b2_dynamicBody )
+			{
+				treeType = b2_kinematicBody;
+				bp.trees[b2_kinematicBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
+				treeType = b2_staticBody;
+				bp.trees[b2_staticBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
 			}
 
-			uint64_t pairKey = B2G_SHAPE_PAIR_KEY( shapeId, queryShapeIndex );
-			if ( bp.pairSet.count( pairKey ) != 0 )
-			{
-				return true;
-			}
-
-			int shapeIdA, shapeIdB;
-			if ( proxyKey < queryProxyKey )
-			{
-				shapeIdA = shapeId;
-				shapeIdB = queryShapeIndex;
-			}
-			else
-			{
-				shapeIdA = queryShapeIndex;
-				shapeIdB = shapeId;
-			}
-
-			Shape* shapeA = &shapes[shapeIdA];
-			Shape* shapeB = &shapes[shapeIdB];
-			if ( shapeA->bodyId == shapeB->bodyId )
-			{
-				return true;
-			}
-			if ( shapeA->sensorIndex != B2_NULL_INDEX || shapeB->sensorIndex != B2_NULL_INDEX )
-			{
-				return true;
-			}
-			if ( ShouldShapesCollide( shapeA->filter, shapeB->filter ) == false )
-			{
-				return true;
-			}
-			if ( CanCollide( shapeA->type, shapeB->type ) == false )
-			{
-				return true;
-			}
-			if ( ShouldBodiesCollide( &bodies[shapeA->bodyId], &bodies[shapeB->bodyId] ) == false )
-			{
-				return true;
-			}
-
-			pairList.push_back( MovePair{ shapeIdA, shapeIdB } );
-			return true;
-		};
-
-		if ( proxyType == b2_dynamicBody )
-		{
-			treeType = b2_kinematicBody;
-			bp.trees[b2_kinematicBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
-			treeType = b2_staticBody;
-			bp.trees[b2_staticBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
+			treeType = b2_dynamicBody;
+			bp.trees[b2_dynamicBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
 		}
-
-		treeType = b2_dynamicBody;
-		bp.trees[b2_dynamicBody].Query( fatAABB, B2_DEFAULT_MASK_BITS, callback );
 	}
 
 	// The tree rebuild runs after the queries (the reference overlaps it with the narrowphase).

@@ -307,6 +307,12 @@ public:
 
 	DeviceWorld* device = nullptr;
 
+	// Pair query: 0 auto (GPU LBVH for large move buffers outside batched stepping), 1 host tree
+	// query, 2 GPU LBVH. Both give the same pairs in the same order.
+	int pairQuery = 0;
+	class PairFinder* pairFinder = nullptr;
+	int lastPairQueryPath = 0; // 1 host, 2 GPU (for tests)
+
 private:
 	friend class DeviceWorld;
 
@@ -343,6 +349,8 @@ private:
 
 	// step stages
 	void UpdateBroadPhasePairs();
+	void QueryPairsGpu( std::vector<std::vector<MovePair>>& results );
+	bool AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair );
 	void ProcessContactStates( const int* changedIds, int count );
 	bool SolveBegin( StepParams& params );
 	void SolveEnd();

@@ -47,6 +47,23 @@ typedef enum b2amd_SolverPath
 
 B2_API void b2amd_World_SetSolverPath( b2WorldId worldId, b2amd_SolverPath path );
 
+/// Broadphase pair query of a world's step (the move-buffer query of reference
+/// src/broad_phase.c:412-529). The GPU query builds a Morton-ordered linear BVH over every proxy
+/// and walks it with one lane per moved proxy; the host orders the hits in the reference trees'
+/// visit order. Both give the same pairs in the same order; Auto uses the GPU query when at least
+/// 1024 proxies moved (outside b2amd_StepWorlds, whose host phase uses the tree query).
+typedef enum b2amd_PairQuery
+{
+	b2amd_pairQueryAuto = 0,
+	b2amd_pairQueryHostTree = 1,
+	b2amd_pairQueryGpu = 2,
+} b2amd_PairQuery;
+
+B2_API void b2amd_World_SetPairQuery( b2WorldId worldId, b2amd_PairQuery mode );
+
+/// Path the last step's pair query took: 1 host tree, 2 GPU, 0 no query (empty move buffer).
+B2_API int b2amd_World_GetLastPairQuery( b2WorldId worldId );
+
 /// Block until all GPU work issued by this library on the current device has completed
 /// (a device-wide synchronize; b2World_Step already returns with its own work complete).
 B2_API void b2amd_Synchronize( void );

@@ -52,6 +52,7 @@ struct RevTape
 };
 
 RevTape* g_tape = nullptr;
+double* g_stepGrads = nullptr;
 
 struct Rev
 {
@@ -136,6 +137,46 @@ template <>
 inline Rev K<Rev>( float x )
 {
 	return Rev{ (double)x, -1 };
+}
+
+// Every branch the replay takes on a computed value goes through Dec. Recording keeps the branch
+// pattern of a run (the float replay's is the recorded forward's); forcing makes a float64 or
+// reverse-mode replay take that pattern, i.e. differentiate the recorded forward's smooth piece
+// with the active set and clamp pattern held fixed. flips counts forced branches that the run's
+// own values would have taken the other way.
+struct Decisions
+{
+	std::vector<uint8_t> bits;
+	size_t pos = 0;
+	int mode = 0; // 0 off, 1 record, 2 force
+	int flips = 0;
+	bool overrun = false;
+};
+
+Decisions* g_dec = nullptr;
+
+inline bool Dec( bool natural )
+{
+	if ( g_dec == nullptr || g_dec->mode == 0 )
+	{
+		return natural;
+	}
+	if ( g_dec->mode == 1 )
+	{
+		g_dec->bits.push_back( natural ? 1 : 0 );
+		return natural;
+	}
+	if ( g_dec->pos >= g_dec->bits.size() )
+	{
+		g_dec->overrun = true;
+		return natural;
+	}
+	bool b = g_dec->bits[g_dec->pos++] != 0;
+	if ( b != natural )
+	{
+		g_dec->flips += 1;
+	}
+	return b;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -243,7 +284,7 @@ template <typename T>
 inline R2<T> NormalizeRot( R2<T> q )
 {
 	T mag = Sqrt( q.s * q.s + q.c * q.c );
-	T invMag = Val( mag ) > 0.0 ? K<T>( 1.0f ) / mag : K<T>( 0.0f );
+	T invMag = Dec( Val( mag ) > 0.0 ) ? K<T>( 1.0f ) / mag : K<T>( 0.0f );
 	return R2<T>{ q.c * invMag, q.s * invMag };
 }
 template <typename T>
@@ -251,13 +292,17 @@ inline R2<T> IntegrateRotation( R2<T> q1, T deltaAngle )
 {
 	R2<T> q2 = { q1.c - deltaAngle * q1.s, q1.s + deltaAngle * q1.c };
 	T mag = Sqrt( q2.s * q2.s + q2.c * q2.c );
-	T invMag = Val( mag ) > 0.0 ? K<T>( 1.0f ) / mag : K<T>( 0.0f );
+	T invMag = Dec( Val( mag ) > 0.0 ) ? K<T>( 1.0f ) / mag : K<T>( 0.0f );
 	return R2<T>{ q2.c * invMag, q2.s * invMag };
 }
 template <typename T>
 inline T Clamp01( T a )
 {
-	return Val( a ) < 0.0 ? K<T>( 0.0f ) : ( Val( a ) > 1.0 ? K<T>( 1.0f ) : a );
+	if ( Dec( Val( a ) < 0.0 ) )
+	{
+		return K<T>( 0.0f );
+	}
+	return Dec( Val( a ) > 1.0 ) ? K<T>( 1.0f ) : a;
 }
 
 // Branch and feature record of a run (FNV-1a over 32-bit words).
@@ -333,12 +378,12 @@ double MaxSeparation( int* edge, const Poly<T>& p1, const Poly<T>& p2 )
 		for ( int j = 0; j < p2.count; ++j )
 		{
 			double sij = Val( Dot( p1.n[i], Sub( p2.v[j], p1.v[i] ) ) );
-			if ( sij < si )
+			if ( Dec( sij < si ) )
 			{
 				si = sij;
 			}
 		}
-		if ( si > maxSep )
+		if ( Dec( si > maxSep ) )
 		{
 			maxSep = si;
 			best = i;
@@ -367,18 +412,18 @@ Local<T> Clip( const Poly<T>& polyA, const Poly<T>& polyB, int edgeA, int edgeB,
 	T upper1 = Dot( Sub( v12, v11 ), tangent );
 	T upper2 = Dot( Sub( v21, v11 ), tangent );
 	T lower2 = Dot( Sub( v22, v11 ), tangent );
-	if ( Val( upper2 ) < Val( lower1 ) || Val( upper1 ) < Val( lower2 ) )
+	if ( Dec( Val( upper2 ) < Val( lower1 ) || Val( upper1 ) < Val( lower2 ) ) )
 	{
 		return m;
 	}
 
 	V2<T> vLower = v22;
-	if ( Val( lower2 ) < Val( lower1 ) && Val( upper2 - lower2 ) > FLT_EPSILON )
+	if ( Dec( Val( lower2 ) < Val( lower1 ) && Val( upper2 - lower2 ) > FLT_EPSILON ) )
 	{
 		vLower = Lerp( v22, v21, ( lower1 - lower2 ) / ( upper2 - lower2 ) );
 	}
 	V2<T> vUpper = v21;
-	if ( Val( upper2 ) > Val( upper1 ) && Val( upper2 - lower2 ) > FLT_EPSILON )
+	if ( Dec( Val( upper2 ) > Val( upper1 ) && Val( upper2 - lower2 ) > FLT_EPSILON ) )
 	{
 		vUpper = Lerp( v22, v21, ( upper1 - lower2 ) / ( upper2 - lower2 ) );
 	}
@@ -422,13 +467,13 @@ SegDist<T> SegmentDistance( V2<T> p1, V2<T> q1, V2<T> p2, V2<T> q2 )
 	V2<T> d1 = Sub( q1, p1 ), d2 = Sub( q2, p2 ), rr = Sub( p1, p2 );
 	T dd1 = Dot( d1, d1 ), dd2 = Dot( d2, d2 ), rd1 = Dot( rr, d1 ), rd2 = Dot( rr, d2 );
 	const double epsSqr = (double)( FLT_EPSILON * FLT_EPSILON );
-	if ( Val( dd1 ) < epsSqr || Val( dd2 ) < epsSqr )
+	if ( Dec( Val( dd1 ) < epsSqr || Val( dd2 ) < epsSqr ) )
 	{
-		if ( Val( dd1 ) >= epsSqr )
+		if ( Dec( Val( dd1 ) >= epsSqr ) )
 		{
 			r.f1 = Clamp01( -rd1 / dd1 );
 		}
-		else if ( Val( dd2 ) >= epsSqr )
+		else if ( Dec( Val( dd2 ) >= epsSqr ) )
 		{
 			r.f2 = Clamp01( rd2 / dd2 );
 		}
@@ -438,17 +483,17 @@ SegDist<T> SegmentDistance( V2<T> p1, V2<T> q1, V2<T> p2, V2<T> q2 )
 		T d12 = Dot( d1, d2 );
 		T den = dd1 * dd2 - d12 * d12;
 		T f1 = K<T>( 0.0f );
-		if ( Val( den ) != 0.0 )
+		if ( Dec( Val( den ) != 0.0 ) )
 		{
 			f1 = Clamp01( ( d12 * rd2 - rd1 * dd2 ) / den );
 		}
 		T f2 = ( d12 * f1 + rd2 ) / dd2;
-		if ( Val( f2 ) < 0.0 )
+		if ( Dec( Val( f2 ) < 0.0 ) )
 		{
 			f2 = K<T>( 0.0f );
 			f1 = Clamp01( -rd1 / dd1 );
 		}
-		else if ( Val( f2 ) > 1.0 )
+		else if ( Dec( Val( f2 ) > 1.0 ) )
 		{
 			f2 = K<T>( 1.0f );
 			f1 = Clamp01( ( d12 - rd1 ) / dd1 );
@@ -492,12 +537,12 @@ Local<T> CollidePolygons( const b2Polygon& A, const b2Polygon& B, V2<T> xp, R2<T
 	double sepA = MaxSeparation( &edgeA, pa, pb );
 	double sepB = MaxSeparation( &edgeB, pb, pa );
 	float radius = pa.radius + pb.radius;
-	if ( sepA > (double)( kSpeculative + radius ) || sepB > (double)( kSpeculative + radius ) )
+	if ( Dec( sepA > (double)( kSpeculative + radius ) || sepB > (double)( kSpeculative + radius ) ) )
 	{
 		return EmptyLocal<T>();
 	}
 
-	bool flip = !( sepA >= sepB );
+	bool flip = !Dec( sepA >= sepB );
 	if ( flip == false )
 	{
 		edgeB = 0;
@@ -505,7 +550,7 @@ Local<T> CollidePolygons( const b2Polygon& A, const b2Polygon& B, V2<T> xp, R2<T
 		for ( int i = 0; i < pb.count; ++i )
 		{
 			double d = Val( Dot( pa.n[edgeA], pb.n[i] ) );
-			if ( d < minDot )
+			if ( Dec( d < minDot ) )
 			{
 				minDot = d;
 				edgeB = i;
@@ -519,7 +564,7 @@ Local<T> CollidePolygons( const b2Polygon& A, const b2Polygon& B, V2<T> xp, R2<T
 		for ( int i = 0; i < pa.count; ++i )
 		{
 			double d = Val( Dot( pb.n[edgeB], pa.n[i] ) );
-			if ( d < minDot )
+			if ( Dec( d < minDot ) )
 			{
 				minDot = d;
 				edgeA = i;
@@ -528,7 +573,7 @@ Local<T> CollidePolygons( const b2Polygon& A, const b2Polygon& B, V2<T> xp, R2<T
 	}
 
 	Local<T> m = EmptyLocal<T>();
-	if ( sepA > (double)( 0.1f * kLinearSlop ) || sepB > (double)( 0.1f * kLinearSlop ) )
+	if ( Dec( sepA > (double)( 0.1f * kLinearSlop ) || sepB > (double)( 0.1f * kLinearSlop ) ) )
 	{
 		int i11 = edgeA, i12 = edgeA + 1 < pa.count ? edgeA + 1 : 0;
 		int i21 = edgeB, i22 = edgeB + 1 < pb.count ? edgeB + 1 : 0;
@@ -536,7 +581,7 @@ Local<T> CollidePolygons( const b2Polygon& A, const b2Polygon& B, V2<T> xp, R2<T
 		SegDist<T> r = SegmentDistance( v11, v12, v21, v22 );
 		T distance = Sqrt( r.dd );
 		T separation = distance - K<T>( radius );
-		if ( Val( distance - K<T>( radius ) ) > (double)kSpeculative )
+		if ( Dec( Val( distance - K<T>( radius ) ) > (double)kSpeculative ) )
 		{
 			return m;
 		}
@@ -544,26 +589,26 @@ Local<T> CollidePolygons( const b2Polygon& A, const b2Polygon& B, V2<T> xp, R2<T
 		double minSep = FLT_MAX;
 		for ( int i = 0; i < m.count; ++i )
 		{
-			minSep = Val( m.p[i].separation ) < minSep ? Val( m.p[i].separation ) : minSep;
+			minSep = Dec( Val( m.p[i].separation ) < minSep ) ? Val( m.p[i].separation ) : minSep;
 		}
-		if ( Val( separation + K<T>( 0.1f * kLinearSlop ) ) < minSep )
+		if ( Dec( Val( separation + K<T>( 0.1f * kLinearSlop ) ) < minSep ) )
 		{
 			double f1 = Val( r.f1 ), f2 = Val( r.f2 );
 			int iA = -1, iB = -1;
 			V2<T> vA = v11, vB = v21;
-			if ( f1 == 0.0 && f2 == 0.0 )
+			if ( Dec( f1 == 0.0 && f2 == 0.0 ) )
 			{
 				iA = i11, iB = i21, vA = v11, vB = v21;
 			}
-			else if ( f1 == 0.0 && f2 == 1.0 )
+			else if ( Dec( f1 == 0.0 && f2 == 1.0 ) )
 			{
 				iA = i11, iB = i22, vA = v11, vB = v22;
 			}
-			else if ( f1 == 1.0 && f2 == 0.0 )
+			else if ( Dec( f1 == 1.0 && f2 == 0.0 ) )
 			{
 				iA = i12, iB = i21, vA = v12, vB = v21;
 			}
-			else if ( f1 == 1.0 && f2 == 1.0 )
+			else if ( Dec( f1 == 1.0 && f2 == 1.0 ) )
 			{
 				iA = i12, iB = i22, vA = v12, vB = v22;
 			}
@@ -835,10 +880,10 @@ struct Replay
 					c.lt[j] = ws * cs.tangentImpulse[j];
 					T rnA = Cross( c.rA[j], c.n ), rnB = Cross( c.rB[j], c.n );
 					T kN = K<T>( c.mA ) + K<T>( c.mB ) + K<T>( c.iA ) * rnA * rnA + K<T>( c.iB ) * rnB * rnB;
-					c.nm[j] = Val( kN ) > 0.0 ? K<T>( 1.0f ) / kN : K<T>( 0.0f );
+					c.nm[j] = Dec( Val( kN ) > 0.0 ) ? K<T>( 1.0f ) / kN : K<T>( 0.0f );
 					T rtA = Cross( c.rA[j], t ), rtB = Cross( c.rB[j], t );
 					T kT = K<T>( c.mA ) + K<T>( c.mB ) + K<T>( c.iA ) * rtA * rtA + K<T>( c.iB ) * rtB * rtB;
-					c.tm[j] = Val( kT ) > 0.0 ? K<T>( 1.0f ) / kT : K<T>( 0.0f );
+					c.tm[j] = Dec( Val( kT ) > 0.0 ) ? K<T>( 1.0f ) / kT : K<T>( 0.0f );
 				}
 				else
 				{
@@ -1009,9 +1054,9 @@ struct Replay
 			T dsx = dpX + ( rsBx - rsAx );
 			T dsy = dpY + ( rsBy - rsAy );
 			T s = ( c.n.x * dsx + c.n.y * dsy ) + c.base[j];
-			bool mask = Val( s ) > 0.0;
+			bool mask = Dec( Val( s ) > 0.0 );
 			T soft = biasRate * s;
-			bool softActive = Val( soft ) >= Val( contactSpeed );
+			bool softActive = Dec( Val( soft ) >= Val( contactSpeed ) );
 			T bias = mask ? s * inv_h : ( softActive ? soft : contactSpeed );
 			T pms = mask ? K<T>( 1.0f ) : massScale;
 			T pis = mask ? K<T>( 0.0f ) : impulseScale;
@@ -1020,7 +1065,7 @@ struct Replay
 			T vn = dvx * c.n.x + dvy * c.n.y;
 			T neg = c.nm[j] * ( pms * vn + bias ) + pis * c.ln[j];
 			T cand = c.ln[j] - neg;
-			bool active = Val( cand ) >= 0.0;
+			bool active = Dec( Val( cand ) >= 0.0 );
 			T newImpulse = active ? cand : K<T>( 0.0f );
 			T impulse = newImpulse - c.ln[j];
 			c.ln[j] = newImpulse;
@@ -1050,10 +1095,10 @@ struct Replay
 				T neg = c.tm[j] * vt;
 				T maxF = c.friction * c.ln[j];
 				T cand = c.lt[j] - neg;
-				bool innerIsCand = Val( cand ) <= Val( maxF );
+				bool innerIsCand = Dec( Val( cand ) <= Val( maxF ) );
 				T inner = innerIsCand ? cand : maxF;
 				T lower = K<T>( 0.0f ) - maxF;
-				bool resIsLower = Val( lower ) >= Val( inner );
+				bool resIsLower = Dec( Val( lower ) >= Val( inner ) );
 				T newImpulse = resIsLower ? lower : inner;
 				T impulse = newImpulse - c.lt[j];
 				c.lt[j] = newImpulse;
@@ -1206,10 +1251,25 @@ int b2o_adjoint_replay( int mode, int stepCount, const b2amd_TapeStep* headers, 
 		r.friction[i] = RevVar( shapeFriction[i] );
 		frictionNodes[i] = r.friction[i].i;
 	}
+	// tape nodes of every body state at each step's start (and the final state)
+	std::vector<int> stepNodes( (size_t)( stepCount + 1 ) * nb * 7, -1 );
+	auto markStep = [&]( int k ) {
+		for ( int s = 1; s <= nb; ++s )
+		{
+			Body<Rev>& bd = r.bodies[s];
+			Rev* vars[7] = { &bd.c.x, &bd.c.y, &bd.q.c, &bd.q.s, &bd.v.x, &bd.v.y, &bd.w };
+			for ( int c = 0; c < 7; ++c )
+			{
+				stepNodes[( (size_t)k * nb + ( s - 1 ) ) * 7 + c] = vars[c]->i;
+			}
+		}
+	};
 	for ( int k = 0; k < stepCount; ++k )
 	{
+		markStep( k );
 		r.Step( k );
 	}
+	markStep( stepCount );
 	const Body<Rev>& b = r.bodies[targetSlot];
 	Rev dx = b.c.x - Rev{ goal[0], -1 };
 	Rev dy = b.c.y - Rev{ goal[1], -1 };
@@ -1248,8 +1308,48 @@ int b2o_adjoint_replay( int mode, int stepCount, const b2amd_TapeStep* headers, 
 	{
 		gradFriction[i] = adj[frictionNodes[i]];
 	}
+	if ( g_stepGrads != nullptr )
+	{
+		for ( size_t k = 0; k < stepNodes.size(); ++k )
+		{
+			g_stepGrads[k] = stepNodes[k] >= 0 ? adj[stepNodes[k]] : 0.0;
+		}
+	}
 	g_tape = nullptr;
 	return 0;
+}
+
+// Branch pattern of the next replays: mode 0 off, 1 record (the pattern is kept), 2 force the kept
+// pattern. b2o_adjoint_decisions_info reports the pattern length, forced branches the run's own
+// values disagreed with, and whether a forced run asked for more branches than were kept.
+static Decisions s_decisions;
+
+void b2o_adjoint_decisions( int mode )
+{
+	s_decisions.mode = mode;
+	s_decisions.pos = 0;
+	s_decisions.flips = 0;
+	s_decisions.overrun = false;
+	if ( mode == 1 )
+	{
+		s_decisions.bits.clear();
+	}
+	g_dec = mode == 0 ? nullptr : &s_decisions;
+}
+
+void b2o_adjoint_decisions_info( int* length, int* flips, int* overrun, int* consumed )
+{
+	*length = (int)s_decisions.bits.size();
+	*flips = s_decisions.flips;
+	*overrun = s_decisions.overrun ? 1 : 0;
+	*consumed = (int)s_decisions.pos;
+}
+
+// Gradients of the loss with respect to every body state at each step's start (steps + 1 blocks
+// of bodies x 7, the last one the final state), from the next mode-2 replay.
+void b2o_adjoint_capture_step_grads( double* out )
+{
+	g_stepGrads = out;
 }
 
 } // extern "C"

@@ -30,6 +30,9 @@ def oracle():
         lib.b2o_adjoint_replay.restype = ctypes.c_int
         lib.b2o_adjoint_replay.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp,
                                            ctypes.c_int, vp, vp, ctypes.c_int, vp, vp, vp, vp, vp, vp]
+        lib.b2o_adjoint_decisions.argtypes = [ctypes.c_int]
+        lib.b2o_adjoint_decisions_info.argtypes = [ctypes.POINTER(ctypes.c_int)] * 4
+        lib.b2o_adjoint_capture_step_grads.argtypes = [vp]
         _lib = lib
     return _lib
 
@@ -106,9 +109,11 @@ class WorldRecord:
         ids = list(self.bodies[:nb]["bodyId"])
         return ids.index(body_id) + 1
 
-    def replay(self, mode, target_slot, goal, state=None, friction=None):
-        """mode 0 float, 1 float64, 2 float64 with reverse-mode gradients. Returns a dict."""
+    def replay(self, mode, target_slot, goal, state=None, friction=None, decisions=0):
+        """mode 0 float, 1 float64, 2 float64 with reverse-mode gradients. decisions: 0 free
+        branches, 1 record the run's branch pattern, 2 force the recorded pattern. Returns a dict."""
         lib = oracle()
+        lib.b2o_adjoint_decisions(decisions)
         nb = int(self.headers[0]["bodyCount"])
         state = self.initial_state() if state is None else np.ascontiguousarray(state, dtype=np.float64)
         fr = self.friction.astype(np.float64) if friction is None else np.ascontiguousarray(friction, dtype=np.float64)
@@ -123,21 +128,40 @@ class WorldRecord:
                                p(self.cons_start), p(self.shapes), p(self.friction), len(self.shapes), p(self.statics),
                                len(self.statics), p(state), p(fr), target_slot, p(goal), p(loss), p(gstate), p(gfr),
                                p(counts), p(sig))
+        info = [ctypes.c_int() for _ in range(4)]
+        lib.b2o_adjoint_decisions_info(*[ctypes.byref(x) for x in info])
+        if decisions != 1:
+            lib.b2o_adjoint_decisions(0)
         return {"loss": float(loss[0]), "grad_state": gstate, "grad_friction": gfr, "float_mismatches": int(counts[0]),
-                "structure_mismatches": int(counts[1]), "signature": int(sig[0])}
+                "structure_mismatches": int(counts[1]), "signature": int(sig[0]), "decisions": info[0].value,
+                "flips": info[1].value, "overrun": bool(info[2].value), "consumed": info[3].value}
 
-    def push_gradients(self, pusher_slot, pusher_shape, target_slot, goal):
-        """Reverse-mode float64 gradient of the loss with respect to (vx, vy, omega, friction)."""
-        r = self.replay(2, target_slot, goal)
+    def record_pattern(self, target_slot, goal):
+        """Record the branch pattern of the float replay (the recorded forward's pattern)."""
+        r = self.replay(0, target_slot, goal, decisions=1)
+        oracle().b2o_adjoint_decisions(0)
+        return r["decisions"]
+
+    def push_gradients(self, pusher_slot, pusher_shape, target_slot, goal, fixed=True):
+        """Reverse-mode float64 gradient of the loss with respect to (vx, vy, omega, friction).
+        fixed: hold the recorded forward's branch pattern (active set, clamps) fixed."""
+        if fixed:
+            self.record_pattern(target_slot, goal)
+        r = self.replay(2, target_slot, goal, decisions=2 if fixed else 0)
         g = r["grad_state"][pusher_slot - 1]
         return np.array([g[4], g[5], g[6], r["grad_friction"][pusher_shape]]), r
 
-    def push_fd(self, pusher_slot, pusher_shape, target_slot, goal, h=1e-4):
-        """Central differences (float64 replay) of the loss in (vx, vy, omega, friction), and
-        whether every perturbed run took the same branches as the unperturbed one."""
-        base = self.replay(1, target_slot, goal)
+    def push_fd(self, pusher_slot, pusher_shape, target_slot, goal, h=1e-4, fixed=True):
+        """Central differences (float64 replay) of the loss in (vx, vy, omega, friction). With
+        fixed, every run is forced onto the recorded forward's branch pattern; stable says whether
+        the perturbed runs' own values took that pattern too (a contact-stable run)."""
+        dec = 0
+        if fixed:
+            self.record_pattern(target_slot, goal)
+            dec = 2
+        base = self.replay(1, target_slot, goal, decisions=dec)
         grads = np.zeros(4)
-        stable = True
+        stable = (base["flips"] == 0) if fixed else True
         for k in range(4):
             vals = []
             for sgn in (1.0, -1.0):
@@ -147,8 +171,11 @@ class WorldRecord:
                     state[pusher_slot - 1, 4 + k] += sgn * h
                 else:
                     fr[pusher_shape] += sgn * h
-                r = self.replay(1, target_slot, goal, state, fr)
-                stable = stable and r["signature"] == base["signature"] and r["structure_mismatches"] == 0
+                r = self.replay(1, target_slot, goal, state, fr, decisions=dec)
+                if fixed:
+                    stable = stable and r["flips"] == 0 and not r["overrun"]
+                else:
+                    stable = stable and r["signature"] == base["signature"] and r["structure_mismatches"] == 0
                 vals.append(r["loss"])
             grads[k] = (vals[0] - vals[1]) / (2.0 * h)
         return grads, stable, base

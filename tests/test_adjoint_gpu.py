@@ -6,8 +6,8 @@ loss = |top box center - goal|^2 after the recorded steps. Checks:
   - the record is a faithful checkpoint: the oracle's float replay reproduces every recorded step
     bit for bit, and the GPU backward's own recompute reports no mismatch;
   - GPU adjoint vs the oracle's float64 reverse-mode adjoint and vs central differences (h = 1e-4)
-    of the float64 replay, on runs whose branch pattern is unchanged by the perturbation
-    (contact-stable runs), to 1e-3 relative;
+    of the float64 replay, all with the recorded run's branch pattern held fixed (active contact
+    set, bias branches, normal and friction clamps), to 1e-3 relative;
   - the torch.autograd.Function returns the same gradient as the tape.
 """
 
@@ -74,7 +74,6 @@ def test_oracle_float_replay_reproduces_records(recorded):
 def test_gpu_adjoint_matches_fp64_adjoint_and_fd(recorded):
     pw, tape, goals, final, ig, fr, stats = recorded
     shape_ids = pw.pusher_shape_ids()
-    stable_count = 0
     for w in range(4):
         pusher = int(pw.body_ids[w][-1])
         top = int(pw.body_ids[w][-2])
@@ -84,13 +83,10 @@ def test_gpu_adjoint_matches_fp64_adjoint_and_fd(recorded):
         ps, ts = rec.slot_of(pusher), rec.slot_of(top)
         g_rev, _ = rec.push_gradients(ps, shape_ids[w], ts, goals[w])
         g_fd, stable, _ = rec.push_fd(ps, shape_ids[w], ts, goals[w])
-        if not stable:
-            continue
-        stable_count += 1
-        assert _rel(g_rev, g_fd) < REL_TOL, (w, g_rev, g_fd)
+        print(f"[gpu] push world {w}: gpu {g_gpu} fp64 adjoint {g_rev} fd {g_fd} contact-stable {stable}", flush=True)
+        assert _rel(g_rev, g_fd) < 1e-4, (w, g_rev, g_fd)
         assert _rel(g_gpu, g_rev) < REL_TOL, (w, g_gpu, g_rev)
         assert _rel(g_gpu, g_fd) < REL_TOL, (w, g_gpu, g_fd)
-    assert stable_count >= 2
 
 
 def test_torch_function_gradient():

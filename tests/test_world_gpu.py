@@ -16,16 +16,24 @@ def scene(world_runs, name):
     return next(s for s in world_runs["scenes"] if s["name"] == name)
 
 
-def run_and_compare(gpu_scenes, fx, path=0):
+def run_and_compare(gpu_scenes, fx, path=0, pair_query=None):
+    """Step the scene and compare with the fixture. pair_query forces the broadphase pair query
+    (1 host tree, 2 GPU LBVH); returns how many steps ran each path."""
     import box2d_amd
 
+    lib = box2d_amd.library()
     h = gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"])
-    box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
+    wid = gpu_scenes.world_id(h)
+    box2d_amd.set_solver_path(wid, path)
+    if pair_query is not None:
+        lib.b2amd_World_SetPairQuery(wid, pair_query)
+    used = {0: 0, 1: 0, 2: 0}
     try:
         assert gpu_scenes.body_count(h) == fx["body_count"]
         assert gpu_scenes.hash(h) == fx["hash_initial"]
         for s in range(1, fx["steps"] + 1):
             gpu_scenes.step(h)
+            used[lib.b2amd_World_GetLastPairQuery(wid)] += 1
             want = fx["hashes"].get(str(s))
             if want is not None:
                 got = gpu_scenes.hash(h)
@@ -39,6 +47,17 @@ def run_and_compare(gpu_scenes, fx, path=0):
             assert np.array_equal(got, np.array(fx["velocities_bits"], dtype=np.uint32))
     finally:
         gpu_scenes.destroy(h)
+    return used
+
+
+@pytest.mark.parametrize("name", ["hello", "pyramid20_sleep", "mixed_pile_1", "mixed_pile_2", "mixed_pile_3",
+                                  "falling_hinges_trace", "rain_small"])
+def test_gpu_pair_query_matches_reference(gpu_scenes, world_runs, name):
+    """Broadphase pairs from the GPU LBVH query on every step (ordered on the host in the
+    reference trees' visit order): the whole run stays bit-identical to the reference."""
+    print(f"[gpu] scene {name} (GPU pair query)", flush=True)
+    used = run_and_compare(gpu_scenes, scene(world_runs, name), pair_query=2)
+    assert used[2] > 0 and used[1] == 0
 
 
 @pytest.mark.parametrize("name", ["hello", "pyramid20", "pyramid100", "pyramid20_sleep"])
@@ -82,7 +101,8 @@ def test_rain_small_matches_reference(gpu_scenes, world_runs, path):
 def test_rain_matches_reference(gpu_scenes, world_runs):
     """The 20,000-body mixed-shape rain, 1000 steps, hash-checked every 10 steps."""
     print("[gpu] rain 20k", flush=True)
-    run_and_compare(gpu_scenes, scene(world_runs, "rain"))
+    used = run_and_compare(gpu_scenes, scene(world_runs, "rain"))
+    print(f"[gpu] rain pair query steps: host {used[1]} gpu {used[2]} none {used[0]}", flush=True)
 
 
 @pytest.mark.parametrize("path", [0, 1])
