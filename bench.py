@@ -101,10 +101,12 @@ def batched_workload(lib, scenes, worlds, steps, loop_steps):
     box2d_amd.step_worlds(ids, 4, DT, SUBSTEPS)  # warm-up; the first three also autotune the workgroup size
     lib.b2amd_Synchronize()
     solve_ms = 0.0
+    solve_bytes = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         st = box2d_amd.step_worlds(ids, 1, DT, SUBSTEPS)
         solve_ms += st.solveKernelMs
+        solve_bytes += st.solveAlgorithmicBytes
     lib.b2amd_Synchronize()
     batched_s = (time.perf_counter() - t0) / steps
     t0 = time.perf_counter()
@@ -118,7 +120,12 @@ def batched_workload(lib, scenes, worlds, steps, loop_steps):
             "body_substeps_per_s": bodies * SUBSTEPS / batched_s, "solve_kernel_ms_per_step": solve_ms / steps,
             "solve_workgroup_threads": st.solveThreads, "solve_workgroups": st.solveGrid,
             "plain_loop_ms_per_step": loop_s * 1e3, "plain_loop_body_substeps_per_s": bodies * SUBSTEPS / loop_s,
-            "speedup_vs_plain_loop": loop_s / batched_s, "plain_loop_steps": loop_steps}
+            "speedup_vs_plain_loop": loop_s / batched_s, "plain_loop_steps": loop_steps,
+            "solve_kernel_roofline": {"bound": "hbm", "achieved": solve_bytes / (solve_ms * 1e-3) / 1e9,
+                                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": solve_bytes / (solve_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                      "algorithmic_bytes_per_launch": solve_bytes / steps,
+                                      "avg_launch_ms": solve_ms / steps}}
 
 
 def rain_workload(lib, scenes, steps, cpu_steps, cpu_threads, no_cpu):
@@ -250,7 +257,22 @@ def main():
     ap.add_argument("--rain-cpu-steps", type=int, default=150, help="bounded CPU-baseline sample of the rain")
     ap.add_argument("--push-worlds", type=int, default=256, help="worlds of the differentiable push (0: skip)")
     ap.add_argument("--push-steps", type=int, default=60)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB megabytes 10 times (scales rocprofv3 FETCH_SIZE and "
+                         "WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration > 0:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_dispatch": n * 4, "dispatches": 10}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported. A single Box2D world "

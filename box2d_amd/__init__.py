@@ -33,7 +33,7 @@ from .scenes_api import (SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HE
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("solveKernelMs", ctypes.c_float), ("solveThreads", ctypes.c_int), ("solveGrid", ctypes.c_int),
-                ("worlds", ctypes.c_int), ("phaseMs", ctypes.c_float * 5)]
+                ("worlds", ctypes.c_int), ("phaseMs", ctypes.c_float * 5), ("solveAlgorithmicBytes", ctypes.c_double)]
 
 
 def step_worlds(world_ids, count=1, dt=1.0 / 60.0, substeps=4):

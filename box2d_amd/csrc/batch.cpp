@@ -180,6 +180,7 @@ int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, Bat
 		stats->solveThreads = s_batch->lastFusedThreads;
 		stats->solveGrid = s_batch->lastFusedGrid;
 		stats->worlds = n;
+		stats->solveBytes = s_batch->lastFusedBytes;
 		for ( int k = 0; k < 5; ++k )
 		{
 			stats->phaseMs[k] = phase[k];

@@ -692,6 +692,7 @@ int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int 
 		{
 			stats->phaseMs[k] = bs.phaseMs[k];
 		}
+		stats->solveAlgorithmicBytes = bs.solveBytes;
 	}
 	return n;
 }

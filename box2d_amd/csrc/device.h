@@ -132,6 +132,7 @@ public:
 	int FusedThreads() const;
 
 	float lastFusedMs = 0.0f;
+	double lastFusedBytes = 0.0;
 	int lastFusedThreads = 0;
 	int lastFusedGrid = 0;
 
@@ -148,6 +149,7 @@ struct BatchStats
 	int solveGrid;		 // its workgroup count
 	int worlds;			 // worlds stepped
 	float phaseMs[5];	 // wall time: host phase 1, GPU collide, host phase 2, GPU solve, host phase 3
+	double solveBytes;	 // algorithmic bytes of the substep-loop launch
 };
 
 // Step count worlds as one batch (defined in batch.cpp). Returns the number of worlds stepped.

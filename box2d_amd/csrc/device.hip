@@ -11,6 +11,7 @@
 // Colors are Gauss-Seidel phases: constraints inside a color touch disjoint dynamic bodies, so a
 // color is solved by one fully parallel launch and the result is independent of lane order.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "box2d_amd.h"
@@ -2068,19 +2069,19 @@ struct DeviceWorld::Impl
 	template <typename F>
 	void Launch( int cls, double bytes, F&& launch )
 	{
+		// Timed launches hand their events to the dispatch itself (hipExtLaunchKernelGGL), so the
+		// events bracket the kernel's execution rather than the queue around it.
 		if ( timing )
 		{
 			int e0 = NextEvent();
 			int e1 = NextEvent();
-			HIP_CHECK( hipEventRecord( eventPool[e0], stream ) );
-			launch();
+			launch( eventPool[e0], eventPool[e1] );
 			HIP_CHECK( hipGetLastError() );
-			HIP_CHECK( hipEventRecord( eventPool[e1], stream ) );
 			pending.push_back( Pending{ cls, bytes, e0, e1 } );
 		}
 		else
 		{
-			launch();
+			launch( nullptr, nullptr );
 			HIP_CHECK( hipGetLastError() );
 		}
 	}
@@ -2433,7 +2434,7 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 
 	HIP_CHECK( hipEventRecord( impl->ev[0], s ) );
 	// Bytes are fixed up after the launch once the recycled count is known.
-	impl->Launch( kkCollide, 0.0, [&]() { LaneKernel<CollideOp><<<Blocks( n ), kBlock, 0, s>>>( a ); } );
+	impl->Launch( kkCollide, 0.0, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<CollideOp>, dim3( Blocks( n ) ), dim3( kBlock ), 0, s, e0, e1, 0, a ); } );
 	HIP_CHECK( hipEventRecord( impl->ev[1], s ) );
 	PackCollideKernel<<<1, kPackThreads, 0, s>>>( pc );
 	HIP_CHECK( hipGetLastError() );
@@ -2516,7 +2517,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		else
 		{
 			impl->Launch( kkPrepareContacts, kBytesPrepare * nc,
-						  [&]() { LaneKernel<PrepareContactsOp><<<Blocks( nc ), kBlock, 0, s>>>( pa ); } );
+						  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<PrepareContactsOp>, dim3( Blocks( nc ) ), dim3( kBlock ), 0, s, e0, e1, 0, pa ); } );
 		}
 	}
 
@@ -2539,7 +2540,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		else
 		{
 			impl->Launch( kkPrepareJoints, kBytesJoint * nj,
-						  [&]() { LaneKernel<PrepareJointsOp><<<Blocks( nj ), kBlock, 0, s>>>( ja ); } );
+						  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<PrepareJointsOp>, dim3( Blocks( nj ) ), dim3( kBlock ), 0, s, e0, e1, 0, ja ); } );
 		}
 	}
 
@@ -2595,20 +2596,20 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 			ca.contactStart = L.overflowContactStart;
 			ca.contactCount = L.overflowContactCount;
 			double bytes = stageBytes( stage ) * ca.contactCount + kBytesJoint * ca.jointCount;
-			impl->Launch( kkOverflow, bytes, [&]() {
+			impl->Launch( kkOverflow, bytes, [&]( hipEvent_t e0, hipEvent_t e1 ) {
 				switch ( stage )
 				{
 					case kStageWarmStart:
-						OverflowKernel<kStageWarmStart><<<1, 64, 0, s>>>( ca );
+						hipExtLaunchKernelGGL( OverflowKernel<kStageWarmStart>, dim3( 1 ), dim3( 64 ), 0, s, e0, e1, 0, ca );
 						break;
 					case kStageSolve:
-						OverflowKernel<kStageSolve><<<1, 64, 0, s>>>( ca );
+						hipExtLaunchKernelGGL( OverflowKernel<kStageSolve>, dim3( 1 ), dim3( 64 ), 0, s, e0, e1, 0, ca );
 						break;
 					case kStageRelax:
-						OverflowKernel<kStageRelax><<<1, 64, 0, s>>>( ca );
+						hipExtLaunchKernelGGL( OverflowKernel<kStageRelax>, dim3( 1 ), dim3( 64 ), 0, s, e0, e1, 0, ca );
 						break;
 					default:
-						OverflowKernel<kStageRestitution><<<1, 64, 0, s>>>( ca );
+						hipExtLaunchKernelGGL( OverflowKernel<kStageRestitution>, dim3( 1 ), dim3( 64 ), 0, s, e0, e1, 0, ca );
 						break;
 				}
 			} );
@@ -2631,20 +2632,20 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 				continue;
 			}
 			double bytes = stageBytes( stage ) * ca.contactCount + kBytesJoint * ca.jointCount;
-			impl->Launch( stageClass( stage ), bytes, [&]() {
+			impl->Launch( stageClass( stage ), bytes, [&]( hipEvent_t e0, hipEvent_t e1 ) {
 				switch ( stage )
 				{
 					case kStageWarmStart:
-						ColorKernel<kStageWarmStart><<<( lanes + cb - 1 ) / cb, cb, 0, s>>>( ca );
+						hipExtLaunchKernelGGL( ColorKernel<kStageWarmStart>, dim3( ( lanes + cb - 1 ) / cb ), dim3( cb ), 0, s, e0, e1, 0, ca );
 						break;
 					case kStageSolve:
-						ColorKernel<kStageSolve><<<( lanes + cb - 1 ) / cb, cb, 0, s>>>( ca );
+						hipExtLaunchKernelGGL( ColorKernel<kStageSolve>, dim3( ( lanes + cb - 1 ) / cb ), dim3( cb ), 0, s, e0, e1, 0, ca );
 						break;
 					case kStageRelax:
-						ColorKernel<kStageRelax><<<( lanes + cb - 1 ) / cb, cb, 0, s>>>( ca );
+						hipExtLaunchKernelGGL( ColorKernel<kStageRelax>, dim3( ( lanes + cb - 1 ) / cb ), dim3( cb ), 0, s, e0, e1, 0, ca );
 						break;
 					default:
-						ColorKernel<kStageRestitution><<<( lanes + cb - 1 ) / cb, cb, 0, s>>>( ca );
+						hipExtLaunchKernelGGL( ColorKernel<kStageRestitution>, dim3( ( lanes + cb - 1 ) / cb ), dim3( cb ), 0, s, e0, e1, 0, ca );
 						break;
 				}
 			} );
@@ -2756,7 +2757,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		}
 		else
 		{
-			impl->Launch( kkSolveFused, bytes, [&]() { SolveFusedKernel<<<1, kFusedThreads, FusedLdsBytes( nb ), s>>>( fa ); } );
+			impl->Launch( kkSolveFused, bytes, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( SolveFusedKernel, dim3( 1 ), dim3( kFusedThreads ), FusedLdsBytes( nb ), s, e0, e1, 0, fa ); } );
 		}
 	}
 	else
@@ -2764,11 +2765,11 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		for ( int sub = 0; sub < params.subStepCount; ++sub )
 		{
 			impl->Launch( kkIntegrateVelocities, kBytesIntegrateVelocities * nb,
-						  [&]() { IntegrateVelocitiesKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba ); } );
+						  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( IntegrateVelocitiesKernel, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, ba ); } );
 			runStage( kStageWarmStart );
 			runStage( kStageSolve );
 			impl->Launch( kkIntegratePositions, kBytesIntegratePositions * nb,
-						  [&]() { IntegratePositionsKernel<<<Blocks( nb ), kBlock, 0, s>>>( ba ); } );
+						  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( IntegratePositionsKernel, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, ba ); } );
 			runStage( kStageRelax );
 		}
 
@@ -2793,7 +2794,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		}
 		else
 		{
-			impl->Launch( kkStore, kBytesStore * nc, [&]() { LaneKernel<StoreImpulsesOp><<<Blocks( nc ), kBlock, 0, s>>>( sa ); } );
+			impl->Launch( kkStore, kBytesStore * nc, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<StoreImpulsesOp>, dim3( Blocks( nc ) ), dim3( kBlock ), 0, s, e0, e1, 0, sa ); } );
 		}
 	}
 
@@ -2819,7 +2820,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	}
 	else
 	{
-		impl->Launch( kkFinalize, kBytesFinalize * nb, [&]() { LaneKernel<FinalizeBodiesOp><<<Blocks( nb ), kBlock, 0, s>>>( fa ); } );
+		impl->Launch( kkFinalize, kBytesFinalize * nb, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<FinalizeBodiesOp>, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, fa ); } );
 	}
 
 	if ( params.enableContinuous )
@@ -2848,7 +2849,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		}
 		else
 		{
-			impl->Launch( kkContinuous, 0.0, [&]() { LaneKernel<ContinuousOp><<<Blocks( nb ), kBlock, 0, s>>>( ka ); } );
+			impl->Launch( kkContinuous, 0.0, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<ContinuousOp>, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, ka ); } );
 		}
 	}
 	if ( rec == false )
@@ -3573,6 +3574,7 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 	StageList<FinalizeArgs> finalize;
 	StageList<ContinuousArgs> continuous;
 	std::vector<PackFinalizeArgs> pack;
+	double fusedBytes = 0.0;
 
 	for ( int i = 0; i < count; ++i )
 	{
@@ -3594,8 +3596,13 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 		}
 		if ( r->fused )
 		{
-			fused.push_back( r->fusedArgs );
-			fusedBodies.push_back( r->fusedArgs.bodyCount );
+			const FusedArgs& fa = r->fusedArgs;
+			fused.push_back( fa );
+			fusedBodies.push_back( fa.bodyCount );
+			double nc = fa.overflowContactStart + fa.overflowContactCount;
+			double nj = fa.colorJointStart[fa.activeColorCount] + fa.overflowJointCount;
+			fusedBytes += fa.subStepCount * ( kBytesFusedContactSubstep * nc + kBytesFusedBodySubstep * fa.bodyCount + kBytesJoint * nj ) +
+						  kBytesFusedBodyStep * fa.bodyCount;
 		}
 		if ( r->store )
 		{
@@ -3691,14 +3698,14 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 		HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, SolveFusedBatchKernel, threads, lds ) );
 		int grid = std::max( 1, std::min( (int)fused.size(), perCu * b->cuCount ) );
 		HIP_CHECK( hipMemsetAsync( b->queue, 0, sizeof( int ), s ) );
-		HIP_CHECK( hipEventRecord( b->ev0, s ) );
-		SolveFusedBatchKernel<<<grid, threads, lds, s>>>( reinterpret_cast<const FusedArgs*>( base + fusedOff ),
-														  reinterpret_cast<const int*>( base + orderOff ), (int)fused.size(),
-														  b->queue );
+		// the events bracket the kernel's execution (recorded by the dispatch itself)
+		hipExtLaunchKernelGGL( SolveFusedBatchKernel, dim3( grid ), dim3( threads ), (uint32_t)lds, s, b->ev0, b->ev1, 0,
+							   reinterpret_cast<const FusedArgs*>( base + fusedOff ), reinterpret_cast<const int*>( base + orderOff ),
+							   (int)fused.size(), b->queue );
 		HIP_CHECK( hipGetLastError() );
-		HIP_CHECK( hipEventRecord( b->ev1, s ) );
 		lastFusedThreads = threads;
 		lastFusedGrid = grid;
+		lastFusedBytes = fusedBytes;
 	}
 
 	if ( storeBlocks > 0 )

@@ -113,6 +113,9 @@ typedef struct b2amd_BatchStats
 	/// wall-clock milliseconds of the step's phases: host phase 1 (pairs, uploads), GPU collide,
 	/// host phase 2 (touch processing, coloring), GPU solve, host phase 3 (refit, sleep)
 	float phaseMs[5];
+	/// algorithmic bytes of the substep-loop launch: per world, substeps x (380 B per contact
+	/// constraint + 32 B per awake body + 256 B per joint) + 64 B per awake body
+	double solveAlgorithmicBytes;
 } b2amd_BatchStats;
 
 /// Step count independent worlds by timeStep with subStepCount substeps, as one batch: host
