@@ -29,6 +29,15 @@ struct SolverLists
 	int overflowContactCount = 0;
 	int overflowJointStart = 0;
 	int overflowJointCount = 0;
+
+	// Warm start by body: for awake index i, entries [warmStart[i], warmStart[i + 1]) of warmList
+	// are the body's constraints in the warm-start stage's order (overflow joints, overflow
+	// contacts, then per color its joints and contacts). Entry: index << 3 | owner << 2 | joint << 1
+	// | side (1: the body is the constraint's body B). index is a constraint index for contacts and
+	// a jointList index for joints; owner marks the one entry per contact that also accumulates
+	// the contact's total normal impulse. Only dynamic bodies have entries.
+	std::vector<int> warmStart;
+	std::vector<int> warmList;
 };
 
 // Results the host needs after the collide stage.
@@ -61,6 +70,8 @@ public:
 	void UploadJoints( const World& world );
 	void UploadContactInits( const std::vector<ContactInit>& inits, int contactCapacity );
 	void UploadLists( const SolverLists& lists );
+	// Upload the per-body warm-start lists of the lists last uploaded (they must match them).
+	void UploadWarmLists( const SolverLists& lists );
 
 	// Device -> host downloads (after a step, on demand).
 	void DownloadBodies( World& world );
@@ -93,6 +104,9 @@ public:
 
 	// Solver path: 0 auto, 1 color kernels, 2 fused when the world fits.
 	void SetSolverPath( int path );
+	// Whether the next solve of a world with these list sizes takes the color-kernel path with the
+	// warm start body by body (the host then builds the per-body warm-start lists).
+	bool WantsWarmLists( int bodyCount, int constraintCount, int activeColorCount ) const;
 
 	// Step recording for the adjoint: while a tape is set, every solve appends one record (body
 	// state at the step's start, constraints in solve order) to the tape. index is the world's

@@ -557,6 +557,39 @@ __global__ void IntegratePositionsKernel( BodyArgs a )
 	IntegratePosition( a.states + a.awake[i], a.p );
 }
 
+struct WarmArgs
+{
+	BodyArgs b;
+	ConstraintSoA c;
+	const JointSim* joints;
+	const int* jointList;
+	const int* warmStart;
+	const int* warmList;
+	int overflowContactStart;
+};
+
+// Velocity integration and the whole warm-start stage of one substep, one lane per awake body
+// (WarmStartBody applies the body's constraint deltas in the stage's order).
+__global__ void IntegrateWarmStartKernel( WarmArgs a )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= a.b.count )
+	{
+		return;
+	}
+	int id = a.b.awake[i];
+	BodyState* state = a.b.states + id;
+	IntegrateVelocity( a.b.sims[id], state, a.b.p );
+	int begin = a.warmStart[i], end = a.warmStart[i + 1];
+	if ( begin < end )
+	{
+		float vx = state->linearVelocity.x, vy = state->linearVelocity.y, w = state->angularVelocity;
+		WarmStartBody( a.c, a.joints, a.jointList, a.overflowContactStart, a.warmList, begin, end, state->deltaRotation, vx, vy, w );
+		state->linearVelocity = { vx, vy };
+		state->angularVelocity = w;
+	}
+}
+
 enum ColorStage
 {
 	kStageWarmStart,
@@ -749,6 +782,9 @@ struct FusedArgs
 	int overflowJointCount;
 	int subStepCount;
 	int anyRestitution;
+	const int* warmStart; // warm start by body (WarmStartBody), indexed by awake index
+	const int* warmList;
+	int warmByBody; // 1: warm start body by body, 0: color by color
 };
 
 template <int Stage>
@@ -840,17 +876,28 @@ B2G_DEV void FusedBody( const FusedArgs& a, char* lds )
 
 	for ( int sub = 0; sub < a.subStepCount; ++sub )
 	{
+		// velocity integration, then the warm-start stage color by color or body by body (one
+		// barrier instead of one per color; the floats are the same either way)
 		for ( int i = threadIdx.x; i < n; i += blockDim.x )
 		{
 			BodyState st = ToState( B.Load( i + 1 ) );
 			IntegrateVelocity( a.sims[a.awake[i]], &st, a.color.p );
-			s_vx[i + 1] = st.linearVelocity.x;
-			s_vy[i + 1] = st.linearVelocity.y;
-			s_w[i + 1] = st.angularVelocity;
+			float vx = st.linearVelocity.x, vy = st.linearVelocity.y, w = st.angularVelocity;
+			if ( a.warmByBody )
+			{
+				WarmStartBody( a.color.c, a.color.joints, a.color.jointList, a.overflowContactStart, a.warmList, a.warmStart[i],
+							   a.warmStart[i + 1], st.deltaRotation, vx, vy, w );
+			}
+			s_vx[i + 1] = vx;
+			s_vy[i + 1] = vy;
+			s_w[i + 1] = w;
 		}
 		__syncthreads();
 
-		FusedStage<kStageWarmStart>( a, B );
+		if ( a.warmByBody == 0 )
+		{
+			FusedStage<kStageWarmStart>( a, B );
+		}
 		FusedStage<kStageSolve>( a, B );
 
 		for ( int i = threadIdx.x; i < n; i += blockDim.x )
@@ -1708,7 +1755,7 @@ enum KernelClass
 };
 
 static const char* s_kernelClassNames[kKernelClassCount] = {
-	"collide", "prepare_contacts", "prepare_joints", "integrate_velocities", "warm_start", "solve",
+	"collide", "prepare_contacts", "prepare_joints", "integrate_velocities_warm_start", "warm_start", "solve",
 	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies", "solve_fused", "continuous",
 };
 
@@ -1911,6 +1958,8 @@ struct DeviceWorld::Impl
 	DevArray<int> bodySlot; // body id -> fused solver LDS slot (awake index), kStaticSlot otherwise
 	DevArray<int> collideList;
 	DevArray<int> jointList;
+	DevArray<int> warmStart; // per awake body: its range of warmList (warm start by body)
+	DevArray<int> warmList;
 	DevArray<int> changes;
 	SolverLists lists;
 	std::vector<int> slotStaging;
@@ -2173,6 +2222,8 @@ DeviceWorld::~DeviceWorld()
 	impl->bodySlot.Free();
 	impl->collideList.Free();
 	impl->jointList.Free();
+	impl->warmStart.Free();
+	impl->warmList.Free();
 	impl->changes.Free();
 	impl->cfields.Free();
 	impl->cints.Free();
@@ -2362,6 +2413,14 @@ void DeviceWorld::DownloadContact( int contactId, ContactSim* out )
 }
 
 
+void DeviceWorld::UploadWarmLists( const SolverLists& lists )
+{
+	impl->lists.warmStart = lists.warmStart;
+	impl->lists.warmList = lists.warmList;
+	impl->StageVector( impl->warmStart, lists.warmStart );
+	impl->StageVector( impl->warmList, lists.warmList );
+}
+
 void DeviceWorld::UploadLists( const SolverLists& lists )
 {
 	impl->lists = lists;
@@ -2381,6 +2440,8 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 	}
 	impl->StageVector( impl->collideList, lists.collideContacts );
 	impl->StageVector( impl->jointList, lists.jointList );
+	impl->StageVector( impl->warmStart, lists.warmStart );
+	impl->StageVector( impl->warmList, lists.warmList );
 
 	size_t nc = lists.constraintContacts.size();
 	impl->EnsureConstraints( std::max<size_t>( nc, 1 ) );
@@ -2481,6 +2542,22 @@ void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 {
 	CollideLaunch( params );
 	CollideFinish( result );
+}
+
+// Warm-start form of a solver path. B2AMD_WARM_BY_BODY: unset = defaults (body by body for the
+// color kernels, color by color in the fused solver), 0 = color by color everywhere, 1 = body by
+// body everywhere. Results are identical; only the time differs.
+static bool WarmByBody( bool fusedPath )
+{
+	static const int mode = []() {
+		const char* v = getenv( "B2AMD_WARM_BY_BODY" );
+		return v != nullptr ? atoi( v ) : -1;
+	}();
+	if ( mode >= 0 )
+	{
+		return mode != 0;
+	}
+	return fusedPath == false;
 }
 
 void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bool needSleepTimes )
@@ -2656,6 +2733,8 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	// color kernels dominates, and loses once a world has enough constraints to need the whole GPU.
 	bool fits = nb > 0 && nb <= kFusedBodyCapacity && L.activeColorCount <= kMaxColorSlots;
 	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && nc <= kFusedMaxContacts ) );
+	// per-body warm-start lists exist when the host expected this path (see WantsWarmLists)
+	bool haveWarmLists = (int)L.warmStart.size() == nb + 1;
 
 	if ( impl->tape != nullptr && nb > 0 )
 	{
@@ -2747,6 +2826,9 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		fa.overflowJointCount = L.overflowJointCount;
 		fa.subStepCount = params.subStepCount;
 		fa.anyRestitution = anyRestitution ? 1 : 0;
+		fa.warmStart = impl->warmStart.ptr;
+		fa.warmList = impl->warmList.ptr;
+		fa.warmByBody = WarmByBody( true ) && haveWarmLists ? 1 : 0;
 		// constraint fields per substep (warm start, solve, relax) + body sims per substep + state in/out
 		double bytes = params.subStepCount * ( kBytesFusedContactSubstep * nc + kBytesFusedBodySubstep * nb + kBytesJoint * nj ) +
 					   kBytesFusedBodyStep * nb;
@@ -2762,11 +2844,28 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	}
 	else
 	{
+		WarmArgs wa;
+		wa.b = ba;
+		wa.c = ca.c;
+		wa.joints = impl->joints.ptr;
+		wa.jointList = impl->jointList.ptr;
+		wa.warmStart = impl->warmStart.ptr;
+		wa.warmList = impl->warmList.ptr;
+		wa.overflowContactStart = L.overflowContactStart;
+		bool byBody = WarmByBody( false ) && haveWarmLists;
 		for ( int sub = 0; sub < params.subStepCount; ++sub )
 		{
-			impl->Launch( kkIntegrateVelocities, kBytesIntegrateVelocities * nb,
-						  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( IntegrateVelocitiesKernel, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, ba ); } );
-			runStage( kStageWarmStart );
+			if ( byBody )
+			{
+				impl->Launch( kkIntegrateVelocities, kBytesIntegrateVelocities * nb + kBytesWarmStart * nc + kBytesJoint * nj,
+							  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( IntegrateWarmStartKernel, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, wa ); } );
+			}
+			else
+			{
+				impl->Launch( kkIntegrateVelocities, kBytesIntegrateVelocities * nb,
+							  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( IntegrateVelocitiesKernel, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, ba ); } );
+				runStage( kStageWarmStart );
+			}
 			runStage( kStageSolve );
 			impl->Launch( kkIntegratePositions, kBytesIntegratePositions * nb,
 						  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( IntegratePositionsKernel, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, ba ); } );
@@ -2986,6 +3085,18 @@ __global__ void CollideBatchKernel( const ShapeDev* shapesA, const ShapeDev* sha
 void DeviceWorld::SetSolverPath( int path )
 {
 	impl->solverPath = path;
+}
+
+bool DeviceWorld::WantsWarmLists( int bodyCount, int constraintCount, int activeColorCount ) const
+{
+	if ( bodyCount == 0 )
+	{
+		return false;
+	}
+	bool fits = bodyCount <= kFusedBodyCapacity && activeColorCount <= kMaxColorSlots;
+	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && constraintCount <= kFusedMaxContacts ) );
+	fused = fused || impl->tape != nullptr || impl->recording;
+	return WarmByBody( fused );
 }
 
 void DeviceWorld::SetTape( TapeStore* tape, int index )

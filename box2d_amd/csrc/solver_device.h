@@ -805,6 +805,102 @@ B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, const Bodies& 
 }
 
 // ---------------------------------------------------------------------------------------------
+// Warm start by body
+
+// The warm-start stage of one dynamic body: the velocity deltas of its constraints' accumulated
+// impulses applied in the stage's order (overflow joints, overflow contacts, then per color its
+// joints and contacts). The deltas depend on impulses, anchors, normals, masses and the body's
+// delta rotation, never on velocities, so the per-body sequence of float operations is exactly the
+// one the color-by-color stage applies to this body (WarmStartContact, OverflowWarmStart,
+// WarmStartRevolute). Entries: index << 3 | owner << 2 | joint << 1 | side; the owner entry of a
+// contact also accumulates the contact's total normal impulses.
+B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, const int* jointList, int overflowContactStart,
+							 const int* list, int begin, int end, b2Rot dq, float& vx, float& vy, float& w )
+{
+	for ( int e = begin; e < end; ++e )
+	{
+		int entry = list[e];
+		int k = entry >> 3;
+		bool sideB = ( entry & 1 ) != 0;
+		if ( entry & 2 )
+		{
+			const JointSim* joint = joints + jointList[k];
+			b2Vec2 L = joint->linearImpulse;
+			float axialImpulse = joint->springImpulse + joint->motorImpulse + joint->lowerImpulse - joint->upperImpulse;
+			if ( sideB == false )
+			{
+				b2Vec2 rA = b2RotateVector( dq, joint->frameA.p );
+				float mA = joint->invMassA, iA = joint->invIA;
+				vx = vx - mA * L.x;
+				vy = vy - mA * L.y;
+				w -= iA * ( b2Cross( rA, L ) + axialImpulse );
+			}
+			else
+			{
+				b2Vec2 rB = b2RotateVector( dq, joint->frameB.p );
+				float mB = joint->invMassB, iB = joint->invIB;
+				vx = vx + mB * L.x;
+				vy = vy + mB * L.y;
+				w += iB * ( b2Cross( rB, L ) + axialImpulse );
+			}
+			continue;
+		}
+
+		bool owner = ( entry & 4 ) != 0;
+		bool overflow = k >= overflowContactStart;
+		float nX = c.f[kcNormalX][k], nY = c.f[kcNormalY][k];
+		// the wide lanes form the tangent as 0 - nX, the scalar overflow solver as -nX
+		float tX = nY;
+		float tY = overflow ? -nX : 0.0f - nX;
+		int points = overflow ? c.pointCount[k] : 2;
+		float m = sideB ? c.f[kcInvMassB][k] : c.f[kcInvMassA][k];
+		float i = sideB ? c.f[kcInvIB][k] : c.f[kcInvIA][k];
+		for ( int j = 0; j < points; ++j )
+		{
+			int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+			float ni = c.f[kcNormalImpulse1 + o][k], ti = c.f[kcTangentImpulse1 + o][k];
+			float Px = ni * nX + ti * tX;
+			float Py = ni * nY + ti * tY;
+			if ( sideB == false )
+			{
+				float rx = c.f[kcAnchorA1X + o][k], ry = c.f[kcAnchorA1Y + o][k];
+				w = w - i * ( rx * Py - ry * Px );
+				vx = vx - m * Px;
+				vy = vy - m * Py;
+			}
+			else
+			{
+				float rx = c.f[kcAnchorB1X + o][k], ry = c.f[kcAnchorB1Y + o][k];
+				w = w + i * ( rx * Py - ry * Px );
+				vx = vx + m * Px;
+				vy = vy + m * Py;
+			}
+		}
+		float rolling = c.f[kcRollingImpulse][k];
+		w = sideB ? w + i * rolling : w - i * rolling;
+		(void)owner;
+	}
+
+	// total normal impulses, once per contact (kept out of the loop above so its loads carry no
+	// dependence on stores)
+	for ( int e = begin; e < end; ++e )
+	{
+		int entry = list[e];
+		if ( ( entry & 6 ) != 4 )
+		{
+			continue;
+		}
+		int k = entry >> 3;
+		int points = k >= overflowContactStart ? c.pointCount[k] : 2;
+		for ( int j = 0; j < points; ++j )
+		{
+			int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+			c.f[kcTotalNormalImpulse1 + o][k] = c.f[kcTotalNormalImpulse1 + o][k] + c.f[kcNormalImpulse1 + o][k];
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
 // Revolute joint
 
 template <typename Bodies>

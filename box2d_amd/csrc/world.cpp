@@ -2490,6 +2490,77 @@ static void BuildSolverLists( World& w, SolverLists& L )
 	L.overflowJointStart = (int)L.jointList.size();
 	L.overflowJointCount = (int)overflow.jointIds.size();
 	L.jointList.insert( L.jointList.end(), overflow.jointIds.begin(), overflow.jointIds.end() );
+
+}
+
+// Per-body warm-start lists for the color-kernel path (SolverLists::warmStart/warmList). The warm
+// start adds velocity deltas that depend only on impulses, anchors and delta rotations, never on
+// velocities, so applying a body's deltas in stage order in one pass gives the same floats as the
+// color-by-color stage.
+static void BuildWarmLists( World& w, SolverLists& L )
+{
+	int nb = (int)L.awakeBodies.size();
+	auto awakeDynamic = [&]( int bodyId ) {
+		const Body& b = w.bodies[bodyId];
+		return b.setIndex == kAwakeSet && b.type == b2_dynamicBody ? b.localIndex : -1;
+	};
+	// visit( awake index, entry ) for every entry in stage order
+	auto forEachEntry = [&]( auto&& visit ) {
+		auto contact = [&]( int k ) {
+			const Contact& c = w.contacts[L.constraintContacts[k]];
+			int a = awakeDynamic( c.edges[0].bodyId );
+			int b = awakeDynamic( c.edges[1].bodyId );
+			if ( a >= 0 )
+			{
+				visit( a, ( k << 3 ) | ( 1 << 2 ) );
+			}
+			if ( b >= 0 )
+			{
+				visit( b, ( k << 3 ) | ( a < 0 ? 1 << 2 : 0 ) | 1 );
+			}
+		};
+		auto joint = [&]( int k ) {
+			const Joint& j = w.joints[L.jointList[k]];
+			int a = awakeDynamic( j.edges[0].bodyId );
+			int b = awakeDynamic( j.edges[1].bodyId );
+			if ( a >= 0 )
+			{
+				visit( a, ( k << 3 ) | 2 );
+			}
+			if ( b >= 0 )
+			{
+				visit( b, ( k << 3 ) | 2 | 1 );
+			}
+		};
+		for ( int k = L.overflowJointStart; k < L.overflowJointStart + L.overflowJointCount; ++k )
+		{
+			joint( k );
+		}
+		for ( int k = L.overflowContactStart; k < L.overflowContactStart + L.overflowContactCount; ++k )
+		{
+			contact( k );
+		}
+		for ( int c = 0; c < L.activeColorCount; ++c )
+		{
+			for ( int k = L.colorJointStart[c]; k < L.colorJointStart[c + 1]; ++k )
+			{
+				joint( k );
+			}
+			for ( int k = L.colorContactStart[c]; k < L.colorContactStart[c + 1]; ++k )
+			{
+				contact( k );
+			}
+		}
+	};
+	L.warmStart.assign( nb + 1, 0 );
+	forEachEntry( [&]( int body, int ) { L.warmStart[body + 1] += 1; } );
+	for ( int i = 0; i < nb; ++i )
+	{
+		L.warmStart[i + 1] += L.warmStart[i];
+	}
+	L.warmList.resize( L.warmStart[nb] );
+	std::vector<int> fill( L.warmStart.begin(), L.warmStart.end() - 1 );
+	forEachEntry( [&]( int body, int entry ) { L.warmList[fill[body]++] = entry; } );
 }
 
 void World::Step( float timeStep, int subStepCount )
@@ -2569,12 +2640,25 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	device->UploadContactInits( pendingContactInits, contactIdPool.Capacity() );
 	pendingContactInits.clear();
 
-	SolverLists lists;
 	if ( listsDirty )
 	{
-		BuildSolverLists( *this, lists );
-		device->UploadLists( lists );
+		BuildSolverLists( *this, solverLists );
+		device->UploadLists( solverLists );
 		listsDirty = false;
+		listsStableSteps = 0;
+	}
+	else
+	{
+		// A contact graph that stayed the same for two steps gets the per-body warm-start lists
+		// (worlds whose graph changes every step keep the color-by-color warm start).
+		listsStableSteps += 1;
+		int nb = (int)solverLists.awakeBodies.size();
+		if ( listsStableSteps >= 2 && solverLists.warmStart.empty() &&
+			 device->WantsWarmLists( nb, (int)solverLists.constraintContacts.size(), solverLists.activeColorCount ) )
+		{
+			BuildWarmLists( *this, solverLists );
+			device->UploadWarmLists( solverLists );
+		}
 	}
 
 	profile.solverSetup = b2GetMilliseconds( setupTicks );
@@ -2643,10 +2727,10 @@ bool World::SolveBegin( StepParams& params )
 
 	if ( listsDirty )
 	{
-		SolverLists lists;
-		BuildSolverLists( *this, lists );
-		device->UploadLists( lists );
+		BuildSolverLists( *this, solverLists );
+		device->UploadLists( solverLists );
 		listsDirty = false;
+		listsStableSteps = 0;
 	}
 	profile.splitIslands = b2GetMilliseconds( splitTicks );
 

@@ -8,6 +8,7 @@
 #pragma once
 
 #include "core.h"
+#include "device.h"
 #include "layout.h"
 #include "tree.h"
 
@@ -296,6 +297,8 @@ public:
 	bool jointsDirty = true;
 	bool jointsStale = false;
 	bool listsDirty = true; // solver work lists (awake bodies, colors, contacts) changed
+	SolverLists solverLists; // the lists last uploaded
+	int listsStableSteps = 0; // steps since the lists last changed
 	int restitutionShapeCount = 0;
 
 	// events and statistics
