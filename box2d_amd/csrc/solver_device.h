@@ -814,71 +814,116 @@ B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, const Bodies& 
 // one the color-by-color stage applies to this body (WarmStartContact, OverflowWarmStart,
 // WarmStartRevolute). Entries: index << 3 | owner << 2 | joint << 1 | side; the owner entry of a
 // contact also accumulates the contact's total normal impulses.
+B2G_DEVI void WarmStartJointSide( const JointSim* joint, bool sideB, b2Rot dq, float& vx, float& vy, float& w )
+{
+	b2Vec2 L = joint->linearImpulse;
+	float axialImpulse = joint->springImpulse + joint->motorImpulse + joint->lowerImpulse - joint->upperImpulse;
+	if ( sideB == false )
+	{
+		b2Vec2 rA = b2RotateVector( dq, joint->frameA.p );
+		float mA = joint->invMassA, iA = joint->invIA;
+		vx = vx - mA * L.x;
+		vy = vy - mA * L.y;
+		w -= iA * ( b2Cross( rA, L ) + axialImpulse );
+	}
+	else
+	{
+		b2Vec2 rB = b2RotateVector( dq, joint->frameB.p );
+		float mB = joint->invMassB, iB = joint->invIB;
+		vx = vx + mB * L.x;
+		vy = vy + mB * L.y;
+		w += iB * ( b2Cross( rB, L ) + axialImpulse );
+	}
+}
+
 B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, const int* jointList, int overflowContactStart,
 							 const int* list, int begin, int end, b2Rot dq, float& vx, float& vy, float& w )
 {
-	for ( int e = begin; e < end; ++e )
+	// Entries go in chunks: all of a chunk's constraint fields are gathered first (independent
+	// loads), then applied in order.
+	constexpr int kChunk = 4;
+	for ( int base = begin; base < end; base += kChunk )
 	{
-		int entry = list[e];
-		int k = entry >> 3;
-		bool sideB = ( entry & 1 ) != 0;
-		if ( entry & 2 )
+		int count = end - base < kChunk ? end - base : kChunk;
+		int ent[kChunk];
+#pragma unroll
+		for ( int q = 0; q < kChunk; ++q )
 		{
-			const JointSim* joint = joints + jointList[k];
-			b2Vec2 L = joint->linearImpulse;
-			float axialImpulse = joint->springImpulse + joint->motorImpulse + joint->lowerImpulse - joint->upperImpulse;
-			if ( sideB == false )
-			{
-				b2Vec2 rA = b2RotateVector( dq, joint->frameA.p );
-				float mA = joint->invMassA, iA = joint->invIA;
-				vx = vx - mA * L.x;
-				vy = vy - mA * L.y;
-				w -= iA * ( b2Cross( rA, L ) + axialImpulse );
-			}
-			else
-			{
-				b2Vec2 rB = b2RotateVector( dq, joint->frameB.p );
-				float mB = joint->invMassB, iB = joint->invIB;
-				vx = vx + mB * L.x;
-				vy = vy + mB * L.y;
-				w += iB * ( b2Cross( rB, L ) + axialImpulse );
-			}
-			continue;
+			ent[q] = q < count ? list[base + q] : 2; // padding reads as a joint entry and is skipped
 		}
 
-		bool owner = ( entry & 4 ) != 0;
-		bool overflow = k >= overflowContactStart;
-		float nX = c.f[kcNormalX][k], nY = c.f[kcNormalY][k];
-		// the wide lanes form the tangent as 0 - nX, the scalar overflow solver as -nX
-		float tX = nY;
-		float tY = overflow ? -nX : 0.0f - nX;
-		int points = overflow ? c.pointCount[k] : 2;
-		float m = sideB ? c.f[kcInvMassB][k] : c.f[kcInvMassA][k];
-		float i = sideB ? c.f[kcInvIB][k] : c.f[kcInvIA][k];
-		for ( int j = 0; j < points; ++j )
+		float nX[kChunk], nY[kChunk], m[kChunk], iv[kChunk], roll[kChunk];
+		float ni[kChunk][2], ti[kChunk][2], rx[kChunk][2], ry[kChunk][2];
+		int points[kChunk];
+#pragma unroll
+		for ( int q = 0; q < kChunk; ++q )
 		{
-			int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
-			float ni = c.f[kcNormalImpulse1 + o][k], ti = c.f[kcTangentImpulse1 + o][k];
-			float Px = ni * nX + ti * tX;
-			float Py = ni * nY + ti * tY;
-			if ( sideB == false )
+			if ( ent[q] & 2 )
 			{
-				float rx = c.f[kcAnchorA1X + o][k], ry = c.f[kcAnchorA1Y + o][k];
-				w = w - i * ( rx * Py - ry * Px );
-				vx = vx - m * Px;
-				vy = vy - m * Py;
+				continue;
 			}
-			else
+			int k = ent[q] >> 3;
+			bool sideB = ( ent[q] & 1 ) != 0;
+			bool overflow = k >= overflowContactStart;
+			nX[q] = c.f[kcNormalX][k];
+			nY[q] = c.f[kcNormalY][k];
+			m[q] = sideB ? c.f[kcInvMassB][k] : c.f[kcInvMassA][k];
+			iv[q] = sideB ? c.f[kcInvIB][k] : c.f[kcInvIA][k];
+			roll[q] = c.f[kcRollingImpulse][k];
+			points[q] = overflow ? c.pointCount[k] : 2;
+#pragma unroll
+			for ( int j = 0; j < 2; ++j )
 			{
-				float rx = c.f[kcAnchorB1X + o][k], ry = c.f[kcAnchorB1Y + o][k];
-				w = w + i * ( rx * Py - ry * Px );
-				vx = vx + m * Px;
-				vy = vy + m * Py;
+				int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
+				ni[q][j] = c.f[kcNormalImpulse1 + o][k];
+				ti[q][j] = c.f[kcTangentImpulse1 + o][k];
+				rx[q][j] = sideB ? c.f[kcAnchorB1X + o][k] : c.f[kcAnchorA1X + o][k];
+				ry[q][j] = sideB ? c.f[kcAnchorB1Y + o][k] : c.f[kcAnchorA1Y + o][k];
 			}
 		}
-		float rolling = c.f[kcRollingImpulse][k];
-		w = sideB ? w + i * rolling : w - i * rolling;
-		(void)owner;
+
+#pragma unroll
+		for ( int q = 0; q < kChunk; ++q )
+		{
+			if ( q >= count )
+			{
+				break;
+			}
+			int entry = ent[q];
+			bool sideB = ( entry & 1 ) != 0;
+			if ( entry & 2 )
+			{
+				WarmStartJointSide( joints + jointList[entry >> 3], sideB, dq, vx, vy, w );
+				continue;
+			}
+			bool overflow = ( entry >> 3 ) >= overflowContactStart;
+			// the wide lanes form the tangent as 0 - nX, the scalar overflow solver as -nX
+			float tX = nY[q];
+			float tY = overflow ? -nX[q] : 0.0f - nX[q];
+#pragma unroll
+			for ( int j = 0; j < 2; ++j )
+			{
+				if ( j >= points[q] )
+				{
+					break;
+				}
+				float Px = ni[q][j] * nX[q] + ti[q][j] * tX;
+				float Py = ni[q][j] * nY[q] + ti[q][j] * tY;
+				if ( sideB == false )
+				{
+					w = w - iv[q] * ( rx[q][j] * Py - ry[q][j] * Px );
+					vx = vx - m[q] * Px;
+					vy = vy - m[q] * Py;
+				}
+				else
+				{
+					w = w + iv[q] * ( rx[q][j] * Py - ry[q][j] * Px );
+					vx = vx + m[q] * Px;
+					vy = vy + m[q] * Py;
+				}
+			}
+			w = sideB ? w + iv[q] * roll[q] : w - iv[q] * roll[q];
+		}
 	}
 
 	// total normal impulses, once per contact (kept out of the loop above so its loads carry no
