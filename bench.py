@@ -270,7 +270,7 @@ def main():
     ap.add_argument("--batched-worlds", type=int, default=4096, help="worlds of the batched workload (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--rain-steps", type=int, default=1000, help="steps of the rain workload (0: skip)")
-    ap.add_argument("--rain-cpu-steps", type=int, default=150, help="bounded CPU-baseline sample of the rain")
+    ap.add_argument("--rain-cpu-steps", type=int, default=1000, help="CPU-baseline sample of the rain (steps from the start)")
     ap.add_argument("--push-worlds", type=int, default=256, help="worlds of the differentiable push (0: skip)")
     ap.add_argument("--push-steps", type=int, default=60)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",

@@ -3,13 +3,18 @@
 // The reference finds new pairs by querying its three dynamic trees with the fat AABB of every
 // proxy in the move buffer (src/broad_phase.c:333-399, src/dynamic_tree.c:1085-1135) and filtering
 // each hit in b2PairQueryCallback (src/broad_phase.c:172-324). Here the overlap search runs on the
-// GPU: the fat AABBs of all proxies are sorted along a 2D Morton curve, a binary radix tree is
+// GPU: the fat AABBs of all proxies are sorted along a 2D Morton curve and a binary radix tree is
 // built over the sorted codes (Karras, "Maximizing Parallelism in the Construction of BVHs,
-// Octrees, and k-d Trees", HPG 2012), node boxes are refitted bottom-up, and one lane per moved
-// proxy walks the tree. Each lane applies the callback's order-free rules (self, which trees a
-// proxy type queries, the moved-proxy duplicate rule) and appends its hits. The candidate set is
-// exactly the set of leaves the reference queries visit and keep at that point; the host sorts
-// each query's candidates into the reference's visit order and applies the remaining filters.
+// Octrees, and k-d Trees", HPG 2012). Each internal node's box is the union of its sorted-leaf
+// range, taken from leaf boxes and 64-leaf chunk boxes by one wavefront per node (exact: min and
+// max are order-free), so no bottom-up refit pass is needed. Every node carries a rope, the next
+// node of a depth-first walk once its subtree is skipped, and one lane per moved proxy walks the
+// tree without a stack. Each lane applies the callback's order-free rules (self, which trees a
+// proxy type queries, the moved-proxy duplicate rule) and emits (query, proxy index) keys, which
+// are radix sorted on the device. The caller numbers the proxies in the reference's visit order
+// (kinematic tree, static tree, dynamic tree, each depth first), so the sorted keys list every
+// query's candidates in the order the reference's queries visit them; the host applies the
+// remaining filters in that order.
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -36,13 +41,44 @@ namespace b2g
 {
 
 constexpr int kLbvhBlock = 256;
-constexpr int kLbvhStack = 96;
+constexpr int kLbvhChunk = 64; // leaves per chunk box (one wavefront)
 
 __device__ __forceinline__ bool Overlaps( const b2AABB& a, const b2AABB& b )
 {
 	// b2AABB_Overlaps (include/box2d/math_functions.h)
 	return !( b.lowerBound.x > a.upperBound.x || b.lowerBound.y > a.upperBound.y || a.lowerBound.x > b.upperBound.x ||
 			  a.lowerBound.y > b.upperBound.y );
+}
+
+__device__ __forceinline__ b2AABB EmptyBox()
+{
+	b2AABB b;
+	b.lowerBound = { FLT_MAX, FLT_MAX };
+	b.upperBound = { -FLT_MAX, -FLT_MAX };
+	return b;
+}
+
+__device__ __forceinline__ void Grow( b2AABB& a, const b2AABB& b )
+{
+	a.lowerBound.x = fminf( a.lowerBound.x, b.lowerBound.x );
+	a.lowerBound.y = fminf( a.lowerBound.y, b.lowerBound.y );
+	a.upperBound.x = fmaxf( a.upperBound.x, b.upperBound.x );
+	a.upperBound.y = fmaxf( a.upperBound.y, b.upperBound.y );
+}
+
+// Union over the 64 lanes of a wavefront (every lane must be active).
+__device__ __forceinline__ b2AABB WaveUnion( b2AABB a )
+{
+	for ( int m = 1; m < 64; m <<= 1 )
+	{
+		b2AABB o;
+		o.lowerBound.x = __shfl_xor( a.lowerBound.x, m, 64 );
+		o.lowerBound.y = __shfl_xor( a.lowerBound.y, m, 64 );
+		o.upperBound.x = __shfl_xor( a.upperBound.x, m, 64 );
+		o.upperBound.y = __shfl_xor( a.upperBound.y, m, 64 );
+		Grow( a, o );
+	}
+	return a;
 }
 
 // Scene bounds of the box centers (one workgroup).
@@ -92,8 +128,9 @@ __device__ __forceinline__ uint32_t SpreadBits( uint32_t v )
 	return v;
 }
 
-// 64-bit keys: 32-bit Morton code of the box center above the proxy index (unique keys).
-__global__ void LbvhMortonKernel( const LbvhProxy* proxies, int n, const float* bounds, uint64_t* keys )
+// 32-bit Morton code of each box center; the proxy index rides along as the sort value (the sort
+// is stable, so equal codes stay in proxy order).
+__global__ void LbvhMortonKernel( const LbvhProxy* proxies, int n, const float* bounds, uint32_t* codes, uint32_t* index )
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( i >= n )
@@ -109,48 +146,73 @@ __global__ void LbvhMortonKernel( const LbvhProxy* proxies, int n, const float* 
 	float fy = ey > 0.0f ? ( cy - bounds[1] ) / ey : 0.0f;
 	uint32_t qx = (uint32_t)fminf( fmaxf( fx * 65536.0f, 0.0f ), 65535.0f );
 	uint32_t qy = (uint32_t)fminf( fmaxf( fy * 65536.0f, 0.0f ), 65535.0f );
-	uint32_t code = ( SpreadBits( qx ) << 1 ) | SpreadBits( qy );
-	keys[i] = ( (uint64_t)code << 32 ) | (uint32_t)i;
+	codes[i] = ( SpreadBits( qx ) << 1 ) | SpreadBits( qy );
+	index[i] = (uint32_t)i;
 }
+
+// Tree nodes, 32 bytes: internal nodes 0..n-2 (node 0 is the root), leaf k of the sorted order at
+// n-1+k (for n == 1 the single leaf is node 0).
+struct LbvhNode
+{
+	b2AABB box;
+	int child;	  // internal: node index of the left child; leaf: ~(proxy index)
+	int rope;	  // next node once this subtree is skipped, -1 at the end of the walk
+	int proxyKey; // leaves only
+	int shapeId;  // leaves only
+};
 
 struct LbvhTree
 {
-	b2AABB* nodeBox; // internal nodes, n - 1
-	int* left;		 // child encoding: >= 0 internal node, < 0 leaf ~k
-	int* right;
-	int* parent;	   // internal parents (n - 1), then leaf parents (n)
-	int* flags;		   // refit arrival counters, n - 1
-	b2AABB* leafBox;   // sorted leaves
-	LbvhProxy* leaves; // sorted proxies
+	LbvhNode* nodes;	 // 2n - 1
+	b2AABB* chunkBox;	 // union of each 64 consecutive sorted leaves
+	int* rightOf;		 // n - 1: node index of the right child of the internal node that splits at k | k+1
+	int* rangeLo;		 // n - 1: sorted-leaf range of each internal node
+	int* rangeHi;
+	const uint32_t* codes; // sorted Morton codes
 	int n;
 };
 
-__global__ void LbvhLeafKernel( const LbvhProxy* proxies, const uint64_t* sortedKeys, LbvhTree t )
+// Sorted leaves and their chunk boxes (blocks of whole wavefronts, so each wavefront is a chunk).
+__global__ __launch_bounds__( kLbvhBlock ) void LbvhLeafKernel( const LbvhProxy* proxies, const uint32_t* sortedIndex, LbvhTree t )
 {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	b2AABB box = EmptyBox();
 	if ( k < t.n )
 	{
-		int idx = (int)( sortedKeys[k] & 0xFFFFFFFFull );
-		t.leaves[k] = proxies[idx];
-		t.leafBox[k] = proxies[idx].aabb;
+		int idx = (int)sortedIndex[k];
+		const LbvhProxy p = proxies[idx];
+		box = p.aabb;
+		LbvhNode& node = t.nodes[t.n - 1 + k];
+		node.box = box;
+		node.child = ~idx;
+		node.proxyKey = p.proxyKey;
+		node.shapeId = p.shapeId;
 	}
-	if ( k < t.n - 1 )
+	box = WaveUnion( box );
+	if ( ( threadIdx.x & 63 ) == 0 && k < t.n )
 	{
-		t.flags[k] = 0;
+		t.chunkBox[k / kLbvhChunk] = box;
 	}
 }
 
-__device__ __forceinline__ int Delta( const uint64_t* keys, int n, int i, int j )
+// Common prefix length of sorted keys i and j, -1 outside [0, n); keys are (code, position) so
+// they are unique.
+__device__ __forceinline__ int Delta( const uint32_t* codes, int n, int i, int j )
 {
 	if ( j < 0 || j >= n )
 	{
 		return -1;
 	}
-	return __clzll( keys[i] ^ keys[j] );
+	uint32_t a = codes[i], b = codes[j];
+	if ( a != b )
+	{
+		return __clz( a ^ b );
+	}
+	return 32 + __clz( (uint32_t)i ^ (uint32_t)j );
 }
 
-// Binary radix tree over the sorted unique keys (Karras 2012, algorithm 1).
-__global__ void LbvhBuildKernel( const uint64_t* keys, LbvhTree t )
+// Binary radix tree over the sorted keys (Karras 2012, algorithm 1).
+__global__ void LbvhBuildKernel( LbvhTree t )
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	int n = t.n;
@@ -158,29 +220,30 @@ __global__ void LbvhBuildKernel( const uint64_t* keys, LbvhTree t )
 	{
 		return;
 	}
-	int d = Delta( keys, n, i, i + 1 ) - Delta( keys, n, i, i - 1 ) >= 0 ? 1 : -1;
-	int dmin = Delta( keys, n, i, i - d );
+	const uint32_t* codes = t.codes;
+	int d = Delta( codes, n, i, i + 1 ) - Delta( codes, n, i, i - 1 ) >= 0 ? 1 : -1;
+	int dmin = Delta( codes, n, i, i - d );
 	int lmax = 2;
-	while ( Delta( keys, n, i, i + lmax * d ) > dmin )
+	while ( Delta( codes, n, i, i + lmax * d ) > dmin )
 	{
 		lmax <<= 1;
 	}
 	int l = 0;
 	for ( int step = lmax >> 1; step >= 1; step >>= 1 )
 	{
-		if ( Delta( keys, n, i, i + ( l + step ) * d ) > dmin )
+		if ( Delta( codes, n, i, i + ( l + step ) * d ) > dmin )
 		{
 			l += step;
 		}
 	}
 	int j = i + l * d;
-	int dnode = Delta( keys, n, i, j );
+	int dnode = Delta( codes, n, i, j );
 	int s = 0;
 	int step = l;
 	do
 	{
 		step = ( step + 1 ) >> 1;
-		if ( Delta( keys, n, i, i + ( s + step ) * d ) > dnode )
+		if ( Delta( codes, n, i, i + ( s + step ) * d ) > dnode )
 		{
 			s += step;
 		}
@@ -189,120 +252,112 @@ __global__ void LbvhBuildKernel( const uint64_t* keys, LbvhTree t )
 	int gamma = i + s * d + ( d < 0 ? d : 0 );
 	int lo = i < j ? i : j;
 	int hi = i < j ? j : i;
-	int leftChild = lo == gamma ? ~gamma : gamma;
-	int rightChild = hi == gamma + 1 ? ~( gamma + 1 ) : gamma + 1;
-	t.left[i] = leftChild;
-	t.right[i] = rightChild;
-	if ( leftChild >= 0 )
-	{
-		t.parent[leftChild] = i;
-	}
-	else
-	{
-		t.parent[n - 1 + ~leftChild] = i;
-	}
-	if ( rightChild >= 0 )
-	{
-		t.parent[rightChild] = i;
-	}
-	else
-	{
-		t.parent[n - 1 + ~rightChild] = i;
-	}
-	if ( i == 0 )
-	{
-		t.parent[0] = -1;
-	}
+	int left = lo == gamma ? n - 1 + gamma : gamma;
+	int right = hi == gamma + 1 ? n - 1 + gamma + 1 : gamma + 1;
+	t.nodes[i].child = left;
+	t.nodes[i].proxyKey = -1;
+	t.nodes[i].shapeId = -1;
+	t.rightOf[gamma] = right;
+	t.rangeLo[i] = lo;
+	t.rangeHi[i] = hi;
 }
 
-__device__ __forceinline__ float CoherentLoad( const float* p )
+// Ropes of every node and boxes of the internal nodes. Blocks [0, leafBlocks) take one leaf per
+// lane; the rest take one internal node per wavefront.
+__global__ __launch_bounds__( kLbvhBlock ) void LbvhFinishKernel( LbvhTree t, int leafBlocks )
 {
-	return __uint_as_float( __hip_atomic_load( reinterpret_cast<const unsigned int*>( p ), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) );
-}
-
-__device__ __forceinline__ b2AABB ChildBox( const LbvhTree& t, int child )
-{
-	const float* p = child >= 0 ? &t.nodeBox[child].lowerBound.x : &t.leafBox[~child].lowerBound.x;
-	b2AABB b;
-	b.lowerBound.x = CoherentLoad( p + 0 );
-	b.lowerBound.y = CoherentLoad( p + 1 );
-	b.upperBound.x = CoherentLoad( p + 2 );
-	b.upperBound.y = CoherentLoad( p + 3 );
-	return b;
-}
-
-// Bottom-up refit: the second lane to reach a node unions its children and climbs on.
-__global__ void LbvhRefitKernel( LbvhTree t )
-{
-	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if ( k >= t.n || t.n < 2 )
+	int n = t.n;
+	if ( (int)blockIdx.x < leafBlocks )
 	{
+		int k = blockIdx.x * blockDim.x + threadIdx.x;
+		if ( k < n )
+		{
+			t.nodes[n - 1 + k].rope = k < n - 1 ? t.rightOf[k] : -1;
+		}
 		return;
 	}
-	int node = t.parent[t.n - 1 + k];
-	while ( node >= 0 )
+	int i = ( blockIdx.x - leafBlocks ) * ( blockDim.x / 64 ) + threadIdx.x / 64;
+	int lane = threadIdx.x & 63;
+	if ( i >= n - 1 )
 	{
-		__threadfence();
-		int old = atomicAdd( t.flags + node, 1 );
-		if ( old == 0 )
+		return; // whole wavefront
+	}
+	int lo = t.rangeLo[i], hi = t.rangeHi[i];
+	b2AABB box = EmptyBox();
+	int c0 = ( lo + kLbvhChunk - 1 ) / kLbvhChunk; // first whole chunk
+	int c1 = ( hi + 1 ) / kLbvhChunk;			   // one past the last whole chunk
+	if ( c0 >= c1 )
+	{
+		for ( int k = lo + lane; k <= hi; k += 64 )
 		{
-			return;
+			Grow( box, t.nodes[n - 1 + k].box );
 		}
-		b2AABB a = ChildBox( t, t.left[node] );
-		b2AABB b = ChildBox( t, t.right[node] );
-		b2AABB u;
-		u.lowerBound.x = fminf( a.lowerBound.x, b.lowerBound.x );
-		u.lowerBound.y = fminf( a.lowerBound.y, b.lowerBound.y );
-		u.upperBound.x = fmaxf( a.upperBound.x, b.upperBound.x );
-		u.upperBound.y = fmaxf( a.upperBound.y, b.upperBound.y );
-		t.nodeBox[node] = u;
-		node = t.parent[node];
+	}
+	else
+	{
+		for ( int k = lo + lane; k < c0 * kLbvhChunk; k += 64 )
+		{
+			Grow( box, t.nodes[n - 1 + k].box );
+		}
+		for ( int c = c0 + lane; c < c1; c += 64 )
+		{
+			Grow( box, t.chunkBox[c] );
+		}
+		for ( int k = c1 * kLbvhChunk + lane; k <= hi; k += 64 )
+		{
+			Grow( box, t.nodes[n - 1 + k].box );
+		}
+	}
+	box = WaveUnion( box );
+	if ( lane == 0 )
+	{
+		t.nodes[i].box = box;
+		t.nodes[i].rope = hi < n - 1 ? t.rightOf[hi] : -1;
 	}
 }
 
 struct LbvhQueryArgs
 {
-	LbvhTree t;
+	const LbvhNode* nodes;
+	int nodeCount;
 	const LbvhProxy* queries;
 	int queryCount;
 	const uint8_t* moved; // by shape id: the shape's proxy is in the move buffer
-	PairCandidate* out;
+	uint64_t* out;		  // (query << 32 | proxy index)
 	int capacity;
-	int* counters; // [0] candidates, [1] stack overflow
+	int* counters; // [0] candidates, [1] walks cut off by the step guard
 };
 
 // Hit rules of b2PairQueryCallback that do not depend on host state (src/broad_phase.c:172-216).
-__device__ __forceinline__ void LbvhVisit( const LbvhQueryArgs& a, int q, const LbvhProxy& query, const LbvhProxy& hit )
+__device__ __forceinline__ bool LbvhKeep( const LbvhQueryArgs& a, int queryKey, int hitKey, int hitShape )
 {
-	int queryType = query.proxyKey & 3;
-	int hitType = hit.proxyKey & 3;
-	if ( hit.proxyKey == query.proxyKey )
+	int queryType = queryKey & 3;
+	int hitType = hitKey & 3;
+	if ( hitKey == queryKey )
 	{
-		return;
+		return false;
 	}
 	if ( queryType == b2_dynamicBody )
 	{
-		if ( hitType == b2_dynamicBody && hit.proxyKey < query.proxyKey && a.moved[hit.shapeId] )
+		if ( hitType == b2_dynamicBody && hitKey < queryKey && a.moved[hitShape] )
 		{
-			return;
+			return false;
 		}
 	}
 	else
 	{
 		// static and kinematic proxies only query the dynamic tree
-		if ( hitType != b2_dynamicBody || a.moved[hit.shapeId] )
+		if ( hitType != b2_dynamicBody || a.moved[hitShape] )
 		{
-			return;
+			return false;
 		}
 	}
-	int slot = atomicAdd( a.counters, 1 );
-	if ( slot < a.capacity )
-	{
-		a.out[slot] = PairCandidate{ q, hit.proxyKey, hit.shapeId };
-	}
+	return true;
 }
 
-__global__ void LbvhQueryKernel( LbvhQueryArgs a )
+// One lane per query: stackless depth-first walk along the ropes. Every node is entered at most
+// once, so the walk takes at most nodeCount steps; the guard only bounds a malformed tree.
+__global__ __launch_bounds__( kLbvhBlock ) void LbvhQueryKernel( LbvhQueryArgs a )
 {
 	int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( q >= a.queryCount )
@@ -311,44 +366,51 @@ __global__ void LbvhQueryKernel( LbvhQueryArgs a )
 	}
 	const LbvhProxy query = a.queries[q];
 	const b2AABB box = query.aabb;
-	const LbvhTree& t = a.t;
-	if ( t.n == 1 )
+	const int lane = threadIdx.x & 63;
+	int node = 0;
+	int steps = 0;
+	while ( node >= 0 )
 	{
-		if ( Overlaps( t.leafBox[0], box ) )
+		if ( ++steps > a.nodeCount )
 		{
-			LbvhVisit( a, q, query, t.leaves[0] );
+			atomicAdd( a.counters + 1, 1 );
+			break;
 		}
-		return;
-	}
-	int stack[kLbvhStack];
-	int sp = 0;
-	stack[sp++] = 0;
-	while ( sp > 0 )
-	{
-		int node = stack[--sp];
-		int children[2] = { t.left[node], t.right[node] };
-		for ( int c = 0; c < 2; ++c )
+		const LbvhNode e = a.nodes[node];
+		bool hit = false;
+		int next = e.rope;
+		if ( Overlaps( e.box, box ) )
 		{
-			int child = children[c];
-			if ( child < 0 )
+			if ( e.child < 0 )
 			{
-				if ( Overlaps( t.leafBox[~child], box ) )
-				{
-					LbvhVisit( a, q, query, t.leaves[~child] );
-				}
+				hit = LbvhKeep( a, query.proxyKey, e.proxyKey, e.shapeId );
 			}
-			else if ( Overlaps( t.nodeBox[child], box ) )
+			else
 			{
-				if ( sp < kLbvhStack )
-				{
-					stack[sp++] = child;
-				}
-				else
-				{
-					atomicAdd( a.counters + 1, 1 );
-				}
+				next = e.child;
 			}
 		}
+		// one counter atomic per wavefront for all of its lanes' hits of this step
+		uint64_t mask = __ballot( hit );
+		if ( mask != 0 )
+		{
+			int leader = __ffsll( (unsigned long long)mask ) - 1;
+			int base = 0;
+			if ( lane == leader )
+			{
+				base = atomicAdd( a.counters, __popcll( mask ) );
+			}
+			base = __shfl( base, leader, 64 );
+			if ( hit )
+			{
+				int slot = base + __popcll( mask & ( ( 1ull << lane ) - 1ull ) );
+				if ( slot < a.capacity )
+				{
+					a.out[slot] = ( (uint64_t)q << 32 ) | (uint32_t)( ~e.child );
+				}
+			}
+		}
+		node = next;
 	}
 }
 
@@ -358,21 +420,43 @@ __global__ void LbvhQueryKernel( LbvhQueryArgs a )
 struct PairFinder::Impl
 {
 	hipStream_t stream = nullptr;
-	size_t proxyCap = 0, queryCap = 0, candCap = 0, tempBytes = 0;
+	size_t proxyCap = 0, queryCap = 0, candCap = 0;
 	LbvhProxy* proxies = nullptr;
 	LbvhProxy* queries = nullptr;
 	uint8_t* moved = nullptr;
 	size_t movedCap = 0;
-	uint64_t* keys = nullptr;
-	uint64_t* sorted = nullptr;
+	uint32_t* codes = nullptr;
+	uint32_t* sortedCodes = nullptr;
+	uint32_t* index = nullptr;
+	uint32_t* sortedIndex = nullptr;
 	void* temp = nullptr;
+	size_t tempBytes = 0;
 	float* bounds = nullptr;
 	LbvhTree tree = {};
-	PairCandidate* cands = nullptr;
+	uint64_t* cands = nullptr;
+	uint64_t* sortedCands = nullptr;
 	int* counters = nullptr;
 	int* hostCounters = nullptr;
-	PairCandidate* hostCands = nullptr;
+
+	// pinned host staging
+	LbvhProxy* hostProxies = nullptr;
+	size_t hostProxyCap = 0;
+	LbvhProxy* hostQueries = nullptr;
+	size_t hostQueryCap = 0;
+	uint8_t* hostMoved = nullptr;
+	size_t hostMovedCap = 0;
+	uint64_t* hostCands = nullptr;
 	size_t hostCandCap = 0;
+
+	void Temp( size_t need )
+	{
+		if ( need > tempBytes )
+		{
+			hipFree( temp );
+			tempBytes = need + need / 2;
+			HIP_CHECK( hipMalloc( &temp, tempBytes ) );
+		}
+	}
 
 	void Reserve( size_t n )
 	{
@@ -382,36 +466,43 @@ struct PairFinder::Impl
 		}
 		size_t cap = std::max<size_t>( 256, n + n / 2 );
 		hipFree( proxies );
-		hipFree( keys );
-		hipFree( sorted );
-		hipFree( tree.nodeBox );
-		hipFree( tree.left );
-		hipFree( tree.right );
-		hipFree( tree.parent );
-		hipFree( tree.flags );
-		hipFree( tree.leafBox );
-		hipFree( tree.leaves );
+		hipFree( codes );
+		hipFree( sortedCodes );
+		hipFree( index );
+		hipFree( sortedIndex );
+		hipFree( tree.nodes );
+		hipFree( tree.chunkBox );
+		hipFree( tree.rightOf );
+		hipFree( tree.rangeLo );
+		hipFree( tree.rangeHi );
 		HIP_CHECK( hipMalloc( &proxies, cap * sizeof( LbvhProxy ) ) );
-		HIP_CHECK( hipMalloc( &keys, cap * sizeof( uint64_t ) ) );
-		HIP_CHECK( hipMalloc( &sorted, cap * sizeof( uint64_t ) ) );
-		HIP_CHECK( hipMalloc( &tree.nodeBox, cap * sizeof( b2AABB ) ) );
-		HIP_CHECK( hipMalloc( &tree.left, cap * sizeof( int ) ) );
-		HIP_CHECK( hipMalloc( &tree.right, cap * sizeof( int ) ) );
-		HIP_CHECK( hipMalloc( &tree.parent, 2 * cap * sizeof( int ) ) );
-		HIP_CHECK( hipMalloc( &tree.flags, cap * sizeof( int ) ) );
-		HIP_CHECK( hipMalloc( &tree.leafBox, cap * sizeof( b2AABB ) ) );
-		HIP_CHECK( hipMalloc( &tree.leaves, cap * sizeof( LbvhProxy ) ) );
+		HIP_CHECK( hipMalloc( &codes, cap * sizeof( uint32_t ) ) );
+		HIP_CHECK( hipMalloc( &sortedCodes, cap * sizeof( uint32_t ) ) );
+		HIP_CHECK( hipMalloc( &index, cap * sizeof( uint32_t ) ) );
+		HIP_CHECK( hipMalloc( &sortedIndex, cap * sizeof( uint32_t ) ) );
+		HIP_CHECK( hipMalloc( &tree.nodes, 2 * cap * sizeof( LbvhNode ) ) );
+		HIP_CHECK( hipMalloc( &tree.chunkBox, ( cap / kLbvhChunk + 1 ) * sizeof( b2AABB ) ) );
+		HIP_CHECK( hipMalloc( &tree.rightOf, cap * sizeof( int ) ) );
+		HIP_CHECK( hipMalloc( &tree.rangeLo, cap * sizeof( int ) ) );
+		HIP_CHECK( hipMalloc( &tree.rangeHi, cap * sizeof( int ) ) );
 		proxyCap = cap;
 		size_t need = 0;
-		HIP_CHECK( hipcub::DeviceRadixSort::SortKeys( nullptr, need, keys, sorted, (int)cap, 0, 64, stream ) );
-		if ( need > tempBytes )
-		{
-			hipFree( temp );
-			HIP_CHECK( hipMalloc( &temp, need ) );
-			tempBytes = need;
-		}
+		HIP_CHECK( hipcub::DeviceRadixSort::SortPairs( nullptr, need, codes, sortedCodes, index, sortedIndex, (int)cap, 0, 32, stream ) );
+		Temp( need );
 	}
 };
+
+template <typename T>
+static T* GrowPinned( T*& p, size_t& cap, size_t need )
+{
+	if ( need > cap )
+	{
+		hipHostFree( p );
+		cap = std::max<size_t>( 256, need + need / 2 );
+		HIP_CHECK( hipHostMalloc( &p, cap * sizeof( T ), hipHostMallocDefault ) );
+	}
+	return p;
+}
 
 PairFinder::PairFinder()
 {
@@ -428,36 +519,52 @@ PairFinder::~PairFinder()
 	hipFree( m->proxies );
 	hipFree( m->queries );
 	hipFree( m->moved );
-	hipFree( m->keys );
-	hipFree( m->sorted );
+	hipFree( m->codes );
+	hipFree( m->sortedCodes );
+	hipFree( m->index );
+	hipFree( m->sortedIndex );
 	hipFree( m->temp );
 	hipFree( m->bounds );
-	hipFree( m->tree.nodeBox );
-	hipFree( m->tree.left );
-	hipFree( m->tree.right );
-	hipFree( m->tree.parent );
-	hipFree( m->tree.flags );
-	hipFree( m->tree.leafBox );
-	hipFree( m->tree.leaves );
+	hipFree( m->tree.nodes );
+	hipFree( m->tree.chunkBox );
+	hipFree( m->tree.rightOf );
+	hipFree( m->tree.rangeLo );
+	hipFree( m->tree.rangeHi );
 	hipFree( m->cands );
+	hipFree( m->sortedCands );
 	hipFree( m->counters );
 	hipHostFree( m->hostCounters );
+	hipHostFree( m->hostProxies );
+	hipHostFree( m->hostQueries );
+	hipHostFree( m->hostMoved );
 	hipHostFree( m->hostCands );
 	hipStreamDestroy( m->stream );
 	delete m;
 }
 
-void PairFinder::Find( const std::vector<LbvhProxy>& proxies, const std::vector<LbvhProxy>& queries,
-					   const std::vector<uint8_t>& moved, std::vector<PairCandidate>* out )
+LbvhProxy* PairFinder::ProxyStaging( int count )
+{
+	return GrowPinned( impl->hostProxies, impl->hostProxyCap, (size_t)count );
+}
+
+LbvhProxy* PairFinder::QueryStaging( int count )
+{
+	return GrowPinned( impl->hostQueries, impl->hostQueryCap, (size_t)count );
+}
+
+uint8_t* PairFinder::MovedStaging( int count )
+{
+	return GrowPinned( impl->hostMoved, impl->hostMovedCap, (size_t)count );
+}
+
+const uint64_t* PairFinder::Find( int n, int nq, int movedCount, int* count )
 {
 	Impl* m = impl;
 	hipStream_t s = m->stream;
-	out->clear();
-	int n = (int)proxies.size();
-	int nq = (int)queries.size();
+	*count = 0;
 	if ( n == 0 || nq == 0 )
 	{
-		return;
+		return nullptr;
 	}
 	m->Reserve( n );
 	if ( (size_t)nq > m->queryCap )
@@ -466,41 +573,46 @@ void PairFinder::Find( const std::vector<LbvhProxy>& proxies, const std::vector<
 		m->queryCap = std::max<size_t>( 256, nq + nq / 2 );
 		HIP_CHECK( hipMalloc( &m->queries, m->queryCap * sizeof( LbvhProxy ) ) );
 	}
-	if ( moved.size() > m->movedCap )
+	if ( (size_t)movedCount > m->movedCap )
 	{
 		hipFree( m->moved );
-		m->movedCap = std::max<size_t>( 1024, moved.size() + moved.size() / 2 );
+		m->movedCap = std::max<size_t>( 1024, movedCount + movedCount / 2 );
 		HIP_CHECK( hipMalloc( &m->moved, m->movedCap ) );
 	}
 	if ( m->candCap == 0 )
 	{
 		m->candCap = std::max<size_t>( 4096, 8 * (size_t)nq );
-		HIP_CHECK( hipMalloc( &m->cands, m->candCap * sizeof( PairCandidate ) ) );
+		HIP_CHECK( hipMalloc( &m->cands, m->candCap * sizeof( uint64_t ) ) );
+		HIP_CHECK( hipMalloc( &m->sortedCands, m->candCap * sizeof( uint64_t ) ) );
 	}
 
-	HIP_CHECK( hipMemcpyAsync( m->proxies, proxies.data(), n * sizeof( LbvhProxy ), hipMemcpyHostToDevice, s ) );
-	HIP_CHECK( hipMemcpyAsync( m->queries, queries.data(), nq * sizeof( LbvhProxy ), hipMemcpyHostToDevice, s ) );
-	HIP_CHECK( hipMemcpyAsync( m->moved, moved.data(), moved.size(), hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( hipMemcpyAsync( m->proxies, m->hostProxies, n * sizeof( LbvhProxy ), hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( hipMemcpyAsync( m->queries, m->hostQueries, nq * sizeof( LbvhProxy ), hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( hipMemcpyAsync( m->moved, m->hostMoved, movedCount, hipMemcpyHostToDevice, s ) );
 
 	LbvhTree t = m->tree;
 	t.n = n;
+	t.codes = m->sortedCodes;
 	LbvhBoundsKernel<<<1, 1024, 0, s>>>( m->proxies, n, m->bounds );
 	int blocks = ( n + kLbvhBlock - 1 ) / kLbvhBlock;
-	LbvhMortonKernel<<<blocks, kLbvhBlock, 0, s>>>( m->proxies, n, m->bounds, m->keys );
+	LbvhMortonKernel<<<blocks, kLbvhBlock, 0, s>>>( m->proxies, n, m->bounds, m->codes, m->index );
 	size_t temp = m->tempBytes;
-	HIP_CHECK( hipcub::DeviceRadixSort::SortKeys( m->temp, temp, m->keys, m->sorted, n, 0, 64, s ) );
-	LbvhLeafKernel<<<blocks, kLbvhBlock, 0, s>>>( m->proxies, m->sorted, t );
+	HIP_CHECK( hipcub::DeviceRadixSort::SortPairs( m->temp, temp, m->codes, m->sortedCodes, m->index, m->sortedIndex, n, 0, 32, s ) );
+	LbvhLeafKernel<<<blocks, kLbvhBlock, 0, s>>>( m->proxies, m->sortedIndex, t );
 	if ( n > 1 )
 	{
-		LbvhBuildKernel<<<blocks, kLbvhBlock, 0, s>>>( m->sorted, t );
-		LbvhRefitKernel<<<blocks, kLbvhBlock, 0, s>>>( t );
+		LbvhBuildKernel<<<blocks, kLbvhBlock, 0, s>>>( t );
 	}
+	int internalBlocks = n > 1 ? ( n - 1 + kLbvhBlock / 64 - 1 ) / ( kLbvhBlock / 64 ) : 0;
+	LbvhFinishKernel<<<blocks + internalBlocks, kLbvhBlock, 0, s>>>( t, blocks );
 	HIP_CHECK( hipGetLastError() );
 
+	int total = 0;
 	for ( ;; )
 	{
 		LbvhQueryArgs qa;
-		qa.t = t;
+		qa.nodes = t.nodes;
+		qa.nodeCount = 2 * n - 1;
 		qa.queries = m->queries;
 		qa.queryCount = nq;
 		qa.moved = m->moved;
@@ -512,33 +624,44 @@ void PairFinder::Find( const std::vector<LbvhProxy>& proxies, const std::vector<
 		HIP_CHECK( hipGetLastError() );
 		HIP_CHECK( hipMemcpyAsync( m->hostCounters, m->counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
 		HIP_CHECK( hipStreamSynchronize( s ) );
-		int count = m->hostCounters[0];
+		total = m->hostCounters[0];
 		if ( m->hostCounters[1] != 0 )
 		{
-			fprintf( stderr, "box2d_amd: LBVH query stack overflow\n" );
+			fprintf( stderr, "box2d_amd: LBVH walk exceeded the node count (malformed tree)\n" );
 			abort();
 		}
-		if ( count > (int)m->candCap )
+		if ( total > (int)m->candCap )
 		{
 			hipFree( m->cands );
-			m->candCap = (size_t)count + count / 2;
-			HIP_CHECK( hipMalloc( &m->cands, m->candCap * sizeof( PairCandidate ) ) );
+			hipFree( m->sortedCands );
+			m->candCap = (size_t)total + total / 2;
+			HIP_CHECK( hipMalloc( &m->cands, m->candCap * sizeof( uint64_t ) ) );
+			HIP_CHECK( hipMalloc( &m->sortedCands, m->candCap * sizeof( uint64_t ) ) );
 			continue;
 		}
-		if ( (size_t)count > m->hostCandCap )
-		{
-			hipHostFree( m->hostCands );
-			m->hostCandCap = (size_t)count + count / 2;
-			HIP_CHECK( hipHostMalloc( &m->hostCands, m->hostCandCap * sizeof( PairCandidate ), hipHostMallocDefault ) );
-		}
-		if ( count > 0 )
-		{
-			HIP_CHECK( hipMemcpyAsync( m->hostCands, m->cands, count * sizeof( PairCandidate ), hipMemcpyDeviceToHost, s ) );
-			HIP_CHECK( hipStreamSynchronize( s ) );
-		}
-		out->assign( m->hostCands, m->hostCands + count );
 		break;
 	}
+	if ( total == 0 )
+	{
+		return nullptr;
+	}
+
+	// (query, proxy index) order: bits 0..31 proxy index, 32.. query index
+	int endBit = 32;
+	while ( endBit < 64 && ( 1ull << ( endBit - 32 ) ) < (uint64_t)nq )
+	{
+		++endBit;
+	}
+	size_t need = 0;
+	HIP_CHECK( hipcub::DeviceRadixSort::SortKeys( nullptr, need, m->cands, m->sortedCands, total, 0, endBit, s ) );
+	m->Temp( need );
+	size_t temp2 = m->tempBytes;
+	HIP_CHECK( hipcub::DeviceRadixSort::SortKeys( m->temp, temp2, m->cands, m->sortedCands, total, 0, endBit, s ) );
+	GrowPinned( m->hostCands, m->hostCandCap, (size_t)total );
+	HIP_CHECK( hipMemcpyAsync( m->hostCands, m->sortedCands, total * sizeof( uint64_t ), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	*count = total;
+	return m->hostCands;
 }
 
 } // namespace b2g

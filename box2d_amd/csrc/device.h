@@ -178,23 +178,23 @@ struct LbvhProxy
 	b2AABB aabb; // the tree's fat box
 };
 
-struct PairCandidate
-{
-	int query; // index into the query list
-	int proxyKey;
-	int shapeId;
-};
-
 class PairFinder
 {
 public:
 	PairFinder();
 	~PairFinder();
 
+	// Pinned host staging the caller fills before Find: every proxy of the world in the order
+	// pairs are to be listed, the moved proxies' queries, and 1 per shape id whose proxy is in the
+	// move buffer.
+	LbvhProxy* ProxyStaging( int count );
+	LbvhProxy* QueryStaging( int count );
+	uint8_t* MovedStaging( int count );
+
 	// Every (query, proxy) whose fat boxes overlap and that passes the pair callback's self, tree
-	// and moved-proxy rules (moved: 1 per shape id whose proxy is in the move buffer). Unordered.
-	void Find( const std::vector<LbvhProxy>& proxies, const std::vector<LbvhProxy>& queries, const std::vector<uint8_t>& moved,
-			   std::vector<PairCandidate>* out );
+	// and moved-proxy rules, as (query index << 32 | proxy index) keys in ascending order. The
+	// array stays valid until the next call.
+	const uint64_t* Find( int proxyCount, int queryCount, int movedCount, int* count );
 
 	struct Impl;
 

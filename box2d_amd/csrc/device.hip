@@ -742,6 +742,43 @@ __global__ void ColorKernel( ColorArgs a )
 	ColorLane<Stage>( a, r, t, GlobalBodies{ a.states } );
 }
 
+// Consecutive small colors of one stage in one workgroup, separated by workgroup barriers instead
+// of kernel boundaries (each color's lanes and arithmetic are those of ColorKernel).
+constexpr int kTailThreads = 1024;
+constexpr int kTailMaxColors = 24;
+
+struct TailArgs
+{
+	ColorArgs color;
+	int colorCount;
+	int jointStart[kTailMaxColors + 1];
+	int contactStart[kTailMaxColors + 1];
+};
+
+template <int Stage>
+__global__ __launch_bounds__( kTailThreads ) void ColorTailKernel( TailArgs a )
+{
+	const GlobalBodies B{ a.color.states };
+	for ( int c = 0; c < a.colorCount; ++c )
+	{
+		ColorRange r;
+		r.jointStart = a.jointStart[c];
+		r.jointCount = a.jointStart[c + 1] - r.jointStart;
+		r.contactStart = a.contactStart[c];
+		r.contactCount = a.contactStart[c + 1] - r.contactStart;
+		int lanes = r.jointCount + r.contactCount;
+		if ( lanes == 0 || ( Stage == kStageRestitution && r.contactCount == 0 ) )
+		{
+			continue;
+		}
+		for ( int t = threadIdx.x; t < lanes; t += blockDim.x )
+		{
+			ColorLane<Stage>( a.color, r, t, B );
+		}
+		__syncthreads();
+	}
+}
+
 template <int Stage>
 __global__ void OverflowKernel( ColorArgs a )
 {
@@ -1751,12 +1788,14 @@ enum KernelClass
 	kkFinalize,
 	kkSolveFused,
 	kkContinuous,
+	kkColorTail,
 	kKernelClassCount
 };
 
 static const char* s_kernelClassNames[kKernelClassCount] = {
 	"collide", "prepare_contacts", "prepare_joints", "integrate_velocities_warm_start", "warm_start", "solve",
 	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies", "solve_fused", "continuous",
+	"color_tail",
 };
 
 constexpr double kBytesCollideFull = 800.0;		// per contact with a new manifold
@@ -2692,7 +2731,28 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 			} );
 		}
 
-		for ( int c = 0; c < L.activeColorCount; ++c )
+		// the trailing colors of at most kTailThreads lanes each go in one single-workgroup launch
+		auto stageLanes = [&]( int c ) {
+			int joints = L.colorJointStart[c + 1] - L.colorJointStart[c];
+			int contacts = L.colorContactStart[c + 1] - L.colorContactStart[c];
+			return stage == kStageRestitution ? ( contacts == 0 ? 0 : joints + contacts ) : joints + contacts;
+		};
+		int tail = L.activeColorCount;
+		while ( tail > 0 && stageLanes( tail - 1 ) <= kTailThreads )
+		{
+			--tail;
+		}
+		int launched = 0;
+		for ( int c = tail; c < L.activeColorCount; ++c )
+		{
+			launched += stageLanes( c ) > 0 ? 1 : 0;
+		}
+		if ( launched < 2 || L.activeColorCount - tail > kTailMaxColors )
+		{
+			tail = L.activeColorCount;
+		}
+
+		for ( int c = 0; c < tail; ++c )
 		{
 			ca.stage = stage;
 			ca.jointStart = L.colorJointStart[c];
@@ -2723,6 +2783,44 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 						break;
 					default:
 						hipExtLaunchKernelGGL( ColorKernel<kStageRestitution>, dim3( ( lanes + cb - 1 ) / cb ), dim3( cb ), 0, s, e0, e1, 0, ca );
+						break;
+				}
+			} );
+		}
+
+		if ( tail < L.activeColorCount )
+		{
+			TailArgs ta;
+			ta.color = ca;
+			ta.color.stage = stage;
+			ta.colorCount = L.activeColorCount - tail;
+			double bytes = 0.0;
+			for ( int k = 0; k <= ta.colorCount; ++k )
+			{
+				ta.jointStart[k] = L.colorJointStart[tail + k];
+				ta.contactStart[k] = L.colorContactStart[tail + k];
+			}
+			for ( int k = 0; k < ta.colorCount; ++k )
+			{
+				if ( stageLanes( tail + k ) > 0 )
+				{
+					bytes += stageBytes( stage ) * ( ta.contactStart[k + 1] - ta.contactStart[k] ) + kBytesJoint * ( ta.jointStart[k + 1] - ta.jointStart[k] );
+				}
+			}
+			impl->Launch( kkColorTail, bytes, [&]( hipEvent_t e0, hipEvent_t e1 ) {
+				switch ( stage )
+				{
+					case kStageWarmStart:
+						hipExtLaunchKernelGGL( ColorTailKernel<kStageWarmStart>, dim3( 1 ), dim3( kTailThreads ), 0, s, e0, e1, 0, ta );
+						break;
+					case kStageSolve:
+						hipExtLaunchKernelGGL( ColorTailKernel<kStageSolve>, dim3( 1 ), dim3( kTailThreads ), 0, s, e0, e1, 0, ta );
+						break;
+					case kStageRelax:
+						hipExtLaunchKernelGGL( ColorTailKernel<kStageRelax>, dim3( 1 ), dim3( kTailThreads ), 0, s, e0, e1, 0, ta );
+						break;
+					default:
+						hipExtLaunchKernelGGL( ColorTailKernel<kStageRestitution>, dim3( 1 ), dim3( kTailThreads ), 0, s, e0, e1, 0, ta );
 						break;
 				}
 			} );

@@ -744,30 +744,4 @@ int DynamicTree::Rebuild( bool fullBuild )
 	return leafCount;
 }
 
-void DynamicTree::LeafOrder( std::vector<int>& out ) const
-{
-	out.clear();
-	if ( root == B2_NULL_INDEX )
-	{
-		return;
-	}
-	std::vector<int> stack;
-	stack.push_back( root );
-	while ( stack.empty() == false )
-	{
-		int nodeId = stack.back();
-		stack.pop_back();
-		const TreeNode& node = nodes[nodeId];
-		if ( node.flags & kNodeLeaf )
-		{
-			out.push_back( nodeId );
-		}
-		else
-		{
-			stack.push_back( node.child1 );
-			stack.push_back( node.child2 );
-		}
-	}
-}
-
 } // namespace b2g

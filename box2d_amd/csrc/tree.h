@@ -114,9 +114,40 @@ public:
 		return root;
 	}
 
-	// Leaf proxy ids in query order (child2 first depth-first traversal). Used by the GPU pair
-	// finder to order pairs exactly like the host query would.
-	void LeafOrder( std::vector<int>& out ) const;
+	// Calls f(proxyId, node) for every leaf in query order (the child2-first depth-first order of
+	// Query). The GPU pair finder numbers proxies this way so its pairs come out in the order the
+	// host query would produce them.
+	template <typename F>
+	void ForEachLeafInQueryOrder( F&& f ) const
+	{
+		if ( root == B2_NULL_INDEX )
+		{
+			return;
+		}
+		int stack[1024];
+		int stackCount = 0;
+		stack[stackCount++] = root;
+		while ( stackCount > 0 )
+		{
+			int nodeId = stack[--stackCount];
+			const TreeNode& node = nodes[nodeId];
+			if ( node.flags & kNodeLeaf )
+			{
+				f( nodeId, node );
+			}
+			else
+			{
+				// at most height + 1 entries are live; the rebuild and rotations keep the height
+				// logarithmic, and a deeper tree is a logic error
+				if ( stackCount + 2 > 1024 )
+				{
+					Fatal( "dynamic tree deeper than the leaf walk supports" );
+				}
+				stack[stackCount++] = node.child1;
+				stack[stackCount++] = node.child2;
+			}
+		}
+	}
 
 private:
 	int AllocateNode();

@@ -2175,9 +2175,10 @@ bool World::AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, in
 	return true;
 }
 
-// New pairs of the moved proxies on the GPU (LBVH over all proxies), then ordered per query the
-// way the reference's stack query visits leaves: kinematic tree, static tree, dynamic tree, each
-// in depth-first child2-first leaf order (src/broad_phase.c:370-397, src/dynamic_tree.c:1085-1135).
+// New pairs of the moved proxies on the GPU (LBVH over all proxies). Proxies are numbered the way
+// the reference's stack query visits leaves: kinematic tree, static tree, dynamic tree, each in
+// depth-first child2-first leaf order (src/broad_phase.c:370-397, src/dynamic_tree.c:1085-1135), so
+// the finder's (query, proxy number) order is the reference's per-query hit order.
 void World::QueryPairsGpu( std::vector<std::vector<MovePair>>& results )
 {
 	BroadPhase& bp = broadPhase;
@@ -2186,24 +2187,37 @@ void World::QueryPairsGpu( std::vector<std::vector<MovePair>>& results )
 		pairFinder = new PairFinder();
 	}
 
-	std::vector<LbvhProxy> proxies;
-	for ( int type = 0; type < b2_bodyTypeCount; ++type )
+	static const b2BodyType kVisitOrder[3] = { b2_kinematicBody, b2_staticBody, b2_dynamicBody };
+	int proxyCount = 0;
+	for ( b2BodyType type : kVisitOrder )
 	{
-		const std::vector<TreeNode>& nodes = bp.trees[type].Nodes();
-		for ( int id = 0; id < (int)nodes.size(); ++id )
-		{
-			const TreeNode& node = nodes[id];
-			if ( ( node.flags & kNodeAllocated ) && ( node.flags & kNodeLeaf ) )
+		proxyCount += bp.trees[type].ProxyCount();
+	}
+	LbvhProxy* proxies = pairFinder->ProxyStaging( proxyCount );
+	int n = 0;
+	for ( b2BodyType type : kVisitOrder )
+	{
+		bp.trees[type].ForEachLeafInQueryOrder( [&]( int id, const TreeNode& node ) {
+			if ( n < proxyCount )
 			{
-				proxies.push_back( LbvhProxy{ B2G_PROXY_KEY( id, type ), (int)node.userData, node.aabb } );
+				proxies[n] = LbvhProxy{ B2G_PROXY_KEY( id, type ), (int)node.userData, node.aabb };
 			}
-		}
+			++n;
+		} );
+	}
+	if ( n != proxyCount )
+	{
+		Fatal( "broadphase trees hold %d leaves but count %d proxies", n, proxyCount );
 	}
 
 	int moveCount = (int)bp.moveArray.size();
-	std::vector<LbvhProxy> queries;
+	LbvhProxy* queries = pairFinder->QueryStaging( std::max( moveCount, 1 ) );
+	int movedCount = std::max<int>( (int)shapes.size(), 1 );
+	uint8_t* moved = pairFinder->MovedStaging( movedCount );
+	memset( moved, 0, movedCount );
 	std::vector<int> queryMove;
-	std::vector<uint8_t> moved( std::max<size_t>( shapes.size(), 1 ), 0 );
+	queryMove.reserve( moveCount );
+	int nq = 0;
 	for ( int i = 0; i < moveCount; ++i )
 	{
 		int key = bp.moveArray[i];
@@ -2213,68 +2227,22 @@ void World::QueryPairsGpu( std::vector<std::vector<MovePair>>& results )
 		}
 		const DynamicTree& tree = bp.trees[B2G_PROXY_TYPE( key )];
 		int shapeId = (int)tree.GetUserData( B2G_PROXY_ID( key ) );
-		queries.push_back( LbvhProxy{ key, shapeId, tree.GetAABB( B2G_PROXY_ID( key ) ) } );
+		queries[nq++] = LbvhProxy{ key, shapeId, tree.GetAABB( B2G_PROXY_ID( key ) ) };
 		queryMove.push_back( i );
 		moved[shapeId] = 1;
 	}
 
-	std::vector<PairCandidate> cands;
-	pairFinder->Find( proxies, queries, moved, &cands );
-
-	// leaf visit rank of every proxy of each tree
-	std::vector<int> rank[b2_bodyTypeCount];
-	for ( int type = 0; type < b2_bodyTypeCount; ++type )
+	int count = 0;
+	const uint64_t* keys = pairFinder->Find( n, nq, movedCount, &count );
+	for ( int c = 0; c < count; ++c )
 	{
-		std::vector<int> order;
-		bp.trees[type].LeafOrder( order );
-		rank[type].assign( bp.trees[type].Nodes().size(), 0 );
-		for ( int k = 0; k < (int)order.size(); ++k )
-		{
-			rank[type][order[k]] = k;
-		}
-	}
-
-	// bucket by query, then sort each bucket into visit order
-	int nq = (int)queries.size();
-	std::vector<int> start( nq + 1, 0 );
-	for ( const PairCandidate& c : cands )
-	{
-		start[c.query + 1] += 1;
-	}
-	for ( int q = 0; q < nq; ++q )
-	{
-		start[q + 1] += start[q];
-	}
-	std::vector<PairCandidate> sorted( cands.size() );
-	{
-		std::vector<int> fill( start.begin(), start.end() - 1 );
-		for ( const PairCandidate& c : cands )
-		{
-			sorted[fill[c.query]++] = c;
-		}
-	}
-	auto treeOrder = []( int type ) { return type == b2_kinematicBody ? 0 : ( type == b2_staticBody ? 1 : 2 ); };
-	for ( int q = 0; q < nq; ++q )
-	{
-		auto begin = sorted.begin() + start[q];
-		auto end = sorted.begin() + start[q + 1];
-		std::sort( begin, end, [&]( const PairCandidate& x, const PairCandidate& y ) {
-			int tx = B2G_PROXY_TYPE( x.proxyKey ), ty = B2G_PROXY_TYPE( y.proxyKey );
-			if ( tx != ty )
-			{
-				return treeOrder( tx ) < treeOrder( ty );
-			}
-			return rank[tx][B2G_PROXY_ID( x.proxyKey )] < rank[ty][B2G_PROXY_ID( y.proxyKey )];
-		} );
-		std::vector<MovePair>& pairList = results[queryMove[q]];
+		int q = (int)( keys[c] >> 32 );
+		const LbvhProxy& hit = proxies[(uint32_t)keys[c]];
 		const LbvhProxy& query = queries[q];
-		for ( auto it = begin; it != end; ++it )
+		MovePair pair;
+		if ( AcceptPair( query.proxyKey, query.shapeId, hit.proxyKey, hit.shapeId, &pair ) )
 		{
-			MovePair pair;
-			if ( AcceptPair( query.proxyKey, query.shapeId, it->proxyKey, it->shapeId, &pair ) )
-			{
-				pairList.push_back( pair );
-			}
+			results[queryMove[q]].push_back( pair );
 		}
 	}
 }
