@@ -7,6 +7,7 @@
 // persistent launch. A step of N worlds costs two host-GPU round trips instead of 2N.
 
 #include "device.h"
+#include "host_threads.h"
 #include "world.h"
 
 #include <algorithm>
@@ -21,107 +22,9 @@
 namespace b2g
 {
 
-// Fixed worker threads running index ranges; the calling thread works too.
-class ThreadPool
-{
-public:
-	explicit ThreadPool( int workerCount )
-	{
-		for ( int i = 0; i < workerCount; ++i )
-		{
-			threads.emplace_back( [this]() { Worker(); } );
-		}
-	}
-
-	~ThreadPool()
-	{
-		{
-			std::lock_guard<std::mutex> lock( mutex );
-			stop = true;
-		}
-		wake.notify_all();
-		for ( std::thread& t : threads )
-		{
-			t.join();
-		}
-	}
-
-	// Run fn(i) for i in [0, count) and return when every call has finished.
-	void For( int count, const std::function<void( int )>& fn )
-	{
-		{
-			std::lock_guard<std::mutex> lock( mutex );
-			job = &fn;
-			jobCount = count;
-			next.store( 0 );
-			pending = (int)threads.size();
-			generation += 1;
-		}
-		wake.notify_all();
-		Drain();
-		std::unique_lock<std::mutex> lock( mutex );
-		finished.wait( lock, [this]() { return pending == 0; } );
-		job = nullptr;
-	}
-
-private:
-	void Drain()
-	{
-		for ( ;; )
-		{
-			int i = next.fetch_add( 1 );
-			if ( i >= jobCount )
-			{
-				return;
-			}
-			( *job )( i );
-		}
-	}
-
-	void Worker()
-	{
-		uint64_t seen = 0;
-		for ( ;; )
-		{
-			{
-				std::unique_lock<std::mutex> lock( mutex );
-				wake.wait( lock, [&]() { return stop || generation != seen; } );
-				if ( stop )
-				{
-					return;
-				}
-				seen = generation;
-			}
-			Drain();
-			{
-				std::lock_guard<std::mutex> lock( mutex );
-				pending -= 1;
-			}
-			finished.notify_one();
-		}
-	}
-
-	std::vector<std::thread> threads;
-	std::mutex mutex;
-	std::condition_variable wake, finished;
-	const std::function<void( int )>* job = nullptr;
-	int jobCount = 0;
-	std::atomic<int> next{ 0 };
-	int pending = 0;
-	uint64_t generation = 0;
-	bool stop = false;
-};
-
 static std::mutex s_batchMutex;
 static ThreadPool* s_pool = nullptr;
 static Batch* s_batch = nullptr;
-
-static int HostThreads()
-{
-	const char* v = getenv( "B2AMD_HOST_THREADS" );
-	int n = v != nullptr ? atoi( v ) : (int)std::thread::hardware_concurrency();
-	return std::max( 1, std::min( n, 16 ) );
-}
 
 int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, BatchStats* stats )
 {
