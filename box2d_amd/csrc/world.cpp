@@ -1703,7 +1703,7 @@ void World::CreateContact( Shape* shapeA, Shape* shapeB )
 		bodyB->contactCount += 1;
 	}
 
-	broadPhase.pairSet.insert( B2G_SHAPE_PAIR_KEY( shapeIdA, shapeIdB ) );
+	broadPhase.pairSet.Insert( B2G_SHAPE_PAIR_KEY( shapeIdA, shapeIdB ) );
 
 	set.contactIds.push_back( contactId );
 
@@ -1727,7 +1727,7 @@ void World::CreateContact( Shape* shapeA, Shape* shapeB )
 
 void World::DestroyContact( Contact* contact )
 {
-	broadPhase.pairSet.erase( B2G_SHAPE_PAIR_KEY( contact->shapeIdA, contact->shapeIdB ) );
+	broadPhase.pairSet.Erase( B2G_SHAPE_PAIR_KEY( contact->shapeIdA, contact->shapeIdB ) );
 
 	ContactEdge* edgeA = contact->edges + 0;
 	ContactEdge* edgeB = contact->edges + 1;
@@ -2132,7 +2132,7 @@ b2JointId World::CreateRevoluteJoint( const b2RevoluteJointDef* def )
 bool World::AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair )
 {
 	uint64_t pairKey = B2G_SHAPE_PAIR_KEY( shapeId, queryShapeIndex );
-	if ( broadPhase.pairSet.count( pairKey ) != 0 )
+	if ( broadPhase.pairSet.Contains( pairKey ) )
 	{
 		return false;
 	}

@@ -186,13 +186,131 @@ struct MovePair
 	int shapeIndexB;
 };
 
+// Set of shape pair keys (never 0: a pair has two distinct shape ids), open addressing with
+// linear probing in a power-of-two table kept at most half full; erase shifts the following
+// run back so probes never meet tombstones. Holds the pairs that have a contact.
+class PairKeySet
+{
+public:
+	bool Contains( uint64_t key ) const
+	{
+		if ( count == 0 )
+		{
+			return false;
+		}
+		for ( size_t i = Slot( key );; i = ( i + 1 ) & mask )
+		{
+			uint64_t k = slots[i];
+			if ( k == key )
+			{
+				return true;
+			}
+			if ( k == 0 )
+			{
+				return false;
+			}
+		}
+	}
+
+	void Insert( uint64_t key )
+	{
+		if ( 2 * ( count + 1 ) > slots.size() )
+		{
+			Grow();
+		}
+		for ( size_t i = Slot( key );; i = ( i + 1 ) & mask )
+		{
+			if ( slots[i] == key )
+			{
+				return;
+			}
+			if ( slots[i] == 0 )
+			{
+				slots[i] = key;
+				count += 1;
+				return;
+			}
+		}
+	}
+
+	void Erase( uint64_t key )
+	{
+		if ( count == 0 )
+		{
+			return;
+		}
+		size_t i = Slot( key );
+		while ( slots[i] != key )
+		{
+			if ( slots[i] == 0 )
+			{
+				return;
+			}
+			i = ( i + 1 ) & mask;
+		}
+		// backward shift: move later members of the run into the hole when their home slot
+		// does not lie cyclically in (hole, position]
+		size_t hole = i;
+		for ( size_t j = ( i + 1 ) & mask; slots[j] != 0; j = ( j + 1 ) & mask )
+		{
+			size_t home = Slot( slots[j] );
+			bool stays = hole <= j ? ( hole < home && home <= j ) : ( hole < home || home <= j );
+			if ( stays == false )
+			{
+				slots[hole] = slots[j];
+				hole = j;
+			}
+		}
+		slots[hole] = 0;
+		count -= 1;
+	}
+
+	size_t Size() const
+	{
+		return count;
+	}
+
+private:
+	size_t Slot( uint64_t key ) const
+	{
+		// 64-bit finalizer of MurmurHash3
+		key ^= key >> 33;
+		key *= 0xff51afd7ed558ccdull;
+		key ^= key >> 33;
+		key *= 0xc4ceb9fe1a85ec53ull;
+		key ^= key >> 33;
+		return (size_t)key & mask;
+	}
+
+	void Grow()
+	{
+		std::vector<uint64_t> old;
+		old.swap( slots );
+		size_t cap = old.empty() ? 1024 : 2 * old.size();
+		slots.assign( cap, 0 );
+		mask = cap - 1;
+		count = 0;
+		for ( uint64_t k : old )
+		{
+			if ( k != 0 )
+			{
+				Insert( k );
+			}
+		}
+	}
+
+	std::vector<uint64_t> slots;
+	size_t mask = 0;
+	size_t count = 0;
+};
+
 class BroadPhase
 {
 public:
 	DynamicTree trees[b2_bodyTypeCount];
 	BitSet movedProxies[b2_bodyTypeCount];
 	std::vector<int> moveArray;
-	std::unordered_set<uint64_t> pairSet;
+	PairKeySet pairSet;
 
 	void BufferMove( int queryProxy )
 	{
