@@ -570,7 +570,7 @@ struct WarmArgs
 
 // Velocity integration and the whole warm-start stage of one substep, one lane per awake body
 // (WarmStartBody applies the body's constraint deltas in the stage's order).
-__global__ void IntegrateWarmStartKernel( WarmArgs a )
+__global__ __launch_bounds__( kBlock ) void IntegrateWarmStartKernel( WarmArgs a )
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if ( i >= a.b.count )
@@ -578,16 +578,16 @@ __global__ void IntegrateWarmStartKernel( WarmArgs a )
 		return;
 	}
 	int id = a.b.awake[i];
-	BodyState* state = a.b.states + id;
-	IntegrateVelocity( a.b.sims[id], state, a.b.p );
 	int begin = a.warmStart[i], end = a.warmStart[i + 1];
-	if ( begin < end )
-	{
-		float vx = state->linearVelocity.x, vy = state->linearVelocity.y, w = state->angularVelocity;
-		WarmStartBody( a.c, a.joints, a.jointList, a.overflowContactStart, a.warmList, begin, end, state->deltaRotation, vx, vy, w );
-		state->linearVelocity = { vx, vy };
-		state->angularVelocity = w;
-	}
+	BodyState* state = a.b.states + id;
+	// integrate and warm start in registers, one store at the end, so every gather can issue
+	// before it
+	BodyState st = *state;
+	IntegrateVelocity( a.b.sims[id], &st, a.b.p );
+	float vx = st.linearVelocity.x, vy = st.linearVelocity.y, w = st.angularVelocity;
+	WarmStartBody( a.c, a.joints, a.jointList, a.overflowContactStart, a.warmList, begin, end, st.deltaRotation, vx, vy, w );
+	state->linearVelocity = { vx, vy };
+	state->angularVelocity = w;
 }
 
 enum ColorStage

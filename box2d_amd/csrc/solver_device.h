@@ -836,12 +836,12 @@ B2G_DEVI void WarmStartJointSide( const JointSim* joint, bool sideB, b2Rot dq, f
 	}
 }
 
+template <int kChunk = 4>
 B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, const int* jointList, int overflowContactStart,
 							 const int* list, int begin, int end, b2Rot dq, float& vx, float& vy, float& w )
 {
-	// Entries go in chunks: all of a chunk's constraint fields are gathered first (independent
-	// loads), then applied in order.
-	constexpr int kChunk = 4;
+	// Entries go in chunks of kChunk: all of a chunk's constraint fields are gathered first
+	// (independent loads), then applied in order.
 	for ( int base = begin; base < end; base += kChunk )
 	{
 		int count = end - base < kChunk ? end - base : kChunk;
