@@ -567,6 +567,66 @@ b2JointId b2CreateRevoluteJoint( b2WorldId worldId, const b2RevoluteJointDef* de
 	return world->CreateRevoluteJoint( def );
 }
 
+b2JointId b2CreateDistanceJoint( b2WorldId worldId, const b2DistanceJointDef* def )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2_nullJointId;
+	}
+	return world->CreateDistanceJoint( def );
+}
+
+b2JointId b2CreateFilterJoint( b2WorldId worldId, const b2FilterJointDef* def )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2_nullJointId;
+	}
+	return world->CreateFilterJoint( def );
+}
+
+b2JointId b2CreateMotorJoint( b2WorldId worldId, const b2MotorJointDef* def )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2_nullJointId;
+	}
+	return world->CreateMotorJoint( def );
+}
+
+b2JointId b2CreatePrismaticJoint( b2WorldId worldId, const b2PrismaticJointDef* def )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2_nullJointId;
+	}
+	return world->CreatePrismaticJoint( def );
+}
+
+b2JointId b2CreateWeldJoint( b2WorldId worldId, const b2WeldJointDef* def )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2_nullJointId;
+	}
+	return world->CreateWeldJoint( def );
+}
+
+b2JointId b2CreateWheelJoint( b2WorldId worldId, const b2WheelJointDef* def )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2_nullJointId;
+	}
+	return world->CreateWheelJoint( def );
+}
+
 bool b2Joint_IsValid( b2JointId id )
 {
 	Joint* joint;
@@ -728,6 +788,12 @@ static_assert( sizeof( b2Polygon ) == 144, "b2Polygon layout" );
 static_assert( sizeof( b2Hull ) == 68, "b2Hull layout" );
 static_assert( sizeof( b2RevoluteJointDef ) == 128, "b2RevoluteJointDef layout" );
 static_assert( offsetof( b2RevoluteJointDef, targetAngle ) == 80, "b2RevoluteJointDef layout" );
+static_assert( sizeof( b2DistanceJointDef ) == 136 && offsetof( b2DistanceJointDef, motorSpeed ) == 124, "b2DistanceJointDef layout" );
+static_assert( sizeof( b2FilterJointDef ) == 88, "b2FilterJointDef layout" );
+static_assert( sizeof( b2MotorJointDef ) == 128 && offsetof( b2MotorJointDef, maxSpringTorque ) == 120, "b2MotorJointDef layout" );
+static_assert( sizeof( b2PrismaticJointDef ) == 128 && offsetof( b2PrismaticJointDef, motorSpeed ) == 116, "b2PrismaticJointDef layout" );
+static_assert( sizeof( b2WeldJointDef ) == 104 && offsetof( b2WeldJointDef, angularDampingRatio ) == 92, "b2WeldJointDef layout" );
+static_assert( sizeof( b2WheelJointDef ) == 120 && offsetof( b2WheelJointDef, motorSpeed ) == 112, "b2WheelJointDef layout" );
 static_assert( sizeof( b2JointDef ) == 80, "b2JointDef layout" );
 static_assert( offsetof( b2JointDef, localFrameA ) == 24, "b2JointDef layout" );
 static_assert( sizeof( b2Profile ) == 92, "b2Profile layout" );

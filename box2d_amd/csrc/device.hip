@@ -506,23 +506,155 @@ B2G_DEV void PrepareJointsBody( const PrepareJointArgs& a, int k )
 	joint->slotA = a.aux[joint->bodyIdA].type == b2_staticBody ? kStaticSlot : a.bodySlot[joint->bodyIdA];
 	joint->slotB = a.aux[joint->bodyIdB].type == b2_staticBody ? kStaticSlot : a.bodySlot[joint->bodyIdB];
 
+	// Type terms (src/revolute_joint.c:217-281, src/weld_joint.c:148-220, src/prismatic_joint.c:295-350,
+	// src/wheel_joint.c:197-282, src/distance_joint.c:250-313, src/motor_joint.c:179-245).
+	int type = joint->type;
+	if ( type == b2_filterJoint )
+	{
+		return;
+	}
+	joint->deltaCenter = b2SubPos( simB.center, simA.center );
+	if ( type == b2_distanceJoint )
+	{
+		joint->anchorA = b2RotateVector( simA.transform.q, b2Sub( joint->localFrameA.p, simA.localCenter ) );
+		joint->anchorB = b2RotateVector( simB.transform.q, b2Sub( joint->localFrameB.p, simB.localCenter ) );
+
+		b2Vec2 rA = joint->anchorA;
+		b2Vec2 rB = joint->anchorB;
+		b2Vec2 separation = b2Add( b2Sub( rB, rA ), joint->deltaCenter );
+		b2Vec2 axis = b2Normalize( separation );
+
+		float crA = b2Cross( rA, axis );
+		float crB = b2Cross( rB, axis );
+		float k = mA + mB + iA * crA * crA + iB * crB * crB;
+		joint->axialMass = k > 0.0f ? 1.0f / k : 0.0f;
+		joint->springSoftness = MakeSoft( joint->hertz, joint->dampingRatio, a.p.h );
+
+		if ( a.p.enableWarmStarting == 0 )
+		{
+			joint->distanceImpulse = 0.0f;
+			joint->lowerImpulse = 0.0f;
+			joint->upperImpulse = 0.0f;
+			joint->motorImpulse = 0.0f;
+		}
+		return;
+	}
+
 	joint->frameA.q = b2MulRot( simA.transform.q, joint->localFrameA.q );
 	joint->frameA.p = b2RotateVector( simA.transform.q, b2Sub( joint->localFrameA.p, simA.localCenter ) );
 	joint->frameB.q = b2MulRot( simB.transform.q, joint->localFrameB.q );
 	joint->frameB.p = b2RotateVector( simB.transform.q, b2Sub( joint->localFrameB.p, simB.localCenter ) );
-	joint->deltaCenter = b2SubPos( simB.center, simA.center );
 
-	float k2 = iA + iB;
-	joint->axialMass = k2 > 0.0f ? 1.0f / k2 : 0.0f;
-	joint->springSoftness = MakeSoft( joint->hertz, joint->dampingRatio, a.p.h );
-
-	if ( a.p.enableWarmStarting == 0 )
+	switch ( type )
 	{
-		joint->linearImpulse = { 0.0f, 0.0f };
-		joint->springImpulse = 0.0f;
-		joint->motorImpulse = 0.0f;
-		joint->lowerImpulse = 0.0f;
-		joint->upperImpulse = 0.0f;
+		case b2_revoluteJoint:
+		{
+			float k2 = iA + iB;
+			joint->axialMass = k2 > 0.0f ? 1.0f / k2 : 0.0f;
+			joint->springSoftness = MakeSoft( joint->hertz, joint->dampingRatio, a.p.h );
+			if ( a.p.enableWarmStarting == 0 )
+			{
+				joint->linearImpulse = { 0.0f, 0.0f };
+				joint->springImpulse = 0.0f;
+				joint->motorImpulse = 0.0f;
+				joint->lowerImpulse = 0.0f;
+				joint->upperImpulse = 0.0f;
+			}
+			break;
+		}
+		case b2_weldJoint:
+		{
+			float ka = iA + iB;
+			joint->axialMass = ka > 0.0f ? 1.0f / ka : 0.0f;
+			joint->linearSpring = joint->linearHertz == 0.0f ? joint->constraintSoftness
+															 : MakeSoft( joint->linearHertz, joint->linearDampingRatio, a.p.h );
+			joint->angularSpring = joint->angularHertz == 0.0f ? joint->constraintSoftness
+															   : MakeSoft( joint->angularHertz, joint->angularDampingRatio, a.p.h );
+			if ( a.p.enableWarmStarting == 0 )
+			{
+				joint->linearImpulse = { 0.0f, 0.0f };
+				joint->angularImpulse = 0.0f;
+			}
+			break;
+		}
+		case b2_prismaticJoint:
+		{
+			joint->springSoftness = MakeSoft( joint->hertz, joint->dampingRatio, a.p.h );
+			if ( a.p.enableWarmStarting == 0 )
+			{
+				joint->pointImpulse = { 0.0f, 0.0f };
+				joint->springImpulse = 0.0f;
+				joint->motorImpulse = 0.0f;
+				joint->lowerImpulse = 0.0f;
+				joint->upperImpulse = 0.0f;
+			}
+			break;
+		}
+		case b2_wheelJoint:
+		{
+			b2Vec2 rA = joint->frameA.p;
+			b2Vec2 rB = joint->frameB.p;
+
+			b2Vec2 d = b2Add( joint->deltaCenter, b2Sub( rB, rA ) );
+			b2Vec2 axisA = b2RotateVector( joint->frameA.q, b2Vec2{ 1.0f, 0.0f } );
+			b2Vec2 perpA = b2LeftPerp( axisA );
+
+			// perpendicular constraint (keeps the wheel on its line)
+			float s1 = b2Cross( b2Add( d, rA ), perpA );
+			float s2 = b2Cross( rB, perpA );
+			float kp = mA + mB + iA * s1 * s1 + iB * s2 * s2;
+			joint->perpMass = kp > 0.0f ? 1.0f / kp : 0.0f;
+
+			// spring
+			float a1 = b2Cross( b2Add( d, rA ), axisA );
+			float a2 = b2Cross( rB, axisA );
+			float ka = mA + mB + iA * a1 * a1 + iB * a2 * a2;
+			joint->axialMass = ka > 0.0f ? 1.0f / ka : 0.0f;
+
+			joint->springSoftness = MakeSoft( joint->hertz, joint->dampingRatio, a.p.h );
+
+			float km = iA + iB;
+			joint->motorMass = km > 0.0f ? 1.0f / km : 0.0f;
+
+			if ( a.p.enableWarmStarting == 0 )
+			{
+				joint->perpImpulse = 0.0f;
+				joint->springImpulse = 0.0f;
+				joint->motorImpulse = 0.0f;
+				joint->lowerImpulse = 0.0f;
+				joint->upperImpulse = 0.0f;
+			}
+			break;
+		}
+		case b2_motorJoint:
+		{
+			b2Vec2 rA = joint->frameA.p;
+			b2Vec2 rB = joint->frameB.p;
+
+			joint->linearSpring = MakeSoft( joint->linearHertz, joint->linearDampingRatio, a.p.h );
+			joint->angularSpring = MakeSoft( joint->angularHertz, joint->angularDampingRatio, a.p.h );
+
+			b2Mat22 kl;
+			kl.cx.x = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
+			kl.cx.y = -rA.y * rA.x * iA - rB.y * rB.x * iB;
+			kl.cy.x = kl.cx.y;
+			kl.cy.y = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
+			joint->linearMass = b2GetInverse22( kl );
+
+			float ka = iA + iB;
+			joint->angularMass = ka > 0.0f ? 1.0f / ka : 0.0f;
+
+			if ( a.p.enableWarmStarting == 0 )
+			{
+				joint->linearVelocityImpulse = { 0.0f, 0.0f };
+				joint->angularVelocityImpulse = 0.0f;
+				joint->linearSpringImpulse = { 0.0f, 0.0f };
+				joint->angularSpringImpulse = 0.0f;
+			}
+			break;
+		}
+		default:
+			break;
 	}
 }
 
@@ -585,7 +717,8 @@ __global__ __launch_bounds__( kBlock ) void IntegrateWarmStartKernel( WarmArgs a
 	BodyState st = *state;
 	IntegrateVelocity( a.b.sims[id], &st, a.b.p );
 	float vx = st.linearVelocity.x, vy = st.linearVelocity.y, w = st.angularVelocity;
-	WarmStartBody( a.c, a.joints, a.jointList, a.overflowContactStart, a.warmList, begin, end, st.deltaRotation, vx, vy, w );
+	WarmStartBody( a.c, a.joints, a.jointList, a.overflowContactStart, a.warmList, begin, end, GlobalBodies{ a.b.states },
+				   st.deltaRotation, st.deltaPosition, vx, vy, w );
 	state->linearVelocity = { vx, vy };
 	state->angularVelocity = w;
 }
@@ -647,15 +780,15 @@ B2G_DEV void ColorLane( const ColorArgs& a, const ColorRange& r, int t, const Bo
 		JointSim* joint = a.joints + a.jointList[r.jointStart + t];
 		if ( Stage == kStageWarmStart )
 		{
-			WarmStartRevolute( joint, B );
+			WarmStartJoint( joint, B );
 		}
 		else if ( Stage == kStageSolve )
 		{
-			SolveRevolute( joint, B, true, a.p.h, a.p.inv_h );
+			SolveJoint( joint, B, true, a.p.h, a.p.inv_h );
 		}
 		else if ( Stage == kStageRelax )
 		{
-			SolveRevolute( joint, B, false, a.p.h, a.p.inv_h );
+			SolveJoint( joint, B, false, a.p.h, a.p.inv_h );
 		}
 		return;
 	}
@@ -699,15 +832,15 @@ B2G_DEV void OverflowSerial( const ColorArgs& a, const ColorRange& r, const Bodi
 		JointSim* joint = a.joints + a.jointList[r.jointStart + t];
 		if ( Stage == kStageWarmStart )
 		{
-			WarmStartRevolute( joint, B );
+			WarmStartJoint( joint, B );
 		}
 		else if ( Stage == kStageSolve )
 		{
-			SolveRevolute( joint, B, true, a.p.h, a.p.inv_h );
+			SolveJoint( joint, B, true, a.p.h, a.p.inv_h );
 		}
 		else if ( Stage == kStageRelax )
 		{
-			SolveRevolute( joint, B, false, a.p.h, a.p.inv_h );
+			SolveJoint( joint, B, false, a.p.h, a.p.inv_h );
 		}
 	}
 
@@ -923,7 +1056,7 @@ B2G_DEV void FusedBody( const FusedArgs& a, char* lds )
 			if ( a.warmByBody )
 			{
 				WarmStartBody( a.color.c, a.color.joints, a.color.jointList, a.overflowContactStart, a.warmList, a.warmStart[i],
-							   a.warmStart[i + 1], st.deltaRotation, vx, vy, w );
+							   a.warmStart[i + 1], B, st.deltaRotation, st.deltaPosition, vx, vy, w );
 			}
 			s_vx[i + 1] = vx;
 			s_vy[i + 1] = vy;

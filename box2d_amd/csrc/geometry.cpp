@@ -291,16 +291,79 @@ b2ChainDef b2DefaultChainDef( void )
 	return def;
 }
 
+// Joint definition defaults (reference src/joint.c:23-113).
+static b2JointDef DefaultJointDef()
+{
+	b2JointDef def = {};
+	def.localFrameA.q = b2Rot{ 1.0f, 0.0f };
+	def.localFrameB.q = b2Rot{ 1.0f, 0.0f };
+	def.forceThreshold = FLT_MAX;
+	def.torqueThreshold = FLT_MAX;
+	def.constraintHertz = 60.0f;
+	def.constraintDampingRatio = 2.0f;
+	def.drawScale = b2GetLengthUnitsPerMeter();
+	return def;
+}
+
 b2RevoluteJointDef b2DefaultRevoluteJointDef( void )
 {
 	b2RevoluteJointDef def = {};
-	def.base.localFrameA.q = b2Rot{ 1.0f, 0.0f };
-	def.base.localFrameB.q = b2Rot{ 1.0f, 0.0f };
-	def.base.forceThreshold = FLT_MAX;
-	def.base.torqueThreshold = FLT_MAX;
-	def.base.constraintHertz = 60.0f;
-	def.base.constraintDampingRatio = 2.0f;
-	def.base.drawScale = b2GetLengthUnitsPerMeter();
+	def.base = DefaultJointDef();
+	def.internalValue = kDefCookie;
+	return def;
+}
+
+b2DistanceJointDef b2DefaultDistanceJointDef( void )
+{
+	b2DistanceJointDef def = {};
+	def.base = DefaultJointDef();
+	def.lowerSpringForce = -FLT_MAX;
+	def.upperSpringForce = FLT_MAX;
+	def.length = 1.0f;
+	def.maxLength = B2_HUGE;
+	def.internalValue = kDefCookie;
+	return def;
+}
+
+b2FilterJointDef b2DefaultFilterJointDef( void )
+{
+	b2FilterJointDef def = {};
+	def.base = DefaultJointDef();
+	def.internalValue = kDefCookie;
+	return def;
+}
+
+b2MotorJointDef b2DefaultMotorJointDef( void )
+{
+	b2MotorJointDef def = {};
+	def.base = DefaultJointDef();
+	def.internalValue = kDefCookie;
+	return def;
+}
+
+b2PrismaticJointDef b2DefaultPrismaticJointDef( void )
+{
+	b2PrismaticJointDef def = {};
+	def.base = DefaultJointDef();
+	def.internalValue = kDefCookie;
+	return def;
+}
+
+b2WeldJointDef b2DefaultWeldJointDef( void )
+{
+	b2WeldJointDef def = {};
+	def.base = DefaultJointDef();
+	def.internalValue = kDefCookie;
+	return def;
+}
+
+b2WheelJointDef b2DefaultWheelJointDef( void )
+{
+	b2WheelJointDef def = {};
+	def.base = DefaultJointDef();
+	def.enableSpring = true;
+	def.hertz = 1.0f;
+	def.dampingRatio = 0.7f;
 	def.internalValue = kDefCookie;
 	return def;
 }

@@ -222,6 +222,49 @@ struct JointSim
 	int enableLimit;
 	int slotA; // LDS slots of the two bodies in the fused solver
 	int slotB;
+
+	// Fields of the other joint types (src/joint.h:64-200). The revolute fields above are shared
+	// where a type has the same quantity: linearImpulse (weld), springImpulse, motorImpulse,
+	// lowerImpulse, upperImpulse, hertz, dampingRatio, motorSpeed, springSoftness (prismatic,
+	// wheel, distance), maxMotorTorque (wheel), axialMass (weld, prismatic, wheel, distance).
+	float lowerTranslation; // prismatic, wheel
+	float upperTranslation;
+	float targetTranslation; // prismatic
+	float maxMotorForce;	 // prismatic, distance
+	b2Vec2 pointImpulse;	 // prismatic: perpendicular and angular impulse
+	float perpImpulse;		 // wheel
+	float perpMass;
+	float motorMass;
+
+	float length; // distance
+	float minLength;
+	float maxLength;
+	float lowerSpringForce;
+	float upperSpringForce;
+	float distanceImpulse;
+	b2Vec2 anchorA;
+	b2Vec2 anchorB;
+
+	float linearHertz; // weld, motor
+	float linearDampingRatio;
+	float angularHertz;
+	float angularDampingRatio;
+	Softness linearSpring;
+	Softness angularSpring;
+	float angularImpulse; // weld
+
+	b2Vec2 linearVelocity; // motor
+	float maxVelocityForce;
+	float angularVelocity;
+	float maxVelocityTorque;
+	float maxSpringForce;
+	float maxSpringTorque;
+	b2Vec2 linearVelocityImpulse;
+	float angularVelocityImpulse;
+	b2Vec2 linearSpringImpulse;
+	float angularSpringImpulse;
+	b2Mat22 linearMass;
+	float angularMass;
 	int pad[3];
 };
 

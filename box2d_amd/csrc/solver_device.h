@@ -5,7 +5,7 @@
 // min-max semantics and the fact that a one-point manifold still runs the second point with zeros
 // (reference src/contact_solver.c:936-1063, 1811-2228). The overflow path reproduces the scalar
 // overflow solver (src/contact_solver.c:24-545), whose expressions differ from the wide ones.
-// Revolute joints follow src/revolute_joint.c:217-500.
+// Joints are in joints_device.h.
 #pragma once
 
 #include "layout.h"
@@ -133,6 +133,14 @@ struct GlobalBodies
 		return r;
 	}
 
+	// Delta position and rotation only (the fields no solver stage writes).
+	B2G_DEVI void LoadDelta( int index, b2Vec2& dp, b2Rot& dq ) const
+	{
+		float4 b = reinterpret_cast<const float4*>( states + index )[1];
+		dp = { b.x, b.y };
+		dq = { b.z, b.w };
+	}
+
 	// Velocities are written back for dynamic bodies only (kinematic and static states are read-only).
 	B2G_DEVI void Store( int index, const LaneBody& b ) const
 	{
@@ -178,6 +186,12 @@ struct LdsBodies
 		r.dqc = dqc[index];
 		r.dqs = dqs[index];
 		return r;
+	}
+
+	B2G_DEVI void LoadDelta( int index, b2Vec2& dp, b2Rot& dq ) const
+	{
+		dp = { dpx[index], dpy[index] };
+		dq = { dqc[index], dqs[index] };
 	}
 
 	B2G_DEVI void Store( int index, const LaneBody& b ) const
@@ -804,6 +818,13 @@ B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, const Bodies& 
 	B.Store( ib, ToLane( sB ) );
 }
 
+} // namespace b2g
+
+#include "joints_device.h"
+
+namespace b2g
+{
+
 // ---------------------------------------------------------------------------------------------
 // Warm start by body
 
@@ -812,33 +833,43 @@ B2G_DEVI void OverflowRestitution( const ConstraintSoA& c, int i, const Bodies& 
 // joints and contacts). The deltas depend on impulses, anchors, normals, masses and the body's
 // delta rotation, never on velocities, so the per-body sequence of float operations is exactly the
 // one the color-by-color stage applies to this body (WarmStartContact, OverflowWarmStart,
-// WarmStartRevolute). Entries: index << 3 | owner << 2 | joint << 1 | side; the owner entry of a
+// WarmStartJoint). Entries: index << 3 | owner << 2 | joint << 1 | side; the owner entry of a
 // contact also accumulates the contact's total normal impulses.
-B2G_DEVI void WarmStartJointSide( const JointSim* joint, bool sideB, b2Rot dq, float& vx, float& vy, float& w )
+template <typename Bodies>
+B2G_DEVI void WarmStartJointSide( const JointSim* joint, bool sideB, const Bodies& B, b2Rot dq, b2Vec2 dp, float& vx, float& vy,
+								  float& w )
 {
-	b2Vec2 L = joint->linearImpulse;
-	float axialImpulse = joint->springImpulse + joint->motorImpulse + joint->lowerImpulse - joint->upperImpulse;
+	b2Vec2 dpOther;
+	b2Rot dqOther;
+	B.LoadDelta( sideB ? Bodies::JointA( joint ) : Bodies::JointB( joint ), dpOther, dqOther );
+	b2Vec2 P;
+	float LA, LB;
+	bool active = sideB ? JointWarmStartTerms( joint, dqOther, dpOther, dq, dp, P, LA, LB )
+						: JointWarmStartTerms( joint, dq, dp, dqOther, dpOther, P, LA, LB );
+	if ( active == false )
+	{
+		return;
+	}
 	if ( sideB == false )
 	{
-		b2Vec2 rA = b2RotateVector( dq, joint->frameA.p );
 		float mA = joint->invMassA, iA = joint->invIA;
-		vx = vx - mA * L.x;
-		vy = vy - mA * L.y;
-		w -= iA * ( b2Cross( rA, L ) + axialImpulse );
+		vx = vx - mA * P.x;
+		vy = vy - mA * P.y;
+		w -= iA * LA;
 	}
 	else
 	{
-		b2Vec2 rB = b2RotateVector( dq, joint->frameB.p );
 		float mB = joint->invMassB, iB = joint->invIB;
-		vx = vx + mB * L.x;
-		vy = vy + mB * L.y;
-		w += iB * ( b2Cross( rB, L ) + axialImpulse );
+		vx = vx + mB * P.x;
+		vy = vy + mB * P.y;
+		w += iB * LB;
 	}
 }
 
-template <int kChunk = 4>
+template <int kChunk = 4, typename Bodies>
 B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, const int* jointList, int overflowContactStart,
-							 const int* list, int begin, int end, b2Rot dq, float& vx, float& vy, float& w )
+							 const int* list, int begin, int end, const Bodies& B, b2Rot dq, b2Vec2 dp, float& vx, float& vy,
+							 float& w )
 {
 	// Entries go in chunks of kChunk: all of a chunk's constraint fields are gathered first
 	// (independent loads), then applied in order.
@@ -893,7 +924,7 @@ B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, con
 			bool sideB = ( entry & 1 ) != 0;
 			if ( entry & 2 )
 			{
-				WarmStartJointSide( joints + jointList[entry >> 3], sideB, dq, vx, vy, w );
+				WarmStartJointSide( joints + jointList[entry >> 3], sideB, B, dq, dp, vx, vy, w );
 				continue;
 			}
 			bool overflow = ( entry >> 3 ) >= overflowContactStart;
@@ -943,203 +974,6 @@ B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, con
 			c.f[kcTotalNormalImpulse1 + o][k] = c.f[kcTotalNormalImpulse1 + o][k] + c.f[kcNormalImpulse1 + o][k];
 		}
 	}
-}
-
-// ---------------------------------------------------------------------------------------------
-// Revolute joint
-
-template <typename Bodies>
-B2G_DEVI void WarmStartRevolute( JointSim* joint, const Bodies& B )
-{
-	float mA = joint->invMassA;
-	float mB = joint->invMassB;
-	float iA = joint->invIA;
-	float iB = joint->invIB;
-	int ia = Bodies::JointA( joint );
-	int ib = Bodies::JointB( joint );
-
-	BodyState sA = ToState( B.Load( ia ) );
-	BodyState* stateA = &sA;
-	BodyState sB = ToState( B.Load( ib ) );
-	BodyState* stateB = &sB;
-
-	b2Vec2 rA = b2RotateVector( stateA->deltaRotation, joint->frameA.p );
-	b2Vec2 rB = b2RotateVector( stateB->deltaRotation, joint->frameB.p );
-
-	float axialImpulse = joint->springImpulse + joint->motorImpulse + joint->lowerImpulse - joint->upperImpulse;
-
-	if ( stateA->flags & kDynamicFlag )
-	{
-		stateA->linearVelocity = b2MulSub( stateA->linearVelocity, mA, joint->linearImpulse );
-		stateA->angularVelocity -= iA * ( b2Cross( rA, joint->linearImpulse ) + axialImpulse );
-	}
-	if ( stateB->flags & kDynamicFlag )
-	{
-		stateB->linearVelocity = b2MulAdd( stateB->linearVelocity, mB, joint->linearImpulse );
-		stateB->angularVelocity += iB * ( b2Cross( rB, joint->linearImpulse ) + axialImpulse );
-	}
-
-	B.Store( ia, ToLane( sA ) );
-	B.Store( ib, ToLane( sB ) );
-}
-
-template <typename Bodies>
-B2G_DEVI void SolveRevolute( JointSim* joint, const Bodies& B, bool useBias, float h, float inv_h )
-{
-	float mA = joint->invMassA;
-	float mB = joint->invMassB;
-	float iA = joint->invIA;
-	float iB = joint->invIB;
-	int ia = Bodies::JointA( joint );
-	int ib = Bodies::JointB( joint );
-
-	BodyState sA = ToState( B.Load( ia ) );
-	BodyState* stateA = &sA;
-	BodyState sB = ToState( B.Load( ib ) );
-	BodyState* stateB = &sB;
-
-	b2Vec2 vA = stateA->linearVelocity;
-	float wA = stateA->angularVelocity;
-	b2Vec2 vB = stateB->linearVelocity;
-	float wB = stateB->angularVelocity;
-
-	b2Rot qA = b2MulRot( stateA->deltaRotation, joint->frameA.q );
-	b2Rot qB = b2MulRot( stateB->deltaRotation, joint->frameB.q );
-	b2Rot relQ = b2InvMulRot( qA, qB );
-
-	bool fixedRotation = ( iA + iB == 0.0f );
-
-	if ( joint->enableSpring && fixedRotation == false )
-	{
-		float jointAngle = Atan2( relQ.s, relQ.c );
-		float jointAngleDelta = b2UnwindAngle( jointAngle - joint->targetAngle );
-		float C = jointAngleDelta;
-		float bias = joint->springSoftness.biasRate * C;
-		float massScale = joint->springSoftness.massScale;
-		float impulseScale = joint->springSoftness.impulseScale;
-		float Cdot = wB - wA;
-		float impulse = -massScale * joint->axialMass * ( Cdot + bias ) - impulseScale * joint->springImpulse;
-		joint->springImpulse += impulse;
-		wA -= iA * impulse;
-		wB += iB * impulse;
-	}
-
-	if ( joint->enableMotor && fixedRotation == false )
-	{
-		float Cdot = wB - wA - joint->motorSpeed;
-		float impulse = -joint->axialMass * Cdot;
-		float oldImpulse = joint->motorImpulse;
-		float maxImpulse = h * joint->maxMotorTorque;
-		joint->motorImpulse = b2ClampFloat( joint->motorImpulse + impulse, -maxImpulse, maxImpulse );
-		impulse = joint->motorImpulse - oldImpulse;
-		wA -= iA * impulse;
-		wB += iB * impulse;
-	}
-
-	if ( joint->enableLimit && fixedRotation == false )
-	{
-		float jointAngle = Atan2( relQ.s, relQ.c );
-
-		{
-			float C = jointAngle - joint->lowerAngle;
-			float bias = 0.0f;
-			float massScale = 1.0f;
-			float impulseScale = 0.0f;
-			if ( C > 0.0f )
-			{
-				bias = C * inv_h;
-			}
-			else if ( useBias )
-			{
-				bias = joint->constraintSoftness.biasRate * C;
-				massScale = joint->constraintSoftness.massScale;
-				impulseScale = joint->constraintSoftness.impulseScale;
-			}
-			float Cdot = wB - wA;
-			float oldImpulse = joint->lowerImpulse;
-			float impulse = -massScale * joint->axialMass * ( Cdot + bias ) - impulseScale * oldImpulse;
-			joint->lowerImpulse = b2MaxFloat( oldImpulse + impulse, 0.0f );
-			impulse = joint->lowerImpulse - oldImpulse;
-			wA -= iA * impulse;
-			wB += iB * impulse;
-		}
-
-		{
-			float C = joint->upperAngle - jointAngle;
-			float bias = 0.0f;
-			float massScale = 1.0f;
-			float impulseScale = 0.0f;
-			if ( C > 0.0f )
-			{
-				bias = C * inv_h;
-			}
-			else if ( useBias )
-			{
-				bias = joint->constraintSoftness.biasRate * C;
-				massScale = joint->constraintSoftness.massScale;
-				impulseScale = joint->constraintSoftness.impulseScale;
-			}
-			float Cdot = wA - wB;
-			float oldImpulse = joint->upperImpulse;
-			float impulse = -massScale * joint->axialMass * ( Cdot + bias ) - impulseScale * oldImpulse;
-			joint->upperImpulse = b2MaxFloat( oldImpulse + impulse, 0.0f );
-			impulse = joint->upperImpulse - oldImpulse;
-			wA += iA * impulse;
-			wB -= iB * impulse;
-		}
-	}
-
-	{
-		b2Vec2 rA = b2RotateVector( stateA->deltaRotation, joint->frameA.p );
-		b2Vec2 rB = b2RotateVector( stateB->deltaRotation, joint->frameB.p );
-
-		b2Vec2 Cdot = b2Sub( b2Add( vB, b2CrossSV( wB, rB ) ), b2Add( vA, b2CrossSV( wA, rA ) ) );
-
-		b2Vec2 bias = { 0.0f, 0.0f };
-		float massScale = 1.0f;
-		float impulseScale = 0.0f;
-		if ( useBias )
-		{
-			b2Vec2 dcA = stateA->deltaPosition;
-			b2Vec2 dcB = stateB->deltaPosition;
-			b2Vec2 separation = b2Add( b2Add( b2Sub( dcB, dcA ), b2Sub( rB, rA ) ), joint->deltaCenter );
-			bias = b2MulSV( joint->constraintSoftness.biasRate, separation );
-			massScale = joint->constraintSoftness.massScale;
-			impulseScale = joint->constraintSoftness.impulseScale;
-		}
-
-		b2Mat22 K;
-		K.cx.x = mA + mB + rA.y * rA.y * iA + rB.y * rB.y * iB;
-		K.cy.x = -rA.y * rA.x * iA - rB.y * rB.x * iB;
-		K.cx.y = K.cy.x;
-		K.cy.y = mA + mB + rA.x * rA.x * iA + rB.x * rB.x * iB;
-		b2Vec2 b = b2Solve22( K, b2Add( Cdot, bias ) );
-
-		b2Vec2 impulse;
-		impulse.x = -massScale * b.x - impulseScale * joint->linearImpulse.x;
-		impulse.y = -massScale * b.y - impulseScale * joint->linearImpulse.y;
-		joint->linearImpulse.x += impulse.x;
-		joint->linearImpulse.y += impulse.y;
-
-		vA = b2MulSub( vA, mA, impulse );
-		wA -= iA * b2Cross( rA, impulse );
-		vB = b2MulAdd( vB, mB, impulse );
-		wB += iB * b2Cross( rB, impulse );
-	}
-
-	if ( stateA->flags & kDynamicFlag )
-	{
-		stateA->linearVelocity = vA;
-		stateA->angularVelocity = wA;
-	}
-	if ( stateB->flags & kDynamicFlag )
-	{
-		stateB->linearVelocity = vB;
-		stateB->angularVelocity = wB;
-	}
-
-	B.Store( ia, ToLane( sA ) );
-	B.Store( ib, ToLane( sB ) );
 }
 
 } // namespace b2g

@@ -400,6 +400,13 @@ World::World( const b2WorldDef* def, int worldIndex, uint16_t gen )
 	userData = def->userData;
 	workerCount = def->workerCount;
 
+	if ( b2GetLengthUnitsPerMeter() != 1.0f )
+	{
+		// the device code's slop and speculative distances are constants for one unit per meter
+		Fatal( "b2SetLengthUnitsPerMeter(%g) is not supported by the GPU step (1 unit per meter only)",
+			   (double)b2GetLengthUnitsPerMeter() );
+	}
+
 	if ( def->frictionCallback != nullptr || def->restitutionCallback != nullptr )
 	{
 		Fatal( "custom friction/restitution callbacks are not supported by the GPU step" );
@@ -1989,19 +1996,22 @@ void World::DestroyContactsBetweenBodies( Body* bodyA, Body* bodyB )
 	}
 }
 
-b2JointId World::CreateRevoluteJoint( const b2RevoluteJointDef* def )
+// Shared part of joint creation: body edges, solver set and graph color, and the base fields of
+// the joint sim (reference src/joint.c:161-391). The caller fills the type fields, then calls
+// FinishJoint.
+JointSim* World::CreateJoint( const b2JointDef* base, b2JointType type, b2JointId* id )
 {
+	*id = b2_nullJointId;
 	if ( locked )
 	{
-		return b2_nullJointId;
+		return nullptr;
 	}
 
-	const b2JointDef* base = &def->base;
 	Body* bodyA = GetBody( base->bodyIdA );
 	Body* bodyB = GetBody( base->bodyIdB );
 	if ( bodyA == nullptr || bodyB == nullptr )
 	{
-		return b2_nullJointId;
+		return nullptr;
 	}
 
 	PullJoints();
@@ -2028,7 +2038,7 @@ b2JointId World::CreateRevoluteJoint( const b2RevoluteJointDef* def )
 	joint->generation = gen + 1;
 	joint->jointId = jointId;
 	joint->userData = base->userData;
-	joint->type = b2_revoluteJoint;
+	joint->type = type;
 	joint->collideConnected = base->collideConnected;
 
 	joint->edges[0].bodyId = bodyIdA;
@@ -2092,7 +2102,7 @@ b2JointId World::CreateRevoluteJoint( const b2RevoluteJointDef* def )
 	js.jointId = jointId;
 	js.bodyIdA = bodyIdA;
 	js.bodyIdB = bodyIdB;
-	js.type = b2_revoluteJoint;
+	js.type = type;
 	js.localFrameA = base->localFrameA;
 	js.localFrameB = base->localFrameB;
 	js.constraintHertz = base->constraintHertz;
@@ -2100,28 +2110,161 @@ b2JointId World::CreateRevoluteJoint( const b2RevoluteJointDef* def )
 	js.constraintSoftness = Softness{ 0.0f, 1.0f, 0.0f };
 	js.forceThreshold = base->forceThreshold;
 	js.torqueThreshold = base->torqueThreshold;
+	*id = b2JointId{ jointId + 1, worldId, joint->generation };
+	return &js;
+}
 
-	js.targetAngle = b2ClampFloat( def->targetAngle, -B2_PI, B2_PI );
-	js.hertz = def->hertz;
-	js.dampingRatio = def->dampingRatio;
-	float lowerAngle = b2MinFloat( def->lowerAngle, def->upperAngle );
-	float upperAngle = b2MaxFloat( def->lowerAngle, def->upperAngle );
-	js.lowerAngle = b2ClampFloat( lowerAngle, -0.99f * B2_PI, 0.99f * B2_PI );
-	js.upperAngle = b2ClampFloat( upperAngle, -0.99f * B2_PI, 0.99f * B2_PI );
-	js.maxMotorTorque = def->maxMotorTorque;
-	js.motorSpeed = def->motorSpeed;
-	js.enableSpring = def->enableSpring;
-	js.enableLimit = def->enableLimit;
-	js.enableMotor = def->enableMotor;
-
+// Island link and dirty flags once the type fields are set.
+void World::FinishJoint( b2JointId id )
+{
+	Joint* joint = &joints[id.index1 - 1];
 	if ( joint->setIndex > kDisabledSet )
 	{
 		LinkJoint( joint );
 	}
-
 	jointsDirty = true;
 	listsDirty = true;
-	return b2JointId{ jointId + 1, worldId, joint->generation };
+}
+
+// Type fields (reference src/joint.c:393-712).
+b2JointId World::CreateRevoluteJoint( const b2RevoluteJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_revoluteJoint, &id );
+	if ( js == nullptr )
+	{
+		return id;
+	}
+	js->targetAngle = b2ClampFloat( def->targetAngle, -B2_PI, B2_PI );
+	js->hertz = def->hertz;
+	js->dampingRatio = def->dampingRatio;
+	float lowerAngle = b2MinFloat( def->lowerAngle, def->upperAngle );
+	float upperAngle = b2MaxFloat( def->lowerAngle, def->upperAngle );
+	js->lowerAngle = b2ClampFloat( lowerAngle, -0.99f * B2_PI, 0.99f * B2_PI );
+	js->upperAngle = b2ClampFloat( upperAngle, -0.99f * B2_PI, 0.99f * B2_PI );
+	js->maxMotorTorque = def->maxMotorTorque;
+	js->motorSpeed = def->motorSpeed;
+	js->enableSpring = def->enableSpring;
+	js->enableLimit = def->enableLimit;
+	js->enableMotor = def->enableMotor;
+	FinishJoint( id );
+	return id;
+}
+
+b2JointId World::CreateDistanceJoint( const b2DistanceJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_distanceJoint, &id );
+	if ( js == nullptr )
+	{
+		return id;
+	}
+	js->length = b2MaxFloat( def->length, B2_LINEAR_SLOP );
+	js->hertz = def->hertz;
+	js->dampingRatio = def->dampingRatio;
+	js->minLength = b2MaxFloat( def->minLength, B2_LINEAR_SLOP );
+	js->maxLength = b2MaxFloat( def->minLength, def->maxLength );
+	js->maxMotorForce = def->maxMotorForce;
+	js->motorSpeed = def->motorSpeed;
+	js->enableSpring = def->enableSpring;
+	js->lowerSpringForce = def->lowerSpringForce;
+	js->upperSpringForce = def->upperSpringForce;
+	js->enableLimit = def->enableLimit;
+	js->enableMotor = def->enableMotor;
+	FinishJoint( id );
+	return id;
+}
+
+b2JointId World::CreateFilterJoint( const b2FilterJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_filterJoint, &id );
+	if ( js != nullptr )
+	{
+		FinishJoint( id );
+	}
+	return id;
+}
+
+b2JointId World::CreateMotorJoint( const b2MotorJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_motorJoint, &id );
+	if ( js == nullptr )
+	{
+		return id;
+	}
+	js->linearVelocity = def->linearVelocity;
+	js->maxVelocityForce = def->maxVelocityForce;
+	js->angularVelocity = def->angularVelocity;
+	js->maxVelocityTorque = def->maxVelocityTorque;
+	js->linearHertz = def->linearHertz;
+	js->linearDampingRatio = def->linearDampingRatio;
+	js->maxSpringForce = def->maxSpringForce;
+	js->angularHertz = def->angularHertz;
+	js->angularDampingRatio = def->angularDampingRatio;
+	js->maxSpringTorque = def->maxSpringTorque;
+	FinishJoint( id );
+	return id;
+}
+
+b2JointId World::CreatePrismaticJoint( const b2PrismaticJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_prismaticJoint, &id );
+	if ( js == nullptr )
+	{
+		return id;
+	}
+	js->hertz = def->hertz;
+	js->dampingRatio = def->dampingRatio;
+	js->targetTranslation = def->targetTranslation;
+	js->lowerTranslation = def->lowerTranslation;
+	js->upperTranslation = def->upperTranslation;
+	js->maxMotorForce = def->maxMotorForce;
+	js->motorSpeed = def->motorSpeed;
+	js->enableSpring = def->enableSpring;
+	js->enableLimit = def->enableLimit;
+	js->enableMotor = def->enableMotor;
+	FinishJoint( id );
+	return id;
+}
+
+b2JointId World::CreateWeldJoint( const b2WeldJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_weldJoint, &id );
+	if ( js == nullptr )
+	{
+		return id;
+	}
+	js->linearHertz = def->linearHertz;
+	js->linearDampingRatio = def->linearDampingRatio;
+	js->angularHertz = def->angularHertz;
+	js->angularDampingRatio = def->angularDampingRatio;
+	FinishJoint( id );
+	return id;
+}
+
+b2JointId World::CreateWheelJoint( const b2WheelJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_wheelJoint, &id );
+	if ( js == nullptr )
+	{
+		return id;
+	}
+	js->lowerTranslation = def->lowerTranslation;
+	js->upperTranslation = def->upperTranslation;
+	js->maxMotorTorque = def->maxMotorTorque;
+	js->motorSpeed = def->motorSpeed;
+	js->hertz = def->hertz;
+	js->dampingRatio = def->dampingRatio;
+	js->enableSpring = def->enableSpring;
+	js->enableLimit = def->enableLimit;
+	js->enableMotor = def->enableMotor;
+	FinishJoint( id );
+	return id;
 }
 
 // ---------------------------------------------------------------------------------------------

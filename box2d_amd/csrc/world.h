@@ -341,6 +341,12 @@ public:
 	int CreateShapeInternal( Body* body, const b2ShapeDef* def, const void* geometry, b2ShapeType type );
 	b2ChainId CreateChain( b2BodyId bodyId, const b2ChainDef* def );
 	b2JointId CreateRevoluteJoint( const b2RevoluteJointDef* def );
+	b2JointId CreateDistanceJoint( const b2DistanceJointDef* def );
+	b2JointId CreateFilterJoint( const b2FilterJointDef* def );
+	b2JointId CreateMotorJoint( const b2MotorJointDef* def );
+	b2JointId CreatePrismaticJoint( const b2PrismaticJointDef* def );
+	b2JointId CreateWeldJoint( const b2WeldJointDef* def );
+	b2JointId CreateWheelJoint( const b2WheelJointDef* def );
 	void Step( float timeStep, int subStepCount );
 
 	// The three phases of Step, split at the two points where the host waits for the GPU, so a
@@ -436,6 +442,9 @@ public:
 
 private:
 	friend class DeviceWorld;
+
+	JointSim* CreateJoint( const b2JointDef* base, b2JointType type, b2JointId* id );
+	void FinishJoint( b2JointId id );
 
 	// bodies / sets / islands (reference body.c, solver_set.c, island.c)
 	void UpdateBodyMassData( Body* body );

@@ -488,6 +488,232 @@ static void b2sBuildPush( b2sScene* scene, int rows, float vx, float vy, float o
 	b2sPushBody( scene, &capacity, pusherId );
 }
 
+static b2BodyId b2sDynamicBox( b2sScene* scene, int* capacity, b2Vec2 position, float angle, float hx, float hy, float friction )
+{
+	b2BodyDef bodyDef = b2DefaultBodyDef();
+	bodyDef.type = b2_dynamicBody;
+	bodyDef.position = ( b2Pos ){ position.x, position.y };
+	bodyDef.rotation = b2MakeRot( angle );
+	b2BodyId bodyId = b2CreateBody( scene->worldId, &bodyDef );
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	shapeDef.material.friction = friction;
+	b2Polygon box = b2MakeBox( hx, hy );
+	b2CreatePolygonShape( bodyId, &shapeDef, &box );
+	b2sPushBody( scene, capacity, bodyId );
+	return bodyId;
+}
+
+static void b2sBuildJointZoo( b2sScene* scene, int enableSleep )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+	int capacity = 0;
+
+	b2BodyId groundId;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Pos ){ 0.0f, -1.0f };
+		groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( 80.0f, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+
+	// weld cantilevers from a static post: rigid, soft angular, soft linear and angular
+	for ( int variant = 0; variant < 3; ++variant )
+	{
+		float x0 = -70.0f + 8.0f * variant;
+		b2BodyDef postDef = b2DefaultBodyDef();
+		postDef.position = ( b2Pos ){ x0, 6.0f };
+		b2BodyId postId = b2CreateBody( worldId, &postDef );
+		b2Polygon postBox = b2MakeBox( 0.25f, 0.25f );
+		b2ShapeDef postShape = b2DefaultShapeDef();
+		b2CreatePolygonShape( postId, &postShape, &postBox );
+
+		b2BodyId prev = postId;
+		for ( int i = 0; i < 6; ++i )
+		{
+			b2BodyId link = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0 + 0.75f + 1.0f * i, 6.0f }, 0.0f, 0.5f, 0.125f, 0.6f );
+			b2WeldJointDef wd = b2DefaultWeldJointDef();
+			wd.base.bodyIdA = prev;
+			wd.base.bodyIdB = link;
+			wd.base.localFrameA.p = ( b2Vec2 ){ i == 0 ? 0.25f : 0.5f, 0.0f };
+			wd.base.localFrameB.p = ( b2Vec2 ){ -0.5f, 0.0f };
+			wd.linearHertz = variant == 2 ? 5.0f : 0.0f;
+			wd.linearDampingRatio = 0.7f;
+			wd.angularHertz = variant >= 1 ? 3.0f : 0.0f;
+			wd.angularDampingRatio = 0.5f;
+			b2CreateWeldJoint( worldId, &wd );
+			prev = link;
+		}
+	}
+
+	// prismatic sliders on the ground: limit only, motor only, spring and limit, all three, tilted
+	for ( int variant = 0; variant < 4; ++variant )
+	{
+		float x0 = -40.0f + 6.0f * variant;
+		b2BodyId slider = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0, 3.0f }, 0.0f, 0.5f, 0.5f, 0.6f );
+		b2PrismaticJointDef pd = b2DefaultPrismaticJointDef();
+		pd.base.bodyIdA = groundId;
+		pd.base.bodyIdB = slider;
+		pd.base.localFrameA.p = ( b2Vec2 ){ x0, 4.0f };
+		pd.base.localFrameA.q = b2MakeRot( 0.25f * B2_PI + 0.2f * variant );
+		pd.base.collideConnected = variant == 3;
+		pd.enableLimit = variant != 1;
+		pd.lowerTranslation = -2.0f;
+		pd.upperTranslation = 1.5f;
+		pd.enableMotor = variant == 1 || variant == 3;
+		pd.motorSpeed = variant == 3 ? -1.0f : 2.0f;
+		pd.maxMotorForce = 40.0f;
+		pd.enableSpring = variant >= 2;
+		pd.hertz = 2.0f;
+		pd.dampingRatio = 0.4f;
+		pd.targetTranslation = 0.5f;
+		b2CreatePrismaticJoint( worldId, &pd );
+
+		// a box dropped on the slider
+		b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0 + 0.2f, 6.0f + variant }, 0.3f, 0.3f, 0.3f, 0.6f );
+	}
+
+	// wheel cars driving on the ground
+	for ( int variant = 0; variant < 3; ++variant )
+	{
+		float x0 = -10.0f + 10.0f * variant;
+		b2BodyId chassis = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0, 1.4f }, 0.0f, 1.5f, 0.3f, 0.6f );
+		for ( int side = 0; side < 2; ++side )
+		{
+			b2BodyDef wheelDef = b2DefaultBodyDef();
+			wheelDef.type = b2_dynamicBody;
+			wheelDef.position = ( b2Pos ){ x0 + ( side == 0 ? -1.0f : 1.0f ), 0.45f };
+			b2BodyId wheel = b2CreateBody( worldId, &wheelDef );
+			b2ShapeDef wheelShape = b2DefaultShapeDef();
+			wheelShape.material.friction = 0.9f;
+			b2Circle circle = { { 0.0f, 0.0f }, 0.4f };
+			b2CreateCircleShape( wheel, &wheelShape, &circle );
+			b2sPushBody( scene, &capacity, wheel );
+
+			b2WheelJointDef jd = b2DefaultWheelJointDef();
+			jd.base.bodyIdA = chassis;
+			jd.base.bodyIdB = wheel;
+			jd.base.localFrameA.p = ( b2Vec2 ){ side == 0 ? -1.0f : 1.0f, -0.95f };
+			jd.base.localFrameA.q = b2MakeRot( 0.5f * B2_PI );
+			jd.enableSpring = variant != 1;
+			jd.hertz = 4.0f;
+			jd.dampingRatio = 0.7f;
+			jd.enableLimit = variant >= 1;
+			jd.lowerTranslation = -0.25f;
+			jd.upperTranslation = 0.25f;
+			jd.enableMotor = side == 0;
+			jd.motorSpeed = variant == 2 ? 4.0f : -3.0f;
+			jd.maxMotorTorque = 15.0f;
+			b2CreateWheelJoint( worldId, &jd );
+		}
+	}
+
+	// distance ropes from static anchors: rigid, spring, spring with limits, spring with motor
+	for ( int variant = 0; variant < 4; ++variant )
+	{
+		float x0 = 25.0f + 6.0f * variant;
+		b2BodyId prev = groundId;
+		b2Vec2 prevAnchor = { x0, 9.0f };
+		for ( int i = 0; i < 5; ++i )
+		{
+			b2BodyId bob = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0 + 0.9f * ( i + 1 ), 9.0f - 0.3f * ( i + 1 ) }, 0.1f * i, 0.2f, 0.2f, 0.6f );
+			b2DistanceJointDef dd = b2DefaultDistanceJointDef();
+			dd.base.bodyIdA = prev;
+			dd.base.bodyIdB = bob;
+			dd.base.localFrameA.p = prevAnchor;
+			dd.base.localFrameB.p = ( b2Vec2 ){ -0.15f, 0.0f };
+			dd.length = 1.0f;
+			dd.enableSpring = variant >= 1;
+			dd.hertz = 3.0f;
+			dd.dampingRatio = 0.3f;
+			dd.enableLimit = variant == 2;
+			dd.minLength = 0.6f;
+			dd.maxLength = 1.4f;
+			dd.enableMotor = variant == 3;
+			dd.maxMotorForce = 10.0f;
+			dd.motorSpeed = 0.5f;
+			if ( variant == 3 )
+			{
+				dd.lowerSpringForce = -20.0f;
+				dd.upperSpringForce = 30.0f;
+			}
+			b2CreateDistanceJoint( worldId, &dd );
+			prev = bob;
+			prevAnchor = ( b2Vec2 ){ 0.15f, 0.0f };
+		}
+	}
+
+	// motor-joint followers relative to the ground: velocity targets, springs, both
+	for ( int variant = 0; variant < 3; ++variant )
+	{
+		float x0 = 55.0f + 5.0f * variant;
+		b2BodyId body = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0, 2.0f + variant }, 0.2f * variant, 0.4f, 0.4f, 0.6f );
+		b2MotorJointDef md = b2DefaultMotorJointDef();
+		md.base.bodyIdA = groundId;
+		md.base.bodyIdB = body;
+		md.base.localFrameA.p = ( b2Vec2 ){ x0, 4.0f };
+		md.base.collideConnected = true;
+		if ( variant != 1 )
+		{
+			md.linearVelocity = ( b2Vec2 ){ 0.5f, 0.2f };
+			md.maxVelocityForce = 20.0f;
+			md.angularVelocity = 1.0f;
+			md.maxVelocityTorque = 5.0f;
+		}
+		if ( variant != 0 )
+		{
+			md.linearHertz = 2.0f;
+			md.linearDampingRatio = 0.5f;
+			md.maxSpringForce = 50.0f;
+			md.angularHertz = 1.0f;
+			md.angularDampingRatio = 0.3f;
+			md.maxSpringTorque = 10.0f;
+		}
+		b2CreateMotorJoint( worldId, &md );
+	}
+
+	// filter joints: overlapping box pairs that fall without colliding with each other
+	for ( int variant = 0; variant < 3; ++variant )
+	{
+		float x0 = 72.0f + 2.0f * variant;
+		b2BodyId a = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0, 2.0f + variant }, 0.0f, 0.5f, 0.5f, 0.6f );
+		b2BodyId b = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0 + 0.3f, 2.2f + variant }, 0.4f, 0.5f, 0.5f, 0.6f );
+		b2FilterJointDef fd = b2DefaultFilterJointDef();
+		fd.base.bodyIdA = a;
+		fd.base.bodyIdB = b;
+		b2CreateFilterJoint( worldId, &fd );
+	}
+
+	// revolute hinges with spring, limit and motor hanging from a static bar
+	{
+		b2BodyId prev = groundId;
+		b2Vec2 anchor = { 0.0f, 12.0f };
+		for ( int i = 0; i < 6; ++i )
+		{
+			b2BodyId link = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ 0.5f + 1.0f * i, 12.0f }, 0.0f, 0.5f, 0.1f, 0.6f );
+			b2RevoluteJointDef rd = b2DefaultRevoluteJointDef();
+			rd.base.bodyIdA = prev;
+			rd.base.bodyIdB = link;
+			rd.base.localFrameA.p = anchor;
+			rd.base.localFrameB.p = ( b2Vec2 ){ -0.5f, 0.0f };
+			rd.enableSpring = ( i & 1 ) == 0;
+			rd.hertz = 2.0f;
+			rd.dampingRatio = 0.5f;
+			rd.enableLimit = i >= 2;
+			rd.lowerAngle = -0.25f * B2_PI;
+			rd.upperAngle = 0.3f * B2_PI;
+			rd.enableMotor = i == 3;
+			rd.motorSpeed = 1.0f;
+			rd.maxMotorTorque = 2.0f;
+			b2CreateRevoluteJoint( worldId, &rd );
+			prev = link;
+			anchor = ( b2Vec2 ){ 0.5f, 0.0f };
+		}
+	}
+}
+
 int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount )
 {
 	int handle = -1;
@@ -565,6 +791,10 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 			b2sBuildPush( scene, rows, vx, vy, omega, friction );
 		}
 		break;
+
+		case B2S_JOINT_ZOO:
+			b2sBuildJointZoo( scene, iparamCount > 0 ? iparams[0] : 0 );
+			break;
 
 		case B2S_RAIN:
 		{

@@ -58,6 +58,14 @@ enum
 	// with initial velocity (fparams[0], fparams[1]), spin fparams[2] and friction fparams[3].
 	// Bodies: the pyramid row by row (top box last), then the pusher.
 	B2S_PUSH = 6,
+
+	// Every joint type on a 80 x 1 m ground box (half extents), with the type's spring, limit
+	// and motor settings varied so each solver branch runs, and bodies that also touch the ground
+	// or each other: weld cantilevers (rigid and soft), prismatic sliders, wheel cars driving on
+	// the ground, distance ropes (rigid, spring, limits, motor), motor-joint followers, filter
+	// joints between overlapping boxes, and revolute hinges with spring, limit and motor.
+	// iparams: [0] enableSleep. Bodies are listed in creation order.
+	B2S_JOINT_ZOO = 7,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.

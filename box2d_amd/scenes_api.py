@@ -19,6 +19,7 @@ SCENE_MIXED_PILE = 3
 SCENE_RAIN = 4
 SCENE_BATCH_WORLD = 5
 SCENE_PUSH = 6
+SCENE_JOINT_ZOO = 7
 
 
 class SceneLib:

@@ -267,6 +267,106 @@ typedef struct b2RevoluteJointDef
 
 B2_API b2RevoluteJointDef b2DefaultRevoluteJointDef( void );
 
+/// Distance joint definition (reference: include/box2d/types.h:641-685)
+typedef struct b2DistanceJointDef
+{
+	b2JointDef base;
+	float length;
+	bool enableSpring;
+	float lowerSpringForce;
+	float upperSpringForce;
+	float hertz;
+	float dampingRatio;
+	bool enableLimit;
+	float minLength;
+	float maxLength;
+	bool enableMotor;
+	float maxMotorForce;
+	float motorSpeed;
+	int internalValue;
+} b2DistanceJointDef;
+
+B2_API b2DistanceJointDef b2DefaultDistanceJointDef( void );
+
+/// Filter joint definition: disables collision between two bodies (reference:
+/// include/box2d/types.h:694-701)
+typedef struct b2FilterJointDef
+{
+	b2JointDef base;
+	int internalValue;
+} b2FilterJointDef;
+
+B2_API b2FilterJointDef b2DefaultFilterJointDef( void );
+
+/// Motor joint definition (reference: include/box2d/types.h:710-747)
+typedef struct b2MotorJointDef
+{
+	b2JointDef base;
+	b2Vec2 linearVelocity;
+	float maxVelocityForce;
+	float angularVelocity;
+	float maxVelocityTorque;
+	float linearHertz;
+	float linearDampingRatio;
+	float maxSpringForce;
+	float angularHertz;
+	float angularDampingRatio;
+	float maxSpringTorque;
+	int internalValue;
+} b2MotorJointDef;
+
+B2_API b2MotorJointDef b2DefaultMotorJointDef( void );
+
+/// Prismatic joint definition (reference: include/box2d/types.h:820-858)
+typedef struct b2PrismaticJointDef
+{
+	b2JointDef base;
+	bool enableSpring;
+	float hertz;
+	float dampingRatio;
+	float targetTranslation;
+	bool enableLimit;
+	float lowerTranslation;
+	float upperTranslation;
+	bool enableMotor;
+	float maxMotorForce;
+	float motorSpeed;
+	int internalValue;
+} b2PrismaticJointDef;
+
+B2_API b2PrismaticJointDef b2DefaultPrismaticJointDef( void );
+
+/// Weld joint definition (reference: include/box2d/types.h:916-935)
+typedef struct b2WeldJointDef
+{
+	b2JointDef base;
+	float linearHertz;
+	float angularHertz;
+	float linearDampingRatio;
+	float angularDampingRatio;
+	int internalValue;
+} b2WeldJointDef;
+
+B2_API b2WeldJointDef b2DefaultWeldJointDef( void );
+
+/// Wheel joint definition (reference: include/box2d/types.h:945-979)
+typedef struct b2WheelJointDef
+{
+	b2JointDef base;
+	bool enableSpring;
+	float hertz;
+	float dampingRatio;
+	bool enableLimit;
+	float lowerTranslation;
+	float upperTranslation;
+	bool enableMotor;
+	float maxMotorTorque;
+	float motorSpeed;
+	int internalValue;
+} b2WheelJointDef;
+
+B2_API b2WheelJointDef b2DefaultWheelJointDef( void );
+
 typedef struct b2ContactBeginTouchEvent
 {
 	b2ShapeId shapeIdA;

@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from box2d_amd.scenes_api import SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES, SCENE_RAIN  # noqa: E402
+from box2d_amd.scenes_api import (SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES,  # noqa: E402
+                                  SCENE_RAIN, SCENE_JOINT_ZOO)
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
 
@@ -82,6 +83,8 @@ def main():
                                   set(range(10, 1001, 10)) | {1, 2, 5}, False))
     fixtures.append(scene_fixture(lib, "falling_hinges_trace", SCENE_FALLING_HINGES, (), (), 300, set(range(1, 301)),
                                   True))
+    fixtures.append(scene_fixture(lib, "joint_zoo", SCENE_JOINT_ZOO, (0,), (), 400, set(range(1, 401)), True))
+    fixtures.append(scene_fixture(lib, "joint_zoo_sleep", SCENE_JOINT_ZOO, (1,), (), 400, set(range(1, 401)), True))
     ok, step, hsh = lib.falling_hinges(1, 500)
     hinges = {"name": "falling_hinges", "slept": ok, "sleep_step": step, "hash": hsh}
     out = {"generator": "tests/golden/gen_golden.py", "reference": "erincatto/box2d v3.2 (src/*.c, -O2 -ffp-contract=off)",

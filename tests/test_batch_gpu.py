@@ -55,3 +55,24 @@ def test_batch_matches_single_worlds(gpu_scenes):
         for h in ref_handles:
             ref.destroy(h)
 
+
+
+def test_batch_of_joint_worlds_matches_reference(gpu_scenes, world_runs):
+    """Worlds holding every joint type, stepped as one batch (persistent fused solver): each
+    world's hash after every step equals the reference's."""
+    import box2d_amd
+
+    fx = next(s for s in world_runs["scenes"] if s["name"] == "joint_zoo")
+    handles = [gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"]) for _ in range(6)]
+    ids = (box2d_amd.WorldId * len(handles))(*[gpu_scenes.world_id(h) for h in handles])
+    try:
+        print(f"[gpu] batch of {len(handles)} joint worlds", flush=True)
+        for step in range(1, fx["steps"] + 1):
+            box2d_amd.step_worlds(ids)
+            want = fx["hashes"][str(step)]
+            for i, h in enumerate(handles):
+                got = gpu_scenes.hash(h)
+                assert got == want, f"joint world {i}: {got:#x} != reference {want:#x} at step {step}"
+    finally:
+        for h in handles:
+            gpu_scenes.destroy(h)

@@ -90,6 +90,16 @@ def test_continuous_scenes_match_reference(gpu_scenes, world_runs, name, path):
     run_and_compare(gpu_scenes, scene(world_runs, name), path=path)
 
 
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("name", ["joint_zoo", "joint_zoo_sleep"])
+def test_joint_types_match_reference(gpu_scenes, world_runs, name, path):
+    """Weld, prismatic, wheel, distance, motor, filter and revolute joints with their springs,
+    limits and motors, among contacts: every step's transform hash equals the reference's, on the
+    color kernels (per-body warm start once the graph is stable) and on the fused solver."""
+    print(f"[gpu] scene {name} path {path}", flush=True)
+    run_and_compare(gpu_scenes, scene(world_runs, name), path=path)
+
+
 @pytest.mark.parametrize("path", [0, 1, 2])
 def test_rain_small_matches_reference(gpu_scenes, world_runs, path):
     """2000 mixed shapes raining into a static U-shaped chain container (one-sided chain segments
