@@ -597,6 +597,26 @@ b2JointId b2CreateMotorJoint( b2WorldId worldId, const b2MotorJointDef* def )
 	return world->CreateMotorJoint( def );
 }
 
+b2JointId b2CreateMoverJoint( b2WorldId worldId, const b2MoverJointDef* def )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2_nullJointId;
+	}
+	return world->CreateMoverJoint( def );
+}
+
+b2JointId b2CreatePogoJoint( b2WorldId worldId, const b2PogoJointDef* def )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2_nullJointId;
+	}
+	return world->CreatePogoJoint( def );
+}
+
 b2JointId b2CreatePrismaticJoint( b2WorldId worldId, const b2PrismaticJointDef* def )
 {
 	World* world = GetWorld( worldId );
@@ -793,6 +813,8 @@ static_assert( sizeof( b2FilterJointDef ) == 88, "b2FilterJointDef layout" );
 static_assert( sizeof( b2MotorJointDef ) == 128 && offsetof( b2MotorJointDef, maxSpringTorque ) == 120, "b2MotorJointDef layout" );
 static_assert( sizeof( b2PrismaticJointDef ) == 128 && offsetof( b2PrismaticJointDef, motorSpeed ) == 116, "b2PrismaticJointDef layout" );
 static_assert( sizeof( b2WeldJointDef ) == 104 && offsetof( b2WeldJointDef, angularDampingRatio ) == 92, "b2WeldJointDef layout" );
+static_assert( sizeof( b2MoverJointDef ) == 104 && offsetof( b2MoverJointDef, maxVelocityForce ) == 88, "b2MoverJointDef layout" );
+static_assert( sizeof( b2PogoJointDef ) == 120 && offsetof( b2PogoJointDef, velocity ) == 112, "b2PogoJointDef layout" );
 static_assert( sizeof( b2WheelJointDef ) == 120 && offsetof( b2WheelJointDef, motorSpeed ) == 112, "b2WheelJointDef layout" );
 static_assert( sizeof( b2JointDef ) == 80, "b2JointDef layout" );
 static_assert( offsetof( b2JointDef, localFrameA ) == 24, "b2JointDef layout" );

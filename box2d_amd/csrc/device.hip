@@ -507,10 +507,24 @@ B2G_DEV void PrepareJointsBody( const PrepareJointArgs& a, int k )
 	joint->slotB = a.aux[joint->bodyIdB].type == b2_staticBody ? kStaticSlot : a.bodySlot[joint->bodyIdB];
 
 	// Type terms (src/revolute_joint.c:217-281, src/weld_joint.c:148-220, src/prismatic_joint.c:295-350,
-	// src/wheel_joint.c:197-282, src/distance_joint.c:250-313, src/motor_joint.c:179-245).
+	// src/wheel_joint.c:197-282, src/distance_joint.c:250-313, src/motor_joint.c:179-245,
+	// src/mover_joint.c:47-84, src/pogo_joint.c:98-151).
 	int type = joint->type;
 	if ( type == b2_filterJoint )
 	{
+		return;
+	}
+	if ( type == b2_moverJoint )
+	{
+		// linear only
+		joint->invIA = 0.0f;
+		joint->invIB = 0.0f;
+		float k = mA + mB;
+		joint->pointMass = k > 0.0f ? 1.0f / k : 0.0f;
+		if ( a.p.enableWarmStarting == 0 )
+		{
+			joint->linearVelocityImpulse = { 0.0f, 0.0f };
+		}
 		return;
 	}
 	joint->deltaCenter = b2SubPos( simB.center, simA.center );
@@ -623,6 +637,24 @@ B2G_DEV void PrepareJointsBody( const PrepareJointArgs& a, int k )
 				joint->motorImpulse = 0.0f;
 				joint->lowerImpulse = 0.0f;
 				joint->upperImpulse = 0.0f;
+			}
+			break;
+		}
+		case b2_pogoJoint:
+		{
+			b2Vec2 rA = joint->frameA.p;
+			b2Vec2 rB = joint->frameB.p;
+			float crA = b2Cross( rA, joint->normal );
+			float crB = b2Cross( rB, joint->normal );
+			float k = mA + mB + iA * crA * crA + iB * crB * crB;
+			joint->pointMass = k > 0.0f ? 1.0f / k : 0.0f;
+
+			// The reference zeroes a spring-less pogo's impulse in its warm start; nothing reads
+			// the impulse in between, so it is zeroed here, where the per-body warm start (which
+			// only reads joints) does not need to write it.
+			if ( a.p.enableWarmStarting == 0 || joint->hertz == 0.0f )
+			{
+				joint->pogoImpulse = 0.0f;
 			}
 			break;
 		}

@@ -341,6 +341,22 @@ b2MotorJointDef b2DefaultMotorJointDef( void )
 	return def;
 }
 
+b2MoverJointDef b2DefaultMoverJointDef( void )
+{
+	b2MoverJointDef def = {};
+	def.base = DefaultJointDef();
+	def.internalValue = kDefCookie;
+	return def;
+}
+
+b2PogoJointDef b2DefaultPogoJointDef( void )
+{
+	b2PogoJointDef def = {};
+	def.base = DefaultJointDef();
+	def.internalValue = kDefCookie;
+	return def;
+}
+
 b2PrismaticJointDef b2DefaultPrismaticJointDef( void )
 {
 	b2PrismaticJointDef def = {};

@@ -1,6 +1,5 @@
-// GPU joint solver: preparation terms, warm start and solve of every joint type the step supports
-// (revolute, weld, prismatic, wheel, distance, motor; filter joints only take part in collision
-// filtering and islands). Each function is one lane's work on one joint and keeps the reference's
+// GPU joint solver: warm start and solve of every joint type (revolute, weld, prismatic, wheel,
+// distance, motor, mover, pogo; filter joints only take part in collision filtering and islands). Each function is one lane's work on one joint and keeps the reference's
 // operation order, so its floats are the reference's:
 //   revolute  src/revolute_joint.c:217-500
 //   weld      src/weld_joint.c:148-452 (the non-block solve, B2_WELD_BLOCK_SOLVE 0)
@@ -8,6 +7,8 @@
 //   wheel     src/wheel_joint.c:197-520
 //   distance  src/distance_joint.c:250-541
 //   motor     src/motor_joint.c:179-430
+//   mover     src/mover_joint.c:47-170
+//   pogo      src/pogo_joint.c:98-280
 // Bodies is a body view of solver_device.h (device memory by body id, or LDS by slot).
 #pragma once
 
@@ -20,7 +21,7 @@ constexpr float kLengthUnitsPerMeter = 1.0f;
 // Warm-start impulses of one joint: body A receives v -= mA * P, w -= iA * LA and body B
 // v += mB * P, w += iB * LB. They depend on the accumulated impulses, the joint frames and the two
 // bodies' delta positions and rotations, never on velocities. Returns false for joints with no
-// warm start (filter).
+// warm start (filter, and pogo joints without a spring).
 B2G_DEVI bool JointWarmStartTerms( const JointSim* j, b2Rot dqA, b2Vec2 dpA, b2Rot dqB, b2Vec2 dpB, b2Vec2& P, float& LA,
 								   float& LB )
 {
@@ -110,6 +111,28 @@ B2G_DEVI bool JointWarmStartTerms( const JointSim* j, b2Rot dqA, b2Vec2 dpA, b2R
 
 			float axialImpulse = j->distanceImpulse + j->lowerImpulse - j->upperImpulse + j->motorImpulse;
 			P = b2MulSV( axialImpulse, axis );
+			LA = b2Cross( rA, P );
+			LB = b2Cross( rB, P );
+			return true;
+		}
+		case b2_moverJoint:
+		{
+			// linear only; the mover's preparation zeroes its inverse inertias
+			P = j->linearVelocityImpulse;
+			LA = 0.0f;
+			LB = 0.0f;
+			return true;
+		}
+		case b2_pogoJoint:
+		{
+			// a pogo without spring has no warm start (its impulse is zeroed when prepared)
+			if ( j->hertz == 0.0f )
+			{
+				return false;
+			}
+			b2Vec2 rA = b2RotateVector( dqA, j->frameA.p );
+			b2Vec2 rB = b2RotateVector( dqB, j->frameB.p );
+			P = b2MulSV( j->pogoImpulse, j->normal );
 			LA = b2Cross( rA, P );
 			LB = b2Cross( rB, P );
 			return true;
@@ -1096,6 +1119,131 @@ B2G_DEVI void SolveMotor( JointSim* joint, const Bodies& B, float h )
 }
 
 // ---------------------------------------------------------------------------------------------
+// Mover
+
+template <typename Bodies>
+B2G_DEVI void SolveMover( JointSim* joint, const Bodies& B, float h )
+{
+	float mA = joint->invMassA;
+	float mB = joint->invMassB;
+	int ia = Bodies::JointA( joint );
+	int ib = Bodies::JointB( joint );
+	BodyState sA = ToState( B.Load( ia ) );
+	BodyState sB = ToState( B.Load( ib ) );
+
+	b2Vec2 vA = sA.linearVelocity;
+	b2Vec2 vB = sB.linearVelocity;
+
+	if ( joint->maxVelocityForce2.x > 0.0f || joint->maxVelocityForce2.y > 0.0f )
+	{
+		b2Vec2 cdot = b2Sub( vB, vA );
+		cdot = b2Sub( cdot, joint->linearVelocity );
+		b2Vec2 b = b2MulSV( joint->pointMass, cdot );
+		b2Vec2 impulse = { -b.x, -b.y };
+
+		b2Vec2 oldImpulse = joint->linearVelocityImpulse;
+		joint->linearVelocityImpulse = b2Add( joint->linearVelocityImpulse, impulse );
+		b2Vec2 maxImpulse = b2MulSV( h, joint->maxVelocityForce2 );
+
+		joint->linearVelocityImpulse = b2Clamp( joint->linearVelocityImpulse, b2Neg( maxImpulse ), maxImpulse );
+
+		impulse = b2Sub( joint->linearVelocityImpulse, oldImpulse );
+		vA = b2MulSub( vA, mA, impulse );
+		vB = b2MulAdd( vB, mB, impulse );
+	}
+	else
+	{
+		joint->linearVelocityImpulse = { 0.0f, 0.0f };
+	}
+
+	if ( sA.flags & kDynamicFlag )
+	{
+		sA.linearVelocity = vA;
+	}
+	if ( sB.flags & kDynamicFlag )
+	{
+		sB.linearVelocity = vB;
+	}
+	B.Store( ia, ToLane( sA ) );
+	B.Store( ib, ToLane( sB ) );
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pogo
+
+template <typename Bodies>
+B2G_DEVI void SolvePogo( JointSim* joint, const Bodies& B, bool useBias, float h )
+{
+	if ( joint->hertz == 0.0f )
+	{
+		joint->pogoImpulse = 0.0f;
+		return;
+	}
+
+	float mA = joint->invMassA;
+	float mB = joint->invMassB;
+	float iA = joint->invIA;
+	float iB = joint->invIB;
+	int ia = Bodies::JointA( joint );
+	int ib = Bodies::JointB( joint );
+	BodyState sA = ToState( B.Load( ia ) );
+	BodyState sB = ToState( B.Load( ib ) );
+
+	b2Vec2 vA = sA.linearVelocity;
+	float wA = sA.angularVelocity;
+	b2Vec2 vB = sB.linearVelocity;
+	float wB = sB.angularVelocity;
+
+	b2Vec2 rA = b2RotateVector( sA.deltaRotation, joint->frameA.p );
+	b2Vec2 rB = b2RotateVector( sB.deltaRotation, joint->frameB.p );
+
+	float bias = 0.0f;
+	if ( useBias )
+	{
+		// position correction through a spring-damper on the pogo axis (body B's local y axis),
+		// so no velocity is fed into the driven body
+		b2Vec2 dcA = sA.deltaPosition;
+		b2Vec2 dcB = sB.deltaPosition;
+		b2Vec2 d = b2Add( b2Add( b2Sub( dcB, dcA ), b2Sub( rB, rA ) ), joint->deltaCenter );
+
+		b2Vec2 pogoAxis = b2RotateVector( joint->frameB.q, b2Vec2{ 0.0f, 1.0f } );
+		float c = b2Dot( pogoAxis, d ) - joint->restLength;
+
+		joint->pogoVelocity = b2SpringDamper( joint->hertz, joint->dampingRatio, c, joint->pogoVelocity, h );
+		bias = -joint->pogoVelocity;
+	}
+
+	b2Vec2 vr = b2Sub( b2Add( vB, b2CrossSV( wB, rB ) ), b2Add( vA, b2CrossSV( wA, rA ) ) );
+	float cdot = b2Dot( joint->normal, vr );
+
+	float maxTensionImpulse = h * joint->maxTensionForce;
+	float maxCompressionImpulse = h * joint->maxCompressionForce;
+	float oldImpulse = joint->pogoImpulse;
+	float impulse = -joint->pointMass * ( cdot + bias );
+	joint->pogoImpulse = b2ClampFloat( joint->pogoImpulse + impulse, -maxTensionImpulse, maxCompressionImpulse );
+	impulse = joint->pogoImpulse - oldImpulse;
+
+	b2Vec2 P = b2MulSV( impulse, joint->normal );
+	vA = b2MulSub( vA, mA, P );
+	wA -= iA * b2Cross( rA, P );
+	vB = b2MulAdd( vB, mB, P );
+	wB += iB * b2Cross( rB, P );
+
+	if ( sA.flags & kDynamicFlag )
+	{
+		sA.linearVelocity = vA;
+		sA.angularVelocity = wA;
+	}
+	if ( sB.flags & kDynamicFlag )
+	{
+		sB.linearVelocity = vB;
+		sB.angularVelocity = wB;
+	}
+	B.Store( ia, ToLane( sA ) );
+	B.Store( ib, ToLane( sB ) );
+}
+
+// ---------------------------------------------------------------------------------------------
 // Dispatch (src/joint.c:1406-1532)
 
 template <typename Bodies>
@@ -1120,6 +1268,12 @@ B2G_DEVI void SolveJoint( JointSim* joint, const Bodies& B, bool useBias, float 
 			break;
 		case b2_motorJoint:
 			SolveMotor( joint, B, h );
+			break;
+		case b2_moverJoint:
+			SolveMover( joint, B, h );
+			break;
+		case b2_pogoJoint:
+			SolvePogo( joint, B, useBias, h );
 			break;
 		default:
 			break;

@@ -265,7 +265,17 @@ struct JointSim
 	float angularSpringImpulse;
 	b2Mat22 linearMass;
 	float angularMass;
-	int pad[3];
+
+	b2Vec2 maxVelocityForce2; // mover: per-axis force limits
+	float pointMass;		  // mover and pogo: scalar linear effective mass
+
+	b2Vec2 normal; // pogo
+	float restLength;
+	float maxTensionForce;
+	float maxCompressionForce;
+	float pogoImpulse;
+	float pogoVelocity;
+	int pad[2];
 };
 
 // Contact constraint fields, one device array per field (structure of arrays).

@@ -2208,6 +2208,40 @@ b2JointId World::CreateMotorJoint( const b2MotorJointDef* def )
 	return id;
 }
 
+b2JointId World::CreateMoverJoint( const b2MoverJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_moverJoint, &id );
+	if ( js == nullptr )
+	{
+		return id;
+	}
+	js->linearVelocity = def->linearVelocity;
+	js->maxVelocityForce2 = def->maxVelocityForce;
+	FinishJoint( id );
+	return id;
+}
+
+b2JointId World::CreatePogoJoint( const b2PogoJointDef* def )
+{
+	b2JointId id;
+	JointSim* js = CreateJoint( &def->base, b2_pogoJoint, &id );
+	if ( js == nullptr )
+	{
+		return id;
+	}
+	js->restLength = def->restLength;
+	js->hertz = def->hertz;
+	js->dampingRatio = def->dampingRatio;
+	js->maxTensionForce = def->maxTensionForce;
+	js->maxCompressionForce = def->maxCompressionForce;
+	js->normal = def->normal;
+	js->pogoImpulse = def->impulse;
+	js->pogoVelocity = def->velocity;
+	FinishJoint( id );
+	return id;
+}
+
 b2JointId World::CreatePrismaticJoint( const b2PrismaticJointDef* def )
 {
 	b2JointId id;

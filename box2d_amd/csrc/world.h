@@ -344,6 +344,8 @@ public:
 	b2JointId CreateDistanceJoint( const b2DistanceJointDef* def );
 	b2JointId CreateFilterJoint( const b2FilterJointDef* def );
 	b2JointId CreateMotorJoint( const b2MotorJointDef* def );
+	b2JointId CreateMoverJoint( const b2MoverJointDef* def );
+	b2JointId CreatePogoJoint( const b2PogoJointDef* def );
 	b2JointId CreatePrismaticJoint( const b2PrismaticJointDef* def );
 	b2JointId CreateWeldJoint( const b2WeldJointDef* def );
 	b2JointId CreateWheelJoint( const b2WheelJointDef* def );

@@ -686,6 +686,42 @@ static void b2sBuildJointZoo( b2sScene* scene, int enableSleep )
 		b2CreateFilterJoint( worldId, &fd );
 	}
 
+	// mover joints relative to the ground: driven, and with zero force limits (impulse cleared)
+	for ( int variant = 0; variant < 2; ++variant )
+	{
+		float x0 = -78.0f + 2.5f * variant;
+		b2BodyId body = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0, 1.0f + variant }, 0.3f, 0.4f, 0.4f, 0.6f );
+		b2MoverJointDef md = b2DefaultMoverJointDef();
+		md.base.bodyIdA = groundId;
+		md.base.bodyIdB = body;
+		md.base.collideConnected = true;
+		md.linearVelocity = ( b2Vec2 ){ 0.8f, 0.6f };
+		md.maxVelocityForce = variant == 0 ? ( b2Vec2 ){ 30.0f, 15.0f } : ( b2Vec2 ){ 0.0f, 0.0f };
+		b2CreateMoverJoint( worldId, &md );
+	}
+
+	// pogo joints holding boxes above the ground: sprung with tension and compression limits,
+	// and spring-less with a nonzero initial impulse (cleared by the solver)
+	for ( int variant = 0; variant < 2; ++variant )
+	{
+		float x0 = -73.0f + 2.5f * variant;
+		b2BodyId body = b2sDynamicBox( scene, &capacity, ( b2Vec2 ){ x0, 2.0f }, 0.0f, 0.3f, 0.5f, 0.6f );
+		b2PogoJointDef pd = b2DefaultPogoJointDef();
+		pd.base.bodyIdA = groundId;
+		pd.base.bodyIdB = body;
+		pd.base.localFrameA.p = ( b2Vec2 ){ x0, 0.0f };
+		pd.base.collideConnected = true;
+		pd.normal = ( b2Vec2 ){ 0.0f, 1.0f };
+		pd.restLength = 1.5f;
+		pd.hertz = variant == 0 ? 4.0f : 0.0f;
+		pd.dampingRatio = 0.6f;
+		pd.maxTensionForce = 5.0f;
+		pd.maxCompressionForce = 60.0f;
+		pd.impulse = variant == 1 ? 0.25f : 0.0f;
+		pd.velocity = 0.1f;
+		b2CreatePogoJoint( worldId, &pd );
+	}
+
 	// revolute hinges with spring, limit and motor hanging from a static bar
 	{
 		b2BodyId prev = groundId;

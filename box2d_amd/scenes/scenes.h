@@ -63,7 +63,8 @@ enum
 	// and motor settings varied so each solver branch runs, and bodies that also touch the ground
 	// or each other: weld cantilevers (rigid and soft), prismatic sliders, wheel cars driving on
 	// the ground, distance ropes (rigid, spring, limits, motor), motor-joint followers, filter
-	// joints between overlapping boxes, and revolute hinges with spring, limit and motor.
+	// joints between overlapping boxes, mover and pogo joints, and revolute hinges with spring,
+	// limit and motor.
 	// iparams: [0] enableSleep. Bodies are listed in creation order.
 	B2S_JOINT_ZOO = 7,
 };

@@ -317,6 +317,36 @@ typedef struct b2MotorJointDef
 
 B2_API b2MotorJointDef b2DefaultMotorJointDef( void );
 
+/// Mover joint definition: drives body B's velocity relative to body A with per-axis force
+/// limits (reference: include/box2d/types.h:756-769)
+typedef struct b2MoverJointDef
+{
+	b2JointDef base;
+	b2Vec2 linearVelocity;
+	b2Vec2 maxVelocityForce;
+	int internalValue;
+} b2MoverJointDef;
+
+B2_API b2MoverJointDef b2DefaultMoverJointDef( void );
+
+/// Pogo joint definition: a one-sided spring along a normal (reference:
+/// include/box2d/types.h:777-810)
+typedef struct b2PogoJointDef
+{
+	b2JointDef base;
+	b2Vec2 normal;
+	float hertz;
+	float dampingRatio;
+	float restLength;
+	float maxTensionForce;
+	float maxCompressionForce;
+	float impulse;
+	float velocity;
+	int internalValue;
+} b2PogoJointDef;
+
+B2_API b2PogoJointDef b2DefaultPogoJointDef( void );
+
 /// Prismatic joint definition (reference: include/box2d/types.h:820-858)
 typedef struct b2PrismaticJointDef
 {
