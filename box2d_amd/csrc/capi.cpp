@@ -154,9 +154,12 @@ b2BodyEvents b2World_GetBodyEvents( b2WorldId worldId )
 
 b2ContactEvents b2World_GetContactEvents( b2WorldId worldId )
 {
-	(void)worldId;
-	b2ContactEvents events = {};
-	return events;
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return b2ContactEvents{};
+	}
+	return world->GetContactEvents();
 }
 
 void b2World_EnableSleeping( b2WorldId worldId, bool flag )

@@ -49,8 +49,20 @@ struct CollideResult
 };
 
 // Results the host needs after body finalization.
+// A contact hit event found by the device (contact and shape ids, world point, normal, speed).
+struct HitRecord
+{
+	int contactId;
+	int shapeIdA;
+	int shapeIdB;
+	b2Vec2 point;
+	b2Vec2 normal;
+	float approachSpeed;
+};
+
 struct FinalizeResult
 {
+	std::vector<HitRecord> hits; // ascending contact id
 	std::vector<int> enlargedBodies; // awake indices of bodies with an enlarged shape, ascending
 	std::vector<int> enlargedShapes; // shape ids whose fat AABB grew
 	std::vector<b2AABB> enlargedShapeAABBs;

@@ -1169,6 +1169,9 @@ struct StoreArgs
 	ContactSim* contacts;
 	int coloredCount;
 	int totalCount;
+	int* hitCandidates; // contact ids of colored contacts flagged for a hit event
+	int* hitCount;
+	float negHitThreshold; // -hitEventThreshold
 };
 
 // Impulses back to the manifolds (reference src/contact_solver.c:2238-2331 for colored
@@ -1193,6 +1196,21 @@ B2G_DEV void StoreImpulsesBody( const StoreArgs& a, int i )
 		m->points[j].normalVelocity = c.f[kcRelativeVelocity1 + o][i];
 	}
 	m->rollingImpulse = c.f[kcRollingImpulse][i];
+
+	// hit-event candidates, colored contacts only (reference src/contact_solver.c:2305-2319); the
+	// point total impulse rules out speculative points that did not collide
+	if ( i < a.coloredCount && ( a.contacts[c.contactId[i]].simFlags & kSimEnableHitEvent ) != 0 )
+	{
+		for ( int k = 0; k < m->pointCount; ++k )
+		{
+			const b2ManifoldPoint* mp = m->points + k;
+			if ( mp->normalVelocity < a.negHitThreshold && mp->totalNormalImpulse > 0.0f )
+			{
+				a.hitCandidates[atomicAdd( a.hitCount, 1 )] = c.contactId[i];
+				break;
+			}
+		}
+	}
 }
 
 // Fat shape AABB: float world points, double extremes and inflation, one narrowing cast
@@ -2026,8 +2044,20 @@ __global__ __launch_bounds__( kPackThreads ) void PackCollideBatchKernel( const 
 	PackCollideBody( args[blockIdx.x] );
 }
 
+// One hit event in the readback mailbox: contact id, shape ids, world point, normal and approach
+// speed (floats as bits).
+constexpr int kHitRecordInts = 8;
+
 struct PackFinalizeArgs
 {
+	// hit events, built after finalize and continuous collision moved the bodies
+	const ContactSim* contacts;
+	const BodySim* sims;
+	const BodyAux* aux;
+	const int* hitCandidates;
+	int* hitCount;
+	float hitThreshold;
+	int hitOffset; // mailbox offset in ints
 	int* counters; // enlarged bodies, enlarged shapes, fast bodies, continuous-collision error
 	const int* enlargedBodies;
 	const int* enlargedShapes;
@@ -2068,9 +2098,54 @@ B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
 			dst[i] = a.sleepTimes[i];
 		}
 	}
+
+	// Hit events (reference src/solver.c:1703-1820): the fastest approaching point that carried
+	// impulse; the world point follows a static body's anchor when the other body is not static.
+	int hits = *a.hitCount;
+	for ( int h = threadIdx.x; h < hits; h += blockDim.x )
+	{
+		int contactId = a.hitCandidates[h];
+		const ContactSim& cs = a.contacts[contactId];
+		float approach = a.hitThreshold;
+		int best = -1;
+		for ( int p = 0; p < cs.manifold.pointCount; ++p )
+		{
+			const b2ManifoldPoint* mp = cs.manifold.points + p;
+			float approachSpeed = -mp->normalVelocity;
+			if ( approachSpeed > approach && mp->totalNormalImpulse > 0.0f )
+			{
+				approach = approachSpeed;
+				best = p;
+			}
+		}
+		int* rec = a.mail + a.hitOffset + kHitRecordInts * h;
+		rec[0] = best >= 0 ? contactId : -1;
+		rec[1] = cs.shapeIdA;
+		rec[2] = cs.shapeIdB;
+		if ( best >= 0 )
+		{
+			const b2ManifoldPoint* mp = cs.manifold.points + best;
+			b2Pos point;
+			if ( a.aux[cs.bodyIdA].type != b2_staticBody && a.aux[cs.bodyIdB].type == b2_staticBody )
+			{
+				point = b2OffsetPos( a.sims[cs.bodyIdB].center, mp->anchorB );
+			}
+			else
+			{
+				point = b2OffsetPos( a.sims[cs.bodyIdA].center, mp->anchorA );
+			}
+			rec[3] = __float_as_int( point.x );
+			rec[4] = __float_as_int( point.y );
+			rec[5] = __float_as_int( cs.manifold.normal.x );
+			rec[6] = __float_as_int( cs.manifold.normal.y );
+			rec[7] = __float_as_int( approach );
+		}
+	}
 	__syncthreads();
 	if ( threadIdx.x == 0 )
 	{
+		a.mail[4] = hits;
+		*a.hitCount = 0;
 		a.mail[0] = bodies;
 		a.mail[1] = shapes;
 		a.mail[2] = fast;
@@ -2191,6 +2266,8 @@ struct DeviceWorld::Impl
 	DevArray<int> enlargedShapes;
 	DevArray<b2AABB> enlargedShapeAABBs;
 	DevArray<float> sleepTimes;
+	DevArray<int> hitCandidates;
+	int solveHitCapacity = 0;
 
 	// counters: [0..1] collide, [4..6] finalize
 	int* counters = nullptr;
@@ -3148,6 +3225,10 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		sa.contacts = impl->contacts.ptr;
 		sa.coloredCount = L.overflowContactStart;
 		sa.totalCount = nc;
+		impl->hitCandidates.Reserve( std::max<size_t>( nc, 1 ), false, s, 0 );
+		sa.hitCandidates = impl->hitCandidates.ptr;
+		sa.hitCount = impl->counters + 8;
+		sa.negHitThreshold = -params.hitEventThreshold;
 		if ( rec )
 		{
 			impl->rec.store = true;
@@ -3231,7 +3312,16 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		pk.shapeOffset = kMailHeader + nb;
 		pk.aabbOffset = pk.shapeOffset + (int)shapeCap;
 		pk.sleepOffset = pk.aabbOffset + 4 * (int)shapeCap;
-		impl->EnsureMail( (size_t)pk.sleepOffset + nb );
+		pk.hitOffset = pk.sleepOffset + nb;
+		impl->solveHitCapacity = nc;
+		impl->hitCandidates.Reserve( std::max<size_t>( nc, 1 ), false, s, 0 );
+		pk.contacts = impl->contacts.ptr;
+		pk.sims = impl->sims.ptr;
+		pk.aux = impl->aux.ptr;
+		pk.hitCandidates = impl->hitCandidates.ptr;
+		pk.hitCount = impl->counters + 8;
+		pk.hitThreshold = params.hitEventThreshold;
+		impl->EnsureMail( (size_t)pk.hitOffset + (size_t)kHitRecordInts * std::max( nc, 0 ) );
 		pk.mail = impl->mail;
 		impl->solveBodyCount = nb;
 		impl->solveNeedSleepTimes = needSleepTimes;
@@ -3282,6 +3372,31 @@ void DeviceWorld::SolveFinish( FinalizeResult* result )
 			memcpy( result->sleepTimes.data(), mail + pk.sleepOffset, nb * sizeof( float ) );
 		}
 		std::sort( result->enlargedBodies.begin(), result->enlargedBodies.end() );
+
+		int hitCount = mail[4];
+		if ( hitCount > impl->solveHitCapacity )
+		{
+			Fatal( "hit event count %d exceeds the colored contact count %d", hitCount, impl->solveHitCapacity );
+		}
+		result->hits.clear();
+		for ( int h = 0; h < hitCount; ++h )
+		{
+			const int* rec = mail + pk.hitOffset + kHitRecordInts * h;
+			if ( rec[0] < 0 )
+			{
+				continue;
+			}
+			HitRecord hr;
+			hr.contactId = rec[0];
+			hr.shapeIdA = rec[1];
+			hr.shapeIdB = rec[2];
+			memcpy( &hr.point, rec + 3, 2 * sizeof( float ) );
+			memcpy( &hr.normal, rec + 5, 2 * sizeof( float ) );
+			memcpy( &hr.approachSpeed, rec + 7, sizeof( float ) );
+			result->hits.push_back( hr );
+		}
+		// the reference reports hits in ascending contact id
+		std::sort( result->hits.begin(), result->hits.end(), []( const HitRecord& x, const HitRecord& y ) { return x.contactId < y.contactId; } );
 	}
 
 	if ( impl->recording == false )

@@ -1745,6 +1745,13 @@ void World::DestroyContact( Contact* contact )
 
 	bool touching = ( contact->flags & kContactTouching ) != 0;
 
+	// end touch event (reference src/contact.c:389-412)
+	if ( touching && ( contact->flags & kContactEnableContactEvents ) != 0 )
+	{
+		contactEndEvents[endEventArrayIndex].push_back(
+			b2ContactEndTouchEvent{ ShapeIdOf( contact->shapeIdA ), ShapeIdOf( contact->shapeIdB ), ContactIdOf( *contact ) } );
+	}
+
 	if ( edgeA->prevKey != B2_NULL_INDEX )
 	{
 		contacts[edgeA->prevKey >> 1].edges[edgeA->prevKey & 1].nextKey = edgeA->nextKey;
@@ -2558,6 +2565,11 @@ void World::ProcessContactStates( const int* changes, int count )
 		}
 		else if ( kind == 2 )
 		{
+			if ( contact->flags & kContactEnableContactEvents )
+			{
+				contactBeginEvents.push_back( b2ContactBeginTouchEvent{ ShapeIdOf( contact->shapeIdA ), ShapeIdOf( contact->shapeIdB ),
+																		ContactIdOf( *contact ) } );
+			}
 			contact->flags |= kContactTouching;
 			LinkContact( contact );
 			AddContactToGraph( contact );
@@ -2571,6 +2583,11 @@ void World::ProcessContactStates( const int* changes, int count )
 		else if ( kind == 3 )
 		{
 			contact->flags &= ~kContactTouching;
+			if ( contact->flags & kContactEnableContactEvents )
+			{
+				contactEndEvents[endEventArrayIndex].push_back(
+					b2ContactEndTouchEvent{ ShapeIdOf( contact->shapeIdA ), ShapeIdOf( contact->shapeIdB ), ContactIdOf( *contact ) } );
+			}
 			UnlinkContact( contact );
 			int bodyIdA = contact->edges[0].bodyId;
 			int bodyIdB = contact->edges[1].bodyId;
@@ -2729,6 +2746,8 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	moveEvents.clear();
 	moveEventBodyIds.clear();
 	moveEventsNeedTransforms = false;
+	contactBeginEvents.clear();
+	contactHitEvents.clear();
 	profile = b2Profile{};
 	recycledContactCount = 0;
 
@@ -2846,7 +2865,29 @@ void World::StepEnd()
 		solvePending = false;
 	}
 	profile.step = b2GetMilliseconds( stepTicks );
+
+	// swap the end event buffers
+	endEventArrayIndex = 1 - endEventArrayIndex;
+	contactEndEvents[endEventArrayIndex].clear();
 	locked = false;
+}
+
+b2ContactEvents World::GetContactEvents()
+{
+	b2ContactEvents events = {};
+	if ( locked )
+	{
+		return events;
+	}
+	// the previous buffer holds the last step's end events
+	std::vector<b2ContactEndTouchEvent>& ends = contactEndEvents[1 - endEventArrayIndex];
+	events.beginEvents = contactBeginEvents.data();
+	events.endEvents = ends.data();
+	events.hitEvents = contactHitEvents.data();
+	events.beginCount = (int)contactBeginEvents.size();
+	events.endCount = (int)ends.size();
+	events.hitCount = (int)contactHitEvents.size();
+	return events;
 }
 
 bool World::SolveBegin( StepParams& params )
@@ -2892,6 +2933,19 @@ void World::SolveEnd()
 	FinalizeResult fr;
 	device->SolveFinish( &fr );
 	profile.constraints = b2GetMilliseconds( constraintTicks );
+
+	// hit events in ascending contact id (reference src/solver.c:1703-1820)
+	for ( const HitRecord& hr : fr.hits )
+	{
+		b2ContactHitEvent event = {};
+		event.shapeIdA = ShapeIdOf( hr.shapeIdA );
+		event.shapeIdB = ShapeIdOf( hr.shapeIdB );
+		event.contactId = ContactIdOf( contacts[hr.contactId] );
+		event.point = hr.point;
+		event.normal = hr.normal;
+		event.approachSpeed = hr.approachSpeed;
+		contactHitEvents.push_back( event );
+	}
 
 	bodiesStale = true;
 	shapesStale = true;

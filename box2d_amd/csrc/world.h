@@ -442,6 +442,23 @@ public:
 	class PairFinder* pairFinder = nullptr;
 	int lastPairQueryPath = 0; // 1 host, 2 GPU (for tests)
 
+	// Contact events of the last step (reference src/physics_world.c:860-866, 975-983,
+	// 1478-1505): begin and hit events are cleared when a step starts; end events are double
+	// buffered because contacts can also end between steps (shape or body destruction).
+	std::vector<b2ContactBeginTouchEvent> contactBeginEvents;
+	std::vector<b2ContactEndTouchEvent> contactEndEvents[2];
+	std::vector<b2ContactHitEvent> contactHitEvents;
+	int endEventArrayIndex = 0;
+	b2ContactEvents GetContactEvents();
+	b2ShapeId ShapeIdOf( int shapeId ) const
+	{
+		return b2ShapeId{ shapeId + 1, worldId, shapes[shapeId].generation };
+	}
+	b2ContactId ContactIdOf( const Contact& contact ) const
+	{
+		return b2ContactId{ contact.contactId + 1, worldId, 0, contact.generation };
+	}
+
 private:
 	friend class DeviceWorld;
 

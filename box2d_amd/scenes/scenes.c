@@ -750,6 +750,60 @@ static void b2sBuildJointZoo( b2sScene* scene, int enableSleep )
 	}
 }
 
+static void b2sBuildEvents( b2sScene* scene, int count, uint32_t seed )
+{
+	b2WorldId worldId = scene->worldId;
+	s_rngState = seed;
+	int capacity = 0;
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.position = ( b2Pos ){ 0.0f, -1.0f };
+		b2BodyId groundId = b2CreateBody( worldId, &bodyDef );
+		b2Polygon box = b2MakeBox( 40.0f, 1.0f );
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.enableContactEvents = true;
+		b2CreatePolygonShape( groundId, &shapeDef, &box );
+	}
+	int columns = 12;
+	for ( int i = 0; i < count; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ -18.0f + 3.0f * ( i % columns ), 2.0f + 2.5f * ( i / columns ) };
+		bodyDef.rotation = b2MakeRot( b2sRandomRange( -1.0f, 1.0f ) );
+		bodyDef.linearVelocity = ( b2Vec2 ){ b2sRandomRange( -3.0f, 3.0f ), b2sRandomRange( -8.0f, 0.0f ) };
+		bodyDef.angularVelocity = b2sRandomRange( -2.0f, 2.0f );
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+
+		b2ShapeDef shapeDef = b2DefaultShapeDef();
+		shapeDef.enableContactEvents = ( i % 3 ) != 2;
+		shapeDef.enableHitEvents = ( i % 2 ) == 0;
+		shapeDef.material.restitution = ( i % 4 ) == 1 ? 0.3f : 0.0f;
+		switch ( i % 3 )
+		{
+			case 0:
+			{
+				b2Circle circle = { { 0.0f, 0.0f }, b2sRandomRange( 0.3f, 0.6f ) };
+				b2CreateCircleShape( bodyId, &shapeDef, &circle );
+			}
+			break;
+			case 1:
+			{
+				b2Polygon box = b2MakeBox( b2sRandomRange( 0.3f, 0.8f ), b2sRandomRange( 0.3f, 0.8f ) );
+				b2CreatePolygonShape( bodyId, &shapeDef, &box );
+			}
+			break;
+			default:
+			{
+				b2Capsule capsule = { { -0.4f, 0.0f }, { 0.4f, 0.0f }, b2sRandomRange( 0.2f, 0.4f ) };
+				b2CreateCapsuleShape( bodyId, &shapeDef, &capsule );
+			}
+			break;
+		}
+		b2sPushBody( scene, &capacity, bodyId );
+	}
+}
+
 int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount )
 {
 	int handle = -1;
@@ -827,6 +881,10 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 			b2sBuildPush( scene, rows, vx, vy, omega, friction );
 		}
 		break;
+
+		case B2S_EVENTS:
+			b2sBuildEvents( scene, iparamCount > 0 ? iparams[0] : 60, iparamCount > 1 ? (uint32_t)iparams[1] : 12345u );
+			break;
 
 		case B2S_JOINT_ZOO:
 			b2sBuildJointZoo( scene, iparamCount > 0 ? iparams[0] : 0 );
@@ -1001,6 +1059,64 @@ int b2s_move_count( int handle )
 	}
 	b2BodyEvents events = b2World_GetBodyEvents( scene->worldId );
 	return events.moveCount;
+}
+
+static uint32_t b2sFloatBits( float x )
+{
+	uint32_t u;
+	memcpy( &u, &x, sizeof( u ) );
+	return u;
+}
+
+int b2s_contact_events( int handle, uint32_t* out, int capacity )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	b2ContactEvents events = b2World_GetContactEvents( scene->worldId );
+	int total = events.beginCount + events.endCount + events.hitCount;
+	int n = 0;
+	for ( int k = 0; k < total && 10 * ( n + 1 ) <= capacity; ++k )
+	{
+		uint32_t* r = out + 10 * n;
+		memset( r, 0, 10 * sizeof( uint32_t ) );
+		if ( k < events.beginCount )
+		{
+			const b2ContactBeginTouchEvent* e = events.beginEvents + k;
+			r[0] = 0;
+			r[1] = (uint32_t)e->shapeIdA.index1;
+			r[2] = (uint32_t)e->shapeIdB.index1;
+			r[3] = (uint32_t)e->contactId.index1;
+			r[4] = e->contactId.generation;
+		}
+		else if ( k < events.beginCount + events.endCount )
+		{
+			const b2ContactEndTouchEvent* e = events.endEvents + ( k - events.beginCount );
+			r[0] = 1;
+			r[1] = (uint32_t)e->shapeIdA.index1;
+			r[2] = (uint32_t)e->shapeIdB.index1;
+			r[3] = (uint32_t)e->contactId.index1;
+			r[4] = e->contactId.generation;
+		}
+		else
+		{
+			const b2ContactHitEvent* e = events.hitEvents + ( k - events.beginCount - events.endCount );
+			r[0] = 2;
+			r[1] = (uint32_t)e->shapeIdA.index1;
+			r[2] = (uint32_t)e->shapeIdB.index1;
+			r[3] = (uint32_t)e->contactId.index1;
+			r[4] = e->contactId.generation;
+			r[5] = b2sFloatBits( (float)e->point.x );
+			r[6] = b2sFloatBits( (float)e->point.y );
+			r[7] = b2sFloatBits( e->normal.x );
+			r[8] = b2sFloatBits( e->normal.y );
+			r[9] = b2sFloatBits( e->approachSpeed );
+		}
+		n += 1;
+	}
+	return n;
 }
 
 int b2s_awake_body_count( int handle )

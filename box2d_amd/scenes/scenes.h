@@ -67,6 +67,12 @@ enum
 	// limit and motor.
 	// iparams: [0] enableSleep. Bodies are listed in creation order.
 	B2S_JOINT_ZOO = 7,
+
+	// Contact events: circles, boxes and capsules thrown onto a 40 x 1 m ground box (half extents)
+	// from a grid of heights with random velocities. Contact events are on for two shapes in
+	// three and for the ground, hit events for every other shape. Sleep is on.
+	// iparams: [0] body count, [1] seed.
+	B2S_EVENTS = 8,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.
@@ -105,6 +111,13 @@ B2S_API int b2s_get_velocities( int handle, float* out, int capacity );
 // Body move event count and awake body count of the last step.
 B2S_API int b2s_move_count( int handle );
 B2S_API int b2s_awake_body_count( int handle );
+
+// The contact events of the last step, flattened in 10-word records: begin (kind 0), end (kind 1)
+// and hit (kind 2) events in the order b2World_GetContactEvents lists them. A record is
+// [kind, shapeA index1, shapeB index1, contact index1, contact generation, point x, point y,
+// normal x, normal y, approach speed] with floats as their bits (zero for begin and end events).
+// Writes at most capacity / 10 records and returns the record count.
+B2S_API int b2s_contact_events( int handle, uint32_t* out, int capacity );
 
 // World counters: out[0..5] = bodyCount, shapeCount, contactCount, jointCount, islandCount,
 // awakeContactCount; out[6..29] = colorCounts[0..23]. Returns the number of ints written (30).

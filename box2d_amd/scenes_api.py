@@ -20,6 +20,7 @@ SCENE_RAIN = 4
 SCENE_BATCH_WORLD = 5
 SCENE_PUSH = 6
 SCENE_JOINT_ZOO = 7
+SCENE_EVENTS = 8
 
 
 class SceneLib:
@@ -112,6 +113,12 @@ class SceneLib:
 
     def awake_body_count(self, h):
         return self.lib.b2s_awake_body_count(h)
+
+    def contact_events(self, h, capacity=1 << 16):
+        """The last step's contact events as a (records, 10) uint32 array (see b2s_contact_events)."""
+        out = np.zeros(capacity * 10, dtype=np.uint32)
+        n = self.lib.b2s_contact_events(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), out.size)
+        return out[: 10 * n].reshape(n, 10)
 
     def falling_hinges(self, workers=1, step_limit=500):
         s = ctypes.c_int()

@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from box2d_amd.scenes_api import (SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES,  # noqa: E402
-                                  SCENE_RAIN, SCENE_JOINT_ZOO)
+                                  SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS)
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
 
@@ -63,6 +63,33 @@ def scene_fixture(lib, name, kind, iparams, fparams, steps, every, keep_transfor
     return fx
 
 
+def fnv1a32(words):
+    h = 0x811C9DC5
+    for w in np.asarray(words, dtype=np.uint32).reshape(-1).tolist():
+        for b in range(4):
+            h ^= (w >> (8 * b)) & 0xFF
+            h = (h * 0x01000193) & 0xFFFFFFFF
+    return h
+
+
+def events_fixture(lib, name, count, seed, steps):
+    """Per-step contact events of the events scene: counts of begin, end and hit events and an
+    FNV-1a hash of the flattened records (b2s_contact_events), the transform hash, and the full
+    records every 25 steps."""
+    h = lib.create(SCENE_EVENTS, (count, seed), (), workers=8)
+    fx = {"name": name, "kind": SCENE_EVENTS, "iparams": [count, seed], "fparams": [], "dt": 1.0 / 60.0,
+          "substeps": 4, "steps": steps, "body_count": lib.body_count(h), "per_step": [], "records": {}}
+    for step in range(1, steps + 1):
+        lib.step(h)
+        ev = lib.contact_events(h)
+        kinds = ev[:, 0].tolist() if len(ev) else []
+        fx["per_step"].append([kinds.count(0), kinds.count(1), kinds.count(2), fnv1a32(ev), lib.hash(h)])
+        if step % 25 == 0:
+            fx["records"][str(step)] = ev.reshape(-1).tolist()
+    lib.destroy(h)
+    return fx
+
+
 def main():
     if not os.path.exists(REF_LIB):
         sys.exit(f"{REF_LIB} missing: run `make -C oracle ref` first")
@@ -85,10 +112,11 @@ def main():
                                   True))
     fixtures.append(scene_fixture(lib, "joint_zoo", SCENE_JOINT_ZOO, (0,), (), 400, set(range(1, 401)), True))
     fixtures.append(scene_fixture(lib, "joint_zoo_sleep", SCENE_JOINT_ZOO, (1,), (), 400, set(range(1, 401)), True))
+    events = [events_fixture(lib, "events_60", 60, 12345, 300), events_fixture(lib, "events_150", 150, 7, 240)]
     ok, step, hsh = lib.falling_hinges(1, 500)
     hinges = {"name": "falling_hinges", "slept": ok, "sleep_step": step, "hash": hsh}
     out = {"generator": "tests/golden/gen_golden.py", "reference": "erincatto/box2d v3.2 (src/*.c, -O2 -ffp-contract=off)",
-           "scenes": fixtures, "falling_hinges": hinges}
+           "scenes": fixtures, "falling_hinges": hinges, "events": events}
     path = os.path.join(HERE, "world_runs.json")
     with open(path, "w") as f:
         json.dump(out, f, separators=(",", ":"))
