@@ -448,6 +448,11 @@ struct PairFinder::Impl
 	uint64_t* hostCands = nullptr;
 	size_t hostCandCap = 0;
 
+	// the launched query, finished by Finish
+	int launchedProxies = 0;
+	int launchedQueries = 0;
+	void Query();
+
 	void Temp( size_t need )
 	{
 		if ( need > tempBytes )
@@ -557,14 +562,15 @@ uint8_t* PairFinder::MovedStaging( int count )
 	return GrowPinned( impl->hostMoved, impl->hostMovedCap, (size_t)count );
 }
 
-const uint64_t* PairFinder::Find( int n, int nq, int movedCount, int* count )
+void PairFinder::Launch( int n, int nq, int movedCount )
 {
 	Impl* m = impl;
 	hipStream_t s = m->stream;
-	*count = 0;
+	m->launchedProxies = n;
+	m->launchedQueries = nq;
 	if ( n == 0 || nq == 0 )
 	{
-		return nullptr;
+		return;
 	}
 	m->Reserve( n );
 	if ( (size_t)nq > m->queryCap )
@@ -606,23 +612,42 @@ const uint64_t* PairFinder::Find( int n, int nq, int movedCount, int* count )
 	int internalBlocks = n > 1 ? ( n - 1 + kLbvhBlock / 64 - 1 ) / ( kLbvhBlock / 64 ) : 0;
 	LbvhFinishKernel<<<blocks + internalBlocks, kLbvhBlock, 0, s>>>( t, blocks );
 	HIP_CHECK( hipGetLastError() );
+	m->tree.n = n;
+	m->Query();
+}
+
+// Query over the built tree; the candidate count and overflow flag come back asynchronously.
+void PairFinder::Impl::Query()
+{
+	LbvhQueryArgs qa;
+	qa.nodes = tree.nodes;
+	qa.nodeCount = 2 * launchedProxies - 1;
+	qa.queries = queries;
+	qa.queryCount = launchedQueries;
+	qa.moved = moved;
+	qa.out = cands;
+	qa.capacity = (int)candCap;
+	qa.counters = counters;
+	HIP_CHECK( hipMemsetAsync( counters, 0, 2 * sizeof( int ), stream ) );
+	LbvhQueryKernel<<<( launchedQueries + kLbvhBlock - 1 ) / kLbvhBlock, kLbvhBlock, 0, stream>>>( qa );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipMemcpyAsync( hostCounters, counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, stream ) );
+}
+
+const uint64_t* PairFinder::Finish( int* count )
+{
+	Impl* m = impl;
+	hipStream_t s = m->stream;
+	*count = 0;
+	int nq = m->launchedQueries;
+	if ( m->launchedProxies == 0 || nq == 0 )
+	{
+		return nullptr;
+	}
 
 	int total = 0;
 	for ( ;; )
 	{
-		LbvhQueryArgs qa;
-		qa.nodes = t.nodes;
-		qa.nodeCount = 2 * n - 1;
-		qa.queries = m->queries;
-		qa.queryCount = nq;
-		qa.moved = m->moved;
-		qa.out = m->cands;
-		qa.capacity = (int)m->candCap;
-		qa.counters = m->counters;
-		HIP_CHECK( hipMemsetAsync( m->counters, 0, 2 * sizeof( int ), s ) );
-		LbvhQueryKernel<<<( nq + kLbvhBlock - 1 ) / kLbvhBlock, kLbvhBlock, 0, s>>>( qa );
-		HIP_CHECK( hipGetLastError() );
-		HIP_CHECK( hipMemcpyAsync( m->hostCounters, m->counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
 		HIP_CHECK( hipStreamSynchronize( s ) );
 		total = m->hostCounters[0];
 		if ( m->hostCounters[1] != 0 )
@@ -637,6 +662,7 @@ const uint64_t* PairFinder::Find( int n, int nq, int movedCount, int* count )
 			m->candCap = (size_t)total + total / 2;
 			HIP_CHECK( hipMalloc( &m->cands, m->candCap * sizeof( uint64_t ) ) );
 			HIP_CHECK( hipMalloc( &m->sortedCands, m->candCap * sizeof( uint64_t ) ) );
+			m->Query();
 			continue;
 		}
 		break;

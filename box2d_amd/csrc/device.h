@@ -204,9 +204,11 @@ public:
 	uint8_t* MovedStaging( int count );
 
 	// Every (query, proxy) whose fat boxes overlap and that passes the pair callback's self, tree
-	// and moved-proxy rules, as (query index << 32 | proxy index) keys in ascending order. The
-	// array stays valid until the next call.
-	const uint64_t* Find( int proxyCount, int queryCount, int movedCount, int* count );
+	// and moved-proxy rules, as (query index << 32 | proxy index) keys in ascending order. Launch
+	// uploads the staging and starts the device work without waiting (the staging may be refilled
+	// only after Finish); Finish waits and returns the keys, valid until the next Launch.
+	void Launch( int proxyCount, int queryCount, int movedCount );
+	const uint64_t* Finish( int* count );
 
 	struct Impl;
 

@@ -2363,7 +2363,7 @@ bool World::AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, in
 // the reference's stack query visits leaves: kinematic tree, static tree, dynamic tree, each in
 // depth-first child2-first leaf order (src/broad_phase.c:370-397, src/dynamic_tree.c:1085-1135), so
 // the finder's (query, proxy number) order is the reference's per-query hit order.
-void World::QueryPairsGpu( std::vector<std::vector<MovePair>>& results )
+void World::QueryPairsGpuLaunch()
 {
 	BroadPhase& bp = broadPhase;
 	if ( pairFinder == nullptr )
@@ -2399,7 +2399,8 @@ void World::QueryPairsGpu( std::vector<std::vector<MovePair>>& results )
 	int movedCount = std::max<int>( (int)shapes.size(), 1 );
 	uint8_t* moved = pairFinder->MovedStaging( movedCount );
 	memset( moved, 0, movedCount );
-	std::vector<int> queryMove;
+	std::vector<int>& queryMove = pairQueryMove;
+	queryMove.clear();
 	queryMove.reserve( moveCount );
 	int nq = 0;
 	for ( int i = 0; i < moveCount; ++i )
@@ -2416,8 +2417,17 @@ void World::QueryPairsGpu( std::vector<std::vector<MovePair>>& results )
 		moved[shapeId] = 1;
 	}
 
+	pairFinder->Launch( n, nq, movedCount );
+}
+
+// Candidates of the launched query, filtered in the reference's per-query visit order.
+void World::QueryPairsGpuFinish( std::vector<std::vector<MovePair>>& results )
+{
+	const LbvhProxy* proxies = pairFinder->ProxyStaging( 0 );
+	const LbvhProxy* queries = pairFinder->QueryStaging( 0 );
+	const std::vector<int>& queryMove = pairQueryMove;
 	int count = 0;
-	const uint64_t* keys = pairFinder->Find( n, nq, movedCount, &count );
+	const uint64_t* keys = pairFinder->Finish( &count );
 	for ( int c = 0; c < count; ++c )
 	{
 		int q = (int)( keys[c] >> 32 );
@@ -2453,7 +2463,9 @@ void World::UpdateBroadPhasePairs()
 
 	if ( gpu )
 	{
-		QueryPairsGpu( results );
+		// the device query works on staged copies of the trees' boxes, so the host rebuilds the
+		// trees while it runs; the candidate filters read no tree
+		QueryPairsGpuLaunch();
 	}
 	else
 	{
@@ -2523,6 +2535,11 @@ void World::UpdateBroadPhasePairs()
 	// The tree rebuild runs after the queries (the reference overlaps it with the narrowphase).
 	bp.trees[b2_dynamicBody].Rebuild( false );
 	bp.trees[b2_kinematicBody].Rebuild( false );
+
+	if ( gpu )
+	{
+		QueryPairsGpuFinish( results );
+	}
 
 	// Create contacts serially in move-buffer order; each proxy's pairs were linked head-first,
 	// so they are created in reverse query order.

@@ -498,7 +498,9 @@ private:
 
 	// step stages
 	void UpdateBroadPhasePairs();
-	void QueryPairsGpu( std::vector<std::vector<MovePair>>& results );
+	void QueryPairsGpuLaunch();
+	void QueryPairsGpuFinish( std::vector<std::vector<MovePair>>& results );
+	std::vector<int> pairQueryMove; // move-buffer index of each launched query
 	bool AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair );
 	void ProcessContactStates( const int* changedIds, int count );
 	bool SolveBegin( StepParams& params );
