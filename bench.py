@@ -98,10 +98,23 @@ def cpu_baseline(rows, ground, seconds, threads):
         steps += 10
     ref.destroy(h)
     ms = total_ms / steps
+
+    # thread sweep on a shorter sample: steps 2..101, minimum of two runs per worker count
+    sweep = []
+    for t in sorted({1, 2, 4, 8, threads}):
+        best = None
+        for _ in range(2):
+            hs = ref.pyramid(rows, ground, sleep=False, workers=t)
+            ref.step(hs)
+            run = ref.step_timed(hs, 100) / 100
+            ref.destroy(hs)
+            best = run if best is None else min(best, run)
+        sweep.append({"threads": t, "ms_per_step": best, "body_substeps_per_s": bodies * SUBSTEPS / (best * 1e-3)})
     return {"value": bodies * SUBSTEPS / (ms * 1e-3), "unit": "body-substeps/s", "cores": threads,
             "kind": "reference", "ms_per_step": ms,
             "sample": f"steps 2..{steps + 1} of the same {rows}-row pyramid, workerCount={threads} "
-                      f"(reference built-in scheduler), monotonic clock"}
+                      f"(reference built-in scheduler), monotonic clock",
+            "thread_sweep": sweep}
 
 
 def batched_workload(lib, scenes, worlds, steps, loop_steps):

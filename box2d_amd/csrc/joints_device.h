@@ -1,6 +1,7 @@
 // GPU joint solver: warm start and solve of every joint type (revolute, weld, prismatic, wheel,
-// distance, motor, mover, pogo; filter joints only take part in collision filtering and islands). Each function is one lane's work on one joint and keeps the reference's
-// operation order, so its floats are the reference's:
+// distance, motor, mover, pogo; filter joints only take part in collision filtering and islands).
+// Each function is one lane's work on one joint and keeps the reference's operation order, so its
+// floats are the reference's:
 //   revolute  src/revolute_joint.c:217-500
 //   weld      src/weld_joint.c:148-452 (the non-block solve, B2_WELD_BLOCK_SOLVE 0)
 //   prismatic src/prismatic_joint.c:295-664
