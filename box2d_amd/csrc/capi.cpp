@@ -469,6 +469,243 @@ b2WorldId b2Body_GetWorld( b2BodyId bodyId )
 	return b2WorldId{ (uint16_t)( bodyId.world0 + 1 ), world->generation };
 }
 
+void b2DestroyBody( b2BodyId bodyId )
+{
+	World* world = GetWorldFromIndex( bodyId.world0 );
+	if ( world != nullptr )
+	{
+		world->DestroyBody( bodyId );
+	}
+}
+
+b2Vec2 b2Body_GetLocalPoint( b2BodyId bodyId, b2Pos worldPoint )
+{
+	return b2InvTransformWorldPoint( b2Body_GetTransform( bodyId ), worldPoint );
+}
+
+b2Pos b2Body_GetWorldPoint( b2BodyId bodyId, b2Vec2 localPoint )
+{
+	return b2TransformWorldPoint( b2Body_GetTransform( bodyId ), localPoint );
+}
+
+b2Vec2 b2Body_GetLocalVector( b2BodyId bodyId, b2Vec2 worldVector )
+{
+	return b2InvRotateVector( b2Body_GetTransform( bodyId ).q, worldVector );
+}
+
+b2Vec2 b2Body_GetWorldVector( b2BodyId bodyId, b2Vec2 localVector )
+{
+	return b2RotateVector( b2Body_GetTransform( bodyId ).q, localVector );
+}
+
+void b2Body_SetTransform( b2BodyId bodyId, b2Pos position, b2Rot rotation )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	if ( world == nullptr || world->locked )
+	{
+		return;
+	}
+	world->SetBodyTransform( body, position, rotation );
+}
+
+// Velocity that carries the body to the target in one step (reference src/body.c:842-893).
+void b2Body_SetTargetTransform( b2BodyId bodyId, b2WorldTransform target, float timeStep, bool wake )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	if ( world == nullptr || body->setIndex == kDisabledSet || body->type == b2_staticBody || timeStep <= 0.0f )
+	{
+		return;
+	}
+	if ( body->setIndex != kAwakeSet && wake == false )
+	{
+		return;
+	}
+	world->PullBodies();
+	const BodySim& sim = world->bodySims[body->id];
+	b2Vec2 delta = b2SubPos( b2TransformWorldPoint( target, sim.localCenter ), sim.center );
+	float invTimeStep = 1.0f / timeStep;
+	b2Vec2 linearVelocity = b2MulSV( invTimeStep, delta );
+	float deltaAngle = b2RelativeAngle( sim.transform.q, target.q );
+	float angularVelocity = invTimeStep * deltaAngle;
+
+	if ( body->setIndex != kAwakeSet )
+	{
+		float maxVelocity = b2Length( linearVelocity ) + b2AbsFloat( angularVelocity ) * sim.maxExtent;
+		if ( maxVelocity < world->bodyAux[body->id].sleepThreshold )
+		{
+			return;
+		}
+		world->WakeBodyPublic( body );
+	}
+	BodyState& state = world->bodyStates[body->id];
+	state.linearVelocity = linearVelocity;
+	state.angularVelocity = angularVelocity;
+	world->bodiesDirty = true;
+}
+
+// Shared prologue of the force and impulse calls (reference src/body.c:935-1095): dynamic,
+// enabled bodies only; optionally wake a sleeping body; act only on awake bodies.
+static World* AwakeDynamicBody( b2BodyId bodyId, bool wake, Body** bodyOut )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	if ( world == nullptr || body->type != b2_dynamicBody || body->setIndex == kDisabledSet )
+	{
+		return nullptr;
+	}
+	world->PullBodies();
+	if ( wake && body->setIndex >= kFirstSleepingSet )
+	{
+		world->WakeBodyPublic( body );
+	}
+	if ( body->setIndex != kAwakeSet )
+	{
+		return nullptr;
+	}
+	world->bodiesDirty = true;
+	*bodyOut = body;
+	return world;
+}
+
+static void LimitVelocity( BodyState* state, float maxLinearSpeed )
+{
+	float v2 = b2LengthSquared( state->linearVelocity );
+	if ( v2 > maxLinearSpeed * maxLinearSpeed )
+	{
+		state->linearVelocity = b2MulSV( maxLinearSpeed / sqrtf( v2 ), state->linearVelocity );
+	}
+}
+
+void b2Body_ApplyForce( b2BodyId bodyId, b2Vec2 force, b2Pos point, bool wake )
+{
+	Body* body;
+	World* world = AwakeDynamicBody( bodyId, wake, &body );
+	if ( world != nullptr )
+	{
+		BodySim& sim = world->bodySims[body->id];
+		sim.force = b2Add( sim.force, force );
+		sim.torque += b2Cross( b2SubPos( point, sim.center ), force );
+	}
+}
+
+void b2Body_ApplyForceToCenter( b2BodyId bodyId, b2Vec2 force, bool wake )
+{
+	Body* body;
+	World* world = AwakeDynamicBody( bodyId, wake, &body );
+	if ( world != nullptr )
+	{
+		BodySim& sim = world->bodySims[body->id];
+		sim.force = b2Add( sim.force, force );
+	}
+}
+
+void b2Body_ApplyTorque( b2BodyId bodyId, float torque, bool wake )
+{
+	Body* body;
+	World* world = AwakeDynamicBody( bodyId, wake, &body );
+	if ( world != nullptr )
+	{
+		world->bodySims[body->id].torque += torque;
+	}
+}
+
+void b2Body_ApplyLinearImpulse( b2BodyId bodyId, b2Vec2 impulse, b2Pos point, bool wake )
+{
+	Body* body;
+	World* world = AwakeDynamicBody( bodyId, wake, &body );
+	if ( world != nullptr )
+	{
+		const BodySim& sim = world->bodySims[body->id];
+		BodyState& state = world->bodyStates[body->id];
+		state.linearVelocity = b2MulAdd( state.linearVelocity, sim.invMass, impulse );
+		state.angularVelocity += sim.invInertia * b2Cross( b2SubPos( point, sim.center ), impulse );
+		LimitVelocity( &state, world->maxLinearSpeed );
+	}
+}
+
+void b2Body_ApplyLinearImpulseToCenter( b2BodyId bodyId, b2Vec2 impulse, bool wake )
+{
+	Body* body;
+	World* world = AwakeDynamicBody( bodyId, wake, &body );
+	if ( world != nullptr )
+	{
+		const BodySim& sim = world->bodySims[body->id];
+		BodyState& state = world->bodyStates[body->id];
+		state.linearVelocity = b2MulAdd( state.linearVelocity, sim.invMass, impulse );
+		LimitVelocity( &state, world->maxLinearSpeed );
+	}
+}
+
+void b2Body_ApplyAngularImpulse( b2BodyId bodyId, float impulse, bool wake )
+{
+	Body* body;
+	World* world = AwakeDynamicBody( bodyId, wake, &body );
+	if ( world != nullptr )
+	{
+		world->bodyStates[body->id].angularVelocity += world->bodySims[body->id].invInertia * impulse;
+	}
+}
+
+void b2Body_ApplyMassFromShapes( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	if ( world == nullptr || world->locked )
+	{
+		return;
+	}
+	world->ApplyMassFromShapes( body );
+}
+
+int b2Body_GetShapes( b2BodyId bodyId, b2ShapeId* shapeArray, int capacity )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	if ( world == nullptr )
+	{
+		return 0;
+	}
+	int shapeId = body->headShapeId;
+	int shapeCount = 0;
+	while ( shapeId != B2_NULL_INDEX && shapeCount < capacity )
+	{
+		const Shape& shape = world->shapes[shapeId];
+		shapeArray[shapeCount] = b2ShapeId{ shape.id + 1, bodyId.world0, shape.generation };
+		shapeCount += 1;
+		shapeId = shape.nextShapeId;
+	}
+	return shapeCount;
+}
+
+int b2Body_GetJointCount( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	return world != nullptr ? body->jointCount : 0;
+}
+
+int b2Body_GetJoints( b2BodyId bodyId, b2JointId* jointArray, int capacity )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	if ( world == nullptr )
+	{
+		return 0;
+	}
+	int jointKey = body->headJointKey;
+	int jointCount = 0;
+	while ( jointKey != B2_NULL_INDEX && jointCount < capacity )
+	{
+		const Joint& joint = world->joints[jointKey >> 1];
+		jointArray[jointCount] = b2JointId{ joint.jointId + 1, bodyId.world0, joint.generation };
+		jointCount += 1;
+		jointKey = joint.edges[jointKey & 1].nextKey;
+	}
+	return jointCount;
+}
+
 // Shapes ---------------------------------------------------------------------------------------
 
 static b2ShapeId CreateShapeEntry( b2BodyId bodyId, const b2ShapeDef* def, const void* geometry, b2ShapeType type )
@@ -546,6 +783,92 @@ b2AABB b2Shape_GetAABB( b2ShapeId shapeId )
 	}
 	world->device->DownloadShapes( *world );
 	return world->shapeData[shape->id].aabb;
+}
+
+void b2Shape_EnableSensorEvents( b2ShapeId shapeId, bool flag )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	if ( world != nullptr && world->locked == false )
+	{
+		shape->enableSensorEvents = flag;
+	}
+}
+
+bool b2Shape_AreSensorEventsEnabled( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr && shape->enableSensorEvents;
+}
+
+// Geometry accessors (reference src/shape.c:1489-1604). A shape of another type returns a
+// zeroed value where the reference asserts.
+extern "C++" template <typename T>
+static T ShapeGeometry( b2ShapeId shapeId, b2ShapeType type, T ShapeDev::*member )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	if ( world == nullptr || shape->type != type )
+	{
+		return T{};
+	}
+	return world->shapeData[shape->id].*member;
+}
+
+b2Circle b2Shape_GetCircle( b2ShapeId shapeId )
+{
+	return ShapeGeometry( shapeId, b2_circleShape, &ShapeDev::circle );
+}
+
+b2Segment b2Shape_GetSegment( b2ShapeId shapeId )
+{
+	return ShapeGeometry( shapeId, b2_segmentShape, &ShapeDev::segment );
+}
+
+b2Capsule b2Shape_GetCapsule( b2ShapeId shapeId )
+{
+	return ShapeGeometry( shapeId, b2_capsuleShape, &ShapeDev::capsule );
+}
+
+b2Polygon b2Shape_GetPolygon( b2ShapeId shapeId )
+{
+	return ShapeGeometry( shapeId, b2_polygonShape, &ShapeDev::polygon );
+}
+
+static void SetGeometry( b2ShapeId shapeId, b2ShapeType type, const void* geometry )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	if ( world == nullptr || world->locked )
+	{
+		return;
+	}
+	world->SetShapeGeometry( shape, type, geometry );
+}
+
+void b2Shape_SetCircle( b2ShapeId shapeId, const b2Circle* circle )
+{
+	SetGeometry( shapeId, b2_circleShape, circle );
+}
+
+void b2Shape_SetCapsule( b2ShapeId shapeId, const b2Capsule* capsule )
+{
+	if ( b2DistanceSquared( capsule->center1, capsule->center2 ) <= B2_LINEAR_SLOP * B2_LINEAR_SLOP )
+	{
+		return;
+	}
+	SetGeometry( shapeId, b2_capsuleShape, capsule );
+}
+
+void b2Shape_SetSegment( b2ShapeId shapeId, const b2Segment* segment )
+{
+	SetGeometry( shapeId, b2_segmentShape, segment );
+}
+
+void b2Shape_SetPolygon( b2ShapeId shapeId, const b2Polygon* polygon )
+{
+	SetGeometry( shapeId, b2_polygonShape, polygon );
 }
 
 b2ChainId b2CreateChain( b2BodyId bodyId, const b2ChainDef* def )
@@ -686,6 +1009,242 @@ b2BodyId b2Joint_GetBodyB( b2JointId jointId )
 	Body& body = world->bodies[joint->edges[1].bodyId];
 	return b2BodyId{ body.id + 1, jointId.world0, body.generation };
 }
+
+void b2DestroyJoint( b2JointId jointId )
+{
+	World* world = GetWorldFromIndex( jointId.world0 );
+	if ( world != nullptr )
+	{
+		world->DestroyJoint( jointId );
+	}
+}
+
+// The joint's solver record, current on the host. Edits mark the joint array for upload.
+static JointSim* JointSimOf( b2JointId jointId, World** worldOut, bool edit, int type = -1 )
+{
+	Joint* joint;
+	World* world = WorldFromJoint( jointId, &joint );
+	if ( world == nullptr || ( type >= 0 && joint->type != type ) )
+	{
+		return nullptr;
+	}
+	world->PullJoints();
+	if ( edit )
+	{
+		world->jointsDirty = true;
+	}
+	if ( worldOut != nullptr )
+	{
+		*worldOut = world;
+	}
+	return &world->jointSims[joint->jointId];
+}
+
+b2Transform b2Joint_GetLocalFrameA( b2JointId jointId )
+{
+	JointSim* js = JointSimOf( jointId, nullptr, false );
+	return js != nullptr ? js->localFrameA : b2Transform_identity;
+}
+
+b2Transform b2Joint_GetLocalFrameB( b2JointId jointId )
+{
+	JointSim* js = JointSimOf( jointId, nullptr, false );
+	return js != nullptr ? js->localFrameB : b2Transform_identity;
+}
+
+void b2Joint_SetLocalFrameA( b2JointId jointId, b2Transform localFrame )
+{
+	JointSim* js = JointSimOf( jointId, nullptr, true );
+	if ( js != nullptr )
+	{
+		js->localFrameA = localFrame;
+	}
+}
+
+void b2Joint_SetLocalFrameB( b2JointId jointId, b2Transform localFrame )
+{
+	JointSim* js = JointSimOf( jointId, nullptr, true );
+	if ( js != nullptr )
+	{
+		js->localFrameB = localFrame;
+	}
+}
+
+// Revolute joint accessors (reference src/revolute_joint.c:17-203). Toggling the spring, motor
+// or limit clears the matching accumulated impulse, as the reference does.
+#define B2G_REVOLUTE( EDIT )                                                                           \
+	World* world;                                                                                      \
+	JointSim* js = JointSimOf( jointId, &world, EDIT, b2_revoluteJoint );                              \
+	(void)world
+
+void b2RevoluteJoint_EnableSpring( b2JointId jointId, bool enableSpring )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr && enableSpring != ( js->enableSpring != 0 ) )
+	{
+		js->enableSpring = enableSpring;
+		js->springImpulse = 0.0f;
+	}
+}
+
+bool b2RevoluteJoint_IsSpringEnabled( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr && js->enableSpring != 0;
+}
+
+void b2RevoluteJoint_SetSpringHertz( b2JointId jointId, float hertz )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr )
+	{
+		js->hertz = hertz;
+	}
+}
+
+float b2RevoluteJoint_GetSpringHertz( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr ? js->hertz : 0.0f;
+}
+
+void b2RevoluteJoint_SetSpringDampingRatio( b2JointId jointId, float dampingRatio )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr )
+	{
+		js->dampingRatio = dampingRatio;
+	}
+}
+
+float b2RevoluteJoint_GetSpringDampingRatio( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr ? js->dampingRatio : 0.0f;
+}
+
+void b2RevoluteJoint_SetTargetAngle( b2JointId jointId, float angle )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr )
+	{
+		js->targetAngle = angle;
+	}
+}
+
+float b2RevoluteJoint_GetTargetAngle( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr ? js->targetAngle : 0.0f;
+}
+
+float b2RevoluteJoint_GetAngle( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	if ( js == nullptr )
+	{
+		return 0.0f;
+	}
+	world->PullBodies();
+	b2Rot qA = b2MulRot( world->bodySims[js->bodyIdA].transform.q, js->localFrameA.q );
+	b2Rot qB = b2MulRot( world->bodySims[js->bodyIdB].transform.q, js->localFrameB.q );
+	return b2RelativeAngle( qA, qB );
+}
+
+void b2RevoluteJoint_EnableLimit( b2JointId jointId, bool enableLimit )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr && enableLimit != ( js->enableLimit != 0 ) )
+	{
+		js->enableLimit = enableLimit;
+		js->lowerImpulse = 0.0f;
+		js->upperImpulse = 0.0f;
+	}
+}
+
+bool b2RevoluteJoint_IsLimitEnabled( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr && js->enableLimit != 0;
+}
+
+float b2RevoluteJoint_GetLowerLimit( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr ? js->lowerAngle : 0.0f;
+}
+
+float b2RevoluteJoint_GetUpperLimit( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr ? js->upperAngle : 0.0f;
+}
+
+void b2RevoluteJoint_SetLimits( b2JointId jointId, float lower, float upper )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr )
+	{
+		float lowerAngle = b2MinFloat( lower, upper );
+		float upperAngle = b2MaxFloat( lower, upper );
+		js->lowerAngle = b2ClampFloat( lowerAngle, -0.99f * B2_PI, 0.99f * B2_PI );
+		js->upperAngle = b2ClampFloat( upperAngle, -0.99f * B2_PI, 0.99f * B2_PI );
+	}
+}
+
+void b2RevoluteJoint_EnableMotor( b2JointId jointId, bool enableMotor )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr && enableMotor != ( js->enableMotor != 0 ) )
+	{
+		js->enableMotor = enableMotor;
+		js->motorImpulse = 0.0f;
+	}
+}
+
+bool b2RevoluteJoint_IsMotorEnabled( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr && js->enableMotor != 0;
+}
+
+void b2RevoluteJoint_SetMotorSpeed( b2JointId jointId, float motorSpeed )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr )
+	{
+		js->motorSpeed = motorSpeed;
+	}
+}
+
+float b2RevoluteJoint_GetMotorSpeed( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr ? js->motorSpeed : 0.0f;
+}
+
+float b2RevoluteJoint_GetMotorTorque( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr ? world->inv_h * js->motorImpulse : 0.0f;
+}
+
+void b2RevoluteJoint_SetMaxMotorTorque( b2JointId jointId, float torque )
+{
+	B2G_REVOLUTE( true );
+	if ( js != nullptr )
+	{
+		js->maxMotorTorque = torque;
+	}
+}
+
+float b2RevoluteJoint_GetMaxMotorTorque( b2JointId jointId )
+{
+	B2G_REVOLUTE( false );
+	return js != nullptr ? js->maxMotorTorque : 0.0f;
+}
+
+#undef B2G_REVOLUTE
 
 // Extensions (include/box2d_amd.h) -----------------------------------------------------------
 

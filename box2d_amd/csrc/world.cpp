@@ -361,6 +361,12 @@ void BroadPhase::DestroyProxy( int proxyKey )
 	trees[B2G_PROXY_TYPE( proxyKey )].DestroyProxy( B2G_PROXY_ID( proxyKey ) );
 }
 
+void BroadPhase::MoveProxy( int proxyKey, b2AABB aabb )
+{
+	trees[B2G_PROXY_TYPE( proxyKey )].MoveProxy( B2G_PROXY_ID( proxyKey ), aabb );
+	BufferMove( proxyKey );
+}
+
 void BroadPhase::EnlargeProxy( int proxyKey, b2AABB aabb )
 {
 	trees[B2G_PROXY_TYPE( proxyKey )].EnlargeProxy( B2G_PROXY_ID( proxyKey ), aabb );
@@ -1816,6 +1822,341 @@ void World::DestroyContact( Contact* contact )
 		WakeBody( bodyB );
 	}
 
+	listsDirty = true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Destruction (reference src/body.c:120-156, 346-436; src/joint.c:721-840; src/island.c:311-337)
+
+void World::UnlinkJoint( Joint* joint )
+{
+	if ( joint->islandId == B2_NULL_INDEX )
+	{
+		return;
+	}
+	Island* island = &islands[joint->islandId];
+	int removeIndex = joint->islandIndex;
+	int movedIndex = RemoveSwap( island->joints, removeIndex );
+	if ( movedIndex != B2_NULL_INDEX )
+	{
+		joints[island->joints[removeIndex].jointId].islandIndex = removeIndex;
+	}
+	joint->islandId = B2_NULL_INDEX;
+	joint->islandIndex = B2_NULL_INDEX;
+	island->constraintRemoveCount += 1;
+}
+
+void World::RemoveBodyFromIsland( Body* body )
+{
+	if ( body->islandId == B2_NULL_INDEX )
+	{
+		return;
+	}
+	int islandId = body->islandId;
+	Island* island = &islands[islandId];
+	int localIndex = body->islandIndex;
+	int movedBodyId = island->bodies.back();
+	island->bodies[localIndex] = movedBodyId;
+	bodies[movedBodyId].islandIndex = localIndex;
+	island->bodies.pop_back();
+	if ( island->bodies.empty() )
+	{
+		DestroyIsland( islandId );
+	}
+	body->islandId = B2_NULL_INDEX;
+	body->islandIndex = B2_NULL_INDEX;
+}
+
+void World::DestroyJointInternal( Joint* joint )
+{
+	int jointId = joint->jointId;
+	JointEdge* edgeA = joint->edges + 0;
+	JointEdge* edgeB = joint->edges + 1;
+	Body* bodyA = &bodies[edgeA->bodyId];
+	Body* bodyB = &bodies[edgeB->bodyId];
+
+	// remove from the two bodies' joint lists
+	if ( edgeA->prevKey != B2_NULL_INDEX )
+	{
+		joints[edgeA->prevKey >> 1].edges[edgeA->prevKey & 1].nextKey = edgeA->nextKey;
+	}
+	if ( edgeA->nextKey != B2_NULL_INDEX )
+	{
+		joints[edgeA->nextKey >> 1].edges[edgeA->nextKey & 1].prevKey = edgeA->prevKey;
+	}
+	if ( bodyA->headJointKey == ( ( jointId << 1 ) | 0 ) )
+	{
+		bodyA->headJointKey = edgeA->nextKey;
+	}
+	bodyA->jointCount -= 1;
+
+	if ( edgeB->prevKey != B2_NULL_INDEX )
+	{
+		joints[edgeB->prevKey >> 1].edges[edgeB->prevKey & 1].nextKey = edgeB->nextKey;
+	}
+	if ( edgeB->nextKey != B2_NULL_INDEX )
+	{
+		joints[edgeB->nextKey >> 1].edges[edgeB->nextKey & 1].prevKey = edgeB->prevKey;
+	}
+	if ( bodyB->headJointKey == ( ( jointId << 1 ) | 1 ) )
+	{
+		bodyB->headJointKey = edgeB->nextKey;
+	}
+	bodyB->jointCount -= 1;
+
+	if ( joint->islandId != B2_NULL_INDEX )
+	{
+		UnlinkJoint( joint );
+	}
+
+	// remove from the solver set that owns the joint
+	int setIndex = joint->setIndex;
+	int localIndex = joint->localIndex;
+	if ( setIndex == kAwakeSet )
+	{
+		RemoveJointFromGraph( edgeA->bodyId, edgeB->bodyId, joint->colorIndex, localIndex );
+	}
+	else
+	{
+		SolverSet& set = solverSets[setIndex];
+		int movedIndex = RemoveSwap( set.jointIds, localIndex );
+		if ( movedIndex != B2_NULL_INDEX )
+		{
+			joints[set.jointIds[localIndex]].localIndex = localIndex;
+		}
+	}
+
+	joint->setIndex = B2_NULL_INDEX;
+	joint->localIndex = B2_NULL_INDEX;
+	joint->colorIndex = B2_NULL_INDEX;
+	joint->jointId = B2_NULL_INDEX;
+	jointIdPool.Free( jointId );
+
+	WakeBody( bodyA );
+	WakeBody( bodyB );
+	jointsDirty = true;
+	listsDirty = true;
+}
+
+void World::DestroyJoint( b2JointId id )
+{
+	if ( locked )
+	{
+		return;
+	}
+	int index = id.index1 - 1;
+	if ( index < 0 || index >= (int)joints.size() || joints[index].jointId != index || joints[index].generation != id.generation )
+	{
+		return;
+	}
+	// the device holds the other joints' impulses; the host copy must be current before the
+	// joint array is uploaded again
+	PullJoints();
+	DestroyJointInternal( &joints[index] );
+}
+
+void World::DestroyBody( b2BodyId id )
+{
+	if ( locked )
+	{
+		return;
+	}
+	Body* body = GetBody( id );
+	if ( body == nullptr )
+	{
+		return;
+	}
+	PullBodies();
+	PullShapes();
+	PullJoints();
+
+	// attached joints
+	int edgeKey = body->headJointKey;
+	while ( edgeKey != B2_NULL_INDEX )
+	{
+		Joint* joint = &joints[edgeKey >> 1];
+		edgeKey = joint->edges[edgeKey & 1].nextKey;
+		DestroyJointInternal( joint );
+	}
+
+	// attached contacts
+	int contactKey = body->headContactKey;
+	while ( contactKey != B2_NULL_INDEX )
+	{
+		Contact* contact = &contacts[contactKey >> 1];
+		contactKey = contact->edges[contactKey & 1].nextKey;
+		DestroyContact( contact );
+	}
+
+	// shapes and their broadphase proxies
+	int shapeId = body->headShapeId;
+	while ( shapeId != B2_NULL_INDEX )
+	{
+		Shape* shape = &shapes[shapeId];
+		if ( shape->proxyKey != B2_NULL_INDEX )
+		{
+			broadPhase.DestroyProxy( shape->proxyKey );
+			shape->proxyKey = B2_NULL_INDEX;
+		}
+		if ( shape->material.restitution > 0.0f )
+		{
+			restitutionShapeCount -= 1;
+		}
+		shapeIdPool.Free( shapeId );
+		shape->id = B2_NULL_INDEX;
+		shapeId = shape->nextShapeId;
+	}
+
+	// chains (their shapes went above)
+	int chainId = body->headChainId;
+	while ( chainId != B2_NULL_INDEX )
+	{
+		ChainShape* chain = &chains[chainId];
+		chain->shapeIndices.clear();
+		chain->materials.clear();
+		chainIdPool.Free( chainId );
+		chain->id = B2_NULL_INDEX;
+		chainId = chain->nextChainId;
+	}
+
+	RemoveBodyFromIsland( body );
+
+	int setIndex = body->setIndex;
+	SolverSet& set = solverSets[setIndex];
+	int movedIndex = RemoveSwap( set.bodyIds, body->localIndex );
+	if ( movedIndex != B2_NULL_INDEX )
+	{
+		bodies[set.bodyIds[body->localIndex]].localIndex = body->localIndex;
+	}
+	if ( setIndex >= kFirstSleepingSet && set.bodyIds.empty() )
+	{
+		DestroySolverSet( setIndex );
+	}
+
+	bodyIdPool.Free( body->id );
+	body->setIndex = B2_NULL_INDEX;
+	body->localIndex = B2_NULL_INDEX;
+	body->id = B2_NULL_INDEX;
+
+	// the static tree copy used by continuous collision and the shape array go up again
+	shapesDirty = true;
+	listsDirty = true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host edits between steps (reference src/body.c:713-757, 1452-1463; src/shape.c:1317-1357,
+// 1545-1604)
+
+void World::SetBodyTransform( Body* body, b2Vec2 position, b2Rot rotation )
+{
+	PullBodies();
+	PullShapes();
+	BodySim& sim = bodySims[body->id];
+	sim.transform.p = position;
+	sim.transform.q = rotation;
+	sim.center = b2TransformWorldPoint( sim.transform, sim.localCenter );
+	sim.rotation0 = sim.transform.q;
+	sim.center0 = sim.center;
+
+	b2Transform transform = sim.transform;
+	int shapeId = body->headShapeId;
+	while ( shapeId != B2_NULL_INDEX )
+	{
+		Shape& shape = shapes[shapeId];
+		ShapeDev& sd = shapeData[shapeId];
+		b2AABB aabb = HostFatShapeAABB( sd, transform, B2_SPECULATIVE_DISTANCE );
+		sd.aabb = aabb;
+		if ( b2AABB_Contains( sd.fatAABB, aabb ) == false )
+		{
+			float margin = sd.aabbMargin;
+			b2AABB fatAABB;
+			fatAABB.lowerBound.x = aabb.lowerBound.x - margin;
+			fatAABB.lowerBound.y = aabb.lowerBound.y - margin;
+			fatAABB.upperBound.x = aabb.upperBound.x + margin;
+			fatAABB.upperBound.y = aabb.upperBound.y + margin;
+			sd.fatAABB = fatAABB;
+			if ( shape.proxyKey != B2_NULL_INDEX )
+			{
+				broadPhase.MoveProxy( shape.proxyKey, fatAABB );
+			}
+		}
+		shapeId = shape.nextShapeId;
+	}
+	bodiesDirty = true;
+	shapesDirty = true;
+}
+
+void World::ApplyMassFromShapes( Body* body )
+{
+	PullBodies();
+	PullShapes();
+	UpdateBodyMassData( body );
+	bodiesDirty = true;
+}
+
+// The shape's contacts go, its AABBs follow the new geometry and its proxy is created again
+// with forced pair creation.
+void World::ResetShapeProxy( Shape* shape )
+{
+	Body* body = &bodies[shape->bodyId];
+	int shapeId = shape->id;
+	int contactKey = body->headContactKey;
+	while ( contactKey != B2_NULL_INDEX )
+	{
+		Contact* contact = &contacts[contactKey >> 1];
+		contactKey = contact->edges[contactKey & 1].nextKey;
+		if ( contact->shapeIdA == shapeId || contact->shapeIdB == shapeId )
+		{
+			DestroyContact( contact );
+		}
+	}
+
+	ShapeDev& sd = shapeData[shapeId];
+	b2Transform transform = bodySims[body->id].transform;
+	b2BodyType proxyType = shape->proxyKey != B2_NULL_INDEX ? B2G_PROXY_TYPE( shape->proxyKey ) : body->type;
+	b2AABB aabb = HostFatShapeAABB( sd, transform, B2_SPECULATIVE_DISTANCE );
+	sd.aabb = aabb;
+	float margin = proxyType == b2_staticBody ? B2_SPECULATIVE_DISTANCE : sd.aabbMargin;
+	sd.fatAABB.lowerBound.x = aabb.lowerBound.x - margin;
+	sd.fatAABB.lowerBound.y = aabb.lowerBound.y - margin;
+	sd.fatAABB.upperBound.x = aabb.upperBound.x + margin;
+	sd.fatAABB.upperBound.y = aabb.upperBound.y + margin;
+
+	if ( shape->proxyKey != B2_NULL_INDEX )
+	{
+		broadPhase.DestroyProxy( shape->proxyKey );
+		shape->proxyKey = broadPhase.CreateProxy( sd.fatAABB, proxyType, shape->filter.categoryBits, shapeId, true );
+	}
+}
+
+void World::SetShapeGeometry( Shape* shape, b2ShapeType type, const void* geometry )
+{
+	PullBodies();
+	PullShapes();
+	ShapeDev& sd = shapeData[shape->id];
+	switch ( type )
+	{
+		case b2_capsuleShape:
+			sd.capsule = *(const b2Capsule*)geometry;
+			break;
+		case b2_circleShape:
+			sd.circle = *(const b2Circle*)geometry;
+			break;
+		case b2_polygonShape:
+			sd.polygon = *(const b2Polygon*)geometry;
+			break;
+		case b2_segmentShape:
+			sd.segment = *(const b2Segment*)geometry;
+			break;
+		default:
+			Fatal( "unsupported shape type in a geometry update" );
+	}
+	// the reference keeps the centroid of the original geometry (src/shape.c:1545-1604)
+	shape->type = type;
+	sd.type = type;
+	sd.aabbMargin = ComputeShapeMargin( sd );
+	ResetShapeProxy( shape );
+	shapesDirty = true;
 	listsDirty = true;
 }
 

@@ -326,6 +326,7 @@ public:
 	void UnBufferMove( int proxyKey );
 	int CreateProxy( b2AABB aabb, b2BodyType proxyType, uint64_t categoryBits, int shapeIndex, bool forcePairCreation );
 	void DestroyProxy( int proxyKey );
+	void MoveProxy( int proxyKey, b2AABB aabb );
 	void EnlargeProxy( int proxyKey, b2AABB aabb );
 };
 
@@ -350,6 +351,11 @@ public:
 	b2JointId CreateWeldJoint( const b2WeldJointDef* def );
 	b2JointId CreateWheelJoint( const b2WheelJointDef* def );
 	void Step( float timeStep, int subStepCount );
+	void DestroyBody( b2BodyId id );
+	void DestroyJoint( b2JointId id );
+	void SetBodyTransform( Body* body, b2Vec2 position, b2Rot rotation );
+	void ApplyMassFromShapes( Body* body );
+	void SetShapeGeometry( Shape* shape, b2ShapeType type, const void* geometry );
 
 	// The three phases of Step, split at the two points where the host waits for the GPU, so a
 	// batch can run each phase for many worlds around one launch per stage.
@@ -361,6 +367,8 @@ public:
 	// Host copies of GPU state are refreshed lazily after a step.
 	void PullBodies();
 	void PushBodiesIfDirty();
+	void PullShapes();
+	void PullJoints();
 
 	b2BodyEvents GetBodyEvents();
 	b2Counters GetCounters();
@@ -482,6 +490,9 @@ private:
 	void MergeSolverSets( int setId1, int setId2 );
 	void DestroySolverSet( int setIndex );
 	bool WakeBody( Body* body );
+	void UnlinkJoint( Joint* joint );
+	void RemoveBodyFromIsland( Body* body );
+	void DestroyJointInternal( Joint* joint );
 	void RemoveBodyFromAwakeSet( Body* body );
 	bool ShouldBodiesCollide( Body* bodyA, Body* bodyB );
 
@@ -509,9 +520,7 @@ private:
 	bool solvePending = false;
 	uint64_t stepTicks = 0, collideTicks = 0, solveTicks = 0, constraintTicks = 0;
 	void FinishSolve( StepParams& params );
-
-	void PullShapes();
-	void PullJoints();
+	void ResetShapeProxy( Shape* shape );
 };
 
 World* GetWorld( b2WorldId id );

@@ -54,6 +54,23 @@ B2_API int b2Body_GetShapeCount( b2BodyId bodyId );
 B2_API void b2Body_SetUserData( b2BodyId bodyId, void* userData );
 B2_API void* b2Body_GetUserData( b2BodyId bodyId );
 B2_API b2WorldId b2Body_GetWorld( b2BodyId bodyId );
+B2_API void b2DestroyBody( b2BodyId bodyId );
+B2_API void b2Body_SetTransform( b2BodyId bodyId, b2Pos position, b2Rot rotation );
+B2_API b2Vec2 b2Body_GetLocalPoint( b2BodyId bodyId, b2Pos worldPoint );
+B2_API b2Pos b2Body_GetWorldPoint( b2BodyId bodyId, b2Vec2 localPoint );
+B2_API b2Vec2 b2Body_GetLocalVector( b2BodyId bodyId, b2Vec2 worldVector );
+B2_API b2Vec2 b2Body_GetWorldVector( b2BodyId bodyId, b2Vec2 localVector );
+B2_API void b2Body_SetTargetTransform( b2BodyId bodyId, b2WorldTransform target, float timeStep, bool wake );
+B2_API void b2Body_ApplyForce( b2BodyId bodyId, b2Vec2 force, b2Pos point, bool wake );
+B2_API void b2Body_ApplyForceToCenter( b2BodyId bodyId, b2Vec2 force, bool wake );
+B2_API void b2Body_ApplyTorque( b2BodyId bodyId, float torque, bool wake );
+B2_API void b2Body_ApplyLinearImpulse( b2BodyId bodyId, b2Vec2 impulse, b2Pos point, bool wake );
+B2_API void b2Body_ApplyLinearImpulseToCenter( b2BodyId bodyId, b2Vec2 impulse, bool wake );
+B2_API void b2Body_ApplyAngularImpulse( b2BodyId bodyId, float impulse, bool wake );
+B2_API void b2Body_ApplyMassFromShapes( b2BodyId bodyId );
+B2_API int b2Body_GetShapes( b2BodyId bodyId, b2ShapeId* shapeArray, int capacity );
+B2_API int b2Body_GetJointCount( b2BodyId bodyId );
+B2_API int b2Body_GetJoints( b2BodyId bodyId, b2JointId* jointArray, int capacity );
 
 /// Shapes (reference: include/box2d/box2d.h:659-908)
 B2_API b2ShapeId b2CreateCircleShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Circle* circle );
@@ -64,6 +81,16 @@ B2_API bool b2Shape_IsValid( b2ShapeId id );
 B2_API b2ShapeType b2Shape_GetType( b2ShapeId shapeId );
 B2_API b2BodyId b2Shape_GetBody( b2ShapeId shapeId );
 B2_API b2AABB b2Shape_GetAABB( b2ShapeId shapeId );
+B2_API void b2Shape_EnableSensorEvents( b2ShapeId shapeId, bool flag );
+B2_API bool b2Shape_AreSensorEventsEnabled( b2ShapeId shapeId );
+B2_API b2Circle b2Shape_GetCircle( b2ShapeId shapeId );
+B2_API b2Segment b2Shape_GetSegment( b2ShapeId shapeId );
+B2_API b2Capsule b2Shape_GetCapsule( b2ShapeId shapeId );
+B2_API b2Polygon b2Shape_GetPolygon( b2ShapeId shapeId );
+B2_API void b2Shape_SetCircle( b2ShapeId shapeId, const b2Circle* circle );
+B2_API void b2Shape_SetCapsule( b2ShapeId shapeId, const b2Capsule* capsule );
+B2_API void b2Shape_SetSegment( b2ShapeId shapeId, const b2Segment* segment );
+B2_API void b2Shape_SetPolygon( b2ShapeId shapeId, const b2Polygon* polygon );
 B2_API b2ChainId b2CreateChain( b2BodyId bodyId, const b2ChainDef* def );
 
 /// Joints (reference: include/box2d/box2d.h:919-1120)
@@ -80,3 +107,31 @@ B2_API bool b2Joint_IsValid( b2JointId id );
 B2_API b2JointType b2Joint_GetType( b2JointId jointId );
 B2_API b2BodyId b2Joint_GetBodyA( b2JointId jointId );
 B2_API b2BodyId b2Joint_GetBodyB( b2JointId jointId );
+B2_API void b2DestroyJoint( b2JointId jointId );
+B2_API void b2Joint_SetLocalFrameA( b2JointId jointId, b2Transform localFrame );
+B2_API b2Transform b2Joint_GetLocalFrameA( b2JointId jointId );
+B2_API void b2Joint_SetLocalFrameB( b2JointId jointId, b2Transform localFrame );
+B2_API b2Transform b2Joint_GetLocalFrameB( b2JointId jointId );
+
+/// Revolute joint (reference: include/box2d/box2d.h:1336-1398)
+B2_API void b2RevoluteJoint_EnableSpring( b2JointId jointId, bool enableSpring );
+B2_API bool b2RevoluteJoint_IsSpringEnabled( b2JointId jointId );
+B2_API void b2RevoluteJoint_SetSpringHertz( b2JointId jointId, float hertz );
+B2_API float b2RevoluteJoint_GetSpringHertz( b2JointId jointId );
+B2_API void b2RevoluteJoint_SetSpringDampingRatio( b2JointId jointId, float dampingRatio );
+B2_API float b2RevoluteJoint_GetSpringDampingRatio( b2JointId jointId );
+B2_API void b2RevoluteJoint_SetTargetAngle( b2JointId jointId, float angle );
+B2_API float b2RevoluteJoint_GetTargetAngle( b2JointId jointId );
+B2_API float b2RevoluteJoint_GetAngle( b2JointId jointId );
+B2_API void b2RevoluteJoint_EnableLimit( b2JointId jointId, bool enableLimit );
+B2_API bool b2RevoluteJoint_IsLimitEnabled( b2JointId jointId );
+B2_API float b2RevoluteJoint_GetLowerLimit( b2JointId jointId );
+B2_API float b2RevoluteJoint_GetUpperLimit( b2JointId jointId );
+B2_API void b2RevoluteJoint_SetLimits( b2JointId jointId, float lower, float upper );
+B2_API void b2RevoluteJoint_EnableMotor( b2JointId jointId, bool enableMotor );
+B2_API bool b2RevoluteJoint_IsMotorEnabled( b2JointId jointId );
+B2_API void b2RevoluteJoint_SetMotorSpeed( b2JointId jointId, float motorSpeed );
+B2_API float b2RevoluteJoint_GetMotorSpeed( b2JointId jointId );
+B2_API float b2RevoluteJoint_GetMotorTorque( b2JointId jointId );
+B2_API void b2RevoluteJoint_SetMaxMotorTorque( b2JointId jointId, float torque );
+B2_API float b2RevoluteJoint_GetMaxMotorTorque( b2JointId jointId );

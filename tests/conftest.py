@@ -27,6 +27,11 @@ def world_runs():
 
 
 @pytest.fixture(scope="session")
+def bench_scenes_golden():
+    return {s["name"]: s for s in load_golden("bench_scenes.json")["scenes"]}
+
+
+@pytest.fixture(scope="session")
 def gpu_scenes():
     import box2d_amd
 
