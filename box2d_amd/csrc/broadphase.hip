@@ -15,6 +15,12 @@
 // (kinematic tree, static tree, dynamic tree, each depth first), so the sorted keys list every
 // query's candidates in the order the reference's queries visit them; the host applies the
 // remaining filters in that order.
+//
+// A query box much larger than the typical proxy (a rotating container wall, a spinner arm)
+// overlaps most of the tree, and one lane walking thousands of nodes would stall the whole query.
+// Queries whose box perimeter exceeds kLargeQueryFactor times the mean proxy perimeter are
+// therefore not walked: one lane per proxy tests every such query instead. Both paths apply the
+// same rules and emit the same keys, so the sorted result does not depend on the split.
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -42,6 +48,12 @@ namespace b2g
 
 constexpr int kLbvhBlock = 256;
 constexpr int kLbvhChunk = 64; // leaves per chunk box (one wavefront)
+constexpr float kLargeQueryFactor = 8.0f;
+
+__device__ __forceinline__ float Perimeter( const b2AABB& a )
+{
+	return ( a.upperBound.x - a.lowerBound.x ) + ( a.upperBound.y - a.lowerBound.y );
+}
 
 __device__ __forceinline__ bool Overlaps( const b2AABB& a, const b2AABB& b )
 {
@@ -81,11 +93,12 @@ __device__ __forceinline__ b2AABB WaveUnion( b2AABB a )
 	return a;
 }
 
-// Scene bounds of the box centers (one workgroup).
+// Scene bounds of the box centers and the large-query perimeter threshold (one workgroup):
+// bounds[0..3] = center bounds, bounds[4] = kLargeQueryFactor x mean proxy perimeter.
 __global__ __launch_bounds__( 1024 ) void LbvhBoundsKernel( const LbvhProxy* proxies, int n, float* bounds )
 {
-	__shared__ float s[4][1024];
-	float lx = FLT_MAX, ly = FLT_MAX, ux = -FLT_MAX, uy = -FLT_MAX;
+	__shared__ float s[5][1024];
+	float lx = FLT_MAX, ly = FLT_MAX, ux = -FLT_MAX, uy = -FLT_MAX, per = 0.0f;
 	for ( int i = threadIdx.x; i < n; i += blockDim.x )
 	{
 		b2AABB a = proxies[i].aabb;
@@ -95,11 +108,13 @@ __global__ __launch_bounds__( 1024 ) void LbvhBoundsKernel( const LbvhProxy* pro
 		ly = fminf( ly, cy );
 		ux = fmaxf( ux, cx );
 		uy = fmaxf( uy, cy );
+		per += Perimeter( a );
 	}
 	s[0][threadIdx.x] = lx;
 	s[1][threadIdx.x] = ly;
 	s[2][threadIdx.x] = ux;
 	s[3][threadIdx.x] = uy;
+	s[4][threadIdx.x] = per;
 	__syncthreads();
 	for ( int w = blockDim.x / 2; w > 0; w >>= 1 )
 	{
@@ -109,12 +124,17 @@ __global__ __launch_bounds__( 1024 ) void LbvhBoundsKernel( const LbvhProxy* pro
 			s[1][threadIdx.x] = fminf( s[1][threadIdx.x], s[1][threadIdx.x + w] );
 			s[2][threadIdx.x] = fmaxf( s[2][threadIdx.x], s[2][threadIdx.x + w] );
 			s[3][threadIdx.x] = fmaxf( s[3][threadIdx.x], s[3][threadIdx.x + w] );
+			s[4][threadIdx.x] = s[4][threadIdx.x] + s[4][threadIdx.x + w];
 		}
 		__syncthreads();
 	}
 	if ( threadIdx.x < 4 )
 	{
 		bounds[threadIdx.x] = s[threadIdx.x][0];
+	}
+	if ( threadIdx.x == 4 )
+	{
+		bounds[4] = kLargeQueryFactor * s[4][0] / (float)n;
 	}
 }
 
@@ -325,8 +345,38 @@ struct LbvhQueryArgs
 	const uint8_t* moved; // by shape id: the shape's proxy is in the move buffer
 	uint64_t* out;		  // (query << 32 | proxy index)
 	int capacity;
-	int* counters; // [0] candidates, [1] walks cut off by the step guard
+	int* counters; // [0] candidates, [1] walks cut off by the step guard, [2] large queries
+	const float* bounds; // [4]: large-query perimeter threshold
+	uint8_t* large;		 // per query: 1 if tested by LbvhLargeQueryKernel instead of walked
+	int* largeList;		 // the large queries' indices (counters[2] of them)
+	const LbvhProxy* proxies;
+	int proxyCount;
 };
+
+// Appends a hit's key with one counter atomic per wavefront for all of its lanes' hits.
+__device__ __forceinline__ void EmitHits( const LbvhQueryArgs& a, bool hit, uint64_t key )
+{
+	const int lane = threadIdx.x & 63;
+	uint64_t mask = __ballot( hit );
+	if ( mask != 0 )
+	{
+		int leader = __ffsll( (unsigned long long)mask ) - 1;
+		int base = 0;
+		if ( lane == leader )
+		{
+			base = atomicAdd( a.counters, __popcll( mask ) );
+		}
+		base = __shfl( base, leader, 64 );
+		if ( hit )
+		{
+			int slot = base + __popcll( mask & ( ( 1ull << lane ) - 1ull ) );
+			if ( slot < a.capacity )
+			{
+				a.out[slot] = key;
+			}
+		}
+	}
+}
 
 // Hit rules of b2PairQueryCallback that do not depend on host state (src/broad_phase.c:172-216).
 __device__ __forceinline__ bool LbvhKeep( const LbvhQueryArgs& a, int queryKey, int hitKey, int hitShape )
@@ -364,9 +414,12 @@ __global__ __launch_bounds__( kLbvhBlock ) void LbvhQueryKernel( LbvhQueryArgs a
 	{
 		return;
 	}
+	if ( a.large[q] )
+	{
+		return;
+	}
 	const LbvhProxy query = a.queries[q];
 	const b2AABB box = query.aabb;
-	const int lane = threadIdx.x & 63;
 	int node = 0;
 	int steps = 0;
 	while ( node >= 0 )
@@ -390,27 +443,57 @@ __global__ __launch_bounds__( kLbvhBlock ) void LbvhQueryKernel( LbvhQueryArgs a
 				next = e.child;
 			}
 		}
-		// one counter atomic per wavefront for all of its lanes' hits of this step
-		uint64_t mask = __ballot( hit );
-		if ( mask != 0 )
-		{
-			int leader = __ffsll( (unsigned long long)mask ) - 1;
-			int base = 0;
-			if ( lane == leader )
-			{
-				base = atomicAdd( a.counters, __popcll( mask ) );
-			}
-			base = __shfl( base, leader, 64 );
-			if ( hit )
-			{
-				int slot = base + __popcll( mask & ( ( 1ull << lane ) - 1ull ) );
-				if ( slot < a.capacity )
-				{
-					a.out[slot] = ( (uint64_t)q << 32 ) | (uint32_t)( ~e.child );
-				}
-			}
-		}
+		EmitHits( a, hit, ( (uint64_t)q << 32 ) | (uint32_t)( ~e.child ) );
 		node = next;
+	}
+}
+
+// Marks and lists the queries whose box is too large to walk.
+__global__ __launch_bounds__( kLbvhBlock ) void LbvhLargeListKernel( LbvhQueryArgs a )
+{
+	int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( q >= a.queryCount )
+	{
+		return;
+	}
+	bool large = Perimeter( a.queries[q].aabb ) > a.bounds[4];
+	a.large[q] = large ? 1 : 0;
+	if ( large )
+	{
+		a.largeList[atomicAdd( a.counters + 2, 1 )] = q;
+	}
+}
+
+// Large queries, one lane per proxy: every proxy is tested against every large query box, with
+// the walk's rules and keys.
+__global__ __launch_bounds__( kLbvhBlock ) void LbvhLargeQueryKernel( LbvhQueryArgs a )
+{
+	__shared__ LbvhProxy sq[kLbvhBlock];
+	__shared__ int sIndex[kLbvhBlock];
+	int p = blockIdx.x * blockDim.x + threadIdx.x;
+	int largeCount = a.counters[2];
+	LbvhProxy proxy = {};
+	if ( p < a.proxyCount )
+	{
+		proxy = a.proxies[p];
+	}
+	for ( int base = 0; base < largeCount; base += kLbvhBlock )
+	{
+		int m = min( kLbvhBlock, largeCount - base );
+		__syncthreads();
+		if ( (int)threadIdx.x < m )
+		{
+			int q = a.largeList[base + threadIdx.x];
+			sIndex[threadIdx.x] = q;
+			sq[threadIdx.x] = a.queries[q];
+		}
+		__syncthreads();
+		for ( int j = 0; j < m; ++j )
+		{
+			bool hit = p < a.proxyCount && Overlaps( proxy.aabb, sq[j].aabb ) &&
+					   LbvhKeep( a, sq[j].proxyKey, proxy.proxyKey, proxy.shapeId );
+			EmitHits( a, hit, ( (uint64_t)sIndex[j] << 32 ) | (uint32_t)p );
+		}
 	}
 }
 
@@ -423,6 +506,8 @@ struct PairFinder::Impl
 	size_t proxyCap = 0, queryCap = 0, candCap = 0;
 	LbvhProxy* proxies = nullptr;
 	LbvhProxy* queries = nullptr;
+	uint8_t* large = nullptr; // per query
+	int* largeList = nullptr;
 	uint8_t* moved = nullptr;
 	size_t movedCap = 0;
 	uint32_t* codes = nullptr;
@@ -513,8 +598,8 @@ PairFinder::PairFinder()
 {
 	impl = new Impl();
 	HIP_CHECK( hipStreamCreate( &impl->stream ) );
-	HIP_CHECK( hipMalloc( &impl->bounds, 4 * sizeof( float ) ) );
-	HIP_CHECK( hipMalloc( &impl->counters, 2 * sizeof( int ) ) );
+	HIP_CHECK( hipMalloc( &impl->bounds, 5 * sizeof( float ) ) );
+	HIP_CHECK( hipMalloc( &impl->counters, 3 * sizeof( int ) ) );
 	HIP_CHECK( hipHostMalloc( &impl->hostCounters, 2 * sizeof( int ), hipHostMallocDefault ) );
 }
 
@@ -523,6 +608,8 @@ PairFinder::~PairFinder()
 	Impl* m = impl;
 	hipFree( m->proxies );
 	hipFree( m->queries );
+	hipFree( m->large );
+	hipFree( m->largeList );
 	hipFree( m->moved );
 	hipFree( m->codes );
 	hipFree( m->sortedCodes );
@@ -576,8 +663,12 @@ void PairFinder::Launch( int n, int nq, int movedCount )
 	if ( (size_t)nq > m->queryCap )
 	{
 		hipFree( m->queries );
+		hipFree( m->large );
+		hipFree( m->largeList );
 		m->queryCap = std::max<size_t>( 256, nq + nq / 2 );
 		HIP_CHECK( hipMalloc( &m->queries, m->queryCap * sizeof( LbvhProxy ) ) );
+		HIP_CHECK( hipMalloc( &m->large, m->queryCap ) );
+		HIP_CHECK( hipMalloc( &m->largeList, m->queryCap * sizeof( int ) ) );
 	}
 	if ( (size_t)movedCount > m->movedCap )
 	{
@@ -628,8 +719,16 @@ void PairFinder::Impl::Query()
 	qa.out = cands;
 	qa.capacity = (int)candCap;
 	qa.counters = counters;
-	HIP_CHECK( hipMemsetAsync( counters, 0, 2 * sizeof( int ), stream ) );
-	LbvhQueryKernel<<<( launchedQueries + kLbvhBlock - 1 ) / kLbvhBlock, kLbvhBlock, 0, stream>>>( qa );
+	qa.bounds = bounds;
+	qa.large = large;
+	qa.largeList = largeList;
+	qa.proxies = proxies;
+	qa.proxyCount = launchedProxies;
+	HIP_CHECK( hipMemsetAsync( counters, 0, 3 * sizeof( int ), stream ) );
+	int queryBlocks = ( launchedQueries + kLbvhBlock - 1 ) / kLbvhBlock;
+	LbvhLargeListKernel<<<queryBlocks, kLbvhBlock, 0, stream>>>( qa );
+	LbvhQueryKernel<<<queryBlocks, kLbvhBlock, 0, stream>>>( qa );
+	LbvhLargeQueryKernel<<<( launchedProxies + kLbvhBlock - 1 ) / kLbvhBlock, kLbvhBlock, 0, stream>>>( qa );
 	HIP_CHECK( hipGetLastError() );
 	HIP_CHECK( hipMemcpyAsync( hostCounters, counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, stream ) );
 }

@@ -104,8 +104,9 @@ void bs_destroy( void )
 	}
 }
 
-// Builds scene `index` in a fresh world. Returns the world id packed as index1 | generation << 16.
-uint32_t bs_create( int index )
+// Builds scene `index` in a fresh world stepped by `workerCount` threads (the reference's own
+// scheduler; box2d_amd ignores it). Returns the world id packed as index1 | generation << 16.
+uint32_t bs_create_workers( int index, int workerCount )
 {
 	bs_destroy();
 	if ( index < 0 || index >= BS_SCENE_COUNT )
@@ -114,11 +115,36 @@ uint32_t bs_create( int index )
 	}
 	b2WorldDef worldDef = b2DefaultWorldDef();
 	worldDef.enableContinuous = true;
+	worldDef.workerCount = workerCount;
 	s_worldId = b2CreateWorld( &worldDef );
 	s_scenes[index].createFcn( s_worldId );
 	s_scene = index;
 	s_stepIndex = 0;
 	return (uint32_t)s_worldId.index1 | (uint32_t)s_worldId.generation << 16;
+}
+
+uint32_t bs_create( int index )
+{
+	return bs_create_workers( index, 1 );
+}
+
+// Runs `count` benchmark steps with no reduction (the timed loop of benchmark/main.c:343-352).
+void bs_run( int count )
+{
+	if ( s_scene < 0 )
+	{
+		return;
+	}
+	const BenchScene* scene = s_scenes + s_scene;
+	for ( int i = 0; i < count; ++i )
+	{
+		if ( scene->stepFcn != NULL )
+		{
+			scene->stepFcn( s_worldId, s_stepIndex );
+		}
+		b2World_Step( s_worldId, 1.0f / 60.0f, 4 );
+		s_stepIndex += 1;
+	}
 }
 
 // Runs one benchmark step and writes {move count, awake body count, sorted move-event hash}.

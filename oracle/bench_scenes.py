@@ -20,16 +20,21 @@ class BenchScenes:
         self.lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
         self.lib.bs_scene_name.restype = ctypes.c_char_p
         self.lib.bs_create.restype = ctypes.c_uint32
+        self.lib.bs_create_workers.restype = ctypes.c_uint32
         self._out = (ctypes.c_uint32 * 3)()
         self.names = [self.lib.bs_scene_name(i).decode() for i in range(self.lib.bs_scene_count())]
 
     def steps(self, name):
         return self.lib.bs_scene_steps(self.names.index(name))
 
-    def create(self, name):
+    def create(self, name, workers=1):
         """Builds the scene in a fresh world; returns (index1, generation) of the world id."""
-        packed = self.lib.bs_create(self.names.index(name))
+        packed = self.lib.bs_create_workers(self.names.index(name), workers)
         return packed & 0xFFFF, packed >> 16
+
+    def run(self, count):
+        """count benchmark steps with no per-step reduction (for timing)."""
+        self.lib.bs_run(count)
 
     def step(self):
         """One benchmark step; returns (move count, awake body count, sorted move-event hash)."""
