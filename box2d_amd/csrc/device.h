@@ -38,7 +38,22 @@ struct SolverLists
 	// the contact's total normal impulse. Only dynamic bodies have entries.
 	std::vector<int> warmStart;
 	std::vector<int> warmList;
+
+	// Level schedule of the overflow contacts (empty when there are none). The overflow joints
+	// come first in every stage and are walked in order before the contacts. The overflow sequence is cut into chunks of at most
+	// kOverflowChunkContacts consecutive contacts; within a chunk each contact's level is one more
+	// than the last level that touched either of its dynamic bodies, so contacts of one level share
+	// no dynamic body and every body still sees its contacts in sequence order. Layout (ints):
+	// [0] chunk count, [1] bodies offset, [2] slots offset, [3] entries offset, [4] level bounds
+	// offset, then per chunk {contact begin, contact end, body begin, body end, level begin, level
+	// end}; bodies: each chunk's distinct body ids; slots: per contact the chunk-local slots of its
+	// bodies A and B; entries: per chunk its contacts (chunk-local index) sorted by level; level
+	// bounds: per chunk the entry positions where each level starts, plus the end.
+	std::vector<int> overflowPlan;
 };
+
+constexpr int kOverflowChunkContacts = 512;
+constexpr int kOverflowChunkBodies = 2 * kOverflowChunkContacts;
 
 // Results the host needs after the collide stage.
 struct CollideResult
@@ -118,7 +133,7 @@ public:
 	void SetSolverPath( int path );
 	// Whether the next solve of a world with these list sizes takes the color-kernel path with the
 	// warm start body by body (the host then builds the per-body warm-start lists).
-	bool WantsWarmLists( int bodyCount, int constraintCount, int activeColorCount ) const;
+	bool WantsWarmLists( int bodyCount, int constraintCount, int activeColorCount, int overflowCount ) const;
 
 	// Step recording for the adjoint: while a tape is set, every solve appends one record (body
 	// state at the step's start, constraints in solve order) to the tape. index is the world's

@@ -955,6 +955,137 @@ __global__ void OverflowKernel( ColorArgs a )
 	OverflowSerial<Stage>( a, r, GlobalBodies{ a.states } );
 }
 
+// Overflow constraints by level (SolverLists::overflowPlan), one workgroup. The overflow joints go
+// first, walked in order by one lane as in OverflowSerial. Then each chunk's constraint
+// rows and body states are copied into LDS, the levels run one after another with a workgroup
+// barrier between them (a level's contacts share no dynamic body), and the rows and dynamic
+// velocities go back to device memory. The per-contact arithmetic is OverflowSerial's; only
+// contacts that touch disjoint state run at the same time.
+constexpr int kOverflowThreads = 256;
+
+constexpr size_t OverflowLdsBytes()
+{
+	return (size_t)kOverflowChunkContacts * ( kcFieldCount * sizeof( float ) + 4 * sizeof( int ) ) +
+		   (size_t)kOverflowChunkBodies * ( 7 * sizeof( float ) + sizeof( uint16_t ) );
+}
+
+static_assert( OverflowLdsBytes() <= 160 * 1024, "overflow chunk must fit in LDS" );
+
+template <int Stage>
+__global__ __launch_bounds__( kOverflowThreads ) void OverflowLevelKernel( ColorArgs a, const int* plan )
+{
+	extern __shared__ float lds[];
+	const int C = kOverflowChunkContacts;
+	const int NB = kOverflowChunkBodies;
+	float* fields = lds;
+	int* ints = reinterpret_cast<int*>( fields + kcFieldCount * C );
+	float* bf = reinterpret_cast<float*>( ints + 4 * C );
+	uint16_t* bflags = reinterpret_cast<uint16_t*>( bf + 7 * NB );
+
+	ConstraintSoA lc = a.c;
+	lc.f = FieldTable{ fields, C };
+	lc.bodyA = ints;
+	lc.bodyB = ints + C;
+	lc.pointCount = ints + 2 * C;
+	lc.contactId = ints + 3 * C;
+	const LdsBodies B{ bf, bf + NB, bf + 2 * NB, bf + 3 * NB, bf + 4 * NB, bf + 5 * NB, bf + 6 * NB, bflags };
+	const GlobalBodies G{ a.states };
+
+	if ( a.jointCount > 0 && Stage != kStageRestitution )
+	{
+		if ( threadIdx.x == 0 )
+		{
+			ColorRange r = { a.jointStart, a.jointCount, a.contactStart, 0 };
+			OverflowSerial<Stage>( a, r, G );
+		}
+		__syncthreads();
+	}
+
+	const int chunkCount = plan[0];
+	const int* bodies = plan + plan[1];
+	const int* slots = plan + plan[2];
+	const int* entries = plan + plan[3];
+	const int* levels = plan + plan[4];
+	const int tid = threadIdx.x;
+
+	for ( int ch = 0; ch < chunkCount; ++ch )
+	{
+		const int* t = plan + 5 + 6 * ch;
+		int c0 = t[0], c1 = t[1], b0 = t[2], b1 = t[3], l0 = t[4], l1 = t[5];
+		int n = c1 - c0;
+		int first = a.contactStart + c0;
+
+		for ( int x = tid; x < kcFieldCount * n; x += kOverflowThreads )
+		{
+			int k = x / n;
+			int j = x - k * n;
+			fields[k * C + j] = a.c.f[k][first + j];
+		}
+		for ( int j = tid; j < n; j += kOverflowThreads )
+		{
+			ints[j] = slots[2 * ( c0 + j )];
+			ints[C + j] = slots[2 * ( c0 + j ) + 1];
+			ints[2 * C + j] = a.c.pointCount[first + j];
+			ints[3 * C + j] = a.c.contactId[first + j];
+		}
+		for ( int j = tid; j < b1 - b0; j += kOverflowThreads )
+		{
+			LaneBody lb = G.Load( bodies[b0 + j] );
+			B.vx[j] = lb.vx;
+			B.vy[j] = lb.vy;
+			B.w[j] = lb.w;
+			B.dpx[j] = lb.dpx;
+			B.dpy[j] = lb.dpy;
+			B.dqc[j] = lb.dqc;
+			B.dqs[j] = lb.dqs;
+			B.flags[j] = (uint16_t)lb.flags;
+		}
+		__syncthreads();
+
+		for ( int l = l0; l < l1; ++l )
+		{
+			for ( int e = levels[l] + tid; e < levels[l + 1]; e += kOverflowThreads )
+			{
+				int j = entries[e];
+				if ( Stage == kStageWarmStart )
+				{
+					OverflowWarmStart( lc, j, B );
+				}
+				else if ( Stage == kStageSolve )
+				{
+					OverflowSolve( lc, j, B, true, a.p.inv_h, a.p.contactSpeed );
+				}
+				else if ( Stage == kStageRelax )
+				{
+					OverflowSolve( lc, j, B, false, a.p.inv_h, a.p.contactSpeed );
+				}
+				else
+				{
+					OverflowRestitution( lc, j, B, a.p.restitutionThreshold );
+				}
+			}
+			__syncthreads();
+		}
+
+		for ( int x = tid; x < kcFieldCount * n; x += kOverflowThreads )
+		{
+			int k = x / n;
+			int j = x - k * n;
+			a.c.f[k][first + j] = fields[k * C + j];
+		}
+		for ( int j = tid; j < b1 - b0; j += kOverflowThreads )
+		{
+			LaneBody lb;
+			lb.vx = B.vx[j];
+			lb.vy = B.vy[j];
+			lb.w = B.w[j];
+			lb.flags = B.flags[j];
+			G.Store( bodies[b0 + j], lb );
+		}
+		__syncthreads();
+	}
+}
+
 // ---------------------------------------------------------------------------------------------
 // Fused solver: the whole substep loop of one world in one workgroup.
 //
@@ -966,6 +1097,14 @@ __global__ void OverflowKernel( ColorArgs a )
 
 constexpr int kFusedThreads = 1024;
 constexpr int kFusedMaxContacts = 4096;
+constexpr int kFusedMaxOverflow = 8;
+
+// Automatic solver path choice: the fused solver for worlds with at most kFusedMaxContacts
+// contact constraints and at most kFusedMaxOverflow overflow constraints.
+static bool AutoFused( int constraintCount, int overflowCount )
+{
+	return constraintCount <= kFusedMaxContacts && overflowCount <= kFusedMaxOverflow;
+}
 constexpr int kMaxColorSlots = 24;
 
 struct FusedArgs
@@ -2239,6 +2378,7 @@ struct DeviceWorld::Impl
 	DevArray<int> jointList;
 	DevArray<int> warmStart; // per awake body: its range of warmList (warm start by body)
 	DevArray<int> warmList;
+	DevArray<int> overflowPlan;
 	DevArray<int> changes;
 	SolverLists lists;
 	std::vector<int> slotStaging;
@@ -2465,6 +2605,14 @@ static void EnableLargeLds()
 										hipFuncAttributeMaxDynamicSharedMemorySize, FusedLdsBytes( kFusedBodyCapacity ) ) );
 		HIP_CHECK( hipFuncSetAttribute( reinterpret_cast<const void*>( SolveFusedBatchKernel ),
 										hipFuncAttributeMaxDynamicSharedMemorySize, FusedLdsBytes( kFusedBodyCapacity ) ) );
+		const void* overflowKernels[4] = { reinterpret_cast<const void*>( OverflowLevelKernel<kStageWarmStart> ),
+										   reinterpret_cast<const void*>( OverflowLevelKernel<kStageSolve> ),
+										   reinterpret_cast<const void*>( OverflowLevelKernel<kStageRelax> ),
+										   reinterpret_cast<const void*>( OverflowLevelKernel<kStageRestitution> ) };
+		for ( const void* k : overflowKernels )
+		{
+			HIP_CHECK( hipFuncSetAttribute( k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)OverflowLdsBytes() ) );
+		}
 		done = true;
 	}
 }
@@ -2723,6 +2871,7 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 	impl->StageVector( impl->jointList, lists.jointList );
 	impl->StageVector( impl->warmStart, lists.warmStart );
 	impl->StageVector( impl->warmList, lists.warmList );
+	impl->StageVector( impl->overflowPlan, lists.overflowPlan );
 
 	size_t nc = lists.constraintContacts.size();
 	impl->EnsureConstraints( std::max<size_t>( nc, 1 ) );
@@ -2954,7 +3103,29 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 			ca.contactStart = L.overflowContactStart;
 			ca.contactCount = L.overflowContactCount;
 			double bytes = stageBytes( stage ) * ca.contactCount + kBytesJoint * ca.jointCount;
+			const int* plan = impl->overflowPlan.ptr;
 			impl->Launch( kkOverflow, bytes, [&]( hipEvent_t e0, hipEvent_t e1 ) {
+				if ( L.overflowPlan.empty() == false )
+				{
+					const dim3 g( 1 ), b( kOverflowThreads );
+					const size_t lds = OverflowLdsBytes();
+					switch ( stage )
+					{
+						case kStageWarmStart:
+							hipExtLaunchKernelGGL( OverflowLevelKernel<kStageWarmStart>, g, b, lds, s, e0, e1, 0, ca, plan );
+							break;
+						case kStageSolve:
+							hipExtLaunchKernelGGL( OverflowLevelKernel<kStageSolve>, g, b, lds, s, e0, e1, 0, ca, plan );
+							break;
+						case kStageRelax:
+							hipExtLaunchKernelGGL( OverflowLevelKernel<kStageRelax>, g, b, lds, s, e0, e1, 0, ca, plan );
+							break;
+						default:
+							hipExtLaunchKernelGGL( OverflowLevelKernel<kStageRestitution>, g, b, lds, s, e0, e1, 0, ca, plan );
+							break;
+					}
+					return;
+				}
 				switch ( stage )
 				{
 					case kStageWarmStart:
@@ -3070,9 +3241,10 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	};
 
 	// The fused solver runs on one compute unit: it wins while the per-color launch latency of the
-	// color kernels dominates, and loses once a world has enough constraints to need the whole GPU.
+	// color kernels dominates, and loses once a world has enough constraints to need the whole GPU
+	// or overflow constraints that its single lane walks from device memory.
 	bool fits = nb > 0 && nb <= kFusedBodyCapacity && L.activeColorCount <= kMaxColorSlots;
-	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && nc <= kFusedMaxContacts ) );
+	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && AutoFused( nc, L.overflowContactCount + L.overflowJointCount ) ) );
 	// per-body warm-start lists exist when the host expected this path (see WantsWarmLists)
 	bool haveWarmLists = (int)L.warmStart.size() == nb + 1;
 
@@ -3465,14 +3637,14 @@ void DeviceWorld::SetSolverPath( int path )
 	impl->solverPath = path;
 }
 
-bool DeviceWorld::WantsWarmLists( int bodyCount, int constraintCount, int activeColorCount ) const
+bool DeviceWorld::WantsWarmLists( int bodyCount, int constraintCount, int activeColorCount, int overflowCount ) const
 {
 	if ( bodyCount == 0 )
 	{
 		return false;
 	}
 	bool fits = bodyCount <= kFusedBodyCapacity && activeColorCount <= kMaxColorSlots;
-	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && constraintCount <= kFusedMaxContacts ) );
+	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && AutoFused( constraintCount, overflowCount ) ) );
 	fused = fused || impl->tape != nullptr || impl->recording;
 	return WarmByBody( fused );
 }

@@ -2975,6 +2975,99 @@ static Softness HostMakeSoft( float hertz, float zeta, float h )
 	return Softness{ omega / a1, a2 * a3, a3 };
 }
 
+// Level schedule of the overflow contacts (SolverLists::overflowPlan). The reference solves the
+// overflow constraints one after another on one thread (src/solver.c:1100-1168); two contacts that
+// share no dynamic body touch disjoint state (static and kinematic states are never written), so
+// running a level's contacts together and the levels in order gives every body the same sequence
+// of updates and the same floats.
+static void BuildOverflowPlan( World& w, SolverLists& L )
+{
+	L.overflowPlan.clear();
+	int n = L.overflowContactCount;
+	if ( n == 0 )
+	{
+		return;
+	}
+	int chunkCount = ( n + kOverflowChunkContacts - 1 ) / kOverflowChunkContacts;
+	std::vector<int> table, bodyList, slots( 2 * n ), entries( n ), levelBounds;
+	std::vector<int> stamp( w.bodies.size(), -1 ), slotOf( w.bodies.size() ), last( w.bodies.size() );
+	std::vector<int> contactLevel( kOverflowChunkContacts ), levelCount;
+	for ( int ch = 0; ch < chunkCount; ++ch )
+	{
+		int c0 = ch * kOverflowChunkContacts;
+		int c1 = std::min( n, c0 + kOverflowChunkContacts );
+		int b0 = (int)bodyList.size();
+		int maxLevel = 0;
+		for ( int k = c0; k < c1; ++k )
+		{
+			const Contact& contact = w.contacts[L.constraintContacts[L.overflowContactStart + k]];
+			int level = 0;
+			for ( int side = 0; side < 2; ++side )
+			{
+				int id = contact.edges[side].bodyId;
+				if ( stamp[id] != ch )
+				{
+					stamp[id] = ch;
+					slotOf[id] = (int)bodyList.size() - b0;
+					bodyList.push_back( id );
+					last[id] = -1;
+				}
+				slots[2 * k + side] = slotOf[id];
+				if ( w.bodies[id].type == b2_dynamicBody )
+				{
+					level = std::max( level, last[id] + 1 );
+				}
+			}
+			for ( int side = 0; side < 2; ++side )
+			{
+				int id = contact.edges[side].bodyId;
+				if ( w.bodies[id].type == b2_dynamicBody )
+				{
+					last[id] = level;
+				}
+			}
+			contactLevel[k - c0] = level;
+			maxLevel = std::max( maxLevel, level );
+		}
+
+		// stable counting sort of the chunk's contacts by level
+		levelCount.assign( maxLevel + 2, 0 );
+		for ( int k = c0; k < c1; ++k )
+		{
+			levelCount[contactLevel[k - c0] + 1] += 1;
+		}
+		for ( int l = 0; l <= maxLevel; ++l )
+		{
+			levelCount[l + 1] += levelCount[l];
+		}
+		int levelBegin = (int)levelBounds.size();
+		for ( int l = 0; l <= maxLevel + 1; ++l )
+		{
+			levelBounds.push_back( c0 + levelCount[l] );
+		}
+		for ( int k = c0; k < c1; ++k )
+		{
+			entries[c0 + levelCount[contactLevel[k - c0]]++] = k - c0;
+		}
+		int b1 = (int)bodyList.size();
+		int levelEnd = (int)levelBounds.size() - 1;
+		table.insert( table.end(), { c0, c1, b0, b1, levelBegin, levelEnd } );
+	}
+
+	std::vector<int>& plan = L.overflowPlan;
+	plan.assign( 5, 0 );
+	plan[0] = chunkCount;
+	plan.insert( plan.end(), table.begin(), table.end() );
+	plan[1] = (int)plan.size();
+	plan.insert( plan.end(), bodyList.begin(), bodyList.end() );
+	plan[2] = (int)plan.size();
+	plan.insert( plan.end(), slots.begin(), slots.end() );
+	plan[3] = (int)plan.size();
+	plan.insert( plan.end(), entries.begin(), entries.end() );
+	plan[4] = (int)plan.size();
+	plan.insert( plan.end(), levelBounds.begin(), levelBounds.end() );
+}
+
 static void BuildSolverLists( World& w, SolverLists& L )
 {
 	L = SolverLists{};
@@ -3011,6 +3104,7 @@ static void BuildSolverLists( World& w, SolverLists& L )
 	L.overflowJointCount = (int)overflow.jointIds.size();
 	L.jointList.insert( L.jointList.end(), overflow.jointIds.begin(), overflow.jointIds.end() );
 
+	BuildOverflowPlan( w, L );
 }
 
 // Per-body warm-start lists for the color-kernel path (SolverLists::warmStart/warmList). The warm
@@ -3176,7 +3270,8 @@ bool World::StepBegin( float timeStep, int subStepCount )
 		listsStableSteps += 1;
 		int nb = (int)solverLists.awakeBodies.size();
 		if ( listsStableSteps >= 2 && solverLists.warmStart.empty() &&
-			 device->WantsWarmLists( nb, (int)solverLists.constraintContacts.size(), solverLists.activeColorCount ) )
+			 device->WantsWarmLists( nb, (int)solverLists.constraintContacts.size(), solverLists.activeColorCount,
+									 solverLists.overflowContactCount + solverLists.overflowJointCount ) )
 		{
 			BuildWarmLists( *this, solverLists );
 			device->UploadWarmLists( solverLists );
