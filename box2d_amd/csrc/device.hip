@@ -965,7 +965,7 @@ constexpr int kOverflowThreads = 256;
 
 constexpr size_t OverflowLdsBytes()
 {
-	return (size_t)kOverflowChunkContacts * ( kcFieldCount * sizeof( float ) + 4 * sizeof( int ) ) +
+	return (size_t)kOverflowChunkContacts * ( kcFieldCount * sizeof( float ) + 6 * sizeof( int ) ) + sizeof( int ) +
 		   (size_t)kOverflowChunkBodies * ( 7 * sizeof( float ) + sizeof( uint16_t ) );
 }
 
@@ -979,7 +979,9 @@ __global__ __launch_bounds__( kOverflowThreads ) void OverflowLevelKernel( Color
 	const int NB = kOverflowChunkBodies;
 	float* fields = lds;
 	int* ints = reinterpret_cast<int*>( fields + kcFieldCount * C );
-	float* bf = reinterpret_cast<float*>( ints + 4 * C );
+	int* sEntries = ints + 4 * C;	  // the chunk's entries (chunk-local contact index) by level
+	int* sLevels = sEntries + C;	  // the chunk's level bounds, relative to its first entry
+	float* bf = reinterpret_cast<float*>( sLevels + C + 1 );
 	uint16_t* bflags = reinterpret_cast<uint16_t*>( bf + 7 * NB );
 
 	ConstraintSoA lc = a.c;
@@ -1027,6 +1029,11 @@ __global__ __launch_bounds__( kOverflowThreads ) void OverflowLevelKernel( Color
 			ints[C + j] = slots[2 * ( c0 + j ) + 1];
 			ints[2 * C + j] = a.c.pointCount[first + j];
 			ints[3 * C + j] = a.c.contactId[first + j];
+			sEntries[j] = entries[c0 + j];
+		}
+		for ( int l = tid; l <= l1 - l0; l += kOverflowThreads )
+		{
+			sLevels[l] = levels[l0 + l] - c0;
 		}
 		for ( int j = tid; j < b1 - b0; j += kOverflowThreads )
 		{
@@ -1042,27 +1049,53 @@ __global__ __launch_bounds__( kOverflowThreads ) void OverflowLevelKernel( Color
 		}
 		__syncthreads();
 
-		for ( int l = l0; l < l1; ++l )
-		{
-			for ( int e = levels[l] + tid; e < levels[l + 1]; e += kOverflowThreads )
+		auto solve = [&]( int j ) {
+			if ( Stage == kStageWarmStart )
 			{
-				int j = entries[e];
-				if ( Stage == kStageWarmStart )
+				OverflowWarmStart( lc, j, B );
+			}
+			else if ( Stage == kStageSolve )
+			{
+				OverflowSolve( lc, j, B, true, a.p.inv_h, a.p.contactSpeed );
+			}
+			else if ( Stage == kStageRelax )
+			{
+				OverflowSolve( lc, j, B, false, a.p.inv_h, a.p.contactSpeed );
+			}
+			else
+			{
+				OverflowRestitution( lc, j, B, a.p.restitutionThreshold );
+			}
+		};
+		const int levelCount = l1 - l0;
+		int l = 0;
+		while ( l < levelCount )
+		{
+			if ( sLevels[l + 1] - sLevels[l] == 1 )
+			{
+				// a run of one-contact levels (a chain through one body) goes to one lane, with no
+				// barrier between its levels
+				int r = l + 1;
+				while ( r < levelCount && sLevels[r + 1] - sLevels[r] == 1 )
 				{
-					OverflowWarmStart( lc, j, B );
+					++r;
 				}
-				else if ( Stage == kStageSolve )
+				if ( tid == 0 )
 				{
-					OverflowSolve( lc, j, B, true, a.p.inv_h, a.p.contactSpeed );
+					for ( int m = l; m < r; ++m )
+					{
+						solve( sEntries[sLevels[m]] );
+					}
 				}
-				else if ( Stage == kStageRelax )
+				l = r;
+			}
+			else
+			{
+				for ( int e = sLevels[l] + tid; e < sLevels[l + 1]; e += kOverflowThreads )
 				{
-					OverflowSolve( lc, j, B, false, a.p.inv_h, a.p.contactSpeed );
+					solve( sEntries[e] );
 				}
-				else
-				{
-					OverflowRestitution( lc, j, B, a.p.restitutionThreshold );
-				}
+				l += 1;
 			}
 			__syncthreads();
 		}
