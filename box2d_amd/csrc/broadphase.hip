@@ -351,7 +351,34 @@ struct LbvhQueryArgs
 	int* largeList;		 // the large queries' indices (counters[2] of them)
 	const LbvhProxy* proxies;
 	int proxyCount;
+	const uint64_t* pairSlots; // mirror of the existing-pair set (nullptr: empty)
+	uint64_t pairMask;
 };
+
+// Whether the shape pair already has a contact: the host set's probe sequence on its mirror.
+__device__ __forceinline__ bool InPairSet( const LbvhQueryArgs& a, int shapeA, int shapeB )
+{
+	if ( a.pairSlots == nullptr )
+	{
+		return false;
+	}
+	uint64_t key = shapeA < shapeB ? ( (uint64_t)shapeA << 32 ) | (uint32_t)shapeB : ( (uint64_t)shapeB << 32 ) | (uint32_t)shapeA;
+	uint64_t i = PairKeyHash( key ) & a.pairMask;
+	for ( uint64_t probes = 0; probes <= a.pairMask; ++probes )
+	{
+		uint64_t k = a.pairSlots[i];
+		if ( k == key )
+		{
+			return true;
+		}
+		if ( k == 0 )
+		{
+			return false;
+		}
+		i = ( i + 1 ) & a.pairMask;
+	}
+	return false;
+}
 
 // Appends a hit's key with one counter atomic per wavefront for all of its lanes' hits.
 __device__ __forceinline__ void EmitHits( const LbvhQueryArgs& a, bool hit, uint64_t key )
@@ -378,8 +405,9 @@ __device__ __forceinline__ void EmitHits( const LbvhQueryArgs& a, bool hit, uint
 	}
 }
 
-// Hit rules of b2PairQueryCallback that do not depend on host state (src/broad_phase.c:172-216).
-__device__ __forceinline__ bool LbvhKeep( const LbvhQueryArgs& a, int queryKey, int hitKey, int hitShape )
+// Hit rules of b2PairQueryCallback that do not depend on host state (src/broad_phase.c:172-216),
+// and its existing-pair test (:218-226) on the mirrored pair set.
+__device__ __forceinline__ bool LbvhKeep( const LbvhQueryArgs& a, int queryKey, int queryShape, int hitKey, int hitShape )
 {
 	int queryType = queryKey & 3;
 	int hitType = hitKey & 3;
@@ -402,7 +430,7 @@ __device__ __forceinline__ bool LbvhKeep( const LbvhQueryArgs& a, int queryKey, 
 			return false;
 		}
 	}
-	return true;
+	return InPairSet( a, queryShape, hitShape ) == false;
 }
 
 // One lane per query: stackless depth-first walk along the ropes. Every node is entered at most
@@ -436,7 +464,7 @@ __global__ __launch_bounds__( kLbvhBlock ) void LbvhQueryKernel( LbvhQueryArgs a
 		{
 			if ( e.child < 0 )
 			{
-				hit = LbvhKeep( a, query.proxyKey, e.proxyKey, e.shapeId );
+				hit = LbvhKeep( a, query.proxyKey, query.shapeId, e.proxyKey, e.shapeId );
 			}
 			else
 			{
@@ -491,7 +519,7 @@ __global__ __launch_bounds__( kLbvhBlock ) void LbvhLargeQueryKernel( LbvhQueryA
 		for ( int j = 0; j < m; ++j )
 		{
 			bool hit = p < a.proxyCount && Overlaps( proxy.aabb, sq[j].aabb ) &&
-					   LbvhKeep( a, sq[j].proxyKey, proxy.proxyKey, proxy.shapeId );
+					   LbvhKeep( a, sq[j].proxyKey, sq[j].shapeId, proxy.proxyKey, proxy.shapeId );
 			EmitHits( a, hit, ( (uint64_t)sIndex[j] << 32 ) | (uint32_t)p );
 		}
 	}
@@ -520,6 +548,11 @@ struct PairFinder::Impl
 	LbvhTree tree = {};
 	uint64_t* cands = nullptr;
 	uint64_t* sortedCands = nullptr;
+	uint64_t* pairSlots = nullptr; // mirror of the existing-pair set
+	size_t pairCap = 0;			   // allocated slots
+	size_t pairSize = 0;		   // slots of the mirrored table (0: empty set)
+	uint64_t* pairUpdates = nullptr; // (slot, key) pairs of a partial update
+	size_t pairUpdateCap = 0;
 	int* counters = nullptr;
 	int* hostCounters = nullptr;
 
@@ -532,6 +565,8 @@ struct PairFinder::Impl
 	size_t hostMovedCap = 0;
 	uint64_t* hostCands = nullptr;
 	size_t hostCandCap = 0;
+	uint64_t* hostPairs = nullptr; // whole table or (slot, key) pairs
+	size_t hostPairCap = 0;
 
 	// the launched query, finished by Finish
 	int launchedProxies = 0;
@@ -624,6 +659,9 @@ PairFinder::~PairFinder()
 	hipFree( m->tree.rangeHi );
 	hipFree( m->cands );
 	hipFree( m->sortedCands );
+	hipFree( m->pairSlots );
+	hipFree( m->pairUpdates );
+	hipHostFree( m->hostPairs );
 	hipFree( m->counters );
 	hipHostFree( m->hostCounters );
 	hipHostFree( m->hostProxies );
@@ -632,6 +670,61 @@ PairFinder::~PairFinder()
 	hipHostFree( m->hostCands );
 	hipStreamDestroy( m->stream );
 	delete m;
+}
+
+// Writes the (slot, key) pairs of a partial pair-set update into the mirror.
+__global__ __launch_bounds__( kLbvhBlock ) void PairSetScatterKernel( const uint64_t* updates, int count, uint64_t* slots )
+{
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( k < count )
+	{
+		slots[updates[2 * k]] = updates[2 * k + 1];
+	}
+}
+
+void PairFinder::SyncPairSet( const uint64_t* slots, size_t capacity, const uint32_t* dirty, size_t dirtyCount, bool all )
+{
+	Impl* m = impl;
+	hipStream_t s = m->stream;
+	if ( capacity == 0 )
+	{
+		m->pairSize = 0;
+		return;
+	}
+	if ( all || capacity != m->pairSize )
+	{
+		if ( capacity > m->pairCap )
+		{
+			hipFree( m->pairSlots );
+			HIP_CHECK( hipMalloc( &m->pairSlots, capacity * sizeof( uint64_t ) ) );
+			m->pairCap = capacity;
+		}
+		GrowPinned( m->hostPairs, m->hostPairCap, capacity );
+		memcpy( m->hostPairs, slots, capacity * sizeof( uint64_t ) );
+		HIP_CHECK( hipMemcpyAsync( m->pairSlots, m->hostPairs, capacity * sizeof( uint64_t ), hipMemcpyHostToDevice, s ) );
+		m->pairSize = capacity;
+		return;
+	}
+	if ( dirtyCount == 0 )
+	{
+		return;
+	}
+	GrowPinned( m->hostPairs, m->hostPairCap, 2 * dirtyCount );
+	for ( size_t k = 0; k < dirtyCount; ++k )
+	{
+		m->hostPairs[2 * k] = dirty[k];
+		m->hostPairs[2 * k + 1] = slots[dirty[k]];
+	}
+	if ( 2 * dirtyCount > m->pairUpdateCap )
+	{
+		hipFree( m->pairUpdates );
+		m->pairUpdateCap = std::max<size_t>( 1024, 3 * dirtyCount );
+		HIP_CHECK( hipMalloc( &m->pairUpdates, m->pairUpdateCap * sizeof( uint64_t ) ) );
+	}
+	HIP_CHECK( hipMemcpyAsync( m->pairUpdates, m->hostPairs, 2 * dirtyCount * sizeof( uint64_t ), hipMemcpyHostToDevice, s ) );
+	PairSetScatterKernel<<<( (int)dirtyCount + kLbvhBlock - 1 ) / kLbvhBlock, kLbvhBlock, 0, s>>>( m->pairUpdates, (int)dirtyCount,
+																								  m->pairSlots );
+	HIP_CHECK( hipGetLastError() );
 }
 
 LbvhProxy* PairFinder::ProxyStaging( int count )
@@ -724,6 +817,8 @@ void PairFinder::Impl::Query()
 	qa.largeList = largeList;
 	qa.proxies = proxies;
 	qa.proxyCount = launchedProxies;
+	qa.pairSlots = pairSize > 0 ? pairSlots : nullptr;
+	qa.pairMask = pairSize > 0 ? pairSize - 1 : 0;
 	HIP_CHECK( hipMemsetAsync( counters, 0, 3 * sizeof( int ), stream ) );
 	int queryBlocks = ( launchedQueries + kLbvhBlock - 1 ) / kLbvhBlock;
 	LbvhLargeListKernel<<<queryBlocks, kLbvhBlock, 0, stream>>>( qa );

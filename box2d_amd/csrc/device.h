@@ -205,11 +205,29 @@ struct LbvhProxy
 	b2AABB aabb; // the tree's fat box
 };
 
+// Home-slot hash of the existing-pair set: the 64-bit MurmurHash3 finalizer. The host set and the
+// device query's mirror of it probe the same slots.
+constexpr uint64_t PairKeyHash( uint64_t key )
+{
+	key ^= key >> 33;
+	key *= 0xff51afd7ed558ccdull;
+	key ^= key >> 33;
+	key *= 0xc4ceb9fe1a85ec53ull;
+	key ^= key >> 33;
+	return key;
+}
+
 class PairFinder
 {
 public:
 	PairFinder();
 	~PairFinder();
+
+	// Device mirror of the world's existing-pair set (open addressing, linear probing, 0 = empty),
+	// updated before a Launch: the whole table when `all` (or its capacity changed), otherwise
+	// only the listed slots, which take their current values. The query then drops candidates
+	// whose shape pair is already in the set.
+	void SyncPairSet( const uint64_t* slots, size_t capacity, const uint32_t* dirty, size_t dirtyCount, bool all );
 
 	// Pinned host staging the caller fills before Find: every proxy of the world in the order
 	// pairs are to be listed, the moved proxies' queries, and 1 per shape id whose proxy is in the
@@ -218,8 +236,9 @@ public:
 	LbvhProxy* QueryStaging( int count );
 	uint8_t* MovedStaging( int count );
 
-	// Every (query, proxy) whose fat boxes overlap and that passes the pair callback's self, tree
-	// and moved-proxy rules, as (query index << 32 | proxy index) keys in ascending order. Launch
+	// Every (query, proxy) whose fat boxes overlap, that passes the pair callback's self, tree
+	// and moved-proxy rules and whose shape pair is not in the mirrored pair set, as (query index
+	// << 32 | proxy index) keys in ascending order. Launch
 	// uploads the staging and starts the device work without waiting (the staging may be refilled
 	// only after Finish); Finish waits and returns the keys, valid until the next Launch.
 	void Launch( int proxyCount, int queryCount, int movedCount );

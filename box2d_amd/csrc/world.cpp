@@ -2758,6 +2758,10 @@ void World::QueryPairsGpuLaunch()
 		moved[shapeId] = 1;
 	}
 
+	PairKeySet& pairSet = bp.pairSet;
+	pairFinder->SyncPairSet( pairSet.slots.data(), pairSet.slots.size(), pairSet.dirty.data(), pairSet.dirty.size(),
+							 pairSet.allDirty );
+	pairSet.ClearDirty();
 	pairFinder->Launch( n, nq, movedCount );
 }
 

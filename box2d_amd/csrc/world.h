@@ -227,6 +227,7 @@ public:
 			if ( slots[i] == 0 )
 			{
 				slots[i] = key;
+				MarkDirty( i );
 				count += 1;
 				return;
 			}
@@ -258,10 +259,12 @@ public:
 			if ( stays == false )
 			{
 				slots[hole] = slots[j];
+				MarkDirty( hole );
 				hole = j;
 			}
 		}
 		slots[hole] = 0;
+		MarkDirty( hole );
 		count -= 1;
 	}
 
@@ -271,15 +274,22 @@ public:
 	}
 
 private:
+	void MarkDirty( size_t i )
+	{
+		if ( allDirty == false )
+		{
+			dirty.push_back( (uint32_t)i );
+			if ( dirty.size() > slots.size() / 4 )
+			{
+				dirty.clear();
+				allDirty = true;
+			}
+		}
+	}
+
 	size_t Slot( uint64_t key ) const
 	{
-		// 64-bit finalizer of MurmurHash3
-		key ^= key >> 33;
-		key *= 0xff51afd7ed558ccdull;
-		key ^= key >> 33;
-		key *= 0xc4ceb9fe1a85ec53ull;
-		key ^= key >> 33;
-		return (size_t)key & mask;
+		return (size_t)PairKeyHash( key ) & mask;
 	}
 
 	void Grow()
@@ -297,9 +307,22 @@ private:
 				Insert( k );
 			}
 		}
+		dirty.clear();
+		allDirty = true;
 	}
 
+public:
+	// slots written since the device mirror was last updated (allDirty: the whole table)
 	std::vector<uint64_t> slots;
+	std::vector<uint32_t> dirty;
+	bool allDirty = true;
+	void ClearDirty()
+	{
+		dirty.clear();
+		allDirty = false;
+	}
+
+private:
 	size_t mask = 0;
 	size_t count = 0;
 };
