@@ -15,12 +15,23 @@
 
 #define B2S_MAX_HANDLES 4096
 
+#define B2S_EDIT_LINKS 5
+
 typedef struct b2sScene
 {
 	b2WorldId worldId;
 	b2BodyId* bodies;
 	int bodyCount;
 	int inUse;
+
+	// B2S_EDITS: the edit script's state
+	int kind;
+	int stepIndex;
+	int bodyCapacity;
+	uint32_t probeHash;
+	b2BodyId links[B2S_EDIT_LINKS];
+	b2JointId hinges[B2S_EDIT_LINKS];
+	b2BodyId kinematicId;
 } b2sScene;
 
 static b2sScene s_scenes[B2S_MAX_HANDLES];
@@ -804,6 +815,301 @@ static void b2sBuildEvents( b2sScene* scene, int count, uint32_t seed )
 	}
 }
 
+// Bodies, a hinge chain and a kinematic paddle that the edit script changes between steps.
+static void b2sBuildEdits( b2sScene* scene, int enableSleep )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+	scene->bodyCapacity = 0;
+
+	b2BodyDef groundDef = b2DefaultBodyDef();
+	b2BodyId groundId = b2CreateBody( worldId, &groundDef );
+	b2ShapeDef shapeDef = b2DefaultShapeDef();
+	b2Polygon groundBox = b2MakeOffsetBox( 30.0f, 1.0f, ( b2Vec2 ){ 0.0f, -1.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &shapeDef, &groundBox );
+
+	// 24 dynamic bodies: boxes, circles and capsules in turn
+	for ( int i = 0; i < 24; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ -7.0f + 2.0f * (float)( i % 6 ), 1.0f + 1.5f * (float)( i / 6 ) };
+		bodyDef.rotation = b2MakeRot( 0.1f * (float)i );
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.material.friction = 0.4f + 0.02f * (float)i;
+		if ( i % 3 == 0 )
+		{
+			b2Polygon box = b2MakeBox( 0.4f, 0.3f );
+			b2CreatePolygonShape( bodyId, &def, &box );
+		}
+		else if ( i % 3 == 1 )
+		{
+			b2Circle circle = { { 0.0f, 0.0f }, 0.4f };
+			b2CreateCircleShape( bodyId, &def, &circle );
+		}
+		else
+		{
+			b2Capsule capsule = { { -0.3f, 0.0f }, { 0.3f, 0.0f }, 0.25f };
+			b2CreateCapsuleShape( bodyId, &def, &capsule );
+		}
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+
+	// a hinge chain of capsule links hanging from the ground body
+	b2BodyId prev = groundId;
+	b2Vec2 anchor = { 10.0f, 8.0f };
+	for ( int i = 0; i < B2S_EDIT_LINKS; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ anchor.x + 0.5f + 1.0f * (float)i, anchor.y };
+		b2BodyId link = b2CreateBody( worldId, &bodyDef );
+		b2Capsule capsule = { { -0.5f, 0.0f }, { 0.5f, 0.0f }, 0.15f };
+		b2CreateCapsuleShape( link, &shapeDef, &capsule );
+
+		b2RevoluteJointDef jd = b2DefaultRevoluteJointDef();
+		jd.base.bodyIdA = prev;
+		jd.base.bodyIdB = link;
+		b2Pos pivot = { anchor.x + 1.0f * (float)i, anchor.y };
+		jd.base.localFrameA.p = b2Body_GetLocalPoint( prev, pivot );
+		jd.base.localFrameB.p = b2Body_GetLocalPoint( link, pivot );
+		scene->hinges[i] = b2CreateRevoluteJoint( worldId, &jd );
+		scene->links[i] = link;
+		b2sPushBody( scene, &scene->bodyCapacity, link );
+		prev = link;
+	}
+
+	// a kinematic paddle
+	b2BodyDef kinDef = b2DefaultBodyDef();
+	kinDef.type = b2_kinematicBody;
+	kinDef.position = ( b2Pos ){ -10.0f, 3.0f };
+	kinDef.angularVelocity = 1.0f;
+	scene->kinematicId = b2CreateBody( worldId, &kinDef );
+	b2Polygon paddle = b2MakeBox( 1.5f, 0.2f );
+	b2CreatePolygonShape( scene->kinematicId, &shapeDef, &paddle );
+	b2sPushBody( scene, &scene->bodyCapacity, scene->kinematicId );
+}
+
+static uint32_t b2sFnvWord( uint32_t hash, uint32_t word )
+{
+	for ( int b = 0; b < 4; ++b )
+	{
+		hash ^= ( word >> ( 8 * b ) ) & 0xFFu;
+		hash *= 0x01000193u;
+	}
+	return hash;
+}
+
+static uint32_t b2sFnvFloat( uint32_t hash, float x )
+{
+	uint32_t u;
+	memcpy( &u, &x, sizeof( u ) );
+	return b2sFnvWord( hash, u );
+}
+
+static void b2sRemoveBody( b2sScene* scene, b2BodyId id )
+{
+	for ( int i = 0; i < scene->bodyCount; ++i )
+	{
+		if ( scene->bodies[i].index1 == id.index1 && scene->bodies[i].generation == id.generation )
+		{
+			memmove( scene->bodies + i, scene->bodies + i + 1, (size_t)( scene->bodyCount - i - 1 ) * sizeof( b2BodyId ) );
+			scene->bodyCount -= 1;
+			return;
+		}
+	}
+}
+
+// The edit script, run before world step k (0-based): body, shape and joint calls made between
+// steps, as a user program makes them.
+static void b2sEditsBeforeStep( b2sScene* scene, int k, float timeStep )
+{
+	b2WorldId worldId = scene->worldId;
+	b2BodyId* b = scene->bodies;
+	if ( k == 10 )
+	{
+		b2Body_SetTransform( b[3], ( b2Pos ){ 2.0f, 6.0f }, b2MakeRot( 0.5f ) );
+	}
+	if ( k == 12 )
+	{
+		b2Body_ApplyForce( b[5], ( b2Vec2 ){ 30.0f, 200.0f }, b2Body_GetWorldPoint( b[5], ( b2Vec2 ){ 0.2f, 0.1f } ), true );
+		b2Body_ApplyTorque( b[6], 20.0f, true );
+	}
+	if ( k >= 15 && k < 40 )
+	{
+		b2Body_ApplyForceToCenter( b[4], ( b2Vec2 ){ 0.0f, 15.0f }, true );
+	}
+	if ( k == 20 )
+	{
+		b2Body_ApplyLinearImpulse( b[7], ( b2Vec2 ){ 2.0f, 5.0f }, b2Body_GetWorldPoint( b[7], ( b2Vec2 ){ 0.1f, 0.2f } ), true );
+		b2Body_ApplyLinearImpulseToCenter( b[8], ( b2Vec2 ){ -3.0f, 2.0f }, true );
+		b2Body_ApplyAngularImpulse( b[9], 1.5f, true );
+	}
+	if ( k == 25 )
+	{
+		for ( int i = 10; i < 13; ++i )
+		{
+			b2ShapeId shapeId;
+			if ( b2Body_GetShapes( b[i], &shapeId, 1 ) != 1 )
+			{
+				continue;
+			}
+			b2ShapeType type = b2Shape_GetType( shapeId );
+			if ( type == b2_polygonShape )
+			{
+				b2Polygon box = b2MakeBox( 0.6f, 0.3f );
+				b2Shape_SetPolygon( shapeId, &box );
+			}
+			else if ( type == b2_circleShape )
+			{
+				b2Circle circle = { { 0.1f, 0.0f }, 0.6f };
+				b2Shape_SetCircle( shapeId, &circle );
+			}
+			else if ( type == b2_capsuleShape )
+			{
+				b2Capsule capsule = { { -0.5f, 0.0f }, { 0.5f, 0.1f }, 0.2f };
+				b2Shape_SetCapsule( shapeId, &capsule );
+			}
+			b2Body_ApplyMassFromShapes( b[i] );
+		}
+	}
+	if ( k == 30 )
+	{
+		b2RevoluteJoint_EnableMotor( scene->hinges[1], true );
+		b2RevoluteJoint_SetMotorSpeed( scene->hinges[1], 2.0f );
+		b2RevoluteJoint_SetMaxMotorTorque( scene->hinges[1], 50.0f );
+		b2RevoluteJoint_EnableSpring( scene->hinges[2], true );
+		b2RevoluteJoint_SetSpringHertz( scene->hinges[2], 2.0f );
+		b2RevoluteJoint_SetSpringDampingRatio( scene->hinges[2], 0.5f );
+		b2RevoluteJoint_SetTargetAngle( scene->hinges[2], 0.3f );
+		b2RevoluteJoint_EnableLimit( scene->hinges[3], true );
+		b2RevoluteJoint_SetLimits( scene->hinges[3], 0.4f, -0.2f );
+	}
+	if ( k == 40 )
+	{
+		b2RevoluteJoint_EnableMotor( scene->hinges[1], false );
+		b2RevoluteJoint_EnableLimit( scene->hinges[3], false );
+	}
+	if ( k == 45 )
+	{
+		b2DestroyJoint( scene->hinges[4] );
+	}
+	if ( k == 50 )
+	{
+		b2BodyId gone = b[13];
+		b2DestroyBody( gone );
+		b2sRemoveBody( scene, gone );
+	}
+	if ( k == 55 )
+	{
+		b2Transform frame = { { 0.5f, 0.0f }, b2MakeRot( 0.1f ) };
+		b2Joint_SetLocalFrameA( scene->hinges[0], frame );
+	}
+	if ( k >= 60 )
+	{
+		b2WorldTransform target = { { -10.0f + 0.05f * (float)( k - 60 ), 3.0f }, b2MakeRot( 0.02f * (float)k ) };
+		b2Body_SetTargetTransform( scene->kinematicId, target, timeStep, true );
+	}
+	if ( k == 70 )
+	{
+		b2BodyId gone = scene->links[2];
+		b2DestroyBody( gone );
+		b2sRemoveBody( scene, gone );
+	}
+	if ( k == 80 )
+	{
+		for ( int i = 0; i < 3; ++i )
+		{
+			b2BodyDef bodyDef = b2DefaultBodyDef();
+			bodyDef.type = b2_dynamicBody;
+			bodyDef.position = ( b2Pos ){ -4.0f + 3.0f * (float)i, 9.0f };
+			b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+			b2ShapeDef def = b2DefaultShapeDef();
+			b2Polygon box = b2MakeBox( 0.5f, 0.5f );
+			b2CreatePolygonShape( bodyId, &def, &box );
+			b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+		}
+	}
+	if ( k == 150 )
+	{
+		// a body that may be asleep: no wake, then a transform that wakes nothing by itself
+		b2Body_ApplyForceToCenter( b[0], ( b2Vec2 ){ 0.0f, 500.0f }, false );
+		b2Body_SetTransform( b[1], ( b2Pos ){ -3.0f, 5.0f }, b2Rot_identity );
+		b2Body_ApplyLinearImpulseToCenter( b[2], ( b2Vec2 ){ 0.0f, 4.0f }, true );
+	}
+}
+
+// Getter results after a step, folded into the scene's probe hash.
+static void b2sEditsProbe( b2sScene* scene )
+{
+	uint32_t h = scene->probeHash;
+	b2BodyId* b = scene->bodies;
+	b2Vec2 p = b2Body_GetLocalPoint( b[0], ( b2Pos ){ 1.0f, 1.0f } );
+	h = b2sFnvFloat( b2sFnvFloat( h, p.x ), p.y );
+	b2Pos w = b2Body_GetWorldPoint( b[2], ( b2Vec2 ){ 0.5f, 0.5f } );
+	h = b2sFnvFloat( b2sFnvFloat( h, w.x ), w.y );
+	b2Vec2 v = b2Body_GetWorldVector( b[1], ( b2Vec2 ){ 1.0f, 0.0f } );
+	h = b2sFnvFloat( b2sFnvFloat( h, v.x ), v.y );
+	v = b2Body_GetLocalVector( b[2], ( b2Vec2 ){ 0.0f, 1.0f } );
+	h = b2sFnvFloat( b2sFnvFloat( h, v.x ), v.y );
+	for ( int i = 0; i < B2S_EDIT_LINKS; ++i )
+	{
+		b2JointId j = scene->hinges[i];
+		if ( b2Joint_IsValid( j ) == false )
+		{
+			h = b2sFnvWord( h, 0xDEADu );
+			continue;
+		}
+		h = b2sFnvFloat( h, b2RevoluteJoint_GetAngle( j ) );
+		h = b2sFnvFloat( h, b2RevoluteJoint_GetMotorTorque( j ) );
+		h = b2sFnvWord( h, (uint32_t)b2RevoluteJoint_IsMotorEnabled( j ) | (uint32_t)b2RevoluteJoint_IsSpringEnabled( j ) << 1 |
+							   (uint32_t)b2RevoluteJoint_IsLimitEnabled( j ) << 2 );
+		h = b2sFnvFloat( h, b2RevoluteJoint_GetLowerLimit( j ) );
+		h = b2sFnvFloat( h, b2RevoluteJoint_GetUpperLimit( j ) );
+		b2Transform frame = b2Joint_GetLocalFrameA( j );
+		h = b2sFnvFloat( b2sFnvFloat( h, frame.p.x ), frame.q.s );
+	}
+	for ( int i = 0; i < B2S_EDIT_LINKS; ++i )
+	{
+		if ( b2Body_IsValid( scene->links[i] ) )
+		{
+			b2JointId joints[4];
+			int n = b2Body_GetJoints( scene->links[i], joints, 4 );
+			h = b2sFnvWord( h, (uint32_t)n | (uint32_t)b2Body_GetJointCount( scene->links[i] ) << 8 );
+		}
+	}
+	for ( int i = 10; i < 13; ++i )
+	{
+		b2ShapeId shapeId;
+		if ( b2Body_GetShapes( b[i], &shapeId, 1 ) != 1 )
+		{
+			continue;
+		}
+		switch ( b2Shape_GetType( shapeId ) )
+		{
+			case b2_polygonShape:
+			{
+				b2Polygon poly = b2Shape_GetPolygon( shapeId );
+				h = b2sFnvFloat( b2sFnvFloat( h, poly.vertices[0].x ), poly.centroid.y );
+			}
+			break;
+			case b2_circleShape:
+				h = b2sFnvFloat( h, b2Shape_GetCircle( shapeId ).radius );
+				break;
+			case b2_capsuleShape:
+				h = b2sFnvFloat( h, b2Shape_GetCapsule( shapeId ).center2.y );
+				break;
+			default:
+				break;
+		}
+		h = b2sFnvFloat( h, b2Body_GetMass( b[i] ) );
+		h = b2sFnvFloat( h, b2Body_GetRotationalInertia( b[i] ) );
+	}
+	h = b2sFnvWord( h, (uint32_t)b2World_GetAwakeBodyCount( scene->worldId ) );
+	scene->probeHash = h;
+}
+
 int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount )
 {
 	int handle = -1;
@@ -833,6 +1139,7 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 	memset( scene, 0, sizeof( *scene ) );
 	scene->worldId = worldId;
 	scene->inUse = 1;
+	scene->kind = kind;
 
 	switch ( kind )
 	{
@@ -884,6 +1191,10 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 
 		case B2S_EVENTS:
 			b2sBuildEvents( scene, iparamCount > 0 ? iparams[0] : 60, iparamCount > 1 ? (uint32_t)iparams[1] : 12345u );
+			break;
+
+		case B2S_EDITS:
+			b2sBuildEdits( scene, iparamCount > 0 ? iparams[0] : 0 );
 			break;
 
 		case B2S_JOINT_ZOO:
@@ -950,8 +1261,23 @@ void b2s_step( int handle, float timeStep, int subStepCount, int count )
 	}
 	for ( int i = 0; i < count; ++i )
 	{
+		if ( scene->kind == B2S_EDITS )
+		{
+			b2sEditsBeforeStep( scene, scene->stepIndex, timeStep );
+		}
 		b2World_Step( scene->worldId, timeStep, subStepCount );
+		scene->stepIndex += 1;
+		if ( scene->kind == B2S_EDITS )
+		{
+			b2sEditsProbe( scene );
+		}
 	}
+}
+
+uint32_t b2s_probe_hash( int handle )
+{
+	b2sScene* scene = b2sGet( handle );
+	return scene != NULL ? scene->probeHash : 0;
 }
 
 static double b2sNowMs( void )

@@ -73,6 +73,15 @@ enum
 	// three and for the ground, hit events for every other shape. Sleep is on.
 	// iparams: [0] body count, [1] seed.
 	B2S_EVENTS = 8,
+
+	// Edits between steps: 24 boxes, circles and capsules on a 30 x 1 m ground box (half extents),
+	// a five-link hinge chain and a kinematic paddle. Before chosen steps b2s_step moves bodies,
+	// applies forces and impulses, replaces shape geometry and recomputes mass, changes the
+	// hinges' motor, spring, limit and frame, destroys a joint and two bodies, creates bodies,
+	// and drives the paddle with b2Body_SetTargetTransform; after each step it folds getter
+	// results (local/world points and vectors, joint angles, motor torques, joint and shape
+	// queries, masses) into b2s_probe_hash. iparams: [0] enableSleep.
+	B2S_EDITS = 9,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.
@@ -100,6 +109,9 @@ B2S_API int b2s_body_count( int handle );
 // djb2 hash (b2Hash) of the world transforms of the scene bodies in creation order,
 // the same construction as the reference determinism check (shared/determinism.c:110-125).
 B2S_API uint32_t b2s_hash( int handle );
+
+// Getter results folded after every step of a B2S_EDITS scene (FNV-1a), 0 for other scenes.
+B2S_API uint32_t b2s_probe_hash( int handle );
 
 // Copy the world transforms (p.x, p.y, q.c, q.s) of the scene bodies in creation order.
 // Returns the number of bodies written.

@@ -21,6 +21,7 @@ SCENE_BATCH_WORLD = 5
 SCENE_PUSH = 6
 SCENE_JOINT_ZOO = 7
 SCENE_EVENTS = 8
+SCENE_EDITS = 9
 
 
 class SceneLib:
@@ -36,6 +37,7 @@ class SceneLib:
         lib.b2s_step_timed.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int]
         lib.b2s_body_count.restype = ctypes.c_int
         lib.b2s_hash.restype = ctypes.c_uint32
+        lib.b2s_probe_hash.restype = ctypes.c_uint32
         lib.b2s_get_transforms.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         lib.b2s_get_velocities.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         lib.b2s_counters.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -85,6 +87,10 @@ class SceneLib:
     def push(self, vx, vy, omega, friction, rows=10):
         """Differentiable push scene: pyramid plus a launched pusher box (SCENE_PUSH)."""
         return self.create(SCENE_PUSH, (rows,), (vx, vy, omega, friction))
+
+    def probe_hash(self, h):
+        """Getter results folded after every step of an edits scene."""
+        return int(self.lib.b2s_probe_hash(h))
 
     def hash(self, h):
         return self.lib.b2s_hash(h)

@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from box2d_amd.scenes_api import (SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES,  # noqa: E402
-                                  SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS)
+                                  SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS, SCENE_EDITS)
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
 
@@ -90,6 +90,18 @@ def events_fixture(lib, name, count, seed, steps):
     return fx
 
 
+def edits_fixture(lib, name, sleep, steps):
+    """Per step of the edits scene: the transform hash of the listed bodies, the getter probe
+    hash and the listed body count."""
+    h = lib.create(SCENE_EDITS, (sleep,), ())
+    per_step = []
+    for _ in range(steps):
+        lib.step(h)
+        per_step.append([lib.hash(h), lib.probe_hash(h), lib.body_count(h)])
+    lib.destroy(h)
+    return {"name": name, "kind": SCENE_EDITS, "iparams": [sleep], "fparams": [], "steps": steps, "per_step": per_step}
+
+
 def main():
     if not os.path.exists(REF_LIB):
         sys.exit(f"{REF_LIB} missing: run `make -C oracle ref` first")
@@ -113,10 +125,11 @@ def main():
     fixtures.append(scene_fixture(lib, "joint_zoo", SCENE_JOINT_ZOO, (0,), (), 400, set(range(1, 401)), True))
     fixtures.append(scene_fixture(lib, "joint_zoo_sleep", SCENE_JOINT_ZOO, (1,), (), 400, set(range(1, 401)), True))
     events = [events_fixture(lib, "events_60", 60, 12345, 300), events_fixture(lib, "events_150", 150, 7, 240)]
+    edits = [edits_fixture(lib, "edits", 0, 240), edits_fixture(lib, "edits_sleep", 1, 240)]
     ok, step, hsh = lib.falling_hinges(1, 500)
     hinges = {"name": "falling_hinges", "slept": ok, "sleep_step": step, "hash": hsh}
     out = {"generator": "tests/golden/gen_golden.py", "reference": "erincatto/box2d v3.2 (src/*.c, -O2 -ffp-contract=off)",
-           "scenes": fixtures, "falling_hinges": hinges, "events": events}
+           "scenes": fixtures, "falling_hinges": hinges, "events": events, "edits": edits}
     path = os.path.join(HERE, "world_runs.json")
     with open(path, "w") as f:
         json.dump(out, f, separators=(",", ":"))

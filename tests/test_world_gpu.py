@@ -165,3 +165,24 @@ def test_world_recreate(gpu_scenes, world_runs):
         gpu_scenes.step(h, 60)
         assert gpu_scenes.hash(h) == fx["hashes"]["60"]
         gpu_scenes.destroy(h)
+
+
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("name", ["edits", "edits_sleep"])
+def test_edits_between_steps_match_reference(gpu_scenes, world_runs, name, path):
+    """Body, shape and joint calls made between steps (transforms, forces and impulses, geometry
+    and mass, hinge motor/spring/limit/frame changes, joint and body destruction, new bodies, a
+    kinematic target) give the reference's transforms and getter results after every step."""
+    import box2d_amd
+
+    fx = next(e for e in world_runs["edits"] if e["name"] == name)
+    print(f"[gpu] edits scene {name} path {path}", flush=True)
+    h = gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"])
+    box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
+    try:
+        for step in range(1, fx["steps"] + 1):
+            gpu_scenes.step(h)
+            got = [gpu_scenes.hash(h), gpu_scenes.probe_hash(h), gpu_scenes.body_count(h)]
+            assert got == fx["per_step"][step - 1], f"{name} step {step}: [transforms, getters, bodies] {got}, reference {fx['per_step'][step - 1]}"
+    finally:
+        gpu_scenes.destroy(h)
