@@ -2877,9 +2877,10 @@ void World::UpdateBroadPhasePairs()
 		}
 	}
 
-	// The tree rebuild runs after the queries (the reference overlaps it with the narrowphase).
-	bp.trees[b2_dynamicBody].Rebuild( false );
-	bp.trees[b2_kinematicBody].Rebuild( false );
+	// The partial rebuild of the dynamic and kinematic trees runs later in the step, while the GPU
+	// solves (the reference overlaps it with the narrowphase). Nothing reads these trees before
+	// the refit at the end of the step.
+	treeRebuildPending = true;
 
 	if ( gpu )
 	{
@@ -3309,6 +3310,13 @@ void World::StepMiddle()
 	{
 		solveTicks = b2GetTicks();
 		solvePending = SolveBegin( stepParams );
+	}
+
+	if ( treeRebuildPending )
+	{
+		broadPhase.trees[b2_dynamicBody].Rebuild( false );
+		broadPhase.trees[b2_kinematicBody].Rebuild( false );
+		treeRebuildPending = false;
 	}
 }
 

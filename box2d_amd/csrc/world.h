@@ -535,6 +535,7 @@ private:
 	void QueryPairsGpuLaunch();
 	void QueryPairsGpuFinish( std::vector<std::vector<MovePair>>& results );
 	std::vector<int> pairQueryMove; // move-buffer index of each launched query
+	bool treeRebuildPending = false; // the pair update's tree rebuild, run while the GPU solves
 	bool AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair );
 	void ProcessContactStates( const int* changedIds, int count );
 	bool SolveBegin( StepParams& params );
