@@ -3366,13 +3366,6 @@ bool World::SolveBegin( StepParams& params )
 		return false;
 	}
 
-	uint64_t splitTicks = b2GetTicks();
-	if ( splitIslandId != B2_NULL_INDEX )
-	{
-		SplitIsland( splitIslandId );
-		splitIslandId = B2_NULL_INDEX;
-	}
-
 	// Woken bodies may have rewritten states on the host.
 	PushBodiesIfDirty();
 
@@ -3383,10 +3376,20 @@ bool World::SolveBegin( StepParams& params )
 		listsDirty = false;
 		listsStableSteps = 0;
 	}
-	profile.splitIslands = b2GetMilliseconds( splitTicks );
 
 	constraintTicks = b2GetTicks();
 	device->SolveLaunch( params, restitutionShapeCount > 0, enableSleep );
+
+	// The island split only changes island bookkeeping, which the GPU solve does not read, so it
+	// runs while the GPU solves (the reference runs it beside the solver stages,
+	// src/solver.c:1475-1494); island sleep reads its result after the solve.
+	uint64_t splitTicks = b2GetTicks();
+	if ( splitIslandId != B2_NULL_INDEX )
+	{
+		SplitIsland( splitIslandId );
+		splitIslandId = B2_NULL_INDEX;
+	}
+	profile.splitIslands = b2GetMilliseconds( splitTicks );
 	return true;
 }
 
