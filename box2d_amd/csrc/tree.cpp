@@ -364,6 +364,7 @@ void DynamicTree::RotateNodes( int iA )
 
 void DynamicTree::InsertLeaf( int leaf, bool shouldRotate )
 {
+	structureVersion += 1;
 	if ( root == B2_NULL_INDEX )
 	{
 		root = leaf;
@@ -424,6 +425,7 @@ void DynamicTree::InsertLeaf( int leaf, bool shouldRotate )
 
 void DynamicTree::RemoveLeaf( int leaf )
 {
+	structureVersion += 1;
 	if ( leaf == root )
 	{
 		root = B2_NULL_INDEX;
@@ -695,6 +697,7 @@ int DynamicTree::BuildTree( int leafCount )
 
 int DynamicTree::Rebuild( bool fullBuild )
 {
+	structureVersion += 1;
 	if ( proxyCount == 0 )
 	{
 		return 0;

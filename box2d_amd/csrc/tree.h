@@ -103,6 +103,12 @@ public:
 	{
 		return proxyCount;
 	}
+	// Changes whenever the tree's structure (and so its leaf order) changes; box-only changes
+	// (EnlargeProxy) keep it.
+	uint32_t StructureVersion() const
+	{
+		return structureVersion;
+	}
 
 	const std::vector<TreeNode>& Nodes() const
 	{
@@ -164,6 +170,7 @@ private:
 	int root = B2_NULL_INDEX;
 	int nodeCount = 0;
 	int proxyCount = 0;
+	uint32_t structureVersion = 0;
 
 	std::vector<int> leafIndices;
 	std::vector<b2Vec2> leafCenters;

@@ -2722,13 +2722,27 @@ void World::QueryPairsGpuLaunch()
 	int n = 0;
 	for ( b2BodyType type : kVisitOrder )
 	{
-		bp.trees[type].ForEachLeafInQueryOrder( [&]( int id, const TreeNode& node ) {
-			if ( n < proxyCount )
+		const DynamicTree& tree = bp.trees[type];
+		ProxyStagingCache& cache = stagingCache[type];
+		if ( cache.valid == false || cache.version != tree.StructureVersion() )
+		{
+			FillStagingCache( type );
+		}
+		int count = (int)cache.proxies.size();
+		if ( n + count > proxyCount )
+		{
+			Fatal( "broadphase trees hold more leaves than their %d proxies", proxyCount );
+		}
+		memcpy( proxies + n, cache.proxies.data(), count * sizeof( LbvhProxy ) );
+		for ( int key : bp.moveArray )
+		{
+			if ( key != B2_NULL_INDEX && B2G_PROXY_TYPE( key ) == type )
 			{
-				proxies[n] = LbvhProxy{ B2G_PROXY_KEY( id, type ), (int)node.userData, node.aabb };
+				int id = B2G_PROXY_ID( key );
+				proxies[n + cache.position[id]].aabb = tree.GetAABB( id );
 			}
-			++n;
-		} );
+		}
+		n += count;
 	}
 	if ( n != proxyCount )
 	{
@@ -2763,6 +2777,20 @@ void World::QueryPairsGpuLaunch()
 							 pairSet.allDirty );
 	pairSet.ClearDirty();
 	pairFinder->Launch( n, nq, movedCount );
+}
+
+void World::FillStagingCache( b2BodyType type )
+{
+	const DynamicTree& tree = broadPhase.trees[type];
+	ProxyStagingCache& cache = stagingCache[type];
+	cache.proxies.clear();
+	cache.position.resize( tree.Nodes().size() );
+	tree.ForEachLeafInQueryOrder( [&]( int id, const TreeNode& node ) {
+		cache.position[id] = (int)cache.proxies.size();
+		cache.proxies.push_back( LbvhProxy{ B2G_PROXY_KEY( id, type ), (int)node.userData, node.aabb } );
+	} );
+	cache.version = tree.StructureVersion();
+	cache.valid = true;
 }
 
 // Candidates of the launched query, filtered in the reference's per-query visit order.
@@ -3317,6 +3345,12 @@ void World::StepMiddle()
 		broadPhase.trees[b2_dynamicBody].Rebuild( false );
 		broadPhase.trees[b2_kinematicBody].Rebuild( false );
 		treeRebuildPending = false;
+		// the rebuilt trees' leaf order, staged ahead for the next GPU pair query
+		if ( lastPairQueryPath == 2 )
+		{
+			FillStagingCache( b2_dynamicBody );
+			FillStagingCache( b2_kinematicBody );
+		}
 	}
 }
 

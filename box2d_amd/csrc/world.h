@@ -536,6 +536,19 @@ private:
 	void QueryPairsGpuFinish( std::vector<std::vector<MovePair>>& results );
 	std::vector<int> pairQueryMove; // move-buffer index of each launched query
 	bool treeRebuildPending = false; // the pair update's tree rebuild, run while the GPU solves
+
+	// A tree's leaves in query order as last staged for the GPU pair query, valid while the
+	// tree's structure version is unchanged. Later box changes come with a move-buffer entry, so
+	// staging copies the cache and refreshes the moved proxies' boxes.
+	struct ProxyStagingCache
+	{
+		uint32_t version = 0;
+		bool valid = false;
+		std::vector<LbvhProxy> proxies;
+		std::vector<int> position; // by proxy id: index in proxies
+	};
+	ProxyStagingCache stagingCache[b2_bodyTypeCount];
+	void FillStagingCache( b2BodyType type );
 	bool AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair );
 	void ProcessContactStates( const int* changedIds, int count );
 	bool SolveBegin( StepParams& params );
