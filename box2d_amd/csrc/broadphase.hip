@@ -353,6 +353,7 @@ struct LbvhQueryArgs
 	int proxyCount;
 	const uint64_t* pairSlots; // mirror of the existing-pair set (nullptr: empty)
 	uint64_t pairMask;
+	const PairShapeFilter* filters; // by shape id (nullptr: not synced)
 };
 
 // Whether the shape pair already has a contact: the host set's probe sequence on its mirror.
@@ -426,6 +427,27 @@ __device__ __forceinline__ bool LbvhKeep( const LbvhQueryArgs& a, int queryKey, 
 	{
 		// static and kinematic proxies only query the dynamic tree
 		if ( hitType != b2_dynamicBody || a.moved[hitShape] )
+		{
+			return false;
+		}
+	}
+	if ( a.filters != nullptr )
+	{
+		// same body and b2ShouldShapesCollide (src/broad_phase.c:245-264, src/shape.h:122-130)
+		const PairShapeFilter fa = a.filters[queryShape];
+		const PairShapeFilter fb = a.filters[hitShape];
+		if ( fa.bodyId == fb.bodyId )
+		{
+			return false;
+		}
+		if ( fa.groupIndex == fb.groupIndex && fa.groupIndex != 0 )
+		{
+			if ( fa.groupIndex <= 0 )
+			{
+				return false;
+			}
+		}
+		else if ( ( fa.maskBits & fb.categoryBits ) == 0 || ( fa.categoryBits & fb.maskBits ) == 0 )
 		{
 			return false;
 		}
@@ -567,6 +589,11 @@ struct PairFinder::Impl
 	size_t hostCandCap = 0;
 	uint64_t* hostPairs = nullptr; // whole table or (slot, key) pairs
 	size_t hostPairCap = 0;
+	PairShapeFilter* filters = nullptr;
+	size_t filterCap = 0;
+	int filterCount = 0;
+	PairShapeFilter* hostFilters = nullptr;
+	size_t hostFilterCap = 0;
 
 	// the launched query, finished by Finish
 	int launchedProxies = 0;
@@ -662,6 +689,8 @@ PairFinder::~PairFinder()
 	hipFree( m->pairSlots );
 	hipFree( m->pairUpdates );
 	hipHostFree( m->hostPairs );
+	hipFree( m->filters );
+	hipHostFree( m->hostFilters );
 	hipFree( m->counters );
 	hipHostFree( m->hostCounters );
 	hipHostFree( m->hostProxies );
@@ -725,6 +754,24 @@ void PairFinder::SyncPairSet( const uint64_t* slots, size_t capacity, const uint
 	PairSetScatterKernel<<<( (int)dirtyCount + kLbvhBlock - 1 ) / kLbvhBlock, kLbvhBlock, 0, s>>>( m->pairUpdates, (int)dirtyCount,
 																								  m->pairSlots );
 	HIP_CHECK( hipGetLastError() );
+}
+
+PairShapeFilter* PairFinder::FilterStaging( int count )
+{
+	return GrowPinned( impl->hostFilters, impl->hostFilterCap, (size_t)count );
+}
+
+void PairFinder::SyncShapeFilters( int count )
+{
+	Impl* m = impl;
+	if ( (size_t)count > m->filterCap )
+	{
+		hipFree( m->filters );
+		m->filterCap = std::max<size_t>( 1024, count + count / 2 );
+		HIP_CHECK( hipMalloc( &m->filters, m->filterCap * sizeof( PairShapeFilter ) ) );
+	}
+	HIP_CHECK( hipMemcpyAsync( m->filters, m->hostFilters, count * sizeof( PairShapeFilter ), hipMemcpyHostToDevice, m->stream ) );
+	m->filterCount = count;
 }
 
 LbvhProxy* PairFinder::ProxyStaging( int count )
@@ -819,6 +866,7 @@ void PairFinder::Impl::Query()
 	qa.proxyCount = launchedProxies;
 	qa.pairSlots = pairSize > 0 ? pairSlots : nullptr;
 	qa.pairMask = pairSize > 0 ? pairSize - 1 : 0;
+	qa.filters = filterCount > 0 ? filters : nullptr;
 	HIP_CHECK( hipMemsetAsync( counters, 0, 3 * sizeof( int ), stream ) );
 	int queryBlocks = ( launchedQueries + kLbvhBlock - 1 ) / kLbvhBlock;
 	LbvhLargeListKernel<<<queryBlocks, kLbvhBlock, 0, stream>>>( qa );

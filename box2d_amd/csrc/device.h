@@ -205,6 +205,15 @@ struct LbvhProxy
 	b2AABB aabb; // the tree's fat box
 };
 
+// What the pair callback's shape tests read of a shape (b2Filter and the owning body).
+struct PairShapeFilter
+{
+	uint64_t categoryBits;
+	uint64_t maskBits;
+	int groupIndex;
+	int bodyId;
+};
+
 // Home-slot hash of the existing-pair set: the 64-bit MurmurHash3 finalizer. The host set and the
 // device query's mirror of it probe the same slots.
 constexpr uint64_t PairKeyHash( uint64_t key )
@@ -229,6 +238,11 @@ public:
 	// whose shape pair is already in the set.
 	void SyncPairSet( const uint64_t* slots, size_t capacity, const uint32_t* dirty, size_t dirtyCount, bool all );
 
+	// Pinned staging of the per-shape filters (indexed by shape id) and their upload; the query
+	// then drops candidates of one body or whose filters reject each other.
+	PairShapeFilter* FilterStaging( int count );
+	void SyncShapeFilters( int count );
+
 	// Pinned host staging the caller fills before Find: every proxy of the world in the order
 	// pairs are to be listed, the moved proxies' queries, and 1 per shape id whose proxy is in the
 	// move buffer.
@@ -236,9 +250,9 @@ public:
 	LbvhProxy* QueryStaging( int count );
 	uint8_t* MovedStaging( int count );
 
-	// Every (query, proxy) whose fat boxes overlap, that passes the pair callback's self, tree
-	// and moved-proxy rules and whose shape pair is not in the mirrored pair set, as (query index
-	// << 32 | proxy index) keys in ascending order. Launch
+	// Every (query, proxy) whose fat boxes overlap, that passes the pair callback's self, tree,
+	// moved-proxy, same-body and filter rules and whose shape pair is not in the mirrored pair set,
+	// as (query index << 32 | proxy index) keys in ascending order. Launch
 	// uploads the staging and starts the device work without waiting (the staging may be refilled
 	// only after Finish); Finish waits and returns the keys, valid until the next Launch.
 	void Launch( int proxyCount, int queryCount, int movedCount );

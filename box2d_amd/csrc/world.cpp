@@ -1495,6 +1495,7 @@ int World::CreateShapeInternal( Body* body, const b2ShapeDef* def, const void* g
 	shape->id = shapeId;
 	shape->bodyId = body->id;
 	shape->type = shapeType;
+	shapeFilterVersion += 1;
 	shape->density = def->density;
 	shape->material = def->material;
 	shape->filter = def->filter;
@@ -2770,6 +2771,19 @@ void World::QueryPairsGpuLaunch()
 		queries[nq++] = LbvhProxy{ key, shapeId, tree.GetAABB( B2G_PROXY_ID( key ) ) };
 		queryMove.push_back( i );
 		moved[shapeId] = 1;
+	}
+
+	if ( syncedFilterVersion != shapeFilterVersion )
+	{
+		int shapeCount = (int)shapes.size();
+		PairShapeFilter* filters = pairFinder->FilterStaging( std::max( shapeCount, 1 ) );
+		for ( int i = 0; i < shapeCount; ++i )
+		{
+			const Shape& shape = shapes[i];
+			filters[i] = PairShapeFilter{ shape.filter.categoryBits, shape.filter.maskBits, shape.filter.groupIndex, shape.bodyId };
+		}
+		pairFinder->SyncShapeFilters( shapeCount );
+		syncedFilterVersion = shapeFilterVersion;
 	}
 
 	PairKeySet& pairSet = bp.pairSet;

@@ -548,6 +548,8 @@ private:
 		std::vector<int> position; // by proxy id: index in proxies
 	};
 	ProxyStagingCache stagingCache[b2_bodyTypeCount];
+	uint64_t shapeFilterVersion = 1; // bumped when a shape id is (re)assigned
+	uint64_t syncedFilterVersion = 0; // the version the pair finder holds
 	void FillStagingCache( b2BodyType type );
 	bool AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair );
 	void ProcessContactStates( const int* changedIds, int count );
