@@ -364,6 +364,7 @@ void DynamicTree::RotateNodes( int iA )
 
 void DynamicTree::InsertLeaf( int leaf, bool shouldRotate )
 {
+	FlushEnlarged();
 	structureVersion += 1;
 	if ( root == B2_NULL_INDEX )
 	{
@@ -425,6 +426,7 @@ void DynamicTree::InsertLeaf( int leaf, bool shouldRotate )
 
 void DynamicTree::RemoveLeaf( int leaf )
 {
+	FlushEnlarged();
 	structureVersion += 1;
 	if ( leaf == root )
 	{
@@ -500,8 +502,28 @@ void DynamicTree::MoveProxy( int proxyId, b2AABB aabb )
 
 void DynamicTree::EnlargeProxy( int proxyId, b2AABB aabb )
 {
+	FlushEnlarged();
 	nodes[proxyId].aabb = aabb;
+	EnlargeAncestors( proxyId, aabb );
+}
 
+void DynamicTree::EnlargeProxyDeferred( int proxyId, b2AABB aabb )
+{
+	nodes[proxyId].aabb = aabb;
+	pendingEnlarge.push_back( PendingEnlarge{ proxyId, aabb } );
+}
+
+void DynamicTree::FlushEnlarged()
+{
+	for ( const PendingEnlarge& p : pendingEnlarge )
+	{
+		EnlargeAncestors( p.proxyId, p.aabb );
+	}
+	pendingEnlarge.clear();
+}
+
+void DynamicTree::EnlargeAncestors( int proxyId, b2AABB aabb )
+{
 	int parentIndex = nodes[proxyId].parent;
 	while ( parentIndex != B2_NULL_INDEX )
 	{
@@ -697,6 +719,7 @@ int DynamicTree::BuildTree( int leafCount )
 
 int DynamicTree::Rebuild( bool fullBuild )
 {
+	FlushEnlarged();
 	structureVersion += 1;
 	if ( proxyCount == 0 )
 	{

@@ -44,6 +44,13 @@ public:
 	void MoveProxy( int proxyId, b2AABB aabb );
 	void EnlargeProxy( int proxyId, b2AABB aabb );
 
+	// EnlargeProxy with the ancestor update deferred: the leaf box is set now, and the ancestors'
+	// box unions and enlarged flags are applied by FlushEnlarged, which every structural change and
+	// the rebuild call first. Unions and flags do not depend on the order of enlargements, so the
+	// tree ends up as with EnlargeProxy. Callers that read internal nodes (Query) flush first.
+	void EnlargeProxyDeferred( int proxyId, b2AABB aabb );
+	void FlushEnlarged();
+
 	// Stack-based depth-first query; child2 is visited before child1. The callback returns false
 	// to stop the query.
 	template <typename F>
@@ -171,6 +178,14 @@ private:
 	int nodeCount = 0;
 	int proxyCount = 0;
 	uint32_t structureVersion = 0;
+
+	struct PendingEnlarge
+	{
+		int proxyId;
+		b2AABB aabb;
+	};
+	std::vector<PendingEnlarge> pendingEnlarge;
+	void EnlargeAncestors( int proxyId, b2AABB aabb );
 
 	std::vector<int> leafIndices;
 	std::vector<b2Vec2> leafCenters;

@@ -369,7 +369,9 @@ void BroadPhase::MoveProxy( int proxyKey, b2AABB aabb )
 
 void BroadPhase::EnlargeProxy( int proxyKey, b2AABB aabb )
 {
-	trees[B2G_PROXY_TYPE( proxyKey )].EnlargeProxy( B2G_PROXY_ID( proxyKey ), aabb );
+	// the ancestors are updated before the trees' internal nodes are next read: in the next
+	// rebuild, which runs while the GPU solves, or before a host query
+	trees[B2G_PROXY_TYPE( proxyKey )].EnlargeProxyDeferred( B2G_PROXY_ID( proxyKey ), aabb );
 	BufferMove( proxyKey );
 }
 
@@ -2856,6 +2858,10 @@ void World::UpdateBroadPhasePairs()
 	}
 	else
 	{
+		for ( DynamicTree& tree : bp.trees )
+		{
+			tree.FlushEnlarged();
+		}
 		for ( int i = 0; i < moveCount; ++i )
 		{
 			std::vector<MovePair>& pairList = results[i];
