@@ -2634,6 +2634,14 @@ static void EnableLargeLds()
 	static bool done = false;
 	if ( done == false )
 	{
+		// A step waits on the GPU three times; spinning instead of yielding to the OS while waiting
+		// keeps the host's wake-up latency out of each wait. This only takes effect when it is the
+		// process's first HIP call, so an error (the runtime already set up) is ignored.
+		if ( getenv( "B2AMD_SCHEDULE_YIELD" ) == nullptr )
+		{
+			(void)hipSetDeviceFlags( hipDeviceScheduleSpin );
+			(void)hipGetLastError();
+		}
 		HIP_CHECK( hipFuncSetAttribute( reinterpret_cast<const void*>( SolveFusedKernel ),
 										hipFuncAttributeMaxDynamicSharedMemorySize, FusedLdsBytes( kFusedBodyCapacity ) ) );
 		HIP_CHECK( hipFuncSetAttribute( reinterpret_cast<const void*>( SolveFusedBatchKernel ),
