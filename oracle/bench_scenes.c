@@ -1,7 +1,7 @@
 // Test-side driver for the reference's own benchmark scenes. Nothing here is part of the product.
 //
 // The scene builders are the reference's: shared/benchmarks.c with its helpers shared/human.c
-// and shared/utils.c, compiled from /root/reference/shared by oracle/Makefile. This file only
+// and shared/utils.c, compiled from the reference's shared/ directory by oracle/Makefile. This file only
 // runs them, the way the reference's benchmark app does (benchmark/main.c:150-159, 318-351:
 // default world def, continuous on, the scene's step function before each b2World_Step of
 // 1/60 s with 4 substeps), and reduces each step to numbers a test can compare.
