@@ -1,3 +1,6 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the chain-segment manifold arithmetic below
+// restates its src/manifold.c in the same operation order)
+// SPDX-License-Identifier: MIT
 // One-sided chain segments against round and convex shapes, and the manifold dispatch with the
 // contact's GJK simplex cache.
 //

@@ -1,3 +1,6 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the hull, polygon, mass and bounding-box arithmetic
+// below restates its src/hull.c and src/geometry.c in the same operation order)
+// SPDX-License-Identifier: MIT
 // Creation-time geometry and math exports: polygon builders, convex hull, mass properties,
 // shape bounding boxes, deterministic trig, default definitions and the determinism hash.
 //

@@ -1,3 +1,6 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the joint solve arithmetic below restates its
+// src/*_joint.c in the same operation order)
+// SPDX-License-Identifier: MIT
 // GPU joint solver: warm start and solve of every joint type (revolute, weld, prismatic, wheel,
 // distance, motor, mover, pogo; filter joints only take part in collision filtering and islands).
 // Each function is one lane's work on one joint and keeps the reference's operation order, so its

@@ -1,3 +1,6 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the manifold arithmetic below restates its
+// src/manifold.c and src/distance.c in the same operation order)
+// SPDX-License-Identifier: MIT
 // GPU narrowphase: contact manifolds for one shape pair per lane.
 //
 // Restated from the reference manifold routines (src/manifold.c) with the same operation order

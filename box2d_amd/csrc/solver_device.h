@@ -1,3 +1,6 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the contact solver arithmetic below restates its
+// src/contact_solver.c and src/solver.c in the same operation order)
+// SPDX-License-Identifier: MIT
 // GPU solver arithmetic for one contact or joint constraint.
 //
 // These functions are the per-lane bodies of the constraint kernels. The colored path reproduces

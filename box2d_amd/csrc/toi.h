@@ -1,3 +1,6 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the GJK and time-of-impact arithmetic below
+// restates its src/distance.c in the same operation order)
+// SPDX-License-Identifier: MIT
 // GJK distance and time of impact, host and device.
 //
 // The continuous-collision kernel runs these per fast body; the library also exports the host

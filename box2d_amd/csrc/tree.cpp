@@ -1,3 +1,6 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; insert, rotate and rebuild below restate its
+// src/dynamic_tree.c so the tree topology, and with it the pair order, is the same)
+// SPDX-License-Identifier: MIT
 // Host-side bounding volume tree (see tree.h for why its topology must match the reference).
 //
 // Node allocation: the reference keeps a singly linked free list that initially threads every

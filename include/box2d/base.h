@@ -1,3 +1,5 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the declarations below follow its public headers)
+// SPDX-License-Identifier: MIT
 // box2d_amd public header: base definitions.
 //
 // Drop-in counterpart of the reference's include/box2d/base.h (export macros, assert/log hooks,

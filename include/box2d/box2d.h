@@ -1,3 +1,5 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the declarations below follow its public headers)
+// SPDX-License-Identifier: MIT
 // box2d_amd public API: the Box2D world, body, shape and joint calls.
 //
 // Drop-in counterpart of the reference's include/box2d/box2d.h for the world step path. Each

@@ -1,3 +1,5 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the declarations below follow its public headers)
+// SPDX-License-Identifier: MIT
 // box2d_amd public header: geometry and contact manifold types.
 //
 // Drop-in counterpart of the subset of the reference's include/box2d/collision.h that the world

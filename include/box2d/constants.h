@@ -1,3 +1,5 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the declarations below follow its public headers)
+// SPDX-License-Identifier: MIT
 // box2d_amd public header: simulation constants.
 //
 // Same values as the reference (include/box2d/constants.h:14-73); the step results depend on them.

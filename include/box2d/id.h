@@ -1,3 +1,5 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the declarations below follow its public headers)
+// SPDX-License-Identifier: MIT
 // box2d_amd public header: object handles.
 //
 // Same handle layouts as the reference (include/box2d/id.h:37-82) so ids pass by value across

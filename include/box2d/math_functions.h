@@ -1,3 +1,5 @@
+// SPDX-FileCopyrightText: 2022 Erin Catto (Box2D; the declarations and inline math below follow its public headers)
+// SPDX-License-Identifier: MIT
 // box2d_amd public header: vector math.
 //
 // Drop-in counterpart of the reference's include/box2d/math_functions.h. The types and the
