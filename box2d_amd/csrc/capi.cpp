@@ -17,6 +17,7 @@ void SetAssertFcn( b2AssertFcn* fcn );
 int CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
 				  b2LocalManifold* manifolds );
 void DeviceSynchronize();
+bool SpinWaitActive();
 int TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs );
 int ShapeDistanceBatch( const b2DistanceInput* inputs, b2SimplexCache* caches, int count, b2DistanceOutput* outputs );
 } // namespace b2g
@@ -1342,6 +1343,11 @@ int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int 
 void b2amd_Synchronize( void )
 {
 	DeviceSynchronize();
+}
+
+int b2amd_SpinWaitActive( void )
+{
+	return SpinWaitActive() ? 1 : 0;
 }
 
 int b2amd_CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 namespace b2g
 {
@@ -2636,11 +2637,14 @@ static void EnableLargeLds()
 	{
 		// A step waits on the GPU three times; spinning instead of yielding to the OS while waiting
 		// keeps the host's wake-up latency out of each wait. This only takes effect when it is the
-		// process's first HIP call, so an error (the runtime already set up) is ignored.
+		// process's first HIP call; when the runtime is already set up the call fails, and only
+		// the error that failure recorded is cleared (b2amd_SpinWaitActive reports the outcome).
 		if ( getenv( "B2AMD_SCHEDULE_YIELD" ) == nullptr )
 		{
-			(void)hipSetDeviceFlags( hipDeviceScheduleSpin );
-			(void)hipGetLastError();
+			if ( hipSetDeviceFlags( hipDeviceScheduleSpin ) != hipSuccess )
+			{
+				(void)hipGetLastError();
+			}
 		}
 		HIP_CHECK( hipFuncSetAttribute( reinterpret_cast<const void*>( SolveFusedKernel ),
 										hipFuncAttributeMaxDynamicSharedMemorySize, FusedLdsBytes( kFusedBodyCapacity ) ) );
@@ -3808,6 +3812,17 @@ void DeviceSynchronize()
 	HIP_CHECK( hipDeviceSynchronize() );
 }
 
+bool SpinWaitActive()
+{
+	unsigned int flags = 0;
+	if ( hipGetDeviceFlags( &flags ) != hipSuccess )
+	{
+		(void)hipGetLastError();
+		return false;
+	}
+	return ( flags & hipDeviceScheduleMask ) == hipDeviceScheduleSpin;
+}
+
 int CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
 				  b2LocalManifold* manifolds )
 {
@@ -3873,6 +3888,7 @@ int CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const 
 	HIP_CHECK( hipFree( dM ) );
 	return 0;
 }
+
 
 } // namespace b2g
 

@@ -52,10 +52,15 @@ public:
 	void FlushEnlarged();
 
 	// Stack-based depth-first query; child2 is visited before child1. The callback returns false
-	// to stop the query.
+	// to stop the query. Internal node boxes are only complete once deferred enlargements are
+	// flushed, so querying with any pending is a caller error.
 	template <typename F>
 	void Query( b2AABB aabb, uint64_t maskBits, F&& callback ) const
 	{
+		if ( pendingEnlarge.empty() == false )
+		{
+			Fatal( "dynamic tree queried with %d deferred enlargements not flushed", (int)pendingEnlarge.size() );
+		}
 		if ( nodeCount == 0 )
 		{
 			return;

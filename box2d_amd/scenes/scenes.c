@@ -1302,6 +1302,29 @@ double b2s_step_timed( int handle, float timeStep, int subStepCount, int count )
 	return b2sNowMs() - start;
 }
 
+double b2s_step_profiled( int handle, float timeStep, int subStepCount, int count, double* profileSums )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return -1.0;
+	}
+	double total = 0.0;
+	for ( int i = 0; i < count; ++i )
+	{
+		double start = b2sNowMs();
+		b2World_Step( scene->worldId, timeStep, subStepCount );
+		total += b2sNowMs() - start;
+		b2Profile p = b2World_GetProfile( scene->worldId );
+		const float* f = &p.step;
+		for ( int k = 0; k < (int)( sizeof( b2Profile ) / sizeof( float ) ); ++k )
+		{
+			profileSums[k] += f[k];
+		}
+	}
+	return total;
+}
+
 int b2s_body_ids( int handle, int* out, int capacity )
 {
 	b2sScene* scene = b2sGet( handle );

@@ -100,6 +100,10 @@ B2S_API void b2s_step( int handle, float timeStep, int subStepCount, int count )
 // Wall-clock milliseconds spent in count steps (monotonic clock around b2World_Step calls).
 B2S_API double b2s_step_timed( int handle, float timeStep, int subStepCount, int count );
 
+// b2s_step_timed that also adds each step's b2World_GetProfile fields (23 floats, b2Profile
+// order) into profileSums[0..22] after the step; the profile read sits outside the clock.
+B2S_API double b2s_step_profiled( int handle, float timeStep, int subStepCount, int count, double* profileSums );
+
 // Body ids (index1 - 1 of each b2BodyId) of the scene bodies in creation order. Returns the count.
 B2S_API int b2s_body_ids( int handle, int* out, int capacity );
 

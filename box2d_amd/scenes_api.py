@@ -12,6 +12,12 @@ class WorldId(ctypes.Structure):
     _fields_ = [("index1", ctypes.c_uint16), ("generation", ctypes.c_uint16)]
 
 
+# b2Profile fields in declaration order (include/box2d/types.h)
+PROFILE_FIELDS = ("step", "pairs", "collide", "solve", "solverSetup", "constraints", "prepareConstraints",
+                  "integrateVelocities", "warmStart", "solveImpulses", "integratePositions", "relaxImpulses",
+                  "applyRestitution", "storeImpulses", "splitIslands", "transforms", "sensorHits", "jointEvents",
+                  "hitEvents", "refit", "bullets", "sleepIslands", "sensors")
+
 SCENE_PYRAMID = 0
 SCENE_FALLING_HINGES = 1
 SCENE_HELLO = 2
@@ -35,6 +41,9 @@ class SceneLib:
         lib.b2s_step.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int]
         lib.b2s_step_timed.restype = ctypes.c_double
         lib.b2s_step_timed.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int]
+        lib.b2s_step_profiled.restype = ctypes.c_double
+        lib.b2s_step_profiled.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_double)]
         lib.b2s_body_count.restype = ctypes.c_int
         lib.b2s_hash.restype = ctypes.c_uint32
         lib.b2s_probe_hash.restype = ctypes.c_uint32
@@ -73,6 +82,12 @@ class SceneLib:
 
     def step_timed(self, h, count, dt=1.0 / 60.0, substeps=4):
         return self.lib.b2s_step_timed(h, ctypes.c_float(dt), substeps, count)
+
+    def step_profiled(self, h, count, dt=1.0 / 60.0, substeps=4):
+        """Wall-clock ms of count steps and the mean b2Profile fields (ms) over those steps."""
+        sums = (ctypes.c_double * len(PROFILE_FIELDS))()
+        ms = self.lib.b2s_step_profiled(h, ctypes.c_float(dt), substeps, count, sums)
+        return ms, {f: sums[i] / max(1, count) for i, f in enumerate(PROFILE_FIELDS)}
 
     def body_count(self, h):
         return self.lib.b2s_body_count(h)

@@ -68,6 +68,11 @@ B2_API int b2amd_World_GetLastPairQuery( b2WorldId worldId );
 /// (a device-wide synchronize; b2World_Step already returns with its own work complete).
 B2_API void b2amd_Synchronize( void );
 
+/// 1 when the current device waits by spinning (hipDeviceScheduleSpin), which the library asks
+/// for when it makes the process's first HIP call (unless B2AMD_SCHEDULE_YIELD is set); 0 when
+/// the runtime was already set up by the host application with another wait mode.
+B2_API int b2amd_SpinWaitActive( void );
+
 /// A shape for batched narrowphase evaluation. type is a b2ShapeType; the union holds the
 /// geometry in the shape's local frame.
 typedef struct b2amd_Shape

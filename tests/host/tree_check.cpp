@@ -1,7 +1,8 @@
 // Host check of the broadphase tree: enlarging proxies with the ancestor update deferred
 // (EnlargeProxyDeferred, flushed by the next structural change or rebuild) leaves the same tree
 // as enlarging them at once (EnlargeProxy), in any enlargement order, through partial and full
-// rebuilds and proxy churn. Prints one FNV-1a hash of every node per variant.
+// rebuilds and proxy churn, and queries made after the flush visit the same leaves in the same
+// order. Prints one FNV-1a hash of every query hit and every node per variant.
 #include "tree.h"
 
 #include <algorithm>
@@ -38,6 +39,15 @@ static uint64_t Run( int n, int variant )
 	s_rng = 12345u;
 	DynamicTree tree;
 	std::vector<int> ids;
+	uint64_t h = 1469598103934665603ull;
+	auto mix = [&]( const void* p, size_t s ) {
+		const unsigned char* c = (const unsigned char*)p;
+		for ( size_t i = 0; i < s; ++i )
+		{
+			h ^= c[i];
+			h *= 1099511628211ull;
+		}
+	};
 	for ( int i = 0; i < n; ++i )
 	{
 		float x = 200.0f * Random(), y = 100.0f * Random(), w = 0.2f + Random(), h = 0.2f + Random();
@@ -68,6 +78,21 @@ static uint64_t Run( int n, int variant )
 				tree.EnlargeProxyDeferred( m.first, m.second );
 			}
 		}
+		if ( variant != 0 )
+		{
+			tree.FlushEnlarged();
+		}
+		// queries between the refit and the rebuild (the pair query's position in a step) visit
+		// the same leaves in the same order after a flush as after immediate enlargement
+		for ( int k = 0; k < n; k += 53 )
+		{
+			b2AABB box = tree.GetAABB( ids[k] );
+			tree.Query( box, ~0ull, [&]( int proxyId, uint64_t userData ) {
+				mix( &proxyId, sizeof( proxyId ) );
+				mix( &userData, sizeof( userData ) );
+				return true;
+			} );
+		}
 		if ( round == 4 )
 		{
 			for ( int k = 0; k < n; k += 37 )
@@ -78,15 +103,6 @@ static uint64_t Run( int n, int variant )
 		}
 		tree.Rebuild( round == 7 );
 	}
-	uint64_t h = 1469598103934665603ull;
-	auto mix = [&]( const void* p, size_t s ) {
-		const unsigned char* c = (const unsigned char*)p;
-		for ( size_t i = 0; i < s; ++i )
-		{
-			h ^= c[i];
-			h *= 1099511628211ull;
-		}
-	};
 	for ( const TreeNode& node : tree.Nodes() )
 	{
 		mix( &node.aabb, sizeof( node.aabb ) );
