@@ -16,6 +16,8 @@ void SetLogFcn( b2LogFcn* fcn );
 void SetAssertFcn( b2AssertFcn* fcn );
 int CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
 				  b2LocalManifold* manifolds );
+int CollideBatchNoAlloc( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
+						 b2LocalManifold* manifolds );
 void DeviceSynchronize();
 bool SpinWaitActive();
 int TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs );
@@ -1354,6 +1356,12 @@ int b2amd_CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, 
 						b2LocalManifold* manifolds )
 {
 	return CollideBatch( shapesA, shapesB, transforms, count, manifolds );
+}
+
+int b2amd_CollideBatchNoAlloc( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
+							   b2LocalManifold* manifolds )
+{
+	return CollideBatchNoAlloc( shapesA, shapesB, transforms, count, manifolds );
 }
 
 } // extern "C"

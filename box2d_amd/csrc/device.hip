@@ -3823,6 +3823,135 @@ bool SpinWaitActive()
 	return ( flags & hipDeviceScheduleMask ) == hipDeviceScheduleSpin;
 }
 
+// A pair the batched narrowphase accepts: known shape types in the reference's primary order.
+static bool CollidePairSupported( int tA, int tB )
+{
+	bool known = ( tA == b2_circleShape || tA == b2_capsuleShape || tA == b2_segmentShape || tA == b2_polygonShape ) &&
+				 ( tB == b2_circleShape || tB == b2_capsuleShape || tB == b2_segmentShape || tB == b2_polygonShape );
+	bool primary = ( tA == tB ) || ( tA == b2_capsuleShape && tB == b2_circleShape ) ||
+				   ( tA == b2_polygonShape && ( tB == b2_circleShape || tB == b2_capsuleShape ) ) ||
+				   ( tA == b2_segmentShape && tB != b2_segmentShape );
+	return known && primary && ( tA == b2_segmentShape && tB == b2_segmentShape ) == false;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched narrowphase without allocation (b2amd_CollideBatchNoAlloc).
+//
+// The pairs and their manifolds are staged in a file-scope device array instead of buffers
+// allocated per call. Each GPU holds its own copy of the array for the life of the process; its
+// address on a device is looked up once, with that device current, and kept per device. Every
+// call on one GPU reuses that GPU's array, so calls that overlap on one GPU race.
+
+constexpr int kNoAllocCollidePairs = 4096;
+constexpr int kNoAllocMaxDevices = 64;
+
+struct CollideScratch
+{
+	b2amd_Shape shapesA[kNoAllocCollidePairs];
+	b2amd_Shape shapesB[kNoAllocCollidePairs];
+	b2Transform transforms[kNoAllocCollidePairs];
+	b2LocalManifold manifolds[kNoAllocCollidePairs];
+};
+
+static __device__ __align__( 256 ) CollideScratch scratch;
+
+static CollideScratch* s_scratchByDevice[kNoAllocMaxDevices] = {};
+static std::mutex s_scratchMutex;
+
+// The current device's address of the scratch array (looked up on the device's first call).
+static CollideScratch* ScratchOfCurrentDevice()
+{
+	int device = 0;
+	if ( hipGetDevice( &device ) != hipSuccess || device < 0 || device >= kNoAllocMaxDevices )
+	{
+		(void)hipGetLastError();
+		Log( "b2amd_CollideBatchNoAlloc: no usable current GPU (device %d)", device );
+		return nullptr;
+	}
+	std::lock_guard<std::mutex> lock( s_scratchMutex );
+	if ( s_scratchByDevice[device] == nullptr )
+	{
+		void* ptr = nullptr;
+		hipError_t e = hipGetSymbolAddress( &ptr, HIP_SYMBOL( scratch ) );
+		if ( e != hipSuccess || ptr == nullptr )
+		{
+			(void)hipGetLastError();
+			Log( "b2amd_CollideBatchNoAlloc: hipGetSymbolAddress failed on device %d: %s", device, hipGetErrorString( e ) );
+			return nullptr;
+		}
+		s_scratchByDevice[device] = static_cast<CollideScratch*>( ptr );
+	}
+	return s_scratchByDevice[device];
+}
+
+// One lane per pair: the geometry is read in its public form and placed in the world step's
+// shape record (all other fields zero, as b2amd_CollideBatch stages it) before the manifold.
+__global__ void CollidePublicShapesKernel( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* xfs,
+										   int count, b2LocalManifold* out )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i >= count )
+	{
+		return;
+	}
+	ShapeDev s[2];
+	const b2amd_Shape* src[2] = { shapesA + i, shapesB + i };
+	for ( int k = 0; k < 2; ++k )
+	{
+		memset( &s[k], 0, sizeof( ShapeDev ) );
+		s[k].type = src[k]->type;
+		switch ( src[k]->type )
+		{
+			case b2_circleShape:
+				s[k].circle = src[k]->circle;
+				break;
+			case b2_capsuleShape:
+				s[k].capsule = src[k]->capsule;
+				break;
+			case b2_segmentShape:
+				s[k].segment = src[k]->segment;
+				break;
+			default:
+				s[k].polygon = src[k]->polygon;
+				break;
+		}
+	}
+	out[i] = ComputeManifold( s[0], s[1], xfs[i] );
+}
+
+int CollideBatchNoAlloc( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
+						 b2LocalManifold* manifolds )
+{
+	if ( count <= 0 )
+	{
+		return 0;
+	}
+	if ( count > kNoAllocCollidePairs )
+	{
+		Log( "b2amd_CollideBatchNoAlloc: %d pairs exceed the scratch capacity of %d pairs", count, kNoAllocCollidePairs );
+		return -2;
+	}
+	for ( int i = 0; i < count; ++i )
+	{
+		if ( CollidePairSupported( shapesA[i].type, shapesB[i].type ) == false )
+		{
+			return -1;
+		}
+	}
+	CollideScratch* ptr = ScratchOfCurrentDevice();
+	if ( ptr == nullptr )
+	{
+		return -3;
+	}
+	HIP_CHECK( hipMemcpy( ptr->shapesA, shapesA, count * sizeof( b2amd_Shape ), hipMemcpyHostToDevice ) );
+	HIP_CHECK( hipMemcpy( ptr->shapesB, shapesB, count * sizeof( b2amd_Shape ), hipMemcpyHostToDevice ) );
+	HIP_CHECK( hipMemcpy( ptr->transforms, transforms, count * sizeof( b2Transform ), hipMemcpyHostToDevice ) );
+	CollidePublicShapesKernel<<<Blocks( count ), kBlock>>>( ptr->shapesA, ptr->shapesB, ptr->transforms, count, ptr->manifolds );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipMemcpy( manifolds, ptr->manifolds, count * sizeof( b2LocalManifold ), hipMemcpyDeviceToHost ) );
+	return 0;
+}
+
 int CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
 				  b2LocalManifold* manifolds )
 {

@@ -96,6 +96,16 @@ typedef struct b2amd_Shape
 B2_API int b2amd_CollideBatch( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms, int count,
 							   b2LocalManifold* manifolds );
 
+/// b2amd_CollideBatch for callers that cannot allocate device memory (code under HIP graph
+/// capture, tight serving loops, hosts that forbid hipMalloc). No counterpart in the reference.
+/// The pairs and manifolds are staged in a fixed device array that each GPU holds for the life of
+/// the process; no call allocates or frees device memory. At most 4096 pairs per call: a larger
+/// count returns -2 and logs the limit. Same arguments, results and -1 as b2amd_CollideBatch;
+/// -3 when the staging array cannot be located on the current device. Calls that overlap on one
+/// GPU share its staging array and race: serialize them.
+B2_API int b2amd_CollideBatchNoAlloc( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB, const b2Transform* transforms,
+									  int count, b2LocalManifold* manifolds );
+
 /// Evaluate count time-of-impact queries on the GPU (the continuous-collision kernel's code;
 /// reference src/distance.c:1148-1441). Host arrays in, host array out; synchronous. Returns 0.
 B2_API int b2amd_TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs );
