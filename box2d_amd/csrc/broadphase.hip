@@ -548,11 +548,77 @@ __global__ __launch_bounds__( kLbvhBlock ) void LbvhLargeQueryKernel( LbvhQueryA
 }
 
 // ---------------------------------------------------------------------------------------------
+// Chunk scan: the default query.
+//
+// The staged proxies are in the reference trees' depth-first leaf order, which keeps neighbours
+// in space near each other in the array, so 64-proxy chunks of it have tight boxes. One wavefront
+// per chunk takes the union of its 64 boxes; then one wavefront per query tests 64 chunk boxes at
+// a time against the query box and, for each overlapping chunk in ascending order, its 64
+// proxies, one per lane. A query is a handful of wavefront-wide loads instead of one lane's
+// dependent walk, and its hits come out in ascending proxy order (chunks ascending, lanes
+// ascending, and each wavefront's appends at increasing counter values), so no sort is needed.
+
+__global__ __launch_bounds__( kLbvhBlock ) void PairChunkBoxKernel( const LbvhProxy* proxies, int n, b2AABB* chunkBox )
+{
+	int chunk = ( blockIdx.x * blockDim.x + threadIdx.x ) >> 6;
+	int lane = threadIdx.x & 63;
+	if ( chunk * kLbvhChunk >= n )
+	{
+		return; // whole wavefront
+	}
+	int p = chunk * kLbvhChunk + lane;
+	b2AABB box = p < n ? proxies[p].aabb : EmptyBox();
+	box = WaveUnion( box );
+	if ( lane == 0 )
+	{
+		chunkBox[chunk] = box;
+	}
+}
+
+__global__ __launch_bounds__( kLbvhBlock ) void PairChunkQueryKernel( LbvhQueryArgs a, const b2AABB* chunkBox, int chunkCount )
+{
+	int q = ( blockIdx.x * blockDim.x + threadIdx.x ) >> 6;
+	int lane = threadIdx.x & 63;
+	if ( q >= a.queryCount )
+	{
+		return; // whole wavefront
+	}
+	const LbvhProxy query = a.queries[q];
+	const b2AABB box = query.aabb;
+	for ( int base = 0; base < chunkCount; base += 64 )
+	{
+		int c = base + lane;
+		bool overlap = c < chunkCount && Overlaps( chunkBox[c], box );
+		uint64_t chunks = __ballot( overlap );
+		while ( chunks != 0 )
+		{
+			int first = __ffsll( (unsigned long long)chunks ) - 1;
+			chunks &= chunks - 1;
+			int p = ( base + first ) * kLbvhChunk + lane;
+			bool hit = false;
+			if ( p < a.proxyCount )
+			{
+				const LbvhProxy proxy = a.proxies[p];
+				hit = Overlaps( proxy.aabb, box ) && LbvhKeep( a, query.proxyKey, query.shapeId, proxy.proxyKey, proxy.shapeId );
+			}
+			EmitHits( a, hit, ( (uint64_t)q << 32 ) | (uint32_t)p );
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
 // Host side
+
+// Keys copied back with the candidate count, before the count is known (one round trip when the
+// candidates fit).
+constexpr int kEagerKeys = 4096;
 
 struct PairFinder::Impl
 {
 	hipStream_t stream = nullptr;
+	int mode = 0; // 0 chunk scan, 1 LBVH walk
+	b2AABB* chunkBoxes = nullptr;
+	size_t chunkCap = 0;
 	size_t proxyCap = 0, queryCap = 0, candCap = 0;
 	LbvhProxy* proxies = nullptr;
 	LbvhProxy* queries = nullptr;
@@ -684,6 +750,7 @@ PairFinder::~PairFinder()
 	hipFree( m->tree.rightOf );
 	hipFree( m->tree.rangeLo );
 	hipFree( m->tree.rangeHi );
+	hipFree( m->chunkBoxes );
 	hipFree( m->cands );
 	hipFree( m->sortedCands );
 	hipFree( m->pairSlots );
@@ -789,6 +856,11 @@ uint8_t* PairFinder::MovedStaging( int count )
 	return GrowPinned( impl->hostMoved, impl->hostMovedCap, (size_t)count );
 }
 
+void PairFinder::SetMode( int mode )
+{
+	impl->mode = mode;
+}
+
 void PairFinder::Launch( int n, int nq, int movedCount )
 {
 	Impl* m = impl;
@@ -826,6 +898,22 @@ void PairFinder::Launch( int n, int nq, int movedCount )
 	HIP_CHECK( hipMemcpyAsync( m->proxies, m->hostProxies, n * sizeof( LbvhProxy ), hipMemcpyHostToDevice, s ) );
 	HIP_CHECK( hipMemcpyAsync( m->queries, m->hostQueries, nq * sizeof( LbvhProxy ), hipMemcpyHostToDevice, s ) );
 	HIP_CHECK( hipMemcpyAsync( m->moved, m->hostMoved, movedCount, hipMemcpyHostToDevice, s ) );
+
+	if ( m->mode == 0 )
+	{
+		int chunkCount = ( n + kLbvhChunk - 1 ) / kLbvhChunk;
+		if ( (size_t)chunkCount > m->chunkCap )
+		{
+			hipFree( m->chunkBoxes );
+			m->chunkCap = (size_t)chunkCount + chunkCount / 2 + 16;
+			HIP_CHECK( hipMalloc( &m->chunkBoxes, m->chunkCap * sizeof( b2AABB ) ) );
+		}
+		int waves = kLbvhBlock / 64;
+		PairChunkBoxKernel<<<( chunkCount + waves - 1 ) / waves, kLbvhBlock, 0, s>>>( m->proxies, n, m->chunkBoxes );
+		HIP_CHECK( hipGetLastError() );
+		m->Query();
+		return;
+	}
 
 	LbvhTree t = m->tree;
 	t.n = n;
@@ -868,6 +956,19 @@ void PairFinder::Impl::Query()
 	qa.pairMask = pairSize > 0 ? pairSize - 1 : 0;
 	qa.filters = filterCount > 0 ? filters : nullptr;
 	HIP_CHECK( hipMemsetAsync( counters, 0, 3 * sizeof( int ), stream ) );
+	if ( mode == 0 )
+	{
+		int waves = kLbvhBlock / 64;
+		int chunkCount = ( launchedProxies + kLbvhChunk - 1 ) / kLbvhChunk;
+		PairChunkQueryKernel<<<( launchedQueries + waves - 1 ) / waves, kLbvhBlock, 0, stream>>>( qa, chunkBoxes, chunkCount );
+		HIP_CHECK( hipGetLastError() );
+		HIP_CHECK( hipMemcpyAsync( hostCounters, counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, stream ) );
+		// the first keys travel with the count
+		int eager = (int)std::min<size_t>( candCap, kEagerKeys );
+		GrowPinned( hostCands, hostCandCap, (size_t)eager );
+		HIP_CHECK( hipMemcpyAsync( hostCands, cands, eager * sizeof( uint64_t ), hipMemcpyDeviceToHost, stream ) );
+		return;
+	}
 	int queryBlocks = ( launchedQueries + kLbvhBlock - 1 ) / kLbvhBlock;
 	LbvhLargeListKernel<<<queryBlocks, kLbvhBlock, 0, stream>>>( qa );
 	LbvhQueryKernel<<<queryBlocks, kLbvhBlock, 0, stream>>>( qa );
@@ -912,6 +1013,19 @@ const uint64_t* PairFinder::Finish( int* count )
 	if ( total == 0 )
 	{
 		return nullptr;
+	}
+
+	if ( m->mode == 0 )
+	{
+		// each query's keys are already in ascending proxy order
+		if ( total > kEagerKeys )
+		{
+			GrowPinned( m->hostCands, m->hostCandCap, (size_t)total );
+			HIP_CHECK( hipMemcpyAsync( m->hostCands, m->cands, total * sizeof( uint64_t ), hipMemcpyDeviceToHost, s ) );
+			HIP_CHECK( hipStreamSynchronize( s ) );
+		}
+		*count = total;
+		return m->hostCands;
 	}
 
 	// (query, proxy index) order: bits 0..31 proxy index, 32.. query index

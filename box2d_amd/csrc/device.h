@@ -196,8 +196,9 @@ struct BatchStats
 // Step count worlds as one batch (defined in batch.cpp). Returns the number of worlds stepped.
 int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, BatchStats* stats );
 
-// GPU broadphase pair query (broadphase.hip): a Morton-ordered LBVH over every proxy of a world,
-// queried with the fat boxes of the moved proxies.
+// GPU broadphase pair query (broadphase.hip): every proxy of a world in the reference trees' visit
+// order, queried with the fat boxes of the moved proxies through 64-proxy chunk boxes (or a
+// Morton-ordered LBVH).
 struct LbvhProxy
 {
 	int proxyKey; // proxy id << 2 | body type of its tree
@@ -250,9 +251,14 @@ public:
 	LbvhProxy* QueryStaging( int count );
 	uint8_t* MovedStaging( int count );
 
+	// Search structure of the following launches: 0 a scan of 64-proxy chunk boxes, one wavefront
+	// per query (the default); 1 a Morton-ordered LBVH walked by one lane per query.
+	void SetMode( int mode );
+
 	// Every (query, proxy) whose fat boxes overlap, that passes the pair callback's self, tree,
 	// moved-proxy, same-body and filter rules and whose shape pair is not in the mirrored pair set,
-	// as (query index << 32 | proxy index) keys in ascending order. Launch
+	// as (query index << 32 | proxy index) keys, each query's keys in ascending proxy order (the
+	// LBVH sorts all keys; the chunk scan may interleave different queries' keys). Launch
 	// uploads the staging and starts the device work without waiting (the staging may be refilled
 	// only after Finish); Finish waits and returns the keys, valid until the next Launch.
 	void Launch( int proxyCount, int queryCount, int movedCount );

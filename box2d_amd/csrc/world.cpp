@@ -2714,6 +2714,7 @@ void World::QueryPairsGpuLaunch()
 	{
 		pairFinder = new PairFinder();
 	}
+	pairFinder->SetMode( pairQuery == 3 ? 1 : 0 );
 
 	static const b2BodyType kVisitOrder[3] = { b2_kinematicBody, b2_staticBody, b2_dynamicBody };
 	int proxyCount = 0;
@@ -2844,9 +2845,9 @@ void World::UpdateBroadPhasePairs()
 
 	static const int s_gpuPairThreshold = []() {
 		const char* v = getenv( "B2AMD_GPU_PAIR_THRESHOLD" );
-		return v != nullptr ? atoi( v ) : 1024;
+		return v != nullptr ? atoi( v ) : kGpuPairThreshold;
 	}();
-	bool gpu = pairQuery == 2 || ( pairQuery == 0 && moveCount >= s_gpuPairThreshold );
+	bool gpu = pairQuery == 2 || pairQuery == 3 || ( pairQuery == 0 && moveCount >= s_gpuPairThreshold );
 	gpu = gpu && device->Recording() == false;
 	lastPairQueryPath = gpu ? 2 : 1;
 

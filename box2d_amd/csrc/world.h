@@ -353,6 +353,9 @@ public:
 	void EnlargeProxy( int proxyKey, b2AABB aabb );
 };
 
+// Move-buffer size from which the automatic pair query runs on the GPU.
+constexpr int kGpuPairThreshold = 128;
+
 class World
 {
 public:
@@ -467,8 +470,9 @@ public:
 
 	DeviceWorld* device = nullptr;
 
-	// Pair query: 0 auto (GPU LBVH for large move buffers outside batched stepping), 1 host tree
-	// query, 2 GPU LBVH. Both give the same pairs in the same order.
+	// Pair query: 0 auto (the GPU chunk scan for move buffers of at least kGpuPairThreshold
+	// proxies, outside batched stepping), 1 host tree query, 2 GPU chunk scan, 3 GPU LBVH walk.
+	// All give the same pairs in the same order.
 	int pairQuery = 0;
 	class PairFinder* pairFinder = nullptr;
 	int lastPairQueryPath = 0; // 1 host, 2 GPU (for tests)

@@ -48,15 +48,18 @@ typedef enum b2amd_SolverPath
 B2_API void b2amd_World_SetSolverPath( b2WorldId worldId, b2amd_SolverPath path );
 
 /// Broadphase pair query of a world's step (the move-buffer query of reference
-/// src/broad_phase.c:412-529). The GPU query builds a Morton-ordered linear BVH over every proxy
-/// and walks it with one lane per moved proxy; the host orders the hits in the reference trees'
-/// visit order. Both give the same pairs in the same order; Auto uses the GPU query when at least
-/// 1024 proxies moved (outside b2amd_StepWorlds, whose host phase uses the tree query).
+/// src/broad_phase.c:412-529). The GPU query stages every proxy in the reference trees' visit
+/// order and tests each moved proxy's box against 64-proxy chunk boxes of that order, one
+/// wavefront per query (b2amd_pairQueryGpu), or walks a Morton-ordered linear BVH with one lane
+/// per query (b2amd_pairQueryGpuLbvh). All give the same pairs in the same order; Auto uses the
+/// chunk scan when at least 128 proxies moved (outside b2amd_StepWorlds, whose host phase uses
+/// the tree query).
 typedef enum b2amd_PairQuery
 {
 	b2amd_pairQueryAuto = 0,
 	b2amd_pairQueryHostTree = 1,
 	b2amd_pairQueryGpu = 2,
+	b2amd_pairQueryGpuLbvh = 3,
 } b2amd_PairQuery;
 
 B2_API void b2amd_World_SetPairQuery( b2WorldId worldId, b2amd_PairQuery mode );

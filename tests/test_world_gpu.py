@@ -18,7 +18,7 @@ def scene(world_runs, name):
 
 def run_and_compare(gpu_scenes, fx, path=0, pair_query=None):
     """Step the scene and compare with the fixture. pair_query forces the broadphase pair query
-    (1 host tree, 2 GPU LBVH); returns how many steps ran each path."""
+    (1 host tree, 2 GPU chunk scan, 3 GPU LBVH walk); returns how many steps ran each path."""
     import box2d_amd
 
     lib = box2d_amd.library()
@@ -50,13 +50,15 @@ def run_and_compare(gpu_scenes, fx, path=0, pair_query=None):
     return used
 
 
+@pytest.mark.parametrize("mode", [2, 3], ids=["chunk_scan", "lbvh"])
 @pytest.mark.parametrize("name", ["hello", "pyramid20_sleep", "mixed_pile_1", "mixed_pile_2", "mixed_pile_3",
                                   "falling_hinges_trace", "rain_small"])
-def test_gpu_pair_query_matches_reference(gpu_scenes, world_runs, name):
-    """Broadphase pairs from the GPU LBVH query on every step (ordered on the host in the
-    reference trees' visit order): the whole run stays bit-identical to the reference."""
-    print(f"[gpu] scene {name} (GPU pair query)", flush=True)
-    used = run_and_compare(gpu_scenes, scene(world_runs, name), pair_query=2)
+def test_gpu_pair_query_matches_reference(gpu_scenes, world_runs, name, mode):
+    """Broadphase pairs from the GPU query on every step, by the chunk-box scan (mode 2) or the
+    LBVH walk (mode 3), ordered in the reference trees' visit order: the whole run stays
+    bit-identical to the reference."""
+    print(f"[gpu] scene {name} (GPU pair query, mode {mode})", flush=True)
+    used = run_and_compare(gpu_scenes, scene(world_runs, name), pair_query=mode)
     assert used[2] > 0 and used[1] == 0
 
 
