@@ -224,7 +224,7 @@ def rain_workload(lib, scenes, steps, cpu_steps, cpu_threads, no_cpu):
     h = scenes.create(4, iparams, fparams)
     wid = scenes.world_id(h)
     bodies = scenes.body_count(h)
-    used = {0: 0, 1: 0, 2: 0}
+    used = {0: 0, 1: 0, 2: 0, 3: 0}
     first_ms = None
     lib.b2amd_Synchronize()
     t0 = time.perf_counter()
@@ -242,7 +242,7 @@ def rain_workload(lib, scenes, steps, cpu_steps, cpu_threads, no_cpu):
            "bodies": bodies, "steps": steps, "gpu_ms_per_step": total * 1e3 / steps,
            "gpu_body_substeps_per_s": bodies * SUBSTEPS * steps / total, "hash": f"{got:#010x}",
            "hash_match": None if want is None else bool(got == want),
-           "pair_query_steps": {"gpu_lbvh": used[2], "host_tree": used[1], "none": used[0]},
+           "pair_query_steps": {"gpu_speculative": used[3], "gpu_at_update": used[2], "host_tree": used[1], "none": used[0]},
            "gpu_ms_per_step_first_steps": first_ms, "first_steps": cpu_steps}
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
     if not no_cpu and os.path.exists(ref_path):

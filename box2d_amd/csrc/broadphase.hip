@@ -354,6 +354,7 @@ struct LbvhQueryArgs
 	const uint64_t* pairSlots; // mirror of the existing-pair set (nullptr: empty)
 	uint64_t pairMask;
 	const PairShapeFilter* filters; // by shape id (nullptr: not synced)
+	const int* queryCountOnDevice;	// when set, the number of queries (queryCount is then a bound)
 };
 
 // Whether the shape pair already has a contact: the host set's probe sequence on its mirror.
@@ -579,7 +580,8 @@ __global__ __launch_bounds__( kLbvhBlock ) void PairChunkQueryKernel( LbvhQueryA
 {
 	int q = ( blockIdx.x * blockDim.x + threadIdx.x ) >> 6;
 	int lane = threadIdx.x & 63;
-	if ( q >= a.queryCount )
+	int queryCount = a.queryCountOnDevice != nullptr ? min( *a.queryCountOnDevice, a.queryCount ) : a.queryCount;
+	if ( q >= queryCount )
 	{
 		return; // whole wavefront
 	}
@@ -606,6 +608,39 @@ __global__ __launch_bounds__( kLbvhBlock ) void PairChunkQueryKernel( LbvhQueryA
 	}
 }
 
+// Queries of a speculative launch from a solve's enlarged shapes: each enlarged shape becomes a
+// query with its new fat box, its staged proxy takes that box, and it is marked moved. shapeSlots
+// maps a shape id to (proxy key, staged position).
+__global__ __launch_bounds__( kLbvhBlock ) void PairSpeculativeBuildKernel( const int* enlargedShapes, const b2AABB* enlargedAABBs,
+																		   const int* enlargedCount, int capacity, const ShapeSlot* shapeSlots,
+																		   int shapeCount, int proxyCount, LbvhProxy* queries, LbvhProxy* proxies,
+																		   uint8_t* moved, int* queryCount, int* error )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	int count = *enlargedCount;
+	if ( i == 0 )
+	{
+		*queryCount = count;
+	}
+	if ( i >= count || i >= capacity )
+	{
+		return;
+	}
+	int shapeId = enlargedShapes[i];
+	b2AABB box = enlargedAABBs[i];
+	ShapeSlot slot = shapeId >= 0 && shapeId < shapeCount ? shapeSlots[shapeId] : ShapeSlot{ -1, -1 };
+	if ( slot.position < 0 || slot.position >= proxyCount )
+	{
+		// not a staged proxy: the result is incomplete and the caller queries again
+		atomicAdd( error, 1 );
+		queries[i] = LbvhProxy{ -1, -1, EmptyBox() };
+		return;
+	}
+	queries[i] = LbvhProxy{ slot.proxyKey, shapeId, box };
+	proxies[slot.position].aabb = box;
+	moved[shapeId] = 1;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Host side
 
@@ -619,6 +654,11 @@ struct PairFinder::Impl
 	int mode = 0; // 0 chunk scan, 1 LBVH walk
 	b2AABB* chunkBoxes = nullptr;
 	size_t chunkCap = 0;
+	ShapeSlot* shapeSlots = nullptr; // speculative launches: shape id -> (proxy key, staged position)
+	size_t shapeSlotCap = 0;
+	ShapeSlot* hostShapeSlots = nullptr;
+	size_t hostShapeSlotCap = 0;
+	bool speculative = false; // the launched query is a speculative one
 	size_t proxyCap = 0, queryCap = 0, candCap = 0;
 	LbvhProxy* proxies = nullptr;
 	LbvhProxy* queries = nullptr;
@@ -727,13 +767,14 @@ PairFinder::PairFinder()
 	impl = new Impl();
 	HIP_CHECK( hipStreamCreate( &impl->stream ) );
 	HIP_CHECK( hipMalloc( &impl->bounds, 5 * sizeof( float ) ) );
-	HIP_CHECK( hipMalloc( &impl->counters, 3 * sizeof( int ) ) );
-	HIP_CHECK( hipHostMalloc( &impl->hostCounters, 2 * sizeof( int ), hipHostMallocDefault ) );
+	HIP_CHECK( hipMalloc( &impl->counters, 4 * sizeof( int ) ) );
+	HIP_CHECK( hipHostMalloc( &impl->hostCounters, 4 * sizeof( int ), hipHostMallocDefault ) );
 }
 
 PairFinder::~PairFinder()
 {
 	Impl* m = impl;
+	(void)hipStreamSynchronize( m->stream );
 	hipFree( m->proxies );
 	hipFree( m->queries );
 	hipFree( m->large );
@@ -751,6 +792,8 @@ PairFinder::~PairFinder()
 	hipFree( m->tree.rangeLo );
 	hipFree( m->tree.rangeHi );
 	hipFree( m->chunkBoxes );
+	hipFree( m->shapeSlots );
+	hipHostFree( m->hostShapeSlots );
 	hipFree( m->cands );
 	hipFree( m->sortedCands );
 	hipFree( m->pairSlots );
@@ -865,6 +908,7 @@ void PairFinder::Launch( int n, int nq, int movedCount )
 {
 	Impl* m = impl;
 	hipStream_t s = m->stream;
+	m->speculative = false;
 	m->launchedProxies = n;
 	m->launchedQueries = nq;
 	if ( n == 0 || nq == 0 )
@@ -955,14 +999,18 @@ void PairFinder::Impl::Query()
 	qa.pairSlots = pairSize > 0 ? pairSlots : nullptr;
 	qa.pairMask = pairSize > 0 ? pairSize - 1 : 0;
 	qa.filters = filterCount > 0 ? filters : nullptr;
-	HIP_CHECK( hipMemsetAsync( counters, 0, 3 * sizeof( int ), stream ) );
-	if ( mode == 0 )
+	qa.queryCountOnDevice = speculative ? counters + 3 : nullptr;
+	if ( speculative == false )
+	{
+		HIP_CHECK( hipMemsetAsync( counters, 0, 3 * sizeof( int ), stream ) );
+	}
+	if ( mode == 0 || speculative )
 	{
 		int waves = kLbvhBlock / 64;
 		int chunkCount = ( launchedProxies + kLbvhChunk - 1 ) / kLbvhChunk;
 		PairChunkQueryKernel<<<( launchedQueries + waves - 1 ) / waves, kLbvhBlock, 0, stream>>>( qa, chunkBoxes, chunkCount );
 		HIP_CHECK( hipGetLastError() );
-		HIP_CHECK( hipMemcpyAsync( hostCounters, counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, stream ) );
+		HIP_CHECK( hipMemcpyAsync( hostCounters, counters, 4 * sizeof( int ), hipMemcpyDeviceToHost, stream ) );
 		// the first keys travel with the count
 		int eager = (int)std::min<size_t>( candCap, kEagerKeys );
 		GrowPinned( hostCands, hostCandCap, (size_t)eager );
@@ -975,6 +1023,118 @@ void PairFinder::Impl::Query()
 	LbvhLargeQueryKernel<<<( launchedProxies + kLbvhBlock - 1 ) / kLbvhBlock, kLbvhBlock, 0, stream>>>( qa );
 	HIP_CHECK( hipGetLastError() );
 	HIP_CHECK( hipMemcpyAsync( hostCounters, counters, 2 * sizeof( int ), hipMemcpyDeviceToHost, stream ) );
+}
+
+ShapeSlot* PairFinder::ShapeSlotStaging( int count )
+{
+	return GrowPinned( impl->hostShapeSlots, impl->hostShapeSlotCap, (size_t)count );
+}
+
+void PairFinder::LaunchSpeculative( int n, int shapeCount, const SolvePairInputs& in )
+{
+	Impl* m = impl;
+	hipStream_t s = m->stream;
+	m->speculative = true;
+	int nq = std::max( in.capacity, 1 );
+	m->launchedProxies = n;
+	m->launchedQueries = nq;
+	m->mode = 0;
+	m->Reserve( std::max( n, 1 ) );
+	if ( (size_t)nq > m->queryCap )
+	{
+		hipFree( m->queries );
+		hipFree( m->large );
+		hipFree( m->largeList );
+		m->queryCap = std::max<size_t>( 256, nq + nq / 2 );
+		HIP_CHECK( hipMalloc( &m->queries, m->queryCap * sizeof( LbvhProxy ) ) );
+		HIP_CHECK( hipMalloc( &m->large, m->queryCap ) );
+		HIP_CHECK( hipMalloc( &m->largeList, m->queryCap * sizeof( int ) ) );
+	}
+	int movedCount = std::max( shapeCount, 1 );
+	if ( (size_t)movedCount > m->movedCap )
+	{
+		hipFree( m->moved );
+		m->movedCap = std::max<size_t>( 1024, movedCount + movedCount / 2 );
+		HIP_CHECK( hipMalloc( &m->moved, m->movedCap ) );
+	}
+	if ( (size_t)movedCount > m->shapeSlotCap )
+	{
+		hipFree( m->shapeSlots );
+		m->shapeSlotCap = std::max<size_t>( 1024, movedCount + movedCount / 2 );
+		HIP_CHECK( hipMalloc( &m->shapeSlots, m->shapeSlotCap * sizeof( ShapeSlot ) ) );
+	}
+	if ( m->candCap == 0 )
+	{
+		m->candCap = std::max<size_t>( 4096, 8 * (size_t)nq );
+		HIP_CHECK( hipMalloc( &m->cands, m->candCap * sizeof( uint64_t ) ) );
+		HIP_CHECK( hipMalloc( &m->sortedCands, m->candCap * sizeof( uint64_t ) ) );
+	}
+	int chunkCount = ( n + kLbvhChunk - 1 ) / kLbvhChunk;
+	if ( (size_t)std::max( chunkCount, 1 ) > m->chunkCap )
+	{
+		hipFree( m->chunkBoxes );
+		m->chunkCap = (size_t)chunkCount + chunkCount / 2 + 16;
+		HIP_CHECK( hipMalloc( &m->chunkBoxes, m->chunkCap * sizeof( b2AABB ) ) );
+	}
+
+	// inputs that do not depend on the solve are copied while it runs
+	if ( n > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( m->proxies, m->hostProxies, n * sizeof( LbvhProxy ), hipMemcpyHostToDevice, s ) );
+	}
+	HIP_CHECK( hipMemcpyAsync( m->shapeSlots, m->hostShapeSlots, shapeCount * sizeof( ShapeSlot ), hipMemcpyHostToDevice, s ) );
+	HIP_CHECK( hipMemsetAsync( m->moved, 0, movedCount, s ) );
+	HIP_CHECK( hipMemsetAsync( m->counters, 0, 4 * sizeof( int ), s ) );
+
+	HIP_CHECK( hipStreamWaitEvent( s, static_cast<hipEvent_t>( in.ready ), 0 ) );
+	PairSpeculativeBuildKernel<<<( nq + kLbvhBlock - 1 ) / kLbvhBlock, kLbvhBlock, 0, s>>>(
+		in.enlargedShapes, in.enlargedShapeAABBs, in.enlargedShapeCount, in.capacity, m->shapeSlots, shapeCount, n, m->queries, m->proxies,
+		m->moved, m->counters + 3, m->counters + 1 );
+	HIP_CHECK( hipGetLastError() );
+	if ( n > 0 )
+	{
+		int waves = kLbvhBlock / 64;
+		PairChunkBoxKernel<<<( chunkCount + waves - 1 ) / waves, kLbvhBlock, 0, s>>>( m->proxies, n, m->chunkBoxes );
+		HIP_CHECK( hipGetLastError() );
+		m->Query();
+	}
+	else
+	{
+		HIP_CHECK( hipMemcpyAsync( m->hostCounters, m->counters, 4 * sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	}
+}
+
+const uint64_t* PairFinder::FinishSpeculative( int* count, int* queryCount, bool* complete )
+{
+	Impl* m = impl;
+	hipStream_t s = m->stream;
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	m->speculative = false;
+	*count = 0;
+	*queryCount = m->hostCounters[3];
+	int total = m->hostCounters[0];
+	*complete = m->hostCounters[1] == 0 && total <= (int)m->candCap && *queryCount <= m->launchedQueries;
+	if ( total > (int)m->candCap )
+	{
+		// the next launch gets room for this many
+		hipFree( m->cands );
+		hipFree( m->sortedCands );
+		m->candCap = (size_t)total + total / 2;
+		HIP_CHECK( hipMalloc( &m->cands, m->candCap * sizeof( uint64_t ) ) );
+		HIP_CHECK( hipMalloc( &m->sortedCands, m->candCap * sizeof( uint64_t ) ) );
+	}
+	if ( *complete == false || total == 0 )
+	{
+		return nullptr;
+	}
+	if ( total > kEagerKeys )
+	{
+		GrowPinned( m->hostCands, m->hostCandCap, (size_t)total );
+		HIP_CHECK( hipMemcpyAsync( m->hostCands, m->cands, total * sizeof( uint64_t ), hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+	}
+	*count = total;
+	return m->hostCands;
 }
 
 const uint64_t* PairFinder::Finish( int* count )

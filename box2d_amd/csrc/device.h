@@ -85,11 +85,27 @@ struct FinalizeResult
 	int fastBodyCount = 0;
 };
 
+// Device results of a launched solve that the next step's speculative pair query reads: the
+// shapes whose fat box grew (finalize and continuous collision, slot order) with their boxes, and
+// an event on the world's stream after which they are complete.
+struct SolvePairInputs
+{
+	const int* enlargedShapes;
+	const b2AABB* enlargedShapeAABBs;
+	const int* enlargedShapeCount; // kept by the pack kernel after it clears its own counter
+	int capacity;				   // entries the enlarged-shape list can hold
+	void* ready;				   // hipEvent_t
+};
+
 class DeviceWorld
 {
 public:
 	DeviceWorld();
 	~DeviceWorld();
+
+	// The inputs of a speculative pair query behind the last SolveLaunch (false when it was
+	// recorded for a batch instead of launched).
+	bool GetSolvePairInputs( SolvePairInputs* out ) const;
 
 	// Host -> device uploads of id-indexed arrays (whole-array copies; used when the host edited them).
 	void UploadBodies( const World& world );
@@ -227,6 +243,14 @@ constexpr uint64_t PairKeyHash( uint64_t key )
 	return key;
 }
 
+// A shape's proxy in a staged pair query: its key and its position in the staged proxies
+// (-1, -1 for a shape without a proxy).
+struct ShapeSlot
+{
+	int proxyKey;
+	int position;
+};
+
 class PairFinder
 {
 public:
@@ -263,6 +287,18 @@ public:
 	// only after Finish); Finish waits and returns the keys, valid until the next Launch.
 	void Launch( int proxyCount, int queryCount, int movedCount );
 	const uint64_t* Finish( int* count );
+
+	// Speculative query for the next step, enqueued behind a launched solve: the queries are the
+	// solve's enlarged shapes (with their new boxes, which also replace their staged proxies'
+	// boxes), taken on the device once in.ready has fired. The caller fills ProxyStaging with the
+	// trees' proxies as they will stand at the next pair update (except for those boxes) and
+	// ShapeSlotStaging with (proxy key, staged position) per shape id, and syncs the pair set and
+	// filters first. FinishSpeculative waits and returns the keys (query index = the enlarged
+	// shape's slot in the solve's list), the number of queries and whether the result is complete
+	// (false when the candidates overflowed: the caller then queries again).
+	ShapeSlot* ShapeSlotStaging( int count );
+	void LaunchSpeculative( int proxyCount, int shapeCount, const SolvePairInputs& in );
+	const uint64_t* FinishSpeculative( int* count, int* queryCount, bool* complete );
 
 	struct Impl;
 

@@ -2179,6 +2179,9 @@ constexpr double kBytesFusedBodyStep = 64.0;
 // point costs one launch and one stream synchronize.
 
 constexpr int kMailHeader = 8;
+// Device counter slot that keeps the last solve's enlarged-shape count (the pack kernel clears
+// the finalize counters it reads).
+constexpr int kKeptShapeCountSlot = 12;
 constexpr int kPackThreads = 1024;
 
 struct PackCollideArgs
@@ -2236,6 +2239,7 @@ struct PackFinalizeArgs
 	const int* enlargedShapes;
 	const b2AABB* enlargedShapeAABBs;
 	const float* sleepTimes; // null when not needed
+	int* keptShapeCount;	 // receives the enlarged-shape count before it is cleared (may be null)
 	int bodyCount;
 	int shapeOffset; // mailbox offsets in ints
 	int aabbOffset;
@@ -2323,6 +2327,10 @@ B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
 		a.mail[1] = shapes;
 		a.mail[2] = fast;
 		a.mail[3] = error;
+		if ( a.keptShapeCount != nullptr )
+		{
+			*a.keptShapeCount = shapes;
+		}
 		a.counters[0] = 0;
 		a.counters[1] = 0;
 		a.counters[2] = 0;
@@ -2541,6 +2549,9 @@ struct DeviceWorld::Impl
 	}
 
 	hipEvent_t ev[4] = {};
+	// recorded after the finalize pack kernel of a launched solve (speculative pair queries wait on it)
+	hipEvent_t solvePacked = nullptr;
+	bool solvePackedValid = false;
 
 	// Per-kernel timing
 	bool timing = false;
@@ -2675,6 +2686,7 @@ DeviceWorld::DeviceWorld()
 	{
 		HIP_CHECK( hipEventCreate( &impl->ev[i] ) );
 	}
+	HIP_CHECK( hipEventCreateWithFlags( &impl->solvePacked, hipEventDisableTiming ) );
 	stream = impl->stream;
 }
 
@@ -2685,6 +2697,7 @@ DeviceWorld::~DeviceWorld()
 		return;
 	}
 	(void)hipStreamSynchronize( impl->stream );
+	(void)hipEventDestroy( impl->solvePacked );
 	impl->sims.Free();
 	impl->states.Free();
 	impl->aux.Free();
@@ -3525,6 +3538,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		pk.enlargedShapes = impl->enlargedShapes.ptr;
 		pk.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 		pk.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
+		pk.keptShapeCount = rec ? nullptr : impl->counters + kKeptShapeCountSlot;
 		pk.bodyCount = nb;
 		pk.shapeOffset = kMailHeader + nb;
 		pk.aabbOffset = pk.shapeOffset + (int)shapeCap;
@@ -3552,8 +3566,24 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		{
 			PackFinalizeKernel<<<1, kPackThreads, 0, s>>>( pk );
 			HIP_CHECK( hipGetLastError() );
+			HIP_CHECK( hipEventRecord( impl->solvePacked, s ) );
+			impl->solvePackedValid = true;
 		}
 	}
+}
+
+bool DeviceWorld::GetSolvePairInputs( SolvePairInputs* out ) const
+{
+	if ( impl->recording || impl->solvePackedValid == false )
+	{
+		return false;
+	}
+	out->enlargedShapes = impl->enlargedShapes.ptr;
+	out->enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
+	out->enlargedShapeCount = impl->counters + kKeptShapeCountSlot;
+	out->capacity = (int)impl->enlargedShapes.capacity;
+	out->ready = impl->solvePacked;
+	return true;
 }
 
 void DeviceWorld::SolveFinish( FinalizeResult* result )

@@ -498,6 +498,7 @@ void DynamicTree::DestroyProxy( int proxyId )
 
 void DynamicTree::MoveProxy( int proxyId, b2AABB aabb )
 {
+	boxVersion += 1;
 	RemoveLeaf( proxyId );
 	nodes[proxyId].aabb = aabb;
 	InsertLeaf( proxyId, false );
@@ -505,6 +506,7 @@ void DynamicTree::MoveProxy( int proxyId, b2AABB aabb )
 
 void DynamicTree::EnlargeProxy( int proxyId, b2AABB aabb )
 {
+	boxVersion += 1;
 	FlushEnlarged();
 	nodes[proxyId].aabb = aabb;
 	EnlargeAncestors( proxyId, aabb );
@@ -512,6 +514,7 @@ void DynamicTree::EnlargeProxy( int proxyId, b2AABB aabb )
 
 void DynamicTree::EnlargeProxyDeferred( int proxyId, b2AABB aabb )
 {
+	boxVersion += 1;
 	nodes[proxyId].aabb = aabb;
 	pendingEnlarge.push_back( PendingEnlarge{ proxyId, aabb } );
 }

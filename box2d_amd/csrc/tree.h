@@ -121,6 +121,11 @@ public:
 	{
 		return structureVersion;
 	}
+	// Changes whenever a leaf box is set (moves and enlargements).
+	uint32_t BoxVersion() const
+	{
+		return boxVersion;
+	}
 
 	const std::vector<TreeNode>& Nodes() const
 	{
@@ -183,6 +188,7 @@ private:
 	int nodeCount = 0;
 	int proxyCount = 0;
 	uint32_t structureVersion = 0;
+	uint32_t boxVersion = 0;
 
 	struct PendingEnlarge
 	{

@@ -2707,22 +2707,16 @@ bool World::AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, in
 // the reference's stack query visits leaves: kinematic tree, static tree, dynamic tree, each in
 // depth-first child2-first leaf order (src/broad_phase.c:370-397, src/dynamic_tree.c:1085-1135), so
 // the finder's (query, proxy number) order is the reference's per-query hit order.
-void World::QueryPairsGpuLaunch()
+int World::StageProxies()
 {
 	BroadPhase& bp = broadPhase;
-	if ( pairFinder == nullptr )
-	{
-		pairFinder = new PairFinder();
-	}
-	pairFinder->SetMode( pairQuery == 3 ? 1 : 0 );
-
 	static const b2BodyType kVisitOrder[3] = { b2_kinematicBody, b2_staticBody, b2_dynamicBody };
 	int proxyCount = 0;
 	for ( b2BodyType type : kVisitOrder )
 	{
 		proxyCount += bp.trees[type].ProxyCount();
 	}
-	LbvhProxy* proxies = pairFinder->ProxyStaging( proxyCount );
+	LbvhProxy* proxies = pairFinder->ProxyStaging( std::max( proxyCount, 1 ) );
 	int n = 0;
 	for ( b2BodyType type : kVisitOrder )
 	{
@@ -2752,6 +2746,150 @@ void World::QueryPairsGpuLaunch()
 	{
 		Fatal( "broadphase trees hold %d leaves but count %d proxies", n, proxyCount );
 	}
+	return n;
+}
+
+// Shape filters and the existing-pair set go to the pair finder's device mirrors.
+void World::SyncPairFinderMirrors()
+{
+	if ( syncedFilterVersion != shapeFilterVersion )
+	{
+		int shapeCount = (int)shapes.size();
+		PairShapeFilter* filters = pairFinder->FilterStaging( std::max( shapeCount, 1 ) );
+		for ( int i = 0; i < shapeCount; ++i )
+		{
+			const Shape& shape = shapes[i];
+			filters[i] = PairShapeFilter{ shape.filter.categoryBits, shape.filter.maskBits, shape.filter.groupIndex, shape.bodyId };
+		}
+		pairFinder->SyncShapeFilters( shapeCount );
+		syncedFilterVersion = shapeFilterVersion;
+	}
+	PairKeySet& pairSet = broadPhase.pairSet;
+	pairFinder->SyncPairSet( pairSet.slots.data(), pairSet.slots.size(), pairSet.dirty.data(), pairSet.dirty.size(),
+							   pairSet.allDirty );
+	pairSet.ClearDirty();
+}
+
+void World::LaunchSpeculativePairs()
+{
+	SolvePairInputs in;
+	if ( device->GetSolvePairInputs( &in ) == false )
+	{
+		return;
+	}
+	if ( pairFinder == nullptr )
+	{
+		pairFinder = new PairFinder();
+	}
+	// the trees are as the next pair update will find them, except for the boxes the solve
+	// enlarges, which the device puts in
+	int n = StageProxies();
+	const LbvhProxy* proxies = pairFinder->ProxyStaging( 0 );
+	int shapeCount = std::max<int>( (int)shapes.size(), 1 );
+	ShapeSlot* slots = pairFinder->ShapeSlotStaging( shapeCount );
+	for ( int i = 0; i < shapeCount; ++i )
+	{
+		slots[i] = ShapeSlot{ -1, -1 };
+	}
+	for ( int p = 0; p < n; ++p )
+	{
+		slots[proxies[p].shapeId] = ShapeSlot{ proxies[p].proxyKey, p };
+	}
+	SyncPairFinderMirrors();
+	pairFinder->LaunchSpeculative( n, (int)shapes.size(), in );
+
+	speculative.launched = true;
+	speculative.snapshot = false;
+	speculative.pairSetVersion = broadPhase.pairSet.Version();
+	speculative.filterVersion = shapeFilterVersion;
+	for ( int t = 0; t < b2_bodyTypeCount; ++t )
+	{
+		speculative.structureVersion[t] = broadPhase.trees[t].StructureVersion();
+	}
+}
+
+// The speculative query's pairs when it still describes this update (see SpeculativePairs).
+bool World::TakeSpeculativePairs( std::vector<std::vector<MovePair>>& results )
+{
+	SpeculativePairs& sp = speculative;
+	sp.launched = false;
+	int count = 0, queryCount = 0;
+	bool complete = false;
+	const uint64_t* keys = pairFinder->FinishSpeculative( &count, &queryCount, &complete );
+
+	BroadPhase& bp = broadPhase;
+	int moveCount = (int)bp.moveArray.size();
+	bool valid = complete && sp.snapshot && queryCount == (int)sp.shapes.size() && queryCount == moveCount &&
+				 bp.pairSet.Version() == sp.pairSetVersion && shapeFilterVersion == sp.filterVersion;
+	for ( int t = 0; valid && t < b2_bodyTypeCount; ++t )
+	{
+		valid = bp.trees[t].StructureVersion() == sp.structureVersion[t] && bp.trees[t].BoxVersion() == sp.boxVersion[t];
+	}
+	if ( valid == false )
+	{
+		return false;
+	}
+
+	// the move buffer must be the enlarged shapes' proxies, each once
+	if ( sp.queryOf.size() < shapes.size() )
+	{
+		sp.queryOf.resize( shapes.size(), -1 );
+	}
+	for ( int q = 0; q < queryCount; ++q )
+	{
+		sp.queryOf[sp.shapes[q]] = q;
+	}
+	sp.moveIndexOf.assign( queryCount, -1 );
+	for ( int i = 0; i < moveCount && valid; ++i )
+	{
+		int key = bp.moveArray[i];
+		if ( key == B2_NULL_INDEX )
+		{
+			valid = false;
+			break;
+		}
+		int shapeId = (int)bp.trees[B2G_PROXY_TYPE( key )].GetUserData( B2G_PROXY_ID( key ) );
+		int q = sp.queryOf[shapeId];
+		if ( q < 0 || sp.moveIndexOf[q] >= 0 || shapes[shapeId].proxyKey != key )
+		{
+			valid = false;
+			break;
+		}
+		sp.moveIndexOf[q] = i;
+	}
+	for ( int q = 0; q < queryCount; ++q )
+	{
+		sp.queryOf[sp.shapes[q]] = -1;
+	}
+	if ( valid == false )
+	{
+		return false;
+	}
+
+	const LbvhProxy* proxies = pairFinder->ProxyStaging( 0 );
+	for ( int c = 0; c < count; ++c )
+	{
+		int q = (int)( keys[c] >> 32 );
+		const LbvhProxy& hit = proxies[(uint32_t)keys[c]];
+		int moveIndex = sp.moveIndexOf[q];
+		MovePair pair;
+		if ( AcceptPair( bp.moveArray[moveIndex], sp.shapes[q], hit.proxyKey, hit.shapeId, &pair ) )
+		{
+			results[moveIndex].push_back( pair );
+		}
+	}
+	return true;
+}
+
+void World::QueryPairsGpuLaunch()
+{
+	BroadPhase& bp = broadPhase;
+	if ( pairFinder == nullptr )
+	{
+		pairFinder = new PairFinder();
+	}
+	pairFinder->SetMode( pairQuery == 3 ? 1 : 0 );
+	int n = StageProxies();
 
 	int moveCount = (int)bp.moveArray.size();
 	LbvhProxy* queries = pairFinder->QueryStaging( std::max( moveCount, 1 ) );
@@ -2776,23 +2914,7 @@ void World::QueryPairsGpuLaunch()
 		moved[shapeId] = 1;
 	}
 
-	if ( syncedFilterVersion != shapeFilterVersion )
-	{
-		int shapeCount = (int)shapes.size();
-		PairShapeFilter* filters = pairFinder->FilterStaging( std::max( shapeCount, 1 ) );
-		for ( int i = 0; i < shapeCount; ++i )
-		{
-			const Shape& shape = shapes[i];
-			filters[i] = PairShapeFilter{ shape.filter.categoryBits, shape.filter.maskBits, shape.filter.groupIndex, shape.bodyId };
-		}
-		pairFinder->SyncShapeFilters( shapeCount );
-		syncedFilterVersion = shapeFilterVersion;
-	}
-
-	PairKeySet& pairSet = bp.pairSet;
-	pairFinder->SyncPairSet( pairSet.slots.data(), pairSet.slots.size(), pairSet.dirty.data(), pairSet.dirty.size(),
-							 pairSet.allDirty );
-	pairSet.ClearDirty();
+	SyncPairFinderMirrors();
 	pairFinder->Launch( n, nq, movedCount );
 }
 
@@ -2836,22 +2958,31 @@ void World::UpdateBroadPhasePairs()
 	BroadPhase& bp = broadPhase;
 	int moveCount = (int)bp.moveArray.size();
 	lastPairQueryPath = 0;
+	std::vector<std::vector<MovePair>> results( moveCount );
+	bool taken = false;
+	if ( speculative.launched )
+	{
+		// always collected: its staging buffers are reused by any later query
+		taken = TakeSpeculativePairs( results ) && moveCount > 0;
+	}
 	if ( moveCount == 0 )
 	{
 		return;
 	}
 
-	std::vector<std::vector<MovePair>> results( moveCount );
-
 	static const int s_gpuPairThreshold = []() {
 		const char* v = getenv( "B2AMD_GPU_PAIR_THRESHOLD" );
 		return v != nullptr ? atoi( v ) : kGpuPairThreshold;
 	}();
-	bool gpu = pairQuery == 2 || pairQuery == 3 || ( pairQuery == 0 && moveCount >= s_gpuPairThreshold );
-	gpu = gpu && device->Recording() == false;
-	lastPairQueryPath = gpu ? 2 : 1;
+	bool gpu = pairQuery >= 2 || ( pairQuery == 0 && moveCount >= s_gpuPairThreshold );
+	gpu = gpu && device->Recording() == false && taken == false;
+	lastPairQueryPath = taken ? 3 : ( gpu ? 2 : 1 );
 
-	if ( gpu )
+	if ( taken )
+	{
+		// pairs already found behind the last solve
+	}
+	else if ( gpu )
 	{
 		// the device query works on staged copies of the trees' boxes, so the host rebuilds the
 		// trees while it runs; the candidate filters read no tree
@@ -3367,11 +3498,19 @@ void World::StepMiddle()
 		broadPhase.trees[b2_kinematicBody].Rebuild( false );
 		treeRebuildPending = false;
 		// the rebuilt trees' leaf order, staged ahead for the next GPU pair query
-		if ( lastPairQueryPath == 2 )
+		if ( lastPairQueryPath >= 2 )
 		{
 			FillStagingCache( b2_dynamicBody );
 			FillStagingCache( b2_kinematicBody );
 		}
+	}
+
+	// The next step's pair query behind this solve, when the last solve enlarged enough proxies
+	// for a GPU query to pay.
+	if ( solvePending && ( pairQuery == 2 || ( pairQuery == 0 && 2 * lastEnlargedShapeCount >= kGpuPairThreshold ) ) &&
+		 device->Recording() == false )
+	{
+		LaunchSpeculativePairs();
 	}
 }
 
@@ -3383,6 +3522,15 @@ void World::StepEnd()
 		SolveEnd();
 		profile.solve = b2GetMilliseconds( solveTicks );
 		solvePending = false;
+	}
+	if ( speculative.launched && speculative.snapshot == false )
+	{
+		// the refit has put the enlarged boxes into the trees: later box changes invalidate
+		for ( int t = 0; t < b2_bodyTypeCount; ++t )
+		{
+			speculative.boxVersion[t] = broadPhase.trees[t].BoxVersion();
+		}
+		speculative.snapshot = true;
 	}
 	profile.step = b2GetMilliseconds( stepTicks );
 
@@ -3456,6 +3604,12 @@ void World::SolveEnd()
 	FinalizeResult fr;
 	device->SolveFinish( &fr );
 	profile.constraints = b2GetMilliseconds( constraintTicks );
+	lastEnlargedShapeCount = (int)fr.enlargedShapes.size();
+	if ( speculative.launched )
+	{
+		// the speculative query's queries, in the device list's slot order
+		speculative.shapes = fr.enlargedShapes;
+	}
 
 	// hit events in ascending contact id (reference src/solver.c:1703-1820)
 	for ( const HitRecord& hr : fr.hits )

@@ -229,6 +229,7 @@ public:
 				slots[i] = key;
 				MarkDirty( i );
 				count += 1;
+				version += 1;
 				return;
 			}
 		}
@@ -266,6 +267,13 @@ public:
 		slots[hole] = 0;
 		MarkDirty( hole );
 		count -= 1;
+		version += 1;
+	}
+
+	// Changes whenever a key is inserted or erased.
+	uint64_t Version() const
+	{
+		return version;
 	}
 
 	size_t Size() const
@@ -325,6 +333,7 @@ public:
 private:
 	size_t mask = 0;
 	size_t count = 0;
+	uint64_t version = 0;
 };
 
 class BroadPhase
@@ -471,11 +480,34 @@ public:
 	DeviceWorld* device = nullptr;
 
 	// Pair query: 0 auto (the GPU chunk scan for move buffers of at least kGpuPairThreshold
-	// proxies, outside batched stepping), 1 host tree query, 2 GPU chunk scan, 3 GPU LBVH walk.
-	// All give the same pairs in the same order.
+	// proxies, outside batched stepping), 1 host tree query, 2 GPU chunk scan, 3 GPU LBVH walk,
+	// 4 GPU chunk scan at the pair update only. All give the same pairs in the same order. Modes
+	// 0 and 2 also run the next step's query speculatively behind the solve (see
+	// SpeculativePairs).
 	int pairQuery = 0;
 	class PairFinder* pairFinder = nullptr;
-	int lastPairQueryPath = 0; // 1 host, 2 GPU (for tests)
+	int lastPairQueryPath = 0; // 1 host, 2 GPU at the update, 3 GPU speculative (for tests)
+
+	// The next step's pair query run on the GPU behind this step's solve: its queries are the
+	// shapes the solve enlarges (exactly the move buffer the refit leaves), its proxies the trees
+	// as rebuilt during the solve with those boxes replaced. The result is used only if, at the
+	// next pair update, the move buffer is that set and the trees, the pair set and the shape
+	// filters are unchanged since (user calls between steps change one of them); otherwise the
+	// update queries as usual.
+	struct SpeculativePairs
+	{
+		bool launched = false;
+		bool snapshot = false; // versions taken after the refit
+		uint64_t pairSetVersion = 0;
+		uint64_t filterVersion = 0;
+		uint32_t structureVersion[b2_bodyTypeCount] = {};
+		uint32_t boxVersion[b2_bodyTypeCount] = {};
+		std::vector<int> shapes;	  // the solve's enlarged shapes in device slot order
+		std::vector<int> queryOf;	  // by shape id: slot in shapes, or -1
+		std::vector<int> moveIndexOf; // by slot: move-buffer index at the update
+	};
+	SpeculativePairs speculative;
+	int lastEnlargedShapeCount = 0;
 
 	// Contact events of the last step (reference src/physics_world.c:860-866, 975-983,
 	// 1478-1505): begin and hit events are cleared when a step starts; end events are double
@@ -555,6 +587,10 @@ private:
 	uint64_t shapeFilterVersion = 1; // bumped when a shape id is (re)assigned
 	uint64_t syncedFilterVersion = 0; // the version the pair finder holds
 	void FillStagingCache( b2BodyType type );
+	int StageProxies(); // every proxy in visit order into the pair finder's staging; returns the count
+	void SyncPairFinderMirrors();
+	void LaunchSpeculativePairs();
+	bool TakeSpeculativePairs( std::vector<std::vector<MovePair>>& results );
 	bool AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair );
 	void ProcessContactStates( const int* changedIds, int count );
 	bool SolveBegin( StepParams& params );

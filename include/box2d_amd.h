@@ -50,21 +50,26 @@ B2_API void b2amd_World_SetSolverPath( b2WorldId worldId, b2amd_SolverPath path 
 /// Broadphase pair query of a world's step (the move-buffer query of reference
 /// src/broad_phase.c:412-529). The GPU query stages every proxy in the reference trees' visit
 /// order and tests each moved proxy's box against 64-proxy chunk boxes of that order, one
-/// wavefront per query (b2amd_pairQueryGpu), or walks a Morton-ordered linear BVH with one lane
-/// per query (b2amd_pairQueryGpuLbvh). All give the same pairs in the same order; Auto uses the
-/// chunk scan when at least 128 proxies moved (outside b2amd_StepWorlds, whose host phase uses
-/// the tree query).
+/// wavefront per query, or walks a Morton-ordered linear BVH with one lane per query
+/// (b2amd_pairQueryGpuLbvh). The chunk scan also runs speculatively: behind each step's solve it
+/// queries the proxies that step enlarges, and the next step uses that result when nothing that
+/// it depends on (move buffer, trees, pair set, filters) changed in between (Auto and
+/// b2amd_pairQueryGpu; b2amd_pairQueryGpuAtUpdate only queries at the pair update). All give the
+/// same pairs in the same order; Auto uses the GPU when at least 128 proxies moved (outside
+/// b2amd_StepWorlds, whose host phase uses the tree query).
 typedef enum b2amd_PairQuery
 {
 	b2amd_pairQueryAuto = 0,
 	b2amd_pairQueryHostTree = 1,
 	b2amd_pairQueryGpu = 2,
 	b2amd_pairQueryGpuLbvh = 3,
+	b2amd_pairQueryGpuAtUpdate = 4,
 } b2amd_PairQuery;
 
 B2_API void b2amd_World_SetPairQuery( b2WorldId worldId, b2amd_PairQuery mode );
 
-/// Path the last step's pair query took: 1 host tree, 2 GPU, 0 no query (empty move buffer).
+/// Path the last step's pair query took: 0 no query (empty move buffer), 1 host tree, 2 GPU at the
+/// pair update, 3 GPU speculative (run behind the previous step's solve).
 B2_API int b2amd_World_GetLastPairQuery( b2WorldId worldId );
 
 /// Block until all GPU work issued by this library on the current device has completed

@@ -18,7 +18,8 @@ def scene(world_runs, name):
 
 def run_and_compare(gpu_scenes, fx, path=0, pair_query=None):
     """Step the scene and compare with the fixture. pair_query forces the broadphase pair query
-    (1 host tree, 2 GPU chunk scan, 3 GPU LBVH walk); returns how many steps ran each path."""
+    (1 host tree, 2 GPU chunk scan, 3 GPU LBVH walk, 4 GPU chunk scan at the update only);
+    returns how many steps ran each path (0 none, 1 host, 2 GPU at the update, 3 GPU speculative)."""
     import box2d_amd
 
     lib = box2d_amd.library()
@@ -27,7 +28,7 @@ def run_and_compare(gpu_scenes, fx, path=0, pair_query=None):
     box2d_amd.set_solver_path(wid, path)
     if pair_query is not None:
         lib.b2amd_World_SetPairQuery(wid, pair_query)
-    used = {0: 0, 1: 0, 2: 0}
+    used = {0: 0, 1: 0, 2: 0, 3: 0}
     try:
         assert gpu_scenes.body_count(h) == fx["body_count"]
         assert gpu_scenes.hash(h) == fx["hash_initial"]
@@ -50,16 +51,22 @@ def run_and_compare(gpu_scenes, fx, path=0, pair_query=None):
     return used
 
 
-@pytest.mark.parametrize("mode", [2, 3], ids=["chunk_scan", "lbvh"])
+@pytest.mark.parametrize("mode", [2, 4, 3], ids=["chunk_scan_speculative", "chunk_scan_at_update", "lbvh"])
 @pytest.mark.parametrize("name", ["hello", "pyramid20_sleep", "mixed_pile_1", "mixed_pile_2", "mixed_pile_3",
                                   "falling_hinges_trace", "rain_small"])
 def test_gpu_pair_query_matches_reference(gpu_scenes, world_runs, name, mode):
-    """Broadphase pairs from the GPU query on every step, by the chunk-box scan (mode 2) or the
-    LBVH walk (mode 3), ordered in the reference trees' visit order: the whole run stays
-    bit-identical to the reference."""
+    """Broadphase pairs from the GPU query on every step: the chunk-box scan run speculatively
+    behind the previous solve where it still applies (mode 2), the chunk-box scan at the pair
+    update (mode 4) or the LBVH walk (mode 3), ordered in the reference trees' visit order. The
+    whole run stays bit-identical to the reference."""
     print(f"[gpu] scene {name} (GPU pair query, mode {mode})", flush=True)
     used = run_and_compare(gpu_scenes, scene(world_runs, name), pair_query=mode)
-    assert used[2] > 0 and used[1] == 0
+    assert used[2] + used[3] > 0 and used[1] == 0
+    if mode == 2 and used[2] + used[3] > 1:
+        # after the first step's query, the speculative query serves the steps that move proxies
+        assert used[3] > 0, used
+    if mode != 2:
+        assert used[3] == 0, used
 
 
 @pytest.mark.parametrize("name", ["hello", "pyramid20", "pyramid100", "pyramid20_sleep"])
@@ -114,7 +121,7 @@ def test_rain_matches_reference(gpu_scenes, world_runs):
     """The 20,000-body mixed-shape rain, 1000 steps, hash-checked every 10 steps."""
     print("[gpu] rain 20k", flush=True)
     used = run_and_compare(gpu_scenes, scene(world_runs, "rain"))
-    print(f"[gpu] rain pair query steps: host {used[1]} gpu {used[2]} none {used[0]}", flush=True)
+    print(f"[gpu] rain pair query steps: host {used[1]} gpu {used[2]} speculative {used[3]} none {used[0]}", flush=True)
 
 
 @pytest.mark.parametrize("path", [0, 1])
