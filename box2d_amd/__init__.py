@@ -60,10 +60,12 @@ def batch_world_specs(count, lo=8, hi=1024, seed=12345):
 SOLVER_AUTO = 0
 SOLVER_COLOR_KERNELS = 1
 SOLVER_FUSED = 2
+SOLVER_XCD = 3
 
 
 def set_solver_path(world_id, path):
-    """Select the solver path of a world (SOLVER_AUTO, SOLVER_COLOR_KERNELS or SOLVER_FUSED)."""
+    """Select the solver path of a world (SOLVER_AUTO, SOLVER_COLOR_KERNELS, SOLVER_FUSED or
+    SOLVER_XCD)."""
     library().b2amd_World_SetSolverPath(world_id, ctypes.c_int(path))
 
 

@@ -6,6 +6,7 @@
 #include "device.h"
 #include "world.h"
 
+#include <cfloat>
 #include <cstring>
 
 namespace b2g
@@ -20,6 +21,7 @@ int CollideBatchNoAlloc( const b2amd_Shape* shapesA, const b2amd_Shape* shapesB,
 						 b2LocalManifold* manifolds );
 void DeviceSynchronize();
 bool SpinWaitActive();
+bool SetBatchSolverThreads( int threads );
 int TimeOfImpactBatch( const b2TOIInput* inputs, int count, b2TOIOutput* outputs );
 int ShapeDistanceBatch( const b2DistanceInput* inputs, b2SimplexCache* caches, int count, b2DistanceOutput* outputs );
 } // namespace b2g
@@ -282,6 +284,81 @@ void* b2World_GetUserData( b2WorldId worldId )
 {
 	World* world = GetWorld( worldId );
 	return world != nullptr ? world->userData : nullptr;
+}
+
+// reference src/physics_world.c:1789-1827
+void b2World_SetRestitutionThreshold( b2WorldId worldId, float value )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr || world->locked )
+	{
+		return;
+	}
+	world->restitutionThreshold = b2ClampFloat( value, 0.0f, FLT_MAX );
+}
+
+float b2World_GetRestitutionThreshold( b2WorldId worldId )
+{
+	World* world = GetWorld( worldId );
+	return world != nullptr ? world->restitutionThreshold : 0.0f;
+}
+
+void b2World_SetHitEventThreshold( b2WorldId worldId, float value )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr || world->locked )
+	{
+		return;
+	}
+	world->hitEventThreshold = b2ClampFloat( value, 0.0f, FLT_MAX );
+}
+
+float b2World_GetHitEventThreshold( b2WorldId worldId )
+{
+	World* world = GetWorld( worldId );
+	return world != nullptr ? world->hitEventThreshold : 0.0f;
+}
+
+// reference src/physics_world.c:2972-2982
+void b2World_SetCustomFilterCallback( b2WorldId worldId, b2CustomFilterFcn* fcn, void* context )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr )
+	{
+		return;
+	}
+	world->customFilterFcn = fcn;
+	world->customFilterContext = context;
+}
+
+// reference src/physics_world.c:1950-1984. A change of callback takes effect for each contact at
+// its next full narrowphase update, as in the reference, where the callback runs in that update.
+void b2World_SetFrictionCallback( b2WorldId worldId, b2FrictionCallback* callback )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr || world->locked )
+	{
+		return;
+	}
+	if ( callback != world->frictionCallback )
+	{
+		world->frictionCallback = callback;
+		world->mixingStale = true;
+	}
+}
+
+void b2World_SetRestitutionCallback( b2WorldId worldId, b2RestitutionCallback* callback )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr || world->locked )
+	{
+		return;
+	}
+	if ( callback != world->restitutionCallback )
+	{
+		world->restitutionCallback = callback;
+		world->mixingStale = true;
+	}
 }
 
 // Bodies ---------------------------------------------------------------------------------------
@@ -662,6 +739,239 @@ void b2Body_ApplyMassFromShapes( b2BodyId bodyId )
 	world->ApplyMassFromShapes( body );
 }
 
+// Damping, gravity scale and sleep settings (reference src/body.c:1466-1550, 1629-1696)
+static World* UnlockedBody( b2BodyId bodyId, Body** body )
+{
+	World* world = WorldFromBody( bodyId, body );
+	return world != nullptr && world->locked == false ? world : nullptr;
+}
+
+void b2Body_SetLinearDamping( b2BodyId bodyId, float linearDamping )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->PullBodies();
+		world->bodySims[body->id].linearDamping = linearDamping;
+		world->bodiesDirty = true;
+	}
+}
+
+float b2Body_GetLinearDamping( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	return world != nullptr ? world->bodySims[body->id].linearDamping : 0.0f;
+}
+
+void b2Body_SetAngularDamping( b2BodyId bodyId, float angularDamping )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->PullBodies();
+		world->bodySims[body->id].angularDamping = angularDamping;
+		world->bodiesDirty = true;
+	}
+}
+
+float b2Body_GetAngularDamping( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	return world != nullptr ? world->bodySims[body->id].angularDamping : 0.0f;
+}
+
+void b2Body_SetGravityScale( b2BodyId bodyId, float gravityScale )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->PullBodies();
+		world->bodySims[body->id].gravityScale = gravityScale;
+		world->bodiesDirty = true;
+	}
+}
+
+float b2Body_GetGravityScale( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	return world != nullptr ? world->bodySims[body->id].gravityScale : 0.0f;
+}
+
+void b2Body_SetSleepThreshold( b2BodyId bodyId, float sleepThreshold )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->PullBodies();
+		world->bodyAux[body->id].sleepThreshold = sleepThreshold;
+		world->bodiesDirty = true;
+	}
+}
+
+float b2Body_GetSleepThreshold( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	return world != nullptr ? world->bodyAux[body->id].sleepThreshold : 0.0f;
+}
+
+void b2Body_EnableSleep( b2BodyId bodyId, bool enableSleep )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->EnableBodySleep( body, enableSleep );
+	}
+}
+
+bool b2Body_IsSleepEnabled( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	return world != nullptr && ( body->flags & kEnableSleep ) != 0;
+}
+
+void b2Body_SetAwake( b2BodyId bodyId, bool awake )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->SetBodyAwake( body, awake );
+	}
+}
+
+void b2Body_WakeTouching( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->PullBodies();
+		world->WakeTouching( body );
+	}
+}
+
+// Contact data (reference src/body.c:436-486, src/shape.c:1749-1800, src/contact.c:45-92)
+static World* WorldFromContact( b2ContactId id, Contact** out )
+{
+	World* world = GetWorldFromIndex( id.world0 );
+	if ( world == nullptr )
+	{
+		return nullptr;
+	}
+	int index = id.index1 - 1;
+	if ( index < 0 || index >= (int)world->contacts.size() )
+	{
+		return nullptr;
+	}
+	Contact* contact = &world->contacts[index];
+	if ( contact->contactId != index || contact->generation != id.generation )
+	{
+		return nullptr;
+	}
+	*out = contact;
+	return world;
+}
+
+static b2ContactData ContactData( World* world, const Contact& contact )
+{
+	b2ContactData data = {};
+	data.contactId = world->ContactIdOf( contact );
+	data.shapeIdA = world->ShapeIdOf( contact.shapeIdA );
+	data.shapeIdB = world->ShapeIdOf( contact.shapeIdB );
+	data.manifold = world->PulledContact( contact.contactId ).manifold;
+	return data;
+}
+
+bool b2Contact_IsValid( b2ContactId id )
+{
+	Contact* contact;
+	return WorldFromContact( id, &contact ) != nullptr;
+}
+
+b2ContactData b2Contact_GetData( b2ContactId contactId )
+{
+	Contact* contact;
+	World* world = WorldFromContact( contactId, &contact );
+	return world != nullptr ? ContactData( world, *contact ) : b2ContactData{};
+}
+
+int b2Body_GetContactCapacity( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	return world != nullptr ? body->contactCount : 0;
+}
+
+int b2Body_GetContactData( b2BodyId bodyId, b2ContactData* contactData, int capacity )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world == nullptr )
+	{
+		return 0;
+	}
+	int contactKey = body->headContactKey;
+	int index = 0;
+	while ( contactKey != B2_NULL_INDEX && index < capacity )
+	{
+		const Contact& contact = world->contacts[contactKey >> 1];
+		if ( contact.flags & kContactTouching )
+		{
+			contactData[index++] = ContactData( world, contact );
+		}
+		contactKey = contact.edges[contactKey & 1].nextKey;
+	}
+	return index;
+}
+
+// Type changes and enabling (reference src/body.c:1134-1316, 1699-1858)
+void b2Body_SetType( b2BodyId bodyId, b2BodyType type )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->SetBodyType( body, type );
+	}
+}
+
+void b2Body_Disable( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->DisableBody( body );
+	}
+}
+
+void b2Body_Enable( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->EnableBody( body );
+	}
+}
+
+bool b2Body_IsEnabled( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	return world != nullptr && body->setIndex != kDisabledSet;
+}
+
 int b2Body_GetShapes( b2BodyId bodyId, b2ShapeId* shapeArray, int capacity )
 {
 	Body* body;
@@ -803,6 +1113,194 @@ bool b2Shape_AreSensorEventsEnabled( b2ShapeId shapeId )
 	Shape* shape;
 	World* world = WorldFromShape( shapeId, &shape );
 	return world != nullptr && shape->enableSensorEvents;
+}
+
+// Materials, density, filter and user data (reference src/shape.c:1190-1316, 1367-1391)
+static World* UnlockedShape( b2ShapeId shapeId, Shape** shape )
+{
+	World* world = WorldFromShape( shapeId, shape );
+	return world != nullptr && world->locked == false ? world : nullptr;
+}
+
+void b2Shape_SetFriction( b2ShapeId shapeId, float friction )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	if ( world != nullptr )
+	{
+		b2SurfaceMaterial m = shape->material;
+		m.friction = friction;
+		world->SetShapeMaterial( shape, m );
+	}
+}
+
+float b2Shape_GetFriction( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr ? shape->material.friction : 0.0f;
+}
+
+void b2Shape_SetRestitution( b2ShapeId shapeId, float restitution )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	if ( world != nullptr )
+	{
+		b2SurfaceMaterial m = shape->material;
+		m.restitution = restitution;
+		world->SetShapeMaterial( shape, m );
+	}
+}
+
+float b2Shape_GetRestitution( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr ? shape->material.restitution : 0.0f;
+}
+
+void b2Shape_SetUserMaterial( b2ShapeId shapeId, uint64_t material )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	if ( world != nullptr )
+	{
+		b2SurfaceMaterial m = shape->material;
+		m.userMaterialId = material;
+		world->SetShapeMaterial( shape, m );
+	}
+}
+
+uint64_t b2Shape_GetUserMaterial( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr ? shape->material.userMaterialId : 0;
+}
+
+void b2Shape_SetSurfaceMaterial( b2ShapeId shapeId, const b2SurfaceMaterial* surfaceMaterial )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	if ( world != nullptr )
+	{
+		world->SetShapeMaterial( shape, *surfaceMaterial );
+	}
+}
+
+b2SurfaceMaterial b2Shape_GetSurfaceMaterial( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr ? shape->material : b2SurfaceMaterial{};
+}
+
+void b2Shape_SetDensity( b2ShapeId shapeId, float density, bool updateBodyMass )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	if ( world != nullptr )
+	{
+		world->SetShapeDensity( shape, density, updateBodyMass );
+	}
+}
+
+float b2Shape_GetDensity( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr ? shape->density : 0.0f;
+}
+
+void b2Shape_SetFilter( b2ShapeId shapeId, b2Filter filter )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	if ( world != nullptr )
+	{
+		world->SetShapeFilter( shape, filter );
+	}
+}
+
+b2Filter b2Shape_GetFilter( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr ? shape->filter : b2Filter{};
+}
+
+void b2Shape_SetUserData( b2ShapeId shapeId, void* userData )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	if ( world != nullptr )
+	{
+		shape->userData = userData;
+	}
+}
+
+void* b2Shape_GetUserData( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr ? shape->userData : nullptr;
+}
+
+void b2DestroyShape( b2ShapeId shapeId, bool updateBodyMass )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	// a segment of a chain goes with its chain (reference src/shape.c:413-418)
+	if ( world == nullptr || ( shape->type == b2_chainSegmentShape && world->shapeData[shape->id].chainSegment.chainId != B2_NULL_INDEX ) )
+	{
+		return;
+	}
+	world->DestroyShape( shape, updateBodyMass );
+}
+
+int b2Shape_GetContactCapacity( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	return world != nullptr && shape->sensorIndex == B2_NULL_INDEX ? world->bodies[shape->bodyId].contactCount : 0;
+}
+
+int b2Shape_GetContactData( b2ShapeId shapeId, b2ContactData* contactData, int capacity )
+{
+	Shape* shape;
+	World* world = UnlockedShape( shapeId, &shape );
+	if ( world == nullptr || shape->sensorIndex != B2_NULL_INDEX )
+	{
+		return 0;
+	}
+	const Body& body = world->bodies[shape->bodyId];
+	int contactKey = body.headContactKey;
+	int index = 0;
+	while ( contactKey != B2_NULL_INDEX && index < capacity )
+	{
+		const Contact& contact = world->contacts[contactKey >> 1];
+		if ( ( contact.shapeIdA == shape->id || contact.shapeIdB == shape->id ) && ( contact.flags & kContactTouching ) )
+		{
+			contactData[index++] = ContactData( world, contact );
+		}
+		contactKey = contact.edges[contactKey & 1].nextKey;
+	}
+	return index;
+}
+
+b2WorldId b2Shape_GetWorld( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr ? b2WorldId{ (uint16_t)( shapeId.world0 + 1 ), world->generation } : b2_nullWorldId;
+}
+
+bool b2Shape_IsSensor( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr && shape->sensorIndex != B2_NULL_INDEX;
 }
 
 // Geometry accessors (reference src/shape.c:1489-1604). A shape of another type returns a
@@ -1345,6 +1843,11 @@ int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int 
 void b2amd_Synchronize( void )
 {
 	DeviceSynchronize();
+}
+
+int b2amd_SetBatchSolverThreads( int threads )
+{
+	return SetBatchSolverThreads( threads ) ? 0 : -1;
 }
 
 int b2amd_SpinWaitActive( void )

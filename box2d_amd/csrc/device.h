@@ -55,10 +55,19 @@ struct SolverLists
 constexpr int kOverflowChunkContacts = 512;
 constexpr int kOverflowChunkBodies = 2 * kOverflowChunkContacts;
 
+// A contact's friction and restitution from the user's mixing callbacks.
+struct ContactMaterial
+{
+	int contactId;
+	float friction;
+	float restitution;
+};
+
 // Results the host needs after the collide stage.
 struct CollideResult
 {
-	// contact id * 4 + kind (1 disjoint, 2 started touching, 3 stopped touching), ascending id
+	// contact id * 4 + kind (0 mixing values out of date, 1 disjoint, 2 started touching,
+	// 3 stopped touching), ascending
 	std::vector<int> changes;
 	int recycledCount = 0;
 };
@@ -111,7 +120,13 @@ public:
 	void UploadBodies( const World& world );
 	void UploadShapes( const World& world );
 	void UploadJoints( const World& world );
+	// Sorted (static shape << 32 | fast shape) keys the custom filter rejects for continuous collision.
+	void UploadContinuousFilterRejects( const std::vector<uint64_t>& keys );
 	void UploadContactInits( const std::vector<ContactInit>& inits, int contactCapacity );
+	// User mixing callbacks: flag every contact's friction and restitution as out of date (the
+	// narrowphase then reports each at its next full update), and upload new values.
+	void MarkContactMixingStale();
+	void UploadContactMaterials( const std::vector<ContactMaterial>& list );
 	void UploadLists( const SolverLists& lists );
 	// Upload the per-body warm-start lists of the lists last uploaded (they must match them).
 	void UploadWarmLists( const SolverLists& lists );
@@ -121,6 +136,7 @@ public:
 	void DownloadShapes( World& world );
 	void DownloadJoints( World& world );
 	void DownloadContact( int contactId, ContactSim* out );
+	void DownloadContacts( int count, ContactSim* out ); // the first count contacts (by id)
 
 	// Stages. Each runs as a launch half (enqueue, or record when recording for a batch) and a
 	// finish half (wait for the stream, read the results back).
@@ -133,6 +149,9 @@ public:
 
 	// Record instead of launching (the batch that steps many worlds launches the records).
 	void SetRecording( bool flag );
+	// A recorded world whose awake bodies exceed the batch's persistent solver records no solve
+	// stages; the batch launches its solve here, on the world's own stream, after its own.
+	void LaunchDeferredSolve();
 	bool Recording() const;
 	void* Record();
 

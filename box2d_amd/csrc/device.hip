@@ -23,6 +23,7 @@
 #include "world.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -106,6 +107,7 @@ struct CollideArgs
 	int* changes;
 	int* counters; // [0] change count, [1] recycled count
 	float recycleDistance;
+	int customMixing; // StepParams::customMixing
 };
 
 B2G_DEV float ShapeRadius( const ShapeDev& s )
@@ -234,8 +236,22 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 		mp->id = local.points[i].id;
 	}
 
-	cs->friction = sqrtf( shapeA.friction * shapeB.friction );
-	cs->restitution = b2MaxFloat( shapeA.restitution, shapeB.restitution );
+	// mixing at every full update (reference src/contact.c:548-551); user callbacks run on the
+	// host, which is asked for new values when the stored ones are out of date (change kind 0)
+	if ( ( a.customMixing & 1 ) == 0 )
+	{
+		cs->friction = sqrtf( shapeA.friction * shapeB.friction );
+	}
+	if ( ( a.customMixing & 2 ) == 0 )
+	{
+		cs->restitution = b2MaxFloat( shapeA.restitution, shapeB.restitution );
+	}
+	if ( a.customMixing != 0 && ( simFlags & kSimMixStale ) != 0 )
+	{
+		simFlags &= ~kSimMixStale;
+		int slot = atomicAdd( a.counters + 0, 1 );
+		a.changes[slot] = contactId * 4 + 0;
+	}
 
 	if ( shapeA.rollingResistance > 0.0f || shapeB.rollingResistance > 0.0f )
 	{
@@ -1336,6 +1352,225 @@ __global__ __launch_bounds__( kFusedThreads ) void SolveFusedBatchKernel( const 
 	}
 }
 
+// ---------------------------------------------------------------------------------------------
+// XCD solver: the whole substep loop of one world in one launch, on the compute units of one XCD.
+//
+// A stage's colors are Gauss-Seidel phases, and in the color-kernel path each phase is a kernel
+// launch with a floor of 3-5 us. Here the phases are separated by a barrier among the workgroups
+// of one XCD instead. The CUs of an XCD share one L2, so a barrier only has to make each
+// workgroup's stores reach that L2 before it arrives (a wait for its outstanding memory
+// operations) and count arrivals with L2 atomics: about 1.2-1.4 us per barrier on MI355X, where
+// coherence across XCDs would mean writing back each L2 at every barrier. The launch places
+// workgroup b on XCD b mod 8 (round robin), so the grid is 8 x kXcdGroups workgroups
+// (cooperative: all resident at once); the workgroups that read XCC_ID 0 take tickets and solve,
+// the others exit at once. Every wait is bounded: a barrier that is not complete after
+// kXcdSpinLimit polls (or sees another workgroup's timeout) raises the error flag and the
+// workgroup exits, and the host stops with an error.
+//
+// What workgroups share must not be read from a CU's L1, which may still hold a line another CU
+// has since rewritten (measured on MI355X: plain loads read stale data, and an L1 invalidate
+// costs about 0.9 us per workgroup per barrier because it also walks the L2). Body states, the
+// only mutable data the phases exchange, are therefore read with agent-scope loads
+// (CoherentBodies), which skip the L1. Constraint and joint rows are otherwise only touched by the
+// wave that owns them: a phase's lanes are cut into 64-lane waves and wave w goes to participant
+// w mod kXcdGroups. The per-body warm start (worlds without joints) is the exception: it reads
+// other waves' impulses with the same kind of loads and leaves the total-normal-impulse sums to
+// each contact's own solve lane (AddWarmTotals); worlds with joints warm start color by color.
+// Each phase runs exactly the lanes one color kernel launch runs, in any order within the phase.
+
+constexpr int kXcdThreads = 1024;
+constexpr int kXcdGroups = 32; // workgroups on the solving XCD (one per CU of an MI355X XCD)
+constexpr int kXcdCount = 8;
+constexpr int kXcdGrid = kXcdGroups * kXcdCount;
+constexpr int kXcdWaves = kXcdThreads / 64;
+constexpr long long kXcdSpinLimit = 1 << 22; // polls before a barrier gives up (~seconds)
+
+// sync[]: 0 ticket, 1 barrier arrivals, 2 error (barrier timeout), 3 unused
+struct XcdArgs
+{
+	FusedArgs f; // color ranges, warm lists and parameters; f.color.c holds body ids
+	int* sync;
+	int zero; // 0, passed at run time so the barrier's polling atomic is not folded into a load
+};
+
+__device__ __forceinline__ int XccId()
+{
+	// XCC_ID hardware register (4 bits)
+	return __builtin_amdgcn_s_getreg( 20 | ( 0 << 6 ) | ( 3 << 11 ) ) & 0xF;
+}
+
+__device__ __forceinline__ int L2Poll( int* p, int zero )
+{
+	return __hip_atomic_fetch_add( p, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+}
+
+// Barrier among the participants; returns false when it timed out (or another one did).
+__device__ bool XcdBarrier( const XcdArgs& a, int& arrivals )
+{
+	__shared__ int s_ok;
+	arrivals += kXcdGroups;
+	__builtin_amdgcn_s_waitcnt( 0 ); // this wave's stores are in L2
+	__syncthreads();
+	if ( threadIdx.x == 0 )
+	{
+		__hip_atomic_fetch_add( a.sync + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+		int ok = 1;
+		long long polls = 0;
+		while ( L2Poll( a.sync + 1, a.zero ) < arrivals )
+		{
+			if ( ++polls > kXcdSpinLimit || L2Poll( a.sync + 2, a.zero ) != 0 )
+			{
+				__hip_atomic_fetch_add( a.sync + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+				ok = 0;
+				break;
+			}
+		}
+		s_ok = ok;
+	}
+	__syncthreads();
+	return s_ok != 0;
+}
+
+// Lanes of one color of one stage, wave-interleaved over the participants.
+template <int Stage>
+B2G_DEV void XcdColor( const XcdArgs& a, const ColorRange& r, int rank, const CoherentBodies& B )
+{
+	int lanes = r.jointCount + r.contactCount;
+	int lane = threadIdx.x & 63;
+	for ( int lw = threadIdx.x >> 6;; lw += kXcdWaves )
+	{
+		int w = lw * kXcdGroups + rank;
+		if ( w * 64 >= lanes )
+		{
+			break;
+		}
+		int t = w * 64 + lane;
+		if ( Stage == kStageSolve && a.f.warmByBody && t >= r.jointCount && t < lanes )
+		{
+			AddWarmTotals( a.f.color.c, r.contactStart + t - r.jointCount );
+		}
+		ColorLane<Stage>( a.f.color, r, t, B );
+	}
+}
+
+template <int Stage>
+B2G_DEV bool XcdStage( const XcdArgs& a, int rank, const CoherentBodies& B, int& arrivals )
+{
+	for ( int color = 0; color < a.f.activeColorCount; ++color )
+	{
+		ColorRange r;
+		r.jointStart = a.f.colorJointStart[color];
+		r.jointCount = a.f.colorJointStart[color + 1] - r.jointStart;
+		r.contactStart = a.f.colorContactStart[color];
+		r.contactCount = a.f.colorContactStart[color + 1] - r.contactStart;
+		int lanes = r.jointCount + r.contactCount;
+		if ( lanes == 0 || ( Stage == kStageRestitution && r.contactCount == 0 ) )
+		{
+			continue;
+		}
+		XcdColor<Stage>( a, r, rank, B );
+		if ( XcdBarrier( a, arrivals ) == false )
+		{
+			return false;
+		}
+	}
+	return true;
+}
+
+__global__ __launch_bounds__( kXcdThreads ) void SolveXcdKernel( XcdArgs a )
+{
+	__shared__ int s_rank;
+	if ( threadIdx.x == 0 )
+	{
+		int rank = -1;
+		if ( XccId() == 0 )
+		{
+			rank = __hip_atomic_fetch_add( a.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+		}
+		s_rank = rank < kXcdGroups ? rank : -1;
+	}
+	__syncthreads();
+	const int rank = s_rank;
+	if ( rank < 0 )
+	{
+		return;
+	}
+
+	const FusedArgs& f = a.f;
+	const CoherentBodies B{ f.states };
+	const StepParams& p = f.color.p;
+	const int n = f.bodyCount;
+	const int lane = threadIdx.x & 63;
+	int arrivals = 0;
+
+	for ( int sub = 0; sub < f.subStepCount; ++sub )
+	{
+		for ( int lw = threadIdx.x >> 6;; lw += kXcdWaves )
+		{
+			int w = lw * kXcdGroups + rank;
+			if ( w * 64 >= n )
+			{
+				break;
+			}
+			int i = w * 64 + lane;
+			if ( i < n )
+			{
+				int id = f.awake[i];
+				BodyState st = B.LoadState( id );
+				IntegrateVelocity( f.sims[id], &st, p );
+				float vx = st.linearVelocity.x, vy = st.linearVelocity.y, wv = st.angularVelocity;
+				if ( f.warmByBody )
+				{
+					WarmStartBody<4, false, true>( f.color.c, f.color.joints, f.color.jointList, f.overflowContactStart, f.warmList,
+												   f.warmStart[i], f.warmStart[i + 1], B, st.deltaRotation, st.deltaPosition, vx, vy, wv );
+				}
+				f.states[id].linearVelocity = { vx, vy };
+				f.states[id].angularVelocity = wv;
+			}
+		}
+		if ( XcdBarrier( a, arrivals ) == false )
+		{
+			return;
+		}
+		if ( f.warmByBody == 0 && XcdStage<kStageWarmStart>( a, rank, B, arrivals ) == false )
+		{
+			return;
+		}
+		if ( XcdStage<kStageSolve>( a, rank, B, arrivals ) == false )
+		{
+			return;
+		}
+		for ( int lw = threadIdx.x >> 6;; lw += kXcdWaves )
+		{
+			int w = lw * kXcdGroups + rank;
+			if ( w * 64 >= n )
+			{
+				break;
+			}
+			int i = w * 64 + lane;
+			if ( i < n )
+			{
+				int id = f.awake[i];
+				BodyState st = B.LoadState( id );
+				IntegratePosition( &st, p );
+				f.states[id] = st;
+			}
+		}
+		if ( XcdBarrier( a, arrivals ) == false )
+		{
+			return;
+		}
+		if ( XcdStage<kStageRelax>( a, rank, B, arrivals ) == false )
+		{
+			return;
+		}
+	}
+	if ( f.anyRestitution )
+	{
+		XcdStage<kStageRestitution>( a, rank, B, arrivals );
+	}
+}
+
 struct StoreArgs
 {
 	ConstraintSoA c;
@@ -1611,6 +1846,8 @@ struct ContinuousArgs
 	int* error; // set when a query stack would overflow
 	const int2* noCollide; // (body, static body) pairs joined with collideConnected off
 	int noCollideCount;
+	const uint64_t* filterRejects; // sorted (static shape << 32 | fast shape) keys the custom filter rejected
+	int filterRejectCount;
 	int* enlargedShapeCount;
 	int* enlargedShapes;
 	b2AABB* enlargedShapeAABBs;
@@ -1728,6 +1965,26 @@ B2G_DEV void EnlargeShape( const ContinuousArgs& a, int shapeId, ShapeDev* shape
 	}
 }
 
+B2G_DEV bool FilterRejected( const ContinuousArgs& a, int staticShapeId, int fastShapeId )
+{
+	uint64_t key = (uint64_t)(uint32_t)staticShapeId << 32 | (uint32_t)fastShapeId;
+	int lo = 0;
+	int hi = a.filterRejectCount;
+	while ( lo < hi )
+	{
+		int mid = ( lo + hi ) >> 1;
+		if ( a.filterRejects[mid] < key )
+		{
+			lo = mid + 1;
+		}
+		else
+		{
+			hi = mid;
+		}
+	}
+	return lo < a.filterRejectCount && a.filterRejects[lo] == key;
+}
+
 B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 {
 	if ( k >= *a.fastCount )
@@ -1810,6 +2067,11 @@ B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 				}
 			}
 			if ( jointed )
+			{
+				continue;
+			}
+			// custom filtering (reference src/solver.c:242-256), asked on the host before the step
+			if ( a.filterRejectCount > 0 && FilterRejected( a, otherId, shapeId ) )
 			{
 				continue;
 			}
@@ -2126,7 +2388,7 @@ __global__ void LaneBatchKernel( const typename Op::Args* args, const int2* bloc
 // ---------------------------------------------------------------------------------------------
 // Kernel classes for timing, with the algorithmic bytes each unit of work must move: every field
 // a lane reads or writes counted once, body states gathered and scattered per constraint
-// (per-unit figures of the step's data layout; see the roofline notes in the design document).
+// (per-unit figures of the step's data layout).
 
 enum KernelClass
 {
@@ -2145,13 +2407,14 @@ enum KernelClass
 	kkSolveFused,
 	kkContinuous,
 	kkColorTail,
+	kkSolveXcd,
 	kKernelClassCount
 };
 
 static const char* s_kernelClassNames[kKernelClassCount] = {
 	"collide", "prepare_contacts", "prepare_joints", "integrate_velocities_warm_start", "warm_start", "solve",
 	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies", "solve_fused", "continuous",
-	"color_tail",
+	"color_tail", "solve_xcd",
 };
 
 constexpr double kBytesCollideFull = 800.0;		// per contact with a new manifold
@@ -2240,6 +2503,7 @@ struct PackFinalizeArgs
 	const b2AABB* enlargedShapeAABBs;
 	const float* sleepTimes; // null when not needed
 	int* keptShapeCount;	 // receives the enlarged-shape count before it is cleared (may be null)
+	const int* xcdSync;		 // XCD solver barrier words of this step (null when it did not run)
 	int bodyCount;
 	int shapeOffset; // mailbox offsets in ints
 	int aabbOffset;
@@ -2331,6 +2595,7 @@ B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
 		{
 			*a.keptShapeCount = shapes;
 		}
+		a.mail[5] = a.xcdSync != nullptr ? a.xcdSync[2] : 0;
 		a.counters[0] = 0;
 		a.counters[1] = 0;
 		a.counters[2] = 0;
@@ -2443,6 +2708,8 @@ struct DeviceWorld::Impl
 	DevArray<CcdNode> ccdTree;
 	int ccdTreeRoot = B2_NULL_INDEX;
 	DevArray<int2> ccdNoCollide;
+	DevArray<uint64_t> ccdFilterRejects;
+	int ccdFilterRejectCount = 0;
 	std::vector<CcdNode> ccdTreeStaging;
 	std::vector<int2> ccdNoCollideStaging;
 	DevArray<int> enlargedShapes;
@@ -2549,6 +2816,16 @@ struct DeviceWorld::Impl
 	}
 
 	hipEvent_t ev[4] = {};
+	// XCD solver: barrier words (see SolveXcdKernel) and whether this device can run it
+	int* xcdSync = nullptr;
+	int xcdUsable = -1; // -1 not checked yet
+	// a recorded (batched) world too large for the batch's persistent solver: its solve is launched
+	// on its own after the batch's (LaunchDeferredSolve) and waited for in SolveFinish
+	bool solveDeferred = false;
+	bool deferredLaunched = false;
+	StepParams deferredParams = {};
+	bool deferredRestitution = false;
+	bool deferredSleepTimes = false;
 	// recorded after the finalize pack kernel of a launched solve (speculative pair queries wait on it)
 	hipEvent_t solvePacked = nullptr;
 	bool solvePackedValid = false;
@@ -2698,6 +2975,10 @@ DeviceWorld::~DeviceWorld()
 	}
 	(void)hipStreamSynchronize( impl->stream );
 	(void)hipEventDestroy( impl->solvePacked );
+	if ( impl->xcdSync != nullptr )
+	{
+		(void)hipFree( impl->xcdSync );
+	}
 	impl->sims.Free();
 	impl->states.Free();
 	impl->aux.Free();
@@ -2718,6 +2999,7 @@ DeviceWorld::~DeviceWorld()
 	impl->fastBodies.Free();
 	impl->ccdTree.Free();
 	impl->ccdNoCollide.Free();
+	impl->ccdFilterRejects.Free();
 	impl->enlargedShapes.Free();
 	impl->enlargedShapeAABBs.Free();
 	impl->sleepTimes.Free();
@@ -2854,6 +3136,16 @@ void DeviceWorld::UploadJoints( const World& world )
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
+void DeviceWorld::UploadContinuousFilterRejects( const std::vector<uint64_t>& keys )
+{
+	if ( keys.empty() && impl->ccdFilterRejectCount == 0 )
+	{
+		return;
+	}
+	UploadVector( impl->ccdFilterRejects, keys, impl->stream );
+	impl->ccdFilterRejectCount = (int)keys.size();
+}
+
 void DeviceWorld::DownloadJoints( World& world )
 {
 	int n = impl->jointCount;
@@ -2890,6 +3182,49 @@ void DeviceWorld::UploadContactInits( const std::vector<ContactInit>& inits, int
 	}
 	LaneKernel<InitContactsOp><<<Blocks( n ), kBlock, 0, impl->stream>>>( ia );
 	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+__global__ void MarkMixStaleKernel( ContactSim* contacts, int count )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i < count )
+	{
+		contacts[i].simFlags |= kSimMixStale;
+	}
+}
+
+void DeviceWorld::MarkContactMixingStale()
+{
+	int n = impl->contactCapacityUsed;
+	if ( n == 0 )
+	{
+		return;
+	}
+	// on the world's own stream and waited for, so a batch's later launches see it too
+	MarkMixStaleKernel<<<Blocks( n ), kBlock, 0, impl->stream>>>( impl->contacts.ptr, n );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+void DeviceWorld::UploadContactMaterials( const std::vector<ContactMaterial>& list )
+{
+	static_assert( offsetof( ContactSim, restitution ) == offsetof( ContactSim, friction ) + sizeof( float ), "adjacent" );
+	for ( const ContactMaterial& m : list )
+	{
+		float values[2] = { m.friction, m.restitution };
+		impl->Stage( &impl->contacts.ptr[m.contactId].friction, values, sizeof( values ) );
+	}
+}
+
+void DeviceWorld::DownloadContacts( int count, ContactSim* out )
+{
+	count = std::min( count, impl->contactCapacityUsed );
+	if ( count <= 0 )
+	{
+		return;
+	}
+	HIP_CHECK( hipMemcpyAsync( out, impl->contacts.ptr, count * sizeof( ContactSim ), hipMemcpyDeviceToHost, impl->stream ) );
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
@@ -2939,7 +3274,8 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 	impl->enlargedBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->fastBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->sleepTimes.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
-	impl->changes.Reserve( std::max<size_t>( lists.collideContacts.size(), 1 ), false, impl->stream, 0 );
+	// at most two entries per contact: a touch change and a mixing refresh
+	impl->changes.Reserve( std::max<size_t>( 2 * lists.collideContacts.size(), 1 ), false, impl->stream, 0 );
 	if ( impl->recording == false )
 	{
 		HIP_CHECK( hipStreamSynchronize( impl->stream ) );
@@ -2956,7 +3292,7 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 	}
 
 	hipStream_t s = impl->stream;
-	impl->EnsureMail( kMailHeader + (size_t)n );
+	impl->EnsureMail( kMailHeader + 2 * (size_t)n );
 
 	CollideArgs a;
 	a.list = impl->collideList.ptr;
@@ -2968,6 +3304,7 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 	a.changes = impl->changes.ptr;
 	a.counters = impl->counters;
 	a.recycleDistance = params.contactRecycleDistance;
+	a.customMixing = params.customMixing;
 
 	PackCollideArgs pc = { impl->counters, impl->changes.ptr, impl->mail };
 
@@ -3048,6 +3385,31 @@ static bool WarmByBody( bool fusedPath )
 	return fusedPath == false;
 }
 
+// Whether the current device can run the XCD solver: an MI355X (8 XCDs of 32 CUs) with
+// cooperative launches and room for one of its workgroups on every CU.
+static bool XcdSolverUsable()
+{
+	if ( getenv( "B2AMD_NO_XCD_SOLVER" ) != nullptr )
+	{
+		return false;
+	}
+	int device = 0;
+	hipDeviceProp_t prop;
+	if ( hipGetDevice( &device ) != hipSuccess || hipGetDeviceProperties( &prop, device ) != hipSuccess )
+	{
+		(void)hipGetLastError();
+		return false;
+	}
+	int perCu = 0;
+	if ( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, SolveXcdKernel, kXcdThreads, 0 ) != hipSuccess )
+	{
+		(void)hipGetLastError();
+		return false;
+	}
+	return prop.cooperativeLaunch != 0 && prop.multiProcessorCount == kXcdGrid && perCu >= 1 &&
+		   strncmp( prop.gcnArchName, "gfx950", 6 ) == 0;
+}
+
 void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bool needSleepTimes )
 {
 	const SolverLists& L = impl->lists;
@@ -3056,6 +3418,16 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	int nc = (int)L.constraintContacts.size();
 	int nj = (int)L.jointList.size();
 	const bool rec = impl->recording;
+
+	if ( rec && nb > 0 && impl->tape == nullptr && ( nb > kFusedBodyCapacity || L.activeColorCount > kMaxColorSlots ) )
+	{
+		// beyond the batch's persistent solver: this world's solve runs alone after the batch's
+		impl->solveDeferred = true;
+		impl->deferredParams = params;
+		impl->deferredRestitution = anyRestitution;
+		impl->deferredSleepTimes = needSleepTimes;
+		return;
+	}
 
 	if ( rec == false )
 	{
@@ -3374,7 +3746,69 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	{
 		fused = true;
 	}
-	if ( fused )
+	// The XCD solver when asked for, for worlds without overflow constraints. Auto keeps the color
+	// kernels: on MI355X both spend about 3.5-4.5 us per color phase (the phases are bound by the
+	// gather-solve-scatter latency chain, not by the launch), and the XCD solver measured 0.46 vs
+	// 0.42 ms/step on the large pyramid and 1.73 vs 1.82 ms/step on the 20k-body rain.
+	bool noOverflow = L.overflowContactCount == 0 && L.overflowJointCount == 0;
+	bool xcd = fused == false && rec == false && impl->deferredLaunched == false && nb > 0 && noOverflow &&
+			   L.activeColorCount <= kMaxColorSlots && impl->solverPath == 3;
+	if ( xcd && impl->xcdUsable < 0 )
+	{
+		impl->xcdUsable = XcdSolverUsable() ? 1 : 0;
+		if ( impl->xcdUsable == 1 )
+		{
+			HIP_CHECK( hipMalloc( &impl->xcdSync, 4 * sizeof( int ) ) );
+		}
+	}
+	xcd = xcd && impl->xcdUsable == 1;
+
+	if ( xcd )
+	{
+		XcdArgs xa;
+		FusedArgs& fa = xa.f;
+		fa.color = ca; // body ids (GlobalBodies)
+		fa.awake = impl->awakeBodies.ptr;
+		fa.bodyCount = nb;
+		fa.sims = impl->sims.ptr;
+		fa.states = impl->states.ptr;
+		fa.activeColorCount = L.activeColorCount;
+		for ( int c = 0; c <= L.activeColorCount; ++c )
+		{
+			fa.colorContactStart[c] = L.colorContactStart[c];
+			fa.colorJointStart[c] = L.colorJointStart[c];
+		}
+		fa.overflowContactStart = L.overflowContactStart;
+		fa.overflowContactCount = 0;
+		fa.overflowJointStart = L.overflowJointStart;
+		fa.overflowJointCount = 0;
+		fa.subStepCount = params.subStepCount;
+		fa.anyRestitution = anyRestitution ? 1 : 0;
+		// per body when the world has no joints (joint rows would cross workgroups)
+		fa.warmStart = impl->warmStart.ptr;
+		fa.warmList = impl->warmList.ptr;
+		fa.warmByBody = WarmByBody( false ) && haveWarmLists && nj == 0 ? 1 : 0;
+		xa.sync = impl->xcdSync;
+		xa.zero = 0;
+		double bytes = params.subStepCount * ( ( kBytesIntegrateVelocities + kBytesIntegratePositions ) * nb +
+											   ( kBytesWarmStart + kBytesSolve + kBytesRelax ) * nc + 3.0 * kBytesJoint * nj ) +
+					   ( anyRestitution ? kBytesRestitution * nc : 0.0 );
+		HIP_CHECK( hipMemsetAsync( impl->xcdSync, 0, 4 * sizeof( int ), s ) );
+		impl->Launch( kkSolveXcd, bytes, [&]( hipEvent_t e0, hipEvent_t e1 ) {
+			if ( e0 != nullptr )
+			{
+				HIP_CHECK( hipEventRecord( e0, s ) );
+			}
+			void* args[] = { &xa };
+			HIP_CHECK( hipLaunchCooperativeKernel( reinterpret_cast<const void*>( SolveXcdKernel ), dim3( kXcdGrid ), dim3( kXcdThreads ), args,
+												   0, s ) );
+			if ( e1 != nullptr )
+			{
+				HIP_CHECK( hipEventRecord( e1, s ) );
+			}
+		} );
+	}
+	else if ( fused )
 	{
 		FusedArgs fa;
 		fa.color = ca;
@@ -3510,6 +3944,8 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		ka.error = impl->counters + 7;
 		ka.noCollide = impl->ccdNoCollide.ptr;
 		ka.noCollideCount = (int)impl->ccdNoCollideStaging.size();
+		ka.filterRejects = impl->ccdFilterRejects.ptr;
+		ka.filterRejectCount = impl->ccdFilterRejectCount;
 		ka.enlargedShapeCount = impl->counters + 5;
 		ka.enlargedShapes = impl->enlargedShapes.ptr;
 		ka.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
@@ -3539,6 +3975,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		pk.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 		pk.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
 		pk.keptShapeCount = rec ? nullptr : impl->counters + kKeptShapeCountSlot;
+		pk.xcdSync = xcd ? impl->xcdSync : nullptr;
 		pk.bodyCount = nb;
 		pk.shapeOffset = kMailHeader + nb;
 		pk.aabbOffset = pk.shapeOffset + (int)shapeCap;
@@ -3586,13 +4023,28 @@ bool DeviceWorld::GetSolvePairInputs( SolvePairInputs* out ) const
 	return true;
 }
 
+void DeviceWorld::LaunchDeferredSolve()
+{
+	if ( impl->solveDeferred == false )
+	{
+		return;
+	}
+	impl->solveDeferred = false;
+	impl->deferredLaunched = true;
+	impl->recording = false;
+	SolveLaunch( impl->deferredParams, impl->deferredRestitution, impl->deferredSleepTimes );
+	impl->recording = true;
+}
+
 void DeviceWorld::SolveFinish( FinalizeResult* result )
 {
+	const bool ownStream = impl->recording == false || impl->deferredLaunched;
+	impl->deferredLaunched = false;
 	{
 		int nb = impl->solveBodyCount;
 		bool needSleepTimes = impl->solveNeedSleepTimes;
 		const PackFinalizeArgs& pk = impl->solvePack;
-		if ( impl->recording == false )
+		if ( ownStream )
 		{
 			HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 		}
@@ -3605,6 +4057,10 @@ void DeviceWorld::SolveFinish( FinalizeResult* result )
 		if ( mail[3] != 0 )
 		{
 			Fatal( "continuous collision: the static tree is deeper than the query stack (%d entries)", kCcdStackSize );
+		}
+		if ( mail[5] != 0 )
+		{
+			Fatal( "XCD solver: a barrier among its workgroups timed out (set B2AMD_NO_XCD_SOLVER to use the color kernels)" );
 		}
 		result->enlargedBodies.assign( mail + kMailHeader, mail + kMailHeader + enlargedBodyCount );
 		result->enlargedShapes.assign( mail + pk.shapeOffset, mail + pk.shapeOffset + enlargedShapeCount );
@@ -3646,7 +4102,7 @@ void DeviceWorld::SolveFinish( FinalizeResult* result )
 		std::sort( result->hits.begin(), result->hits.end(), []( const HitRecord& x, const HitRecord& y ) { return x.contactId < y.contactId; } );
 	}
 
-	if ( impl->recording == false )
+	if ( ownStream )
 	{
 		HIP_CHECK( hipEventElapsedTime( &lastSolveMs, impl->ev[2], impl->ev[3] ) );
 	}
@@ -4440,6 +4896,19 @@ void Batch::Collide( DeviceWorld** worlds, int count )
 	HIP_CHECK( hipStreamSynchronize( b->stream ) );
 }
 
+// Workgroup size of the persistent batched solver forced by b2amd_SetBatchSolverThreads (0: tuned).
+static std::atomic<int> s_forcedBatchThreads{ 0 };
+
+bool SetBatchSolverThreads( int threads )
+{
+	if ( threads != 0 && threads != 256 && threads != 512 && threads != 1024 )
+	{
+		return false;
+	}
+	s_forcedBatchThreads.store( threads );
+	return true;
+}
+
 void Batch::Solve( DeviceWorld** worlds, int count )
 {
 	StageList<PrepareArgs> prepare;
@@ -4565,11 +5034,12 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 																  reinterpret_cast<const int2*>( base + tapeBlocksOff ) );
 	}
 
+	int forced = s_forcedBatchThreads.load();
 	if ( fused.empty() == false )
 	{
 		// Autotune the workgroup size over the first batches: try 256, 512 and 1024 threads once
 		// each and keep the fastest. The choice does not change any result.
-		int threads = b->fusedThreads != 0 ? b->fusedThreads : kBatchThreadCandidates[b->tuneIndex];
+		int threads = forced != 0 ? forced : ( b->fusedThreads != 0 ? b->fusedThreads : kBatchThreadCandidates[b->tuneIndex] );
 		int lds = FusedLdsBytes( maxBodies );
 		int perCu = 0;
 		HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, SolveFusedBatchKernel, threads, lds ) );
@@ -4607,12 +5077,19 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 	HIP_CHECK( hipGetLastError() );
 	HIP_CHECK( hipStreamSynchronize( s ) );
 
+	// worlds beyond the persistent solver: their own solve, now that the batch has applied their
+	// uploads (each one's StepEnd waits for it)
+	for ( int i = 0; i < count; ++i )
+	{
+		worlds[i]->LaunchDeferredSolve();
+	}
+
 	if ( fused.empty() == false )
 	{
 		float ms = 0.0f;
 		HIP_CHECK( hipEventElapsedTime( &ms, b->ev0, b->ev1 ) );
 		lastFusedMs = ms;
-		if ( b->fusedThreads == 0 )
+		if ( b->fusedThreads == 0 && forced == 0 )
 		{
 			b->tuneMs[b->tuneIndex] = ms;
 			b->tuneIndex += 1;

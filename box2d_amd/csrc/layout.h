@@ -55,6 +55,9 @@ enum ContactFlags : uint32_t
 	kSimEnableHitEvent = 0x00100000,
 	kSimEnablePreSolveEvents = 0x00200000,
 	kSimRelativeTransformValid = 0x00400000,
+	// the contact's friction and restitution come from user mixing callbacks and are out of date:
+	// its next full narrowphase update asks the host for new values
+	kSimMixStale = 0x00800000,
 };
 
 // Shape flags
@@ -343,6 +346,9 @@ struct StepParams
 	int enableSleep;
 	int enableContinuous;
 	int enableContactSoftening;
+	// user mixing callbacks: bit 0 friction, bit 1 restitution (the host evaluates them; the
+	// narrowphase keeps the contact's values instead of mixing the shapes' own)
+	int customMixing;
 };
 
 } // namespace b2g

@@ -166,6 +166,87 @@ struct GlobalBodies
 	}
 };
 
+// GlobalBodies for solvers whose workgroups exchange body states inside one launch (the XCD
+// solver): loads are agent-scope relaxed atomic loads, which skip the CU's L1, where another
+// CU's earlier copy of the line may still sit, and read the L2 that every writer's stores reach.
+// Stores are plain: the L1 writes through to the L2.
+struct CoherentBodies
+{
+	BodyState* states;
+
+	B2G_DEVI static uint64_t Word( const BodyState* s, int k )
+	{
+		return __hip_atomic_load( reinterpret_cast<const uint64_t*>( s ) + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+	}
+
+	B2G_DEVI static float Lo( uint64_t u )
+	{
+		return __uint_as_float( (uint32_t)u );
+	}
+
+	B2G_DEVI static float Hi( uint64_t u )
+	{
+		return __uint_as_float( (uint32_t)( u >> 32 ) );
+	}
+
+	B2G_DEVI LaneBody Load( int index ) const
+	{
+		const BodyState* s = states + index;
+		uint64_t a = Word( s, 0 ), b = Word( s, 1 ), c = Word( s, 2 ), d = Word( s, 3 );
+		LaneBody r;
+		r.vx = Lo( a );
+		r.vy = Hi( a );
+		r.w = Lo( b );
+		r.flags = (uint32_t)( b >> 32 );
+		r.dpx = Lo( c );
+		r.dpy = Hi( c );
+		r.dqc = Lo( d );
+		r.dqs = Hi( d );
+		return r;
+	}
+
+	B2G_DEVI void LoadDelta( int index, b2Vec2& dp, b2Rot& dq ) const
+	{
+		const BodyState* s = states + index;
+		uint64_t c = Word( s, 2 ), d = Word( s, 3 );
+		dp = { Lo( c ), Hi( c ) };
+		dq = { Lo( d ), Hi( d ) };
+	}
+
+	B2G_DEVI void Store( int index, const LaneBody& b ) const
+	{
+		if ( b.flags & kDynamicFlag )
+		{
+			BodyState* s = states + index;
+			s->linearVelocity.x = b.vx;
+			s->linearVelocity.y = b.vy;
+			s->angularVelocity = b.w;
+		}
+	}
+
+	// The whole state of a body (for the per-body integration phases).
+	B2G_DEVI BodyState LoadState( int index ) const
+	{
+		LaneBody l = Load( index );
+		BodyState st;
+		st.linearVelocity = { l.vx, l.vy };
+		st.angularVelocity = l.w;
+		st.flags = l.flags;
+		st.deltaPosition = { l.dpx, l.dpy };
+		st.deltaRotation = { l.dqc, l.dqs };
+		return st;
+	}
+
+	B2G_DEVI static int JointA( const JointSim* j )
+	{
+		return j->indexA;
+	}
+	B2G_DEVI static int JointB( const JointSim* j )
+	{
+		return j->indexB;
+	}
+};
+
 struct LdsBodies
 {
 	float* vx;
@@ -869,11 +950,18 @@ B2G_DEVI void WarmStartJointSide( const JointSim* joint, bool sideB, const Bodie
 	}
 }
 
-template <int kChunk = 4, typename Bodies>
+// kOwnerTotals: the owner entries add the contacts' normal impulses to their total normal
+// impulses (otherwise each contact's own solve lane does it first, see AddWarmTotals).
+// kCoherent: the impulses, written by other workgroups inside the same launch, are read with
+// agent-scope loads that skip the CU's L1.
+template <int kChunk = 4, bool kOwnerTotals = true, bool kCoherent = false, typename Bodies>
 B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, const int* jointList, int overflowContactStart,
 							 const int* list, int begin, int end, const Bodies& B, b2Rot dq, b2Vec2 dp, float& vx, float& vy,
 							 float& w )
 {
+	auto impulse = []( const float* p ) {
+		return kCoherent ? __hip_atomic_load( p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) : *p;
+	};
 	// Entries go in chunks of kChunk: all of a chunk's constraint fields are gathered first
 	// (independent loads), then applied in order.
 	for ( int base = begin; base < end; base += kChunk )
@@ -903,14 +991,14 @@ B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, con
 			nY[q] = c.f[kcNormalY][k];
 			m[q] = sideB ? c.f[kcInvMassB][k] : c.f[kcInvMassA][k];
 			iv[q] = sideB ? c.f[kcInvIB][k] : c.f[kcInvIA][k];
-			roll[q] = c.f[kcRollingImpulse][k];
+			roll[q] = impulse( c.f[kcRollingImpulse] + k );
 			points[q] = overflow ? c.pointCount[k] : 2;
 #pragma unroll
 			for ( int j = 0; j < 2; ++j )
 			{
 				int o = j == 0 ? 0 : ( kcAnchorA2X - kcAnchorA1X );
-				ni[q][j] = c.f[kcNormalImpulse1 + o][k];
-				ti[q][j] = c.f[kcTangentImpulse1 + o][k];
+				ni[q][j] = impulse( c.f[kcNormalImpulse1 + o] + k );
+				ti[q][j] = impulse( c.f[kcTangentImpulse1 + o] + k );
 				rx[q][j] = sideB ? c.f[kcAnchorB1X + o][k] : c.f[kcAnchorA1X + o][k];
 				ry[q][j] = sideB ? c.f[kcAnchorB1Y + o][k] : c.f[kcAnchorA1Y + o][k];
 			}
@@ -962,7 +1050,7 @@ B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, con
 
 	// total normal impulses, once per contact (kept out of the loop above so its loads carry no
 	// dependence on stores)
-	for ( int e = begin; e < end; ++e )
+	for ( int e = begin; kOwnerTotals && e < end; ++e )
 	{
 		int entry = list[e];
 		if ( ( entry & 6 ) != 4 )
@@ -977,6 +1065,16 @@ B2G_DEVI void WarmStartBody( const ConstraintSoA& c, const JointSim* joints, con
 			c.f[kcTotalNormalImpulse1 + o][k] = c.f[kcTotalNormalImpulse1 + o][k] + c.f[kcNormalImpulse1 + o][k];
 		}
 	}
+}
+
+// The warm start's contribution to a colored contact's total normal impulses, for warm starts
+// that leave it to the contact's own lane (WarmStartBody with kOwnerTotals false): nothing reads
+// the totals between the warm start and the contact's solve, so adding them first in the solve
+// lane gives the same floats.
+B2G_DEVI void AddWarmTotals( const ConstraintSoA& c, int i )
+{
+	c.f[kcTotalNormalImpulse1][i] = c.f[kcTotalNormalImpulse1][i] + c.f[kcNormalImpulse1][i];
+	c.f[kcTotalNormalImpulse2][i] = c.f[kcTotalNormalImpulse2][i] + c.f[kcNormalImpulse2][i];
 }
 
 } // namespace b2g

@@ -415,10 +415,8 @@ World::World( const b2WorldDef* def, int worldIndex, uint16_t gen )
 			   (double)b2GetLengthUnitsPerMeter() );
 	}
 
-	if ( def->frictionCallback != nullptr || def->restitutionCallback != nullptr )
-	{
-		Fatal( "custom friction/restitution callbacks are not supported by the GPU step" );
-	}
+	frictionCallback = def->frictionCallback;
+	restitutionCallback = def->restitutionCallback;
 
 	device = new DeviceWorld();
 }
@@ -469,6 +467,17 @@ void World::PullShapes()
 		device->DownloadShapes( *this );
 		shapesStale = false;
 	}
+}
+
+const ContactSim& World::PulledContact( int contactId )
+{
+	if ( contactsStale || contactSims.size() < contacts.size() )
+	{
+		contactSims.resize( contacts.size() );
+		device->DownloadContacts( (int)contacts.size(), contactSims.data() );
+		contactsStale = false;
+	}
+	return contactSims[contactId];
 }
 
 void World::PullJoints()
@@ -1429,10 +1438,6 @@ b2ShapeId World::CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void
 	{
 		Fatal( "sensor shapes are not supported by the GPU step" );
 	}
-	if ( def->enablePreSolveEvents || def->enableCustomFiltering )
-	{
-		Fatal( "pre-solve callbacks and custom filtering are not supported by the GPU step" );
-	}
 
 	PullBodies();
 	PullShapes();
@@ -1729,8 +1734,8 @@ void World::CreateContact( Shape* shapeA, Shape* shapeB )
 	init.shapeIdB = shapeIdB;
 	init.bodyIdA = shapeA->bodyId;
 	init.bodyIdB = shapeB->bodyId;
-	init.friction = sqrtf( shapeA->material.friction * shapeB->material.friction );
-	init.restitution = b2MaxFloat( shapeA->material.restitution, shapeB->material.restitution );
+	init.friction = MixFriction( *shapeA, *shapeB );
+	init.restitution = MixRestitution( *shapeA, *shapeB );
 	init.simFlags = contact->flags;
 	if ( shapeA->enablePreSolveEvents || shapeB->enablePreSolveEvents )
 	{
@@ -2046,6 +2051,378 @@ void World::DestroyBody( b2BodyId id )
 	listsDirty = true;
 }
 
+// b2TransferBody (reference src/solver_set.c:511-545): move a body between solver sets, keeping
+// the reference's array order (remove-swap in the source, append to the target).
+void World::TransferBody( int targetSetIndex, int sourceSetIndex, Body* body )
+{
+	if ( targetSetIndex == sourceSetIndex )
+	{
+		return;
+	}
+	int sourceIndex = body->localIndex;
+	int targetIndex = (int)solverSets[targetSetIndex].bodyIds.size();
+	solverSets[targetSetIndex].bodyIds.push_back( body->id );
+
+	body->flags &= ~kTransientFlags;
+	bodySims[body->id].flags &= ~kTransientFlags;
+	bodyAux[body->id].flags &= ~kTransientFlags;
+
+	SolverSet& source = solverSets[sourceSetIndex];
+	int movedIndex = RemoveSwap( source.bodyIds, sourceIndex );
+	if ( movedIndex != B2_NULL_INDEX )
+	{
+		bodies[source.bodyIds[sourceIndex]].localIndex = sourceIndex;
+	}
+	if ( sourceSetIndex != kAwakeSet && targetSetIndex == kAwakeSet )
+	{
+		// a body entering the awake set starts from the identity state with its flags
+		bodyStates[body->id] = BodyState{ { 0.0f, 0.0f }, 0.0f, body->flags, { 0.0f, 0.0f }, { 1.0f, 0.0f } };
+	}
+	else if ( targetSetIndex != kAwakeSet )
+	{
+		// outside the awake set the state is the identity (constraint lanes read a static body's)
+		bodyStates[body->id] = BodyState{ { 0.0f, 0.0f }, 0.0f, 0u, { 0.0f, 0.0f }, { 1.0f, 0.0f } };
+	}
+	body->setIndex = targetSetIndex;
+	body->localIndex = targetIndex;
+	bodiesDirty = true;
+	listsDirty = true;
+}
+
+// b2TransferJoint (reference src/solver_set.c:547-605)
+void World::TransferJoint( int targetSetIndex, int sourceSetIndex, Joint* joint )
+{
+	if ( targetSetIndex == sourceSetIndex )
+	{
+		return;
+	}
+	int localIndex = joint->localIndex;
+	int colorIndex = joint->colorIndex;
+	if ( targetSetIndex == kAwakeSet )
+	{
+		AddJointToGraph( joint );
+		joint->setIndex = kAwakeSet;
+	}
+	else
+	{
+		SolverSet& target = solverSets[targetSetIndex];
+		joint->setIndex = targetSetIndex;
+		joint->localIndex = (int)target.jointIds.size();
+		joint->colorIndex = B2_NULL_INDEX;
+		target.jointIds.push_back( joint->jointId );
+	}
+	if ( sourceSetIndex == kAwakeSet )
+	{
+		RemoveJointFromGraph( joint->edges[0].bodyId, joint->edges[1].bodyId, colorIndex, localIndex );
+	}
+	else
+	{
+		SolverSet& source = solverSets[sourceSetIndex];
+		int movedIndex = RemoveSwap( source.jointIds, localIndex );
+		if ( movedIndex != B2_NULL_INDEX )
+		{
+			joints[source.jointIds[localIndex]].localIndex = localIndex;
+		}
+	}
+	listsDirty = true;
+}
+
+void World::DestroyBodyContacts( Body* body )
+{
+	int contactKey = body->headContactKey;
+	while ( contactKey != B2_NULL_INDEX )
+	{
+		Contact* contact = &contacts[contactKey >> 1];
+		contactKey = contact->edges[contactKey & 1].nextKey;
+		DestroyContact( contact );
+	}
+}
+
+void World::DestroyShapeProxy( Shape* shape )
+{
+	if ( shape->proxyKey != B2_NULL_INDEX )
+	{
+		broadPhase.DestroyProxy( shape->proxyKey );
+		shape->proxyKey = B2_NULL_INDEX;
+	}
+}
+
+void World::SetBodyTypeFlag( Body* body, b2BodyType type )
+{
+	body->type = type;
+	body->flags = type == b2_dynamicBody ? ( body->flags | kDynamicFlag ) : ( body->flags & ~kDynamicFlag );
+	bodyAux[body->id].type = type;
+}
+
+// b2Body_SetType (reference src/body.c:1134-1316)
+void World::SetBodyType( Body* body, b2BodyType type )
+{
+	b2BodyType originalType = body->type;
+	if ( originalType == type )
+	{
+		return;
+	}
+	PullBodies();
+	PullShapes();
+	PullJoints();
+
+	if ( body->setIndex == kDisabledSet )
+	{
+		SetBodyTypeFlag( body, type );
+		SyncBodyFlags( body );
+		UpdateBodyMassData( body );
+		return;
+	}
+
+	DestroyBodyContacts( body );
+	WakeBody( body );
+
+	// joints go to the static set first so they take consistent colors below
+	int jointKey = body->headJointKey;
+	while ( jointKey != B2_NULL_INDEX )
+	{
+		Joint* joint = &joints[jointKey >> 1];
+		jointKey = joint->edges[jointKey & 1].nextKey;
+		if ( joint->setIndex == kDisabledSet )
+		{
+			continue;
+		}
+		WakeBody( &bodies[joint->edges[0].bodyId] );
+		WakeBody( &bodies[joint->edges[1].bodyId] );
+		UnlinkJoint( joint );
+		TransferJoint( kStaticSet, joint->setIndex, joint );
+	}
+
+	SetBodyTypeFlag( body, type );
+	TransferBody( type == b2_staticBody ? kStaticSet : kAwakeSet, body->setIndex, body );
+
+	if ( originalType == b2_staticBody )
+	{
+		CreateIslandForBody( kAwakeSet, body );
+	}
+	else if ( type == b2_staticBody )
+	{
+		RemoveBodyFromIsland( body );
+	}
+
+	jointKey = body->headJointKey;
+	while ( jointKey != B2_NULL_INDEX )
+	{
+		Joint* joint = &joints[jointKey >> 1];
+		jointKey = joint->edges[jointKey & 1].nextKey;
+		if ( joint->setIndex == kDisabledSet )
+		{
+			continue;
+		}
+		const Body& bodyA = bodies[joint->edges[0].bodyId];
+		const Body& bodyB = bodies[joint->edges[1].bodyId];
+		if ( bodyA.type == b2_dynamicBody || bodyB.type == b2_dynamicBody )
+		{
+			TransferJoint( kAwakeSet, kStaticSet, joint );
+		}
+	}
+
+	// proxies go to the tree of the new type
+	int shapeId = body->headShapeId;
+	while ( shapeId != B2_NULL_INDEX )
+	{
+		Shape* shape = &shapes[shapeId];
+		shapeId = shape->nextShapeId;
+		DestroyShapeProxy( shape );
+		CreateShapeProxy( shape, true );
+	}
+
+	jointKey = body->headJointKey;
+	while ( jointKey != B2_NULL_INDEX )
+	{
+		int edgeIndex = jointKey & 1;
+		Joint* joint = &joints[jointKey >> 1];
+		jointKey = joint->edges[edgeIndex].nextKey;
+		const Body& other = bodies[joint->edges[edgeIndex ^ 1].bodyId];
+		if ( other.setIndex == kDisabledSet )
+		{
+			continue;
+		}
+		if ( body->type != b2_dynamicBody && other.type != b2_dynamicBody )
+		{
+			continue;
+		}
+		LinkJoint( joint );
+	}
+
+	SyncBodyFlags( body );
+	UpdateBodyMassData( body );
+	shapesDirty = true;
+	bodiesDirty = true;
+	jointsDirty = true;
+	listsDirty = true;
+}
+
+// b2Body_Disable (reference src/body.c:1699-1768)
+void World::DisableBody( Body* body )
+{
+	if ( body->setIndex == kDisabledSet )
+	{
+		return;
+	}
+	PullBodies();
+	PullShapes();
+	PullJoints();
+
+	DestroyBodyContacts( body );
+	int setIndex = body->setIndex;
+
+	int jointKey = body->headJointKey;
+	while ( jointKey != B2_NULL_INDEX )
+	{
+		Joint* joint = &joints[jointKey >> 1];
+		jointKey = joint->edges[jointKey & 1].nextKey;
+		if ( joint->setIndex == kDisabledSet )
+		{
+			continue;
+		}
+		UnlinkJoint( joint );
+		TransferJoint( kDisabledSet, joint->setIndex, joint );
+	}
+
+	int shapeId = body->headShapeId;
+	while ( shapeId != B2_NULL_INDEX )
+	{
+		Shape* shape = &shapes[shapeId];
+		shapeId = shape->nextShapeId;
+		DestroyShapeProxy( shape );
+	}
+
+	RemoveBodyFromIsland( body );
+	TransferBody( kDisabledSet, setIndex, body );
+	shapesDirty = true;
+	bodiesDirty = true;
+	jointsDirty = true;
+	listsDirty = true;
+}
+
+// b2Body_Enable (reference src/body.c:1770-1858)
+void World::EnableBody( Body* body )
+{
+	if ( body->setIndex != kDisabledSet )
+	{
+		return;
+	}
+	PullBodies();
+	PullShapes();
+	PullJoints();
+
+	int setId = body->type == b2_staticBody ? kStaticSet : kAwakeSet;
+	TransferBody( setId, kDisabledSet, body );
+
+	int shapeId = body->headShapeId;
+	while ( shapeId != B2_NULL_INDEX )
+	{
+		Shape* shape = &shapes[shapeId];
+		shapeId = shape->nextShapeId;
+		CreateShapeProxy( shape, true );
+	}
+
+	if ( setId != kStaticSet )
+	{
+		CreateIslandForBody( setId, body );
+	}
+
+	int jointKey = body->headJointKey;
+	while ( jointKey != B2_NULL_INDEX )
+	{
+		Joint* joint = &joints[jointKey >> 1];
+		jointKey = joint->edges[jointKey & 1].nextKey;
+		const Body& bodyA = bodies[joint->edges[0].bodyId];
+		const Body& bodyB = bodies[joint->edges[1].bodyId];
+		if ( bodyA.setIndex == kDisabledSet || bodyB.setIndex == kDisabledSet )
+		{
+			continue;
+		}
+		int jointSetId;
+		if ( bodyA.setIndex == kStaticSet && bodyB.setIndex == kStaticSet )
+		{
+			jointSetId = kStaticSet;
+		}
+		else if ( bodyA.setIndex == kStaticSet )
+		{
+			jointSetId = bodyB.setIndex;
+		}
+		else
+		{
+			jointSetId = bodyA.setIndex;
+		}
+		TransferJoint( jointSetId, kDisabledSet, joint );
+		if ( jointSetId != kStaticSet )
+		{
+			LinkJoint( joint );
+		}
+	}
+	shapesDirty = true;
+	bodiesDirty = true;
+	jointsDirty = true;
+	listsDirty = true;
+}
+
+// b2DestroyShape (reference src/shape.c:320-431): unlink the shape from its body's list (and the
+// device's copy of the list), drop its proxy and contacts, free its id.
+void World::DestroyShape( Shape* shape, bool updateBodyMass )
+{
+	PullBodies();
+	PullShapes();
+	Body* body = &bodies[shape->bodyId];
+	int shapeId = shape->id;
+
+	if ( shape->prevShapeId != B2_NULL_INDEX )
+	{
+		shapes[shape->prevShapeId].nextShapeId = shape->nextShapeId;
+		shapeData[shape->prevShapeId].nextShapeId = shape->nextShapeId;
+	}
+	if ( shape->nextShapeId != B2_NULL_INDEX )
+	{
+		shapes[shape->nextShapeId].prevShapeId = shape->prevShapeId;
+	}
+	if ( shapeId == body->headShapeId )
+	{
+		body->headShapeId = shape->nextShapeId;
+		bodySims[body->id].headShapeId = shape->nextShapeId;
+	}
+	body->shapeCount -= 1;
+
+	if ( shape->proxyKey != B2_NULL_INDEX )
+	{
+		broadPhase.DestroyProxy( shape->proxyKey );
+		shape->proxyKey = B2_NULL_INDEX;
+	}
+
+	int contactKey = body->headContactKey;
+	while ( contactKey != B2_NULL_INDEX )
+	{
+		Contact* contact = &contacts[contactKey >> 1];
+		contactKey = contact->edges[contactKey & 1].nextKey;
+		if ( contact->shapeIdA == shapeId || contact->shapeIdB == shapeId )
+		{
+			DestroyContact( contact );
+		}
+	}
+
+	if ( shape->material.restitution > 0.0f )
+	{
+		restitutionShapeCount -= 1;
+	}
+	shapeIdPool.Free( shapeId );
+	shape->id = B2_NULL_INDEX;
+	shapeData[shapeId].nextShapeId = B2_NULL_INDEX;
+
+	if ( updateBodyMass )
+	{
+		UpdateBodyMassData( body );
+	}
+	shapesDirty = true;
+	bodiesDirty = true;
+	listsDirty = true;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Host edits between steps (reference src/body.c:713-757, 1452-1463; src/shape.c:1317-1357,
 // 1545-1604)
@@ -2099,7 +2476,7 @@ void World::ApplyMassFromShapes( Body* body )
 
 // The shape's contacts go, its AABBs follow the new geometry and its proxy is created again
 // with forced pair creation.
-void World::ResetShapeProxy( Shape* shape )
+void World::ResetShapeProxy( Shape* shape, bool destroyProxy )
 {
 	Body* body = &bodies[shape->bodyId];
 	int shapeId = shape->id;
@@ -2127,8 +2504,133 @@ void World::ResetShapeProxy( Shape* shape )
 
 	if ( shape->proxyKey != B2_NULL_INDEX )
 	{
-		broadPhase.DestroyProxy( shape->proxyKey );
-		shape->proxyKey = broadPhase.CreateProxy( sd.fatAABB, proxyType, shape->filter.categoryBits, shapeId, true );
+		if ( destroyProxy )
+		{
+			broadPhase.DestroyProxy( shape->proxyKey );
+			shape->proxyKey = broadPhase.CreateProxy( sd.fatAABB, proxyType, shape->filter.categoryBits, shapeId, true );
+		}
+		else
+		{
+			broadPhase.MoveProxy( shape->proxyKey, sd.fatAABB );
+		}
+	}
+}
+
+// b2SyncBodyFlags (reference src/body.c:92-105): the body's flags without the transient bits go
+// to every copy of them.
+void World::SyncBodyFlags( Body* body )
+{
+	PullBodies();
+	uint32_t flags = body->flags & ~kTransientFlags;
+	bodySims[body->id].flags = flags;
+	bodyAux[body->id].flags = flags;
+	if ( body->setIndex == kAwakeSet )
+	{
+		bodyStates[body->id].flags = flags;
+	}
+	bodiesDirty = true;
+}
+
+// b2Body_EnableSleep (reference src/body.c:1670-1696)
+void World::EnableBodySleep( Body* body, bool enable )
+{
+	bool flag = ( body->flags & kEnableSleep ) != 0;
+	if ( enable == flag )
+	{
+		return;
+	}
+	body->flags = enable ? ( body->flags | kEnableSleep ) : ( body->flags & ~kEnableSleep );
+	SyncBodyFlags( body );
+	if ( enable == false )
+	{
+		WakeBody( body );
+	}
+}
+
+// b2Body_SetAwake (reference src/body.c:1550-1575)
+void World::SetBodyAwake( Body* body, bool awake )
+{
+	PullBodies();
+	if ( awake && body->setIndex >= kFirstSleepingSet )
+	{
+		WakeBody( body );
+	}
+	else if ( awake == false && body->setIndex == kAwakeSet )
+	{
+		Island& island = islands[body->islandId];
+		if ( island.constraintRemoveCount > 0 )
+		{
+			SplitIsland( body->islandId );
+		}
+		TrySleepIsland( body->islandId );
+	}
+}
+
+// b2Body_WakeTouching (reference src/body.c:1577-1609)
+void World::WakeTouching( Body* body )
+{
+	int contactKey = body->headContactKey;
+	while ( contactKey != B2_NULL_INDEX )
+	{
+		Contact& contact = contacts[contactKey >> 1];
+		int edgeIndex = contactKey & 1;
+		const Shape& shapeA = shapes[contact.shapeIdA];
+		const Shape& shapeB = shapes[contact.shapeIdB];
+		Body* other = shapeA.bodyId == body->id ? &bodies[shapeB.bodyId] : &bodies[shapeA.bodyId];
+		WakeBody( other );
+		contactKey = contact.edges[edgeIndex].nextKey;
+	}
+}
+
+// Shape material and filter changes (reference src/shape.c:1190-1308, 1367-1391).
+void World::SetShapeMaterial( Shape* shape, const b2SurfaceMaterial& material )
+{
+	PullShapes();
+	bool hadRestitution = shape->material.restitution > 0.0f;
+	shape->material = material;
+	ShapeDev& sd = shapeData[shape->id];
+	sd.friction = material.friction;
+	sd.restitution = material.restitution;
+	sd.rollingResistance = material.rollingResistance;
+	sd.tangentSpeed = material.tangentSpeed;
+	bool hasRestitution = material.restitution > 0.0f;
+	restitutionShapeCount += ( hasRestitution ? 1 : 0 ) - ( hadRestitution ? 1 : 0 );
+	shapesDirty = true;
+	if ( CustomMixing() != 0 )
+	{
+		mixingStale = true;
+	}
+}
+
+void World::SetShapeFilter( Shape* shape, b2Filter filter )
+{
+	if ( filter.maskBits == shape->filter.maskBits && filter.categoryBits == shape->filter.categoryBits &&
+		 filter.groupIndex == shape->filter.groupIndex )
+	{
+		return;
+	}
+	PullBodies();
+	PullShapes();
+	bool destroyProxy = filter.categoryBits != shape->filter.categoryBits;
+	shape->filter = filter;
+	shapeData[shape->id].filter = filter;
+	shapeFilterVersion += 1;
+	ResetShapeProxy( shape, destroyProxy );
+	shapesDirty = true;
+	listsDirty = true;
+}
+
+void World::SetShapeDensity( Shape* shape, float density, bool updateBodyMass )
+{
+	if ( density == shape->density )
+	{
+		return;
+	}
+	PullBodies();
+	shape->density = density;
+	if ( updateBodyMass )
+	{
+		UpdateBodyMassData( &bodies[shape->bodyId] );
 	}
 }
 
@@ -2657,6 +3159,27 @@ b2JointId World::CreateWheelJoint( const b2WheelJointDef* def )
 
 // Remaining rules of b2PairQueryCallback for one hit (reference src/broad_phase.c:216-324):
 // existing pair, shape order, same body, sensors, filters, shape types, joints.
+// Contact friction and restitution: the user's callbacks, or the default mixing rules
+// (reference src/physics_world.c:106-116, src/contact.c:351-354).
+float World::MixFriction( const Shape& a, const Shape& b ) const
+{
+	if ( frictionCallback != nullptr )
+	{
+		return frictionCallback( a.material.friction, a.material.userMaterialId, b.material.friction, b.material.userMaterialId );
+	}
+	return sqrtf( a.material.friction * b.material.friction );
+}
+
+float World::MixRestitution( const Shape& a, const Shape& b ) const
+{
+	if ( restitutionCallback != nullptr )
+	{
+		return restitutionCallback( a.material.restitution, a.material.userMaterialId, b.material.restitution,
+									b.material.userMaterialId );
+	}
+	return b2MaxFloat( a.material.restitution, b.material.restitution );
+}
+
 bool World::AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, int shapeId, MovePair* pair )
 {
 	uint64_t pairKey = B2G_SHAPE_PAIR_KEY( shapeId, queryShapeIndex );
@@ -2699,8 +3222,77 @@ bool World::AcceptPair( int queryProxyKey, int queryShapeIndex, int proxyKey, in
 	{
 		return false;
 	}
+	// custom user filter (reference src/broad_phase.c:281-294)
+	if ( ( shapeA->enableCustomFiltering || shapeB->enableCustomFiltering ) && customFilterFcn != nullptr )
+	{
+		if ( customFilterFcn( ShapeIdOf( shapeIdA ), ShapeIdOf( shapeIdB ), customFilterContext ) == false )
+		{
+			return false;
+		}
+	}
 	*pair = MovePair{ shapeIdA, shapeIdB };
 	return true;
+}
+
+// Continuous collision also asks the custom filter, with the static shape first, about every
+// (static shape, fast shape) candidate it reaches (reference src/solver.c:242-256). The callback
+// lives on the host, so it is asked before the step about each pair that can become such a
+// candidate: a shape of an awake dynamic body against a static shape with a proxy, either of them
+// with custom filtering on and the category/mask filter passing. The rejected pairs reach the
+// continuous kernel as sorted keys.
+void World::BuildContinuousFilterRejects()
+{
+	bool had = ccdFilterRejects.empty() == false;
+	ccdFilterRejects.clear();
+	if ( customFilterFcn != nullptr && enableContinuous )
+	{
+		std::vector<int> fastShapes, staticShapes, customStatic;
+		bool anyCustom = false;
+		for ( const Shape& shape : shapes )
+		{
+			if ( shape.id == B2_NULL_INDEX || shape.proxyKey == B2_NULL_INDEX || shape.sensorIndex != B2_NULL_INDEX )
+			{
+				continue;
+			}
+			const Body& body = bodies[shape.bodyId];
+			if ( body.type == b2_staticBody )
+			{
+				staticShapes.push_back( shape.id );
+				if ( shape.enableCustomFiltering )
+				{
+					customStatic.push_back( shape.id );
+				}
+			}
+			else if ( body.type == b2_dynamicBody && body.setIndex == kAwakeSet )
+			{
+				fastShapes.push_back( shape.id );
+			}
+			anyCustom = anyCustom || shape.enableCustomFiltering;
+		}
+		for ( int fastId = 0; anyCustom && fastId < (int)fastShapes.size(); ++fastId )
+		{
+			const Shape& fast = shapes[fastShapes[fastId]];
+			// a fast shape with custom filtering meets every static shape, others only the custom ones
+			const std::vector<int>& others = fast.enableCustomFiltering ? staticShapes : customStatic;
+			for ( int staticId : others )
+			{
+				const Shape& other = shapes[staticId];
+				if ( ShouldShapesCollide( fast.filter, other.filter ) == false )
+				{
+					continue;
+				}
+				if ( customFilterFcn( ShapeIdOf( other.id ), ShapeIdOf( fast.id ), customFilterContext ) == false )
+				{
+					ccdFilterRejects.push_back( (uint64_t)(uint32_t)other.id << 32 | (uint32_t)fast.id );
+				}
+			}
+		}
+		std::sort( ccdFilterRejects.begin(), ccdFilterRejects.end() );
+	}
+	if ( had || ccdFilterRejects.empty() == false )
+	{
+		device->UploadContinuousFilterRejects( ccdFilterRejects );
+	}
 }
 
 // New pairs of the moved proxies on the GPU (LBVH over all proxies). Proxies are numbered the way
@@ -3099,6 +3691,15 @@ void World::ProcessContactStates( const int* changes, int count )
 		int kind = changes[k] & 3;
 		Contact* contact = &contacts[contactId];
 
+		if ( kind == 0 )
+		{
+			// the narrowphase asks for the mixing callbacks' values (reference src/contact.c:548-551)
+			const Shape& shapeA = shapes[contact->shapeIdA];
+			const Shape& shapeB = shapes[contact->shapeIdB];
+			mixRefresh.push_back( ContactMaterial{ contactId, MixFriction( shapeA, shapeB ), MixRestitution( shapeA, shapeB ) } );
+			continue;
+		}
+
 		int colorIndex = contact->colorIndex;
 		int localIndex = contact->localIndex;
 
@@ -3141,6 +3742,11 @@ void World::ProcessContactStates( const int* changes, int count )
 
 			RemoveContactFromGraph( bodyIdA, bodyIdB, colorIndex, localIndex );
 		}
+	}
+	if ( mixRefresh.empty() == false )
+	{
+		device->UploadContactMaterials( mixRefresh );
+		mixRefresh.clear();
 	}
 }
 
@@ -3424,6 +4030,15 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	params.enableSleep = enableSleep ? 1 : 0;
 	params.enableContinuous = enableContinuous ? 1 : 0;
 	params.enableContactSoftening = enableContactSoftening ? 1 : 0;
+	params.customMixing = CustomMixing();
+	if ( mixingStale )
+	{
+		if ( params.customMixing != 0 )
+		{
+			device->MarkContactMixingStale();
+		}
+		mixingStale = false;
+	}
 
 	// Host edits made since the last step go to the GPU before any stage reads them.
 	uint64_t setupTicks = b2GetTicks();
@@ -3440,6 +4055,7 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	}
 	device->UploadContactInits( pendingContactInits, contactIdPool.Capacity() );
 	pendingContactInits.clear();
+	BuildContinuousFilterRejects();
 
 	if ( listsDirty )
 	{
@@ -3627,6 +4243,7 @@ void World::SolveEnd()
 	bodiesStale = true;
 	shapesStale = true;
 	jointsStale = true;
+	contactsStale = true;
 
 	// Move events: one per awake body in awake order (transforms fetched lazily).
 	uint64_t eventTicks = b2GetTicks();

@@ -391,6 +391,17 @@ public:
 	void SetBodyTransform( Body* body, b2Vec2 position, b2Rot rotation );
 	void ApplyMassFromShapes( Body* body );
 	void SetShapeGeometry( Shape* shape, b2ShapeType type, const void* geometry );
+	void DestroyShape( Shape* shape, bool updateBodyMass );
+	void SetBodyType( Body* body, b2BodyType type );
+	void DisableBody( Body* body );
+	void EnableBody( Body* body );
+	void SyncBodyFlags( Body* body );
+	void EnableBodySleep( Body* body, bool enable );
+	void SetBodyAwake( Body* body, bool awake );
+	void WakeTouching( Body* body );
+	void SetShapeMaterial( Shape* shape, const b2SurfaceMaterial& material );
+	void SetShapeFilter( Shape* shape, b2Filter filter );
+	void SetShapeDensity( Shape* shape, float density, bool updateBodyMass );
 
 	// The three phases of Step, split at the two points where the host waits for the GPU, so a
 	// batch can run each phase for many worlds around one launch per stage.
@@ -404,6 +415,10 @@ public:
 	void PushBodiesIfDirty();
 	void PullShapes();
 	void PullJoints();
+	// Contacts' manifolds and materials as the device last left them (read-only host copy).
+	const ContactSim& PulledContact( int contactId );
+	std::vector<ContactSim> contactSims;
+	bool contactsStale = true;
 
 	b2BodyEvents GetBodyEvents();
 	b2Counters GetCounters();
@@ -435,6 +450,25 @@ public:
 	void* userData = nullptr;
 	int workerCount = 1;
 	int stepIndex = 0;
+
+	// user callbacks, run on the host (reference src/physics_world.c:272-290, src/broad_phase.c:281-294)
+	b2CustomFilterFcn* customFilterFcn = nullptr;
+	void* customFilterContext = nullptr;
+	b2FrictionCallback* frictionCallback = nullptr;
+	b2RestitutionCallback* restitutionCallback = nullptr;
+	// a mixing callback or a shape material changed while a callback is set: every contact's
+	// stored values are out of date until its next full narrowphase update
+	bool mixingStale = false;
+	std::vector<ContactMaterial> mixRefresh;
+	int CustomMixing() const
+	{
+		return ( frictionCallback != nullptr ? 1 : 0 ) | ( restitutionCallback != nullptr ? 2 : 0 );
+	}
+	float MixFriction( const Shape& a, const Shape& b ) const;
+	// (static shape << 32 | fast shape) pairs the custom filter rejects for continuous collision
+	std::vector<uint64_t> ccdFilterRejects;
+	void BuildContinuousFilterRejects();
+	float MixRestitution( const Shape& a, const Shape& b ) const;
 	int splitIslandId = B2_NULL_INDEX;
 	float inv_h = 0.0f;
 	float inv_dt = 0.0f;
@@ -565,6 +599,11 @@ private:
 	void AddJointToGraph( Joint* joint );
 	void RemoveJointFromGraph( int bodyIdA, int bodyIdB, int colorIndex, int localIndex );
 	void DestroyContactsBetweenBodies( Body* bodyA, Body* bodyB );
+	void DestroyBodyContacts( Body* body );
+	void DestroyShapeProxy( Shape* shape );
+	void TransferBody( int targetSetIndex, int sourceSetIndex, Body* body );
+	void TransferJoint( int targetSetIndex, int sourceSetIndex, Joint* joint );
+	void SetBodyTypeFlag( Body* body, b2BodyType type );
 
 	// step stages
 	void UpdateBroadPhasePairs();
@@ -599,7 +638,7 @@ private:
 	bool solvePending = false;
 	uint64_t stepTicks = 0, collideTicks = 0, solveTicks = 0, constraintTicks = 0;
 	void FinishSolve( StepParams& params );
-	void ResetShapeProxy( Shape* shape );
+	void ResetShapeProxy( Shape* shape, bool destroyProxy = true );
 };
 
 World* GetWorld( b2WorldId id );

@@ -32,6 +32,10 @@ typedef struct b2sScene
 	b2BodyId links[B2S_EDIT_LINKS];
 	b2JointId hinges[B2S_EDIT_LINKS];
 	b2BodyId kinematicId;
+
+	// B2S_SETTERS: the chain links and the body with two shapes
+	b2BodyId setterLinks[3];
+	b2ShapeId extraShape;
 } b2sScene;
 
 static b2sScene s_scenes[B2S_MAX_HANDLES];
@@ -1110,6 +1114,318 @@ static void b2sEditsProbe( b2sScene* scene )
 	scene->probeHash = h;
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Setters and callbacks scene
+
+static float b2sFrictionMin( float frictionA, uint64_t materialA, float frictionB, uint64_t materialB )
+{
+	if ( materialA == 7 || materialB == 7 )
+	{
+		return 0.05f;
+	}
+	return frictionA < frictionB ? frictionA : frictionB;
+}
+
+static float b2sFrictionMax( float frictionA, uint64_t materialA, float frictionB, uint64_t materialB )
+{
+	(void)materialA;
+	(void)materialB;
+	return 1.5f * ( frictionA > frictionB ? frictionA : frictionB );
+}
+
+static float b2sRestitution( float restitutionA, uint64_t materialA, float restitutionB, uint64_t materialB )
+{
+	if ( materialA == 3 || materialB == 3 )
+	{
+		return 0.5f;
+	}
+	return restitutionA > restitutionB ? restitutionA : restitutionB;
+}
+
+// Rejects pairs whose bodies both have odd user data.
+static bool b2sOddFilter( b2ShapeId shapeIdA, b2ShapeId shapeIdB, void* context )
+{
+	(void)context;
+	intptr_t a = (intptr_t)b2Body_GetUserData( b2Shape_GetBody( shapeIdA ) );
+	intptr_t b = (intptr_t)b2Body_GetUserData( b2Shape_GetBody( shapeIdB ) );
+	return ( a & 1 ) == 0 || ( b & 1 ) == 0;
+}
+
+static b2ShapeId b2sFirstShape( b2BodyId bodyId )
+{
+	b2ShapeId shapeId = b2_nullShapeId;
+	b2Body_GetShapes( bodyId, &shapeId, 1 );
+	return shapeId;
+}
+
+static void b2sBuildSetters( b2sScene* scene, int enableSleep )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+	b2World_SetFrictionCallback( worldId, b2sFrictionMin );
+	b2World_SetRestitutionCallback( worldId, b2sRestitution );
+	b2World_SetCustomFilterCallback( worldId, b2sOddFilter, scene );
+	scene->bodyCapacity = 0;
+
+	b2BodyDef groundDef = b2DefaultBodyDef();
+	groundDef.userData = (void*)(intptr_t)1001;
+	b2BodyId groundId = b2CreateBody( worldId, &groundDef );
+	b2ShapeDef groundShape = b2DefaultShapeDef();
+	b2Polygon groundBox = b2MakeOffsetBox( 30.0f, 1.0f, ( b2Vec2 ){ 0.0f, -1.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &groundShape, &groundBox );
+
+	for ( int i = 0; i < 30; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ -9.0f + 2.5f * (float)( i % 6 ), 1.0f + 1.6f * (float)( i / 6 ) };
+		bodyDef.rotation = b2MakeRot( 0.07f * (float)i );
+		bodyDef.userData = (void*)(intptr_t)i;
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.material.friction = 0.3f + 0.02f * (float)i;
+		def.material.restitution = ( i % 5 == 0 ) ? 0.2f : 0.0f;
+		def.material.userMaterialId = (uint64_t)( i % 8 );
+		def.enableCustomFiltering = ( i % 3 ) == 0;
+		if ( i % 3 == 0 )
+		{
+			b2Polygon box = b2MakeBox( 0.45f, 0.35f );
+			b2CreatePolygonShape( bodyId, &def, &box );
+		}
+		else if ( i % 3 == 1 )
+		{
+			b2Circle circle = { { 0.0f, 0.0f }, 0.42f };
+			b2CreateCircleShape( bodyId, &def, &circle );
+		}
+		else
+		{
+			b2Capsule capsule = { { -0.35f, 0.0f }, { 0.35f, 0.0f }, 0.25f };
+			b2CreateCapsuleShape( bodyId, &def, &capsule );
+		}
+		if ( i == 11 )
+		{
+			b2Circle extra = { { 0.6f, 0.0f }, 0.2f };
+			scene->extraShape = b2CreateCircleShape( bodyId, &def, &extra );
+		}
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+
+	b2ShapeDef linkShape = b2DefaultShapeDef();
+	b2BodyId prev = groundId;
+	b2Vec2 anchor = { 12.0f, 7.0f };
+	for ( int i = 0; i < 3; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ anchor.x + 0.5f + 1.0f * (float)i, anchor.y };
+		bodyDef.userData = (void*)(intptr_t)( 100 + i );
+		b2BodyId link = b2CreateBody( worldId, &bodyDef );
+		b2Capsule capsule = { { -0.5f, 0.0f }, { 0.5f, 0.0f }, 0.15f };
+		b2CreateCapsuleShape( link, &linkShape, &capsule );
+		b2RevoluteJointDef jd = b2DefaultRevoluteJointDef();
+		jd.base.bodyIdA = prev;
+		jd.base.bodyIdB = link;
+		b2Pos pivot = { anchor.x + 1.0f * (float)i, anchor.y };
+		jd.base.localFrameA.p = b2Body_GetLocalPoint( prev, pivot );
+		jd.base.localFrameB.p = b2Body_GetLocalPoint( link, pivot );
+		b2CreateRevoluteJoint( worldId, &jd );
+		scene->setterLinks[i] = link;
+		b2sPushBody( scene, &scene->bodyCapacity, link );
+		prev = link;
+	}
+
+	b2BodyDef kinDef = b2DefaultBodyDef();
+	kinDef.type = b2_kinematicBody;
+	kinDef.position = ( b2Pos ){ -12.0f, 2.5f };
+	kinDef.angularVelocity = 0.8f;
+	kinDef.userData = (void*)(intptr_t)200;
+	scene->kinematicId = b2CreateBody( worldId, &kinDef );
+	b2Polygon paddle = b2MakeBox( 1.5f, 0.2f );
+	b2CreatePolygonShape( scene->kinematicId, &linkShape, &paddle );
+	b2sPushBody( scene, &scene->bodyCapacity, scene->kinematicId );
+}
+
+static void b2sSettersBeforeStep( b2sScene* scene, int k )
+{
+	b2WorldId worldId = scene->worldId;
+	b2BodyId* b = scene->bodies;
+	if ( k == 20 )
+	{
+		b2Body_SetLinearDamping( b[0], 0.8f );
+		b2Body_SetAngularDamping( b[1], 0.5f );
+		b2Body_SetGravityScale( b[2], -0.3f );
+	}
+	if ( k == 30 )
+	{
+		b2Shape_SetFriction( b2sFirstShape( b[3] ), 0.05f );
+		b2Shape_SetRestitution( b2sFirstShape( b[4] ), 0.6f );
+		b2Shape_SetUserMaterial( b2sFirstShape( b[5] ), 7 );
+	}
+	if ( k == 40 )
+	{
+		b2World_SetFrictionCallback( worldId, NULL );
+	}
+	if ( k == 45 )
+	{
+		b2Body_SetType( b[6], b2_staticBody );
+	}
+	if ( k == 50 )
+	{
+		b2World_SetRestitutionThreshold( worldId, 0.3f );
+		b2World_SetHitEventThreshold( worldId, 0.5f );
+	}
+	if ( k == 55 )
+	{
+		b2Body_SetType( b[6], b2_dynamicBody );
+		b2Body_SetType( b[7], b2_kinematicBody );
+		b2Body_SetLinearVelocity( b[7], ( b2Vec2 ){ 0.5f, 1.0f } );
+	}
+	if ( k == 60 )
+	{
+		b2World_SetFrictionCallback( worldId, b2sFrictionMax );
+	}
+	if ( k == 65 )
+	{
+		b2Body_Disable( b[8] );
+		b2Body_Disable( scene->setterLinks[1] );
+	}
+	if ( k == 75 )
+	{
+		b2Body_Enable( b[8] );
+	}
+	if ( k == 80 )
+	{
+		b2Filter filter = b2DefaultFilter();
+		filter.categoryBits = 2;
+		filter.maskBits = ~(uint64_t)4;
+		b2Shape_SetFilter( b2sFirstShape( b[9] ), filter );
+		b2Filter groupFilter = b2DefaultFilter();
+		groupFilter.groupIndex = -3;
+		b2Shape_SetFilter( b2sFirstShape( b[15] ), groupFilter );
+		b2Shape_SetFilter( b2sFirstShape( b[21] ), groupFilter );
+		b2Shape_SetDensity( b2sFirstShape( b[10] ), 3.0f, true );
+	}
+	if ( k == 85 )
+	{
+		b2DestroyShape( scene->extraShape, true );
+	}
+	if ( k == 90 )
+	{
+		b2Body_Enable( scene->setterLinks[1] );
+		b2World_SetCustomFilterCallback( worldId, NULL, NULL );
+	}
+	if ( k == 100 )
+	{
+		b2Body_SetAwake( b[12], false );
+		b2Body_EnableSleep( b[13], false );
+		b2Body_SetSleepThreshold( b[14], 0.01f );
+	}
+	if ( k == 110 )
+	{
+		b2Body_SetType( scene->setterLinks[0], b2_staticBody );
+	}
+	if ( k == 120 )
+	{
+		b2Body_SetAwake( b[12], true );
+		b2Body_WakeTouching( b[0] );
+		b2Body_EnableSleep( b[13], true );
+	}
+	if ( k == 130 )
+	{
+		b2World_SetRestitutionCallback( worldId, NULL );
+		b2SurfaceMaterial m = b2DefaultSurfaceMaterial();
+		m.friction = 0.9f;
+		m.restitution = 0.3f;
+		m.userMaterialId = 3;
+		b2Shape_SetSurfaceMaterial( b2sFirstShape( b[16] ), &m );
+	}
+	if ( k == 140 )
+	{
+		b2Body_SetType( b[7], b2_dynamicBody );
+		b2Body_SetType( scene->setterLinks[0], b2_dynamicBody );
+	}
+	if ( k == 150 )
+	{
+		b2Body_Disable( scene->kinematicId );
+	}
+	if ( k == 160 )
+	{
+		b2Body_Enable( scene->kinematicId );
+		b2World_SetRestitutionCallback( worldId, b2sRestitution );
+	}
+}
+
+static uint32_t b2sFnvManifold( uint32_t h, const b2Manifold* m )
+{
+	h = b2sFnvFloat( b2sFnvFloat( h, m->normal.x ), m->normal.y );
+	h = b2sFnvWord( h, (uint32_t)m->pointCount );
+	for ( int p = 0; p < m->pointCount; ++p )
+	{
+		const b2ManifoldPoint* mp = m->points + p;
+		h = b2sFnvFloat( b2sFnvFloat( h, mp->anchorA.x ), mp->anchorB.y );
+		h = b2sFnvFloat( b2sFnvFloat( h, mp->separation ), mp->normalImpulse );
+		h = b2sFnvFloat( b2sFnvFloat( h, mp->tangentImpulse ), mp->totalNormalImpulse );
+		h = b2sFnvWord( h, (uint32_t)mp->id | ( mp->persisted ? 0x10000u : 0u ) );
+	}
+	return h;
+}
+
+static void b2sSettersProbe( b2sScene* scene )
+{
+	uint32_t h = scene->probeHash;
+	b2BodyId* b = scene->bodies;
+	b2WorldId worldId = scene->worldId;
+	h = b2sFnvFloat( h, b2World_GetRestitutionThreshold( worldId ) );
+	h = b2sFnvFloat( h, b2World_GetHitEventThreshold( worldId ) );
+	for ( int i = 0; i < 16; ++i )
+	{
+		b2BodyId id = b[i];
+		h = b2sFnvFloat( h, b2Body_GetLinearDamping( id ) );
+		h = b2sFnvFloat( h, b2Body_GetAngularDamping( id ) );
+		h = b2sFnvFloat( h, b2Body_GetGravityScale( id ) );
+		h = b2sFnvFloat( h, b2Body_GetSleepThreshold( id ) );
+		h = b2sFnvWord( h, (uint32_t)b2Body_GetType( id ) | (uint32_t)b2Body_IsEnabled( id ) << 4 |
+							   (uint32_t)b2Body_IsSleepEnabled( id ) << 5 | (uint32_t)b2Body_IsAwake( id ) << 6 );
+		h = b2sFnvFloat( h, b2Body_GetMass( id ) );
+		b2ShapeId shapeId = b2sFirstShape( id );
+		h = b2sFnvFloat( h, b2Shape_GetFriction( shapeId ) );
+		h = b2sFnvFloat( h, b2Shape_GetRestitution( shapeId ) );
+		h = b2sFnvFloat( h, b2Shape_GetDensity( shapeId ) );
+		b2Filter f = b2Shape_GetFilter( shapeId );
+		h = b2sFnvWord( h, (uint32_t)f.categoryBits ^ (uint32_t)( f.maskBits >> 3 ) ^ (uint32_t)f.groupIndex );
+		h = b2sFnvWord( h, (uint32_t)b2Shape_GetUserMaterial( shapeId ) | (uint32_t)b2Body_GetShapeCount( id ) << 8 );
+	}
+	for ( int i = 0; i < 3; ++i )
+	{
+		h = b2sFnvWord( h, (uint32_t)b2Body_GetType( scene->setterLinks[i] ) | (uint32_t)b2Body_IsEnabled( scene->setterLinks[i] ) << 4 );
+	}
+	h = b2sFnvWord( h, (uint32_t)b2Body_IsEnabled( scene->kinematicId ) );
+
+	// contact data of a body, of a shape and of one contact
+	b2ContactData data[16];
+	int n = b2Body_GetContactData( b[0], data, 16 );
+	h = b2sFnvWord( h, (uint32_t)n | (uint32_t)b2Body_GetContactCapacity( b[0] ) << 8 );
+	for ( int i = 0; i < n; ++i )
+	{
+		h = b2sFnvWord( h, (uint32_t)data[i].shapeIdA.index1 | (uint32_t)data[i].shapeIdB.index1 << 16 );
+		h = b2sFnvManifold( h, &data[i].manifold );
+	}
+	if ( n > 0 && b2Contact_IsValid( data[0].contactId ) )
+	{
+		b2ContactData one = b2Contact_GetData( data[0].contactId );
+		h = b2sFnvManifold( h, &one.manifold );
+	}
+	n = b2Shape_GetContactData( b2sFirstShape( b[6] ), data, 16 );
+	h = b2sFnvWord( h, (uint32_t)n | (uint32_t)b2Shape_GetContactCapacity( b2sFirstShape( b[6] ) ) << 8 );
+	for ( int i = 0; i < n; ++i )
+	{
+		h = b2sFnvManifold( h, &data[i].manifold );
+	}
+	h = b2sFnvWord( h, (uint32_t)b2World_GetAwakeBodyCount( worldId ) );
+	scene->probeHash = h;
+}
+
 int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, const float* fparams, int fparamCount )
 {
 	int handle = -1;
@@ -1201,6 +1517,10 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 			b2sBuildJointZoo( scene, iparamCount > 0 ? iparams[0] : 0 );
 			break;
 
+		case B2S_SETTERS:
+			b2sBuildSetters( scene, iparamCount > 0 ? iparams[0] : 0 );
+			break;
+
 		case B2S_RAIN:
 		{
 			int count = iparamCount > 0 ? iparams[0] : 20000;
@@ -1265,11 +1585,19 @@ void b2s_step( int handle, float timeStep, int subStepCount, int count )
 		{
 			b2sEditsBeforeStep( scene, scene->stepIndex, timeStep );
 		}
+		else if ( scene->kind == B2S_SETTERS )
+		{
+			b2sSettersBeforeStep( scene, scene->stepIndex );
+		}
 		b2World_Step( scene->worldId, timeStep, subStepCount );
 		scene->stepIndex += 1;
 		if ( scene->kind == B2S_EDITS )
 		{
 			b2sEditsProbe( scene );
+		}
+		else if ( scene->kind == B2S_SETTERS )
+		{
+			b2sSettersProbe( scene );
 		}
 	}
 }

@@ -82,6 +82,18 @@ enum
 	// results (local/world points and vectors, joint angles, motor torques, joint and shape
 	// queries, masses) into b2s_probe_hash. iparams: [0] enableSleep.
 	B2S_EDITS = 9,
+
+	// Setters and callbacks between steps: 30 boxes, circles and capsules (user data = index,
+	// user material ids 0-7, some with custom filtering, one with a second shape), a three-link
+	// hinge chain and a kinematic paddle on a 30 x 1 m ground box (half extents). The world has
+	// friction and restitution callbacks and a custom filter; before chosen steps b2s_step changes
+	// damping, gravity scale, shape materials, density and filters, body types (dynamic, static,
+	// kinematic, with joints attached), disables and enables bodies, destroys a shape, changes
+	// sleep settings and wakes or sleeps bodies, swaps and clears the callbacks and changes the
+	// restitution and hit-event thresholds. After each step it folds getter results (damping,
+	// gravity scale, materials, filters, types, sleep settings, thresholds, contact data of a body,
+	// a shape and a contact) into b2s_probe_hash. iparams: [0] enableSleep.
+	B2S_SETTERS = 10,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.

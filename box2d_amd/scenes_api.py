@@ -28,6 +28,7 @@ SCENE_PUSH = 6
 SCENE_JOINT_ZOO = 7
 SCENE_EVENTS = 8
 SCENE_EDITS = 9
+SCENE_SETTERS = 10
 
 
 class SceneLib:

@@ -35,6 +35,17 @@ B2_API b2Profile b2World_GetProfile( b2WorldId worldId );
 B2_API b2Counters b2World_GetCounters( b2WorldId worldId );
 B2_API void b2World_SetUserData( b2WorldId worldId, void* userData );
 B2_API void* b2World_GetUserData( b2WorldId worldId );
+/// World settings and callbacks (reference: include/box2d/box2d.h:127-144, 209-213). The callbacks
+/// run on the host: the custom filter where the pair update filters candidates, the mixing
+/// callbacks when a contact is created and whenever a full narrowphase update follows a change of
+/// callback or shape material.
+B2_API void b2World_SetRestitutionThreshold( b2WorldId worldId, float value );
+B2_API float b2World_GetRestitutionThreshold( b2WorldId worldId );
+B2_API void b2World_SetHitEventThreshold( b2WorldId worldId, float value );
+B2_API float b2World_GetHitEventThreshold( b2WorldId worldId );
+B2_API void b2World_SetCustomFilterCallback( b2WorldId worldId, b2CustomFilterFcn* fcn, void* context );
+B2_API void b2World_SetFrictionCallback( b2WorldId worldId, b2FrictionCallback* callback );
+B2_API void b2World_SetRestitutionCallback( b2WorldId worldId, b2RestitutionCallback* callback );
 
 /// Bodies (reference: include/box2d/box2d.h:355-645)
 B2_API b2BodyId b2CreateBody( b2WorldId worldId, const b2BodyDef* def );
@@ -73,6 +84,31 @@ B2_API void b2Body_ApplyMassFromShapes( b2BodyId bodyId );
 B2_API int b2Body_GetShapes( b2BodyId bodyId, b2ShapeId* shapeArray, int capacity );
 B2_API int b2Body_GetJointCount( b2BodyId bodyId );
 B2_API int b2Body_GetJoints( b2BodyId bodyId, b2JointId* jointArray, int capacity );
+/// Damping, gravity scale and sleep (reference: include/box2d/box2d.h:526-582)
+B2_API void b2Body_SetLinearDamping( b2BodyId bodyId, float linearDamping );
+B2_API float b2Body_GetLinearDamping( b2BodyId bodyId );
+B2_API void b2Body_SetAngularDamping( b2BodyId bodyId, float angularDamping );
+B2_API float b2Body_GetAngularDamping( b2BodyId bodyId );
+B2_API void b2Body_SetGravityScale( b2BodyId bodyId, float gravityScale );
+B2_API float b2Body_GetGravityScale( b2BodyId bodyId );
+B2_API void b2Body_SetAwake( b2BodyId bodyId, bool awake );
+B2_API void b2Body_WakeTouching( b2BodyId bodyId );
+B2_API void b2Body_EnableSleep( b2BodyId bodyId, bool enableSleep );
+B2_API bool b2Body_IsSleepEnabled( b2BodyId bodyId );
+B2_API void b2Body_SetSleepThreshold( b2BodyId bodyId, float sleepThreshold );
+B2_API float b2Body_GetSleepThreshold( b2BodyId bodyId );
+B2_API bool b2Body_IsEnabled( b2BodyId bodyId );
+/// Contact data (reference: include/box2d/box2d.h:633-641, 833-841, 1519-1525)
+B2_API int b2Body_GetContactCapacity( b2BodyId bodyId );
+B2_API int b2Body_GetContactData( b2BodyId bodyId, b2ContactData* contactData, int capacity );
+B2_API int b2Shape_GetContactCapacity( b2ShapeId shapeId );
+B2_API int b2Shape_GetContactData( b2ShapeId shapeId, b2ContactData* contactData, int capacity );
+B2_API bool b2Contact_IsValid( b2ContactId id );
+B2_API b2ContactData b2Contact_GetData( b2ContactId contactId );
+/// Type changes and enabling (reference: include/box2d/box2d.h:366-372, 576-582)
+B2_API void b2Body_SetType( b2BodyId bodyId, b2BodyType type );
+B2_API void b2Body_Disable( b2BodyId bodyId );
+B2_API void b2Body_Enable( b2BodyId bodyId );
 
 /// Shapes (reference: include/box2d/box2d.h:659-908)
 B2_API b2ShapeId b2CreateCircleShape( b2BodyId bodyId, const b2ShapeDef* def, const b2Circle* circle );
@@ -93,6 +129,25 @@ B2_API void b2Shape_SetCircle( b2ShapeId shapeId, const b2Circle* circle );
 B2_API void b2Shape_SetCapsule( b2ShapeId shapeId, const b2Capsule* capsule );
 B2_API void b2Shape_SetSegment( b2ShapeId shapeId, const b2Segment* segment );
 B2_API void b2Shape_SetPolygon( b2ShapeId shapeId, const b2Polygon* polygon );
+/// Shape destruction, materials, density, filter and user data (reference:
+/// include/box2d/box2d.h:683-760)
+B2_API void b2DestroyShape( b2ShapeId shapeId, bool updateBodyMass );
+B2_API b2WorldId b2Shape_GetWorld( b2ShapeId shapeId );
+B2_API bool b2Shape_IsSensor( b2ShapeId shapeId );
+B2_API void b2Shape_SetUserData( b2ShapeId shapeId, void* userData );
+B2_API void* b2Shape_GetUserData( b2ShapeId shapeId );
+B2_API void b2Shape_SetDensity( b2ShapeId shapeId, float density, bool updateBodyMass );
+B2_API float b2Shape_GetDensity( b2ShapeId shapeId );
+B2_API void b2Shape_SetFriction( b2ShapeId shapeId, float friction );
+B2_API float b2Shape_GetFriction( b2ShapeId shapeId );
+B2_API void b2Shape_SetRestitution( b2ShapeId shapeId, float restitution );
+B2_API float b2Shape_GetRestitution( b2ShapeId shapeId );
+B2_API void b2Shape_SetUserMaterial( b2ShapeId shapeId, uint64_t material );
+B2_API uint64_t b2Shape_GetUserMaterial( b2ShapeId shapeId );
+B2_API void b2Shape_SetSurfaceMaterial( b2ShapeId shapeId, const b2SurfaceMaterial* surfaceMaterial );
+B2_API b2SurfaceMaterial b2Shape_GetSurfaceMaterial( b2ShapeId shapeId );
+B2_API b2Filter b2Shape_GetFilter( b2ShapeId shapeId );
+B2_API void b2Shape_SetFilter( b2ShapeId shapeId, b2Filter filter );
 B2_API b2ChainId b2CreateChain( b2BodyId bodyId, const b2ChainDef* def );
 
 /// Joints (reference: include/box2d/box2d.h:919-1120)

@@ -24,6 +24,11 @@ typedef void b2FinishTaskCallback( void* userTask, void* userContext );
 typedef float b2FrictionCallback( float frictionA, uint64_t userMaterialIdA, float frictionB, uint64_t userMaterialIdB );
 typedef float b2RestitutionCallback( float restitutionA, uint64_t userMaterialIdA, float restitutionB, uint64_t userMaterialIdB );
 
+/// Custom contact filter, called for a candidate pair when either shape has custom filtering
+/// enabled; return false to keep the shapes from colliding. Must be thread-safe; must not modify
+/// the world. (reference: include/box2d/types.h:1240-1252)
+typedef bool b2CustomFilterFcn( b2ShapeId shapeIdA, b2ShapeId shapeIdB, void* context );
+
 typedef struct b2Capacity
 {
 	int staticShapeCount;

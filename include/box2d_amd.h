@@ -35,14 +35,18 @@ B2_API void b2amd_World_ResetKernelStats( b2WorldId worldId );
 
 /// Solver path of a world's step. The fused path runs the whole substep loop in one workgroup
 /// with the awake body states in LDS (up to 5400 awake bodies); the color-kernel path launches one
-/// kernel per graph color and stage. Auto picks the fused path for small worlds (up to 4096
-/// contact constraints). Results are bit-identical across paths; the switch exists for testing
-/// and measurement. Fused falls back to color kernels for worlds beyond its LDS budget.
+/// kernel per graph color and stage; the XCD path runs the whole substep loop in one launch on the
+/// 32 compute units of one XCD, with a barrier among its workgroups between colors instead of
+/// kernel boundaries (MI355X only; worlds without overflow constraints). Auto picks the fused path
+/// for small worlds (up to 4096 contact constraints), else the color kernels. Results are
+/// bit-identical across paths; the switch exists for testing and measurement. Fused falls back to
+/// color kernels for worlds beyond its LDS budget, XCD to color kernels where it does not apply.
 typedef enum b2amd_SolverPath
 {
 	b2amd_solverAuto = 0,
 	b2amd_solverColorKernels = 1,
 	b2amd_solverFused = 2,
+	b2amd_solverXcd = 3,
 } b2amd_SolverPath;
 
 B2_API void b2amd_World_SetSolverPath( b2WorldId worldId, b2amd_SolverPath path );
@@ -144,9 +148,15 @@ typedef struct b2amd_BatchStats
 /// Step count independent worlds by timeStep with subStepCount substeps, as one batch: host
 /// bookkeeping runs on host threads and every GPU stage is one launch over all the worlds, the
 /// substep loop a persistent launch over a world queue. Each world ends exactly as if stepped
-/// alone with b2World_Step. Every world's awake bodies must fit the fused solver (5400).
+/// alone with b2World_Step. A world whose awake bodies do not fit the persistent solver's LDS
+/// (more than 5400) has its substep loop run on its own, by the per-world path, after the batch's.
 /// Returns the number of worlds stepped; stats may be null.
 B2_API int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int subStepCount, b2amd_BatchStats* stats );
+
+/// Workgroup size of b2amd_StepWorlds' persistent solver: 256, 512 or 1024 threads, or 0 to let the
+/// first three batch steps try each and keep the fastest (the default). The results do not depend
+/// on it. Returns 0, or -1 for another value.
+B2_API int b2amd_SetBatchSolverThreads( int threads );
 
 // ---------------------------------------------------------------------------------------------
 // Differentiation of recorded steps.

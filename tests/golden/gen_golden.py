@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from box2d_amd.scenes_api import (SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES,  # noqa: E402
-                                  SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS, SCENE_EDITS)
+                                  SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS, SCENE_EDITS, SCENE_SETTERS)
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
 
@@ -90,16 +90,16 @@ def events_fixture(lib, name, count, seed, steps):
     return fx
 
 
-def edits_fixture(lib, name, sleep, steps):
-    """Per step of the edits scene: the transform hash of the listed bodies, the getter probe
-    hash and the listed body count."""
-    h = lib.create(SCENE_EDITS, (sleep,), ())
+def edits_fixture(lib, name, sleep, steps, kind=SCENE_EDITS):
+    """Per step of an edits scene (SCENE_EDITS or SCENE_SETTERS): the transform hash of the listed
+    bodies, the getter probe hash and the listed body count."""
+    h = lib.create(kind, (sleep,), ())
     per_step = []
     for _ in range(steps):
         lib.step(h)
         per_step.append([lib.hash(h), lib.probe_hash(h), lib.body_count(h)])
     lib.destroy(h)
-    return {"name": name, "kind": SCENE_EDITS, "iparams": [sleep], "fparams": [], "steps": steps, "per_step": per_step}
+    return {"name": name, "kind": kind, "iparams": [sleep], "fparams": [], "steps": steps, "per_step": per_step}
 
 
 def main():
@@ -125,7 +125,8 @@ def main():
     fixtures.append(scene_fixture(lib, "joint_zoo", SCENE_JOINT_ZOO, (0,), (), 400, set(range(1, 401)), True))
     fixtures.append(scene_fixture(lib, "joint_zoo_sleep", SCENE_JOINT_ZOO, (1,), (), 400, set(range(1, 401)), True))
     events = [events_fixture(lib, "events_60", 60, 12345, 300), events_fixture(lib, "events_150", 150, 7, 240)]
-    edits = [edits_fixture(lib, "edits", 0, 240), edits_fixture(lib, "edits_sleep", 1, 240)]
+    edits = [edits_fixture(lib, "edits", 0, 240), edits_fixture(lib, "edits_sleep", 1, 240),
+             edits_fixture(lib, "setters", 0, 200, SCENE_SETTERS), edits_fixture(lib, "setters_sleep", 1, 200, SCENE_SETTERS)]
     ok, step, hsh = lib.falling_hinges(1, 500)
     hinges = {"name": "falling_hinges", "slept": ok, "sleep_step": step, "hash": hsh}
     out = {"generator": "tests/golden/gen_golden.py", "reference": "erincatto/box2d v3.2 (src/*.c, -O2 -ffp-contract=off)",

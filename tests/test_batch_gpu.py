@@ -16,14 +16,26 @@ from conftest import REF_SCENES
 
 pytestmark = pytest.mark.gpu
 
-CHECK = {1, 2, 10, 30, 60, 90}
+CHECK = {1, 2, 3, 10, 30, 60, 90}
 
 
-def test_batch_matches_single_worlds(gpu_scenes):
+def _set_batch_threads(threads):
+    import box2d_amd
+
+    lib = box2d_amd.library()
+    lib.b2amd_SetBatchSolverThreads.restype = ctypes.c_int
+    assert lib.b2amd_SetBatchSolverThreads(threads) == 0
+
+
+@pytest.mark.parametrize("threads,hi,seed", [(0, 300, 777), (256, 1024, 778), (512, 1024, 779), (1024, 1024, 780)],
+                         ids=["tuned_8-300", "wg256_8-1024", "wg512_8-1024", "wg1024_8-1024"])
+def test_batch_matches_single_worlds(gpu_scenes, threads, hi, seed):
+    """48 uneven worlds (8 to 300 or 1024 bodies) as one batch, with the persistent solver's
+    workgroup size tuned over the first three steps or forced to 256, 512 or 1024 threads."""
     import box2d_amd
     from box2d_amd.scenes_api import SceneLib
 
-    specs = box2d_amd.batch_world_specs(48, lo=8, hi=300, seed=777)
+    specs = box2d_amd.batch_world_specs(48, lo=8, hi=hi, seed=seed)
     batch = [gpu_scenes.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
     single = [gpu_scenes.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
     ref = None
@@ -32,10 +44,14 @@ def test_batch_matches_single_worlds(gpu_scenes):
         ref = SceneLib(REF_SCENES)
         ref_handles = [ref.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
     ids = (box2d_amd.WorldId * len(batch))(*[gpu_scenes.world_id(h) for h in batch])
+    _set_batch_threads(threads)
     try:
-        print(f"[gpu] batch of {len(batch)} worlds, bodies {[gpu_scenes.body_count(h) for h in batch]}", flush=True)
+        print(f"[gpu] batch of {len(batch)} worlds (threads {threads}), bodies {[gpu_scenes.body_count(h) for h in batch]}",
+              flush=True)
         for step in range(1, 91):
             stats = box2d_amd.step_worlds(ids)
+            if threads != 0:
+                assert stats.solveThreads == threads
             for h in single:
                 gpu_scenes.step(h)
             for h in ref_handles:
@@ -50,10 +66,44 @@ def test_batch_matches_single_worlds(gpu_scenes):
         assert stats.worlds == len(batch)
         assert stats.solveThreads in (256, 512, 1024)
     finally:
+        _set_batch_threads(0)
         for h in batch + single:
             gpu_scenes.destroy(h)
         for h in ref_handles:
             ref.destroy(h)
+
+
+def test_batch_with_worlds_beyond_the_persistent_solver(gpu_scenes, world_runs):
+    """Two 100-row pyramids (5050 bodies each) and a 105-row pyramid (5565 awake bodies, more than
+    the persistent solver's 5400 LDS slots) batched with small worlds: the large world's substep
+    loop runs on its own after the batch's, and every world still ends as if stepped alone (the
+    100-row pyramid also as the reference's)."""
+    import box2d_amd
+
+    fx = next(s for s in world_runs["scenes"] if s["name"] == "pyramid100")
+    specs = box2d_amd.batch_world_specs(6, lo=8, hi=300, seed=901)
+    batch = [gpu_scenes.pyramid(100, fx["fparams"][0], sleep=False), gpu_scenes.pyramid(105, 40.0, sleep=False)]
+    batch += [gpu_scenes.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
+    single = [gpu_scenes.pyramid(105, 40.0, sleep=False)] + [gpu_scenes.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
+    ids = (box2d_amd.WorldId * len(batch))(*[gpu_scenes.world_id(h) for h in batch])
+    try:
+        assert gpu_scenes.body_count(batch[1]) > 5400
+        print(f"[gpu] batch with oversized worlds, bodies {[gpu_scenes.body_count(h) for h in batch]}", flush=True)
+        for step in range(1, 41):
+            stats = box2d_amd.step_worlds(ids)
+            for h in single:
+                gpu_scenes.step(h)
+            want = fx["hashes"].get(str(step))
+            if want is not None:
+                assert gpu_scenes.hash(batch[0]) == want, f"100-row pyramid differs from the reference at step {step}"
+            if step in CHECK or step == 40:
+                for i, (hb, hs) in enumerate(zip(batch[1:], single)):
+                    a, b = gpu_scenes.hash(hb), gpu_scenes.hash(hs)
+                    assert a == b, f"world {i + 1}: batch {a:#x} != single {b:#x} at step {step}"
+        assert stats.worlds == len(batch)
+    finally:
+        for h in batch + single:
+            gpu_scenes.destroy(h)
 
 
 

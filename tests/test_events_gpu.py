@@ -38,7 +38,7 @@ def check_step(fx, step, ev, transform_hash):
         assert np.array_equal(ev.reshape(-1), np.array(rec, dtype=np.uint32))
 
 
-@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("path", [1, 2, 3])
 @pytest.mark.parametrize("name", ["events_60", "events_150"])
 def test_contact_events_match_reference(gpu_scenes, world_runs, name, path):
     import box2d_amd

@@ -90,7 +90,15 @@ def test_scene_matches_reference_color_kernels(gpu_scenes, world_runs, name):
     run_and_compare(gpu_scenes, scene(world_runs, name), path=1)
 
 
-@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("name", ["hello", "pyramid20", "pyramid100", "pyramid20_sleep", "falling_hinges_trace"])
+def test_scene_matches_reference_xcd(gpu_scenes, world_runs, name):
+    """XCD solver forced: the substep loop in one launch on one XCD's compute units, colors
+    separated by barriers among its workgroups."""
+    print(f"[gpu] scene {name} (xcd solver)", flush=True)
+    run_and_compare(gpu_scenes, scene(world_runs, name), path=3)
+
+
+@pytest.mark.parametrize("path", [0, 1, 3])
 @pytest.mark.parametrize("name", ["mixed_pile_1", "mixed_pile_2", "mixed_pile_3", "falling_hinges_trace"])
 def test_continuous_scenes_match_reference(gpu_scenes, world_runs, name, path):
     """Circles, capsules, hulls and rounded boxes dropped from height (fast bodies take the
@@ -99,7 +107,7 @@ def test_continuous_scenes_match_reference(gpu_scenes, world_runs, name, path):
     run_and_compare(gpu_scenes, scene(world_runs, name), path=path)
 
 
-@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("path", [1, 2, 3])
 @pytest.mark.parametrize("name", ["joint_zoo", "joint_zoo_sleep"])
 def test_joint_types_match_reference(gpu_scenes, world_runs, name, path):
     """Weld, prismatic, wheel, distance, motor, filter and revolute joints with their springs,
@@ -109,7 +117,7 @@ def test_joint_types_match_reference(gpu_scenes, world_runs, name, path):
     run_and_compare(gpu_scenes, scene(world_runs, name), path=path)
 
 
-@pytest.mark.parametrize("path", [0, 1, 2])
+@pytest.mark.parametrize("path", [0, 1, 2, 3])
 def test_rain_small_matches_reference(gpu_scenes, world_runs, path):
     """2000 mixed shapes raining into a static U-shaped chain container (one-sided chain segments
     with ghost vertices, GJK simplex caches, continuous collision against the chain)."""
@@ -186,6 +194,30 @@ def test_edits_between_steps_match_reference(gpu_scenes, world_runs, name, path)
 
     fx = next(e for e in world_runs["edits"] if e["name"] == name)
     print(f"[gpu] edits scene {name} path {path}", flush=True)
+    h = gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"])
+    box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
+    try:
+        for step in range(1, fx["steps"] + 1):
+            gpu_scenes.step(h)
+            got = [gpu_scenes.hash(h), gpu_scenes.probe_hash(h), gpu_scenes.body_count(h)]
+            assert got == fx["per_step"][step - 1], f"{name} step {step}: [transforms, getters, bodies] {got}, reference {fx['per_step'][step - 1]}"
+    finally:
+        gpu_scenes.destroy(h)
+
+
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("name", ["setters", "setters_sleep"])
+def test_setters_and_callbacks_match_reference(gpu_scenes, world_runs, name, path):
+    """Setters and user callbacks between steps: damping, gravity scale, shape materials,
+    density and filters, body types (with joints attached), disable/enable, shape destruction,
+    sleep settings, awake changes, the friction/restitution callbacks (set, swapped, cleared)
+    evaluated on the host, the custom pair filter and the restitution and hit-event thresholds.
+    Transforms and getter results (contact data included) equal the reference's after every
+    step."""
+    import box2d_amd
+
+    fx = next(e for e in world_runs["edits"] if e["name"] == name)
+    print(f"[gpu] setters scene {name} path {path}", flush=True)
     h = gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"])
     box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
     try:
