@@ -157,6 +157,16 @@ b2BodyEvents b2World_GetBodyEvents( b2WorldId worldId )
 	return world->GetBodyEvents();
 }
 
+b2JointEvents b2World_GetJointEvents( b2WorldId worldId )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr || world->locked )
+	{
+		return b2JointEvents{};
+	}
+	return b2JointEvents{ world->jointEvents.data(), (int)world->jointEvents.size() };
+}
+
 b2ContactEvents b2World_GetContactEvents( b2WorldId worldId )
 {
 	World* world = GetWorld( worldId );
@@ -259,6 +269,16 @@ int b2World_GetAwakeBodyCount( b2WorldId worldId )
 	return world != nullptr ? (int)world->solverSets[kAwakeSet].bodyIds.size() : 0;
 }
 
+uint64_t b2World_GetStateHash( b2WorldId worldId )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr || world->locked )
+	{
+		return 0;
+	}
+	return world->StateHash();
+}
+
 b2Profile b2World_GetProfile( b2WorldId worldId )
 {
 	World* world = GetWorld( worldId );
@@ -333,6 +353,19 @@ void b2World_SetCustomFilterCallback( b2WorldId worldId, b2CustomFilterFcn* fcn,
 
 // reference src/physics_world.c:1950-1984. A change of callback takes effect for each contact at
 // its next full narrowphase update, as in the reference, where the callback runs in that update.
+void b2World_SetPreSolveCallback( b2WorldId worldId, b2PreSolveFcn* preSolveFcn, b2PreContinuousFcn* preContinuousFcn,
+								  void* context )
+{
+	World* world = GetWorld( worldId );
+	if ( world == nullptr || world->locked )
+	{
+		return;
+	}
+	world->preSolveFcn = preSolveFcn;
+	world->preContinuousFcn = preContinuousFcn;
+	world->preSolveContext = context;
+}
+
 void b2World_SetFrictionCallback( b2WorldId worldId, b2FrictionCallback* callback )
 {
 	World* world = GetWorld( worldId );
@@ -1115,6 +1148,23 @@ bool b2Shape_AreSensorEventsEnabled( b2ShapeId shapeId )
 	return world != nullptr && shape->enableSensorEvents;
 }
 
+void b2Shape_EnablePreSolveEvents( b2ShapeId shapeId, bool flag )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	if ( world != nullptr && world->locked == false )
+	{
+		shape->enablePreSolveEvents = flag;
+	}
+}
+
+bool b2Shape_ArePreSolveEventsEnabled( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	return world != nullptr && shape->enablePreSolveEvents;
+}
+
 // Materials, density, filter and user data (reference src/shape.c:1190-1316, 1367-1391)
 static World* UnlockedShape( b2ShapeId shapeId, Shape** shape )
 {
@@ -1539,6 +1589,53 @@ static JointSim* JointSimOf( b2JointId jointId, World** worldOut, bool edit, int
 		*worldOut = world;
 	}
 	return &world->jointSims[joint->jointId];
+}
+
+void b2Joint_SetUserData( b2JointId jointId, void* userData )
+{
+	Joint* joint;
+	if ( WorldFromJoint( jointId, &joint ) != nullptr )
+	{
+		joint->userData = userData;
+	}
+}
+
+void* b2Joint_GetUserData( b2JointId jointId )
+{
+	Joint* joint;
+	return WorldFromJoint( jointId, &joint ) != nullptr ? joint->userData : nullptr;
+}
+
+// Event thresholds (reference src/joint.c:1369-1404); a finite threshold turns on the step's
+// joint-event gather.
+void b2Joint_SetForceThreshold( b2JointId jointId, float threshold )
+{
+	JointSim* js = JointSimOf( jointId, nullptr, true );
+	if ( js != nullptr )
+	{
+		js->forceThreshold = threshold;
+	}
+}
+
+float b2Joint_GetForceThreshold( b2JointId jointId )
+{
+	JointSim* js = JointSimOf( jointId, nullptr, false );
+	return js != nullptr ? js->forceThreshold : 0.0f;
+}
+
+void b2Joint_SetTorqueThreshold( b2JointId jointId, float threshold )
+{
+	JointSim* js = JointSimOf( jointId, nullptr, true );
+	if ( js != nullptr )
+	{
+		js->torqueThreshold = threshold;
+	}
+}
+
+float b2Joint_GetTorqueThreshold( b2JointId jointId )
+{
+	JointSim* js = JointSimOf( jointId, nullptr, false );
+	return js != nullptr ? js->torqueThreshold : 0.0f;
 }
 
 b2Transform b2Joint_GetLocalFrameA( b2JointId jointId )

@@ -70,6 +70,8 @@ struct CollideResult
 	// 3 stopped touching), ascending
 	std::vector<int> changes;
 	int recycledCount = 0;
+	// contacts left to the host's pre-solve callback (ascending id), untouched by the device
+	std::vector<int> deferred;
 };
 
 // Results the host needs after body finalization.
@@ -92,6 +94,7 @@ struct FinalizeResult
 	std::vector<b2AABB> enlargedShapeAABBs;
 	std::vector<float> sleepTimes; // per awake index (only when sleep is enabled)
 	int fastBodyCount = 0;
+	std::vector<int> jointEvents; // ids of joints that reached a force or torque threshold, ascending
 };
 
 // Device results of a launched solve that the next step's speculative pair query reads: the
@@ -120,6 +123,11 @@ public:
 	void UploadBodies( const World& world );
 	void UploadShapes( const World& world );
 	void UploadJoints( const World& world );
+	// The narrowphase of contacts the device deferred for the pre-solve callback, run on the
+	// host copies (bodies, shapes and contacts must be current there) with the world's callback;
+	// the updated contacts go back to the device and their change records are appended.
+	void CollideOnHost( World& world, const std::vector<int>& contactIds, const StepParams& params,
+						std::vector<int>* changes );
 	// Sorted (static shape << 32 | fast shape) keys the custom filter rejects for continuous collision.
 	void UploadContinuousFilterRejects( const std::vector<uint64_t>& keys );
 	void UploadContactInits( const std::vector<ContactInit>& inits, int contactCapacity );

@@ -105,9 +105,45 @@ struct CollideArgs
 	const BodySim* sims;
 	const BodyAux* aux;
 	int* changes;
-	int* counters; // [0] change count, [1] recycled count
+	int* counters; // [0] change count, [1] recycled count, [2] deferred count
 	float recycleDistance;
 	int customMixing; // StepParams::customMixing
+	int preSolve;	  // the world has a pre-solve callback: contacts that enable it go to the host
+	int* deferred;	  // their ids
+};
+
+// Where the device narrowphase reports: change records, recycled contacts, and contacts whose
+// pre-solve callback must run on the host (they are left untouched and listed).
+struct DeviceCollideSink
+{
+	const CollideArgs& a;
+
+	__device__ void Change( int code ) const
+	{
+		int slot = atomicAdd( a.counters + 0, 1 );
+		a.changes[slot] = code;
+	}
+	__device__ void Recycled() const
+	{
+		atomicAdd( a.counters + 1, 1 );
+	}
+	__device__ bool CheckRecycle() const
+	{
+		return true;
+	}
+	__device__ bool Defer( int contactId, uint32_t simFlags ) const
+	{
+		if ( a.preSolve != 0 && ( simFlags & kSimEnablePreSolveEvents ) != 0 )
+		{
+			int slot = atomicAdd( a.counters + 2, 1 );
+			a.deferred[slot] = contactId;
+			return true;
+		}
+		return false;
+	}
+	__device__ void PreSolve( const ContactSim&, b2Manifold* ) const
+	{
+	}
 };
 
 B2G_DEV float ShapeRadius( const ShapeDev& s )
@@ -126,8 +162,11 @@ B2G_DEV float ShapeRadius( const ShapeDev& s )
 }
 
 // Narrowphase update for one contact (reference src/physics_world.c:451-609 and
-// src/contact.c:519-698).
-B2G_DEV void CollideBody( const CollideArgs& a, int k )
+// src/contact.c:519-698). The sink receives the change records and runs the pre-solve callback;
+// the device sink instead defers contacts that need it, before anything is written, and the host
+// runs this same function for them with a sink that calls the callback.
+template <typename Sink>
+B2G_DEV void CollideBody( const CollideArgs& a, int k, const Sink& sink )
 {
 	if ( k >= a.count )
 	{
@@ -144,8 +183,7 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 	{
 		cs->simFlags |= kSimDisjoint;
 		cs->simFlags &= ~kSimTouching;
-		int slot = atomicAdd( a.counters + 0, 1 );
-		a.changes[slot] = contactId * 4 + 1;
+		sink.Change( contactId * 4 + 1 );
 		return;
 	}
 
@@ -166,7 +204,8 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 
 	bool isFast = ( auxA.flags & kIsFast ) || ( auxB.flags & kIsFast );
 
-	if ( isFast == false && recycleDistance > 0.0f && ( simFlags & kSimRelativeTransformValid ) && ( simFlags & kContactRecycle ) )
+	if ( sink.CheckRecycle() && isFast == false && recycleDistance > 0.0f && ( simFlags & kSimRelativeTransformValid ) &&
+		 ( simFlags & kContactRecycle ) )
 	{
 		b2Rot cachedQA = cs->cachedRotationA;
 		b2Rot cachedQB = cs->cachedRotationB;
@@ -203,9 +242,14 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 			}
 			cs->stepInfo = 1u | ( 1u << 2 ) | ( 2u << 4 );
 
-			atomicAdd( a.counters + 1, 1 );
+			sink.Recycled();
 			return;
 		}
+	}
+
+	if ( sink.Defer( contactId, simFlags ) )
+	{
+		return;
 	}
 
 	cs->cachedRotationA = transformA.q;
@@ -249,8 +293,7 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 	if ( a.customMixing != 0 && ( simFlags & kSimMixStale ) != 0 )
 	{
 		simFlags &= ~kSimMixStale;
-		int slot = atomicAdd( a.counters + 0, 1 );
-		a.changes[slot] = contactId * 4 + 0;
+		sink.Change( contactId * 4 + 0 );
 	}
 
 	if ( shapeA.rollingResistance > 0.0f || shapeB.rollingResistance > 0.0f )
@@ -269,6 +312,14 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 
 	int pointCount = m.pointCount;
 	bool touching = pointCount > 0;
+
+	// pre-solve callback (reference src/contact.c:573-583): it may edit the manifold
+	if ( touching && ( simFlags & kSimEnablePreSolveEvents ) != 0 )
+	{
+		sink.PreSolve( *cs, &m );
+		pointCount = m.pointCount;
+		touching = pointCount > 0;
+	}
 
 	if ( touching && ( ( shapeA.flags | shapeB.flags ) & kShapeEnableHitEvents ) )
 	{
@@ -333,13 +384,11 @@ B2G_DEV void CollideBody( const CollideArgs& a, int k )
 
 	if ( touching == true && wasTouching == false )
 	{
-		int slot = atomicAdd( a.counters + 0, 1 );
-		a.changes[slot] = contactId * 4 + 2;
+		sink.Change( contactId * 4 + 2 );
 	}
 	else if ( touching == false && wasTouching == true )
 	{
-		int slot = atomicAdd( a.counters + 0, 1 );
-		a.changes[slot] = contactId * 4 + 3;
+		sink.Change( contactId * 4 + 3 );
 	}
 }
 
@@ -834,6 +883,12 @@ B2G_DEV void ColorLane( const ColorArgs& a, const ColorRange& r, int t, const Bo
 		else if ( Stage == kStageSolve )
 		{
 			SolveJoint( joint, B, true, a.p.h, a.p.inv_h );
+			// joint events: colored joints only, as the reference's overflow pass does not test
+			if ( ( joint->forceThreshold < FLT_MAX || joint->torqueThreshold < FLT_MAX ) && joint->eventFlag == 0 &&
+				 JointReactionReached( joint, a.p.inv_h ) )
+			{
+				joint->eventFlag = 1;
+			}
 		}
 		else if ( Stage == kStageRelax )
 		{
@@ -2294,7 +2349,7 @@ struct CollideOp
 	using Args = CollideArgs;
 	static B2G_DEV void Run( const Args& a, int i )
 	{
-		CollideBody( a, i );
+		CollideBody( a, i, DeviceCollideSink{ a } );
 	}
 };
 
@@ -2452,23 +2507,31 @@ struct PackCollideArgs
 	int* counters;
 	const int* changes;
 	int* mail;
+	int deferredOffset; // changes and mailbox offset of the deferred contact ids (2 per collided contact)
 };
 
 B2G_DEV void PackCollideBody( const PackCollideArgs& a )
 {
 	int count = a.counters[0];
 	int recycled = a.counters[1];
+	int deferred = a.counters[2];
 	for ( int i = threadIdx.x; i < count; i += blockDim.x )
 	{
 		a.mail[kMailHeader + i] = a.changes[i];
+	}
+	for ( int i = threadIdx.x; i < deferred; i += blockDim.x )
+	{
+		a.mail[kMailHeader + a.deferredOffset + i] = a.changes[a.deferredOffset + i];
 	}
 	__syncthreads();
 	if ( threadIdx.x == 0 )
 	{
 		a.mail[0] = count;
 		a.mail[1] = recycled;
+		a.mail[2] = deferred;
 		a.counters[0] = 0;
 		a.counters[1] = 0;
+		a.counters[2] = 0;
 	}
 }
 
@@ -2504,6 +2567,9 @@ struct PackFinalizeArgs
 	const float* sleepTimes; // null when not needed
 	int* keptShapeCount;	 // receives the enlarged-shape count before it is cleared (may be null)
 	const int* xcdSync;		 // XCD solver barrier words of this step (null when it did not run)
+	JointSim* joints;		 // joint events: flagged joints are listed and their flags cleared
+	int jointCount;			 // 0 when no joint has a finite threshold
+	int jointEventOffset;	 // mailbox offset in ints
 	int bodyCount;
 	int shapeOffset; // mailbox offsets in ints
 	int aabbOffset;
@@ -2537,6 +2603,23 @@ B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
 		for ( int i = threadIdx.x; i < a.bodyCount; i += blockDim.x )
 		{
 			dst[i] = a.sleepTimes[i];
+		}
+	}
+
+	// Joint events (reference src/solver.c:1648-1700), listed in any order (the host sorts them)
+	__shared__ int jointEvents;
+	if ( threadIdx.x == 0 )
+	{
+		jointEvents = 0;
+	}
+	__syncthreads();
+	for ( int i = threadIdx.x; i < a.jointCount; i += blockDim.x )
+	{
+		if ( a.joints[i].eventFlag != 0 )
+		{
+			a.joints[i].eventFlag = 0;
+			int slot = atomicAdd( &jointEvents, 1 );
+			a.mail[a.jointEventOffset + slot] = i;
 		}
 	}
 
@@ -2596,6 +2679,7 @@ B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
 			*a.keptShapeCount = shapes;
 		}
 		a.mail[5] = a.xcdSync != nullptr ? a.xcdSync[2] : 0;
+		a.mail[6] = jointEvents;
 		a.counters[0] = 0;
 		a.counters[1] = 0;
 		a.counters[2] = 0;
@@ -3274,8 +3358,9 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 	impl->enlargedBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->fastBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->sleepTimes.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
-	// at most two entries per contact: a touch change and a mixing refresh
-	impl->changes.Reserve( std::max<size_t>( 2 * lists.collideContacts.size(), 1 ), false, impl->stream, 0 );
+	// at most two entries per contact (a touch change and a mixing refresh), then the contacts
+	// deferred to the host's pre-solve callback
+	impl->changes.Reserve( std::max<size_t>( 3 * lists.collideContacts.size(), 1 ), false, impl->stream, 0 );
 	if ( impl->recording == false )
 	{
 		HIP_CHECK( hipStreamSynchronize( impl->stream ) );
@@ -3292,7 +3377,7 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 	}
 
 	hipStream_t s = impl->stream;
-	impl->EnsureMail( kMailHeader + 2 * (size_t)n );
+	impl->EnsureMail( kMailHeader + 3 * (size_t)n );
 
 	CollideArgs a;
 	a.list = impl->collideList.ptr;
@@ -3305,8 +3390,10 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 	a.counters = impl->counters;
 	a.recycleDistance = params.contactRecycleDistance;
 	a.customMixing = params.customMixing;
+	a.preSolve = params.preSolve;
+	a.deferred = impl->changes.ptr + 2 * n;
 
-	PackCollideArgs pc = { impl->counters, impl->changes.ptr, impl->mail };
+	PackCollideArgs pc = { impl->counters, impl->changes.ptr, impl->mail, 2 * n };
 
 	if ( impl->recording )
 	{
@@ -3329,6 +3416,7 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 void DeviceWorld::CollideFinish( CollideResult* result )
 {
 	result->changes.clear();
+	result->deferred.clear();
 	result->recycledCount = 0;
 
 	int n = impl->collideCount;
@@ -3356,11 +3444,73 @@ void DeviceWorld::CollideFinish( CollideResult* result )
 		result->changes.assign( impl->mail + kMailHeader, impl->mail + kMailHeader + changeCount );
 		std::sort( result->changes.begin(), result->changes.end() );
 	}
+	int deferredCount = impl->mail[2];
+	if ( deferredCount < 0 || deferredCount > n )
+	{
+		Fatal( "pre-solve contact count %d exceeds the collided contact count %d", deferredCount, n );
+	}
+	result->deferred.assign( impl->mail + kMailHeader + 2 * n, impl->mail + kMailHeader + 2 * n + deferredCount );
+	std::sort( result->deferred.begin(), result->deferred.end() );
 
 	if ( impl->recording == false )
 	{
 		HIP_CHECK( hipEventElapsedTime( &lastCollideMs, impl->ev[0], impl->ev[1] ) );
 	}
+}
+
+// Host sink of the narrowphase: change records into a vector and the world's pre-solve callback
+// on the new manifold, before anchors move to the centers of mass (reference src/contact.c:573-583).
+struct HostCollideSink
+{
+	std::vector<int>* changes;
+	const World* world;
+
+	void Change( int code ) const
+	{
+		changes->push_back( code );
+	}
+	void Recycled() const
+	{
+	}
+	bool CheckRecycle() const
+	{
+		// the device found these contacts not recyclable before deferring them
+		return false;
+	}
+	bool Defer( int, uint32_t ) const
+	{
+		return false;
+	}
+	void PreSolve( const ContactSim& cs, b2Manifold* manifold ) const
+	{
+		world->preSolveFcn( world->ShapeIdOf( cs.shapeIdA ), world->ShapeIdOf( cs.shapeIdB ), manifold, world->preSolveContext );
+	}
+};
+
+void DeviceWorld::CollideOnHost( World& world, const std::vector<int>& contactIds, const StepParams& params,
+								 std::vector<int>* changes )
+{
+	CollideArgs a = {};
+	a.list = contactIds.data();
+	a.count = (int)contactIds.size();
+	a.contacts = world.contactSims.data();
+	a.shapes = world.shapeData.data();
+	a.sims = world.bodySims.data();
+	a.aux = world.bodyAux.data();
+	a.recycleDistance = params.contactRecycleDistance;
+	a.customMixing = params.customMixing;
+	a.preSolve = 1;
+	HostCollideSink sink{ changes, &world };
+	for ( int k = 0; k < a.count; ++k )
+	{
+		CollideBody( a, k, sink );
+	}
+	for ( int contactId : contactIds )
+	{
+		HIP_CHECK( hipMemcpyAsync( impl->contacts.ptr + contactId, world.contactSims.data() + contactId, sizeof( ContactSim ),
+								   hipMemcpyHostToDevice, impl->stream ) );
+	}
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
 void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
@@ -3981,6 +4131,9 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		pk.aabbOffset = pk.shapeOffset + (int)shapeCap;
 		pk.sleepOffset = pk.aabbOffset + 4 * (int)shapeCap;
 		pk.hitOffset = pk.sleepOffset + nb;
+		pk.joints = impl->joints.ptr;
+		pk.jointCount = params.jointEvents != 0 ? impl->jointCount : 0;
+		pk.jointEventOffset = pk.hitOffset + kHitRecordInts * std::max( nc, 0 );
 		impl->solveHitCapacity = nc;
 		impl->hitCandidates.Reserve( std::max<size_t>( nc, 1 ), false, s, 0 );
 		pk.contacts = impl->contacts.ptr;
@@ -3989,7 +4142,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		pk.hitCandidates = impl->hitCandidates.ptr;
 		pk.hitCount = impl->counters + 8;
 		pk.hitThreshold = params.hitEventThreshold;
-		impl->EnsureMail( (size_t)pk.hitOffset + (size_t)kHitRecordInts * std::max( nc, 0 ) );
+		impl->EnsureMail( (size_t)pk.jointEventOffset + (size_t)pk.jointCount );
 		pk.mail = impl->mail;
 		impl->solveBodyCount = nb;
 		impl->solveNeedSleepTimes = needSleepTimes;
@@ -4100,6 +4253,15 @@ void DeviceWorld::SolveFinish( FinalizeResult* result )
 		}
 		// the reference reports hits in ascending contact id
 		std::sort( result->hits.begin(), result->hits.end(), []( const HitRecord& x, const HitRecord& y ) { return x.contactId < y.contactId; } );
+
+		// and joint events in ascending joint id
+		int jointEventCount = mail[6];
+		if ( jointEventCount < 0 || jointEventCount > pk.jointCount )
+		{
+			Fatal( "joint event count %d exceeds the joint count %d", jointEventCount, pk.jointCount );
+		}
+		result->jointEvents.assign( mail + pk.jointEventOffset, mail + pk.jointEventOffset + jointEventCount );
+		std::sort( result->jointEvents.begin(), result->jointEvents.end() );
 	}
 
 	if ( ownStream )

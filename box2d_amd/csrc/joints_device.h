@@ -1248,6 +1248,61 @@ B2G_DEVI void SolvePogo( JointSim* joint, const Bodies& B, bool useBias, float h
 }
 
 // ---------------------------------------------------------------------------------------------
+// Joint events (reference src/joint.c:993-1072 and the threshold test at src/joint.c:1663-1675):
+// the constraint force and torque after a biased solve, compared with the joint's thresholds.
+
+B2G_DEVI bool JointReactionReached( const JointSim* j, float inv_h )
+{
+	float linearImpulse = 0.0f;
+	float angularImpulse = 0.0f;
+	switch ( j->type )
+	{
+		case b2_distanceJoint:
+			linearImpulse = b2AbsFloat( j->distanceImpulse + j->lowerImpulse - j->upperImpulse + j->motorImpulse );
+			break;
+		case b2_motorJoint:
+			linearImpulse = b2Length( b2Add( j->linearVelocityImpulse, j->linearSpringImpulse ) );
+			angularImpulse = b2AbsFloat( j->angularVelocityImpulse + j->angularSpringImpulse );
+			break;
+		case b2_moverJoint:
+			linearImpulse = b2Length( j->linearVelocityImpulse );
+			break;
+		case b2_pogoJoint:
+			linearImpulse = b2AbsFloat( j->pogoImpulse );
+			break;
+		case b2_prismaticJoint:
+		{
+			float perp = j->pointImpulse.x;
+			float axial = j->motorImpulse + j->lowerImpulse - j->upperImpulse;
+			linearImpulse = sqrtf( perp * perp + axial * axial );
+			angularImpulse = b2AbsFloat( j->pointImpulse.y );
+			break;
+		}
+		case b2_revoluteJoint:
+			linearImpulse = b2Length( j->linearImpulse );
+			angularImpulse = b2AbsFloat( j->motorImpulse + j->lowerImpulse - j->upperImpulse );
+			break;
+		case b2_weldJoint:
+			linearImpulse = b2Length( j->linearImpulse );
+			angularImpulse = b2AbsFloat( j->angularImpulse );
+			break;
+		case b2_wheelJoint:
+		{
+			float perp = j->perpImpulse;
+			float axial = j->springImpulse + j->lowerImpulse - j->upperImpulse;
+			linearImpulse = sqrtf( perp * perp + axial * axial );
+			angularImpulse = b2AbsFloat( j->motorImpulse );
+			break;
+		}
+		default:
+			break;
+	}
+	float force = linearImpulse * inv_h;
+	float torque = angularImpulse * inv_h;
+	return force >= j->forceThreshold || torque >= j->torqueThreshold;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Dispatch (src/joint.c:1406-1532)
 
 template <typename Bodies>

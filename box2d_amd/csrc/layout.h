@@ -278,7 +278,8 @@ struct JointSim
 	float maxCompressionForce;
 	float pogoImpulse;
 	float pogoVelocity;
-	int pad[2];
+	int eventFlag; // a biased solve of this step reached the force or torque threshold
+	int pad;
 };
 
 // Contact constraint fields, one device array per field (structure of arrays).
@@ -349,6 +350,11 @@ struct StepParams
 	// user mixing callbacks: bit 0 friction, bit 1 restitution (the host evaluates them; the
 	// narrowphase keeps the contact's values instead of mixing the shapes' own)
 	int customMixing;
+	// some joint has a finite force or torque threshold: the pack kernel gathers joint events
+	int jointEvents;
+	// the world has a pre-solve callback: the narrowphase leaves contacts that enable pre-solve
+	// events to the host
+	int preSolve;
 };
 
 } // namespace b2g

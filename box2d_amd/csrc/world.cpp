@@ -469,7 +469,7 @@ void World::PullShapes()
 	}
 }
 
-const ContactSim& World::PulledContact( int contactId )
+void World::PullContacts()
 {
 	if ( contactsStale || contactSims.size() < contacts.size() )
 	{
@@ -477,6 +477,11 @@ const ContactSim& World::PulledContact( int contactId )
 		device->DownloadContacts( (int)contacts.size(), contactSims.data() );
 		contactsStale = false;
 	}
+}
+
+const ContactSim& World::PulledContact( int contactId )
+{
+	PullContacts();
 	return contactSims[contactId];
 }
 
@@ -487,6 +492,171 @@ void World::PullJoints()
 		device->DownloadJoints( *this );
 		jointsStale = false;
 	}
+}
+
+// Deep state hash (reference src/world_snapshot.c:1025-1187, FNV-1a 64 from src/recording.h:18-38):
+// body transforms, awake velocities and set indices; contact set/color/local indices and manifold
+// impulses; joint indices and accumulated impulses by type; id pool states; the solver set count.
+// The device copies are pulled first, so the hash describes the state after the last step.
+namespace
+{
+constexpr uint64_t kFnvInit = 14695981039346656037ull;
+constexpr uint64_t kFnvPrime = 1099511628211ull;
+
+uint64_t MixWord( uint64_t hash, uint32_t word )
+{
+	return ( hash ^ (uint64_t)word ) * kFnvPrime;
+}
+
+uint64_t MixFloat( uint64_t hash, float f )
+{
+	uint32_t word;
+	memcpy( &word, &f, 4 );
+	return MixWord( hash, word );
+}
+
+uint64_t MixInt( uint64_t hash, int v )
+{
+	return MixWord( hash, (uint32_t)v );
+}
+
+uint64_t MixVec2Bytes( uint64_t hash, b2Vec2 v )
+{
+	// the reference mixes a b2Vec2 byte by byte
+	unsigned char bytes[8];
+	memcpy( bytes, &v, 8 );
+	for ( unsigned char b : bytes )
+	{
+		hash = ( hash ^ (uint64_t)b ) * kFnvPrime;
+	}
+	return hash;
+}
+
+uint64_t MixJointImpulses( uint64_t hash, const JointSim& j )
+{
+	switch ( j.type )
+	{
+		case b2_distanceJoint:
+			hash = MixFloat( hash, j.distanceImpulse );
+			hash = MixFloat( hash, j.lowerImpulse );
+			hash = MixFloat( hash, j.upperImpulse );
+			return MixFloat( hash, j.motorImpulse );
+		case b2_moverJoint:
+			hash = MixFloat( hash, j.linearVelocityImpulse.x );
+			return MixFloat( hash, j.linearVelocityImpulse.y );
+		case b2_motorJoint:
+			hash = MixFloat( hash, j.linearVelocityImpulse.x );
+			hash = MixFloat( hash, j.linearVelocityImpulse.y );
+			hash = MixFloat( hash, j.angularVelocityImpulse );
+			hash = MixFloat( hash, j.linearSpringImpulse.x );
+			hash = MixFloat( hash, j.linearSpringImpulse.y );
+			return MixFloat( hash, j.angularSpringImpulse );
+		case b2_pogoJoint:
+			hash = MixFloat( hash, j.pogoImpulse );
+			return MixFloat( hash, j.pogoVelocity );
+		case b2_prismaticJoint:
+			hash = MixVec2Bytes( hash, j.pointImpulse );
+			hash = MixFloat( hash, j.springImpulse );
+			hash = MixFloat( hash, j.motorImpulse );
+			hash = MixFloat( hash, j.lowerImpulse );
+			return MixFloat( hash, j.upperImpulse );
+		case b2_revoluteJoint:
+			hash = MixVec2Bytes( hash, j.linearImpulse );
+			hash = MixFloat( hash, j.springImpulse );
+			hash = MixFloat( hash, j.motorImpulse );
+			hash = MixFloat( hash, j.lowerImpulse );
+			return MixFloat( hash, j.upperImpulse );
+		case b2_weldJoint:
+			hash = MixVec2Bytes( hash, j.linearImpulse );
+			return MixFloat( hash, j.angularImpulse );
+		case b2_wheelJoint:
+			hash = MixFloat( hash, j.perpImpulse );
+			hash = MixFloat( hash, j.motorImpulse );
+			hash = MixFloat( hash, j.springImpulse );
+			hash = MixFloat( hash, j.lowerImpulse );
+			return MixFloat( hash, j.upperImpulse );
+		default:
+			return hash;
+	}
+}
+
+uint64_t MixPool( uint64_t hash, const IdPool& pool )
+{
+	hash = MixInt( hash, pool.nextIndex );
+	return MixInt( hash, pool.Count() );
+}
+} // namespace
+
+uint64_t World::StateHash()
+{
+	PullBodies();
+	PullJoints();
+	uint64_t hash = kFnvInit;
+
+	for ( int i = 0; i < (int)bodies.size(); ++i )
+	{
+		const Body& body = bodies[i];
+		if ( body.id != i )
+		{
+			continue;
+		}
+		const BodySim& sim = bodySims[i];
+		hash = MixFloat( hash, sim.transform.p.x );
+		hash = MixFloat( hash, sim.transform.p.y );
+		hash = MixFloat( hash, sim.transform.q.c );
+		hash = MixFloat( hash, sim.transform.q.s );
+		if ( body.setIndex == kAwakeSet )
+		{
+			const BodyState& state = bodyStates[i];
+			hash = MixFloat( hash, state.linearVelocity.x );
+			hash = MixFloat( hash, state.linearVelocity.y );
+			hash = MixFloat( hash, state.angularVelocity );
+		}
+		hash = MixInt( hash, body.setIndex );
+		hash = MixInt( hash, body.localIndex );
+	}
+
+	for ( int i = 0; i < (int)contacts.size(); ++i )
+	{
+		const Contact& contact = contacts[i];
+		if ( contact.contactId != i )
+		{
+			continue;
+		}
+		hash = MixInt( hash, contact.setIndex );
+		hash = MixInt( hash, contact.colorIndex );
+		hash = MixInt( hash, contact.localIndex );
+		const b2Manifold& m = PulledContact( i ).manifold;
+		hash = MixInt( hash, m.pointCount );
+		for ( int p = 0; p < m.pointCount; ++p )
+		{
+			hash = MixFloat( hash, m.points[p].normalImpulse );
+			hash = MixFloat( hash, m.points[p].tangentImpulse );
+			hash = MixFloat( hash, m.points[p].totalNormalImpulse );
+		}
+	}
+
+	for ( int i = 0; i < (int)joints.size(); ++i )
+	{
+		const Joint& joint = joints[i];
+		if ( joint.jointId != i )
+		{
+			continue;
+		}
+		hash = MixInt( hash, joint.setIndex );
+		hash = MixInt( hash, joint.colorIndex );
+		hash = MixInt( hash, joint.localIndex );
+		hash = MixJointImpulses( hash, jointSims[i] );
+	}
+
+	hash = MixPool( hash, bodyIdPool );
+	hash = MixPool( hash, shapeIdPool );
+	hash = MixPool( hash, chainIdPool );
+	hash = MixPool( hash, contactIdPool );
+	hash = MixPool( hash, jointIdPool );
+	hash = MixPool( hash, islandIdPool );
+	hash = MixPool( hash, solverSetIdPool );
+	return MixInt( hash, (int)solverSets.size() );
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3991,6 +4161,7 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	moveEventsNeedTransforms = false;
 	contactBeginEvents.clear();
 	contactHitEvents.clear();
+	jointEvents.clear();
 	profile = b2Profile{};
 	recycledContactCount = 0;
 
@@ -4052,6 +4223,27 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	{
 		device->UploadJoints( *this );
 		jointsDirty = false;
+		jointEventJointCount = 0;
+		for ( const Joint& joint : joints )
+		{
+			if ( joint.jointId != B2_NULL_INDEX )
+			{
+				const JointSim& js = jointSims[joint.jointId];
+				jointEventJointCount += js.forceThreshold < FLT_MAX || js.torqueThreshold < FLT_MAX ? 1 : 0;
+			}
+		}
+	}
+	params.jointEvents = jointEventJointCount > 0 ? 1 : 0;
+	params.preSolve = preSolveFcn != nullptr ? 1 : 0;
+	if ( preContinuousFcn != nullptr && enableContinuous )
+	{
+		for ( const Shape& shape : shapes )
+		{
+			if ( shape.id != B2_NULL_INDEX && shape.enablePreSolveEvents )
+			{
+				Fatal( "the pre-continuous callback is not supported by the GPU step (continuous collision runs on the GPU)" );
+			}
+		}
 	}
 	device->UploadContactInits( pendingContactInits, contactIdPool.Capacity() );
 	pendingContactInits.clear();
@@ -4093,6 +4285,15 @@ void World::StepMiddle()
 	CollideResult cr;
 	device->CollideFinish( &cr );
 	recycledContactCount = cr.recycledCount;
+	if ( cr.deferred.empty() == false )
+	{
+		// contacts with pre-solve events: their narrowphase runs here, around the user callback
+		PullBodies();
+		PullShapes();
+		PullContacts();
+		device->CollideOnHost( *this, cr.deferred, stepParams, &cr.changes );
+		std::sort( cr.changes.begin(), cr.changes.end() );
+	}
 	uint64_t stateTicks = b2GetTicks();
 	if ( cr.changes.empty() == false )
 	{
@@ -4225,6 +4426,13 @@ void World::SolveEnd()
 	{
 		// the speculative query's queries, in the device list's slot order
 		speculative.shapes = fr.enlargedShapes;
+	}
+
+	// joint events in ascending joint id (reference src/solver.c:1648-1700)
+	for ( int jointId : fr.jointEvents )
+	{
+		const Joint& joint = joints[jointId];
+		jointEvents.push_back( b2JointEvent{ b2JointId{ jointId + 1, worldId, joint.generation }, joint.userData } );
 	}
 
 	// hit events in ascending contact id (reference src/solver.c:1703-1820)

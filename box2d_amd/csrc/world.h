@@ -417,6 +417,8 @@ public:
 	void PullJoints();
 	// Contacts' manifolds and materials as the device last left them (read-only host copy).
 	const ContactSim& PulledContact( int contactId );
+	void PullContacts();
+	uint64_t StateHash();
 	std::vector<ContactSim> contactSims;
 	bool contactsStale = true;
 
@@ -454,6 +456,9 @@ public:
 	// user callbacks, run on the host (reference src/physics_world.c:272-290, src/broad_phase.c:281-294)
 	b2CustomFilterFcn* customFilterFcn = nullptr;
 	void* customFilterContext = nullptr;
+	b2PreSolveFcn* preSolveFcn = nullptr;
+	b2PreContinuousFcn* preContinuousFcn = nullptr;
+	void* preSolveContext = nullptr;
 	b2FrictionCallback* frictionCallback = nullptr;
 	b2RestitutionCallback* restitutionCallback = nullptr;
 	// a mixing callback or a shape material changed while a callback is set: every contact's
@@ -549,6 +554,10 @@ public:
 	std::vector<b2ContactBeginTouchEvent> contactBeginEvents;
 	std::vector<b2ContactEndTouchEvent> contactEndEvents[2];
 	std::vector<b2ContactHitEvent> contactHitEvents;
+	// joints whose force or torque reached a threshold in the last step (reference
+	// src/physics_world.c:867, src/solver.c:1648-1700)
+	std::vector<b2JointEvent> jointEvents;
+	int jointEventJointCount = 0; // joints with a finite force or torque threshold at the last upload
 	int endEventArrayIndex = 0;
 	b2ContactEvents GetContactEvents();
 	b2ShapeId ShapeIdOf( int shapeId ) const

@@ -9,6 +9,8 @@
 
 #include "box2d/box2d.h"
 
+#include <float.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -36,6 +38,15 @@ typedef struct b2sScene
 	// B2S_SETTERS: the chain links and the body with two shapes
 	b2BodyId setterLinks[3];
 	b2ShapeId extraShape;
+
+	// B2S_JOINT_ZOO with event thresholds: the joints in ascending id
+	b2JointId* zooJoints;
+	int zooJointCount;
+
+	// B2S_PRESOLVE: the one-way platform and the callback's tally of the last step
+	b2ShapeId platformShape;
+	uint32_t preSolveCalls;
+	uint32_t preSolveSum;
 } b2sScene;
 
 static b2sScene s_scenes[B2S_MAX_HANDLES];
@@ -516,6 +527,56 @@ static b2BodyId b2sDynamicBox( b2sScene* scene, int* capacity, b2Vec2 position, 
 	b2CreatePolygonShape( bodyId, &shapeDef, &box );
 	b2sPushBody( scene, capacity, bodyId );
 	return bodyId;
+}
+
+static int b2sCompareJointIds( const void* a, const void* b )
+{
+	int ia = ( (const b2JointId*)a )->index1;
+	int ib = ( (const b2JointId*)b )->index1;
+	return ( ia > ib ) - ( ia < ib );
+}
+
+// Event thresholds of joint k in round r (r = 0 at creation, 1 after the mid-run change): every
+// awake step, a force level, a torque level, or never.
+static void b2sSetZooThresholds( b2JointId jointId, int k, int round )
+{
+	static const float force[4] = { 0.0f, 40.0f, FLT_MAX, FLT_MAX };
+	static const float torque[4] = { 0.0f, FLT_MAX, 2.0f, FLT_MAX };
+	int v = ( k + round ) & 3;
+	b2Joint_SetForceThreshold( jointId, force[v] );
+	b2Joint_SetTorqueThreshold( jointId, torque[v] );
+}
+
+// Collect the zoo's joints through the bodies' joint lists, give each user data k + 1 and
+// thresholds by its rank k in ascending joint id.
+static void b2sZooThresholds( b2sScene* scene )
+{
+	int capacity = 0;
+	for ( int i = 0; i < scene->bodyCount; ++i )
+	{
+		capacity += b2Body_GetJointCount( scene->bodies[i] );
+	}
+	scene->zooJoints = (b2JointId*)malloc( ( capacity > 0 ? capacity : 1 ) * sizeof( b2JointId ) );
+	int n = 0;
+	for ( int i = 0; i < scene->bodyCount; ++i )
+	{
+		n += b2Body_GetJoints( scene->bodies[i], scene->zooJoints + n, capacity - n );
+	}
+	qsort( scene->zooJoints, n, sizeof( b2JointId ), b2sCompareJointIds );
+	int unique = 0;
+	for ( int i = 0; i < n; ++i )
+	{
+		if ( unique == 0 || scene->zooJoints[unique - 1].index1 != scene->zooJoints[i].index1 )
+		{
+			scene->zooJoints[unique++] = scene->zooJoints[i];
+		}
+	}
+	scene->zooJointCount = unique;
+	for ( int k = 0; k < unique; ++k )
+	{
+		b2Joint_SetUserData( scene->zooJoints[k], (void*)(intptr_t)( k + 1 ) );
+		b2sSetZooThresholds( scene->zooJoints[k], k, 0 );
+	}
 }
 
 static void b2sBuildJointZoo( b2sScene* scene, int enableSleep )
@@ -1159,6 +1220,148 @@ static b2ShapeId b2sFirstShape( b2BodyId bodyId )
 	return shapeId;
 }
 
+static uint32_t b2sMix32( uint32_t x )
+{
+	x ^= x >> 16;
+	x *= 0x7feb352du;
+	x ^= x >> 15;
+	x *= 0x846ca68bu;
+	x ^= x >> 16;
+	return x;
+}
+
+// One-way platform: a contact with the platform keeps its points only when the other shape is
+// above it (normal within 60 degrees of up, pointing from the platform to the other shape); some
+// other pairs lose their second point. The tally is a count and an order-independent sum, since
+// the reference may call this from several threads in any order.
+static void b2sPreSolve( b2ShapeId shapeIdA, b2ShapeId shapeIdB, b2Manifold* manifold, void* context )
+{
+	b2sScene* scene = (b2sScene*)context;
+	scene->preSolveCalls += 1;
+	scene->preSolveSum += b2sMix32( (uint32_t)shapeIdA.index1 * 65599u + (uint32_t)shapeIdB.index1 ) ^ (uint32_t)manifold->pointCount;
+	bool platformA = B2_ID_EQUALS( shapeIdA, scene->platformShape );
+	bool platformB = B2_ID_EQUALS( shapeIdB, scene->platformShape );
+	if ( platformA || platformB )
+	{
+		float up = platformA ? manifold->normal.y : -manifold->normal.y;
+		if ( up < 0.5f )
+		{
+			manifold->pointCount = 0;
+		}
+	}
+	else if ( manifold->pointCount == 2 && ( shapeIdA.index1 + shapeIdB.index1 ) % 5 == 0 )
+	{
+		manifold->pointCount = 1;
+	}
+}
+
+static void b2sBuildPreSolve( b2sScene* scene, int enableSleep )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+	b2World_SetPreSolveCallback( worldId, b2sPreSolve, NULL, scene );
+	scene->bodyCapacity = 0;
+
+	b2BodyDef groundDef = b2DefaultBodyDef();
+	b2BodyId groundId = b2CreateBody( worldId, &groundDef );
+	b2ShapeDef groundShape = b2DefaultShapeDef();
+	b2Polygon groundBox = b2MakeOffsetBox( 30.0f, 1.0f, ( b2Vec2 ){ 0.0f, -1.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &groundShape, &groundBox );
+
+	b2BodyDef platformDef = b2DefaultBodyDef();
+	platformDef.position = ( b2Pos ){ 0.0f, 4.0f };
+	b2BodyId platformId = b2CreateBody( worldId, &platformDef );
+	b2ShapeDef platformShape = b2DefaultShapeDef();
+	platformShape.enablePreSolveEvents = true;
+	b2Polygon platformBox = b2MakeBox( 8.0f, 0.25f );
+	scene->platformShape = b2CreatePolygonShape( platformId, &platformShape, &platformBox );
+
+	// boxes launched up through the platform from below, then shapes dropped onto it from above
+	for ( int i = 0; i < 24; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bool launched = i < 12;
+		float x = -6.5f + 1.2f * (float)( i % 12 );
+		bodyDef.position = launched ? ( b2Pos ){ x, 1.0f + 0.3f * (float)( i % 3 ) } : ( b2Pos ){ x, 8.0f + 0.8f * (float)( i % 4 ) };
+		bodyDef.rotation = b2MakeRot( 0.11f * (float)i );
+		if ( launched )
+		{
+			bodyDef.linearVelocity = ( b2Vec2 ){ 0.3f * (float)( ( i % 5 ) - 2 ), 11.0f + 0.25f * (float)( i % 4 ) };
+		}
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.material.friction = 0.5f;
+		def.enablePreSolveEvents = ( i % 3 ) == 0;
+		if ( launched || i % 2 == 0 )
+		{
+			b2Polygon box = b2MakeBox( 0.4f, 0.3f );
+			b2CreatePolygonShape( bodyId, &def, &box );
+		}
+		else if ( i % 4 == 1 )
+		{
+			b2Circle circle = { { 0.0f, 0.0f }, 0.35f };
+			b2CreateCircleShape( bodyId, &def, &circle );
+		}
+		else
+		{
+			b2Capsule capsule = { { -0.3f, 0.0f }, { 0.3f, 0.0f }, 0.2f };
+			b2CreateCapsuleShape( bodyId, &def, &capsule );
+		}
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+}
+
+// Before step k (0-based) of B2S_PRESOLVE: pre-solve off on the platform (existing contacts keep
+// the flag), bodies launched again, the flag back on, the callback cleared and restored.
+static void b2sPreSolveBeforeStep( b2sScene* scene, int k )
+{
+	b2BodyId* b = scene->bodies;
+	scene->preSolveCalls = 0;
+	scene->preSolveSum = 0;
+	if ( k == 60 )
+	{
+		b2Shape_EnablePreSolveEvents( scene->platformShape, false );
+	}
+	if ( k == 60 || k == 110 )
+	{
+		for ( int i = 0; i < 4; ++i )
+		{
+			b2Body_SetLinearVelocity( b[3 * i + 1], ( b2Vec2 ){ 0.0f, 12.0f } );
+			b2Body_SetAwake( b[3 * i + 1], true );
+		}
+	}
+	if ( k == 90 )
+	{
+		b2Shape_EnablePreSolveEvents( scene->platformShape, true );
+	}
+	if ( k == 130 )
+	{
+		b2World_SetPreSolveCallback( scene->worldId, NULL, NULL, NULL );
+	}
+	if ( k == 150 )
+	{
+		b2World_SetPreSolveCallback( scene->worldId, b2sPreSolve, NULL, scene );
+		for ( int i = 0; i < 4; ++i )
+		{
+			b2Body_SetLinearVelocity( b[3 * i + 2], ( b2Vec2 ){ 0.5f, 13.0f } );
+			b2Body_SetAwake( b[3 * i + 2], true );
+		}
+	}
+}
+
+static void b2sPreSolveProbe( b2sScene* scene )
+{
+	uint32_t h = scene->probeHash;
+	h = b2sFnvWord( h, scene->preSolveCalls );
+	h = b2sFnvWord( h, scene->preSolveSum );
+	h = b2sFnvWord( h, (uint32_t)b2Shape_ArePreSolveEventsEnabled( scene->platformShape ) );
+	b2ContactData data[8];
+	int n = b2Shape_GetContactData( scene->platformShape, data, 8 );
+	h = b2sFnvWord( h, (uint32_t)n );
+	scene->probeHash = h;
+}
+
 static void b2sBuildSetters( b2sScene* scene, int enableSleep )
 {
 	b2WorldId worldId = scene->worldId;
@@ -1515,10 +1718,18 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 
 		case B2S_JOINT_ZOO:
 			b2sBuildJointZoo( scene, iparamCount > 0 ? iparams[0] : 0 );
+			if ( iparamCount > 1 && iparams[1] != 0 )
+			{
+				b2sZooThresholds( scene );
+			}
 			break;
 
 		case B2S_SETTERS:
 			b2sBuildSetters( scene, iparamCount > 0 ? iparams[0] : 0 );
+			break;
+
+		case B2S_PRESOLVE:
+			b2sBuildPreSolve( scene, iparamCount > 0 ? iparams[0] : 0 );
 			break;
 
 		case B2S_RAIN:
@@ -1569,6 +1780,7 @@ void b2s_destroy( int handle )
 	}
 	b2DestroyWorld( scene->worldId );
 	free( scene->bodies );
+	free( scene->zooJoints );
 	memset( scene, 0, sizeof( *scene ) );
 }
 
@@ -1589,6 +1801,17 @@ void b2s_step( int handle, float timeStep, int subStepCount, int count )
 		{
 			b2sSettersBeforeStep( scene, scene->stepIndex );
 		}
+		else if ( scene->kind == B2S_PRESOLVE )
+		{
+			b2sPreSolveBeforeStep( scene, scene->stepIndex );
+		}
+		else if ( scene->kind == B2S_JOINT_ZOO && scene->zooJointCount > 0 && scene->stepIndex == 150 )
+		{
+			for ( int k = 0; k < scene->zooJointCount; ++k )
+			{
+				b2sSetZooThresholds( scene->zooJoints[k], k, 1 );
+			}
+		}
 		b2World_Step( scene->worldId, timeStep, subStepCount );
 		scene->stepIndex += 1;
 		if ( scene->kind == B2S_EDITS )
@@ -1599,6 +1822,10 @@ void b2s_step( int handle, float timeStep, int subStepCount, int count )
 		{
 			b2sSettersProbe( scene );
 		}
+		else if ( scene->kind == B2S_PRESOLVE )
+		{
+			b2sPreSolveProbe( scene );
+		}
 	}
 }
 
@@ -1606,6 +1833,12 @@ uint32_t b2s_probe_hash( int handle )
 {
 	b2sScene* scene = b2sGet( handle );
 	return scene != NULL ? scene->probeHash : 0;
+}
+
+uint64_t b2s_state_hash( int handle )
+{
+	b2sScene* scene = b2sGet( handle );
+	return scene != NULL ? b2World_GetStateHash( scene->worldId ) : 0;
 }
 
 static double b2sNowMs( void )
@@ -1743,6 +1976,25 @@ static uint32_t b2sFloatBits( float x )
 	uint32_t u;
 	memcpy( &u, &x, sizeof( u ) );
 	return u;
+}
+
+int b2s_joint_events( int handle, uint32_t* out, int capacity )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	b2JointEvents events = b2World_GetJointEvents( scene->worldId );
+	int n = 0;
+	for ( int k = 0; k < events.count && 3 * ( n + 1 ) <= capacity; ++k, ++n )
+	{
+		const b2JointEvent* e = events.jointEvents + k;
+		out[3 * n + 0] = (uint32_t)e->jointId.index1;
+		out[3 * n + 1] = e->jointId.generation;
+		out[3 * n + 2] = (uint32_t)(intptr_t)e->userData;
+	}
+	return n;
 }
 
 int b2s_contact_events( int handle, uint32_t* out, int capacity )

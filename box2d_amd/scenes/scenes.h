@@ -65,7 +65,10 @@ enum
 	// the ground, distance ropes (rigid, spring, limits, motor), motor-joint followers, filter
 	// joints between overlapping boxes, mover and pogo joints, and revolute hinges with spring,
 	// limit and motor.
-	// iparams: [0] enableSleep. Bodies are listed in creation order.
+	// iparams: [0] enableSleep, [1] event thresholds (optional): the joints in ascending id get
+	// user data k + 1 and, by k mod 4, thresholds that report every awake step, a force of 40 N,
+	// a torque of 2 N m, or never; before step 151 every joint moves one place in that cycle.
+	// Bodies are listed in creation order.
 	B2S_JOINT_ZOO = 7,
 
 	// Contact events: circles, boxes and capsules thrown onto a 40 x 1 m ground box (half extents)
@@ -94,6 +97,16 @@ enum
 	// gravity scale, materials, filters, types, sleep settings, thresholds, contact data of a body,
 	// a shape and a contact) into b2s_probe_hash. iparams: [0] enableSleep.
 	B2S_SETTERS = 10,
+
+	// Pre-solve: a static one-way platform (16 x 0.5 m at y = 4, pre-solve events on) over a
+	// 30 x 1 m ground box (half extents); 12 boxes launched up through it from below and 12
+	// boxes, circles and capsules dropped onto it, every third with pre-solve events. The
+	// callback drops the points of platform contacts approached from below and the second point
+	// of some other pairs. Before chosen steps b2s_step turns the platform's flag off and on,
+	// relaunches bodies and clears and restores the callback. After each step it folds the
+	// callback's call count and an order-independent sum of its arguments, the platform's flag
+	// and its contact count into b2s_probe_hash. iparams: [0] enableSleep.
+	B2S_PRESOLVE = 11,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.
@@ -129,6 +142,9 @@ B2S_API uint32_t b2s_hash( int handle );
 // Getter results folded after every step of a B2S_EDITS scene (FNV-1a), 0 for other scenes.
 B2S_API uint32_t b2s_probe_hash( int handle );
 
+// b2World_GetStateHash of the scene's world, 0 for an unknown handle.
+B2S_API uint64_t b2s_state_hash( int handle );
+
 // Copy the world transforms (p.x, p.y, q.c, q.s) of the scene bodies in creation order.
 // Returns the number of bodies written.
 B2S_API int b2s_get_transforms( int handle, float* out, int capacity );
@@ -146,6 +162,11 @@ B2S_API int b2s_awake_body_count( int handle );
 // normal x, normal y, approach speed] with floats as their bits (zero for begin and end events).
 // Writes at most capacity / 10 records and returns the record count.
 B2S_API int b2s_contact_events( int handle, uint32_t* out, int capacity );
+
+// Joint events of the last step in the order b2World_GetJointEvents lists them: records of
+// [joint index1, joint generation, user data as an integer]. Writes at most capacity / 3 records
+// and returns the record count.
+B2S_API int b2s_joint_events( int handle, uint32_t* out, int capacity );
 
 // World counters: out[0..5] = bodyCount, shapeCount, contactCount, jointCount, islandCount,
 // awakeContactCount; out[6..29] = colorCounts[0..23]. Returns the number of ints written (30).

@@ -29,6 +29,7 @@ SCENE_JOINT_ZOO = 7
 SCENE_EVENTS = 8
 SCENE_EDITS = 9
 SCENE_SETTERS = 10
+SCENE_PRESOLVE = 11
 
 
 class SceneLib:
@@ -48,6 +49,10 @@ class SceneLib:
         lib.b2s_body_count.restype = ctypes.c_int
         lib.b2s_hash.restype = ctypes.c_uint32
         lib.b2s_probe_hash.restype = ctypes.c_uint32
+        lib.b2s_joint_events.restype = ctypes.c_int
+        lib.b2s_joint_events.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
+        lib.b2s_state_hash.restype = ctypes.c_uint64
+        lib.b2s_state_hash.argtypes = [ctypes.c_int]
         lib.b2s_get_transforms.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         lib.b2s_get_velocities.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         lib.b2s_counters.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -108,6 +113,10 @@ class SceneLib:
         """Getter results folded after every step of an edits scene."""
         return int(self.lib.b2s_probe_hash(h))
 
+    def state_hash(self, h):
+        """b2World_GetStateHash of the scene's world (64-bit)."""
+        return int(self.lib.b2s_state_hash(h))
+
     def hash(self, h):
         return self.lib.b2s_hash(h)
 
@@ -141,6 +150,13 @@ class SceneLib:
         out = np.zeros(capacity * 10, dtype=np.uint32)
         n = self.lib.b2s_contact_events(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), out.size)
         return out[: 10 * n].reshape(n, 10)
+
+    def joint_events(self, h, capacity=1 << 14):
+        """The last step's joint events as a (records, 3) uint32 array: joint index1, generation,
+        user data (see b2s_joint_events)."""
+        out = np.zeros(capacity * 3, dtype=np.uint32)
+        n = self.lib.b2s_joint_events(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), out.size)
+        return out[: 3 * n].reshape(n, 3)
 
     def falling_hinges(self, workers=1, step_limit=500):
         s = ctypes.c_int()

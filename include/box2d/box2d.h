@@ -19,6 +19,9 @@ B2_API bool b2World_IsValid( b2WorldId id );
 B2_API void b2World_Step( b2WorldId worldId, float timeStep, int subStepCount );
 B2_API b2BodyEvents b2World_GetBodyEvents( b2WorldId worldId );
 B2_API b2ContactEvents b2World_GetContactEvents( b2WorldId worldId );
+/// Joints whose force or torque reached the joint's threshold during the last step (reference:
+/// include/box2d/box2d.h:64).
+B2_API b2JointEvents b2World_GetJointEvents( b2WorldId worldId );
 B2_API void b2World_EnableSleeping( b2WorldId worldId, bool flag );
 B2_API bool b2World_IsSleepingEnabled( b2WorldId worldId );
 B2_API void b2World_EnableContinuous( b2WorldId worldId, bool flag );
@@ -44,8 +47,18 @@ B2_API float b2World_GetRestitutionThreshold( b2WorldId worldId );
 B2_API void b2World_SetHitEventThreshold( b2WorldId worldId, float value );
 B2_API float b2World_GetHitEventThreshold( b2WorldId worldId );
 B2_API void b2World_SetCustomFilterCallback( b2WorldId worldId, b2CustomFilterFcn* fcn, void* context );
+/// Pre-solve callback, run on the host for the contacts of shapes with pre-solve events enabled
+/// (their narrowphase runs on the host around it). A pre-continuous callback is accepted, but a
+/// step with it set and a shape with pre-solve events enabled stops with an error, because
+/// continuous collision runs on the GPU (reference: include/box2d/box2d.h:146-148).
+B2_API void b2World_SetPreSolveCallback( b2WorldId worldId, b2PreSolveFcn* preSolveFcn, b2PreContinuousFcn* preContinuousFcn,
+										 void* context );
 B2_API void b2World_SetFrictionCallback( b2WorldId worldId, b2FrictionCallback* callback );
 B2_API void b2World_SetRestitutionCallback( b2WorldId worldId, b2RestitutionCallback* callback );
+/// Deterministic hash of the simulation state: body transforms and awake velocities, contact and
+/// joint impulses, and the set/color/local indices and id pools that drive the solve (reference:
+/// include/box2d/box2d.h:230-235, computed as src/world_snapshot.c:1025-1187 does).
+B2_API uint64_t b2World_GetStateHash( b2WorldId worldId );
 
 /// Bodies (reference: include/box2d/box2d.h:355-645)
 B2_API b2BodyId b2CreateBody( b2WorldId worldId, const b2BodyDef* def );
@@ -121,6 +134,10 @@ B2_API b2BodyId b2Shape_GetBody( b2ShapeId shapeId );
 B2_API b2AABB b2Shape_GetAABB( b2ShapeId shapeId );
 B2_API void b2Shape_EnableSensorEvents( b2ShapeId shapeId, bool flag );
 B2_API bool b2Shape_AreSensorEventsEnabled( b2ShapeId shapeId );
+/// Pre-solve events of a shape; contacts created afterwards follow the flag (reference:
+/// include/box2d/box2d.h:770-774).
+B2_API void b2Shape_EnablePreSolveEvents( b2ShapeId shapeId, bool flag );
+B2_API bool b2Shape_ArePreSolveEventsEnabled( b2ShapeId shapeId );
 B2_API b2Circle b2Shape_GetCircle( b2ShapeId shapeId );
 B2_API b2Segment b2Shape_GetSegment( b2ShapeId shapeId );
 B2_API b2Capsule b2Shape_GetCapsule( b2ShapeId shapeId );
@@ -169,6 +186,13 @@ B2_API void b2Joint_SetLocalFrameA( b2JointId jointId, b2Transform localFrame );
 B2_API b2Transform b2Joint_GetLocalFrameA( b2JointId jointId );
 B2_API void b2Joint_SetLocalFrameB( b2JointId jointId, b2Transform localFrame );
 B2_API b2Transform b2Joint_GetLocalFrameB( b2JointId jointId );
+/// Joint user data and event thresholds (reference: include/box2d/box2d.h:955-994).
+B2_API void b2Joint_SetUserData( b2JointId jointId, void* userData );
+B2_API void* b2Joint_GetUserData( b2JointId jointId );
+B2_API void b2Joint_SetForceThreshold( b2JointId jointId, float threshold );
+B2_API float b2Joint_GetForceThreshold( b2JointId jointId );
+B2_API void b2Joint_SetTorqueThreshold( b2JointId jointId, float threshold );
+B2_API float b2Joint_GetTorqueThreshold( b2JointId jointId );
 
 /// Revolute joint (reference: include/box2d/box2d.h:1336-1398)
 B2_API void b2RevoluteJoint_EnableSpring( b2JointId jointId, bool enableSpring );

@@ -29,6 +29,14 @@ typedef float b2RestitutionCallback( float restitutionA, uint64_t userMaterialId
 /// the world. (reference: include/box2d/types.h:1240-1252)
 typedef bool b2CustomFilterFcn( b2ShapeId shapeIdA, b2ShapeId shapeIdB, void* context );
 
+/// Pre-solve callback: called for a touching contact of a shape with pre-solve events enabled
+/// after its manifold is computed; it may edit the manifold (a point count of zero disables the
+/// contact for the step). (reference: include/box2d/types.h:1254-1267)
+typedef void b2PreSolveFcn( b2ShapeId shapeIdA, b2ShapeId shapeIdB, b2Manifold* manifold, void* context );
+
+/// Pre-continuous callback: may veto a time-of-impact hit (reference: include/box2d/types.h:1269-1281).
+typedef bool b2PreContinuousFcn( b2ShapeId shapeIdA, b2ShapeId shapeIdB, b2Pos point, b2Vec2 normal, void* context );
+
 typedef struct b2Capacity
 {
 	int staticShapeCount;
@@ -602,6 +610,21 @@ typedef struct b2BodyEvents
 	b2BodyMoveEvent* moveEvents;
 	int moveCount;
 } b2BodyEvents;
+
+/// A joint whose constraint force or torque reached its threshold in the last step
+/// (reference: include/box2d/types.h:1196-1205).
+typedef struct b2JointEvent
+{
+	b2JointId jointId;
+	void* userData;
+} b2JointEvent;
+
+/// Joint events of the last step in ascending joint id (reference: include/box2d/types.h:1207-1217).
+typedef struct b2JointEvents
+{
+	b2JointEvent* jointEvents;
+	int count;
+} b2JointEvents;
 
 typedef struct b2ContactData
 {

@@ -6,7 +6,7 @@ driver box2d_amd/scenes/scenes.c). Every fixture is data only: scene parameters,
 protocol, and the reference's outputs (djb2 transform hashes as in shared/determinism.c:110-125,
 world counters, and final transform bits for the small scenes).
 
-    python tests/golden/gen_golden.py
+    python tests/golden/gen_golden.py [--events-and-hashes-only]
 """
 
 import json
@@ -20,7 +20,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from box2d_amd.scenes_api import (SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES,  # noqa: E402
-                                  SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS, SCENE_EDITS, SCENE_SETTERS)
+                                  SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS, SCENE_EDITS, SCENE_SETTERS,
+                                  SCENE_PRESOLVE)
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
 
@@ -102,10 +103,93 @@ def edits_fixture(lib, name, sleep, steps, kind=SCENE_EDITS):
     return {"name": name, "kind": kind, "iparams": [sleep], "fparams": [], "steps": steps, "per_step": per_step}
 
 
+# (name, kind, iparams, fparams, steps, every): scenes whose b2World_GetStateHash is recorded
+STATE_HASH_SCENES = [
+    ("pyramid20_sleep", SCENE_PYRAMID, (20, 1), (40.0,), 500, 1),
+    ("mixed_pile_1", SCENE_MIXED_PILE, (64, 1, 1), (), 300, 1),
+    ("joint_zoo", SCENE_JOINT_ZOO, (0,), (), 400, 1),
+    ("joint_zoo_sleep", SCENE_JOINT_ZOO, (1,), (), 400, 1),
+    ("events_60", SCENE_EVENTS, (60, 12345), (), 300, 1),
+    ("edits_sleep", SCENE_EDITS, (1,), (), 240, 1),
+    ("setters_sleep", SCENE_SETTERS, (1,), (), 200, 1),
+    ("rain_small", SCENE_RAIN, (2000, 40, 0, 7), (48.0,), 400, 10),
+]
+
+
+def state_hash_fixtures(lib):
+    """The reference's deep state hash (b2World_GetStateHash) after every `every`-th step."""
+    out = []
+    for name, kind, iparams, fparams, steps, every in STATE_HASH_SCENES:
+        h = lib.create(kind, iparams, fparams, workers=8)
+        hashes = {"0": lib.state_hash(h)}
+        for step in range(1, steps + 1):
+            lib.step(h)
+            if step % every == 0:
+                hashes[str(step)] = lib.state_hash(h)
+        lib.destroy(h)
+        out.append({"name": name, "kind": kind, "iparams": list(iparams), "fparams": list(fparams),
+                    "steps": steps, "every": every, "hashes": hashes})
+    return out
+
+
+def joint_event_fixture(lib, name, sleep, steps):
+    """Per step of the joint zoo with event thresholds: the joint event count, an FNV-1a hash of
+    the records (b2s_joint_events) and the transform hash; full records every 50 steps."""
+    h = lib.create(SCENE_JOINT_ZOO, (sleep, 1), (), workers=8)
+    fx = {"name": name, "kind": SCENE_JOINT_ZOO, "iparams": [sleep, 1], "fparams": [], "steps": steps,
+          "per_step": [], "records": {}}
+    for step in range(1, steps + 1):
+        lib.step(h)
+        ev = lib.joint_events(h)
+        fx["per_step"].append([len(ev), fnv1a32(ev), lib.hash(h)])
+        if step % 50 == 0:
+            fx["records"][str(step)] = ev.reshape(-1).tolist()
+    lib.destroy(h)
+    return fx
+
+
+def write_joint_events(lib):
+    path = os.path.join(HERE, "joint_events.json")
+    with open(path, "w") as f:
+        json.dump({"generator": "tests/golden/gen_golden.py",
+                   "reference": "erincatto/box2d v3.2 b2World_GetJointEvents (src/solver.c, src/joint.c)",
+                   "scenes": [joint_event_fixture(lib, "joint_zoo_events", 0, 300),
+                              joint_event_fixture(lib, "joint_zoo_events_sleep", 1, 300)]},
+                  f, separators=(",", ":"))
+    print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+def write_presolve(lib):
+    """The pre-solve scene (one-way platform) with sleep off and on: per step the transform hash,
+    the probe hash (callback tally, platform flag and contact count) and the body count."""
+    path = os.path.join(HERE, "presolve.json")
+    with open(path, "w") as f:
+        json.dump({"generator": "tests/golden/gen_golden.py",
+                   "reference": "erincatto/box2d v3.2 pre-solve callback (src/contact.c:573-583)",
+                   "scenes": [edits_fixture(lib, "presolve", 0, 200, SCENE_PRESOLVE),
+                              edits_fixture(lib, "presolve_sleep", 1, 200, SCENE_PRESOLVE)]},
+                  f, separators=(",", ":"))
+    print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+def write_state_hashes(lib):
+    path = os.path.join(HERE, "state_hashes.json")
+    with open(path, "w") as f:
+        json.dump({"generator": "tests/golden/gen_golden.py",
+                   "reference": "erincatto/box2d v3.2 b2World_GetStateHash (src/world_snapshot.c)",
+                   "scenes": state_hash_fixtures(lib)}, f, separators=(",", ":"))
+    print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
 def main():
     if not os.path.exists(REF_LIB):
         sys.exit(f"{REF_LIB} missing: run `make -C oracle ref` first")
     lib = SceneLib(REF_LIB)
+    write_state_hashes(lib)
+    write_joint_events(lib)
+    write_presolve(lib)
+    if "--events-and-hashes-only" in sys.argv:
+        return
     fixtures = []
     fixtures.append(scene_fixture(lib, "hello", SCENE_HELLO, (), (), 120, set(range(1, 121)), True))
     fixtures.append(scene_fixture(lib, "pyramid20", SCENE_PYRAMID, (20, 0), (40.0,), 500,

@@ -227,3 +227,28 @@ def test_setters_and_callbacks_match_reference(gpu_scenes, world_runs, name, pat
             assert got == fx["per_step"][step - 1], f"{name} step {step}: [transforms, getters, bodies] {got}, reference {fx['per_step'][step - 1]}"
     finally:
         gpu_scenes.destroy(h)
+
+
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("name", ["presolve", "presolve_sleep"])
+def test_presolve_callback_matches_reference(gpu_scenes, name, path):
+    """Pre-solve callback on a one-way platform: the narrowphase of contacts with pre-solve
+    events runs on the host around the callback, which drops points of contacts approached from
+    below and second points of some pairs. Transforms, the callback's call count and argument
+    tally, the platform's flag and its contact count equal the reference's after every step,
+    through flag changes, relaunches and the callback cleared and restored
+    (tests/golden/presolve.json)."""
+    import box2d_amd
+    from conftest import load_golden
+
+    fx = next(s for s in load_golden("presolve.json")["scenes"] if s["name"] == name)
+    print(f"[gpu] presolve scene {name} path {path}", flush=True)
+    h = gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"])
+    box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
+    try:
+        for step in range(1, fx["steps"] + 1):
+            gpu_scenes.step(h)
+            got = [gpu_scenes.hash(h), gpu_scenes.probe_hash(h), gpu_scenes.body_count(h)]
+            assert got == fx["per_step"][step - 1], f"{name} step {step}: [transforms, probe, bodies] {got}, reference {fx['per_step'][step - 1]}"
+    finally:
+        gpu_scenes.destroy(h)
