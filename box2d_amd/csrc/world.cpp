@@ -4275,6 +4275,8 @@ bool World::StepBegin( float timeStep, int subStepCount )
 
 	stepParams = params;
 	collideTicks = b2GetTicks();
+	// the narrowphase and the contact scatter rewrite the device contacts
+	contactsStale = true;
 	device->CollideLaunch( params );
 	return true;
 }
