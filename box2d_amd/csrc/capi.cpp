@@ -157,6 +157,12 @@ b2BodyEvents b2World_GetBodyEvents( b2WorldId worldId )
 	return world->GetBodyEvents();
 }
 
+b2SensorEvents b2World_GetSensorEvents( b2WorldId worldId )
+{
+	World* world = GetWorld( worldId );
+	return world != nullptr ? world->GetSensorEvents() : b2SensorEvents{};
+}
+
 b2JointEvents b2World_GetJointEvents( b2WorldId worldId )
 {
 	World* world = GetWorld( worldId );
@@ -1135,10 +1141,42 @@ void b2Shape_EnableSensorEvents( b2ShapeId shapeId, bool flag )
 {
 	Shape* shape;
 	World* world = WorldFromShape( shapeId, &shape );
-	if ( world != nullptr && world->locked == false )
+	if ( world != nullptr && world->locked == false && shape->enableSensorEvents != flag )
 	{
+		world->PullShapes();
 		shape->enableSensorEvents = flag;
+		uint32_t& flags = world->shapeData[shape->id].flags;
+		flags = flag ? ( flags | kShapeEnableSensorEvents ) : ( flags & ~kShapeEnableSensorEvents );
+		world->shapesDirty = true;
 	}
+}
+
+int b2Shape_GetSensorCapacity( b2ShapeId shapeId )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	if ( world == nullptr || world->locked || shape->sensorIndex == B2_NULL_INDEX )
+	{
+		return 0;
+	}
+	return (int)world->sensors[shape->sensorIndex].overlaps2.size();
+}
+
+int b2Shape_GetSensorData( b2ShapeId shapeId, b2ShapeId* visitorIds, int capacity )
+{
+	Shape* shape;
+	World* world = WorldFromShape( shapeId, &shape );
+	if ( world == nullptr || world->locked || shape->sensorIndex == B2_NULL_INDEX )
+	{
+		return 0;
+	}
+	const std::vector<SensorVisitor>& overlaps = world->sensors[shape->sensorIndex].overlaps2;
+	int count = std::min( (int)overlaps.size(), capacity );
+	for ( int i = 0; i < count; ++i )
+	{
+		visitorIds[i] = b2ShapeId{ overlaps[i].shapeId + 1, shapeId.world0, overlaps[i].generation };
+	}
+	return count;
 }
 
 bool b2Shape_AreSensorEventsEnabled( b2ShapeId shapeId )

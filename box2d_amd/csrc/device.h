@@ -55,6 +55,13 @@ struct SolverLists
 constexpr int kOverflowChunkContacts = 512;
 constexpr int kOverflowChunkBodies = 2 * kOverflowChunkContacts;
 
+// A pair of ids: (sensor, visitor) shape ids, or (active sensor index, shape id).
+struct IdPair
+{
+	int first;
+	int second;
+};
+
 // A contact's friction and restitution from the user's mixing callbacks.
 struct ContactMaterial
 {
@@ -123,6 +130,11 @@ public:
 	void UploadBodies( const World& world );
 	void UploadShapes( const World& world );
 	void UploadJoints( const World& world );
+	// Overlaps of the given sensor shapes with every shape in a broadphase tree, after the step's
+	// solve: (index into sensorShapes, shape id), unordered.
+	void SensorOverlaps( const std::vector<int>& sensorShapes, std::vector<IdPair>* overlaps );
+	// The last solve's continuous-collision hits on static sensors: (sensor shape, fast shape).
+	void ContinuousSensorHits( std::vector<IdPair>* hits );
 	// The narrowphase of contacts the device deferred for the pre-solve callback, run on the
 	// host copies (bodies, shapes and contacts must be current there) with the world's callback;
 	// the updated contacts go back to the device and their change records are appended.

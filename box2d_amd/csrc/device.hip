@@ -1907,7 +1907,14 @@ struct ContinuousArgs
 	int* enlargedShapes;
 	b2AABB* enlargedShapeAABBs;
 	float huge;
+	IdPair* sensorHits; // (sensor shape, fast shape) hits on static sensors; null when the world has none
+	int* sensorHitCount;
+	int sensorHitCapacity;
 };
+
+constexpr int kMaxContinuousSensorHits = 8;
+constexpr int kSensorHitCountSlot = 9;	   // device counter slots
+constexpr int kSensorOverlapCountSlot = 10;
 
 B2G_DEV bool ShouldShapesCollide( const b2Filter& a, const b2Filter& b )
 {
@@ -2062,6 +2069,10 @@ B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 
 	float fraction = 1.0f;
 	bool hadImpact = false;
+	// sensor hits no later than the solid hit found so far (reference src/solver.c:319-335)
+	IdPair sensorHits[kMaxContinuousSensorHits];
+	float sensorFractions[kMaxContinuousSensorHits];
+	int sensorCount = 0;
 
 	for ( int shapeId = fastSim->headShapeId; shapeId != B2_NULL_INDEX; shapeId = a.shapes[shapeId].nextShapeId )
 	{
@@ -2073,6 +2084,11 @@ B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 		b2AABB box1 = fastShape->aabb;
 		b2AABB box2 = OffsetAABB( FatShapeAABB( *fastShape, xf2, 0.0f ), base );
 		fastShape->aabb = box2;
+		// no continuous collision for sensors, but their bounds follow (reference src/solver.c:436-440)
+		if ( fastShape->flags & kShapeIsSensor )
+		{
+			continue;
+		}
 		b2AABB swept = b2AABB_Union( box1, box2 );
 
 		int stack[kCcdStackSize];
@@ -2106,6 +2122,12 @@ B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 			int otherId = node.shapeId;
 			const ShapeDev& other = a.shapes[otherId];
 			if ( otherId == shapeId || other.bodyId == fastShape->bodyId )
+			{
+				continue;
+			}
+			// a sensor counts only when both shapes enable sensor events (reference src/solver.c:209-214)
+			const bool otherSensor = ( other.flags & kShapeIsSensor ) != 0;
+			if ( otherSensor && ( ( other.flags & fastShape->flags & kShapeEnableSensorEvents ) == 0 || a.sensorHits == nullptr ) )
 			{
 				continue;
 			}
@@ -2161,6 +2183,16 @@ B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 			input.maxFraction = fraction;
 
 			b2TOIOutput output = TimeOfImpact( input );
+			if ( otherSensor )
+			{
+				if ( output.fraction <= fraction && sensorCount < kMaxContinuousSensorHits )
+				{
+					sensorHits[sensorCount] = IdPair{ otherId, shapeId };
+					sensorFractions[sensorCount] = output.fraction;
+					sensorCount += 1;
+				}
+				continue;
+			}
 			float hitFraction = fraction;
 			bool didHit = false;
 			if ( 0.0f < output.fraction && output.fraction < fraction )
@@ -2193,6 +2225,19 @@ B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 	if ( hadImpact )
 	{
 		fastSim->flags |= kHadTimeOfImpact;
+	}
+
+	// sensor hits that happened before the final solid hit (reference src/solver.c:540-547)
+	for ( int i = 0; i < sensorCount; ++i )
+	{
+		if ( sensorFractions[i] < fraction )
+		{
+			int slot = atomicAdd( a.sensorHitCount, 1 );
+			if ( slot < a.sensorHitCapacity )
+			{
+				a.sensorHits[slot] = sensorHits[i];
+			}
+		}
 	}
 
 	if ( fraction < 1.0f )
@@ -2792,6 +2837,11 @@ struct DeviceWorld::Impl
 	DevArray<CcdNode> ccdTree;
 	int ccdTreeRoot = B2_NULL_INDEX;
 	DevArray<int2> ccdNoCollide;
+	DevArray<IdPair> sensorHits;	  // continuous collision hits on static sensors
+	DevArray<IdPair> sensorOverlaps; // the sensor stage's overlaps
+	DevArray<int> sensorShapes;
+	int sensorHitCapacity = 0;
+	int sensorOverlapCapacity = 0;
 	DevArray<uint64_t> ccdFilterRejects;
 	int ccdFilterRejectCount = 0;
 	std::vector<CcdNode> ccdTreeStaging;
@@ -3228,6 +3278,143 @@ void DeviceWorld::UploadContinuousFilterRejects( const std::vector<uint64_t>& ke
 	}
 	UploadVector( impl->ccdFilterRejects, keys, impl->stream );
 	impl->ccdFilterRejectCount = (int)keys.size();
+}
+
+// Sensor overlaps (reference src/sensor.c:42-126): lane (sensor, shape) applies the tree query's
+// tests (the shape's fat box against the sensor's box, the sensor's mask against the shape's
+// category) and the callback's (sensor events on the visitor, another body, the filter, the GJK
+// distance below 10 FLT_EPSILON). Every shape is tested instead of walking the trees; the tree walk
+// only prunes by those same box and mask tests, so the overlaps are the same.
+struct SensorArgs
+{
+	const int* sensorShapes;
+	int sensorCount;
+	const ShapeDev* shapes;
+	int shapeCount;
+	const BodySim* sims;
+	IdPair* overlaps;
+	int* overlapCount;
+	int capacity;
+};
+
+__global__ void SensorOverlapKernel( SensorArgs a )
+{
+	int s = blockIdx.y;
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( s >= a.sensorCount || j >= a.shapeCount )
+	{
+		return;
+	}
+	int sensorId = a.sensorShapes[s];
+	if ( j == sensorId )
+	{
+		return;
+	}
+	const ShapeDev& other = a.shapes[j];
+	const uint32_t need = kShapeHasProxy | kShapeEnableSensorEvents;
+	if ( ( other.flags & need ) != need )
+	{
+		return;
+	}
+	const ShapeDev& sensor = a.shapes[sensorId];
+	if ( b2AABB_Overlaps( other.fatAABB, sensor.aabb ) == false || ( other.filter.categoryBits & sensor.filter.maskBits ) == 0 )
+	{
+		return;
+	}
+	if ( other.bodyId == sensor.bodyId || ShouldShapesCollide( sensor.filter, other.filter ) == false )
+	{
+		return;
+	}
+	b2DistanceInput input;
+	input.proxyA = ShapeProxy( sensor );
+	input.proxyB = ShapeProxy( other );
+	input.transform = b2InvMulWorldTransforms( a.sims[sensor.bodyId].transform, a.sims[other.bodyId].transform );
+	input.useRadii = true;
+	b2SimplexCache cache = {};
+	b2DistanceOutput output = ShapeDistance( input, &cache );
+	if ( output.distance < 10.0f * FLT_EPSILON )
+	{
+		int slot = atomicAdd( a.overlapCount, 1 );
+		if ( slot < a.capacity )
+		{
+			a.overlaps[slot] = IdPair{ s, j };
+		}
+	}
+}
+
+void DeviceWorld::SensorOverlaps( const std::vector<int>& sensorShapes, std::vector<IdPair>* overlaps )
+{
+	overlaps->clear();
+	int n = (int)sensorShapes.size();
+	if ( n == 0 || impl->shapeCount == 0 )
+	{
+		return;
+	}
+	hipStream_t s = impl->stream;
+	UploadVector( impl->sensorShapes, sensorShapes, s );
+	int* count = impl->counters + kSensorOverlapCountSlot;
+	for ( int attempt = 0; attempt < 2; ++attempt )
+	{
+		if ( impl->sensorOverlapCapacity == 0 )
+		{
+			impl->sensorOverlapCapacity = 4096;
+		}
+		impl->sensorOverlaps.Reserve( impl->sensorOverlapCapacity, false, s, 0 );
+		SensorArgs a;
+		a.sensorShapes = impl->sensorShapes.ptr;
+		a.sensorCount = n;
+		a.shapes = impl->shapes.ptr;
+		a.shapeCount = impl->shapeCount;
+		a.sims = impl->sims.ptr;
+		a.overlaps = impl->sensorOverlaps.ptr;
+		a.overlapCount = count;
+		a.capacity = impl->sensorOverlapCapacity;
+		HIP_CHECK( hipMemsetAsync( count, 0, sizeof( int ), s ) );
+		SensorOverlapKernel<<<dim3( Blocks( impl->shapeCount ), n ), kBlock, 0, s>>>( a );
+		HIP_CHECK( hipGetLastError() );
+		HIP_CHECK( hipMemcpyAsync( impl->hostCounters + kSensorOverlapCountSlot, count, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+		int found = impl->hostCounters[kSensorOverlapCountSlot];
+		if ( found > impl->sensorOverlapCapacity )
+		{
+			// too many to keep: grow and run again
+			impl->sensorOverlapCapacity = 2 * found;
+			continue;
+		}
+		overlaps->resize( found );
+		if ( found > 0 )
+		{
+			HIP_CHECK( hipMemcpyAsync( overlaps->data(), impl->sensorOverlaps.ptr, found * sizeof( IdPair ), hipMemcpyDeviceToHost, s ) );
+			HIP_CHECK( hipStreamSynchronize( s ) );
+		}
+		return;
+	}
+	Fatal( "sensor overlaps kept growing between two identical queries" );
+}
+
+void DeviceWorld::ContinuousSensorHits( std::vector<IdPair>* hits )
+{
+	hits->clear();
+	if ( impl->sensorHitCapacity == 0 )
+	{
+		return;
+	}
+	hipStream_t s = impl->stream;
+	int* count = impl->counters + kSensorHitCountSlot;
+	HIP_CHECK( hipMemcpyAsync( impl->hostCounters + kSensorHitCountSlot, count, sizeof( int ), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipMemsetAsync( count, 0, sizeof( int ), s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
+	int found = impl->hostCounters[kSensorHitCountSlot];
+	if ( found > impl->sensorHitCapacity )
+	{
+		Fatal( "continuous sensor hits %d exceed their capacity %d", found, impl->sensorHitCapacity );
+	}
+	hits->resize( found );
+	if ( found > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( hits->data(), impl->sensorHits.ptr, found * sizeof( IdPair ), hipMemcpyDeviceToHost, s ) );
+		HIP_CHECK( hipStreamSynchronize( s ) );
+	}
 }
 
 void DeviceWorld::DownloadJoints( World& world )
@@ -4100,6 +4287,15 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		ka.enlargedShapes = impl->enlargedShapes.ptr;
 		ka.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
 		ka.huge = B2_HUGE;
+		// at most kMaxContinuousSensorHits per fast body
+		impl->sensorHitCapacity = params.sensorHits != 0 ? kMaxContinuousSensorHits * std::max( nb, 1 ) : 0;
+		if ( impl->sensorHitCapacity > 0 )
+		{
+			impl->sensorHits.Reserve( impl->sensorHitCapacity, false, s, 0 );
+		}
+		ka.sensorHits = impl->sensorHitCapacity > 0 ? impl->sensorHits.ptr : nullptr;
+		ka.sensorHitCount = impl->counters + kSensorHitCountSlot;
+		ka.sensorHitCapacity = impl->sensorHitCapacity;
 		if ( rec )
 		{
 			impl->rec.continuous = true;

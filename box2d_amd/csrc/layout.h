@@ -67,6 +67,8 @@ enum ShapeFlags : uint32_t
 	kShapeEnableContactEvents = 0x2,
 	kShapeEnablePreSolveEvents = 0x4,
 	kShapeIsSensor = 0x8,
+	kShapeEnableSensorEvents = 0x10,
+	kShapeHasProxy = 0x20, // the shape is in a broadphase tree (its body is not disabled)
 };
 
 struct Softness
@@ -355,6 +357,8 @@ struct StepParams
 	// the world has a pre-solve callback: the narrowphase leaves contacts that enable pre-solve
 	// events to the host
 	int preSolve;
+	// the world has a sensor on a static body: continuous collision records its hits
+	int sensorHits;
 };
 
 } // namespace b2g

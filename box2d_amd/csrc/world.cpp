@@ -1589,6 +1589,8 @@ void World::CreateShapeProxy( Shape* shape, bool forcePairCreation )
 	sd.fatAABB = fatAABB;
 
 	shape->proxyKey = broadPhase.CreateProxy( fatAABB, body.type, shape->filter.categoryBits, shape->id, forcePairCreation );
+	sd.flags |= kShapeHasProxy;
+	shapesDirty = true;
 }
 
 b2ShapeId World::CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void* geometry, b2ShapeType shapeType )
@@ -1602,11 +1604,6 @@ b2ShapeId World::CreateShape( b2BodyId bodyId, const b2ShapeDef* def, const void
 	if ( body == nullptr )
 	{
 		return b2_nullShapeId;
-	}
-
-	if ( def->isSensor )
-	{
-		Fatal( "sensor shapes are not supported by the GPU step" );
 	}
 
 	PullBodies();
@@ -1686,7 +1683,8 @@ int World::CreateShapeInternal( Body* body, const b2ShapeDef* def, const void* g
 
 	sd.type = shapeType;
 	sd.bodyId = body->id;
-	sd.flags = ( def->enableHitEvents ? kShapeEnableHitEvents : 0 ) | ( def->enableContactEvents ? kShapeEnableContactEvents : 0 );
+	sd.flags = ( def->enableHitEvents ? kShapeEnableHitEvents : 0 ) | ( def->enableContactEvents ? kShapeEnableContactEvents : 0 ) |
+			   ( def->isSensor ? kShapeIsSensor : 0 ) | ( def->enableSensorEvents ? kShapeEnableSensorEvents : 0 );
 	sd.friction = def->material.friction;
 	sd.restitution = def->material.restitution;
 	sd.rollingResistance = def->material.rollingResistance;
@@ -1717,6 +1715,14 @@ int World::CreateShapeInternal( Body* body, const b2ShapeDef* def, const void* g
 	body->headShapeId = shapeId;
 	bodySims[body->id].headShapeId = shapeId;
 	body->shapeCount += 1;
+
+	// sensors (reference src/shape.c:189-197)
+	if ( def->isSensor )
+	{
+		shape->sensorIndex = (int)sensors.size();
+		sensors.emplace_back();
+		sensors.back().shapeId = shapeId;
+	}
 	return shapeId;
 }
 
@@ -2171,11 +2177,16 @@ void World::DestroyBody( b2BodyId id )
 	while ( shapeId != B2_NULL_INDEX )
 	{
 		Shape* shape = &shapes[shapeId];
+		if ( shape->sensorIndex != B2_NULL_INDEX )
+		{
+			DestroySensor( shape );
+		}
 		if ( shape->proxyKey != B2_NULL_INDEX )
 		{
 			broadPhase.DestroyProxy( shape->proxyKey );
 			shape->proxyKey = B2_NULL_INDEX;
 		}
+		shapeData[shapeId].flags &= ~kShapeHasProxy;
 		if ( shape->material.restitution > 0.0f )
 		{
 			restitutionShapeCount -= 1;
@@ -2315,6 +2326,29 @@ void World::DestroyShapeProxy( Shape* shape )
 		broadPhase.DestroyProxy( shape->proxyKey );
 		shape->proxyKey = B2_NULL_INDEX;
 	}
+	shapeData[shape->id].flags &= ~kShapeHasProxy;
+	shapesDirty = true;
+}
+
+// A destroyed sensor reports the end of its overlaps; the last sensor takes its slot
+// (reference src/shape.c:365-402, src/sensor.c:392-429).
+void World::DestroySensor( Shape* shape )
+{
+	int index = shape->sensorIndex;
+	Sensor& sensor = sensors[index];
+	b2ShapeId sensorId = ShapeIdOf( shape->id );
+	for ( const SensorVisitor& v : sensor.overlaps2 )
+	{
+		sensorEndEvents[endEventArrayIndex].push_back( b2SensorEndTouchEvent{ sensorId, b2ShapeId{ v.shapeId + 1, worldId, v.generation } } );
+	}
+	int last = (int)sensors.size() - 1;
+	if ( index != last )
+	{
+		sensors[index] = std::move( sensors[last] );
+		shapes[sensors[index].shapeId].sensorIndex = index;
+	}
+	sensors.pop_back();
+	shape->sensorIndex = B2_NULL_INDEX;
 }
 
 void World::SetBodyTypeFlag( Body* body, b2BodyType type )
@@ -2564,6 +2598,7 @@ void World::DestroyShape( Shape* shape, bool updateBodyMass )
 		broadPhase.DestroyProxy( shape->proxyKey );
 		shape->proxyKey = B2_NULL_INDEX;
 	}
+	shapeData[shapeId].flags &= ~kShapeHasProxy;
 
 	int contactKey = body->headContactKey;
 	while ( contactKey != B2_NULL_INDEX )
@@ -2574,6 +2609,11 @@ void World::DestroyShape( Shape* shape, bool updateBodyMass )
 		{
 			DestroyContact( contact );
 		}
+	}
+
+	if ( shape->sensorIndex != B2_NULL_INDEX )
+	{
+		DestroySensor( shape );
 	}
 
 	if ( shape->material.restitution > 0.0f )
@@ -3420,11 +3460,16 @@ void World::BuildContinuousFilterRejects()
 		bool anyCustom = false;
 		for ( const Shape& shape : shapes )
 		{
-			if ( shape.id == B2_NULL_INDEX || shape.proxyKey == B2_NULL_INDEX || shape.sensorIndex != B2_NULL_INDEX )
+			if ( shape.id == B2_NULL_INDEX || shape.proxyKey == B2_NULL_INDEX )
 			{
 				continue;
 			}
 			const Body& body = bodies[shape.bodyId];
+			if ( shape.sensorIndex != B2_NULL_INDEX && body.type != b2_staticBody )
+			{
+				// a moving sensor has no continuous collision
+				continue;
+			}
 			if ( body.type == b2_staticBody )
 			{
 				staticShapes.push_back( shape.id );
@@ -4162,6 +4207,7 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	contactBeginEvents.clear();
 	contactHitEvents.clear();
 	jointEvents.clear();
+	sensorBeginEvents.clear();
 	profile = b2Profile{};
 	recycledContactCount = 0;
 
@@ -4235,6 +4281,16 @@ bool World::StepBegin( float timeStep, int subStepCount )
 	}
 	params.jointEvents = jointEventJointCount > 0 ? 1 : 0;
 	params.preSolve = preSolveFcn != nullptr ? 1 : 0;
+	params.sensorHits = 0;
+	for ( const Sensor& sensor : sensors )
+	{
+		const Shape& shape = shapes[sensor.shapeId];
+		if ( shape.enableSensorEvents && bodies[shape.bodyId].type == b2_staticBody )
+		{
+			params.sensorHits = 1;
+			break;
+		}
+	}
 	if ( preContinuousFcn != nullptr && enableContinuous )
 	{
 		for ( const Shape& shape : shapes )
@@ -4351,11 +4407,18 @@ void World::StepEnd()
 		}
 		speculative.snapshot = true;
 	}
+	if ( sensors.empty() == false )
+	{
+		uint64_t sensorTicks = b2GetTicks();
+		OverlapSensors();
+		profile.sensors = b2GetMilliseconds( sensorTicks );
+	}
 	profile.step = b2GetMilliseconds( stepTicks );
 
 	// swap the end event buffers
 	endEventArrayIndex = 1 - endEventArrayIndex;
 	contactEndEvents[endEventArrayIndex].clear();
+	sensorEndEvents[endEventArrayIndex].clear();
 	locked = false;
 }
 
@@ -4375,6 +4438,151 @@ b2ContactEvents World::GetContactEvents()
 	events.endCount = (int)ends.size();
 	events.hitCount = (int)contactHitEvents.size();
 	return events;
+}
+
+b2SensorEvents World::GetSensorEvents()
+{
+	b2SensorEvents events = {};
+	if ( locked )
+	{
+		return events;
+	}
+	std::vector<b2SensorEndTouchEvent>& ends = sensorEndEvents[1 - endEventArrayIndex];
+	events.beginEvents = sensorBeginEvents.data();
+	events.endEvents = ends.data();
+	events.beginCount = (int)sensorBeginEvents.size();
+	events.endCount = (int)ends.size();
+	return events;
+}
+
+// Sensor overlaps of the step (reference src/sensor.c:140-390). The device tests every sensor
+// against every shape (the tree query's box and mask tests, the callback's filters and the GJK
+// distance); the host applies the custom filter, merges the continuous-collision hits, keeps each
+// sensor's overlaps sorted by shape id and reports what began and ended, sensor by sensor.
+void World::OverlapSensors()
+{
+	// continuous collision hits on static sensors, in this step's sensor hit lists
+	std::vector<IdPair> hits;
+	device->ContinuousSensorHits( &hits );
+	for ( IdPair h : hits )
+	{
+		const Shape& sensorShape = shapes[h.first];
+		if ( sensorShape.sensorIndex != B2_NULL_INDEX )
+		{
+			sensors[sensorShape.sensorIndex].hits.push_back( SensorVisitor{ h.second, shapes[h.second].generation } );
+		}
+	}
+
+	int sensorCount = (int)sensors.size();
+	std::vector<char> changed( sensorCount, 0 );
+	std::vector<int> active;
+	std::vector<int> activeShapes;
+	for ( int i = 0; i < sensorCount; ++i )
+	{
+		Sensor& sensor = sensors[i];
+		std::swap( sensor.overlaps1, sensor.overlaps2 );
+		sensor.overlaps2 = sensor.hits;
+		sensor.hits.clear();
+		const Shape& shape = shapes[sensor.shapeId];
+		if ( bodies[shape.bodyId].setIndex == kDisabledSet || shape.enableSensorEvents == false )
+		{
+			// the sensor drops every overlap
+			changed[i] = sensor.overlaps1.empty() ? 0 : 1;
+			continue;
+		}
+		active.push_back( i );
+		activeShapes.push_back( sensor.shapeId );
+	}
+
+	std::vector<IdPair> overlaps;
+	device->SensorOverlaps( activeShapes, &overlaps );
+	for ( IdPair o : overlaps )
+	{
+		Sensor& sensor = sensors[active[o.first]];
+		const Shape& sensorShape = shapes[sensor.shapeId];
+		const Shape& other = shapes[o.second];
+		if ( ( sensorShape.enableCustomFiltering || other.enableCustomFiltering ) && customFilterFcn != nullptr &&
+			 customFilterFcn( ShapeIdOf( sensorShape.id ), ShapeIdOf( other.id ), customFilterContext ) == false )
+		{
+			continue;
+		}
+		sensor.overlaps2.push_back( SensorVisitor{ o.second, other.generation } );
+	}
+
+	for ( int i : active )
+	{
+		std::vector<SensorVisitor>& v = sensors[i].overlaps2;
+		std::sort( v.begin(), v.end(), []( const SensorVisitor& a, const SensorVisitor& b ) { return a.shapeId < b.shapeId; } );
+		v.erase( std::unique( v.begin(), v.end(), []( const SensorVisitor& a, const SensorVisitor& b ) { return a.shapeId == b.shapeId; } ),
+				 v.end() );
+		const std::vector<SensorVisitor>& w = sensors[i].overlaps1;
+		bool same = v.size() == w.size();
+		for ( size_t k = 0; same && k < v.size(); ++k )
+		{
+			same = v[k].shapeId == w[k].shapeId && v[k].generation == w[k].generation;
+		}
+		changed[i] = same ? 0 : 1;
+	}
+
+	for ( int i = 0; i < sensorCount; ++i )
+	{
+		if ( changed[i] == 0 )
+		{
+			continue;
+		}
+		const Sensor& sensor = sensors[i];
+		b2ShapeId sensorId = ShapeIdOf( sensor.shapeId );
+		const std::vector<SensorVisitor>& r1 = sensor.overlaps1;
+		const std::vector<SensorVisitor>& r2 = sensor.overlaps2;
+		auto endEvent = [&]( const SensorVisitor& v ) {
+			sensorEndEvents[endEventArrayIndex].push_back( b2SensorEndTouchEvent{ sensorId, b2ShapeId{ v.shapeId + 1, worldId, v.generation } } );
+		};
+		auto beginEvent = [&]( const SensorVisitor& v ) {
+			sensorBeginEvents.push_back( b2SensorBeginTouchEvent{ sensorId, b2ShapeId{ v.shapeId + 1, worldId, v.generation } } );
+		};
+		size_t i1 = 0, i2 = 0;
+		while ( i1 < r1.size() && i2 < r2.size() )
+		{
+			const SensorVisitor& a = r1[i1];
+			const SensorVisitor& b = r2[i2];
+			if ( a.shapeId == b.shapeId )
+			{
+				if ( a.generation < b.generation )
+				{
+					endEvent( a );
+					i1 += 1;
+				}
+				else if ( a.generation > b.generation )
+				{
+					beginEvent( b );
+					i2 += 1;
+				}
+				else
+				{
+					i1 += 1;
+					i2 += 1;
+				}
+			}
+			else if ( a.shapeId < b.shapeId )
+			{
+				endEvent( a );
+				i1 += 1;
+			}
+			else
+			{
+				beginEvent( b );
+				i2 += 1;
+			}
+		}
+		for ( ; i1 < r1.size(); ++i1 )
+		{
+			endEvent( r1[i1] );
+		}
+		for ( ; i2 < r2.size(); ++i2 )
+		{
+			beginEvent( r2[i2] );
+		}
+	}
 }
 
 bool World::SolveBegin( StepParams& params )

@@ -82,6 +82,23 @@ struct Shape
 	bool enablePreSolveEvents = false;
 };
 
+// A shape a sensor overlaps, with the generation it had then (reference src/sensor.h).
+struct SensorVisitor
+{
+	int shapeId;
+	uint16_t generation;
+};
+
+// Per sensor shape: last step's and this step's overlaps (sorted by shape id) and the continuous
+// collision hits of this step (reference src/sensor.h, src/shape.c:189-197).
+struct Sensor
+{
+	int shapeId = B2_NULL_INDEX;
+	std::vector<SensorVisitor> overlaps1;
+	std::vector<SensorVisitor> overlaps2;
+	std::vector<SensorVisitor> hits;
+};
+
 struct ChainShape
 {
 	int id = B2_NULL_INDEX;
@@ -557,6 +574,14 @@ public:
 	// joints whose force or torque reached a threshold in the last step (reference
 	// src/physics_world.c:867, src/solver.c:1648-1700)
 	std::vector<b2JointEvent> jointEvents;
+	// sensors and their events (reference src/sensor.c:239-390): begin events are cleared when a
+	// step starts, end events are double buffered with the contact end events
+	std::vector<Sensor> sensors;
+	std::vector<b2SensorBeginTouchEvent> sensorBeginEvents;
+	std::vector<b2SensorEndTouchEvent> sensorEndEvents[2];
+	b2SensorEvents GetSensorEvents();
+	void OverlapSensors();
+	void DestroySensor( Shape* shape );
 	int jointEventJointCount = 0; // joints with a finite force or torque threshold at the last upload
 	int endEventArrayIndex = 0;
 	b2ContactEvents GetContactEvents();

@@ -43,6 +43,14 @@ typedef struct b2sScene
 	b2JointId* zooJoints;
 	int zooJointCount;
 
+	// B2S_SENSORS: the sensor shapes, the body carrying one, a visitor destroyed mid-run
+	b2ShapeId zoneShape;
+	b2ShapeId stripShape;
+	b2ShapeId sweepShape;
+	b2ShapeId carriedShape;
+	b2BodyId carrierId;
+	b2ShapeId doomedShape;
+
 	// B2S_PRESOLVE: the one-way platform and the callback's tally of the last step
 	b2ShapeId platformShape;
 	uint32_t preSolveCalls;
@@ -1362,6 +1370,205 @@ static void b2sPreSolveProbe( b2sScene* scene )
 	scene->probeHash = h;
 }
 
+// Sensor filter: visitors with odd user data and custom filtering are not sensed.
+static bool b2sSensorFilter( b2ShapeId shapeIdA, b2ShapeId shapeIdB, void* context )
+{
+	(void)context;
+	intptr_t a = (intptr_t)b2Body_GetUserData( b2Shape_GetBody( shapeIdA ) );
+	intptr_t b = (intptr_t)b2Body_GetUserData( b2Shape_GetBody( shapeIdB ) );
+	return ( ( a | b ) & 1 ) == 0;
+}
+
+static void b2sBuildSensors( b2sScene* scene, int enableSleep )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+	b2World_SetCustomFilterCallback( worldId, b2sSensorFilter, NULL );
+	scene->bodyCapacity = 0;
+
+	b2BodyDef groundDef = b2DefaultBodyDef();
+	groundDef.userData = (void*)(intptr_t)1000;
+	b2BodyId groundId = b2CreateBody( worldId, &groundDef );
+	b2ShapeDef groundShape = b2DefaultShapeDef();
+	groundShape.enableSensorEvents = true;
+	b2Polygon groundBox = b2MakeOffsetBox( 40.0f, 1.0f, ( b2Vec2 ){ 0.0f, -1.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &groundShape, &groundBox );
+	// a wall that stops the projectiles
+	b2Polygon wall = b2MakeOffsetBox( 0.5f, 4.0f, ( b2Vec2 ){ 32.0f, 4.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &groundShape, &wall );
+
+	// a static zone and a thin static strip, both sensors
+	b2ShapeDef sensorDef = b2DefaultShapeDef();
+	sensorDef.isSensor = true;
+	sensorDef.enableSensorEvents = true;
+	sensorDef.filter.maskBits = ~(uint64_t)0x4;
+	b2Polygon zone = b2MakeOffsetBox( 4.0f, 2.5f, ( b2Vec2 ){ -10.0f, 3.0f }, b2Rot_identity );
+	scene->zoneShape = b2CreatePolygonShape( groundId, &sensorDef, &zone );
+	sensorDef.filter.maskBits = B2_DEFAULT_MASK_BITS;
+	b2Polygon strip = b2MakeOffsetBox( 0.05f, 3.0f, ( b2Vec2 ){ 20.0f, 3.0f }, b2Rot_identity );
+	scene->stripShape = b2CreatePolygonShape( groundId, &sensorDef, &strip );
+
+	// a kinematic sensor sweeping across the zone
+	b2BodyDef sweepDef = b2DefaultBodyDef();
+	sweepDef.type = b2_kinematicBody;
+	sweepDef.position = ( b2Pos ){ -25.0f, 1.5f };
+	sweepDef.linearVelocity = ( b2Vec2 ){ 3.0f, 0.0f };
+	sweepDef.userData = (void*)(intptr_t)2000;
+	b2BodyId sweepId = b2CreateBody( worldId, &sweepDef );
+	b2Circle sweepCircle = { { 0.0f, 0.0f }, 1.5f };
+	scene->sweepShape = b2CreateCircleShape( sweepId, &sensorDef, &sweepCircle );
+	b2sPushBody( scene, &scene->bodyCapacity, sweepId );
+
+	// a dynamic carrier with a solid box and a sensor circle around it
+	b2BodyDef carrierDef = b2DefaultBodyDef();
+	carrierDef.type = b2_dynamicBody;
+	carrierDef.position = ( b2Pos ){ 5.0f, 3.0f };
+	carrierDef.userData = (void*)(intptr_t)3000;
+	scene->carrierId = b2CreateBody( worldId, &carrierDef );
+	b2ShapeDef solid = b2DefaultShapeDef();
+	solid.enableSensorEvents = true;
+	b2Polygon carrierBox = b2MakeBox( 0.5f, 0.5f );
+	b2CreatePolygonShape( scene->carrierId, &solid, &carrierBox );
+	b2Circle carried = { { 0.0f, 0.0f }, 2.0f };
+	scene->carriedShape = b2CreateCircleShape( scene->carrierId, &sensorDef, &carried );
+	b2sPushBody( scene, &scene->bodyCapacity, scene->carrierId );
+
+	// visitors dropped over the zone and the carrier: two in three sense, every fifth is outside
+	// the zone's mask, every seventh uses the custom filter
+	for ( int i = 0; i < 30; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ ( i < 20 ? -13.0f : 3.0f ) + 0.9f * (float)( i % 10 ), 7.0f + 1.1f * (float)( i / 10 ) };
+		bodyDef.rotation = b2MakeRot( 0.13f * (float)i );
+		bodyDef.userData = (void*)(intptr_t)i;
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.enableSensorEvents = ( i % 3 ) != 2;
+		def.enableCustomFiltering = ( i % 7 ) == 0;
+		if ( i % 5 == 4 )
+		{
+			def.filter.categoryBits = 0x4;
+		}
+		b2ShapeId shapeId;
+		if ( i % 3 == 0 )
+		{
+			b2Polygon box = b2MakeBox( 0.35f, 0.3f );
+			shapeId = b2CreatePolygonShape( bodyId, &def, &box );
+		}
+		else if ( i % 3 == 1 )
+		{
+			b2Circle circle = { { 0.0f, 0.0f }, 0.3f };
+			shapeId = b2CreateCircleShape( bodyId, &def, &circle );
+		}
+		else
+		{
+			b2Capsule capsule = { { -0.25f, 0.0f }, { 0.25f, 0.0f }, 0.2f };
+			shapeId = b2CreateCapsuleShape( bodyId, &def, &capsule );
+		}
+		if ( i == 12 )
+		{
+			scene->doomedShape = shapeId;
+		}
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+
+	// fast projectiles that cross the thin strip within one step
+	for ( int i = 0; i < 3; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ 8.0f, 1.0f + 1.5f * (float)i };
+		bodyDef.linearVelocity = ( b2Vec2 ){ 110.0f + 7.0f * (float)i, 1.0f };
+		bodyDef.userData = (void*)(intptr_t)( 100 + 2 * i );
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.enableSensorEvents = i != 2;
+		b2Circle circle = { { 0.0f, 0.0f }, 0.1f };
+		b2CreateCircleShape( bodyId, &def, &circle );
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+}
+
+// Before step k (0-based) of B2S_SENSORS.
+static void b2sSensorsBeforeStep( b2sScene* scene, int k )
+{
+	b2BodyId* b = scene->bodies;
+	if ( k == 40 )
+	{
+		b2Shape_EnableSensorEvents( scene->zoneShape, false );
+	}
+	if ( k == 60 )
+	{
+		b2Shape_EnableSensorEvents( scene->zoneShape, true );
+	}
+	if ( k == 70 )
+	{
+		b2DestroyShape( scene->doomedShape, true );
+	}
+	if ( k == 80 )
+	{
+		b2Body_Disable( scene->carrierId );
+	}
+	if ( k == 100 )
+	{
+		b2Body_Enable( scene->carrierId );
+	}
+	if ( k == 110 )
+	{
+		b2DestroyShape( scene->carriedShape, false );
+	}
+	if ( k == 120 )
+	{
+		// a visitor stops sensing, another changes its category into the zone's mask
+		b2Shape_EnableSensorEvents( b2sFirstShape( b[2 + 3] ), false );
+		b2Filter f = b2DefaultFilter();
+		b2Shape_SetFilter( b2sFirstShape( b[2 + 4] ), f );
+	}
+	if ( k == 130 )
+	{
+		// projectiles again
+		for ( int i = 0; i < 3; ++i )
+		{
+			b2BodyId p = b[32 + i];
+			b2Body_SetTransform( p, ( b2Pos ){ 8.0f, 1.0f + 1.5f * (float)i }, b2Rot_identity );
+			b2Body_SetLinearVelocity( p, ( b2Vec2 ){ 115.0f, 0.5f } );
+		}
+	}
+}
+
+static uint32_t b2sFnvShapeId( uint32_t h, b2ShapeId id )
+{
+	return b2sFnvWord( b2sFnvWord( h, (uint32_t)id.index1 ), (uint32_t)id.generation );
+}
+
+static void b2sSensorsProbe( b2sScene* scene )
+{
+	uint32_t h = scene->probeHash;
+	b2SensorEvents ev = b2World_GetSensorEvents( scene->worldId );
+	h = b2sFnvWord( h, (uint32_t)ev.beginCount | (uint32_t)ev.endCount << 16 );
+	for ( int i = 0; i < ev.beginCount; ++i )
+	{
+		h = b2sFnvShapeId( b2sFnvShapeId( h, ev.beginEvents[i].sensorShapeId ), ev.beginEvents[i].visitorShapeId );
+	}
+	for ( int i = 0; i < ev.endCount; ++i )
+	{
+		h = b2sFnvShapeId( b2sFnvShapeId( h, ev.endEvents[i].sensorShapeId ), ev.endEvents[i].visitorShapeId );
+	}
+	b2ShapeId sensors[3] = { scene->zoneShape, scene->stripShape, scene->sweepShape };
+	for ( int s = 0; s < 3; ++s )
+	{
+		b2ShapeId visitors[64];
+		int n = b2Shape_GetSensorData( sensors[s], visitors, 64 );
+		h = b2sFnvWord( h, (uint32_t)n | (uint32_t)b2Shape_GetSensorCapacity( sensors[s] ) << 16 );
+		for ( int i = 0; i < n; ++i )
+		{
+			h = b2sFnvShapeId( h, visitors[i] );
+		}
+	}
+	scene->probeHash = h;
+}
+
 static void b2sBuildSetters( b2sScene* scene, int enableSleep )
 {
 	b2WorldId worldId = scene->worldId;
@@ -1732,6 +1939,10 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 			b2sBuildPreSolve( scene, iparamCount > 0 ? iparams[0] : 0 );
 			break;
 
+		case B2S_SENSORS:
+			b2sBuildSensors( scene, iparamCount > 0 ? iparams[0] : 0 );
+			break;
+
 		case B2S_RAIN:
 		{
 			int count = iparamCount > 0 ? iparams[0] : 20000;
@@ -1805,6 +2016,10 @@ void b2s_step( int handle, float timeStep, int subStepCount, int count )
 		{
 			b2sPreSolveBeforeStep( scene, scene->stepIndex );
 		}
+		else if ( scene->kind == B2S_SENSORS )
+		{
+			b2sSensorsBeforeStep( scene, scene->stepIndex );
+		}
 		else if ( scene->kind == B2S_JOINT_ZOO && scene->zooJointCount > 0 && scene->stepIndex == 150 )
 		{
 			for ( int k = 0; k < scene->zooJointCount; ++k )
@@ -1825,6 +2040,10 @@ void b2s_step( int handle, float timeStep, int subStepCount, int count )
 		else if ( scene->kind == B2S_PRESOLVE )
 		{
 			b2sPreSolveProbe( scene );
+		}
+		else if ( scene->kind == B2S_SENSORS )
+		{
+			b2sSensorsProbe( scene );
 		}
 	}
 }
@@ -1976,6 +2195,30 @@ static uint32_t b2sFloatBits( float x )
 	uint32_t u;
 	memcpy( &u, &x, sizeof( u ) );
 	return u;
+}
+
+int b2s_sensor_events( int handle, uint32_t* out, int capacity )
+{
+	b2sScene* scene = b2sGet( handle );
+	if ( scene == NULL )
+	{
+		return 0;
+	}
+	b2SensorEvents events = b2World_GetSensorEvents( scene->worldId );
+	int n = 0;
+	for ( int k = 0; k < events.beginCount + events.endCount && 5 * ( n + 1 ) <= capacity; ++k, ++n )
+	{
+		bool begin = k < events.beginCount;
+		b2ShapeId sensorId = begin ? events.beginEvents[k].sensorShapeId : events.endEvents[k - events.beginCount].sensorShapeId;
+		b2ShapeId visitorId = begin ? events.beginEvents[k].visitorShapeId : events.endEvents[k - events.beginCount].visitorShapeId;
+		uint32_t* r = out + 5 * n;
+		r[0] = begin ? 0u : 1u;
+		r[1] = (uint32_t)sensorId.index1;
+		r[2] = sensorId.generation;
+		r[3] = (uint32_t)visitorId.index1;
+		r[4] = visitorId.generation;
+	}
+	return n;
 }
 
 int b2s_joint_events( int handle, uint32_t* out, int capacity )

@@ -107,6 +107,18 @@ enum
 	// callback's call count and an order-independent sum of its arguments, the platform's flag
 	// and its contact count into b2s_probe_hash. iparams: [0] enableSleep.
 	B2S_PRESOLVE = 11,
+
+	// Sensors: a static zone (mask excluding category 4) and a thin static strip on a 40 x 1 m
+	// ground box (half extents) with a wall at x = 32, a kinematic sensor circle sweeping across,
+	// a dynamic carrier with a sensor circle, 30 visitors dropped over the zone and the carrier
+	// (two in three sense, every fifth in category 4, every seventh custom-filtered: odd user data
+	// is not sensed) and three projectiles crossing the strip within one step. Before chosen steps
+	// b2s_step toggles the zone's sensor events, destroys a visitor, disables and enables the
+	// carrier, destroys the carried sensor, turns a visitor's events off, changes a filter and
+	// fires the projectiles again. After each step it folds the sensor events and the sensors'
+	// overlaps into b2s_probe_hash. Listed bodies: the kinematic sensor, the carrier, the 30
+	// visitors, the projectiles. iparams: [0] enableSleep.
+	B2S_SENSORS = 12,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.
@@ -167,6 +179,11 @@ B2S_API int b2s_contact_events( int handle, uint32_t* out, int capacity );
 // [joint index1, joint generation, user data as an integer]. Writes at most capacity / 3 records
 // and returns the record count.
 B2S_API int b2s_joint_events( int handle, uint32_t* out, int capacity );
+
+// Sensor events of the last step, begin events then end events as b2World_GetSensorEvents lists
+// them: records of [0 begin / 1 end, sensor index1, sensor generation, visitor index1, visitor
+// generation]. Writes at most capacity / 5 records and returns the record count.
+B2S_API int b2s_sensor_events( int handle, uint32_t* out, int capacity );
 
 // World counters: out[0..5] = bodyCount, shapeCount, contactCount, jointCount, islandCount,
 // awakeContactCount; out[6..29] = colorCounts[0..23]. Returns the number of ints written (30).

@@ -30,6 +30,7 @@ SCENE_EVENTS = 8
 SCENE_EDITS = 9
 SCENE_SETTERS = 10
 SCENE_PRESOLVE = 11
+SCENE_SENSORS = 12
 
 
 class SceneLib:
@@ -49,6 +50,8 @@ class SceneLib:
         lib.b2s_body_count.restype = ctypes.c_int
         lib.b2s_hash.restype = ctypes.c_uint32
         lib.b2s_probe_hash.restype = ctypes.c_uint32
+        lib.b2s_sensor_events.restype = ctypes.c_int
+        lib.b2s_sensor_events.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
         lib.b2s_joint_events.restype = ctypes.c_int
         lib.b2s_joint_events.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
         lib.b2s_state_hash.restype = ctypes.c_uint64
@@ -150,6 +153,13 @@ class SceneLib:
         out = np.zeros(capacity * 10, dtype=np.uint32)
         n = self.lib.b2s_contact_events(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), out.size)
         return out[: 10 * n].reshape(n, 10)
+
+    def sensor_events(self, h, capacity=1 << 14):
+        """The last step's sensor events as a (records, 5) uint32 array: 0 begin / 1 end, sensor
+        index1 and generation, visitor index1 and generation (see b2s_sensor_events)."""
+        out = np.zeros(capacity * 5, dtype=np.uint32)
+        n = self.lib.b2s_sensor_events(h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), out.size)
+        return out[: 5 * n].reshape(n, 5)
 
     def joint_events(self, h, capacity=1 << 14):
         """The last step's joint events as a (records, 3) uint32 array: joint index1, generation,

@@ -22,6 +22,8 @@ B2_API b2ContactEvents b2World_GetContactEvents( b2WorldId worldId );
 /// Joints whose force or torque reached the joint's threshold during the last step (reference:
 /// include/box2d/box2d.h:64).
 B2_API b2JointEvents b2World_GetJointEvents( b2WorldId worldId );
+/// Sensor begin and end overlap events of the last step (reference: include/box2d/box2d.h:58).
+B2_API b2SensorEvents b2World_GetSensorEvents( b2WorldId worldId );
 B2_API void b2World_EnableSleeping( b2WorldId worldId, bool flag );
 B2_API bool b2World_IsSleepingEnabled( b2WorldId worldId );
 B2_API void b2World_EnableContinuous( b2WorldId worldId, bool flag );
@@ -134,6 +136,10 @@ B2_API b2BodyId b2Shape_GetBody( b2ShapeId shapeId );
 B2_API b2AABB b2Shape_GetAABB( b2ShapeId shapeId );
 B2_API void b2Shape_EnableSensorEvents( b2ShapeId shapeId, bool flag );
 B2_API bool b2Shape_AreSensorEventsEnabled( b2ShapeId shapeId );
+/// The shapes a sensor overlapped at the end of the last step, by shape id (reference:
+/// include/box2d/box2d.h:846-856).
+B2_API int b2Shape_GetSensorCapacity( b2ShapeId shapeId );
+B2_API int b2Shape_GetSensorData( b2ShapeId shapeId, b2ShapeId* visitorIds, int capacity );
 /// Pre-solve events of a shape; contacts created afterwards follow the flag (reference:
 /// include/box2d/box2d.h:770-774).
 B2_API void b2Shape_EnablePreSolveEvents( b2ShapeId shapeId, bool flag );

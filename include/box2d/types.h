@@ -611,6 +611,30 @@ typedef struct b2BodyEvents
 	int moveCount;
 } b2BodyEvents;
 
+/// A shape began overlapping a sensor shape (reference: include/box2d/types.h:1028-1037).
+typedef struct b2SensorBeginTouchEvent
+{
+	b2ShapeId sensorShapeId;
+	b2ShapeId visitorShapeId;
+} b2SensorBeginTouchEvent;
+
+/// A shape stopped overlapping a sensor shape, also when either was destroyed or disabled; the ids
+/// may no longer be valid (reference: include/box2d/types.h:1039-1055).
+typedef struct b2SensorEndTouchEvent
+{
+	b2ShapeId sensorShapeId;
+	b2ShapeId visitorShapeId;
+} b2SensorEndTouchEvent;
+
+/// Sensor events of the last step (reference: include/box2d/types.h:1057-1075).
+typedef struct b2SensorEvents
+{
+	b2SensorBeginTouchEvent* beginEvents;
+	b2SensorEndTouchEvent* endEvents;
+	int beginCount;
+	int endCount;
+} b2SensorEvents;
+
 /// A joint whose constraint force or torque reached its threshold in the last step
 /// (reference: include/box2d/types.h:1196-1205).
 typedef struct b2JointEvent

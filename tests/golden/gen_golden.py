@@ -21,7 +21,7 @@ sys.path.insert(0, ROOT)
 
 from box2d_amd.scenes_api import (SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES,  # noqa: E402
                                   SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS, SCENE_EDITS, SCENE_SETTERS,
-                                  SCENE_PRESOLVE)
+                                  SCENE_PRESOLVE, SCENE_SENSORS)
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
 
@@ -160,14 +160,17 @@ def write_joint_events(lib):
 
 
 def write_presolve(lib):
-    """The pre-solve scene (one-way platform) with sleep off and on: per step the transform hash,
-    the probe hash (callback tally, platform flag and contact count) and the body count."""
+    """The pre-solve scene (one-way platform) and the sensor scene, each with sleep off and on: per
+    step the transform hash, the probe hash (the pre-solve callback's tally, the platform's flag
+    and contact count; the sensor events and overlaps) and the body count."""
     path = os.path.join(HERE, "presolve.json")
     with open(path, "w") as f:
         json.dump({"generator": "tests/golden/gen_golden.py",
-                   "reference": "erincatto/box2d v3.2 pre-solve callback (src/contact.c:573-583)",
+                   "reference": "erincatto/box2d v3.2 pre-solve callback (src/contact.c:573-583), sensors (src/sensor.c)",
                    "scenes": [edits_fixture(lib, "presolve", 0, 200, SCENE_PRESOLVE),
-                              edits_fixture(lib, "presolve_sleep", 1, 200, SCENE_PRESOLVE)]},
+                              edits_fixture(lib, "presolve_sleep", 1, 200, SCENE_PRESOLVE),
+                              edits_fixture(lib, "sensors", 0, 200, SCENE_SENSORS),
+                              edits_fixture(lib, "sensors_sleep", 1, 200, SCENE_SENSORS)]},
                   f, separators=(",", ":"))
     print(f"wrote {path} ({os.path.getsize(path)} bytes)")
 

@@ -252,3 +252,28 @@ def test_presolve_callback_matches_reference(gpu_scenes, name, path):
             assert got == fx["per_step"][step - 1], f"{name} step {step}: [transforms, probe, bodies] {got}, reference {fx['per_step'][step - 1]}"
     finally:
         gpu_scenes.destroy(h)
+
+
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("name", ["sensors", "sensors_sleep"])
+def test_sensor_events_match_reference(gpu_scenes, name, path):
+    """Sensors: a static zone with a mask, a thin static strip crossed by projectiles within one
+    step (continuous-collision sensor hits), a kinematic sensor sweeping and a sensor carried by
+    a dynamic body, over visitors with and without sensor events, filtered by category and by the
+    custom filter; sensor events toggled, a visitor and a sensor destroyed, the carrier disabled
+    and enabled, a filter changed. Transforms, every sensor event and the sensors' overlap lists
+    equal the reference's after each of 200 steps (tests/golden/presolve.json)."""
+    import box2d_amd
+    from conftest import load_golden
+
+    fx = next(s for s in load_golden("presolve.json")["scenes"] if s["name"] == name)
+    print(f"[gpu] sensor scene {name} path {path}", flush=True)
+    h = gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"])
+    box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
+    try:
+        for step in range(1, fx["steps"] + 1):
+            gpu_scenes.step(h)
+            got = [gpu_scenes.hash(h), gpu_scenes.probe_hash(h), gpu_scenes.body_count(h)]
+            assert got == fx["per_step"][step - 1], f"{name} step {step}: [transforms, sensors, bodies] {got}, reference {fx['per_step'][step - 1]}"
+    finally:
+        gpu_scenes.destroy(h)
