@@ -17,10 +17,14 @@ the host's cores, timed on a bounded sample of steps.
 """
 
 import argparse
+import csv
 import ctypes
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -49,20 +53,62 @@ def golden_hash(rows, ground, steps):
     return None
 
 
-def measured_traffic(kernel, rows):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy), when one exists for this workload."""
-    if kernel != "relax" or rows != 100:
-        return None
-    path = os.path.join(ROOT, "profiles", "large_pyramid_relax_traffic.md")
+# kernel class (b2amd_KernelStats name) -> kernel symbol in rocprofv3 output
+KERNEL_SYMBOLS = {"relax": "ColorKernel<2>", "solve": "ColorKernel<1>", "color_tail": "ColorTailKernel<2>",
+                  "integrate_velocities_warm_start": "IntegrateWarmStartKernel"}
+COPY_MB = 256
+
+
+def pmc_per_dispatch(counter, child_args, symbol, tmp):
+    """Average of one rocprofv3 counter (KiB per dispatch) over the dispatches of `symbol` in a
+    child run of this script; None when the profiler is missing or the run fails."""
+    out = os.path.join(tmp, counter + "_" + str(abs(hash(tuple(child_args)))))
+    cmd = ["timeout", "-s", "KILL", "90", "rocprofv3", "--pmc", counter, "-f", "csv", "-d", out, "-o", "c", "--",
+           sys.executable, os.path.abspath(__file__)] + child_args
     try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":")[1])
-    except OSError:
-        pass
-    return None
+        subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=tmp, timeout=120, check=True)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    values = []
+    for dirpath, _, files in os.walk(out):
+        for name in files:
+            if name.endswith("counter_collection.csv"):
+                with open(os.path.join(dirpath, name)) as f:
+                    for row in csv.DictReader(f):
+                        if symbol in row.get("Kernel_Name", "") and row.get("Counter_Name") == counter:
+                            values.append(float(row["Counter_Value"]))
+    return sum(values) / len(values) if values else None
+
+
+def live_traffic(kernel, rows, ground, warmup, steps):
+    """HBM bytes per launch of the dominant kernel, measured now: FETCH_SIZE and WRITE_SIZE of the
+    kernel over the same window of the same scene, each scaled by what the counter reads for a
+    plain device copy of COPY_MB MiB (the counters' units differ from bytes by a fixed factor).
+    Returns (bytes or None, details)."""
+    symbol = KERNEL_SYMBOLS.get(kernel)
+    if symbol is None or shutil.which("rocprofv3") is None:
+        return None, {"reason": "no kernel symbol or no rocprofv3"}
+    probe = ["--traffic-probe", "--rows", str(rows), "--ground", str(ground), "--warmup", str(warmup), "--steps",
+             str(steps)]
+    copy = ["--copy-calibration", str(COPY_MB)]
+    copy_kib = COPY_MB * 1024.0
+    tmp = tempfile.mkdtemp(prefix="b2amd_pmc_")
+    try:
+        fetch = pmc_per_dispatch("FETCH_SIZE", probe, symbol, tmp)
+        write = pmc_per_dispatch("WRITE_SIZE", probe, symbol, tmp)
+        copy_fetch = pmc_per_dispatch("FETCH_SIZE", copy, "copyBuffer", tmp)
+        copy_write = pmc_per_dispatch("WRITE_SIZE", copy, "copyBuffer", tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    details = {"kernel_symbol": symbol, "fetch_kib": fetch, "write_kib": write, "copy_mib": COPY_MB,
+               "copy_fetch_kib": copy_fetch, "copy_write_kib": copy_write}
+    if None in (fetch, write, copy_fetch, copy_write) or copy_fetch <= 0 or copy_write <= 0:
+        details["reason"] = "a counter run failed"
+        return None, details
+    details["fetch_scale"] = copy_kib / copy_fetch
+    details["write_scale"] = copy_kib / copy_write
+    traffic = 1024.0 * (fetch * details["fetch_scale"] + write * details["write_scale"])
+    return traffic, details
 
 
 def kernel_stats(lib, world_id):
@@ -340,18 +386,35 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes 10 times (scales rocprofv3 FETCH_SIZE and "
                          "WRITE_SIZE to bytes) and exit")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only step the pyramid warmup + steps times (the child run the counter passes profile)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the live rocprofv3 counter passes")
     args = ap.parse_args()
 
     if args.copy_calibration > 0:
-        import torch
-
-        n = args.copy_calibration * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
+        # a device-to-device copy through the HIP runtime (one copy kernel dispatch each)
+        hip = ctypes.CDLL("libamdhip64.so")
+        n = args.copy_calibration * (1 << 20)
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)):
+            sys.exit("hipMalloc failed")
+        hip.hipMemset(src, 1, ctypes.c_size_t(n))
         for _ in range(10):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_per_dispatch": n * 4, "dispatches": 10}))
+            hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        print(json.dumps({"copy_calibration_bytes_per_dispatch": n, "dispatches": 10}))
+        return
+
+    if args.traffic_probe:
+        import box2d_amd
+
+        scenes = box2d_amd.scenes()
+        h = scenes.pyramid(args.rows, args.ground, sleep=False)
+        scenes.step(h, args.warmup + args.steps, DT, SUBSTEPS)
+        box2d_amd.library().b2amd_Synchronize()
+        scenes.destroy(h)
         return
 
     if args.gpus != 1:
@@ -429,7 +492,7 @@ def main():
         bytes_per_launch = dominant["bytes"] / dominant["launches"]
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant["name"], args.rows),
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                     "kernel": dominant["name"],
                     "avg_launch_us": avg_ms * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch}
     kernels = [{"name": k["name"], "launches_per_step": k["launches"] / min(args.profile_steps, args.steps),
@@ -457,6 +520,15 @@ def main():
     push = None
     if args.push_worlds > 0:
         push = push_workload(lib, args.push_worlds, args.push_steps, min(32, args.push_worlds))
+
+    # HBM traffic of the dominant kernel: rocprofv3 counter passes on child runs of the same
+    # window, after every timed workload so the profiler perturbs none of them
+    if roofline is not None and not args.no_traffic:
+        traffic, details = live_traffic(roofline["kernel"], args.rows, args.ground, args.warmup, args.steps)
+        roofline["traffic"] = traffic
+        roofline["traffic_counters"] = details
+        if traffic is not None:
+            roofline["traffic_over_algorithmic"] = traffic / roofline["algorithmic_bytes_per_launch"]
 
     line = {
         "metric": METRIC,
