@@ -39,7 +39,7 @@ HIP_FLAGS = [
 COMMON = ["-O3", "-fPIC", "-fvisibility=hidden", "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-function",
           "-Wno-missing-braces"]
 
-HOST_SOURCES = ["world.cpp", "tree.cpp", "geometry.cpp", "capi.cpp", "batch.cpp", "host_threads.cpp"]
+HOST_SOURCES = ["world.cpp", "tree.cpp", "geometry.cpp", "capi.cpp", "batch.cpp", "host_threads.cpp", "snapshot.cpp"]
 HIP_SOURCES = ["device.hip", "adjoint.hip", "broadphase.hip"]
 
 

@@ -132,6 +132,36 @@ void b2DestroyWorld( b2WorldId worldId )
 	}
 }
 
+// Snapshots (reference src/world_snapshot.c:477-590, 899-970; this library's own image layout).
+int b2World_Snapshot( b2WorldId worldId, uint8_t* image, int capacity )
+{
+	World* world = GetWorld( worldId );
+	return world != nullptr ? world->Snapshot( image, capacity ) : 0;
+}
+
+bool b2World_Restore( b2WorldId worldId, const uint8_t* image, int size )
+{
+	World* world = GetWorld( worldId );
+	return world != nullptr && world->Restore( image, size );
+}
+
+b2WorldId b2CreateWorldFromSnapshot( const uint8_t* image, int size, int workerCount )
+{
+	b2WorldDef def = b2DefaultWorldDef();
+	def.workerCount = workerCount;
+	b2WorldId id = b2CreateWorld( &def );
+	World* world = GetWorld( id );
+	if ( world == nullptr || world->Restore( image, size ) == false )
+	{
+		if ( world != nullptr )
+		{
+			DestroyWorldInternal( world );
+		}
+		return b2_nullWorldId;
+	}
+	return id;
+}
+
 bool b2World_IsValid( b2WorldId id )
 {
 	return GetWorld( id ) != nullptr;

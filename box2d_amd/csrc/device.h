@@ -130,6 +130,8 @@ public:
 	void UploadBodies( const World& world );
 	void UploadShapes( const World& world );
 	void UploadJoints( const World& world );
+	// The whole contact array from the host copies (a restored world).
+	void UploadContacts( const ContactSim* sims, int count );
 	// Overlaps of the given sensor shapes with every shape in a broadphase tree, after the step's
 	// solve: (index into sensorShapes, shape id), unordered.
 	void SensorOverlaps( const std::vector<int>& sensorShapes, std::vector<IdPair>* overlaps );

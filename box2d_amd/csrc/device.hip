@@ -3499,6 +3499,17 @@ void DeviceWorld::DownloadContacts( int count, ContactSim* out )
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
+void DeviceWorld::UploadContacts( const ContactSim* sims, int count )
+{
+	impl->contacts.Reserve( std::max( count, 1 ), false, impl->stream, 0 );
+	impl->contactCapacityUsed = std::max( impl->contactCapacityUsed, count );
+	if ( count > 0 )
+	{
+		HIP_CHECK( hipMemcpyAsync( impl->contacts.ptr, sims, count * sizeof( ContactSim ), hipMemcpyHostToDevice, impl->stream ) );
+	}
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
 void DeviceWorld::DownloadContact( int contactId, ContactSim* out )
 {
 	HIP_CHECK( hipMemcpyAsync( out, impl->contacts.ptr + contactId, sizeof( ContactSim ), hipMemcpyDeviceToHost, impl->stream ) );

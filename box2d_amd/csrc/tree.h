@@ -132,6 +132,28 @@ public:
 		return nodes;
 	}
 
+	// Every member in a fixed order, for world snapshots (A: the snapshot writer or reader).
+	template <typename A>
+	void Transfer( A& a )
+	{
+		a.Vec( nodes );
+		a.Vec( freeStack );
+		a.Pod( freshIndex );
+		a.Pod( root );
+		a.Pod( nodeCount );
+		a.Pod( proxyCount );
+		a.Pod( structureVersion );
+		a.Pod( boxVersion );
+		a.Vec( pendingEnlarge );
+	}
+
+	// After a restore: versions newer than any a cache may hold.
+	void BumpVersions( uint32_t structureFloor, uint32_t boxFloor )
+	{
+		structureVersion = ( structureVersion > structureFloor ? structureVersion : structureFloor ) + 1;
+		boxVersion = ( boxVersion > boxFloor ? boxVersion : boxFloor ) + 1;
+	}
+
 	int Root() const
 	{
 		return root;

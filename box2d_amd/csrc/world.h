@@ -337,6 +337,27 @@ private:
 	}
 
 public:
+	// Every member in a fixed order, for world snapshots; a restored table is resent whole.
+	template <typename A>
+	void Transfer( A& a )
+	{
+		a.Vec( slots );
+		a.Pod( mask );
+		a.Pod( count );
+		a.Pod( version );
+		if ( A::kReading )
+		{
+			dirty.clear();
+			allDirty = true;
+		}
+	}
+
+	// After a restore: a version newer than any a mirror may hold.
+	void BumpVersion( uint64_t floor )
+	{
+		version = ( version > floor ? version : floor ) + 1;
+	}
+
 	// slots written since the device mirror was last updated (allDirty: the whole table)
 	std::vector<uint64_t> slots;
 	std::vector<uint32_t> dirty;
@@ -435,6 +456,14 @@ public:
 	// Contacts' manifolds and materials as the device last left them (read-only host copy).
 	const ContactSim& PulledContact( int contactId );
 	void PullContacts();
+
+	// Snapshot image of the simulation state (reference src/world_snapshot.c:477-590, 899-970):
+	// returns the image size, writing it when it fits in capacity. Restore returns false and
+	// leaves the world unchanged for an image it rejects.
+	int Snapshot( uint8_t* image, int capacity );
+	bool Restore( const uint8_t* image, int size );
+	template <typename A>
+	void TransferState( A& a );
 	uint64_t StateHash();
 	std::vector<ContactSim> contactSims;
 	bool contactsStale = true;

@@ -61,6 +61,14 @@ B2_API void b2World_SetRestitutionCallback( b2WorldId worldId, b2RestitutionCall
 /// joint impulses, and the set/color/local indices and id pools that drive the solve (reference:
 /// include/box2d/box2d.h:230-235, computed as src/world_snapshot.c:1025-1187 does).
 B2_API uint64_t b2World_GetStateHash( b2WorldId worldId );
+/// Snapshots of a world's simulation state between steps (reference: include/box2d/box2d.h:287-337).
+/// The image layout is this library's own; it round-trips through b2World_Restore and
+/// b2CreateWorldFromSnapshot, and the restored world steps exactly as the original would. User data
+/// and callbacks are not stored. Restore returns false and leaves the world unchanged for an image
+/// it rejects (magic, version, layout or checksum).
+B2_API int b2World_Snapshot( b2WorldId worldId, uint8_t* image, int capacity );
+B2_API bool b2World_Restore( b2WorldId worldId, const uint8_t* image, int size );
+B2_API b2WorldId b2CreateWorldFromSnapshot( const uint8_t* image, int size, int workerCount );
 
 /// Bodies (reference: include/box2d/box2d.h:355-645)
 B2_API b2BodyId b2CreateBody( b2WorldId worldId, const b2BodyDef* def );
