@@ -99,7 +99,8 @@ struct FinalizeResult
 	std::vector<int> enlargedBodies; // awake indices of bodies with an enlarged shape, ascending
 	std::vector<int> enlargedShapes; // shape ids whose fat AABB grew
 	std::vector<b2AABB> enlargedShapeAABBs;
-	std::vector<float> sleepTimes; // per awake index (only when sleep is enabled)
+	std::vector<uint32_t> islandBits; // awake islands by slot in the awake set (sleep enabled)
+	uint64_t splitKey = 0;			  // sleepiest split candidate: sleep time bits << 32 | island id
 	int fastBodyCount = 0;
 	std::vector<int> jointEvents; // ids of joints that reached a force or torque threshold, ascending
 };
@@ -130,6 +131,9 @@ public:
 	void UploadBodies( const World& world );
 	void UploadShapes( const World& world );
 	void UploadJoints( const World& world );
+	// Per awake body: its island's slot in the awake set (negative when the island has removed
+	// constraints) and the island id, read by the finalize kernel's island sleep reduction.
+	void UploadIslands( const std::vector<int>& slots, const std::vector<int>& ids, int islandCount );
 	// The whole contact array from the host copies (a restored world).
 	void UploadContacts( const ContactSim* sims, int count );
 	// Overlaps of the given sensor shapes with every shape in a broadphase tree, after the step's

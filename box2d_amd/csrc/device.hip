@@ -1751,7 +1751,12 @@ struct FinalizeArgs
 	int* enlargedBodies;
 	int* enlargedShapes;
 	b2AABB* enlargedShapeAABBs;
-	float* sleepTimes;
+	// island sleep (null when sleep is off): per awake index the island's slot in the awake set,
+	// negative when the island has removed constraints, and the island id
+	const int* islandSlots;
+	const int* islandIds;
+	uint32_t* islandBits;		  // awake islands, by slot
+	unsigned long long* splitKey; // sleepiest split candidate: sleep time bits << 32 | island id
 };
 
 // Body finalization (reference src/solver.c:553-743).
@@ -1819,9 +1824,22 @@ B2G_DEV void FinalizeBodiesBody( const FinalizeArgs& a, int simIndex )
 
 	aux->sleepTime = sleepTime;
 	aux->flags = bodyFlags;
-	if ( a.sleepTimes != nullptr )
+	// Any body of an island keeps it awake; a sleepy body of an island with removed constraints is
+	// a split candidate, the sleepiest winning and ties going to the larger island id
+	// (reference src/solver.c:676-694).
+	if ( a.islandBits != nullptr )
 	{
-		a.sleepTimes[simIndex] = sleepTime;
+		int slot = a.islandSlots[simIndex];
+		if ( sleepTime < B2_TIME_TO_SLEEP )
+		{
+			int index = slot & 0x7fffffff;
+			atomicOr( a.islandBits + ( index >> 5 ), 1u << ( index & 31 ) );
+		}
+		else if ( slot < 0 )
+		{
+			unsigned long long key = (unsigned long long)__float_as_uint( sleepTime ) << 32 | (uint32_t)a.islandIds[simIndex];
+			atomicMax( a.splitKey, key );
+		}
 	}
 
 	b2Transform transform = sim->transform;
@@ -1914,6 +1932,7 @@ struct ContinuousArgs
 
 constexpr int kMaxContinuousSensorHits = 8;
 constexpr int kSensorHitCountSlot = 9;	   // device counter slots
+constexpr int kSplitKeySlot = 14;		   // 64-bit, slots 14 and 15
 constexpr int kSensorOverlapCountSlot = 10;
 
 B2G_DEV bool ShouldShapesCollide( const b2Filter& a, const b2Filter& b )
@@ -2609,7 +2628,9 @@ struct PackFinalizeArgs
 	const int* enlargedBodies;
 	const int* enlargedShapes;
 	const b2AABB* enlargedShapeAABBs;
-	const float* sleepTimes; // null when not needed
+	uint32_t* islandBits;		  // awake islands (null when sleep is off): copied, then cleared
+	unsigned long long* splitKey; // likewise
+	int islandWords;
 	int* keptShapeCount;	 // receives the enlarged-shape count before it is cleared (may be null)
 	const int* xcdSync;		 // XCD solver barrier words of this step (null when it did not run)
 	JointSim* joints;		 // joint events: flagged joints are listed and their flags cleared
@@ -2642,12 +2663,20 @@ B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
 		dst[2] = box.upperBound.x;
 		dst[3] = box.upperBound.y;
 	}
-	if ( a.sleepTimes != nullptr )
+	if ( a.islandBits != nullptr )
 	{
-		float* dst = reinterpret_cast<float*>( a.mail + a.sleepOffset );
-		for ( int i = threadIdx.x; i < a.bodyCount; i += blockDim.x )
+		uint32_t* dst = reinterpret_cast<uint32_t*>( a.mail + a.sleepOffset );
+		for ( int i = threadIdx.x; i < a.islandWords; i += blockDim.x )
 		{
-			dst[i] = a.sleepTimes[i];
+			dst[i] = a.islandBits[i];
+			a.islandBits[i] = 0;
+		}
+		if ( threadIdx.x == 0 )
+		{
+			unsigned long long key = *a.splitKey;
+			dst[a.islandWords] = (uint32_t)key;
+			dst[a.islandWords + 1] = (uint32_t)( key >> 32 );
+			*a.splitKey = 0;
 		}
 	}
 
@@ -2848,7 +2877,11 @@ struct DeviceWorld::Impl
 	std::vector<int2> ccdNoCollideStaging;
 	DevArray<int> enlargedShapes;
 	DevArray<b2AABB> enlargedShapeAABBs;
-	DevArray<float> sleepTimes;
+	DevArray<int> islandSlots;
+	DevArray<int> islandIds;
+	DevArray<uint32_t> islandBits; // kept zero between steps by the pack kernel
+	size_t islandBitsZeroed = 0;   // words known zero
+	int islandCount = 0;
 	DevArray<int> hitCandidates;
 	int solveHitCapacity = 0;
 
@@ -3136,7 +3169,9 @@ DeviceWorld::~DeviceWorld()
 	impl->ccdFilterRejects.Free();
 	impl->enlargedShapes.Free();
 	impl->enlargedShapeAABBs.Free();
-	impl->sleepTimes.Free();
+	impl->islandSlots.Free();
+	impl->islandIds.Free();
+	impl->islandBits.Free();
 	(void)hipFree( impl->counters );
 	(void)hipHostFree( impl->hostCounters );
 	if ( impl->mail != nullptr )
@@ -3555,7 +3590,7 @@ void DeviceWorld::UploadLists( const SolverLists& lists )
 	size_t nb = lists.awakeBodies.size();
 	impl->enlargedBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
 	impl->fastBodies.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
-	impl->sleepTimes.Reserve( std::max<size_t>( nb, 1 ), false, impl->stream, 0 );
+
 	// at most two entries per contact (a touch change and a mixing refresh), then the contacts
 	// deferred to the host's pre-solve callback
 	impl->changes.Reserve( std::max<size_t>( 3 * lists.collideContacts.size(), 1 ), false, impl->stream, 0 );
@@ -4266,7 +4301,11 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	fa.enlargedBodies = impl->enlargedBodies.ptr;
 	fa.enlargedShapes = impl->enlargedShapes.ptr;
 	fa.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
-	fa.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
+	const bool islands = needSleepTimes && impl->islandCount > 0;
+	fa.islandSlots = islands ? impl->islandSlots.ptr : nullptr;
+	fa.islandIds = islands ? impl->islandIds.ptr : nullptr;
+	fa.islandBits = islands ? impl->islandBits.ptr : nullptr;
+	fa.splitKey = reinterpret_cast<unsigned long long*>( impl->counters + kSplitKeySlot );
 	if ( rec )
 	{
 		impl->rec.finalize = true;
@@ -4330,14 +4369,16 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		pk.enlargedBodies = impl->enlargedBodies.ptr;
 		pk.enlargedShapes = impl->enlargedShapes.ptr;
 		pk.enlargedShapeAABBs = impl->enlargedShapeAABBs.ptr;
-		pk.sleepTimes = needSleepTimes ? impl->sleepTimes.ptr : nullptr;
+		pk.islandBits = islands ? impl->islandBits.ptr : nullptr;
+		pk.splitKey = reinterpret_cast<unsigned long long*>( impl->counters + kSplitKeySlot );
+		pk.islandWords = islands ? ( impl->islandCount + 31 ) / 32 : 0;
 		pk.keptShapeCount = rec ? nullptr : impl->counters + kKeptShapeCountSlot;
 		pk.xcdSync = xcd ? impl->xcdSync : nullptr;
 		pk.bodyCount = nb;
 		pk.shapeOffset = kMailHeader + nb;
 		pk.aabbOffset = pk.shapeOffset + (int)shapeCap;
 		pk.sleepOffset = pk.aabbOffset + 4 * (int)shapeCap;
-		pk.hitOffset = pk.sleepOffset + nb;
+		pk.hitOffset = pk.sleepOffset + std::max( nb, pk.islandWords + 2 );
 		pk.joints = impl->joints.ptr;
 		pk.jointCount = params.jointEvents != 0 ? impl->jointCount : 0;
 		pk.jointEventOffset = pk.hitOffset + kHitRecordInts * std::max( nc, 0 );
@@ -4381,6 +4422,21 @@ bool DeviceWorld::GetSolvePairInputs( SolvePairInputs* out ) const
 	out->capacity = (int)impl->enlargedShapes.capacity;
 	out->ready = impl->solvePacked;
 	return true;
+}
+
+void DeviceWorld::UploadIslands( const std::vector<int>& slots, const std::vector<int>& ids, int islandCount )
+{
+	impl->StageVector( impl->islandSlots, slots );
+	impl->StageVector( impl->islandIds, ids );
+	size_t words = ( (size_t)islandCount + 31 ) / 32;
+	impl->islandBits.Reserve( std::max<size_t>( words, 1 ), false, impl->stream, 0 );
+	if ( impl->islandBits.capacity > impl->islandBitsZeroed )
+	{
+		// the pack kernel clears the words it reads; fresh storage starts at zero once
+		HIP_CHECK( hipMemsetAsync( impl->islandBits.ptr, 0, impl->islandBits.capacity * sizeof( uint32_t ), impl->stream ) );
+		impl->islandBitsZeroed = impl->islandBits.capacity;
+	}
+	impl->islandCount = islandCount;
 }
 
 void DeviceWorld::LaunchDeferredSolve()
@@ -4429,10 +4485,13 @@ void DeviceWorld::SolveFinish( FinalizeResult* result )
 		{
 			memcpy( result->enlargedShapeAABBs.data(), mail + pk.aabbOffset, enlargedShapeCount * sizeof( b2AABB ) );
 		}
-		if ( needSleepTimes && nb > 0 )
+		result->islandBits.clear();
+		result->splitKey = 0;
+		if ( pk.islandBits != nullptr )
 		{
-			result->sleepTimes.resize( nb );
-			memcpy( result->sleepTimes.data(), mail + pk.sleepOffset, nb * sizeof( float ) );
+			const uint32_t* bits = reinterpret_cast<const uint32_t*>( mail + pk.sleepOffset );
+			result->islandBits.assign( bits, bits + pk.islandWords );
+			result->splitKey = (uint64_t)bits[pk.islandWords] | (uint64_t)bits[pk.islandWords + 1] << 32;
 		}
 		std::sort( result->enlargedBodies.begin(), result->enlargedBodies.end() );
 

@@ -386,7 +386,7 @@ bool World::Restore( const uint8_t* image, int size )
 	ccdFilterRejects.clear();
 
 	// every device copy is resent before the next step
-	bodiesDirty = shapesDirty = jointsDirty = listsDirty = true;
+	bodiesDirty = shapesDirty = jointsDirty = listsDirty = islandsDirty = true;
 	bodiesStale = shapesStale = jointsStale = false;
 	solverLists = SolverLists{};
 	listsStableSteps = 0;

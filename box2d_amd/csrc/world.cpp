@@ -664,6 +664,7 @@ uint64_t World::StateHash()
 
 Island* World::CreateIsland( int setIndex )
 {
+	islandsDirty = true;
 	int islandId = islandIdPool.Alloc();
 	if ( islandId == (int)islands.size() )
 	{
@@ -685,6 +686,7 @@ Island* World::CreateIsland( int setIndex )
 
 void World::DestroyIsland( int islandId )
 {
+	islandsDirty = true;
 	if ( splitIslandId == islandId )
 	{
 		splitIslandId = B2_NULL_INDEX;
@@ -711,6 +713,7 @@ void World::DestroyIsland( int islandId )
 
 void World::CreateIslandForBody( int setIndex, Body* body )
 {
+	islandsDirty = true;
 	Island* island = CreateIsland( setIndex );
 	island->bodies.push_back( body->id );
 	body->islandId = island->islandId;
@@ -719,6 +722,7 @@ void World::CreateIslandForBody( int setIndex, Body* body )
 
 int World::MergeIslands( int islandIdA, int islandIdB )
 {
+	islandsDirty = true;
 	if ( islandIdA == islandIdB )
 	{
 		return islandIdA;
@@ -780,6 +784,7 @@ int World::MergeIslands( int islandIdA, int islandIdB )
 
 void World::AddContactToIsland( int islandId, Contact* contact )
 {
+	islandsDirty = true;
 	Island* island = &islands[islandId];
 	contact->islandId = islandId;
 	contact->islandIndex = (int)island->contacts.size();
@@ -808,6 +813,7 @@ void World::LinkContact( Contact* contact )
 
 void World::UnlinkContact( Contact* contact )
 {
+	islandsDirty = true;
 	int islandId = contact->islandId;
 	Island* island = &islands[islandId];
 	int removeIndex = contact->islandIndex;
@@ -823,6 +829,7 @@ void World::UnlinkContact( Contact* contact )
 
 void World::AddJointToIsland( int islandId, Joint* joint )
 {
+	islandsDirty = true;
 	Island* island = &islands[islandId];
 	joint->islandId = islandId;
 	joint->islandIndex = (int)island->joints.size();
@@ -861,6 +868,7 @@ static inline int IslandFindParent( std::vector<int>& parents, int node )
 // Union-find split of an island whose constraints were removed (reference src/island.c:389-651).
 void World::SplitIsland( int baseId )
 {
+	islandsDirty = true;
 	Island* baseIsland = &islands[baseId];
 	int baseBodyCount = (int)baseIsland->bodies.size();
 	std::vector<int> baseBodyIds = baseIsland->bodies;
@@ -1023,6 +1031,7 @@ void World::DestroySolverSet( int setIndex )
 
 void World::WakeSolverSet( int setIndex )
 {
+	islandsDirty = true;
 	// Waking rewrites body states, so the host copy must be current first.
 	PullBodies();
 
@@ -1110,6 +1119,7 @@ void World::RemoveBodyFromAwakeSet( Body* body )
 
 void World::TrySleepIsland( int islandId )
 {
+	islandsDirty = true;
 	Island* island = &islands[islandId];
 	if ( island->constraintRemoveCount > 0 && island->bodies.size() > 1 )
 	{
@@ -2014,6 +2024,7 @@ void World::DestroyContact( Contact* contact )
 
 void World::UnlinkJoint( Joint* joint )
 {
+	islandsDirty = true;
 	if ( joint->islandId == B2_NULL_INDEX )
 	{
 		return;
@@ -2032,6 +2043,7 @@ void World::UnlinkJoint( Joint* joint )
 
 void World::RemoveBodyFromIsland( Body* body )
 {
+	islandsDirty = true;
 	if ( body->islandId == B2_NULL_INDEX )
 	{
 		return;
@@ -4077,6 +4089,8 @@ static void BuildOverflowPlan( World& w, SolverLists& L )
 static void BuildSolverLists( World& w, SolverLists& L )
 {
 	L = SolverLists{};
+	// the awake order may have changed: the island slots go up again
+	w.islandsDirty = true;
 	L.awakeBodies = w.solverSets[kAwakeSet].bodyIds;
 
 	for ( int i = 0; i < kGraphColorCount; ++i )
@@ -4607,12 +4621,9 @@ bool World::SolveBegin( StepParams& params )
 		listsStableSteps = 0;
 	}
 
-	constraintTicks = b2GetTicks();
-	device->SolveLaunch( params, restitutionShapeCount > 0, enableSleep );
-
-	// The island split only changes island bookkeeping, which the GPU solve does not read, so it
-	// runs while the GPU solves (the reference runs it beside the solver stages,
-	// src/solver.c:1475-1494); island sleep reads its result after the solve.
+	// The island split changes only island bookkeeping, which the solve does not read, but the
+	// finalize kernel's island sleep reduction must see the split islands, as the reference's
+	// finalize does (src/solver.c:1475-1494): the split runs before the launch.
 	uint64_t splitTicks = b2GetTicks();
 	if ( splitIslandId != B2_NULL_INDEX )
 	{
@@ -4620,6 +4631,26 @@ bool World::SolveBegin( StepParams& params )
 		splitIslandId = B2_NULL_INDEX;
 	}
 	profile.splitIslands = b2GetMilliseconds( splitTicks );
+
+	// each awake body's island slot and id for the finalize kernel, resent when islands or the
+	// awake order changed
+	if ( enableSleep && islandsDirty )
+	{
+		const std::vector<int>& awake = awakeSet.bodyIds;
+		islandSlotStaging.resize( awake.size() );
+		islandIdStaging.resize( awake.size() );
+		for ( size_t i = 0; i < awake.size(); ++i )
+		{
+			const Island& island = islands[bodies[awake[i]].islandId];
+			islandSlotStaging[i] = island.localIndex | ( island.constraintRemoveCount > 0 ? (int)0x80000000u : 0 );
+			islandIdStaging[i] = island.islandId;
+		}
+		device->UploadIslands( islandSlotStaging, islandIdStaging, (int)awakeSet.islandIds.size() );
+		islandsDirty = false;
+	}
+
+	constraintTicks = b2GetTicks();
+	device->SolveLaunch( params, restitutionShapeCount > 0, enableSleep );
 	return true;
 }
 
@@ -4714,38 +4745,21 @@ void World::SolveEnd()
 		profile.refit = b2GetMilliseconds( refitTicks );
 	}
 
-	// Island sleep (reference src/solver.c:2024-2080)
+	// Island sleep (reference src/solver.c:2024-2080): the finalize kernel marked the islands a
+	// body keeps awake and picked the sleepiest island that needs a split
 	if ( enableSleep )
 	{
 		uint64_t sleepTicks = b2GetTicks();
 		int islandCount = (int)awakeSet.islandIds.size();
-		std::vector<char> awakeIslands( islandCount, 0 );
-		float splitSleepTimer = 0.0f;
-		int candidate = B2_NULL_INDEX;
-		for ( int i = 0; i < awakeBodyCount; ++i )
+		if ( (int)fr.islandBits.size() * 32 < islandCount )
 		{
-			int bodyId = moveEventBodyIds[i];
-			Body& body = bodies[bodyId];
-			float sleepTime = fr.sleepTimes[i];
-			Island& island = islands[body.islandId];
-			if ( sleepTime < B2_TIME_TO_SLEEP )
-			{
-				awakeIslands[island.localIndex] = 1;
-			}
-			else if ( island.constraintRemoveCount > 0 )
-			{
-				if ( sleepTime > splitSleepTimer || ( sleepTime == splitSleepTimer && body.islandId > candidate ) )
-				{
-					candidate = body.islandId;
-					splitSleepTimer = sleepTime;
-				}
-			}
+			Fatal( "island sleep: %d island bits read back for %d awake islands", (int)fr.islandBits.size() * 32, islandCount );
 		}
-		splitIslandId = candidate;
+		splitIslandId = fr.splitKey != 0 ? (int)(uint32_t)fr.splitKey : B2_NULL_INDEX;
 
 		for ( int islandIndex = islandCount - 1; islandIndex >= 0; islandIndex -= 1 )
 		{
-			if ( awakeIslands[islandIndex] )
+			if ( ( fr.islandBits[islandIndex >> 5] >> ( islandIndex & 31 ) ) & 1u )
 			{
 				continue;
 			}

@@ -552,6 +552,9 @@ public:
 	bool jointsStale = false;
 	bool listsDirty = true; // solver work lists (awake bodies, colors, contacts) changed
 	SolverLists solverLists; // the lists last uploaded
+	bool islandsDirty = true; // island membership, slots or removed-constraint counts changed
+	std::vector<int> islandSlotStaging;
+	std::vector<int> islandIdStaging;
 	int listsStableSteps = 0; // steps since the lists last changed
 	int restitutionShapeCount = 0;
 
