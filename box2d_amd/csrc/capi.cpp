@@ -908,6 +908,23 @@ bool b2Body_IsSleepEnabled( b2BodyId bodyId )
 	return world != nullptr && ( body->flags & kEnableSleep ) != 0;
 }
 
+void b2Body_SetBullet( b2BodyId bodyId, bool flag )
+{
+	Body* body;
+	World* world = UnlockedBody( bodyId, &body );
+	if ( world != nullptr )
+	{
+		world->SetBodyBullet( body, flag );
+	}
+}
+
+bool b2Body_IsBullet( b2BodyId bodyId )
+{
+	Body* body;
+	World* world = WorldFromBody( bodyId, &body );
+	return world != nullptr && ( body->flags & kIsBullet ) != 0;
+}
+
 void b2Body_SetAwake( b2BodyId bodyId, bool awake )
 {
 	Body* body;

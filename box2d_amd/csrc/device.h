@@ -146,6 +146,14 @@ public:
 	// the updated contacts go back to the device and their change records are appended.
 	void CollideOnHost( World& world, const std::vector<int>& contactIds, const StepParams& params,
 						std::vector<int>* changes );
+	// Continuous collision of one fast body on the host copies of bodies and shapes (current there)
+	// with the world's custom filter and pre-continuous callbacks: against the static tree, and for
+	// a bullet also against the kinematic and dynamic trees (flushed by the caller). Enlarged shapes
+	// get enlargedAABB and the body kEnlargeBounds; sensor hits are appended as (sensor shape, fast
+	// shape). The device copies are not touched.
+	static void ContinuousOnHost( World& world, int bodyId, std::vector<IdPair>* sensorHits );
+	// The given bodies' sims and all their shapes, from the host copies.
+	void UploadBodyShapes( const World& world, const std::vector<int>& bodyIds );
 	// Sorted (static shape << 32 | fast shape) keys the custom filter rejects for continuous collision.
 	void UploadContinuousFilterRejects( const std::vector<uint64_t>& keys );
 	void UploadContactInits( const std::vector<ContactInit>& inits, int contactCapacity );

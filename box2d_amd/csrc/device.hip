@@ -2075,6 +2075,11 @@ B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 
 	int bodyId = a.awake[a.fastBodies[k]];
 	BodySim* fastSim = a.sims + bodyId;
+	if ( fastSim->flags & kIsBullet )
+	{
+		// bullets sweep after the refit, against the moving bodies too (the host does them)
+		return;
+	}
 
 	b2Vec2 base = fastSim->center0;
 	b2Sweep sweep = RelativeSweep( *fastSim, base );
@@ -3746,6 +3751,234 @@ void DeviceWorld::CollideOnHost( World& world, const std::vector<int>& contactId
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
 }
 
+// Continuous collision of one fast body on the host (reference src/solver.c:181-550): the callback
+// tests of b2ContinuousQueryCallback in its order, including the two the device kernel cannot run
+// (the custom filter and the pre-continuous callback, called here as the reference calls them),
+// and the bullet branch that also sweeps the kinematic and dynamic trees, skipping other bullets.
+// Reads and writes the host copies of bodies and shapes.
+void DeviceWorld::ContinuousOnHost( World& world, int bodyId, std::vector<IdPair>* sensorHits )
+{
+	BodySim* fastSim = &world.bodySims[bodyId];
+	const bool isBullet = ( fastSim->flags & kIsBullet ) != 0;
+	b2Vec2 base = fastSim->center0;
+	b2Sweep sweep = RelativeSweep( *fastSim, base );
+
+	b2Transform xf1;
+	xf1.q = sweep.q1;
+	xf1.p = b2Sub( sweep.c1, b2RotateVector( sweep.q1, sweep.localCenter ) );
+	b2Transform xf2;
+	xf2.q = sweep.q2;
+	xf2.p = b2Sub( sweep.c2, b2RotateVector( sweep.q2, sweep.localCenter ) );
+
+	float fraction = 1.0f;
+	IdPair hits[kMaxContinuousSensorHits];
+	float hitFractions[kMaxContinuousSensorHits];
+	int hitCount = 0;
+	Body* fastBody = &world.bodies[bodyId];
+
+	for ( int shapeId = fastSim->headShapeId; shapeId != B2_NULL_INDEX; shapeId = world.shapeData[shapeId].nextShapeId )
+	{
+		ShapeDev* fastShape = &world.shapeData[shapeId];
+		const Shape& fastHost = world.shapes[shapeId];
+		b2Vec2 centroid1 = b2TransformPoint( xf1, fastShape->localCentroid );
+		b2Vec2 centroid2 = b2TransformPoint( xf2, fastShape->localCentroid );
+
+		b2AABB box1 = fastShape->aabb;
+		b2AABB box2 = OffsetAABB( FatShapeAABB( *fastShape, xf2, 0.0f ), base );
+		fastShape->aabb = box2;
+		if ( fastShape->flags & kShapeIsSensor )
+		{
+			continue;
+		}
+		b2AABB swept = b2AABB_Union( box1, box2 );
+
+		auto visit = [&]( int, uint64_t userData ) -> bool {
+			int otherId = (int)userData;
+			if ( otherId == shapeId )
+			{
+				return true;
+			}
+			const ShapeDev& other = world.shapeData[otherId];
+			const Shape& otherHost = world.shapes[otherId];
+			if ( other.bodyId == fastShape->bodyId )
+			{
+				return true;
+			}
+			const bool otherSensor = ( other.flags & kShapeIsSensor ) != 0;
+			if ( otherSensor && ( other.flags & fastShape->flags & kShapeEnableSensorEvents ) == 0 )
+			{
+				return true;
+			}
+			if ( ShouldShapesCollide( fastShape->filter, other.filter ) == false )
+			{
+				return true;
+			}
+			const BodySim& otherSim = world.bodySims[other.bodyId];
+			if ( otherSim.flags & kIsBullet )
+			{
+				return true;
+			}
+			if ( world.ShouldBodiesCollide( fastBody, &world.bodies[other.bodyId] ) == false )
+			{
+				return true;
+			}
+			if ( ( otherHost.enableCustomFiltering || fastHost.enableCustomFiltering ) && world.customFilterFcn != nullptr &&
+				 world.customFilterFcn( world.ShapeIdOf( otherId ), world.ShapeIdOf( shapeId ), world.customFilterContext ) == false )
+			{
+				return true;
+			}
+
+			if ( other.type == b2_chainSegmentShape )
+			{
+				b2Transform rel = { b2SubPos( otherSim.transform.p, base ), otherSim.transform.q };
+				b2Vec2 p1 = b2TransformPoint( rel, other.chainSegment.segment.point1 );
+				b2Vec2 p2 = b2TransformPoint( rel, other.chainSegment.segment.point2 );
+				float length;
+				b2Vec2 e = b2GetLengthAndNormalize( &length, b2Sub( p2, p1 ) );
+				if ( length > kLinearSlop )
+				{
+					float separation1 = b2Cross( b2Sub( centroid1, p1 ), e );
+					float separation2 = b2Cross( b2Sub( centroid2, p1 ), e );
+					float coreDistance = kCoreFraction * fastSim->minExtent;
+					if ( separation1 < 0.0f || ( separation1 - separation2 < coreDistance && separation2 > coreDistance ) )
+					{
+						return true;
+					}
+				}
+			}
+
+			b2TOIInput input;
+			input.proxyA = ShapeProxy( other );
+			input.proxyB = ShapeProxy( *fastShape );
+			input.sweepA = RelativeSweep( otherSim, base );
+			input.sweepB = sweep;
+			input.maxFraction = fraction;
+
+			b2TOIOutput output = TimeOfImpact( input );
+			if ( otherSensor )
+			{
+				if ( output.fraction <= fraction && hitCount < kMaxContinuousSensorHits )
+				{
+					hits[hitCount] = IdPair{ otherId, shapeId };
+					hitFractions[hitCount] = output.fraction;
+					hitCount += 1;
+				}
+				return true;
+			}
+
+			float hitFraction = fraction;
+			bool didHit = false;
+			if ( 0.0f < output.fraction && output.fraction < fraction )
+			{
+				hitFraction = output.fraction;
+				didHit = true;
+			}
+			else if ( 0.0f == output.fraction )
+			{
+				b2Vec2 centroid = ShapeCentroidDev( *fastShape );
+				float radius = kCoreFraction * ShapeMinExtent( *fastShape, B2_HUGE );
+				input.proxyB = MakeProxy( &centroid, 1, radius );
+				output = TimeOfImpact( input );
+				if ( 0.0f < output.fraction && output.fraction < fraction )
+				{
+					hitFraction = output.fraction;
+					didHit = true;
+				}
+			}
+
+			if ( didHit && ( otherHost.enablePreSolveEvents || fastHost.enablePreSolveEvents ) && world.preContinuousFcn != nullptr )
+			{
+				// the time of impact is in the base frame: the callback gets the world point
+				b2Vec2 worldPoint = b2OffsetPos( base, output.point );
+				didHit = world.preContinuousFcn( world.ShapeIdOf( otherId ), world.ShapeIdOf( shapeId ), worldPoint, output.normal,
+												 world.preSolveContext );
+			}
+
+			if ( didHit )
+			{
+				fastSim->flags |= kHadTimeOfImpact;
+				fraction = hitFraction;
+			}
+			return true;
+		};
+
+		world.broadPhase.trees[b2_staticBody].Query( swept, B2_DEFAULT_MASK_BITS, visit );
+		if ( isBullet )
+		{
+			world.broadPhase.trees[b2_kinematicBody].Query( swept, B2_DEFAULT_MASK_BITS, visit );
+			world.broadPhase.trees[b2_dynamicBody].Query( swept, B2_DEFAULT_MASK_BITS, visit );
+		}
+	}
+
+	auto enlarge = [&]( ShapeDev* shape ) {
+		if ( b2AABB_Contains( shape->fatAABB, shape->aabb ) == false )
+		{
+			float margin = shape->aabbMargin;
+			b2AABB fat;
+			fat.lowerBound.x = shape->aabb.lowerBound.x - margin;
+			fat.lowerBound.y = shape->aabb.lowerBound.y - margin;
+			fat.upperBound.x = shape->aabb.upperBound.x + margin;
+			fat.upperBound.y = shape->aabb.upperBound.y + margin;
+			shape->fatAABB = fat;
+			shape->enlargedAABB = 1;
+			fastSim->flags |= kEnlargeBounds;
+		}
+	};
+
+	if ( fraction < 1.0f )
+	{
+		b2Rot q = b2NLerp( sweep.q1, sweep.q2, fraction );
+		b2Vec2 c = b2Lerp( sweep.c1, sweep.c2, fraction );
+		b2Vec2 origin = b2Sub( c, b2RotateVector( q, sweep.localCenter ) );
+
+		fastSim->transform.q = q;
+		fastSim->transform.p = b2OffsetPos( base, origin );
+		fastSim->center = b2OffsetPos( base, c );
+		fastSim->rotation0 = q;
+		fastSim->center0 = fastSim->center;
+
+		b2Transform transform = fastSim->transform;
+		for ( int shapeId = fastSim->headShapeId; shapeId != B2_NULL_INDEX; shapeId = world.shapeData[shapeId].nextShapeId )
+		{
+			ShapeDev* shape = &world.shapeData[shapeId];
+			shape->aabb = FatShapeAABB( *shape, transform, kSpeculativeDistance );
+			enlarge( shape );
+		}
+	}
+	else
+	{
+		fastSim->rotation0 = fastSim->transform.q;
+		fastSim->center0 = fastSim->center;
+		for ( int shapeId = fastSim->headShapeId; shapeId != B2_NULL_INDEX; shapeId = world.shapeData[shapeId].nextShapeId )
+		{
+			enlarge( &world.shapeData[shapeId] );
+		}
+	}
+
+	for ( int i = 0; i < hitCount; ++i )
+	{
+		if ( hitFractions[i] < fraction )
+		{
+			sensorHits->push_back( hits[i] );
+		}
+	}
+}
+
+void DeviceWorld::UploadBodyShapes( const World& world, const std::vector<int>& bodyIds )
+{
+	for ( int bodyId : bodyIds )
+	{
+		HIP_CHECK( hipMemcpyAsync( impl->sims.ptr + bodyId, world.bodySims.data() + bodyId, sizeof( BodySim ), hipMemcpyHostToDevice,
+								   impl->stream ) );
+		for ( int shapeId = world.bodySims[bodyId].headShapeId; shapeId != B2_NULL_INDEX; shapeId = world.shapeData[shapeId].nextShapeId )
+		{
+			HIP_CHECK( hipMemcpyAsync( impl->shapes.ptr + shapeId, world.shapeData.data() + shapeId, sizeof( ShapeDev ),
+									   hipMemcpyHostToDevice, impl->stream ) );
+		}
+	}
+	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
 void DeviceWorld::Collide( const StepParams& params, CollideResult* result )
 {
 	CollideLaunch( params );
@@ -4317,7 +4550,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		impl->Launch( kkFinalize, kBytesFinalize * nb, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<FinalizeBodiesOp>, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, fa ); } );
 	}
 
-	if ( params.enableContinuous )
+	if ( params.enableContinuous && params.hostContinuous == 0 )
 	{
 		// The fast-body count is only known on the device: size the grid for every awake body.
 		ContinuousArgs ka;
@@ -4356,6 +4589,10 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		{
 			impl->Launch( kkContinuous, 0.0, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<ContinuousOp>, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, ka ); } );
 		}
+	}
+	else
+	{
+		impl->sensorHitCapacity = 0;
 	}
 	if ( rec == false )
 	{

@@ -359,6 +359,9 @@ struct StepParams
 	int preSolve;
 	// the world has a sensor on a static body: continuous collision records its hits
 	int sensorHits;
+	// continuous collision runs on the host for every fast body (a pre-continuous callback with
+	// shapes that enable pre-solve events): the device kernel is not launched
+	int hostContinuous;
 };
 
 } // namespace b2g

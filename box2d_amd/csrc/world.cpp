@@ -1386,10 +1386,7 @@ b2BodyId World::CreateBody( const b2BodyDef* def )
 	sim.angularDamping = def->angularDamping;
 	sim.gravityScale = def->gravityScale;
 	sim.flags = lockFlags;
-	if ( def->isBullet )
-	{
-		Fatal( "bullet bodies (continuous collision against moving bodies) are not supported by the GPU step" );
-	}
+	sim.flags |= def->isBullet ? kIsBullet : 0;
 	sim.flags |= def->allowFastRotation ? kAllowFastRotation : 0;
 	sim.flags |= def->type == b2_dynamicBody ? kDynamicFlag : 0;
 	sim.flags |= def->enableSleep ? kEnableSleep : 0;
@@ -2767,6 +2764,18 @@ void World::EnableBodySleep( Body* body, bool enable )
 	{
 		WakeBody( body );
 	}
+}
+
+// b2Body_SetBullet (reference src/body.c:1922-1944)
+void World::SetBodyBullet( Body* body, bool flag )
+{
+	uint32_t newFlag = flag ? kIsBullet : 0;
+	if ( ( body->flags & kIsBullet ) == newFlag )
+	{
+		return;
+	}
+	body->flags = ( body->flags & ~kIsBullet ) | newFlag;
+	SyncBodyFlags( body );
 }
 
 // b2Body_SetAwake (reference src/body.c:1550-1575)
@@ -4305,16 +4314,21 @@ bool World::StepBegin( float timeStep, int subStepCount )
 			break;
 		}
 	}
+	// the pre-continuous callback decides hits inside the sweep: with it, every fast body sweeps
+	// on the host
+	hostContinuousStep = false;
 	if ( preContinuousFcn != nullptr && enableContinuous )
 	{
 		for ( const Shape& shape : shapes )
 		{
 			if ( shape.id != B2_NULL_INDEX && shape.enablePreSolveEvents )
 			{
-				Fatal( "the pre-continuous callback is not supported by the GPU step (continuous collision runs on the GPU)" );
+				hostContinuousStep = true;
+				break;
 			}
 		}
 	}
+	params.hostContinuous = hostContinuousStep ? 1 : 0;
 	device->UploadContactInits( pendingContactInits, contactIdPool.Capacity() );
 	pendingContactInits.clear();
 	BuildContinuousFilterRejects();
@@ -4727,9 +4741,55 @@ void World::SolveEnd()
 			}
 		}
 
+		// Fast bodies the device left to the host: bullets (their moves are buffered here, in body
+		// order, and they sweep after the refit; reference src/solver.c:1871-1885) and, in a
+		// host-continuous step, every other fast body, which sweeps against the static tree here
+		// as the reference's finalize does before its refit (src/solver.c:648-660).
+		std::vector<int> bullets, hostSwept;
+		std::vector<IdPair> hostSensorHits;
+		bool pulled = false;
 		for ( int awakeIndex : fr.enlargedBodies )
 		{
 			int bodyId = moveEventBodyIds[awakeIndex];
+			if ( ( bodies[bodyId].flags & kIsBullet ) != 0 || hostContinuousStep )
+			{
+				if ( pulled == false )
+				{
+					PullBodies();
+					PullShapes();
+					broadPhase.trees[b2_staticBody].FlushEnlarged();
+					pulled = true;
+				}
+				if ( bodyAux[bodyId].flags & kIsFast )
+				{
+					if ( bodySims[bodyId].flags & kIsBullet )
+					{
+						for ( int shapeId = bodies[bodyId].headShapeId; shapeId != B2_NULL_INDEX; shapeId = shapes[shapeId].nextShapeId )
+						{
+							broadPhase.BufferMove( shapes[shapeId].proxyKey );
+						}
+						bullets.push_back( bodyId );
+						continue;
+					}
+					if ( hostContinuousStep )
+					{
+						DeviceWorld::ContinuousOnHost( *this, bodyId, &hostSensorHits );
+						bodySims[bodyId].flags &= ~kEnlargeBounds;
+						for ( int shapeId = bodies[bodyId].headShapeId; shapeId != B2_NULL_INDEX; shapeId = shapes[shapeId].nextShapeId )
+						{
+							ShapeDev& sd = shapeData[shapeId];
+							if ( sd.enlargedAABB )
+							{
+								broadPhase.EnlargeProxy( shapes[shapeId].proxyKey, sd.fatAABB );
+								sd.enlargedAABB = 0;
+							}
+						}
+						hostSwept.push_back( bodyId );
+						continue;
+					}
+				}
+			}
+
 			int shapeId = bodies[bodyId].headShapeId;
 			while ( shapeId != B2_NULL_INDEX )
 			{
@@ -4743,6 +4803,54 @@ void World::SolveEnd()
 			}
 		}
 		profile.refit = b2GetMilliseconds( refitTicks );
+
+		// Bullets (reference src/solver.c:1917-1980): each sweeps against the refit trees, then the
+		// proxies of its enlarged shapes grow in the dynamic tree (their moves are buffered above).
+		if ( bullets.empty() == false )
+		{
+			uint64_t bulletTicks = b2GetTicks();
+			for ( DynamicTree& tree : broadPhase.trees )
+			{
+				tree.FlushEnlarged();
+			}
+			for ( int bodyId : bullets )
+			{
+				DeviceWorld::ContinuousOnHost( *this, bodyId, &hostSensorHits );
+			}
+			DynamicTree& dynamicTree = broadPhase.trees[b2_dynamicBody];
+			for ( int bodyId : bullets )
+			{
+				BodySim& sim = bodySims[bodyId];
+				if ( ( sim.flags & kEnlargeBounds ) == 0 )
+				{
+					continue;
+				}
+				sim.flags &= ~kEnlargeBounds;
+				for ( int shapeId = bodies[bodyId].headShapeId; shapeId != B2_NULL_INDEX; shapeId = shapes[shapeId].nextShapeId )
+				{
+					ShapeDev& sd = shapeData[shapeId];
+					if ( sd.enlargedAABB )
+					{
+						sd.enlargedAABB = 0;
+						dynamicTree.EnlargeProxy( B2G_PROXY_ID( shapes[shapeId].proxyKey ), sd.fatAABB );
+					}
+				}
+			}
+			hostSwept.insert( hostSwept.end(), bullets.begin(), bullets.end() );
+			profile.bullets = b2GetMilliseconds( bulletTicks );
+		}
+		if ( hostSwept.empty() == false )
+		{
+			device->UploadBodyShapes( *this, hostSwept );
+		}
+		for ( IdPair h : hostSensorHits )
+		{
+			const Shape& sensorShape = shapes[h.first];
+			if ( sensorShape.sensorIndex != B2_NULL_INDEX )
+			{
+				sensors[sensorShape.sensorIndex].hits.push_back( SensorVisitor{ h.second, shapes[h.second].generation } );
+			}
+		}
 	}
 
 	// Island sleep (reference src/solver.c:2024-2080): the finalize kernel marked the islands a

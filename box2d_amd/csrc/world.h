@@ -435,6 +435,7 @@ public:
 	void EnableBody( Body* body );
 	void SyncBodyFlags( Body* body );
 	void EnableBodySleep( Body* body, bool enable );
+	void SetBodyBullet( Body* body, bool flag );
 	void SetBodyAwake( Body* body, bool awake );
 	void WakeTouching( Body* body );
 	void SetShapeMaterial( Shape* shape, const b2SurfaceMaterial& material );
@@ -519,6 +520,9 @@ public:
 	// (static shape << 32 | fast shape) pairs the custom filter rejects for continuous collision
 	std::vector<uint64_t> ccdFilterRejects;
 	void BuildContinuousFilterRejects();
+	// this step's fast bodies all sweep on the host, from the refit (the pre-continuous callback
+	// is in use); fast bullets always do, after the refit
+	bool hostContinuousStep = false;
 	float MixRestitution( const Shape& a, const Shape& b ) const;
 	int splitIslandId = B2_NULL_INDEX;
 	float inv_h = 0.0f;

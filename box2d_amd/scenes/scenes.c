@@ -52,9 +52,14 @@ typedef struct b2sScene
 	b2ShapeId doomedShape;
 
 	// B2S_PRESOLVE: the one-way platform and the callback's tally of the last step
+	// (B2S_BULLETS: the pre-continuous callback's tally)
 	b2ShapeId platformShape;
 	uint32_t preSolveCalls;
 	uint32_t preSolveSum;
+
+	// B2S_BULLETS: the dynamic sensor and the ghost wall
+	b2ShapeId ringShape;
+	b2ShapeId ghostShape;
 } b2sScene;
 
 static b2sScene s_scenes[B2S_MAX_HANDLES];
@@ -1569,6 +1574,237 @@ static void b2sSensorsProbe( b2sScene* scene )
 	scene->probeHash = h;
 }
 
+// Pre-continuous callback of B2S_BULLETS: a hit on the ghost wall above y = 3 is ignored, every
+// other hit stands. The tally is a count and an order-independent sum of the arguments.
+static bool b2sPreContinuous( b2ShapeId shapeIdA, b2ShapeId shapeIdB, b2Pos point, b2Vec2 normal, void* context )
+{
+	b2sScene* scene = (b2sScene*)context;
+	uint32_t px, py, nx, ny;
+	float fx = (float)point.x, fy = (float)point.y;
+	memcpy( &px, &fx, 4 );
+	memcpy( &py, &fy, 4 );
+	memcpy( &nx, &normal.x, 4 );
+	memcpy( &ny, &normal.y, 4 );
+	scene->preSolveCalls += 1;
+	scene->preSolveSum += b2sMix32( (uint32_t)shapeIdA.index1 * 65599u + (uint32_t)shapeIdB.index1 ) ^ b2sMix32( px ^ b2sMix32( py ) ) ^
+						  b2sMix32( nx + 3u * ny );
+	bool ghost = B2_ID_EQUALS( shapeIdA, scene->ghostShape ) || B2_ID_EQUALS( shapeIdB, scene->ghostShape );
+	return ghost == false || fy <= 3.0f;
+}
+
+static void b2sFire( b2BodyId id, float x, float y, float vx, float vy )
+{
+	b2Body_SetTransform( id, ( b2Pos ){ x, y }, b2Rot_identity );
+	b2Body_SetLinearVelocity( id, ( b2Vec2 ){ vx, vy } );
+	b2Body_SetAngularVelocity( id, 0.0f );
+}
+
+static void b2sBuildBullets( b2sScene* scene, int enableSleep, int preContinuous )
+{
+	b2WorldId worldId = scene->worldId;
+	b2World_EnableSleeping( worldId, enableSleep != 0 );
+	if ( preContinuous )
+	{
+		b2World_SetPreSolveCallback( worldId, NULL, b2sPreContinuous, scene );
+	}
+	scene->bodyCapacity = 0;
+
+	b2BodyDef groundDef = b2DefaultBodyDef();
+	b2BodyId groundId = b2CreateBody( worldId, &groundDef );
+	b2ShapeDef groundShape = b2DefaultShapeDef();
+	groundShape.enableSensorEvents = true;
+	b2Polygon groundBox = b2MakeOffsetBox( 40.0f, 1.0f, ( b2Vec2 ){ 0.0f, -1.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &groundShape, &groundBox );
+	b2Polygon left = b2MakeOffsetBox( 0.1f, 6.0f, ( b2Vec2 ){ -35.0f, 6.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &groundShape, &left );
+	b2Polygon right = b2MakeOffsetBox( 0.1f, 6.0f, ( b2Vec2 ){ 35.0f, 6.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &groundShape, &right );
+	b2Polygon roof = b2MakeOffsetBox( 35.0f, 0.1f, ( b2Vec2 ){ 0.0f, 12.0f }, b2Rot_identity );
+	b2CreatePolygonShape( groundId, &groundShape, &roof );
+
+	// the ghost wall: pre-solve events on, the callback lets hits above y = 3 through
+	b2ShapeDef ghostDef = b2DefaultShapeDef();
+	ghostDef.enablePreSolveEvents = true;
+	b2Polygon ghost = b2MakeOffsetBox( 0.1f, 4.0f, ( b2Vec2 ){ -22.0f, 4.0f }, b2Rot_identity );
+	scene->ghostShape = b2CreatePolygonShape( groundId, &ghostDef, &ghost );
+
+	// bullets: a circle, a box, a capsule, a circle with sensor and pre-solve events, a hexagon
+	for ( int i = 0; i < 5; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.isBullet = true;
+		bodyDef.position = ( b2Pos ){ 30.0f, 0.6f + 1.0f * (float)i };
+		bodyDef.linearVelocity = ( b2Vec2 ){ -240.0f - 10.0f * (float)i, 0.0f };
+		bodyDef.gravityScale = ( i % 2 ) == 0 ? 0.0f : 1.0f;
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.enableSensorEvents = i == 0 || i == 3;
+		def.enablePreSolveEvents = i == 3;
+		if ( i == 1 )
+		{
+			b2Polygon box = b2MakeBox( 0.1f, 0.1f );
+			b2CreatePolygonShape( bodyId, &def, &box );
+		}
+		else if ( i == 2 )
+		{
+			b2Capsule capsule = { { -0.15f, 0.0f }, { 0.15f, 0.0f }, 0.08f };
+			b2CreateCapsuleShape( bodyId, &def, &capsule );
+		}
+		else if ( i == 4 )
+		{
+			b2Vec2 points[6] = { { 0.12f, 0.0f }, { 0.06f, 0.1f }, { -0.06f, 0.1f }, { -0.12f, 0.0f }, { -0.06f, -0.1f }, { 0.06f, -0.1f } };
+			b2Hull hull = b2ComputeHull( points, 6 );
+			b2Polygon hex = b2MakePolygon( &hull, 0.0f );
+			b2CreatePolygonShape( bodyId, &def, &hex );
+		}
+		else
+		{
+			b2Circle circle = { { 0.0f, 0.0f }, i == 3 ? 0.15f : 0.1f };
+			b2CreateCircleShape( bodyId, &def, &circle );
+		}
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+
+	// fast bodies that are not bullets: they sweep against static shapes only
+	for ( int i = 0; i < 2; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ 30.0f, 1.2f + 2.0f * (float)i };
+		bodyDef.linearVelocity = ( b2Vec2 ){ -230.0f, 0.0f };
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.enableSensorEvents = true;
+		b2Circle circle = { { 0.0f, 0.0f }, 0.1f };
+		b2CreateCircleShape( bodyId, &def, &circle );
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+
+	// targets: a stack of boxes, a thin standing plank, a rotating kinematic paddle and a dynamic
+	// body carrying a sensor ring
+	for ( int i = 0; i < 12; ++i )
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ 10.0f + 0.8f * (float)( i % 2 ), 0.4f + 0.8f * (float)( i / 2 ) };
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.enableSensorEvents = true;
+		b2Polygon box = b2MakeBox( 0.4f, 0.4f );
+		b2CreatePolygonShape( bodyId, &def, &box );
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ 0.0f, 2.0f };
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		b2Polygon plank = b2MakeBox( 0.05f, 2.0f );
+		b2CreatePolygonShape( bodyId, &def, &plank );
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_kinematicBody;
+		bodyDef.position = ( b2Pos ){ -10.0f, 3.0f };
+		bodyDef.angularVelocity = 1.5f;
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		b2Polygon paddle = b2MakeBox( 0.1f, 2.5f );
+		b2CreatePolygonShape( bodyId, &def, &paddle );
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+	{
+		b2BodyDef bodyDef = b2DefaultBodyDef();
+		bodyDef.type = b2_dynamicBody;
+		bodyDef.position = ( b2Pos ){ -15.0f, 0.5f };
+		b2BodyId bodyId = b2CreateBody( worldId, &bodyDef );
+		b2ShapeDef def = b2DefaultShapeDef();
+		def.enableSensorEvents = true;
+		b2Polygon box = b2MakeBox( 0.5f, 0.5f );
+		b2CreatePolygonShape( bodyId, &def, &box );
+		b2ShapeDef ringDef = b2DefaultShapeDef();
+		ringDef.isSensor = true;
+		ringDef.enableSensorEvents = true;
+		b2Circle ring = { { 0.0f, 1.5f }, 1.2f };
+		scene->ringShape = b2CreateCircleShape( bodyId, &ringDef, &ring );
+		b2sPushBody( scene, &scene->bodyCapacity, bodyId );
+	}
+}
+
+// Before step k (0-based) of B2S_BULLETS: the bullets and the fast bodies are fired again from
+// either end and at angles, and bullet flags are switched.
+static void b2sBulletsBeforeStep( b2sScene* scene, int k )
+{
+	b2BodyId* b = scene->bodies;
+	if ( k == 30 || k == 140 )
+	{
+		for ( int i = 0; i < 7; ++i )
+		{
+			b2sFire( b[i], 30.0f, 0.7f + 0.8f * (float)i, -260.0f, 5.0f - 1.5f * (float)i );
+		}
+	}
+	if ( k == 50 )
+	{
+		b2Body_SetBullet( b[5], true );
+		b2Body_SetBullet( b[4], false );
+		for ( int i = 0; i < 7; ++i )
+		{
+			b2sFire( b[i], -30.0f, 0.8f + 0.75f * (float)i, 250.0f, 2.0f );
+		}
+	}
+	if ( k == 80 )
+	{
+		for ( int i = 0; i < 7; ++i )
+		{
+			b2sFire( b[i], 30.0f, 6.0f + 0.5f * (float)i, -200.0f, -25.0f );
+		}
+	}
+	if ( k == 110 )
+	{
+		b2Body_SetBullet( b[4], true );
+		for ( int i = 0; i < 7; ++i )
+		{
+			b2sFire( b[i], -30.0f, 1.0f + 0.6f * (float)i, 220.0f, 30.0f );
+		}
+	}
+}
+
+static void b2sBulletsProbe( b2sScene* scene )
+{
+	uint32_t h = scene->probeHash;
+	b2SensorEvents ev = b2World_GetSensorEvents( scene->worldId );
+	h = b2sFnvWord( h, (uint32_t)ev.beginCount | (uint32_t)ev.endCount << 16 );
+	for ( int i = 0; i < ev.beginCount; ++i )
+	{
+		h = b2sFnvShapeId( b2sFnvShapeId( h, ev.beginEvents[i].sensorShapeId ), ev.beginEvents[i].visitorShapeId );
+	}
+	for ( int i = 0; i < ev.endCount; ++i )
+	{
+		h = b2sFnvShapeId( b2sFnvShapeId( h, ev.endEvents[i].sensorShapeId ), ev.endEvents[i].visitorShapeId );
+	}
+	b2ShapeId visitors[64];
+	int n = b2Shape_GetSensorData( scene->ringShape, visitors, 64 );
+	h = b2sFnvWord( h, (uint32_t)n );
+	for ( int i = 0; i < n; ++i )
+	{
+		h = b2sFnvShapeId( h, visitors[i] );
+	}
+	uint32_t bits = 0;
+	for ( int i = 0; i < 7; ++i )
+	{
+		bits |= b2Body_IsBullet( scene->bodies[i] ) ? 1u << i : 0u;
+	}
+	h = b2sFnvWord( h, bits );
+	h = b2sFnvWord( h, scene->preSolveCalls );
+	h = b2sFnvWord( h, scene->preSolveSum );
+	scene->preSolveCalls = 0;
+	scene->preSolveSum = 0;
+	scene->probeHash = h;
+}
+
 static void b2sBuildSetters( b2sScene* scene, int enableSleep )
 {
 	b2WorldId worldId = scene->worldId;
@@ -1943,6 +2179,10 @@ int b2s_create( int kind, int workerCount, const int* iparams, int iparamCount, 
 			b2sBuildSensors( scene, iparamCount > 0 ? iparams[0] : 0 );
 			break;
 
+		case B2S_BULLETS:
+			b2sBuildBullets( scene, iparamCount > 0 ? iparams[0] : 0, iparamCount > 1 ? iparams[1] : 0 );
+			break;
+
 		case B2S_RAIN:
 		{
 			int count = iparamCount > 0 ? iparams[0] : 20000;
@@ -2020,6 +2260,10 @@ void b2s_step( int handle, float timeStep, int subStepCount, int count )
 		{
 			b2sSensorsBeforeStep( scene, scene->stepIndex );
 		}
+		else if ( scene->kind == B2S_BULLETS )
+		{
+			b2sBulletsBeforeStep( scene, scene->stepIndex );
+		}
 		else if ( scene->kind == B2S_JOINT_ZOO && scene->zooJointCount > 0 && scene->stepIndex == 150 )
 		{
 			for ( int k = 0; k < scene->zooJointCount; ++k )
@@ -2044,6 +2288,10 @@ void b2s_step( int handle, float timeStep, int subStepCount, int count )
 		else if ( scene->kind == B2S_SENSORS )
 		{
 			b2sSensorsProbe( scene );
+		}
+		else if ( scene->kind == B2S_BULLETS )
+		{
+			b2sBulletsProbe( scene );
 		}
 	}
 }

@@ -119,6 +119,18 @@ enum
 	// overlaps into b2s_probe_hash. Listed bodies: the kinematic sensor, the carrier, the 30
 	// visitors, the projectiles. iparams: [0] enableSleep.
 	B2S_SENSORS = 12,
+
+	// Bullets: five bullets (circle, box, capsule, a circle with sensor and pre-solve events, a
+	// hexagon) and two fast bodies that are not bullets, fired at 200-260 m/s through a stack of
+	// twelve boxes, a thin standing plank, a rotating kinematic paddle, a dynamic body carrying a
+	// sensor ring and a static ghost wall with pre-solve events, inside a 70 x 12 m box of static
+	// walls. Before chosen steps b2s_step fires them again from either end and at angles, turns a
+	// fast body into a bullet and a bullet off and on again. After each step it folds the sensor
+	// events, the ring's overlaps, the bullet flags and the pre-continuous callback's tally into
+	// b2s_probe_hash. Listed bodies: the bullets, the two fast bodies, the stack, the plank, the
+	// paddle and the ring's body. iparams: [0] enableSleep, [1] pre-continuous callback (hits on
+	// the ghost wall above y = 3 are ignored).
+	B2S_BULLETS = 13,
 };
 
 // Create a world with workerCount threads (0 or 1 means serial) and build a scene in it.

@@ -31,6 +31,7 @@ SCENE_EDITS = 9
 SCENE_SETTERS = 10
 SCENE_PRESOLVE = 11
 SCENE_SENSORS = 12
+SCENE_BULLETS = 13
 
 
 class SceneLib:

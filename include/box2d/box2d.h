@@ -121,6 +121,9 @@ B2_API bool b2Body_IsSleepEnabled( b2BodyId bodyId );
 B2_API void b2Body_SetSleepThreshold( b2BodyId bodyId, float sleepThreshold );
 B2_API float b2Body_GetSleepThreshold( b2BodyId bodyId );
 B2_API bool b2Body_IsEnabled( b2BodyId bodyId );
+/// Bullets: continuous collision against moving bodies too (reference: include/box2d/box2d.h:592-596)
+B2_API void b2Body_SetBullet( b2BodyId bodyId, bool flag );
+B2_API bool b2Body_IsBullet( b2BodyId bodyId );
 /// Contact data (reference: include/box2d/box2d.h:633-641, 833-841, 1519-1525)
 B2_API int b2Body_GetContactCapacity( b2BodyId bodyId );
 B2_API int b2Body_GetContactData( b2BodyId bodyId, b2ContactData* contactData, int capacity );

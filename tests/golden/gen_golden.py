@@ -21,7 +21,7 @@ sys.path.insert(0, ROOT)
 
 from box2d_amd.scenes_api import (SceneLib, SCENE_HELLO, SCENE_PYRAMID, SCENE_MIXED_PILE, SCENE_FALLING_HINGES,  # noqa: E402
                                   SCENE_RAIN, SCENE_JOINT_ZOO, SCENE_EVENTS, SCENE_EDITS, SCENE_SETTERS,
-                                  SCENE_PRESOLVE, SCENE_SENSORS)
+                                  SCENE_PRESOLVE, SCENE_SENSORS, SCENE_BULLETS)
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libb2ref_scenes.so")
 
@@ -91,16 +91,17 @@ def events_fixture(lib, name, count, seed, steps):
     return fx
 
 
-def edits_fixture(lib, name, sleep, steps, kind=SCENE_EDITS):
-    """Per step of an edits scene (SCENE_EDITS or SCENE_SETTERS): the transform hash of the listed
-    bodies, the getter probe hash and the listed body count."""
-    h = lib.create(kind, (sleep,), ())
+def edits_fixture(lib, name, sleep, steps, kind=SCENE_EDITS, extra=()):
+    """Per step of a scripted scene (edits, setters, pre-solve, sensors, bullets): the transform
+    hash of the listed bodies, the probe hash and the listed body count."""
+    iparams = (sleep,) + tuple(extra)
+    h = lib.create(kind, iparams, ())
     per_step = []
     for _ in range(steps):
         lib.step(h)
         per_step.append([lib.hash(h), lib.probe_hash(h), lib.body_count(h)])
     lib.destroy(h)
-    return {"name": name, "kind": kind, "iparams": [sleep], "fparams": [], "steps": steps, "per_step": per_step}
+    return {"name": name, "kind": kind, "iparams": list(iparams), "fparams": [], "steps": steps, "per_step": per_step}
 
 
 # (name, kind, iparams, fparams, steps, every): scenes whose b2World_GetStateHash is recorded
@@ -113,6 +114,7 @@ STATE_HASH_SCENES = [
     ("edits_sleep", SCENE_EDITS, (1,), (), 240, 1),
     ("setters_sleep", SCENE_SETTERS, (1,), (), 200, 1),
     ("rain_small", SCENE_RAIN, (2000, 40, 0, 7), (48.0,), 400, 10),
+    ("bullets_sleep", SCENE_BULLETS, (1, 0), (), 200, 1),
 ]
 
 
@@ -166,11 +168,14 @@ def write_presolve(lib):
     path = os.path.join(HERE, "presolve.json")
     with open(path, "w") as f:
         json.dump({"generator": "tests/golden/gen_golden.py",
-                   "reference": "erincatto/box2d v3.2 pre-solve callback (src/contact.c:573-583), sensors (src/sensor.c)",
+                   "reference": "erincatto/box2d v3.2 pre-solve callback (src/contact.c:573-583), sensors (src/sensor.c), bullets and the pre-continuous callback (src/solver.c:181-550, 1917-1989)",
                    "scenes": [edits_fixture(lib, "presolve", 0, 200, SCENE_PRESOLVE),
                               edits_fixture(lib, "presolve_sleep", 1, 200, SCENE_PRESOLVE),
                               edits_fixture(lib, "sensors", 0, 200, SCENE_SENSORS),
-                              edits_fixture(lib, "sensors_sleep", 1, 200, SCENE_SENSORS)]},
+                              edits_fixture(lib, "sensors_sleep", 1, 200, SCENE_SENSORS),
+                              edits_fixture(lib, "bullets", 0, 200, SCENE_BULLETS, (0,)),
+                              edits_fixture(lib, "bullets_sleep", 1, 200, SCENE_BULLETS, (0,)),
+                              edits_fixture(lib, "bullets_precontinuous", 0, 200, SCENE_BULLETS, (1,))]},
                   f, separators=(",", ":"))
     print(f"wrote {path} ({os.path.getsize(path)} bytes)")
 

@@ -28,7 +28,7 @@ def by_name(name):
 
 def test_fixture_is_well_formed():
     scenes = state_scenes()
-    assert len(scenes) == 8
+    assert len(scenes) == 9
     for fx in scenes:
         hashes = fx["hashes"]
         assert len(hashes) == 1 + fx["steps"] // fx["every"]

@@ -277,3 +277,30 @@ def test_sensor_events_match_reference(gpu_scenes, name, path):
             assert got == fx["per_step"][step - 1], f"{name} step {step}: [transforms, sensors, bodies] {got}, reference {fx['per_step'][step - 1]}"
     finally:
         gpu_scenes.destroy(h)
+
+
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("name", ["bullets", "bullets_sleep", "bullets_precontinuous"])
+def test_bullets_match_reference(gpu_scenes, name, path):
+    """Bullets: five bullets and two fast non-bullet bodies fired at 200-260 m/s through a box
+    stack, a thin dynamic plank, a rotating kinematic paddle and a dynamic body carrying a sensor
+    ring. Bullets sweep against the moving bodies after the refit and stop at them; the other fast
+    bodies sweep against static shapes only and pass. A fast body becomes a bullet and a bullet
+    stops being one mid-run. With the pre-continuous callback every fast body sweeps on the host
+    and the callback drops hits on the upper half of a ghost wall. Transforms, sensor events, the
+    ring's overlaps, the bullet flags and the callback's call count and argument sum equal the
+    reference's after each of 200 steps (tests/golden/presolve.json)."""
+    import box2d_amd
+    from conftest import load_golden
+
+    fx = next(s for s in load_golden("presolve.json")["scenes"] if s["name"] == name)
+    print(f"[gpu] bullet scene {name} path {path}", flush=True)
+    h = gpu_scenes.create(fx["kind"], fx["iparams"], fx["fparams"])
+    box2d_amd.set_solver_path(gpu_scenes.world_id(h), path)
+    try:
+        for step in range(1, fx["steps"] + 1):
+            gpu_scenes.step(h)
+            got = [gpu_scenes.hash(h), gpu_scenes.probe_hash(h), gpu_scenes.body_count(h)]
+            assert got == fx["per_step"][step - 1], f"{name} step {step}: [transforms, probe, bodies] {got}, reference {fx['per_step'][step - 1]}"
+    finally:
+        gpu_scenes.destroy(h)
