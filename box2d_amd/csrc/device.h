@@ -179,6 +179,12 @@ public:
 	void CollideLaunch( const StepParams& params );
 	void CollideFinish( CollideResult* result );
 	void SolveLaunch( const StepParams& params, bool anyRestitution, bool needSleepTimes );
+	// SolveLaunch on the process's helper thread when it is free (else here), so that the caller's
+	// host work overlaps the launches; every later device call of this world joins it first.
+	void SolveLaunchAsync( const StepParams& params, bool anyRestitution, bool needSleepTimes );
+	void JoinLaunch() const;
+	// Wake the helper thread ahead of a step's solve launch.
+	void ArmLaunchHelper();
 	void SolveFinish( FinalizeResult* result );
 
 	// Record instead of launching (the batch that steps many worlds launches the records).

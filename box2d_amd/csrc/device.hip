@@ -19,6 +19,7 @@
 #include "narrowphase.h"
 #include "solver_device.h"
 #include "chain.h"
+#include "host_threads.h"
 #include "toi.h"
 #include "world.h"
 
@@ -3001,6 +3002,8 @@ struct DeviceWorld::Impl
 	// recorded after the finalize pack kernel of a launched solve (speculative pair queries wait on it)
 	hipEvent_t solvePacked = nullptr;
 	bool solvePackedValid = false;
+	// the helper thread running this world's solve launches, until JoinLaunch
+	StepHelper* launchHelper = nullptr;
 
 	// Per-kernel timing
 	bool timing = false;
@@ -3145,6 +3148,7 @@ DeviceWorld::~DeviceWorld()
 	{
 		return;
 	}
+	JoinLaunch();
 	(void)hipStreamSynchronize( impl->stream );
 	(void)hipEventDestroy( impl->solvePacked );
 	if ( impl->xcdSync != nullptr )
@@ -3203,6 +3207,7 @@ static void UploadVector( DevArray<T>& dst, const std::vector<T>& src, hipStream
 
 void DeviceWorld::UploadBodies( const World& world )
 {
+	JoinLaunch();
 	int n = (int)world.bodySims.size();
 	impl->sims.Reserve( n, false, impl->stream, 0 );
 	impl->states.Reserve( n, false, impl->stream, 0 );
@@ -3220,6 +3225,7 @@ void DeviceWorld::UploadBodies( const World& world )
 
 void DeviceWorld::DownloadBodies( World& world )
 {
+	JoinLaunch();
 	int n = impl->bodyCount;
 	B2G_CHECK( n <= (int)world.bodySims.size() );
 	if ( n > 0 )
@@ -3234,6 +3240,7 @@ void DeviceWorld::DownloadBodies( World& world )
 
 void DeviceWorld::UploadShapes( const World& world )
 {
+	JoinLaunch();
 	int n = (int)world.shapeData.size();
 	impl->shapes.Reserve( n, false, impl->stream, 0 );
 	if ( n > 0 )
@@ -3268,6 +3275,7 @@ void DeviceWorld::UploadShapes( const World& world )
 
 void DeviceWorld::DownloadShapes( World& world )
 {
+	JoinLaunch();
 	int n = impl->shapeCount;
 	B2G_CHECK( n <= (int)world.shapeData.size() );
 	if ( n > 0 )
@@ -4026,6 +4034,44 @@ static bool XcdSolverUsable()
 		   strncmp( prop.gcnArchName, "gfx950", 6 ) == 0;
 }
 
+void DeviceWorld::ArmLaunchHelper()
+{
+	StepHelper* helper = impl->recording ? nullptr : SharedStepHelper();
+	if ( helper != nullptr )
+	{
+		helper->Arm();
+	}
+}
+
+void DeviceWorld::SolveLaunchAsync( const StepParams& params, bool anyRestitution, bool needSleepTimes )
+{
+	StepHelper* helper = impl->recording ? nullptr : SharedStepHelper();
+	if ( helper != nullptr )
+	{
+		int device = 0;
+		HIP_CHECK( hipGetDevice( &device ) );
+		StepParams p = params;
+		if ( helper->TryPost( [this, p, anyRestitution, needSleepTimes, device]() {
+				 HIP_CHECK( hipSetDevice( device ) );
+				 SolveLaunch( p, anyRestitution, needSleepTimes );
+			 } ) )
+		{
+			impl->launchHelper = helper;
+			return;
+		}
+	}
+	SolveLaunch( params, anyRestitution, needSleepTimes );
+}
+
+void DeviceWorld::JoinLaunch() const
+{
+	if ( impl->launchHelper != nullptr )
+	{
+		impl->launchHelper->Wait();
+		impl->launchHelper = nullptr;
+	}
+}
+
 void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bool needSleepTimes )
 {
 	const SolverLists& L = impl->lists;
@@ -4649,6 +4695,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 
 bool DeviceWorld::GetSolvePairInputs( SolvePairInputs* out ) const
 {
+	JoinLaunch();
 	if ( impl->recording || impl->solvePackedValid == false )
 	{
 		return false;
@@ -4691,6 +4738,7 @@ void DeviceWorld::LaunchDeferredSolve()
 
 void DeviceWorld::SolveFinish( FinalizeResult* result )
 {
+	JoinLaunch();
 	const bool ownStream = impl->recording == false || impl->deferredLaunched;
 	impl->deferredLaunched = false;
 	{

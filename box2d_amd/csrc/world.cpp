@@ -3600,17 +3600,13 @@ void World::SyncPairFinderMirrors()
 
 void World::LaunchSpeculativePairs()
 {
-	SolvePairInputs in;
-	if ( device->GetSolvePairInputs( &in ) == false )
-	{
-		return;
-	}
 	if ( pairFinder == nullptr )
 	{
 		pairFinder = new PairFinder();
 	}
 	// the trees are as the next pair update will find them, except for the boxes the solve
-	// enlarges, which the device puts in
+	// enlarges, which the device puts in; the staging is host work that overlaps the solve's
+	// launches, the query itself waits for them
 	int n = StageProxies();
 	const LbvhProxy* proxies = pairFinder->ProxyStaging( 0 );
 	int shapeCount = std::max<int>( (int)shapes.size(), 1 );
@@ -3624,6 +3620,11 @@ void World::LaunchSpeculativePairs()
 		slots[proxies[p].shapeId] = ShapeSlot{ proxies[p].proxyKey, p };
 	}
 	SyncPairFinderMirrors();
+	SolvePairInputs in;
+	if ( device->GetSolvePairInputs( &in ) == false )
+	{
+		return;
+	}
 	pairFinder->LaunchSpeculative( n, (int)shapes.size(), in );
 
 	speculative.launched = true;
@@ -4224,6 +4225,12 @@ bool World::StepBegin( float timeStep, int subStepCount )
 		return false;
 	}
 
+	if ( solverSets[kAwakeSet].bodyIds.empty() == false )
+	{
+		// the solve's launches go to a helper thread, which starts spinning now
+		device->ArmLaunchHelper();
+	}
+
 	moveEvents.clear();
 	moveEventBodyIds.clear();
 	moveEventsNeedTransforms = false;
@@ -4664,7 +4671,7 @@ bool World::SolveBegin( StepParams& params )
 	}
 
 	constraintTicks = b2GetTicks();
-	device->SolveLaunch( params, restitutionShapeCount > 0, enableSleep );
+	device->SolveLaunchAsync( params, restitutionShapeCount > 0, enableSleep );
 	return true;
 }
 
