@@ -200,7 +200,7 @@ def batched_workload(lib, scenes, worlds, steps, sample_stride=8, warmup=4):
     handles = [scenes.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
     bodies = sum(scenes.body_count(h) for h in handles)
     ids = (box2d_amd.WorldId * worlds)(*[scenes.world_id(h) for h in handles])
-    box2d_amd.step_worlds(ids, warmup, DT, SUBSTEPS)  # warm-up; the first three also autotune the workgroup size
+    box2d_amd.step_worlds(ids, warmup, DT, SUBSTEPS)  # warm-up
     lib.b2amd_Synchronize()
     solve_ms = 0.0
     solve_bytes = 0.0
@@ -241,6 +241,10 @@ def batched_workload(lib, scenes, worlds, steps, sample_stride=8, warmup=4):
     return {"worlds": worlds, "bodies": bodies, "steps": steps, "warmup": warmup, "ms_per_step": batched_s * 1e3,
             "body_substeps_per_s": batch_rate, "solve_kernel_ms_per_step": solve_ms / steps,
             "solve_workgroup_threads": st.solveThreads, "solve_workgroups": st.solveGrid,
+            "solve_buckets": [{"bucket": k, "worlds": st.bucketWorlds[k], "threads": st.bucketThreads[k],
+                               "warm_by_body": st.bucketWarmByBody[k], "workgroups": st.bucketWorkgroups[k],
+                               "gpu_ms": st.bucketMs[k]} for k in range(4) if st.bucketWorlds[k]],
+            "solver_tuning_table": box2d_amd.batch_tuning(),
             "phase_ms_per_step": {names[k]: phase_ms[k] / steps for k in range(5)},
             "hash_match": not mismatched, "hash_sample_worlds": len(sample), "hash_mismatched_worlds": mismatched[:16],
             "plain_loop_worlds": len(sample), "plain_loop_bodies": loop_bodies,

@@ -33,7 +33,25 @@ from .scenes_api import (SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HE
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("solveKernelMs", ctypes.c_float), ("solveThreads", ctypes.c_int), ("solveGrid", ctypes.c_int),
-                ("worlds", ctypes.c_int), ("phaseMs", ctypes.c_float * 5), ("solveAlgorithmicBytes", ctypes.c_double)]
+                ("worlds", ctypes.c_int), ("phaseMs", ctypes.c_float * 5), ("solveAlgorithmicBytes", ctypes.c_double),
+                ("bucketWorlds", ctypes.c_int * 4), ("bucketThreads", ctypes.c_int * 4),
+                ("bucketWarmByBody", ctypes.c_int * 4), ("bucketWorkgroups", ctypes.c_int * 4),
+                ("bucketMs", ctypes.c_float * 4)]
+
+
+def batch_tuning():
+    """The batched solver's configuration per world-size bucket: a list of dicts with the bucket's
+    largest awake body count, the workgroup size and warm-start form in force and the offline
+    table's."""
+    lib = library()
+    out = []
+    for k in range(4):
+        v = [ctypes.c_int() for _ in range(5)]
+        if lib.b2amd_GetBatchTuning(k, *[ctypes.byref(x) for x in v]) != 0:
+            break
+        out.append({"bucket": k, "max_bodies": v[0].value, "threads": v[1].value, "warm_by_body": v[2].value,
+                    "table_threads": v[3].value, "table_warm_by_body": v[4].value})
+    return out
 
 
 def step_worlds(world_ids, count=1, dt=1.0 / 60.0, substeps=4):

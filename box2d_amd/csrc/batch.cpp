@@ -80,8 +80,23 @@ int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, Bat
 	if ( stats != nullptr )
 	{
 		stats->solveKernelMs = s_batch->lastFusedMs;
-		stats->solveThreads = s_batch->lastFusedThreads;
-		stats->solveGrid = s_batch->lastFusedGrid;
+		stats->solveThreads = 0;
+		stats->solveGrid = 0;
+		int most = 0;
+		for ( int k = 0; k < kBatchBuckets; ++k )
+		{
+			stats->bucketWorlds[k] = s_batch->lastBucketWorlds[k];
+			stats->bucketThreads[k] = s_batch->lastBucketThreads[k];
+			stats->bucketWarmByBody[k] = s_batch->lastBucketWarmByBody[k];
+			stats->bucketGrid[k] = s_batch->lastBucketGrid[k];
+			stats->bucketMs[k] = s_batch->lastBucketMs[k];
+			stats->solveGrid += s_batch->lastBucketGrid[k];
+			if ( s_batch->lastBucketWorlds[k] > most )
+			{
+				most = s_batch->lastBucketWorlds[k];
+				stats->solveThreads = s_batch->lastBucketThreads[k];
+			}
+		}
 		stats->worlds = n;
 		stats->solveBytes = s_batch->lastFusedBytes;
 		for ( int k = 0; k < 5; ++k )

@@ -2018,6 +2018,14 @@ int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int 
 			stats->phaseMs[k] = bs.phaseMs[k];
 		}
 		stats->solveAlgorithmicBytes = bs.solveBytes;
+		for ( int k = 0; k < kBatchBuckets; ++k )
+		{
+			stats->bucketWorlds[k] = bs.bucketWorlds[k];
+			stats->bucketThreads[k] = bs.bucketThreads[k];
+			stats->bucketWarmByBody[k] = bs.bucketWarmByBody[k];
+			stats->bucketWorkgroups[k] = bs.bucketGrid[k];
+			stats->bucketMs[k] = bs.bucketMs[k];
+		}
 	}
 	return n;
 }
@@ -2030,6 +2038,42 @@ void b2amd_Synchronize( void )
 int b2amd_SetBatchSolverThreads( int threads )
 {
 	return SetBatchSolverThreads( threads ) ? 0 : -1;
+}
+
+int b2amd_SetBatchTuning( int bucket, int threads, int warmByBody )
+{
+	return SetBatchTuning( bucket, threads, warmByBody ) ? 0 : -1;
+}
+
+int b2amd_GetBatchTuning( int bucket, int* maxBodies, int* threads, int* warmByBody, int* tableThreads, int* tableWarmByBody )
+{
+	if ( bucket < 0 || bucket >= kBatchBuckets )
+	{
+		return -1;
+	}
+	BatchTuning t = BatchTuningFor( bucket );
+	BatchTuning table = BatchTuningTable( bucket );
+	if ( maxBodies != nullptr )
+	{
+		*maxBodies = t.maxBodies;
+	}
+	if ( threads != nullptr )
+	{
+		*threads = t.threads;
+	}
+	if ( warmByBody != nullptr )
+	{
+		*warmByBody = t.warmByBody;
+	}
+	if ( tableThreads != nullptr )
+	{
+		*tableThreads = table.threads;
+	}
+	if ( tableWarmByBody != nullptr )
+	{
+		*tableWarmByBody = table.warmByBody;
+	}
+	return 0;
 }
 
 int b2amd_SpinWaitActive( void )

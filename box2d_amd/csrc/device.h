@@ -244,13 +244,15 @@ public:
 	// Launch the recorded solve stages (prepare .. finalize, continuous, readback) and wait.
 	void Solve( DeviceWorld** worlds, int count );
 
-	// Workgroup size of the persistent solver (0 while the autotuner is still trying sizes).
-	int FusedThreads() const;
-
-	float lastFusedMs = 0.0f;
+	float lastFusedMs = 0.0f; // all buckets' persistent launches, first start to last end
 	double lastFusedBytes = 0.0;
-	int lastFusedThreads = 0;
-	int lastFusedGrid = 0;
+	// per world-size bucket of the last batch step: worlds and their warm-start form, and the
+	// workgroup size, workgroups and GPU time of the launch that ran them (zeros: no worlds)
+	int lastBucketWorlds[4] = {};
+	int lastBucketThreads[4] = {};
+	int lastBucketWarmByBody[4] = {};
+	int lastBucketGrid[4] = {};
+	float lastBucketMs[4] = {};
 
 	struct Impl;
 
@@ -260,13 +262,39 @@ private:
 
 struct BatchStats
 {
-	float solveKernelMs; // persistent substep-loop launch of the last batch step
-	int solveThreads;	 // its workgroup size
-	int solveGrid;		 // its workgroup count
+	float solveKernelMs; // persistent substep-loop launches of the last batch step
+	int solveThreads;	 // workgroup size of the bucket with the most worlds
+	int solveGrid;		 // workgroups over all buckets
 	int worlds;			 // worlds stepped
 	float phaseMs[5];	 // wall time: host phase 1, GPU collide, host phase 2, GPU solve, host phase 3
-	double solveBytes;	 // algorithmic bytes of the substep-loop launch
+	double solveBytes;	 // algorithmic bytes of the substep-loop launches
+	int bucketWorlds[4];
+	int bucketThreads[4];
+	int bucketWarmByBody[4];
+	int bucketGrid[4];
+	float bucketMs[4];
 };
+
+// The batched persistent solver's configuration for worlds of up to maxBodies awake bodies: the
+// workgroup size that steps one world (a batch's single launch takes the size of the bucket with
+// the most worlds) and the warm-start form of the bucket's worlds (1 body by body, 0 color by
+// color). Neither changes any result. Four buckets by awake body count: 1-64, 65-256, 257-1024,
+// 1025 up to the persistent solver's LDS capacity.
+constexpr int kBatchBuckets = 4;
+struct BatchTuning
+{
+	int maxBodies;
+	int threads;
+	int warmByBody;
+};
+int BatchBucketOf( int bodyCount );
+// The configuration in force for a bucket: the offline-tuned table unless set otherwise.
+BatchTuning BatchTuningFor( int bucket );
+// Override a bucket's configuration (threads 64, 128, 256, 512 or 1024; warmByBody 0 or 1), or
+// restore the table's with threads 0. False for an invalid bucket or value.
+bool SetBatchTuning( int bucket, int threads, int warmByBody );
+// The table's configuration for a bucket (what the offline tuner chose).
+BatchTuning BatchTuningTable( int bucket );
 
 // Step count worlds as one batch (defined in batch.cpp). Returns the number of worlds stepped.
 int StepWorlds( World** worlds, int count, float timeStep, int subStepCount, BatchStats* stats );

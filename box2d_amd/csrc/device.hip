@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "box2d_amd.h"
+#include "batch_tuning.h"
 #include "device.h"
 #include "narrowphase.h"
 #include "solver_device.h"
@@ -4009,6 +4010,102 @@ static bool WarmByBody( bool fusedPath )
 	return fusedPath == false;
 }
 
+// Batched persistent solver configuration by world-size bucket (the offline table, overridable).
+static const BatchTuning kBatchTuningTable[kBatchBuckets] = B2AMD_BATCH_TUNING_TABLE;
+static std::atomic<int> s_bucketThreads[kBatchBuckets];	 // 0: the table's
+static std::atomic<int> s_bucketWarmByBody[kBatchBuckets]; // -1: the table's
+static std::atomic<bool> s_bucketOverridesInit{ false };
+// Workgroup size forced for every bucket by b2amd_SetBatchSolverThreads (0: per bucket).
+static std::atomic<int> s_forcedBatchThreads{ 0 };
+
+static bool ValidBatchThreads( int threads )
+{
+	return threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024;
+}
+
+static void InitBucketOverrides()
+{
+	if ( s_bucketOverridesInit.load() == false )
+	{
+		for ( int k = 0; k < kBatchBuckets; ++k )
+		{
+			s_bucketThreads[k].store( 0 );
+			s_bucketWarmByBody[k].store( -1 );
+		}
+		const char* v = getenv( "B2AMD_BATCH_THREADS" );
+		if ( v != nullptr && ValidBatchThreads( atoi( v ) ) )
+		{
+			s_forcedBatchThreads.store( atoi( v ) );
+		}
+		s_bucketOverridesInit.store( true );
+	}
+}
+
+int BatchBucketOf( int bodyCount )
+{
+	for ( int k = 0; k < kBatchBuckets - 1; ++k )
+	{
+		if ( bodyCount <= kBatchTuningTable[k].maxBodies )
+		{
+			return k;
+		}
+	}
+	return kBatchBuckets - 1;
+}
+
+BatchTuning BatchTuningTable( int bucket )
+{
+	return kBatchTuningTable[bucket];
+}
+
+BatchTuning BatchTuningFor( int bucket )
+{
+	InitBucketOverrides();
+	BatchTuning t = kBatchTuningTable[bucket];
+	int threads = s_bucketThreads[bucket].load();
+	int warm = s_bucketWarmByBody[bucket].load();
+	int forced = s_forcedBatchThreads.load();
+	t.threads = forced != 0 ? forced : ( threads != 0 ? threads : t.threads );
+	t.warmByBody = warm >= 0 ? warm : t.warmByBody;
+	return t;
+}
+
+bool SetBatchTuning( int bucket, int threads, int warmByBody )
+{
+	if ( bucket < 0 || bucket >= kBatchBuckets || ( threads != 0 && ValidBatchThreads( threads ) == false ) ||
+		 ( threads != 0 && warmByBody != 0 && warmByBody != 1 ) )
+	{
+		return false;
+	}
+	InitBucketOverrides();
+	s_bucketThreads[bucket].store( threads );
+	s_bucketWarmByBody[bucket].store( threads == 0 ? -1 : warmByBody );
+	return true;
+}
+
+bool SetBatchSolverThreads( int threads )
+{
+	if ( threads != 0 && ValidBatchThreads( threads ) == false )
+	{
+		return false;
+	}
+	InitBucketOverrides();
+	s_forcedBatchThreads.store( threads );
+	return true;
+}
+
+// The warm-start form of a batched world with bodyCount awake bodies: B2AMD_WARM_BY_BODY when set,
+// else its bucket's.
+static bool BatchWarmByBody( int bodyCount )
+{
+	const char* v = getenv( "B2AMD_WARM_BY_BODY" );
+	if ( v != nullptr )
+	{
+		return atoi( v ) != 0;
+	}
+	return BatchTuningFor( BatchBucketOf( bodyCount ) ).warmByBody != 0;
+}
+
 // Whether the current device can run the XCD solver: an MI355X (8 XCDs of 32 CUs) with
 // cooperative launches and room for one of its workgroups on every CU.
 static bool XcdSolverUsable()
@@ -4494,7 +4591,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		fa.anyRestitution = anyRestitution ? 1 : 0;
 		fa.warmStart = impl->warmStart.ptr;
 		fa.warmList = impl->warmList.ptr;
-		fa.warmByBody = WarmByBody( true ) && haveWarmLists ? 1 : 0;
+		fa.warmByBody = ( rec ? BatchWarmByBody( nb ) : WarmByBody( true ) ) && haveWarmLists ? 1 : 0;
 		// constraint fields per substep (warm start, solve, relax) + body sims per substep + state in/out
 		double bytes = params.subStepCount * ( kBytesFusedContactSubstep * nc + kBytesFusedBodySubstep * nb + kBytesJoint * nj ) +
 					   kBytesFusedBodyStep * nb;
@@ -4889,7 +4986,11 @@ bool DeviceWorld::WantsWarmLists( int bodyCount, int constraintCount, int active
 	}
 	bool fits = bodyCount <= kFusedBodyCapacity && activeColorCount <= kMaxColorSlots;
 	bool fused = fits && ( impl->solverPath == 2 || ( impl->solverPath == 0 && AutoFused( constraintCount, overflowCount ) ) );
-	fused = fused || impl->tape != nullptr || impl->recording;
+	if ( impl->recording )
+	{
+		return BatchWarmByBody( bodyCount );
+	}
+	fused = fused || impl->tape != nullptr;
 	return WarmByBody( fused );
 }
 
@@ -5400,11 +5501,6 @@ struct Batch::Impl
 	int* queue = nullptr;
 	int cuCount = 0;
 
-	// autotuned workgroup size of the persistent solver
-	int fusedThreads = 0;
-	int tuneIndex = 0;
-	float tuneMs[3] = {};
-
 	size_t used = 0;
 
 	void EnsureStaging( size_t bytes )
@@ -5442,12 +5538,6 @@ Batch::Batch()
 	int device = 0;
 	HIP_CHECK( hipGetDevice( &device ) );
 	HIP_CHECK( hipDeviceGetAttribute( &impl->cuCount, hipDeviceAttributeMultiprocessorCount, device ) );
-	const char* v = getenv( "B2AMD_BATCH_THREADS" );
-	if ( v != nullptr )
-	{
-		int t = atoi( v );
-		impl->fusedThreads = ( t == 256 || t == 512 || t == 1024 ) ? t : 0;
-	}
 }
 
 Batch::~Batch()
@@ -5464,13 +5554,6 @@ Batch::~Batch()
 	(void)hipStreamDestroy( impl->stream );
 	delete impl;
 }
-
-int Batch::FusedThreads() const
-{
-	return impl->fusedThreads;
-}
-
-static const int kBatchThreadCandidates[3] = { 256, 512, 1024 };
 
 // Per-stage lists gathered from the worlds' records.
 template <typename Args>
@@ -5609,18 +5692,6 @@ void Batch::Collide( DeviceWorld** worlds, int count )
 	HIP_CHECK( hipStreamSynchronize( b->stream ) );
 }
 
-// Workgroup size of the persistent batched solver forced by b2amd_SetBatchSolverThreads (0: tuned).
-static std::atomic<int> s_forcedBatchThreads{ 0 };
-
-bool SetBatchSolverThreads( int threads )
-{
-	if ( threads != 0 && threads != 256 && threads != 512 && threads != 1024 )
-	{
-		return false;
-	}
-	s_forcedBatchThreads.store( threads );
-	return true;
-}
 
 void Batch::Solve( DeviceWorld** worlds, int count )
 {
@@ -5713,11 +5784,6 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 		order[i] = (int)i;
 	}
 	std::stable_sort( order.begin(), order.end(), [&]( int x, int y ) { return fusedBodies[x] > fusedBodies[y]; } );
-	int maxBodies = 0;
-	for ( int nb : fusedBodies )
-	{
-		maxBodies = std::max( maxBodies, nb );
-	}
 	fusedOff = b->Put( fused.data(), fused.size() * sizeof( FusedArgs ) );
 	orderOff = b->Put( order.data(), order.size() * sizeof( int ) );
 	packOff = b->Put( pack.data(), pack.size() * sizeof( PackFinalizeArgs ) );
@@ -5747,24 +5813,47 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 																  reinterpret_cast<const int2*>( base + tapeBlocksOff ) );
 	}
 
-	int forced = s_forcedBatchThreads.load();
+	for ( int k = 0; k < kBatchBuckets; ++k )
+	{
+		lastBucketWorlds[k] = 0;
+		lastBucketGrid[k] = 0;
+		lastBucketMs[k] = 0.0f;
+	}
 	if ( fused.empty() == false )
 	{
-		// Autotune the workgroup size over the first batches: try 256, 512 and 1024 threads once
-		// each and keep the fastest. The choice does not change any result.
-		int threads = forced != 0 ? forced : ( b->fusedThreads != 0 ? b->fusedThreads : kBatchThreadCandidates[b->tuneIndex] );
-		int lds = FusedLdsBytes( maxBodies );
+		// One persistent launch over every world, largest first. Its workgroup size is the tuned
+		// table's for the bucket with the most worlds (the worlds of one launch share one size; a
+		// launch per bucket, tried side by side on separate streams, lost to the single queue's load
+		// balance); each world's warm-start form is its own bucket's. Neither changes any result.
+		const int total = (int)fused.size();
+		int bucketCount[kBatchBuckets] = {};
+		for ( int i = 0; i < total; ++i )
+		{
+			bucketCount[BatchBucketOf( fusedBodies[i] )] += 1;
+		}
+		int most = 0;
+		for ( int k = 1; k < kBatchBuckets; ++k )
+		{
+			most = bucketCount[k] > bucketCount[most] ? k : most;
+		}
+		const int threads = BatchTuningFor( most ).threads;
+		int lds = FusedLdsBytes( fusedBodies[order[0]] );
 		int perCu = 0;
 		HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, SolveFusedBatchKernel, threads, lds ) );
-		int grid = std::max( 1, std::min( (int)fused.size(), perCu * b->cuCount ) );
+		int grid = std::max( 1, std::min( total, perCu * b->cuCount ) );
 		HIP_CHECK( hipMemsetAsync( b->queue, 0, sizeof( int ), s ) );
 		// the events bracket the kernel's execution (recorded by the dispatch itself)
 		hipExtLaunchKernelGGL( SolveFusedBatchKernel, dim3( grid ), dim3( threads ), (uint32_t)lds, s, b->ev0, b->ev1, 0,
 							   reinterpret_cast<const FusedArgs*>( base + fusedOff ), reinterpret_cast<const int*>( base + orderOff ),
-							   (int)fused.size(), b->queue );
+							   total, b->queue );
 		HIP_CHECK( hipGetLastError() );
-		lastFusedThreads = threads;
-		lastFusedGrid = grid;
+		for ( int k = 0; k < kBatchBuckets; ++k )
+		{
+			lastBucketWorlds[k] = bucketCount[k];
+			lastBucketThreads[k] = bucketCount[k] > 0 ? threads : 0;
+			lastBucketWarmByBody[k] = bucketCount[k] > 0 ? BatchTuningFor( k ).warmByBody : 0;
+			lastBucketGrid[k] = bucketCount[k] > 0 ? grid : 0;
+		}
 		lastFusedBytes = fusedBytes;
 	}
 
@@ -5802,19 +5891,9 @@ void Batch::Solve( DeviceWorld** worlds, int count )
 		float ms = 0.0f;
 		HIP_CHECK( hipEventElapsedTime( &ms, b->ev0, b->ev1 ) );
 		lastFusedMs = ms;
-		if ( b->fusedThreads == 0 && forced == 0 )
+		for ( int k = 0; k < kBatchBuckets; ++k )
 		{
-			b->tuneMs[b->tuneIndex] = ms;
-			b->tuneIndex += 1;
-			if ( b->tuneIndex == 3 )
-			{
-				int best = 0;
-				for ( int k = 1; k < 3; ++k )
-				{
-					best = b->tuneMs[k] < b->tuneMs[best] ? k : best;
-				}
-				b->fusedThreads = kBatchThreadCandidates[best];
-			}
+			lastBucketMs[k] = lastBucketWorlds[k] > 0 ? ms : 0.0f;
 		}
 	}
 }

@@ -4672,6 +4672,23 @@ bool World::SolveBegin( StepParams& params )
 
 	constraintTicks = b2GetTicks();
 	device->SolveLaunchAsync( params, restitutionShapeCount > 0, enableSleep );
+
+	// Move events: one per awake body in awake order, filled while the GPU solves (the awake set
+	// cannot change before the solve ends; transforms are fetched lazily).
+	uint64_t eventTicks = b2GetTicks();
+	moveEventBodyIds = awakeSet.bodyIds;
+	moveEvents.resize( awakeBodyCount );
+	for ( int i = 0; i < awakeBodyCount; ++i )
+	{
+		int bodyId = moveEventBodyIds[i];
+		Body& body = bodies[bodyId];
+		body.bodyMoveIndex = i;
+		moveEvents[i].userData = body.userData;
+		moveEvents[i].bodyId = b2BodyId{ bodyId + 1, worldId, body.generation };
+		moveEvents[i].fellAsleep = false;
+	}
+	moveEventsNeedTransforms = true;
+	profile.transforms = b2GetMilliseconds( eventTicks );
 	return true;
 }
 
@@ -4714,22 +4731,6 @@ void World::SolveEnd()
 	shapesStale = true;
 	jointsStale = true;
 	contactsStale = true;
-
-	// Move events: one per awake body in awake order (transforms fetched lazily).
-	uint64_t eventTicks = b2GetTicks();
-	moveEventBodyIds = awakeSet.bodyIds;
-	moveEvents.resize( awakeBodyCount );
-	for ( int i = 0; i < awakeBodyCount; ++i )
-	{
-		int bodyId = moveEventBodyIds[i];
-		Body& body = bodies[bodyId];
-		body.bodyMoveIndex = i;
-		moveEvents[i].userData = body.userData;
-		moveEvents[i].bodyId = b2BodyId{ bodyId + 1, worldId, body.generation };
-		moveEvents[i].fellAsleep = false;
-	}
-	moveEventsNeedTransforms = true;
-	profile.transforms = b2GetMilliseconds( eventTicks );
 
 	// Refit: enlarge proxies of enlarged bodies in awake order (reference src/solver.c:1822-1915).
 	{

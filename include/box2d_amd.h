@@ -143,6 +143,15 @@ typedef struct b2amd_BatchStats
 	/// algorithmic bytes of the substep-loop launch: per world, substeps x (380 B per contact
 	/// constraint + 32 B per awake body + 256 B per joint) + 64 B per awake body
 	double solveAlgorithmicBytes;
+	/// per world-size bucket (awake bodies 1-64, 65-256, 257-1024, more): worlds in the bucket and
+	/// their warm-start form (1 body by body); the workgroup size, workgroup count and GPU
+	/// milliseconds of the persistent launch that ran them (one launch runs every bucket, with the
+	/// workgroup size of the bucket that has the most worlds); zeros for an empty bucket
+	int bucketWorlds[4];
+	int bucketThreads[4];
+	int bucketWarmByBody[4];
+	int bucketWorkgroups[4];
+	float bucketMs[4];
 } b2amd_BatchStats;
 
 /// Step count independent worlds by timeStep with subStepCount substeps, as one batch: host
@@ -153,10 +162,23 @@ typedef struct b2amd_BatchStats
 /// Returns the number of worlds stepped; stats may be null.
 B2_API int b2amd_StepWorlds( const b2WorldId* worldIds, int count, float timeStep, int subStepCount, b2amd_BatchStats* stats );
 
-/// Workgroup size of b2amd_StepWorlds' persistent solver: 256, 512 or 1024 threads, or 0 to let the
-/// first three batch steps try each and keep the fastest (the default). The results do not depend
-/// on it. Returns 0, or -1 for another value.
+/// Workgroup size of b2amd_StepWorlds' persistent solver for every world-size bucket: 64, 128, 256,
+/// 512 or 1024 threads, or 0 for each bucket's own (the default). The results do not depend on it.
+/// Returns 0, or -1 for another value. No counterpart in the reference (its workers are threads).
 B2_API int b2amd_SetBatchSolverThreads( int threads );
+
+/// The persistent solver's configuration of one world-size bucket (0: 1-64 awake bodies, 1: 65-256,
+/// 2: 257-1024, 3: more): workgroup size (64, 128, 256, 512 or 1024; a batch's launch uses the size
+/// of the bucket with the most worlds) and warm-start form of the bucket's worlds (1 body by body,
+/// 0 color by color), overriding the offline-tuned table; threads 0 restores the table's. The form
+/// applies to worlds whose solver lists are built afterwards. Results do not depend on either.
+/// Returns 0, or -1 for an invalid bucket or value. No counterpart in the reference.
+B2_API int b2amd_SetBatchTuning( int bucket, int threads, int warmByBody );
+
+/// The configuration in force for a bucket and the offline table's (any pointer may be null).
+/// Returns 0, or -1 for an invalid bucket. No counterpart in the reference.
+B2_API int b2amd_GetBatchTuning( int bucket, int* maxBodies, int* threads, int* warmByBody, int* tableThreads,
+								 int* tableWarmByBody );
 
 // ---------------------------------------------------------------------------------------------
 // Differentiation of recorded steps.

@@ -82,3 +82,27 @@ def test_layouts_match_reference_sizes():
     cookie = d.internalValue
     assert cookie == 1152023
     assert ctypes.sizeof(WorldId) == 4
+
+
+def test_batch_tuning_table_is_well_formed():
+    """The offline-tuned table of the batched solver (host calls only): four buckets with rising
+    body limits, the last one reaching the persistent solver's 5400-body capacity, each with a valid
+    workgroup size and warm-start form; overrides are validated and restored."""
+    import box2d_amd
+
+    lib = box2d_amd.library()
+    table = box2d_amd.batch_tuning()
+    assert [t["bucket"] for t in table] == [0, 1, 2, 3]
+    limits = [t["max_bodies"] for t in table]
+    assert limits == sorted(limits) and limits[-1] >= 5400
+    for t in table:
+        assert t["table_threads"] in (64, 128, 256, 512, 1024)
+        assert t["table_warm_by_body"] in (0, 1)
+        assert (t["threads"], t["warm_by_body"]) == (t["table_threads"], t["table_warm_by_body"])
+    assert lib.b2amd_SetBatchTuning(4, 256, 0) == -1
+    assert lib.b2amd_SetBatchTuning(0, 96, 0) == -1
+    assert lib.b2amd_SetBatchTuning(0, 256, 2) == -1
+    assert lib.b2amd_SetBatchTuning(1, 128, 1) == 0
+    assert box2d_amd.batch_tuning()[1]["threads"] == 128 and box2d_amd.batch_tuning()[1]["warm_by_body"] == 1
+    assert lib.b2amd_SetBatchTuning(1, 0, 0) == 0
+    assert box2d_amd.batch_tuning()[1]["threads"] == table[1]["table_threads"]

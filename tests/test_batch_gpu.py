@@ -27,11 +27,13 @@ def _set_batch_threads(threads):
     assert lib.b2amd_SetBatchSolverThreads(threads) == 0
 
 
-@pytest.mark.parametrize("threads,hi,seed", [(0, 300, 777), (256, 1024, 778), (512, 1024, 779), (1024, 1024, 780)],
-                         ids=["tuned_8-300", "wg256_8-1024", "wg512_8-1024", "wg1024_8-1024"])
+@pytest.mark.parametrize("threads,hi,seed", [(0, 300, 777), (64, 1024, 781), (256, 1024, 778), (512, 1024, 779),
+                                             (1024, 1024, 780)],
+                         ids=["tuned_8-300", "wg64_8-1024", "wg256_8-1024", "wg512_8-1024", "wg1024_8-1024"])
 def test_batch_matches_single_worlds(gpu_scenes, threads, hi, seed):
     """48 uneven worlds (8 to 300 or 1024 bodies) as one batch, with the persistent solver's
-    workgroup size tuned over the first three steps or forced to 256, 512 or 1024 threads."""
+    workgroup size from the offline-tuned table per world-size bucket or forced to 64, 256, 512 or
+    1024 threads for every bucket."""
     import box2d_amd
     from box2d_amd.scenes_api import SceneLib
 
@@ -64,13 +66,57 @@ def test_batch_matches_single_worlds(gpu_scenes, threads, hi, seed):
                         c = ref.hash(ref_handles[i])
                         assert a == c, f"world {i} {specs[i]}: batch {a:#x} != reference {c:#x} at step {step}"
         assert stats.worlds == len(batch)
-        assert stats.solveThreads in (256, 512, 1024)
+        assert stats.solveThreads in (64, 128, 256, 512, 1024)
+        assert sum(stats.bucketWorlds) == len(batch)
+        if threads == 0:
+            # one launch: the workgroup size of the bucket with the most worlds, each bucket's warm form
+            table = box2d_amd.batch_tuning()
+            most = max(range(4), key=lambda k: stats.bucketWorlds[k])
+            for k in range(4):
+                if stats.bucketWorlds[k]:
+                    assert stats.bucketThreads[k] == table[most]["table_threads"]
+                    assert stats.bucketWarmByBody[k] == table[k]["table_warm_by_body"]
     finally:
         _set_batch_threads(0)
         for h in batch + single:
             gpu_scenes.destroy(h)
         for h in ref_handles:
             ref.destroy(h)
+
+
+@pytest.mark.parametrize("threads,warm", [(64, 1), (128, 0), (1024, 1)], ids=["wg64_warm_by_body", "wg128_by_color",
+                                                                           "wg1024_warm_by_body"])
+def test_batch_bucket_overrides_match_single_worlds(gpu_scenes, threads, warm):
+    """Every bucket's configuration overridden (b2amd_SetBatchTuning) before the worlds are built:
+    32 uneven worlds (8 to 2000 bodies, so all four buckets) stepped 40 times as a batch end as if
+    stepped alone, with the warm start body by body or color by color."""
+    import box2d_amd
+
+    lib = box2d_amd.library()
+    specs = box2d_amd.batch_world_specs(32, lo=8, hi=2000, seed=900 + threads + warm)
+    for k in range(4):
+        assert lib.b2amd_SetBatchTuning(k, threads, warm) == 0
+    batch = [gpu_scenes.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
+    single = [gpu_scenes.create(box2d_amd.SCENE_BATCH_WORLD, spec) for spec in specs]
+    ids = (box2d_amd.WorldId * len(batch))(*[gpu_scenes.world_id(h) for h in batch])
+    try:
+        print(f"[gpu] bucket overrides threads {threads} warm {warm}", flush=True)
+        for step in range(1, 41):
+            stats = box2d_amd.step_worlds(ids)
+            for h in single:
+                gpu_scenes.step(h)
+            assert all(stats.bucketThreads[k] == threads for k in range(4) if stats.bucketWorlds[k])
+            assert all(stats.bucketWarmByBody[k] == warm for k in range(4) if stats.bucketWorlds[k])
+            if step in CHECK or step == 40:
+                for i, (hb, hs) in enumerate(zip(batch, single)):
+                    a, b = gpu_scenes.hash(hb), gpu_scenes.hash(hs)
+                    assert a == b, f"world {i} {specs[i]}: batch {a:#x} != single {b:#x} at step {step}"
+        assert sum(1 for k in range(4) if stats.bucketWorlds[k]) >= 3
+    finally:
+        for k in range(4):
+            lib.b2amd_SetBatchTuning(k, 0, 0)
+        for h in batch + single:
+            gpu_scenes.destroy(h)
 
 
 def test_batch_with_worlds_beyond_the_persistent_solver(gpu_scenes, world_runs):
