@@ -29,7 +29,6 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 SUBSTEPS = 4
@@ -112,7 +111,7 @@ def live_traffic(kernel, rows, ground, warmup, steps):
 
 
 def kernel_stats(lib, world_id):
-    from b2types import KernelStats
+    from box2d_amd import KernelStats
 
     arr = (KernelStats * 32)()
     lib.b2amd_World_GetKernelStats.restype = ctypes.c_int
@@ -427,7 +426,7 @@ def main():
         sys.exit(2)
 
     import box2d_amd
-    from b2types import WorldId
+    from box2d_amd import WorldId
 
     lib = box2d_amd.library()
     scenes = box2d_amd.scenes()
@@ -470,6 +469,29 @@ def main():
     window_2_500 = {"ms_per_step": long_s * 1e3, "body_substeps_per_s": bodies * SUBSTEPS / long_s,
                     "vs_published": (bodies * SUBSTEPS / long_s) / PUBLISHED_BODY_SUBSTEPS if args.rows == 100 else None,
                     "hash_match": None if long_want is None else bool(long_hash == long_want)}
+
+    # the same window with the move events read after every step (b2World_GetBodyEvents), as a
+    # caller that draws or syncs transforms does: the transforms come back from the GPU per step
+    class BodyEvents(ctypes.Structure):
+        _fields_ = [("moveEvents", ctypes.c_void_p), ("moveCount", ctypes.c_int)]
+
+    lib.b2World_GetBodyEvents.restype = BodyEvents
+    lib.b2World_GetBodyEvents.argtypes = [WorldId]
+    he = scenes.pyramid(args.rows, args.ground, sleep=False)
+    pe = scenes.lib.b2s_world_id(he)
+    wide = WorldId(pe & 0xFFFF, pe >> 16)
+    scenes.step(he, args.warmup, DT, SUBSTEPS)
+    lib.b2amd_Synchronize()
+    moved = 0
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        scenes.step(he, 1, DT, SUBSTEPS)
+        moved += lib.b2World_GetBodyEvents(wide).moveCount
+    lib.b2amd_Synchronize()
+    events_ms = (time.perf_counter() - t2) * 1e3 / args.steps
+    scenes.destroy(he)
+    with_body_events = {"ms_per_step": events_ms, "move_events_per_step": moved / args.steps,
+                        "note": "same window, b2World_GetBodyEvents read after every step from Python"}
 
     total_steps = args.warmup + args.steps
     want = golden_hash(args.rows, args.ground, total_steps)
@@ -559,6 +581,7 @@ def main():
         "gpu_busy_ms_per_step": gpu_busy_ms,
         "spin_wait": bool(lib.b2amd_SpinWaitActive()),
         "window_steps_2_500": window_2_500,
+        "with_body_events": with_body_events,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -31,6 +31,12 @@ from .scenes_api import (SceneLib, SCENE_PYRAMID, SCENE_FALLING_HINGES, SCENE_HE
                          SCENE_RAIN, SCENE_BATCH_WORLD, SCENE_PUSH, WorldId)
 
 
+class KernelStats(ctypes.Structure):
+    """b2amd_KernelStats (include/box2d_amd.h): one kernel class's launches, time and bytes."""
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int), ("totalMs", ctypes.c_double),
+                ("algorithmicBytes", ctypes.c_double)]
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [("solveKernelMs", ctypes.c_float), ("solveThreads", ctypes.c_int), ("solveGrid", ctypes.c_int),
                 ("worlds", ctypes.c_int), ("phaseMs", ctypes.c_float * 5), ("solveAlgorithmicBytes", ctypes.c_double),

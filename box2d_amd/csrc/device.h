@@ -168,6 +168,8 @@ public:
 	// Device -> host downloads (after a step, on demand).
 	void DownloadBodies( World& world );
 	void DownloadShapes( World& world );
+	// The transforms of the given bodies (ids below the uploaded body count), in the given order.
+	void DownloadTransforms( const int* bodyIds, int count, b2Transform* out );
 	void DownloadJoints( World& world );
 	void DownloadContact( int contactId, ContactSim* out );
 	void DownloadContacts( int count, ContactSim* out ); // the first count contacts (by id)

@@ -3005,6 +3005,9 @@ struct DeviceWorld::Impl
 	bool solvePackedValid = false;
 	// the helper thread running this world's solve launches, until JoinLaunch
 	StepHelper* launchHelper = nullptr;
+	// move-event transforms: the bodies asked for and their gathered transforms
+	DevArray<int> transformIds;
+	DevArray<b2Transform> transformOut;
 
 	// Per-kernel timing
 	bool timing = false;
@@ -3157,6 +3160,8 @@ DeviceWorld::~DeviceWorld()
 		(void)hipFree( impl->xcdSync );
 	}
 	impl->sims.Free();
+	impl->transformIds.Free();
+	impl->transformOut.Free();
 	impl->states.Free();
 	impl->aux.Free();
 	impl->shapes.Free();
@@ -3272,6 +3277,34 @@ void DeviceWorld::UploadShapes( const World& world )
 		UploadVector( impl->ccdTree, impl->ccdTreeStaging, impl->stream );
 	}
 	HIP_CHECK( hipStreamSynchronize( impl->stream ) );
+}
+
+// The transforms of the listed bodies, in list order (the move events' transforms).
+__global__ void GatherTransformsKernel( const int* ids, const BodySim* sims, b2Transform* out, int count )
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if ( i < count )
+	{
+		out[i] = sims[ids[i]].transform;
+	}
+}
+
+void DeviceWorld::DownloadTransforms( const int* bodyIds, int count, b2Transform* out )
+{
+	JoinLaunch();
+	if ( count <= 0 )
+	{
+		return;
+	}
+	B2G_CHECK( impl->sims.ptr != nullptr );
+	hipStream_t s = impl->stream;
+	impl->transformIds.Reserve( count, false, s, 0 );
+	impl->transformOut.Reserve( count, false, s, 0 );
+	HIP_CHECK( hipMemcpyAsync( impl->transformIds.ptr, bodyIds, count * sizeof( int ), hipMemcpyHostToDevice, s ) );
+	GatherTransformsKernel<<<Blocks( count ), kBlock, 0, s>>>( impl->transformIds.ptr, impl->sims.ptr, impl->transformOut.ptr, count );
+	HIP_CHECK( hipGetLastError() );
+	HIP_CHECK( hipMemcpyAsync( out, impl->transformOut.ptr, count * sizeof( b2Transform ), hipMemcpyDeviceToHost, s ) );
+	HIP_CHECK( hipStreamSynchronize( s ) );
 }
 
 void DeviceWorld::DownloadShapes( World& world )

@@ -4914,10 +4914,22 @@ b2BodyEvents World::GetBodyEvents()
 {
 	if ( moveEventsNeedTransforms )
 	{
-		PullBodies();
-		for ( size_t i = 0; i < moveEvents.size(); ++i )
+		if ( bodiesStale )
 		{
-			moveEvents[i].transform = bodySims[moveEventBodyIds[i]].transform;
+			// only the moved bodies' transforms come back, not the whole body arrays
+			moveEventTransforms.resize( moveEvents.size() );
+			device->DownloadTransforms( moveEventBodyIds.data(), (int)moveEventBodyIds.size(), moveEventTransforms.data() );
+			for ( size_t i = 0; i < moveEvents.size(); ++i )
+			{
+				moveEvents[i].transform = moveEventTransforms[i];
+			}
+		}
+		else
+		{
+			for ( size_t i = 0; i < moveEvents.size(); ++i )
+			{
+				moveEvents[i].transform = bodySims[moveEventBodyIds[i]].transform;
+			}
 		}
 		moveEventsNeedTransforms = false;
 	}

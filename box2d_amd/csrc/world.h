@@ -565,6 +565,7 @@ public:
 	// events and statistics
 	std::vector<b2BodyMoveEvent> moveEvents;
 	std::vector<int> moveEventBodyIds;
+	std::vector<b2Transform> moveEventTransforms; // gathered from the device for the move events
 	bool moveEventsNeedTransforms = false;
 	b2Profile profile = {};
 	int recycledContactCount = 0;
