@@ -4407,12 +4407,27 @@ void World::StepMiddle()
 		broadPhase.trees[b2_dynamicBody].Rebuild( false );
 		broadPhase.trees[b2_kinematicBody].Rebuild( false );
 		treeRebuildPending = false;
-		// the rebuilt trees' leaf order, staged ahead for the next GPU pair query
-		if ( lastPairQueryPath >= 2 )
+	}
+	// the trees' leaf order, staged ahead for the next GPU pair query: when the last update was on
+	// the GPU, or when a world this large may need one after this solve (host work that overlaps
+	// the solve)
+	const bool largeAwake = solvePending && (int)solverSets[kAwakeSet].bodyIds.size() >= kGpuPairThreshold;
+	if ( lastPairQueryPath >= 2 || largeAwake )
+	{
+		for ( b2BodyType type : { b2_dynamicBody, b2_kinematicBody } )
 		{
-			FillStagingCache( b2_dynamicBody );
-			FillStagingCache( b2_kinematicBody );
+			const ProxyStagingCache& cache = stagingCache[type];
+			if ( cache.valid == false || cache.version != broadPhase.trees[type].StructureVersion() )
+			{
+				FillStagingCache( type );
+			}
 		}
+	}
+	// the GPU pair finder's copies of the shape filters and the pair set follow the host's while
+	// the GPU solves, so that a query launched after the solve uploads no backlog
+	if ( largeAwake && pairFinder != nullptr && device->Recording() == false )
+	{
+		SyncPairFinderMirrors();
 	}
 
 	// The next step's pair query behind this solve, when the last solve enlarged enough proxies
@@ -4700,6 +4715,14 @@ void World::SolveEnd()
 	FinalizeResult fr;
 	device->SolveFinish( &fr );
 	profile.constraints = b2GetMilliseconds( constraintTicks );
+	// A solve that enlarged many proxies after one that enlarged few: the next pair update would
+	// run the GPU query as a round trip of its own, so the query is launched now instead, behind the
+	// solve's results, and runs while the host refits (the speculative query's inputs are ready).
+	if ( speculative.launched == false && pairQuery == 0 && 2 * (int)fr.enlargedShapes.size() >= kGpuPairThreshold &&
+		 device->Recording() == false )
+	{
+		LaunchSpeculativePairs();
+	}
 	lastEnlargedShapeCount = (int)fr.enlargedShapes.size();
 	if ( speculative.launched )
 	{
