@@ -89,6 +89,54 @@ def test_gpu_adjoint_matches_fp64_adjoint_and_fd(recorded):
         assert _rel(g_gpu, g_fd) < REL_TOL, (w, g_gpu, g_fd)
 
 
+def test_gpu_adjoint_matches_fd_of_the_unforced_forward():
+    """Central differences of the plain float64 forward, with no branch forced: each perturbed run
+    makes its own contact, bias and clamp decisions. Eight single-box targets are pushed for 20
+    steps. A world counts as contact-stable when its unforced perturbed runs keep the recorded
+    run's branch signature and land where the runs forced onto the recorded pattern land (the two
+    difference quotients agree to 1e-3 relative), so that no decision of the forward with a visible
+    effect changed within the step h = 1e-6. In every contact-stable world the GPU gradient must
+    equal the unforced differences to 1e-3 relative, and at least two of the eight worlds must be
+    contact-stable (on MI355X three are: the others cross a decision the signature does not hold,
+    and there the unforced differences leave the forced ones by 1e-3 to 2x)."""
+    from box2d_amd.adjoint import BODY_STATE, PushWorlds
+
+    rng = np.random.default_rng(8)
+    params = np.stack([rng.uniform(2, 6, 8), rng.uniform(-0.5, 0.5, 8), rng.uniform(-1, 1, 8),
+                       rng.uniform(0.3, 0.9, 8)], 1)
+    goals = np.array([[0.0, 2.0]] * 8)
+    pw = PushWorlds(params, rows=1)
+    try:
+        tape = pw.record(20)
+        final = tape.final_states()
+        fg = np.zeros((8, tape.body_stride), dtype=BODY_STATE)
+        for w in range(8):
+            top = int(pw.body_ids[w][-2])
+            fg[w]["center"][top] = 2.0 * (final[w]["center"][top] - goals[w])
+        ig, fr, stats = tape.backward(fg)
+        assert stats.unsupported == 0
+        shape_ids = pw.pusher_shape_ids()
+        stable_worlds = []
+        for w in range(8):
+            pusher = int(pw.body_ids[w][-1])
+            top = int(pw.body_ids[w][-2])
+            g_gpu = np.array([ig[w]["linearVelocity"][pusher][0], ig[w]["linearVelocity"][pusher][1],
+                              ig[w]["angularVelocity"][pusher], fr[w, shape_ids[w]]], dtype=np.float64)
+            rec = WorldRecord.from_tape(tape, w)
+            ps, ts = rec.slot_of(pusher), rec.slot_of(top)
+            g_free, same_signature, _ = rec.push_fd(ps, shape_ids[w], ts, goals[w], h=1e-6, fixed=False)
+            g_forced, _, _ = rec.push_fd(ps, shape_ids[w], ts, goals[w], h=1e-6, fixed=True)
+            stable = same_signature and np.linalg.norm(g_free) > 0 and _rel(g_free, g_forced) < 1e-3
+            print(f"[gpu] push world {w}: gpu {g_gpu} unforced fd {g_free} forced fd {g_forced} contact-stable {stable}",
+                  flush=True)
+            if stable:
+                stable_worlds.append(w)
+                assert _rel(g_gpu, g_free) < REL_TOL, (w, g_gpu, g_free)
+        assert len(stable_worlds) >= 2, stable_worlds
+    finally:
+        pw.destroy()
+
+
 def test_torch_function_gradient():
     import torch
 
