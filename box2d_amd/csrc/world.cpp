@@ -4411,7 +4411,9 @@ void World::StepMiddle()
 	// the trees' leaf order, staged ahead for the next GPU pair query: when the last update was on
 	// the GPU, or when a world this large may need one after this solve (host work that overlaps
 	// the solve)
-	const bool largeAwake = solvePending && (int)solverSets[kAwakeSet].bodyIds.size() >= kGpuPairThreshold;
+	// (a batched world never runs the GPU pair query, so it stages nothing)
+	const bool largeAwake = solvePending && (int)solverSets[kAwakeSet].bodyIds.size() >= kGpuPairThreshold &&
+							device->Recording() == false;
 	if ( lastPairQueryPath >= 2 || largeAwake )
 	{
 		for ( b2BodyType type : { b2_dynamicBody, b2_kinematicBody } )
@@ -4425,7 +4427,7 @@ void World::StepMiddle()
 	}
 	// the GPU pair finder's copies of the shape filters and the pair set follow the host's while
 	// the GPU solves, so that a query launched after the solve uploads no backlog
-	if ( largeAwake && pairFinder != nullptr && device->Recording() == false )
+	if ( largeAwake && pairFinder != nullptr )
 	{
 		SyncPairFinderMirrors();
 	}
