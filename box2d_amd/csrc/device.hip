@@ -1936,6 +1936,7 @@ constexpr int kMaxContinuousSensorHits = 8;
 constexpr int kSensorHitCountSlot = 9;	   // device counter slots
 constexpr int kSplitKeySlot = 14;		   // 64-bit, slots 14 and 15
 constexpr int kSensorOverlapCountSlot = 10;
+constexpr int kFinishDoneSlot = 11; // workgroups of FinishKernel done (the last one resets it)
 
 B2G_DEV bool ShouldShapesCollide( const b2Filter& a, const b2Filter& b )
 {
@@ -2068,14 +2069,20 @@ B2G_DEV bool FilterRejected( const ContinuousArgs& a, int staticShapeId, int fas
 	return lo < a.filterRejectCount && a.filterRejects[lo] == key;
 }
 
+B2G_DEV void ContinuousBodyAt( const ContinuousArgs& a, int bodyId );
+
 B2G_DEV void ContinuousBody( const ContinuousArgs& a, int k )
 {
 	if ( k >= *a.fastCount )
 	{
 		return;
 	}
+	ContinuousBodyAt( a, a.awake[a.fastBodies[k]] );
+}
 
-	int bodyId = a.awake[a.fastBodies[k]];
+// The sweep of one fast body (by body id), after its finalization.
+B2G_DEV void ContinuousBodyAt( const ContinuousArgs& a, int bodyId )
+{
 	BodySim* fastSim = a.sims + bodyId;
 	if ( fastSim->flags & kIsBullet )
 	{
@@ -2534,13 +2541,14 @@ enum KernelClass
 	kkContinuous,
 	kkColorTail,
 	kkSolveXcd,
+	kkFinish,
 	kKernelClassCount
 };
 
 static const char* s_kernelClassNames[kKernelClassCount] = {
 	"collide", "prepare_contacts", "prepare_joints", "integrate_velocities_warm_start", "warm_start", "solve",
 	"integrate_positions", "relax", "restitution", "overflow", "store_impulses", "finalize_bodies", "solve_fused", "continuous",
-	"color_tail", "solve_xcd",
+	"color_tail", "solve_xcd", "finish",
 };
 
 constexpr double kBytesCollideFull = 800.0;		// per contact with a new manifold
@@ -2776,6 +2784,70 @@ __global__ __launch_bounds__( kPackThreads ) void PackFinalizeKernel( PackFinali
 __global__ __launch_bounds__( kPackThreads ) void PackFinalizeBatchKernel( const PackFinalizeArgs* args )
 {
 	PackFinalizeBody( args[blockIdx.x] );
+}
+
+// The end of a single world's solve in one launch: the first storeBlocks workgroups store the
+// contact impulses, the others finalize the awake bodies, each fast non-bullet body sweeping
+// right after its own finalization (its sweep reads only static shapes and its own state). The
+// last workgroup to finish packs the results for the host. Every thread releases its writes at
+// device scope before its workgroup counts itself done, and the last workgroup acquires before it
+// reads, so the pack sees every other workgroup's results; no workgroup waits for another.
+struct FinishArgs
+{
+	StoreArgs store;
+	FinalizeArgs fin;
+	ContinuousArgs ccd;
+	PackFinalizeArgs pack;
+	int storeBlocks;
+	int sweep; // run the continuous sweeps
+	int* done;
+};
+
+// workgroups of 256 threads (1024-thread workgroups measured at twice the time: too few of them)
+constexpr int kFinishThreads = 256;
+
+__global__ __launch_bounds__( kFinishThreads ) void FinishKernel( FinishArgs a )
+{
+	if ( (int)blockIdx.x < a.storeBlocks )
+	{
+		int i = blockIdx.x * blockDim.x + threadIdx.x;
+		if ( i < a.store.totalCount )
+		{
+			StoreImpulsesBody( a.store, i );
+		}
+	}
+	else
+	{
+		int i = ( blockIdx.x - a.storeBlocks ) * blockDim.x + threadIdx.x;
+		FinalizeBodiesBody( a.fin, i );
+		if ( a.sweep && i < a.fin.count )
+		{
+			int bodyId = a.fin.awake[i];
+			if ( ( a.fin.aux[bodyId].flags & kIsFast ) != 0 && ( a.fin.sims[bodyId].flags & kIsBullet ) == 0 )
+			{
+				ContinuousBodyAt( a.ccd, bodyId );
+			}
+		}
+	}
+
+	__shared__ int s_last;
+	__threadfence();
+	__syncthreads();
+	if ( threadIdx.x == 0 )
+	{
+		s_last = atomicAdd( a.done, 1 ) == (int)gridDim.x - 1 ? 1 : 0;
+	}
+	__syncthreads();
+	if ( s_last == 0 )
+	{
+		return;
+	}
+	__threadfence();
+	PackFinalizeBody( a.pack );
+	if ( threadIdx.x == 0 )
+	{
+		*a.done = 0;
+	}
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -4674,9 +4746,14 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		}
 	}
 
+	// Without recording, store, finalize, the continuous sweeps and the pack run as one launch
+	// (FinishKernel); B2AMD_NO_FUSED_FINISH keeps them separate launches.
+	static const bool fuseFinishAllowed = getenv( "B2AMD_NO_FUSED_FINISH" ) == nullptr;
+	const bool fuseFinish = rec == false && fuseFinishAllowed;
+	FinishArgs fin = {};
+	StoreArgs sa = {};
 	if ( nc > 0 )
 	{
-		StoreArgs sa;
 		sa.c = impl->soa;
 		sa.contacts = impl->contacts.ptr;
 		sa.coloredCount = L.overflowContactStart;
@@ -4691,7 +4768,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 			impl->rec.storeArgs = sa;
 			impl->rec.storeLanes = nc;
 		}
-		else
+		else if ( fuseFinish == false )
 		{
 			impl->Launch( kkStore, kBytesStore * nc, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<StoreImpulsesOp>, dim3( Blocks( nc ) ), dim3( kBlock ), 0, s, e0, e1, 0, sa ); } );
 		}
@@ -4721,7 +4798,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		impl->rec.finalizeArgs = fa;
 		impl->rec.finalizeLanes = nb;
 	}
-	else
+	else if ( fuseFinish == false )
 	{
 		impl->Launch( kkFinalize, kBytesFinalize * nb, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<FinalizeBodiesOp>, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, fa ); } );
 	}
@@ -4761,6 +4838,11 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 			impl->rec.continuousArgs = ka;
 			impl->rec.continuousLanes = nb;
 		}
+		else if ( fuseFinish )
+		{
+			fin.ccd = ka;
+			fin.sweep = 1;
+		}
 		else
 		{
 			impl->Launch( kkContinuous, 0.0, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<ContinuousOp>, dim3( Blocks( nb ) ), dim3( kBlock ), 0, s, e0, e1, 0, ka ); } );
@@ -4770,7 +4852,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	{
 		impl->sensorHitCapacity = 0;
 	}
-	if ( rec == false )
+	if ( rec == false && fuseFinish == false )
 	{
 		HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
 	}
@@ -4812,6 +4894,21 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		{
 			impl->rec.packFinalize = true;
 			impl->rec.packFinalizeArgs = pk;
+		}
+		else if ( fuseFinish )
+		{
+			fin.store = sa;
+			fin.fin = fa;
+			fin.pack = pk;
+			fin.storeBlocks = ( nc + kFinishThreads - 1 ) / kFinishThreads;
+			fin.done = impl->counters + kFinishDoneSlot;
+			const int blocks = std::max( 1, fin.storeBlocks + ( nb + kFinishThreads - 1 ) / kFinishThreads );
+			impl->Launch( kkFinish, kBytesStore * nc + kBytesFinalize * nb, [&]( hipEvent_t e0, hipEvent_t e1 ) {
+				hipExtLaunchKernelGGL( FinishKernel, dim3( blocks ), dim3( kFinishThreads ), 0, s, e0, e1, 0, fin );
+			} );
+			HIP_CHECK( hipEventRecord( impl->ev[3], s ) );
+			HIP_CHECK( hipEventRecord( impl->solvePacked, s ) );
+			impl->solvePackedValid = true;
 		}
 		else
 		{
