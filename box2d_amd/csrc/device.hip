@@ -1765,6 +1765,11 @@ struct FinalizeArgs
 	const int* islandIds;
 	uint32_t* islandBits;		  // awake islands, by slot
 	unsigned long long* splitKey; // sleepiest split candidate: sleep time bits << 32 | island id
+	// the pinned mailbox's enlarged-body, enlarged-shape and box lists, written beside the device
+	// lists (null: the pack copies the device lists instead)
+	int* mailBodies;
+	int* mailShapes;
+	float* mailAabbs;
 };
 
 // Body finalization (reference src/solver.c:553-743).
@@ -1880,6 +1885,15 @@ B2G_DEV void FinalizeBodiesBody( const FinalizeArgs& a, int simIndex )
 				int slot = atomicAdd( a.counters + 1, 1 );
 				a.enlargedShapes[slot] = shapeId;
 				a.enlargedShapeAABBs[slot] = fatAABB;
+				if ( a.mailShapes != nullptr )
+				{
+					a.mailShapes[slot] = shapeId;
+					float* dst = a.mailAabbs + 4 * slot;
+					dst[0] = fatAABB.lowerBound.x;
+					dst[1] = fatAABB.lowerBound.y;
+					dst[2] = fatAABB.upperBound.x;
+					dst[3] = fatAABB.upperBound.y;
+				}
 			}
 		}
 		shapeId = shape->nextShapeId;
@@ -1890,6 +1904,10 @@ B2G_DEV void FinalizeBodiesBody( const FinalizeArgs& a, int simIndex )
 	{
 		int slot = atomicAdd( a.counters + 0, 1 );
 		a.enlargedBodies[slot] = simIndex;
+		if ( a.mailBodies != nullptr )
+		{
+			a.mailBodies[slot] = simIndex;
+		}
 	}
 }
 
@@ -1936,6 +1954,8 @@ struct ContinuousArgs
 	IdPair* sensorHits; // (sensor shape, fast shape) hits on static sensors; null when the world has none
 	int* sensorHitCount;
 	int sensorHitCapacity;
+	int* mailShapes; // as FinalizeArgs::mailShapes
+	float* mailAabbs;
 };
 
 constexpr int kMaxContinuousSensorHits = 8;
@@ -2052,6 +2072,15 @@ B2G_DEV void EnlargeShape( const ContinuousArgs& a, int shapeId, ShapeDev* shape
 		int slot = atomicAdd( a.enlargedShapeCount, 1 );
 		a.enlargedShapes[slot] = shapeId;
 		a.enlargedShapeAABBs[slot] = fat;
+		if ( a.mailShapes != nullptr )
+		{
+			a.mailShapes[slot] = shapeId;
+			float* dst = a.mailAabbs + 4 * slot;
+			dst[0] = fat.lowerBound.x;
+			dst[1] = fat.lowerBound.y;
+			dst[2] = fat.upperBound.x;
+			dst[3] = fat.upperBound.y;
+		}
 	}
 }
 
@@ -2662,6 +2691,7 @@ struct PackFinalizeArgs
 	int aabbOffset;
 	int sleepOffset;
 	int* mail;
+	int listsInMail; // finalize and the sweeps wrote the enlarged lists into the mailbox already
 };
 
 B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
@@ -2670,11 +2700,11 @@ B2G_DEV void PackFinalizeBody( const PackFinalizeArgs& a )
 	int shapes = a.counters[1];
 	int fast = a.counters[2];
 	int error = a.counters[3];
-	for ( int i = threadIdx.x; i < bodies; i += blockDim.x )
+	for ( int i = threadIdx.x; a.listsInMail == 0 && i < bodies; i += blockDim.x )
 	{
 		a.mail[kMailHeader + i] = a.enlargedBodies[i];
 	}
-	for ( int i = threadIdx.x; i < shapes; i += blockDim.x )
+	for ( int i = threadIdx.x; a.listsInMail == 0 && i < shapes; i += blockDim.x )
 	{
 		a.mail[a.shapeOffset + i] = a.enlargedShapes[i];
 		b2AABB box = a.enlargedShapeAABBs[i];
@@ -2809,8 +2839,9 @@ struct FinishArgs
 	int* done;
 };
 
-// workgroups of 256 threads (1024-thread workgroups measured at twice the time: too few of them)
-constexpr int kFinishThreads = 256;
+// 64-lane workgroups: the bodies spread over as many compute units as possible (1024-thread
+// workgroups measured at twice the time of 256); the last one's pack has little left to copy
+constexpr int kFinishThreads = 64;
 
 __global__ __launch_bounds__( kFinishThreads ) void FinishKernel( FinishArgs a )
 {
@@ -4780,7 +4811,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		}
 	}
 
-	FinalizeArgs fa;
+	FinalizeArgs fa = {};
 	fa.awake = impl->awakeBodies.ptr;
 	fa.count = nb;
 	fa.sims = impl->sims.ptr;
@@ -4812,7 +4843,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 	if ( params.enableContinuous && params.hostContinuous == 0 )
 	{
 		// The fast-body count is only known on the device: size the grid for every awake body.
-		ContinuousArgs ka;
+		ContinuousArgs ka = {};
 		ka.fastBodies = impl->fastBodies.ptr;
 		ka.fastCount = impl->counters + 6;
 		ka.awake = impl->awakeBodies.ptr;
@@ -4865,7 +4896,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 
 	{
 		size_t shapeCap = std::max<size_t>( impl->shapeCount, 1 );
-		PackFinalizeArgs pk;
+		PackFinalizeArgs pk = {};
 		pk.counters = impl->counters + 4;
 		pk.enlargedBodies = impl->enlargedBodies.ptr;
 		pk.enlargedShapes = impl->enlargedShapes.ptr;
@@ -4906,6 +4937,13 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 			fin.store = sa;
 			fin.fin = fa;
 			fin.pack = pk;
+			// the lanes write the enlarged lists straight into the mailbox
+			fin.fin.mailBodies = impl->mail + kMailHeader;
+			fin.fin.mailShapes = impl->mail + pk.shapeOffset;
+			fin.fin.mailAabbs = reinterpret_cast<float*>( impl->mail + pk.aabbOffset );
+			fin.ccd.mailShapes = fin.fin.mailShapes;
+			fin.ccd.mailAabbs = fin.fin.mailAabbs;
+			fin.pack.listsInMail = 1;
 			fin.storeBlocks = ( nc + kFinishThreads - 1 ) / kFinishThreads;
 			fin.done = impl->counters + kFinishDoneSlot;
 			const int blocks = std::max( 1, fin.storeBlocks + ( nb + kFinishThreads - 1 ) / kFinishThreads );
