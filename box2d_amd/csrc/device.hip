@@ -112,6 +112,10 @@ struct CollideArgs
 	int customMixing; // StepParams::customMixing
 	int preSolve;	  // the world has a pre-solve callback: contacts that enable it go to the host
 	int* deferred;	  // their ids
+	// the pinned mailbox's change and deferred lists, written beside the device lists (null: the
+	// pack copies the device lists)
+	int* mailChanges;
+	int* mailDeferred;
 };
 
 // Where the device narrowphase reports: change records, recycled contacts, and contacts whose
@@ -124,6 +128,10 @@ struct DeviceCollideSink
 	{
 		int slot = atomicAdd( a.counters + 0, 1 );
 		a.changes[slot] = code;
+		if ( a.mailChanges != nullptr )
+		{
+			a.mailChanges[slot] = code;
+		}
 	}
 	__device__ void Recycled() const
 	{
@@ -139,6 +147,10 @@ struct DeviceCollideSink
 		{
 			int slot = atomicAdd( a.counters + 2, 1 );
 			a.deferred[slot] = contactId;
+			if ( a.mailDeferred != nullptr )
+			{
+				a.mailDeferred[slot] = contactId;
+			}
 			return true;
 		}
 		return false;
@@ -1963,6 +1975,7 @@ constexpr int kSensorHitCountSlot = 9;	   // device counter slots
 constexpr int kSplitKeySlot = 14;		   // 64-bit, slots 14 and 15
 constexpr int kSensorOverlapCountSlot = 10;
 constexpr int kFinishDoneSlot = 11; // workgroups of FinishKernel done (the last one resets it)
+constexpr int kCollideDoneSlot = 3;	 // workgroups of CollideAndPackKernel done (likewise)
 
 B2G_DEV bool ShouldShapesCollide( const b2Filter& a, const b2Filter& b )
 {
@@ -2622,6 +2635,7 @@ struct PackCollideArgs
 	const int* changes;
 	int* mail;
 	int deferredOffset; // changes and mailbox offset of the deferred contact ids (2 per collided contact)
+	int listsInMail;	// the narrowphase lanes wrote the lists into the mailbox already
 };
 
 B2G_DEV void PackCollideBody( const PackCollideArgs& a )
@@ -2629,11 +2643,11 @@ B2G_DEV void PackCollideBody( const PackCollideArgs& a )
 	int count = a.counters[0];
 	int recycled = a.counters[1];
 	int deferred = a.counters[2];
-	for ( int i = threadIdx.x; i < count; i += blockDim.x )
+	for ( int i = threadIdx.x; a.listsInMail == 0 && i < count; i += blockDim.x )
 	{
 		a.mail[kMailHeader + i] = a.changes[i];
 	}
-	for ( int i = threadIdx.x; i < deferred; i += blockDim.x )
+	for ( int i = threadIdx.x; a.listsInMail == 0 && i < deferred; i += blockDim.x )
 	{
 		a.mail[kMailHeader + a.deferredOffset + i] = a.changes[a.deferredOffset + i];
 	}
@@ -2652,6 +2666,35 @@ B2G_DEV void PackCollideBody( const PackCollideArgs& a )
 __global__ __launch_bounds__( kPackThreads ) void PackCollideKernel( PackCollideArgs a )
 {
 	PackCollideBody( a );
+}
+
+// The narrowphase of a single world with its pack: the last workgroup to finish (each workgroup
+// releases its writes at device scope before it counts itself done; the last one acquires)
+// copies the change records into the mailbox, writes the counts and clears the counters. No
+// workgroup waits for another.
+__global__ void CollideAndPackKernel( CollideArgs a, PackCollideArgs pc, int* done )
+{
+	CollideOp::Run( a, blockIdx.x * blockDim.x + threadIdx.x );
+	// The barrier completes every thread's stores (they reach the L2); thread 0's device-scope
+	// fence then releases the whole workgroup's writes before it counts the workgroup done.
+	__shared__ int s_last;
+	__syncthreads();
+	if ( threadIdx.x == 0 )
+	{
+		__threadfence();
+		s_last = atomicAdd( done, 1 ) == (int)gridDim.x - 1 ? 1 : 0;
+	}
+	__syncthreads();
+	if ( s_last == 0 )
+	{
+		return;
+	}
+	__threadfence();
+	PackCollideBody( pc );
+	if ( threadIdx.x == 0 )
+	{
+		*done = 0;
+	}
 }
 
 // one workgroup per world
@@ -2825,9 +2868,9 @@ __global__ __launch_bounds__( kPackThreads ) void PackFinalizeBatchKernel( const
 // The end of a single world's solve in one launch: the first storeBlocks workgroups store the
 // contact impulses, the others finalize the awake bodies, each fast non-bullet body sweeping
 // right after its own finalization (its sweep reads only static shapes and its own state). The
-// last workgroup to finish packs the results for the host. Every thread releases its writes at
-// device scope before its workgroup counts itself done, and the last workgroup acquires before it
-// reads, so the pack sees every other workgroup's results; no workgroup waits for another.
+// last workgroup to finish packs the results for the host. Each workgroup releases its writes at
+// device scope before it counts itself done, and the last workgroup acquires before it reads, so
+// the pack sees every other workgroup's results; no workgroup waits for another.
 struct FinishArgs
 {
 	StoreArgs store;
@@ -2867,11 +2910,13 @@ __global__ __launch_bounds__( kFinishThreads ) void FinishKernel( FinishArgs a )
 		}
 	}
 
+	// The barrier completes every thread's stores (they reach the L2); thread 0's device-scope
+	// fence then releases the whole workgroup's writes before it counts the workgroup done.
 	__shared__ int s_last;
-	__threadfence();
 	__syncthreads();
 	if ( threadIdx.x == 0 )
 	{
+		__threadfence();
 		s_last = atomicAdd( a.done, 1 ) == (int)gridDim.x - 1 ? 1 : 0;
 	}
 	__syncthreads();
@@ -3768,7 +3813,7 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 	hipStream_t s = impl->stream;
 	impl->EnsureMail( kMailHeader + 3 * (size_t)n );
 
-	CollideArgs a;
+	CollideArgs a = {};
 	a.list = impl->collideList.ptr;
 	a.count = n;
 	a.contacts = impl->contacts.ptr;
@@ -3782,7 +3827,7 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 	a.preSolve = params.preSolve;
 	a.deferred = impl->changes.ptr + 2 * n;
 
-	PackCollideArgs pc = { impl->counters, impl->changes.ptr, impl->mail, 2 * n };
+	PackCollideArgs pc = { impl->counters, impl->changes.ptr, impl->mail, 2 * n, 0 };
 
 	if ( impl->recording )
 	{
@@ -3795,7 +3840,18 @@ void DeviceWorld::CollideLaunch( const StepParams& params )
 	}
 
 	HIP_CHECK( hipEventRecord( impl->ev[0], s ) );
-	// Bytes are fixed up after the launch once the recycled count is known.
+	// Bytes are fixed up after the launch once the recycled count is known. One launch with its
+	// own pack (B2AMD_NO_FUSED_FINISH: the narrowphase and the pack as two launches).
+	static const bool fused = getenv( "B2AMD_NO_FUSED_FINISH" ) == nullptr;
+	if ( fused )
+	{
+		int* done = impl->counters + kCollideDoneSlot;
+		impl->Launch( kkCollide, 0.0, [&]( hipEvent_t e0, hipEvent_t e1 ) {
+			hipExtLaunchKernelGGL( CollideAndPackKernel, dim3( Blocks( n ) ), dim3( kBlock ), 0, s, e0, e1, 0, a, pc, done );
+		} );
+		HIP_CHECK( hipEventRecord( impl->ev[1], s ) );
+		return;
+	}
 	impl->Launch( kkCollide, 0.0, [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<CollideOp>, dim3( Blocks( n ) ), dim3( kBlock ), 0, s, e0, e1, 0, a ); } );
 	HIP_CHECK( hipEventRecord( impl->ev[1], s ) );
 	PackCollideKernel<<<1, kPackThreads, 0, s>>>( pc );
