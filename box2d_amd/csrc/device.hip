@@ -4411,7 +4411,7 @@ void DeviceWorld::SolveLaunch( const StepParams& params, bool anyRestitution, bo
 		else
 		{
 			impl->Launch( kkPrepareContacts, kBytesPrepare * nc,
-						  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<PrepareContactsOp>, dim3( Blocks( nc ) ), dim3( kBlock ), 0, s, e0, e1, 0, pa ); } );
+						  [&]( hipEvent_t e0, hipEvent_t e1 ) { hipExtLaunchKernelGGL( LaneKernel<PrepareContactsOp>, dim3( ( nc + kWarmBlock - 1 ) / kWarmBlock ), dim3( kWarmBlock ), 0, s, e0, e1, 0, pa ); } );
 		}
 	}
 
